@@ -1,0 +1,437 @@
+// The C ABI of libbeamtracer_hip.so (declared in include/beamtracer_hip.h).
+//
+// Host forms (bt_<name>): the drop-in for the reference's host kernels.  They validate
+// the shapes, copy the arrays to the device (grow-only buffers kept per device), launch,
+// copy the outputs back and return when done.
+// Device forms (bt_<name>_device): take device pointers and launch asynchronously on the
+// caller's stream; only the small patch and plane tables of the trace are read on the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/beamtracer_hip.h"
+#include "bt_internal.h"
+
+using bt::i64;
+using bt::u64;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& message) {
+    g_error = message;
+    return code;
+}
+
+int hip_fail(hipError_t err, const char* what) {
+    return fail(BT_E_LAUNCH, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+constexpr int kMaxPatches = 16;
+constexpr int kMaxPlanes = 256;
+constexpr size_t kTableBytes = 4096;
+
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct DeviceState {
+    std::vector<Buffer> buffers;   // host-form staging, by slot
+    Buffer workspace;              // library-owned workspace
+    hipStream_t stream = nullptr;  // host-form stream
+    int cus = 0;
+};
+
+std::mutex g_mutex;
+std::map<int, DeviceState> g_devices;
+
+int current_device(int* dev) {
+    hipError_t err = hipGetDevice(dev);
+    if (err != hipSuccess) return hip_fail(err, "hipGetDevice");
+    return BT_OK;
+}
+
+// per-device state, created on first use (caller holds g_mutex)
+int device_state(DeviceState** out) {
+    int dev = 0;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    DeviceState& s = g_devices[dev];
+    if (s.cus == 0) {
+        hipDeviceProp_t prop;
+        hipError_t err = hipGetDeviceProperties(&prop, dev);
+        if (err != hipSuccess) return hip_fail(err, "hipGetDeviceProperties");
+        s.cus = prop.multiProcessorCount;
+        s.buffers.resize(32);
+    }
+    *out = &s;
+    return BT_OK;
+}
+
+int ensure(Buffer& b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.bytes >= bytes) return BT_OK;
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    hipError_t err = hipMalloc(&b.ptr, bytes);
+    if (err != hipSuccess) return fail(BT_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(err));
+    b.bytes = bytes;
+    return BT_OK;
+}
+
+size_t workspace_bytes(i64 nodes) { return round256((size_t)(16 * nodes)) + 256 + kTableBytes; }
+
+// The fixed launch-independent checks of a trace (host-readable tables only).
+int check_trace(const bt_trace2d_args* a) {
+    if (a == nullptr) return fail(BT_E_ARG, "args is NULL");
+    if (a->n < 0 || a->n_active < 0) return fail(BT_E_ARG, "negative trajectory count");
+    if (a->n_patches < 1 || a->n_patches > kMaxPatches)
+        return fail(BT_E_ARG, "n_patches must be 1..16");
+    if (a->n_planes < 0 || a->n_planes > kMaxPlanes) return fail(BT_E_ARG, "n_planes must be 0..256");
+    if (a->cylindrical < 0 || a->cylindrical > 2) return fail(BT_E_ARG, "cylindrical must be 0, 1 or 2");
+    if (a->integrator < 0 || a->integrator > 1) return fail(BT_E_ARG, "integrator must be 0 or 1");
+    if (!(a->h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    if (a->refine < 1) return fail(BT_E_ARG, "refine must be positive");
+    if (a->capacity < 0 || a->path_capacity < 0 || a->n_paths < 0)
+        return fail(BT_E_ARG, "negative buffer size");
+    if (a->patch_int == nullptr || a->patch_real == nullptr)
+        return fail(BT_E_ARG, "patch tables are NULL");
+    if (a->n_planes > 0 && a->planes == nullptr) return fail(BT_E_ARG, "planes is NULL");
+    if (a->graded != 0 && a->n_x_nodes < 2) return fail(BT_E_ARG, "graded axis needs 2+ nodes");
+    i64 nodes = 0;
+    for (i64 q = 0; q < a->n_patches; ++q) {
+        const int64_t* pi = a->patch_int + q * 6;
+        const i64 nx = pi[0], ny = pi[1];
+        if (nx < 2 || ny < 2) return fail(BT_E_ARG, "a patch has fewer than 2 nodes per axis");
+        if (pi[2] < 0 || pi[2] + (nx + 1) * ny > a->n_fxs) return fail(BT_E_ARG, "fx offset beyond fxs");
+        if (pi[3] < 0 || pi[3] + nx * (ny + 1) > a->n_fys) return fail(BT_E_ARG, "fy offset beyond fys");
+        if (pi[5] < 0 || pi[5] + nx * ny > a->charge_cols)
+            return fail(BT_E_ARG, "node offset beyond charge");
+        if (pi[5] + nx * ny > nodes) nodes = pi[5] + nx * ny;
+        if (!(a->patch_real[q * 7 + 2] > 0.0)) return fail(BT_E_ARG, "a patch spacing is not positive");
+    }
+    const int64_t* base = a->patch_int + (a->n_patches - 1) * 6;
+    if (a->cells_rows < base[0] - 1 || a->cells_cols < base[1] - 1)
+        return fail(BT_E_ARG, "cells must be at least (nx - 1, ny - 1)");
+    if (a->n_subcells < 1) return fail(BT_E_ARG, "subcells is empty");
+    if (a->charge_rows < 1) return fail(BT_E_ARG, "charge has no rows");
+    return BT_OK;
+}
+
+bt::TraceParams make_params(const bt_trace2d_args* a) {
+    bt::TraceParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.active = a->active; p.n_active = a->n_active;
+    p.x = a->x; p.v = a->v; p.n = a->n;
+    p.qm = a->qm; p.dq = a->dq;
+    p.steps = (i64*)a->steps; p.status = (i64*)a->status; p.electrode = (i64*)a->electrode;
+    p.fxs = a->fxs; p.fys = a->fys;
+    p.n_patches = (int)a->n_patches;
+    p.origin_x = a->origin_x; p.origin_y = a->origin_y; p.h = a->h;
+    p.upper_x = a->upper_x; p.upper_y = a->upper_y;
+    p.step_fraction = a->step_fraction; p.max_steps = a->max_steps;
+    p.cells = (const i64*)a->cells; p.cells_rows = a->cells_rows; p.cells_cols = a->cells_cols;
+    p.subcells = (const i64*)a->subcells; p.n_subcells = a->n_subcells; p.refine = a->refine;
+    p.n_planes = (int)a->n_planes;
+    p.hits = a->hits; p.capacity = a->capacity; p.counts = (i64*)a->counts;
+    p.path_slot = (const i64*)a->path_slot; p.paths = a->paths; p.n_paths = a->n_paths;
+    p.path_capacity = a->path_capacity; p.path_count = (i64*)a->path_count;
+    p.graded = (int)a->graded; p.x_nodes = a->x_nodes; p.n_x_nodes = a->n_x_nodes;
+    p.inv_c2 = a->inv_c2;
+    if (a->has_b) { p.b0 = a->b0; p.b1 = a->b1; p.b2 = a->b2; }
+    const int64_t* base = a->patch_int + (a->n_patches - 1) * 6;
+    const double* basr = a->patch_real + (a->n_patches - 1) * 7;
+    p.nx = base[0]; p.ny = base[1];
+    p.bofx = base[2]; p.bofy = base[3]; p.bofn = base[5];
+    p.box = basr[0]; p.boy = basr[1]; p.bh = basr[2];
+    p.deposit_exponent = a->deposit_exponent;
+    return p;
+}
+
+i64 trace_nodes(const bt_trace2d_args* a) {
+    i64 nodes = 0;
+    for (i64 q = 0; q < a->n_patches; ++q) {
+        const int64_t* pi = a->patch_int + q * 6;
+        if (pi[5] + pi[0] * pi[1] > nodes) nodes = pi[5] + pi[0] * pi[1];
+    }
+    return nodes;
+}
+
+// Place the accumulator, queue and tables in the workspace, upload the tables, launch.
+int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t stream, int cus) {
+    const i64 nodes = trace_nodes(a);
+    char* base = static_cast<char*>(ws);
+    p.acc = reinterpret_cast<u64*>(base);
+    p.queue = reinterpret_cast<u64*>(base + round256((size_t)(16 * nodes)));
+    char* tables = base + round256((size_t)(16 * nodes)) + 256;
+    const size_t pi_bytes = (size_t)a->n_patches * 6 * 8, pr_bytes = (size_t)a->n_patches * 7 * 8;
+    i64* d_pi = reinterpret_cast<i64*>(tables);
+    double* d_pr = reinterpret_cast<double*>(tables + pi_bytes);
+    double* d_pl = reinterpret_cast<double*>(tables + pi_bytes + pr_bytes);
+    hipError_t err = hipMemcpyAsync(d_pi, a->patch_int, pi_bytes, hipMemcpyHostToDevice, stream);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(d_pr, a->patch_real, pr_bytes, hipMemcpyHostToDevice, stream);
+    if (err == hipSuccess && a->n_planes > 0)
+        err = hipMemcpyAsync(d_pl, a->planes, (size_t)a->n_planes * 8, hipMemcpyHostToDevice, stream);
+    if (err != hipSuccess) return hip_fail(err, "table upload");
+    p.patch_int = d_pi; p.patch_real = d_pr; p.planes = d_pl;
+
+    const int integ = (int)a->integrator, geom = (int)a->cylindrical;
+    const bool has_b = a->has_b != 0;
+    const bool general = a->n_patches > 1 || a->graded != 0 || a->inv_c2 != 0.0;
+    int block = a->block_size > 0 ? a->block_size : 64;
+    if (block % 64 != 0 || block > 256) return fail(BT_E_ARG, "block_size must be a multiple of 64, at most 256");
+    int grid = 0;
+    if (a->n_active > 0) {
+        int per_cu = bt::trace_occupancy(integ, geom, has_b, general, block);
+        if (per_cu < 1) per_cu = 1;
+        i64 full = (i64)per_cu * cus;
+        i64 need = (a->n_active + block - 1) / block;
+        grid = (int)(need < full ? need : full);
+        if (a->max_blocks > 0 && grid > a->max_blocks) grid = a->max_blocks;
+        if (grid < 1) grid = 1;
+    }
+    p.charge = a->charge;  // row 0
+    int rc = bt::launch_trace(p, block, grid, stream, integ, geom, has_b, general, nodes);
+    if (rc) return fail(BT_E_LAUNCH, std::string("trace launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
+template <typename T>
+int upload(DeviceState& s, int slot, const T* host, size_t count, T** dev, hipStream_t st) {
+    int rc = ensure(s.buffers[slot], count * sizeof(T));
+    if (rc) return rc;
+    *dev = static_cast<T*>(s.buffers[slot].ptr);
+    if (count > 0 && host != nullptr) {
+        hipError_t err = hipMemcpyAsync(*dev, host, count * sizeof(T), hipMemcpyHostToDevice, st);
+        if (err != hipSuccess) return hip_fail(err, "hipMemcpyAsync H2D");
+    }
+    return BT_OK;
+}
+
+template <typename T>
+int download(T* host, const T* dev, size_t count, hipStream_t st) {
+    if (count == 0 || host == nullptr) return BT_OK;
+    hipError_t err = hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemcpyAsync D2H");
+    return BT_OK;
+}
+
+int host_stream(DeviceState& s) {
+    if (s.stream) return BT_OK;
+    hipError_t err = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (err != hipSuccess) return hip_fail(err, "hipStreamCreate");
+    return BT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bt_version(void) { return "beamtracer_hip 0.1.0 (gfx950, fp64, fixed-point deposit)"; }
+
+const char* bt_last_error(void) { return g_error.c_str(); }
+
+int64_t bt_trace_args_size(void) { return (int64_t)sizeof(bt_trace2d_args); }
+
+int32_t bt_deposit_exponent(double scale) {
+    if (!(scale > 0.0) || !std::isfinite(scale)) return -1074 + 60;
+    return (int32_t)std::ilogb(scale) - 52;
+}
+
+int64_t bt_trace_workspace_bytes(int64_t n_nodes) { return (int64_t)workspace_bytes(n_nodes); }
+
+int bt_trace_trajectories_2d_device(const bt_trace2d_args* a, void* stream) {
+    int rc = check_trace(a);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    if ((rc = device_state(&s))) return rc;
+    const i64 nodes = trace_nodes(a);
+    void* ws = a->workspace;
+    if (ws == nullptr) {
+        if ((rc = ensure(s->workspace, workspace_bytes(nodes)))) return rc;
+        ws = s->workspace.ptr;
+    } else if (a->workspace_bytes < (int64_t)workspace_bytes(nodes)) {
+        return fail(BT_E_ARG, "workspace smaller than bt_trace_workspace_bytes()");
+    }
+    return run_trace(a, make_params(a), ws, static_cast<hipStream_t>(stream), s->cus);
+}
+
+int bt_trace_trajectories_2d(const bt_trace2d_args* a) {
+    int rc = check_trace(a);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    if ((rc = device_state(&s))) return rc;
+    if ((rc = host_stream(*s))) return rc;
+    hipStream_t st = s->stream;
+    const i64 n = a->n, nodes = trace_nodes(a);
+    bt_trace2d_args d = *a;
+    double *x, *v, *hits, *paths, *charge, *fxs, *fys, *xn, *b0 = nullptr, *b1 = nullptr, *b2 = nullptr;
+    const double *qm, *dq;
+    int64_t *active, *steps, *status, *electrode, *counts, *slot, *pcount, *cells, *subcells;
+    const size_t nh = (size_t)n * a->n_planes * a->capacity * 8;
+    const size_t np = (size_t)a->n_paths * a->path_capacity * 2;
+    const int64_t* bpi = a->patch_int + (a->n_patches - 1) * 6;
+    const size_t nb = (size_t)bpi[0] * bpi[1];
+    double* qm_d; double* dq_d;
+    if ((rc = upload(*s, 0, a->active, (size_t)a->n_active, &active, st))) return rc;
+    if ((rc = upload(*s, 1, a->x, (size_t)(3 * n), &x, st))) return rc;
+    if ((rc = upload(*s, 2, a->v, (size_t)(3 * n), &v, st))) return rc;
+    if ((rc = upload(*s, 3, a->qm, (size_t)n, &qm_d, st))) return rc;
+    if ((rc = upload(*s, 4, a->dq, (size_t)n, &dq_d, st))) return rc;
+    qm = qm_d; dq = dq_d;
+    if ((rc = upload(*s, 5, a->steps, (size_t)n, &steps, st))) return rc;
+    if ((rc = upload(*s, 6, a->status, (size_t)n, &status, st))) return rc;
+    if ((rc = upload(*s, 7, a->electrode, (size_t)n, &electrode, st))) return rc;
+    if ((rc = upload(*s, 8, a->fxs, (size_t)a->n_fxs, &fxs, st))) return rc;
+    if ((rc = upload(*s, 9, a->fys, (size_t)a->n_fys, &fys, st))) return rc;
+    if ((rc = upload(*s, 10, a->cells, (size_t)(a->cells_rows * a->cells_cols), &cells, st))) return rc;
+    if ((rc = upload(*s, 11, a->subcells, (size_t)a->n_subcells, &subcells, st))) return rc;
+    if ((rc = upload(*s, 12, a->hits, nh, &hits, st))) return rc;
+    if ((rc = upload(*s, 13, a->counts, (size_t)n * a->n_planes, &counts, st))) return rc;
+    if ((rc = upload(*s, 14, a->path_slot, (size_t)n, &slot, st))) return rc;
+    if ((rc = upload(*s, 15, a->paths, np, &paths, st))) return rc;
+    if ((rc = upload(*s, 16, a->path_count, (size_t)a->n_paths, &pcount, st))) return rc;
+    if ((rc = upload(*s, 17, (const double*)nullptr, (size_t)nodes, &charge, st))) return rc;
+    if ((rc = upload(*s, 18, a->x_nodes, (size_t)a->n_x_nodes, &xn, st))) return rc;
+    if (a->has_b) {
+        if (a->b0 && (rc = upload(*s, 19, a->b0, nb, &b0, st))) return rc;
+        if (a->b1 && (rc = upload(*s, 20, a->b1, nb, &b1, st))) return rc;
+        if (a->b2 && (rc = upload(*s, 21, a->b2, nb, &b2, st))) return rc;
+    }
+    hipError_t err = hipMemsetAsync(charge, 0, (size_t)nodes * 8, st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    d.active = active; d.x = x; d.v = v; d.qm = qm; d.dq = dq; d.steps = steps; d.status = status;
+    d.electrode = electrode; d.fxs = fxs; d.fys = fys; d.cells = cells; d.subcells = subcells;
+    d.hits = hits; d.counts = counts; d.path_slot = slot; d.paths = paths; d.path_count = pcount;
+    d.charge = charge; d.x_nodes = xn; d.b0 = b0; d.b1 = b1; d.b2 = b2;
+    if ((rc = ensure(s->workspace, workspace_bytes(nodes)))) return rc;
+    if ((rc = run_trace(&d, make_params(&d), s->workspace.ptr, st, s->cus))) return rc;
+    std::vector<double> added((size_t)nodes);
+    if ((rc = download(a->x, x, (size_t)(3 * n), st))) return rc;
+    if ((rc = download(a->v, v, (size_t)(3 * n), st))) return rc;
+    if ((rc = download(a->steps, steps, (size_t)n, st))) return rc;
+    if ((rc = download(a->status, status, (size_t)n, st))) return rc;
+    if ((rc = download(a->electrode, electrode, (size_t)n, st))) return rc;
+    if ((rc = download(a->hits, hits, nh, st))) return rc;
+    if ((rc = download(a->counts, counts, (size_t)n * a->n_planes, st))) return rc;
+    if ((rc = download(a->paths, paths, np, st))) return rc;
+    if ((rc = download(a->path_count, pcount, (size_t)a->n_paths, st))) return rc;
+    if ((rc = download(added.data(), charge, (size_t)nodes, st))) return rc;
+    err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return hip_fail(err, "trace");
+    for (i64 c = 0; c < nodes; ++c) a->charge[c] += added[(size_t)c];
+    return BT_OK;
+}
+
+int bt_staggered_field_2d_device(const double* px, const double* py, int64_t n, const double* fx,
+                                 const double* fy, int64_t nx, int64_t ny, double origin_x,
+                                 double origin_y, double h, double* ex, double* ey, void* stream) {
+    if (n < 0) return fail(BT_E_ARG, "negative point count");
+    if (nx < 2 || ny < 2) return fail(BT_E_ARG, "the mesh needs 2+ nodes per axis");
+    if (!(h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    int rc = bt::launch_gather(px, py, n, fx, fy, nx, ny, origin_x, origin_y, h, ex, ey,
+                               static_cast<hipStream_t>(stream));
+    if (rc) return fail(BT_E_LAUNCH, std::string("gather launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
+int bt_staggered_field_2d(const double* px, const double* py, int64_t n, const double* fx,
+                          const double* fy, int64_t nx, int64_t ny, double origin_x, double origin_y,
+                          double h, double* ex, double* ey) {
+    if (n < 0) return fail(BT_E_ARG, "negative point count");
+    if (nx < 2 || ny < 2) return fail(BT_E_ARG, "the mesh needs 2+ nodes per axis");
+    if (!(h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    int rc;
+    if ((rc = device_state(&s))) return rc;
+    if ((rc = host_stream(*s))) return rc;
+    hipStream_t st = s->stream;
+    double *dpx, *dpy, *dfx, *dfy, *dex, *dey;
+    if ((rc = upload(*s, 0, px, (size_t)n, &dpx, st))) return rc;
+    if ((rc = upload(*s, 1, py, (size_t)n, &dpy, st))) return rc;
+    if ((rc = upload(*s, 2, fx, (size_t)((nx + 1) * ny), &dfx, st))) return rc;
+    if ((rc = upload(*s, 3, fy, (size_t)(nx * (ny + 1)), &dfy, st))) return rc;
+    if ((rc = upload(*s, 4, (const double*)nullptr, (size_t)n, &dex, st))) return rc;
+    if ((rc = upload(*s, 5, (const double*)nullptr, (size_t)n, &dey, st))) return rc;
+    if (bt::launch_gather(dpx, dpy, n, dfx, dfy, nx, ny, origin_x, origin_y, h, dex, dey, st))
+        return fail(BT_E_LAUNCH, std::string("gather launch: ") + hipGetErrorString(hipGetLastError()));
+    if ((rc = download(ex, dex, (size_t)n, st))) return rc;
+    if ((rc = download(ey, dey, (size_t)n, st))) return rc;
+    hipError_t err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return hip_fail(err, "gather");
+    return BT_OK;
+}
+
+int bt_deposit_trajectories_2d_device(const double* px, const double* py, const double* q, int64_t n,
+                                      double origin_x, double origin_y, double h,
+                                      int64_t cylindrical, double* charge, int64_t nx, int64_t ny,
+                                      int32_t deposit_exponent, void* workspace,
+                                      int64_t workspace_bytes_, void* stream) {
+    if (n < 0) return fail(BT_E_ARG, "negative point count");
+    if (nx < 2 || ny < 2) return fail(BT_E_ARG, "the mesh needs 2+ nodes per axis");
+    if (!(h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    void* ws = workspace;
+    if (ws == nullptr) {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        DeviceState* s = nullptr;
+        int rc;
+        if ((rc = device_state(&s))) return rc;
+        if ((rc = ensure(s->workspace, workspace_bytes(nx * ny)))) return rc;
+        ws = s->workspace.ptr;
+    } else if (workspace_bytes_ < (int64_t)workspace_bytes(nx * ny)) {
+        return fail(BT_E_ARG, "workspace smaller than bt_trace_workspace_bytes()");
+    }
+    if (bt::launch_deposit(px, py, q, n, origin_x, origin_y, h, cylindrical, charge, nx, ny,
+                           deposit_exponent, static_cast<u64*>(ws), static_cast<hipStream_t>(stream)))
+        return fail(BT_E_LAUNCH, std::string("deposit launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
+int bt_deposit_trajectories_2d(const double* px, const double* py, const double* q, int64_t n,
+                               double origin_x, double origin_y, double h, int64_t cylindrical,
+                               double* charge, int64_t nx, int64_t ny, int32_t deposit_exponent) {
+    if (n < 0) return fail(BT_E_ARG, "negative point count");
+    if (nx < 2 || ny < 2) return fail(BT_E_ARG, "the mesh needs 2+ nodes per axis");
+    if (!(h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    int rc;
+    if ((rc = device_state(&s))) return rc;
+    if ((rc = host_stream(*s))) return rc;
+    hipStream_t st = s->stream;
+    const i64 nodes = nx * ny;
+    double *dpx, *dpy, *dq, *dc;
+    if ((rc = upload(*s, 0, px, (size_t)n, &dpx, st))) return rc;
+    if ((rc = upload(*s, 1, py, (size_t)n, &dpy, st))) return rc;
+    if ((rc = upload(*s, 2, q, (size_t)n, &dq, st))) return rc;
+    if ((rc = upload(*s, 3, (const double*)nullptr, (size_t)nodes, &dc, st))) return rc;
+    hipError_t err = hipMemsetAsync(dc, 0, (size_t)nodes * 8, st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if ((rc = ensure(s->workspace, workspace_bytes(nodes)))) return rc;
+    if (bt::launch_deposit(dpx, dpy, dq, n, origin_x, origin_y, h, cylindrical, dc, nx, ny,
+                           deposit_exponent, static_cast<u64*>(s->workspace.ptr), st))
+        return fail(BT_E_LAUNCH, std::string("deposit launch: ") + hipGetErrorString(hipGetLastError()));
+    std::vector<double> added((size_t)nodes);
+    if ((rc = download(added.data(), dc, (size_t)nodes, st))) return rc;
+    err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return hip_fail(err, "deposit");
+    for (i64 c = 0; c < nodes; ++c) charge[c] += added[(size_t)c];
+    return BT_OK;
+}
+
+}  // extern "C"

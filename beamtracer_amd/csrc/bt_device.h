@@ -1,0 +1,120 @@
+// Device helpers shared by the particle-stage kernels (gfx950, FP64).
+//
+// * the Lorentz factor, graded-axis location and patch search of the reference tracer
+//   (kernels/trace_trajectories_2d/trace_trajectories_2d_pyccel.py:19-39, 165-171, 215-222);
+// * the clamped staggered bilinear gather (staggered_field_2d_pyccel.py:36-55);
+// * 128-bit fixed-point accumulation: a deposit is rounded once to a multiple of 2^e and
+//   added as an exact integer, so a node's total does not depend on the order of the
+//   atomic adds (the reference adds FP64 in an order set by its threads).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bt {
+
+typedef int64_t i64;
+typedef unsigned long long u64;
+
+// gamma of the proper velocity u: sqrt(1 + u.u / c^2); exactly 1 for inv_c2 == 0.
+__device__ __forceinline__ double lorentz_factor(double u0, double u1, double u2, double inv_c2) {
+    return sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2) * inv_c2);
+}
+
+// The edge [nodes[c], nodes[c+1]] holding x (pyccel.py:19-33).
+__device__ __forceinline__ i64 node_cell(double x, const double* __restrict__ nodes, i64 n) {
+    i64 lo = 0, hi = n - 1;
+    if (x >= nodes[n - 1]) {
+        lo = n - 2;
+    } else if (x > nodes[0]) {
+        while (hi - lo > 1) {
+            const i64 mid = (lo + hi) / 2;
+            if (nodes[mid] <= x) lo = mid; else hi = mid;
+        }
+    }
+    return lo;
+}
+
+// The uniform coordinate x0 + h * index of x on a graded axis (pyccel.py:36-39).
+__device__ __forceinline__ double uniform_x(double x, const double* __restrict__ nodes, i64 n,
+                                            double x0, double h) {
+    const i64 c = node_cell(x, nodes, n);
+    return x0 + h * ((double)c + (x - nodes[c]) / (nodes[c + 1] - nodes[c]));
+}
+
+// floor(s) as an index clamped to [lo, hi].  Clamping the double first keeps the
+// conversion in range for points far off the mesh; the clamped index is the reference's.
+__device__ __forceinline__ int clamped_floor(double s, int lo, int hi) {
+    double f = floor(s);
+    f = fmax(f, (double)lo);
+    f = fmin(f, (double)hi);
+    return (int)f;
+}
+
+// Clamped bilinear of the staggered grid f (rows of `cols` values) at fractional index
+// (s, t): i in [0, imax], j in [0, jmax] (staggered_field_2d_pyccel.py:38-55).
+__device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f, i64 cols,
+                                                   double s, double t, int imax, int jmax) {
+    const int i = clamped_floor(s, 0, imax);
+    const int j = clamped_floor(t, 0, jmax);
+    const double u = s - (double)i;
+    const double w = t - (double)j;
+    const i64 c = (i64)i * cols + j;
+    const double f00 = f[c], f01 = f[c + 1], f10 = f[c + cols], f11 = f[c + cols + 1];
+    const double low = f00 + w * (f01 - f00);
+    const double high = f10 + w * (f11 - f10);
+    return low + u * (high - low);
+}
+
+// ---- 128-bit fixed point ------------------------------------------------------------
+// A node holds {u64 lo, i64 hi}: the value (hi * 2^64 + lo) * 2^e in two's complement.
+
+__device__ __forceinline__ void fixed_add(u64* __restrict__ acc, i64 node, double value, int e) {
+    const double s = rint(ldexp(value, -e));
+    if (!(fabs(s) < 0x1p126)) return;  // zero is handled below; NaN/overflow are dropped
+    u64 lo;
+    i64 hi;
+    if (fabs(s) < 0x1p63) {
+        const i64 q = (i64)s;
+        if (q == 0) return;
+        lo = (u64)q;
+        hi = q < 0 ? -1 : 0;
+    } else {
+        const double hd = floor(s * 0x1p-64);
+        hi = (i64)hd;
+        lo = (u64)(s - hd * 0x1p64);  // exact: 0 <= s - hd 2^64 < 2^64
+    }
+    u64* p = acc + 2 * node;
+    const u64 old = atomicAdd(p, lo);
+    const i64 dh = hi + ((old + lo) < old ? 1 : 0);
+    if (dh != 0) atomicAdd(p + 1, (u64)dh);
+}
+
+// The node value as a double, rounded once (correctly) from the 128-bit integer.
+__device__ __forceinline__ double fixed_value(u64 lo, i64 hi_signed, int e) {
+    u64 hi = (u64)hi_signed;
+    const bool neg = hi_signed < 0;
+    if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0 ? 1ull : 0ull);
+    }
+    double m;
+    if (hi == 0) {
+        m = (double)lo;
+    } else {
+        const int lz = __clzll((long long)hi);
+        const int bits = 64 - lz;  // significant bits of hi, 1..64
+        u64 top, sticky;
+        if (lz == 0) {
+            top = hi;
+            sticky = lo != 0;
+        } else {
+            top = (hi << lz) | (lo >> bits);
+            sticky = (lo << lz) != 0;
+        }
+        m = ldexp((double)(top | sticky), bits);
+    }
+    return ldexp(neg ? -m : m, e);
+}
+
+}  // namespace bt

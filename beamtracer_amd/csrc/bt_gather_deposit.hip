@@ -1,0 +1,118 @@
+// The standalone gather and deposit kernels for gfx950.
+//
+// staggered_field_2d: (E_x, E_y) at points from the staggered edge fields, clamped bilinear
+// (kernels/staggered_field_2d/staggered_field_2d_pyccel.py:13-55).  One point per lane,
+// grid-stride; px/py/ex/ey are streamed (coalesced, HBM-bound) while the 8 field values
+// of a point come from L2.
+//
+// deposit_trajectories_2d: the area-weighted deposit of charges at points, bilinear in
+// planar geometry and linear in r^2 in cylindrical geometry, charge beyond the mesh on
+// the boundary nodes (deposit_trajectories_2d_pyccel.py:13-59).  Each weight is rounded
+// once to the fixed-point grid 2^e and added with integer atomics into a 128-bit node
+// accumulator, then converted and added to `charge`: the result does not depend on the
+// order in which lanes arrive.
+#include <hip/hip_runtime.h>
+
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ px,
+                                                     const double* __restrict__ py, i64 n,
+                                                     const double* __restrict__ fx,
+                                                     const double* __restrict__ fy, i64 nx, i64 ny,
+                                                     double ox, double oy, double h,
+                                                     double* __restrict__ ex,
+                                                     double* __restrict__ ey) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const double sx = (px[k] - ox) / h;
+        const double sy = (py[k] - oy) / h;
+        ex[k] = clamped_bilinear(fx, ny, sx + 0.5, sy, (int)nx - 1, (int)ny - 2);
+        ey[k] = clamped_bilinear(fy, ny + 1, sx, sy + 0.5, (int)nx - 2, (int)ny - 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void deposit_kernel(const double* __restrict__ px,
+                                                      const double* __restrict__ py,
+                                                      const double* __restrict__ q, i64 n,
+                                                      double ox, double oy, double h,
+                                                      i64 cylindrical, i64 nx, i64 ny, int e,
+                                                      u64* __restrict__ acc) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const double x = px[k], y = py[k];
+        const double fi = floor((x - ox) / h);
+        const double fj = floor((y - oy) / h);
+        double t = (x - (ox + h * fi)) / h;
+        double u;
+        if (cylindrical != 0) {
+            const double r1 = oy + h * fj;
+            const double r2 = r1 + h;
+            u = (y * y - r1 * r1) / (r2 * r2 - r1 * r1);
+        } else {
+            u = (y - (oy + h * fj)) / h;
+        }
+        i64 i, j;
+        // !(f >= 0) also sends a NaN point to the first node (its deposit is dropped)
+        if (!(fi >= 0.0)) { t = 0.0; i = 0; }
+        else if (fi >= (double)(nx - 1)) { t = 1.0; i = nx - 2; }
+        else i = (i64)fi;
+        if (!(fj >= 0.0)) { u = 0.0; j = 0; }
+        else if (fj >= (double)(ny - 1)) { u = 1.0; j = ny - 2; }
+        else j = (i64)fj;
+        const double value = q[k];
+        const i64 c = i * ny + j;
+        fixed_add(acc, c, (1.0 - t) * (1.0 - u) * value, e);
+        fixed_add(acc, c + 1, (1.0 - t) * u * value, e);
+        fixed_add(acc, c + ny, t * (1.0 - u) * value, e);
+        fixed_add(acc, c + ny + 1, t * u * value, e);
+    }
+}
+
+__global__ void zero_words(u64* __restrict__ acc, i64 words) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < words; t += stride) acc[t] = 0ull;
+}
+
+__global__ void add_fixed(const u64* __restrict__ acc, i64 nodes, int e, double* __restrict__ charge) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < nodes; c += stride) {
+        const u64 lo = acc[2 * c];
+        const i64 hi = (i64)acc[2 * c + 1];
+        if (lo != 0 || hi != 0) charge[c] += fixed_value(lo, hi, e);
+    }
+}
+
+static int grid_for(i64 n, int block, int cap) {
+    i64 g = (n + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
+                  i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,
+                  hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, stream, px, py,
+                       n, fx, fy, nx, ny, ox, oy, h, ex, ey);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_deposit(const double* px, const double* py, const double* q, i64 n, double ox,
+                   double oy, double h, i64 cylindrical, double* charge, i64 nx, i64 ny, int e,
+                   u64* acc, hipStream_t stream) {
+    const i64 nodes = nx * ny;
+    hipLaunchKernelGGL(zero_words, dim3(grid_for(2 * nodes, 256, 4096)), dim3(256), 0, stream, acc,
+                       2 * nodes);
+    if (n > 0)
+        hipLaunchKernelGGL(deposit_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, stream, px,
+                           py, q, n, ox, oy, h, cylindrical, nx, ny, e, acc);
+    hipLaunchKernelGGL(add_fixed, dim3(grid_for(nodes, 256, 4096)), dim3(256), 0, stream, acc, nodes,
+                       e, charge);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace bt

@@ -1,0 +1,54 @@
+// Internal launch parameters shared by the C ABI (bt_api.hip) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bt {
+
+typedef int64_t i64;
+typedef unsigned long long u64;
+
+// Device pointers and scalars of one trace launch (see bt_trace2d_args in
+// include/beamtracer_hip.h for the meaning of each).
+struct TraceParams {
+    const i64* active; i64 n_active;
+    double* x; double* v; i64 n;
+    const double* qm; const double* dq;
+    i64* steps; i64* status; i64* electrode;
+    const double* fxs; const double* fys;
+    const i64* patch_int; const double* patch_real; int n_patches;
+    double origin_x, origin_y, h, upper_x, upper_y;
+    double step_fraction; i64 max_steps;
+    const i64* cells; i64 cells_rows, cells_cols;
+    const i64* subcells; i64 n_subcells; i64 refine;
+    const double* planes; int n_planes;
+    double* hits; i64 capacity; i64* counts;
+    const i64* path_slot; double* paths; i64 n_paths; i64 path_capacity; i64* path_count;
+    int graded; const double* x_nodes; i64 n_x_nodes;
+    double inv_c2;
+    const double* b0; const double* b1; const double* b2;
+    // base (last) patch = the mesh; filled on the host
+    i64 nx, ny;
+    i64 bofx, bofy, bofn;
+    double box, boy, bh;
+    // deposit accumulator and work queue (workspace)
+    u64* acc; int deposit_exponent; u64* queue;
+    double* charge;
+};
+
+typedef void (*TraceKernel)(TraceParams);
+
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general);
+int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
+                 bool has_b, bool general, i64 nodes);
+int trace_occupancy(int integ, int geom, bool has_b, bool general, int block);
+
+int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
+                  i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,
+                  hipStream_t stream);
+int launch_deposit(const double* px, const double* py, const double* q, i64 n, double ox,
+                   double oy, double h, i64 cylindrical, double* charge, i64 nx, i64 ny, int e,
+                   u64* acc, hipStream_t stream);
+
+}  // namespace bt

@@ -1,0 +1,553 @@
+// The fused particle tracer for gfx950: trace_trajectories_2d.
+//
+// Behaviour: kernels/trace_trajectories_2d/trace_trajectories_2d_pyccel.py:42-507 (and its
+// CUDA twin trace_trajectories_2d_cuda.cu:165-362): every active trajectory is stepped
+// until it leaves the mesh, enters an electrode sub-cell (status 3), stops in a sub-cell
+// that an electrode surface cuts (status -2, the caller decides), or reaches max_steps
+// (status 4).  Each step takes the classical Runge--Kutta step (integrator 0, four field
+// evaluations) or Boris's kick--drift--kick (1, one evaluation: the end field is the next
+// start's), deposits dq dt at the Hermite midpoint, records plane crossings and path
+// points, and tests the end point.  The arithmetic follows the reference statement by
+// statement in the same order; the library is built with -ffp-contract=off, so the
+// results are those of the CPU restatement bit for bit where the math library agrees.
+//
+// MI355X design:
+// * persistent grid, one trajectory per lane, refilled from a device work queue: a
+//   wave whose lanes finish claims that many new trajectories with one atomic, so lanes
+//   freed by short trajectories pick up the remaining ones;
+// * state in registers (x, v component rows are coalesced loads / stores);
+// * the FP64 edge fields (2.6 MB for an 801 x 201 mesh) stay resident in each XCD's 4 MB
+//   L2 and are read with plain loads;
+// * the deposit goes into a 128-bit fixed-point node array with integer atomics
+//   (deterministic), converted and added to charge row 0 by a finalize kernel;
+// * compile-time variants on integrator, geometry, magnetic field and the general
+//   features (refined patches, graded axis, relativistic motion), so the common case
+//   carries no dead branches.
+//
+// Extension (not in the reference's compiled kernel, which refuses B): a node magnetic
+// field acting through q/m (E + u/gamma x B) (solver/tracer.py:519-529) in every RK4
+// stage, and through Boris's rotation (tracer.py:618-650) for integrator 1.
+#include <hip/hip_runtime.h>
+
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+struct LaneState {
+    double x0, x1, x2, v0, v1, v2;
+    double qmk, dqk;
+    double ca0, ca1, ca2;  // Boris (E only): the last end's acceleration
+    double ce0, ce1, ce2, cb0, cb1, cb2;  // with B: the last end's E and B
+    i64 k, slot, ns;
+    int have;
+};
+
+// Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
+__device__ __forceinline__ double node_bilinear(const double* __restrict__ f, double sx, double sy,
+                                                i64 nx, i64 ny) {
+    if (f == nullptr) return 0.0;
+    sx = fmin(fmax(sx, 0.0), (double)(nx - 1));
+    sy = fmin(fmax(sy, 0.0), (double)(ny - 1));
+    i64 i = (i64)sx; if (i > nx - 2) i = nx - 2;
+    i64 j = (i64)sy; if (j > ny - 2) j = ny - 2;
+    const double a = sx - (double)i, b = sy - (double)j;
+    return (1.0 - a) * ((1.0 - b) * f[i * ny + j] + b * f[i * ny + j + 1])
+         + a * ((1.0 - b) * f[(i + 1) * ny + j] + b * f[(i + 1) * ny + j + 1]);
+}
+
+template <int GEOM>
+__device__ __forceinline__ void magnetic_at(const TraceParams& p, double lq, double qy, double cy,
+                                            double cz, double& B0, double& B1, double& B2) {
+    const double sx = (lq - p.origin_x) / p.h;
+    const double sy = (qy - p.origin_y) / p.h;
+    const double c0 = node_bilinear(p.b0, sx, sy, p.nx, p.ny);
+    const double c1 = node_bilinear(p.b1, sx, sy, p.nx, p.ny);
+    const double c2 = node_bilinear(p.b2, sx, sy, p.nx, p.ny);
+    if (GEOM == 1) {
+        B0 = c0; B1 = c1 * cy - c2 * cz; B2 = c1 * cz + c2 * cy;
+    } else if (GEOM == 2) {
+        B0 = cy * c0; B1 = c1; B2 = cy * c2;
+    } else {
+        B0 = c0; B1 = c1; B2 = c2;
+    }
+}
+
+// Boris's velocity update over tau: half kick, rotation, half kick (tracer.py:618-632).
+__device__ __forceinline__ void boris_kick(double& u0, double& u1, double& u2, double e0, double e1,
+                                           double e2, double b0, double b1, double b2, double tau,
+                                           double qm, double inv_c2) {
+    const double hq = 0.5 * tau * qm;
+    const double m0 = u0 + hq * e0, m1 = u1 + hq * e1, m2 = u2 + hq * e2;
+    const double g = 1.0 / lorentz_factor(m0, m1, m2, inv_c2);
+    const double t0 = hq * b0 * g, t1 = hq * b1 * g, t2 = hq * b2 * g;
+    const double f = 2.0 / (1.0 + (t0 * t0 + t1 * t1 + t2 * t2));
+    const double p0 = m0 + (m1 * t2 - m2 * t1);
+    const double p1 = m1 + (m2 * t0 - m0 * t2);
+    const double p2 = m2 + (m0 * t1 - m1 * t0);
+    const double s0 = f * t0, s1 = f * t1, s2 = f * t2;
+    const double n0 = m0 + (p1 * s2 - p2 * s1);
+    const double n1 = m1 + (p2 * s0 - p0 * s2);
+    const double n2 = m2 + (p0 * s1 - p1 * s0);
+    u0 = n0 + hq * e0; u1 = n1 + hq * e1; u2 = n2 + hq * e2;
+}
+
+// The finest patch containing (px, py); the base mesh (last) if none (pyccel.py:215-222).
+template <bool GENERAL>
+__device__ __forceinline__ int patch_of(const TraceParams& p, double px, double py) {
+    if (!GENERAL) return p.n_patches - 1;
+    for (int q = 0; q < p.n_patches - 1; ++q) {
+        const double* pr = p.patch_real + q * 7;
+        if (px >= pr[3] && px <= pr[5] && py >= pr[4] && py <= pr[6]) return q;
+    }
+    return p.n_patches - 1;
+}
+
+struct PatchView {
+    i64 nx, ny, ofx, ofy, ofn;
+    double ox, oy, h;
+};
+
+template <bool GENERAL>
+__device__ __forceinline__ PatchView patch_view(const TraceParams& p, int pp) {
+    PatchView v;
+    if (!GENERAL) {
+        v.nx = p.nx; v.ny = p.ny; v.ofx = p.bofx; v.ofy = p.bofy; v.ofn = p.bofn;
+        v.ox = p.box; v.oy = p.boy; v.h = p.bh;
+        return v;
+    }
+    const i64* pi = p.patch_int + (i64)pp * 6;
+    const double* pr = p.patch_real + (i64)pp * 7;
+    v.nx = pi[0]; v.ny = pi[1]; v.ofx = pi[2]; v.ofy = pi[3]; v.ofn = pi[5];
+    v.ox = pr[0]; v.oy = pr[1]; v.h = pr[2];
+    return v;
+}
+
+template <bool GENERAL>
+__device__ __forceinline__ double locate_x(const TraceParams& p, double x) {
+    if (GENERAL && p.graded != 0) return uniform_x(x, p.x_nodes, p.n_x_nodes, p.origin_x, p.h);
+    return x;
+}
+
+template <bool GENERAL>
+__device__ __forceinline__ double lorentz(double u0, double u1, double u2, double inv_c2) {
+    if (!GENERAL) return 1.0;
+    return lorentz_factor(u0, u1, u2, inv_c2);
+}
+
+// E_x, E_y of the finest patch at the Cartesian point q, with the radial direction and
+// the mesh coordinates (pyccel.py:196-269).
+template <int GEOM, bool GENERAL>
+__device__ __forceinline__ void edge_field(const TraceParams& p, double q0, double q1, double q2,
+                                           double& ex, double& ey, double& cy, double& cz,
+                                           double& lq, double& qy) {
+    qy = q1; cy = 0.0; cz = 0.0;
+    if (GEOM == 1) {
+        const double r = hypot(q1, q2);
+        if (r > 0.0) { cy = q1 / r; cz = q2 / r; }
+        qy = r;
+    } else if (GEOM == 2) {
+        cy = 1.0;
+        if (q1 < p.origin_y) cy = -1.0;
+        qy = p.origin_y + fabs(q1 - p.origin_y);
+    }
+    lq = locate_x<GENERAL>(p, q0);
+    const PatchView v = patch_view<GENERAL>(p, patch_of<GENERAL>(p, lq, qy));
+    const double sx = (lq - v.ox) / v.h;
+    const double sy = (qy - v.oy) / v.h;
+    ex = clamped_bilinear(p.fxs + v.ofx, v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
+    ey = clamped_bilinear(p.fys + v.ofy, v.ny + 1, sx, sy + 0.5, (int)v.nx - 2, (int)v.ny - 1);
+}
+
+// One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+__device__ __forceinline__ int step(const TraceParams& p, LaneState& s) {
+    const double inv_c2 = p.inv_c2;
+    double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
+    const double qmk = s.qmk;
+    // the spacing of the finest patch at the step's start (pyccel.py:155-177)
+    double hloc = p.h;
+    if (GENERAL) {
+        double rr = x1;
+        if (GEOM == 1) rr = hypot(x1, x2);
+        else if (GEOM == 2) rr = p.origin_y + fabs(x1 - p.origin_y);
+        const double lx = locate_x<GENERAL>(p, x0);
+        for (int qq = 0; qq < p.n_patches - 1; ++qq) {
+            const double* pr = p.patch_real + qq * 7;
+            if (lx >= pr[3] && lx <= pr[5] && rr >= pr[4] && rr <= pr[6]) { hloc = pr[2]; break; }
+        }
+        if (p.graded != 0) {
+            const i64 cg = node_cell(x0, p.x_nodes, p.n_x_nodes);
+            const double d2 = (p.x_nodes[cg + 1] - p.x_nodes[cg]) / p.h;
+            if (d2 < 1.0) hloc = hloc * d2;
+        }
+    }
+    const double gv = 1.0 / lorentz<GENERAL>(v0, v1, v2, inv_c2);
+    double q0 = x0, q1 = x1, q2 = x2;
+    double kx0 = v0, kx1 = v1, kx2 = v2;
+    double sx0 = 0.0, sx1 = 0.0, sx2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, y0, y1, y2;
+    double dt = 0.0, hdt = 0.0, dt6 = 0.0;
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    constexpr int NSTAGE = INTEG == 0 ? 4 : 2;
+#pragma unroll
+    for (int stage = 0; stage < NSTAGE; ++stage) {
+        double gk = 1.0 / lorentz<GENERAL>(kx0, kx1, kx2, inv_c2);
+        double g0, g1, g2;
+        if (INTEG != 0 && stage == 0 && s.have != 0) {
+            if (HAS_B) {
+                e0 = s.ce0; e1 = s.ce1; e2 = s.ce2; b0 = s.cb0; b1 = s.cb1; b2 = s.cb2;
+                const double c0 = gk * kx0, c1 = gk * kx1, c2 = gk * kx2;
+                g0 = qmk * (e0 + (c1 * b2 - c2 * b1));
+                g1 = qmk * (e1 + (c2 * b0 - c0 * b2));
+                g2 = qmk * (e2 + (c0 * b1 - c1 * b0));
+            } else {
+                g0 = s.ca0; g1 = s.ca1; g2 = s.ca2;
+            }
+        } else {
+            double ex, ey, cy, cz, lq, qy;
+            edge_field<GEOM, GENERAL>(p, q0, q1, q2, ex, ey, cy, cz, lq, qy);
+            if (GEOM != 0) {
+                g0 = qmk * ex;
+                g1 = qmk * (ey * cy);
+                g2 = qmk * (ey * cz);
+            } else {
+                g0 = qmk * ex;
+                g1 = qmk * ey;
+                g2 = qmk * 0.0;
+            }
+            if (HAS_B) {
+                e0 = ex;
+                e1 = GEOM != 0 ? ey * cy : ey;
+                e2 = GEOM == 1 ? ey * cz : 0.0;
+                magnetic_at<GEOM>(p, lq, qy, cy, cz, b0, b1, b2);
+                const double c0 = gk * kx0, c1 = gk * kx1, c2 = gk * kx2;
+                g0 = qmk * (e0 + (c1 * b2 - c2 * b1));
+                g1 = qmk * (e1 + (c2 * b0 - c0 * b2));
+                g2 = qmk * (e2 + (c0 * b1 - c1 * b0));
+            }
+        }
+        if (stage == 0) {
+            const double a0 = g0, a1 = g1, a2 = g2;
+            const double speed = gv * sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+            const double accel = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+            dt = -1.0;
+            if (speed > 0.0) dt = hloc / speed;
+            if (accel > 0.0) {
+                const double d2 = sqrt(2.0 * hloc / accel);
+                if (dt < 0.0 || d2 < dt) dt = d2;
+            }
+            if (dt < 0.0) dt = hloc;
+            dt = p.step_fraction * dt;
+            hdt = 0.5 * dt;
+            dt6 = dt / 6.0;
+            if (INTEG != 0) {
+                if (HAS_B) {
+                    kx0 = v0; kx1 = v1; kx2 = v2;
+                    boris_kick(kx0, kx1, kx2, e0, e1, e2, b0, b1, b2, hdt, qmk, GENERAL ? inv_c2 : 0.0);
+                } else {
+                    kx0 = v0 + hdt * a0;
+                    kx1 = v1 + hdt * a1;
+                    kx2 = v2 + hdt * a2;
+                }
+                gk = 1.0 / lorentz<GENERAL>(kx0, kx1, kx2, inv_c2);
+                q0 = x0 + dt * gk * kx0;
+                q1 = x1 + dt * gk * kx1;
+                q2 = x2 + dt * gk * kx2;
+                continue;
+            }
+            sx0 = gv * v0; sx1 = gv * v1; sx2 = gv * v2;
+            sv0 = a0; sv1 = a1; sv2 = a2;
+            q0 = x0 + hdt * gv * v0;
+            q1 = x1 + hdt * gv * v1;
+            q2 = x2 + hdt * gv * v2;
+            kx0 = v0 + hdt * a0;
+            kx1 = v1 + hdt * a1;
+            kx2 = v2 + hdt * a2;
+        } else if (INTEG != 0) {
+            if (HAS_B) {
+                w0 = kx0; w1 = kx1; w2 = kx2;
+                boris_kick(w0, w1, w2, e0, e1, e2, b0, b1, b2, hdt, qmk, GENERAL ? inv_c2 : 0.0);
+                s.ce0 = e0; s.ce1 = e1; s.ce2 = e2; s.cb0 = b0; s.cb1 = b1; s.cb2 = b2;
+            } else {
+                w0 = kx0 + hdt * g0;
+                w1 = kx1 + hdt * g1;
+                w2 = kx2 + hdt * g2;
+                s.ca0 = g0; s.ca1 = g1; s.ca2 = g2;
+            }
+            s.have = 1;
+        } else if (stage == 1) {
+            sx0 = sx0 + 2.0 * gk * kx0; sx1 = sx1 + 2.0 * gk * kx1; sx2 = sx2 + 2.0 * gk * kx2;
+            sv0 = sv0 + 2.0 * g0; sv1 = sv1 + 2.0 * g1; sv2 = sv2 + 2.0 * g2;
+            q0 = x0 + hdt * gk * kx0; q1 = x1 + hdt * gk * kx1; q2 = x2 + hdt * gk * kx2;
+            kx0 = v0 + hdt * g0; kx1 = v1 + hdt * g1; kx2 = v2 + hdt * g2;
+        } else if (stage == 2) {
+            sx0 = sx0 + 2.0 * gk * kx0; sx1 = sx1 + 2.0 * gk * kx1; sx2 = sx2 + 2.0 * gk * kx2;
+            sv0 = sv0 + 2.0 * g0; sv1 = sv1 + 2.0 * g1; sv2 = sv2 + 2.0 * g2;
+            q0 = x0 + dt * gk * kx0; q1 = x1 + dt * gk * kx1; q2 = x2 + dt * gk * kx2;
+            kx0 = v0 + dt * g0; kx1 = v1 + dt * g1; kx2 = v2 + dt * g2;
+        } else {
+            sx0 = sx0 + gk * kx0; sx1 = sx1 + gk * kx1; sx2 = sx2 + gk * kx2;
+            sv0 = sv0 + g0; sv1 = sv1 + g1; sv2 = sv2 + g2;
+        }
+    }
+    if (INTEG == 0) {
+        y0 = x0 + dt6 * sx0; y1 = x1 + dt6 * sx1; y2 = x2 + dt6 * sx2;
+        w0 = v0 + dt6 * sv0; w1 = v1 + dt6 * sv1; w2 = v2 + dt6 * sv2;
+    } else {
+        y0 = q0; y1 = q1; y2 = q2;
+    }
+
+    // space charge: dq dt at the Hermite midpoint, area-weighted (pyccel.py:372-423)
+    const double gw = 1.0 / lorentz<GENERAL>(w0, w1, w2, inv_c2);
+    double m0 = 0.5 * (x0 + y0) + dt * (gv * v0 - gw * w0) / 8.0;
+    double m1 = 0.5 * (x1 + y1) + dt * (gv * v1 - gw * w1) / 8.0;
+    const double m2 = 0.5 * (x2 + y2) + dt * (gv * v2 - gw * w2) / 8.0;
+    if (GEOM == 1) m1 = hypot(m1, m2);
+    else if (GEOM == 2) m1 = p.origin_y + fabs(m1 - p.origin_y);
+    m0 = locate_x<GENERAL>(p, m0);
+    {
+        const PatchView pv = patch_view<GENERAL>(p, patch_of<GENERAL>(p, m0, m1));
+        const double fi = floor((m0 - pv.ox) / pv.h);
+        const double fj = floor((m1 - pv.oy) / pv.h);
+        double u = (m0 - (pv.ox + pv.h * fi)) / pv.h;
+        double w;
+        if (GEOM == 1) {
+            const double low = pv.oy + pv.h * fj;
+            const double high = low + pv.h;
+            w = (m1 * m1 - low * low) / (high * high - low * low);
+        } else {
+            w = (m1 - (pv.oy + pv.h * fj)) / pv.h;
+        }
+        i64 i, j;
+        // !(f >= 0) also sends a NaN midpoint to the first node (its deposit is dropped)
+        if (!(fi >= 0.0)) { u = 0.0; i = 0; }
+        else if (fi >= (double)(pv.nx - 1)) { u = 1.0; i = pv.nx - 2; }
+        else i = (i64)fi;
+        if (!(fj >= 0.0)) { w = 0.0; j = 0; }
+        else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = pv.ny - 2; }
+        else j = (i64)fj;
+        const double deposit = s.dqk * dt;
+        const i64 c = pv.ofn + i * pv.ny + j;
+        const int e = p.deposit_exponent;
+        fixed_add(p.acc, c, (1.0 - u) * (1.0 - w) * deposit, e);
+        fixed_add(p.acc, c + 1, (1.0 - u) * w * deposit, e);
+        fixed_add(p.acc, c + pv.ny, u * (1.0 - w) * deposit, e);
+        fixed_add(p.acc, c + pv.ny + 1, u * w * deposit, e);
+    }
+    const i64 ns = s.ns + 1;
+    s.ns = ns;
+
+    // plane crossings (pyccel.py:426-442)
+    for (int pl = 0; pl < p.n_planes; ++pl) {
+        const double xp = p.planes[pl];
+        const double before = x0 - xp;
+        const double after = y0 - xp;
+        if ((before < 0.0) != (after < 0.0)) {
+            const double sfrac = before / (before - after);
+            const i64 idx = s.k * p.n_planes + pl;
+            const i64 c = p.counts[idx];
+            if (c < p.capacity) {
+                double* rec = p.hits + (idx * p.capacity + c) * 8;
+                rec[0] = xp;
+                rec[1] = x1 + sfrac * (y1 - x1);
+                rec[2] = x2 + sfrac * (y2 - x2);
+                rec[3] = gv * v0 + sfrac * (gw * w0 - gv * v0);
+                rec[4] = gv * v1 + sfrac * (gw * w1 - gv * v1);
+                rec[5] = gv * v2 + sfrac * (gw * w2 - gv * v2);
+                rec[6] = after >= 0.0 ? 1.0 : -1.0;
+                rec[7] = 1.0 * (double)ns;
+            }
+            p.counts[idx] = c + 1;
+        }
+    }
+    x0 = y0; x1 = y1; x2 = y2;
+    v0 = w0; v1 = w1; v2 = w2;
+    s.x0 = x0; s.x1 = x1; s.x2 = x2; s.v0 = v0; s.v1 = v1; s.v2 = v2;
+
+    // the end of the step: off the mesh or in an electrode (pyccel.py:450-497)
+    double px = x0, py = x1;
+    if (GEOM == 1) py = hypot(x1, x2);
+    else if (GEOM == 2) py = p.origin_y + fabs(x1 - p.origin_y);
+    if (s.slot >= 0) {
+        const i64 c = p.path_count[s.slot];
+        if (c < p.path_capacity) {
+            p.paths[(s.slot * p.path_capacity + c) * 2 + 0] = px;
+            p.paths[(s.slot * p.path_capacity + c) * 2 + 1] = py;
+        }
+        p.path_count[s.slot] = c + 1;
+    }
+    px = locate_x<GENERAL>(p, px);
+    int st = -1;
+    if (px >= p.upper_x) {
+        st = 0;
+    } else if (px < p.origin_x) {
+        st = 1;
+    } else if (py > p.upper_y || (GEOM == 0 && py < p.origin_y)) {
+        st = 2;
+    } else {
+        const double ci = (px - p.origin_x) / p.h;
+        const double cj = (py - p.origin_y) / p.h;
+        // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341
+        i64 i = (i64)fmax(ci, 0.0);
+        if (i > p.nx - 2) i = p.nx - 2;
+        if (i > p.cells_rows - 1) i = p.cells_rows - 1;
+        i64 j = (i64)fmax(cj, 0.0);
+        if (j > p.ny - 2) j = p.ny - 2;
+        if (j > p.cells_cols - 1) j = p.cells_cols - 1;
+        const i64 cell = p.cells[i * p.cells_cols + j];
+        if (cell >= 0) {
+            const i64 R = p.refine;
+            i64 si = (i64)((ci - (double)i) * (double)R);
+            if (si < 0) si = 0; else if (si > R - 1) si = R - 1;
+            i64 sj = (i64)((cj - (double)j) * (double)R);
+            if (sj < 0) sj = 0; else if (sj > R - 1) sj = R - 1;
+            const i64 at = (cell * R + si) * R + sj;
+            const i64 e = at < p.n_subcells ? p.subcells[at] : -1;
+            if (e >= 0) {
+                st = 3;
+                p.electrode[s.k] = e;
+            } else if (e == -2) {
+                st = -2;
+            }
+        }
+    }
+    return st;
+}
+
+__device__ __forceinline__ void store_end(const TraceParams& p, const LaneState& s, int st) {
+    const i64 N = p.n, k = s.k;
+    p.x[k] = s.x0; p.x[N + k] = s.x1; p.x[2 * N + k] = s.x2;
+    p.v[k] = s.v0; p.v[N + k] = s.v1; p.v[2 * N + k] = s.v2;
+    p.steps[k] = s.ns;
+    p.status[k] = st;
+}
+
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+__global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
+    const unsigned lane = __lane_id();
+    const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    LaneState s;
+    s.k = -1;
+    bool exhausted = false;
+    for (;;) {
+        const bool want = s.k < 0 && !exhausted;
+        const u64 need = __ballot(want);
+        if (need != 0) {
+            const int count = __popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            u64 base = 0;
+            if ((int)lane == leader) base = atomicAdd(p.queue, (u64)count);
+            base = __shfl(base, leader);
+            if (want) {
+                const u64 m = base + (u64)__popcll(need & below);
+                if (m < (u64)p.n_active) {
+                    const i64 k = p.active[m];
+                    const i64 N = p.n;
+                    // an index that is not a trajectory is skipped (the lane asks again)
+                    if (k >= 0 && k < N) {
+                    s.k = k;
+                    s.x0 = p.x[k]; s.x1 = p.x[N + k]; s.x2 = p.x[2 * N + k];
+                    s.v0 = p.v[k]; s.v1 = p.v[N + k]; s.v2 = p.v[2 * N + k];
+                    s.qmk = p.qm[k];
+                    s.dqk = p.dq[k];
+                    s.slot = p.path_slot[k];
+                    if (s.slot >= p.n_paths) s.slot = -1;
+                    s.ns = p.steps[k];
+                    s.have = 0;
+                    s.ca0 = s.ca1 = s.ca2 = 0.0;
+                    s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
+                    if (s.ns >= p.max_steps) {  // no step: the loop of the reference ends at once
+                        store_end(p, s, 4);
+                        s.k = -1;
+                    }
+                    }
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(s.k >= 0) == 0) {
+            if (__ballot(!exhausted) == 0) break;
+            continue;
+        }
+        if (s.k >= 0) {
+            const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s);
+            if (st != -1) {
+                store_end(p, s, st);
+                s.k = -1;
+            } else if (s.ns >= p.max_steps) {
+                store_end(p, s, 4);
+                s.k = -1;
+            }
+        }
+    }
+}
+
+// zero the fixed-point accumulator and the queue head
+__global__ void prepare_kernel(u64* __restrict__ acc, i64 words, u64* __restrict__ queue) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < words; t += stride) acc[t] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0ull;
+}
+
+// charge[c] += the fixed-point total of node c
+__global__ void finalize_kernel(const u64* __restrict__ acc, i64 nodes, int e,
+                                double* __restrict__ charge) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < nodes; c += stride) {
+        const u64 lo = acc[2 * c];
+        const i64 hi = (i64)acc[2 * c + 1];
+        if (lo != 0 || hi != 0) charge[c] += fixed_value(lo, hi, e);
+    }
+}
+
+typedef void (*TraceKernel)(TraceParams);
+
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+static TraceKernel kernel_ptr() {
+    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL>;
+}
+
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general) {
+#define BT_SEL(I, G, B, X) \
+    if (integ == I && geom == G && has_b == B && general == X) return kernel_ptr<I, G, B, X>();
+#define BT_SEL_G(I, G) BT_SEL(I, G, false, false) BT_SEL(I, G, false, true) \
+                       BT_SEL(I, G, true, false) BT_SEL(I, G, true, true)
+    BT_SEL_G(0, 0) BT_SEL_G(0, 1) BT_SEL_G(0, 2)
+    BT_SEL_G(1, 0) BT_SEL_G(1, 1) BT_SEL_G(1, 2)
+#undef BT_SEL_G
+#undef BT_SEL
+    return nullptr;
+}
+
+int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
+                 bool has_b, bool general, i64 nodes) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general);
+    if (kern == nullptr) return -1;
+    const i64 words = 2 * nodes;
+    int zgrid = (int)((words + 255) / 256);
+    if (zgrid > 4096) zgrid = 4096;
+    if (zgrid < 1) zgrid = 1;
+    hipLaunchKernelGGL(prepare_kernel, dim3(zgrid), dim3(256), 0, stream, p.acc, words, p.queue);
+    if (p.n_active > 0 && grid > 0)
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, p);
+    int fgrid = (int)((nodes + 255) / 256);
+    if (fgrid > 4096) fgrid = 4096;
+    if (fgrid < 1) fgrid = 1;
+    hipLaunchKernelGGL(finalize_kernel, dim3(fgrid), dim3(256), 0, stream, p.acc, nodes,
+                       p.deposit_exponent, p.charge);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int trace_occupancy(int integ, int geom, bool has_b, bool general, int block) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general);
+    if (kern == nullptr) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),
+                                                     block, 0) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+}  // namespace bt

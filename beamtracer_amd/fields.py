@@ -1,0 +1,189 @@
+"""Fields on the host: the staggered edge fields of a potential, and node magnetic fields.
+
+* :func:`edge_field` restates ``solver/fields.py:18-67``: ``(phi[i-1] - phi[i]) / h`` on the
+  edges along one axis, taken over the bracketed surface distance next to an electrode,
+  extrapolated from the one adjacent surface on edges inside an electrode, and mirrored
+  (``"symmetric"``) or linearly extrapolated on the two end faces.
+* :class:`StaggeredElectricField` restates ``solver/fields.py:105-166``: ``fx`` ``(nx + 1, ny)``
+  and ``fy`` ``(nx, ny + 1)``; calling it gathers ``(E_x, E_y)`` with the hip
+  ``staggered_field_2d`` kernel.
+* :class:`AxisymmetricMagneticField` and :func:`solenoid_on_axis` restate
+  ``solver/fields.py:205-321`` (the paraxial expansion of an on-axis profile), and
+  :func:`node_magnetic_field` samples such a field onto the mesh nodes: the node field the
+  hip tracer gathers bilinearly.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Sequence
+
+import numpy as np
+
+from beamtracer_amd.mesh import IonOpticsMesh
+
+
+def edge_field(phi, solid, near, dist_lo, dist_hi, h, extrapolation) -> np.ndarray:
+    """The ``(n + 1, ...)`` edge field ``(phi[i-1] - phi[i]) / h`` along axis 0."""
+    n = phi.shape[0]
+    out = np.zeros((n + 1, *phi.shape[1:]))
+    column = (-1,) + (1,) * (phi.ndim - 1)
+    lengths = np.broadcast_to(np.asarray(h, dtype=float), (n - 1,)).reshape(column)
+    h = lengths
+    h_before = np.ones_like(lengths)
+    h_before[1:] = lengths[:-1]
+    h_after = np.ones_like(lengths)
+    h_after[:-1] = lengths[1:]
+    a, b = phi[:-1], phi[1:]
+    sa, sb = solid[:-1], solid[1:]
+    na, nb = near[:-1], near[1:]
+    field = (a - b) / h
+    case = na & sb
+    field = np.where(case, (a - b) / (dist_hi[:-1] * h), field)
+    case = nb & sa
+    field = np.where(case, (a - b) / (dist_lo[1:] * h), field)
+    both = sa & sb
+    near0 = np.zeros_like(na)
+    near0[1:] = near[:-2]
+    near3 = np.zeros_like(na)
+    near3[:-1] = near[2:]
+    phi0 = np.zeros_like(a)
+    phi0[1:] = phi[:-2]
+    phi3 = np.zeros_like(a)
+    phi3[:-1] = phi[2:]
+    d0 = np.ones_like(a)
+    d0[1:] = dist_hi[:-2]
+    d3 = np.ones_like(a)
+    d3[:-1] = dist_lo[2:]
+    inside = np.zeros_like(field)
+    inside = np.where(near0 & ~near3, (phi0 - a) / (d0 * h_before), inside)
+    inside = np.where(near3 & ~near0, (b - phi3) / (d3 * h_after), inside)
+    field = np.where(both, inside, field)
+    out[1:-1] = field
+    for end, (edge, first, second) in enumerate(((0, 1, 2), (n, n - 1, n - 2))):
+        if extrapolation[end] == "symmetric":
+            out[edge] = -out[first]
+        else:
+            out[edge] = 2.0 * out[first] - out[second]
+    return out
+
+
+class StaggeredElectricField:
+    """The electric field of a potential on the edges between nodes (``fx``, ``fy``)."""
+
+    def __init__(self, mesh: IonOpticsMesh, potential: np.ndarray, extrapolation: Sequence[str] | None = None):
+        if extrapolation is None:
+            extrapolation = ["extrapolate"] * 4
+            if mesh.cylindrical or mesh.symmetry_plane:
+                extrapolation[2] = "symmetric"
+        self.mesh = mesh
+        self.extrapolation = tuple(extrapolation)
+        phi = np.asarray(potential, dtype=float)
+        solid = mesh.solid >= 0
+        near = mesh.near_solid
+        self.fx = edge_field(phi, solid, near, mesh.distance[0], mesh.distance[1], mesh.dx, self.extrapolation[0:2])
+        self.fy = np.ascontiguousarray(
+            edge_field(phi.T, solid.T, near.T, mesh.distance[2].T, mesh.distance[3].T, mesh.h, self.extrapolation[2:4]).T
+        )
+
+    def __call__(self, x: np.ndarray, y: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """``(E_x, E_y)`` [V/m] at the points ``(x, y)``, gathered on the GPU."""
+        from beamtracer_amd.kernels import staggered_field_2d
+
+        mesh = self.mesh
+        x = np.ascontiguousarray(x, dtype=float).ravel()
+        y = np.ascontiguousarray(y, dtype=float).ravel()
+        ex, ey = np.empty_like(x), np.empty_like(y)
+        if x.size:
+            staggered_field_2d(
+                np.ascontiguousarray(mesh.logical_x(x)), y, self.fx, self.fy,
+                *map(float, mesh.logical_origin), mesh.h, ex, ey,
+            )  # fmt: skip
+        return ex, ey
+
+
+class AxisymmetricMagneticField:
+    """A static axisymmetric magnetic field from its on-axis profile ``B_z(z)``.
+
+    The power series in ``r`` up to ``order`` (at most 6) with 7-point finite-difference
+    derivatives of the sampled profile (``solver/fields.py:205-279``).
+    """
+
+    STENCILS = (
+        (0, 0, 0, 1, 0, 0, 0),
+        (-1 / 60, 9 / 60, -45 / 60, 0, 45 / 60, -9 / 60, 1 / 60),
+        (2 / 180, -27 / 180, 270 / 180, -490 / 180, 270 / 180, -27 / 180, 2 / 180),
+        (1 / 8, -8 / 8, 13 / 8, 0, -13 / 8, 8 / 8, -1 / 8),
+        (-1 / 6, 12 / 6, -39 / 6, 56 / 6, -39 / 6, 12 / 6, -1 / 6),
+        (-1 / 2, 4 / 2, -5 / 2, 0, 5 / 2, -4 / 2, 1 / 2),
+        (1, -6, 15, -20, 15, -6, 1),
+    )
+
+    def __init__(self, z0: float, hz: float, bz, order: int = 6):
+        samples = np.asarray(bz, dtype=float)
+        if samples.ndim != 1 or samples.size < 8:
+            raise ValueError("the on-axis profile needs at least 8 samples")
+        if not 0 <= order <= 6:
+            raise ValueError(f"order must be 0 to 6, got {order}")
+        self.z0, self.hz, self.order, self.bz = float(z0), float(hz), int(order), samples
+        n = samples.size
+        centre = np.arange(3, n - 3)
+        self._derivatives = np.zeros((self.order + 1, n))
+        for k, stencil in enumerate(self.STENCILS[: self.order + 1]):
+            d = sum(c * samples[centre + a - 3] for a, c in enumerate(stencil))
+            self._derivatives[k, centre] = d / self.hz**k
+        self._samples = np.ascontiguousarray(self._derivatives.T)
+
+    def derivatives(self, z: np.ndarray) -> np.ndarray:
+        n = self.bz.size
+        s = (np.asarray(z, dtype=float) - self.z0) / self.hz
+        i = np.clip(np.floor(s).astype(np.int64), 3, n - 5)
+        t = (s - i)[:, None]
+        low = self._samples[i]
+        return (low + t * (self._samples[i + 1] - low)).T
+
+    def __call__(self, z: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        d = self.derivatives(np.ravel(z))
+        r = np.ravel(np.asarray(r, dtype=float))
+        r2 = r * r
+        bz_c = (1.0, -1.0 / 4.0, 1.0 / 64.0, -1.0 / 2304.0)
+        br_c = (-1.0 / 2.0, 1.0 / 16.0, -1.0 / 384.0)
+        bz = d[0].copy()
+        br = np.zeros_like(bz)
+        power = np.ones_like(r2)
+        for k in range(1, self.order // 2 + 1):
+            power = power * r2
+            bz += bz_c[k] * power * d[2 * k]
+        power = r.copy()
+        for k in range((self.order + 1) // 2):
+            br += br_c[k] * power * d[2 * k + 1]
+            power = power * r2
+        return bz, br
+
+
+def solenoid_on_axis(z, centre: float, length: float, radius: float, peak: float) -> np.ndarray:
+    """On-axis ``B_z`` of a thin-walled solenoid scaled to ``peak`` at its centre (``fields.py:307-321``)."""
+    z = np.asarray(z, dtype=float)
+
+    def shape(zz):
+        a = zz - centre + 0.5 * length
+        b = zz - centre - 0.5 * length
+        return a / np.hypot(a, radius) - b / np.hypot(b, radius)
+
+    return peak * shape(z) / shape(np.asarray(centre))
+
+
+def current_loop_on_axis(z, centre: float, radius: float, peak: float) -> np.ndarray:
+    """On-axis ``B_z`` of a circular current loop, ``B0 a^3 / (a^2 + (z - c)^2)^{3/2}``."""
+    d = np.asarray(z, dtype=float) - centre
+    return peak * radius**3 / (radius * radius + d * d) ** 1.5
+
+
+def node_magnetic_field(mesh: IonOpticsMesh, magnetic) -> tuple[np.ndarray, np.ndarray, None]:
+    """Sample ``magnetic(x, r) -> (B_x, B_r)`` on the mesh nodes: ``(B_x, B_r, None)``."""
+    x, y = mesh.nodes()
+    bx, br = magnetic(x.ravel(), y.ravel())
+    return (
+        np.ascontiguousarray(np.reshape(bx, mesh.shape), dtype=float),
+        np.ascontiguousarray(np.reshape(br, mesh.shape), dtype=float),
+        None,
+    )

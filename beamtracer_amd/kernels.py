@@ -1,0 +1,307 @@
+"""The ``hip`` kernel backend: beamtracer's particle-stage kernels on MI355X.
+
+Each function takes exactly the arguments of the reference kernel it replaces
+(``src/beamtracer/kernels/<name>/<name>_pyccel.py``) and mutates its output arrays in
+place the same way:
+
+* :func:`staggered_field_2d` -- ``staggered_field_2d_pyccel.py:13-23``
+* :func:`deposit_trajectories_2d` -- ``deposit_trajectories_2d_pyccel.py:13-22``
+* :func:`trace_trajectories_2d` -- ``trace_trajectories_2d_pyccel.py:42-78``
+
+NumPy arrays run the synchronous host form of the C ABI (copies to and from the GPU);
+``torch`` tensors on the GPU run the device form on the current stream with no copies
+(the small ``patch_int``, ``patch_real`` and ``planes`` tables of the trace are read on
+the host and may be NumPy arrays or CPU tensors). There is no CPU fallback: without the
+built library every call raises :class:`beamtracer_amd._lib.HipLibraryError`.
+
+``KERNELS`` is the catalog by name, like ``kernels/catalog.py:39-44``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Any
+
+import numpy as np
+
+from beamtracer_amd import _lib
+
+KERNEL_NAMES = ("staggered_field_2d", "deposit_trajectories_2d", "trace_trajectories_2d")
+
+_I64 = np.int64
+_F64 = np.float64
+
+
+def _is_torch(a: Any) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _host(a: Any, dtype, name: str) -> np.ndarray:
+    """A C-contiguous NumPy array of ``dtype`` (the reference's Pyccel types)."""
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"{name} must be a numpy array, got {type(a).__name__}")
+    if a.dtype != dtype:
+        raise TypeError(f"{name} must be {np.dtype(dtype)}, got {a.dtype}")
+    if not a.flags.c_contiguous:
+        raise ValueError(f"{name} must be C-contiguous")
+    return a
+
+
+def _dev(a: Any, dtype, name: str):
+    """A contiguous CUDA (HIP) torch tensor of ``dtype``."""
+    import torch
+
+    want = {np.dtype(np.int64): torch.int64, np.dtype(np.float64): torch.float64}[np.dtype(dtype)]
+    if not (_is_torch(a) and a.is_cuda):
+        raise TypeError(f"{name} must be a GPU torch tensor like the other arrays")
+    if a.dtype != want:
+        raise TypeError(f"{name} must be {want}, got {a.dtype}")
+    if not a.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return a
+
+
+def _ptr(a: Any) -> int | None:
+    if a is None:
+        return None
+    if _is_torch(a):
+        return a.data_ptr() if a.numel() else None
+    return a.ctypes.data if a.size else None
+
+
+def _table(a: Any, dtype, name: str) -> np.ndarray:
+    """A small table read on the host: a NumPy array or a (CPU or GPU) tensor."""
+    if _is_torch(a):
+        a = a.detach().cpu().numpy()
+    return _host(np.ascontiguousarray(a, dtype=dtype), dtype, name)
+
+
+def _stream(stream: Any) -> int | None:
+    if stream is not None:
+        return int(getattr(stream, "cuda_stream", stream)) or None
+    import torch
+
+    return int(torch.cuda.current_stream().cuda_stream) or None
+
+
+def deposit_exponent(scale: float) -> int:
+    """The fixed-point exponent e (LSB 2^e) for deposits of typical size ``scale``."""
+    return int(_lib.lib().bt_deposit_exponent(float(scale)))
+
+
+def staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey, *, stream=None) -> None:
+    """``(E_x, E_y)`` at the points ``(px, py)`` from the edge fields ``fx``, ``fy``.
+
+    ``fx`` ``(nx + 1, ny)`` holds ``E_x`` on the x edges, ``fy`` ``(nx, ny + 1)`` ``E_y`` on
+    the y edges; each is interpolated bilinearly with its cell clamped to the array.
+    """
+    L = _lib.lib()
+    if _is_torch(px):
+        arrays = [_dev(a, _F64, n) for a, n in ((px, "px"), (py, "py"), (fx, "fx"), (fy, "fy"), (ex, "ex"), (ey, "ey"))]
+        px, py, fx, fy, ex, ey = arrays
+        nx, ny = fy.shape[0], fx.shape[1]
+        _shape_gather(px.shape, py.shape, tuple(fx.shape), tuple(fy.shape), ex.shape, ey.shape)
+        rc = L.bt_staggered_field_2d_device(
+            _ptr(px), _ptr(py), px.shape[0], _ptr(fx), _ptr(fy), nx, ny,
+            float(origin_x), float(origin_y), float(h), _ptr(ex), _ptr(ey), _stream(stream),
+        )  # fmt: skip
+    else:
+        for a, n in ((px, "px"), (py, "py"), (fx, "fx"), (fy, "fy"), (ex, "ex"), (ey, "ey")):
+            _host(a, _F64, n)
+        _shape_gather(px.shape, py.shape, fx.shape, fy.shape, ex.shape, ey.shape)
+        nx, ny = fy.shape[0], fx.shape[1]
+        rc = L.bt_staggered_field_2d(
+            _ptr(px), _ptr(py), px.shape[0], _ptr(fx), _ptr(fy), nx, ny,
+            float(origin_x), float(origin_y), float(h), _ptr(ex), _ptr(ey),
+        )  # fmt: skip
+    _lib.check(rc, "staggered_field_2d")
+
+
+def _shape_gather(spx, spy, sfx, sfy, sex, sey) -> None:
+    if len(spx) != 1 or spy != spx or sex != spx or sey != spx:
+        raise ValueError("px, py, ex and ey must be 1D arrays of one length")
+    if len(sfx) != 2 or len(sfy) != 2:
+        raise ValueError("fx and fy must be 2D")
+    nx, ny = sfy[0], sfx[1]
+    if sfx != (nx + 1, ny) or sfy != (nx, ny + 1):
+        raise ValueError(f"fx {sfx} and fy {sfy} must be (nx + 1, ny) and (nx, ny + 1)")
+
+
+def deposit_trajectories_2d(
+    px, py, q, origin_x, origin_y, h, cylindrical, charge, *, exponent=None, stream=None
+) -> None:
+    """Add the charges ``q`` at ``(px, py)`` to the node charges ``charge``, area-weighted.
+
+    Bilinear in planar geometry, linear in ``r**2`` along the radius in cylindrical geometry;
+    charge beyond the mesh goes to its boundary nodes. The sum is exact in 128-bit fixed
+    point with LSB ``2**exponent`` (default: 52 bits below ``max|q|``), so it does not depend
+    on the order of the charges.
+    """
+    L = _lib.lib()
+    if _is_torch(px):
+        px, py, q, charge = (_dev(a, _F64, n) for a, n in ((px, "px"), (py, "py"), (q, "q"), (charge, "charge")))
+        if px.dim() != 1 or py.shape != px.shape or q.shape != px.shape or charge.dim() != 2:
+            raise ValueError("px, py, q must be 1D of one length and charge 2D")
+        if exponent is None:
+            exponent = deposit_exponent(float(q.abs().max().item()) if q.numel() else 1.0)
+        nx, ny = charge.shape
+        rc = L.bt_deposit_trajectories_2d_device(
+            _ptr(px), _ptr(py), _ptr(q), px.shape[0], float(origin_x), float(origin_y), float(h),
+            int(cylindrical), _ptr(charge), nx, ny, int(exponent), None, 0, _stream(stream),
+        )  # fmt: skip
+    else:
+        for a, n in ((px, "px"), (py, "py"), (q, "q"), (charge, "charge")):
+            _host(a, _F64, n)
+        if px.ndim != 1 or py.shape != px.shape or q.shape != px.shape or charge.ndim != 2:
+            raise ValueError("px, py, q must be 1D of one length and charge 2D")
+        if exponent is None:
+            exponent = deposit_exponent(float(np.abs(q).max()) if q.size else 1.0)
+        nx, ny = charge.shape
+        rc = L.bt_deposit_trajectories_2d(
+            _ptr(px), _ptr(py), _ptr(q), px.shape[0], float(origin_x), float(origin_y), float(h),
+            int(cylindrical), _ptr(charge), nx, ny, int(exponent),
+        )  # fmt: skip
+    _lib.check(rc, "deposit_trajectories_2d")
+
+
+def trace_exponent(active, v, dq, qm, fxs, fys, patch_real, step_fraction, inv_c2=0.0) -> int:
+    """The deposit exponent of a trace: 52 bits below ``max|dq|`` times a typical step.
+
+    The step is ``step_fraction * min(h / v_max, sqrt(2 h / a_max))`` with the finest
+    spacing, the fastest start and the strongest field, as the step rule of
+    ``trace_trajectories_2d_pyccel.py:282-293``; it sets only the rounding grid of the
+    deposit, never a result of the trace.
+    """
+    if _is_torch(v):
+        import torch
+
+        idx = active.long()
+        if idx.numel() == 0:
+            return deposit_exponent(1.0)
+        u = v[:, idx]
+        speed = torch.sqrt((u * u).sum(0))
+        if inv_c2:
+            speed = speed / torch.sqrt(1.0 + (u * u).sum(0) * inv_c2)
+        vmax = float(speed.max().item())
+        dqmax = float(dq[idx].abs().max().item())
+        qmax = float(qm[idx].abs().max().item())
+        emax = max(float(fxs.abs().max().item()) if fxs.numel() else 0.0,
+                   float(fys.abs().max().item()) if fys.numel() else 0.0)
+    else:
+        if active.size == 0:
+            return deposit_exponent(1.0)
+        u = v[:, active]
+        speed = np.sqrt((u * u).sum(0))
+        if inv_c2:
+            speed = speed / np.sqrt(1.0 + (u * u).sum(0) * inv_c2)
+        vmax = float(speed.max())
+        dqmax = float(np.abs(dq[active]).max())
+        qmax = float(np.abs(qm[active]).max())
+        emax = max(float(np.abs(fxs).max()) if fxs.size else 0.0, float(np.abs(fys).max()) if fys.size else 0.0)
+    hmin = float(np.min(np.asarray(patch_real)[:, 2]))
+    dt = math.inf
+    if vmax > 0.0:
+        dt = hmin / vmax
+    if qmax * emax > 0.0:
+        dt = min(dt, math.sqrt(2.0 * hmin / (qmax * emax)))
+    if not math.isfinite(dt) or dt <= 0.0:
+        dt = hmin
+    scale = dqmax * step_fraction * dt
+    return deposit_exponent(scale if scale > 0.0 and math.isfinite(scale) else 1.0)
+
+
+def trace_trajectories_2d(
+    active, x, v, qm, dq, steps, status, electrode, fxs, fys, patch_int, patch_real,
+    origin_x, origin_y, h, upper_x, upper_y, cylindrical, integrator, step_fraction,
+    max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
+    path_count, num_threads, charge, graded, x_nodes, inv_c2,
+    *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
+) -> None:  # fmt: skip
+    """Trace the ``active`` trajectories until they end, or reach a surface.
+
+    The reference's 35 arguments with its meaning (``trace_trajectories_2d_pyccel.py:79-120``);
+    the whole deposit goes to ``charge[0]`` (as the reference's CUDA kernel), exact in
+    fixed point. ``magnetic`` (an extension) is None or three node arrays ``(nx, ny)`` of
+    the base mesh, ``(B_x, B_r, B_theta)`` cylindrical or ``(B_x, B_y, B_z)`` planar
+    (None entries are zero). ``block_size`` and ``max_blocks`` set the persistent launch
+    (0: defaults); ``exponent`` the fixed-point LSB ``2**exponent`` (default
+    :func:`trace_exponent`).
+    """
+    L = _lib.lib()
+    device = _is_torch(x)
+    pi = _table(patch_int, _I64, "patch_int")
+    pr = _table(patch_real, _F64, "patch_real")
+    pl = _table(planes, _F64, "planes")
+    if pi.ndim != 2 or pi.shape[1] != 6 or pr.shape != (pi.shape[0], 7):
+        raise ValueError("patch_int must be (P, 6) and patch_real (P, 7)")
+    check = _dev if device else _host
+    names_i = ("active", "steps", "status", "electrode", "cells", "subcells", "counts", "path_slot", "path_count")
+    names_f = ("x", "v", "qm", "dq", "fxs", "fys", "hits", "paths", "charge", "x_nodes")
+    vals = dict(active=active, steps=steps, status=status, electrode=electrode, cells=cells,
+                subcells=subcells, counts=counts, path_slot=path_slot, path_count=path_count,
+                x=x, v=v, qm=qm, dq=dq, fxs=fxs, fys=fys, hits=hits, paths=paths, charge=charge,
+                x_nodes=x_nodes)
+    for n in names_i:
+        check(vals[n], _I64, n)
+    for n in names_f:
+        check(vals[n], _F64, n)
+    n = x.shape[1]
+    if tuple(x.shape) != (3, n) or tuple(v.shape) != (3, n):
+        raise ValueError("x and v must be (3, N)")
+    for a, nm in ((qm, "qm"), (dq, "dq"), (steps, "steps"), (status, "status"), (electrode, "electrode"), (path_slot, "path_slot")):
+        if tuple(a.shape) != (n,):
+            raise ValueError(f"{nm} must be ({n},)")
+    if hits.ndim != 4 or tuple(hits.shape[:2]) != (n, pl.shape[0]) or hits.shape[3] != 8:
+        raise ValueError("hits must be (N, n_planes, capacity, 8)")
+    if tuple(counts.shape) != (n, pl.shape[0]):
+        raise ValueError("counts must be (N, n_planes)")
+    if paths.ndim != 3 or paths.shape[2] != 2 or tuple(path_count.shape) != (paths.shape[0],):
+        raise ValueError("paths must be (S, capacity, 2) and path_count (S,)")
+    if cells.ndim != 2 or charge.ndim != 2 or active.ndim != 1:
+        raise ValueError("cells and charge must be 2D, active 1D")
+    bx = by = bz = None
+    if magnetic is not None:
+        nxb, nyb = int(pi[-1, 0]), int(pi[-1, 1])
+        comps = []
+        for c in magnetic:
+            if c is None:
+                comps.append(None)
+                continue
+            check(c, _F64, "magnetic")
+            if tuple(c.shape) != (nxb, nyb):
+                raise ValueError(f"magnetic components must be ({nxb}, {nyb})")
+            comps.append(c)
+        bx, by, bz = comps
+    if exponent is None:
+        exponent = trace_exponent(active, v, dq, qm, fxs, fys, pr, float(step_fraction), float(inv_c2))
+    args = _lib.TraceArgs(
+        _ptr(active), active.shape[0], _ptr(x), _ptr(v), n, _ptr(qm), _ptr(dq),
+        _ptr(steps), _ptr(status), _ptr(electrode),
+        _ptr(fxs), fxs.shape[0], _ptr(fys), fys.shape[0],
+        pi.ctypes.data, pr.ctypes.data, pi.shape[0],
+        float(origin_x), float(origin_y), float(h), float(upper_x), float(upper_y),
+        int(cylindrical), int(integrator), float(step_fraction), int(max_steps),
+        _ptr(cells), cells.shape[0], cells.shape[1], _ptr(subcells), subcells.shape[0], int(refine),
+        pl.ctypes.data if pl.size else None, pl.shape[0],
+        _ptr(hits), hits.shape[2], _ptr(counts), _ptr(path_slot),
+        _ptr(paths), paths.shape[0], paths.shape[1], _ptr(path_count),
+        int(num_threads), _ptr(charge), charge.shape[0], charge.shape[1],
+        int(graded), _ptr(x_nodes), x_nodes.shape[0], float(inv_c2),
+        int(magnetic is not None), _ptr(bx), _ptr(by), _ptr(bz),
+        int(exponent), int(block_size), int(max_blocks), 0,
+        _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
+    )  # fmt: skip
+    if device:
+        rc = L.bt_trace_trajectories_2d_device(ctypes.byref(args), _stream(stream))
+    else:
+        rc = L.bt_trace_trajectories_2d(ctypes.byref(args))
+    _lib.check(rc, "trace_trajectories_2d")
+
+
+#: The kernels by name (``kernels/catalog.py:39-44``): the hip backend.
+KERNELS = {
+    "staggered_field_2d": staggered_field_2d,
+    "deposit_trajectories_2d": deposit_trajectories_2d,
+    "trace_trajectories_2d": trace_trajectories_2d,
+}

@@ -1,0 +1,233 @@
+"""Benchmark: particle-steps/s of the hip trace on the cylindrical einzel lens.
+
+Workload (one GPU): the einzel lens of 801 x 201 nodes over 80 x 20 mm (h = 0.1 mm), three
+coaxial tubes of 10 mm bore and 24 mm length with 4 mm gaps at 0, +6 kV and 0; 1e5 H+
+trajectories at 10 keV from r in [0, 3 mm] (20 A/m^2, transverse temperature 1 eV, Sobol
+spreads), RK4, step fraction 0.5, a diagnostic plane at 75 mm. The field is the vacuum
+solution of the host Poisson solve. A step is one trace of every trajectory from the
+emitter to its end (reset of the states, then one persistent launch), with all arrays
+resident in HBM. The metric counts particle-steps (the sum of every trajectory's steps).
+
+The line also carries the FP64 roofline of the trace kernel (algorithmic FLOPs per
+particle-step times the steps of one launch over its average launch time, HIP events on
+the launch stream) and the CPU baseline: the oracle's C restatement of the reference's
+Pyccel tracer (OpenMP over the host cores) on an evenly strided sample of the same
+trajectories, sized to about ten seconds.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s plain vs batched at 1e-10 rel err; adjoint/forward time ratio"
+#: Algorithmic FP64 work per RK4 step in cylindrical geometry: +, -, *, /, sqrt count 1,
+#: an fma 2; index arithmetic, compares and clamps 0 (derivation in DESIGN.md).
+FLOP_PER_STEP_RK4_CYL = 322
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(message: str) -> None:
+    print(message, file=sys.stderr, flush=True)
+
+
+def build_workload(particles: int):
+    from beamtracer_amd import designs, fields, poisson
+    from beamtracer_amd.tracer import HipBeamTracer
+
+    design = designs.einzel_lens(particles=particles)
+    solver = poisson.IonOpticsPoisson(design.mesh)
+    phi = solver.solve()
+    field = fields.StaggeredElectricField(design.mesh, phi)
+    tracer = HipBeamTracer(design.mesh, field, planes=design.planes, step_fraction=design.step_fraction,
+                           max_steps=design.max_steps, integrator=design.integrator)
+    return design, tracer
+
+
+def kernel_arguments(design, tracer, capacity: int = 4):
+    """The 35 host arrays and scalars of one plain trace of every trajectory."""
+    from beamtracer_amd.tracer import INTEGRATORS, SUBCELLS, geometry_code
+
+    mesh = design.mesh
+    p = design.particles
+    n = len(p)
+    x = np.array(p.position, dtype=float, order="C")
+    v = np.array(p.velocity, dtype=float, order="C")
+    qm = np.ascontiguousarray(p.charge / p.mass)
+    dq = np.ascontiguousarray(np.sign(p.charge) * np.abs(p.current))
+    fxs, fys, patch_int, patch_real = tracer.patch_arrays()
+    cells, subcells = tracer.electrode_lookup()
+    planes = np.array(design.planes, dtype=float)
+    return [
+        np.arange(n, dtype=np.int64), x, v, qm, dq, np.zeros(n, np.int64), np.full(n, -1, np.int64),
+        np.full(n, -1, np.int64), fxs, fys, patch_int, patch_real, *map(float, mesh.logical_origin), mesh.h,
+        *map(float, mesh.logical_upper), geometry_code(mesh), INTEGRATORS.index(design.integrator),
+        design.step_fraction, design.max_steps, cells, subcells, SUBCELLS, planes,
+        np.zeros((n, planes.size, capacity, 8)), np.zeros((n, planes.size), np.int64),
+        np.full(n, -1, np.int64), np.zeros((1, 2, 2)), np.ones(1, np.int64), 1,
+        np.zeros((1, int(patch_int[:, 4].sum()))), 0, np.zeros(2), 0.0,
+    ]  # fmt: skip
+
+
+def cpu_baseline(args_host, target_seconds: float) -> dict:
+    """The oracle's OpenMP tracer on a strided sample, timed on this host's cores."""
+    import oracle
+    from oracle import cases
+
+    threads = len(os.sched_getaffinity(0))
+    n = args_host[1].shape[1]
+
+    def run(count: int) -> tuple[float, int]:
+        index = np.linspace(0, n - 1, count).round().astype(np.int64)
+        a = cases.copy_args(args_host)
+        a[0] = index
+        a[30] = threads
+        a[31] = np.zeros((threads, a[31].shape[1]))
+        t0 = time.perf_counter()
+        oracle.trace_trajectories_2d(*a)
+        return time.perf_counter() - t0, int(a[5][index].sum())
+
+    count = min(n, 2 * threads)
+    seconds, steps = run(count)
+    while seconds < 0.5 and count < n:
+        count = min(n, count * 4)
+        seconds, steps = run(count)
+    if count < n:
+        scaled = int(count * target_seconds / max(seconds, 1e-6))
+        count = max(count, min(n, scaled))
+        seconds, steps = run(count)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": steps / seconds,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{count} of {n} trajectories (evenly strided), {steps} particle-steps in {seconds:.2f} s; "
+        f"oracle/oracle.c (restatement of trace_trajectories_2d_pyccel.py), OpenMP static(1) on {threads} "
+        f"threads of {model or 'the host CPU'}",
+    }
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--particles", type=int, default=100_000)
+    parser.add_argument("--block-size", type=int, default=0)
+    parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--cpu-seconds", type=float, default=10.0)
+    opts = parser.parse_args()
+    if opts.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from beamtracer_amd import kernels
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    log("building the einzel-lens workload (mesh, host Poisson, edge fields, emitter)")
+    design, tracer = build_workload(opts.particles)
+    host = kernel_arguments(design, tracer)
+    n = host[1].shape[1]
+    dev = [torch.from_numpy(a).cuda() if isinstance(a, np.ndarray) else a for a in host]
+    for i in (10, 11, 24):
+        dev[i] = host[i]  # patch tables and planes are read on the host
+    initial = {i: dev[i].clone() for i in (1, 2, 5, 6, 7, 25, 26, 31)}
+    exponent = kernels.trace_exponent(host[0], host[2], host[4], host[3], host[8], host[9], host[11], host[19])
+    stream = torch.cuda.current_stream()
+
+    def one_step() -> None:
+        for i, a in initial.items():
+            dev[i].copy_(a)
+        kernels.trace_trajectories_2d(*dev, exponent=exponent, block_size=opts.block_size, stream=stream)
+
+    log(f"warmup: {opts.warmup} traces of {n} trajectories")
+    for _ in range(opts.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    steps_per_trace = int(dev[5].sum().item())
+    status = torch.bincount(dev[6].clamp(min=-2) + 2, minlength=7).cpu().numpy()
+    log(f"particle-steps per trace {steps_per_trace}; status counts (-2..4) {status.tolist()}")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(opts.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(opts.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(opts.steps):
+        for i, a in initial.items():
+            dev[i].copy_(a)
+        starts[k].record(stream)
+        kernels.trace_trajectories_2d(*dev, exponent=exponent, block_size=opts.block_size, stream=stream)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    value = steps_per_trace * opts.steps / wall
+    log(f"timed: {opts.steps} traces in {wall:.3f} s; launch avg {avg_launch_s * 1e3:.3f} ms")
+
+    achieved = FLOP_PER_STEP_RK4_CYL * steps_per_trace / avg_launch_s / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": opts.steps,
+        "warmup": opts.warmup,
+        "ms_per_step": wall / opts.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "einzel lens, cylindrical 801x201 (h=0.1 mm), tubes 0/+6 kV/0, 1e5 H+ at 10 keV, "
+            "RK4, plain hip trace (one persistent launch per step), vacuum field",
+            "trajectories": n,
+            "particle_steps_per_trace": steps_per_trace,
+            "integrator": "rk4",
+            "geometry": "cylindrical",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "fp64 vector (no MFMA: nothing here is a contraction; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "flop_per_step": FLOP_PER_STEP_RK4_CYL,
+            "launch_ms": avg_launch_s * 1e3,
+        },
+    }
+    if not opts.no_cpu:
+        log("cpu baseline: oracle tracer on the host cores")
+        line["cpu_baseline"] = cpu_baseline(host, opts.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

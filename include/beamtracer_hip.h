@@ -1,0 +1,133 @@
+/*
+ * beamtracer_hip.h -- C ABI of the MI355X (gfx950) particle-stage kernels.
+ *
+ * The three kernels of beamtracer's particle stage, with the argument meaning of the
+ * reference kernels they replace (paths relative to the reference's src/beamtracer/):
+ *
+ *   bt_staggered_field_2d      kernels/staggered_field_2d/staggered_field_2d_pyccel.py:13-23
+ *   bt_deposit_trajectories_2d kernels/deposit_trajectories_2d/deposit_trajectories_2d_pyccel.py:13-22
+ *   bt_trace_trajectories_2d   kernels/trace_trajectories_2d/trace_trajectories_2d_pyccel.py:42-78
+ *                              (called from solver/tracer.py:1237-1247)
+ *
+ * Every entry point exists in two forms:
+ *   bt_<name>          host arrays (NumPy memory), synchronous: the drop-in for the Pyccel
+ *                      kernel.  Arrays are copied to the device and the outputs back.
+ *   bt_<name>_device   device arrays (hipMalloc / torch memory), asynchronous on `stream`
+ *                      (a hipStream_t, NULL for the null stream).
+ *
+ * Arrays are C-contiguous; int64_t where the reference has Pyccel `int`, double where it
+ * has `float`.  Outputs are mutated in place exactly as the reference kernels do, except
+ * that `charge` receives the whole deposit in its row 0 (as the reference's CUDA kernel,
+ * kernels/trace_trajectories_2d/trace_trajectories_2d_cuda.cu:1-4), summed in 128-bit
+ * fixed point with least significant bit 2^deposit_exponent, so the node charges do not
+ * depend on the order of the deposits.
+ *
+ * Return value: 0 on success, a negative BT_E_* code otherwise (never aborts);
+ * bt_last_error() describes the last failure of the calling thread.
+ */
+#ifndef BEAMTRACER_HIP_H
+#define BEAMTRACER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BT_OK 0
+#define BT_E_ARG (-1)        /* an argument is inconsistent (shape, count, enum)       */
+#define BT_E_NOMEM (-2)      /* a device or host allocation failed                     */
+#define BT_E_LAUNCH (-3)     /* a kernel launch or a HIP runtime call failed           */
+#define BT_E_DEVICE (-4)     /* no usable HIP device                                   */
+#define BT_E_UNSUPPORTED (-5)
+
+/* Every argument of trace_trajectories_2d (trace_trajectories_2d_pyccel.py:42-78) as a
+ * pointer with its extents, then the node magnetic field that the reference's compiled
+ * kernel lacks (its NumPy path takes B, solver/tracer.py:519-529, 605-650), then launch
+ * and workspace controls. */
+typedef struct bt_trace2d_args {
+    const int64_t* active; int64_t n_active;          /* active (n_active,)          */
+    double* x; double* v; int64_t n;                  /* x, v (3, n)                 */
+    const double* qm; const double* dq;               /* (n,)                        */
+    int64_t* steps; int64_t* status; int64_t* electrode;  /* (n,)                    */
+    const double* fxs; int64_t n_fxs;                 /* flat fx of every patch      */
+    const double* fys; int64_t n_fys;                 /* flat fy of every patch      */
+    const int64_t* patch_int; const double* patch_real; int64_t n_patches; /* (P,6),(P,7) */
+    double origin_x, origin_y, h, upper_x, upper_y;
+    int64_t cylindrical;                              /* 0 planar, 1 (x,r), 2 mirror */
+    int64_t integrator;                               /* 0 RK4, 1 Boris              */
+    double step_fraction;
+    int64_t max_steps;
+    const int64_t* cells; int64_t cells_rows, cells_cols;   /* (nx-1, ny-1)          */
+    const int64_t* subcells; int64_t n_subcells; int64_t refine;
+    const double* planes; int64_t n_planes;
+    double* hits; int64_t capacity;                   /* (n, n_planes, capacity, 8)  */
+    int64_t* counts;                                  /* (n, n_planes)               */
+    const int64_t* path_slot;                         /* (n,)                        */
+    double* paths; int64_t n_paths; int64_t path_capacity;   /* (n_paths, cap, 2)    */
+    int64_t* path_count;                              /* (n_paths,)                  */
+    int64_t num_threads;                              /* ignored (one slab)          */
+    double* charge; int64_t charge_rows, charge_cols; /* row 0 receives the deposit  */
+    int64_t graded; const double* x_nodes; int64_t n_x_nodes;
+    double inv_c2;
+    /* extension: a node magnetic field on the base (last) patch's nodes (nx, ny),
+     * (B_x, B_r, B_theta) cylindrical or (B_x, B_y, B_z) planar; NULL components are 0 */
+    int64_t has_b; const double* b0; const double* b1; const double* b2;
+    /* controls */
+    int32_t deposit_exponent;   /* fixed-point LSB 2^e of the deposit; see bt_deposit_exponent */
+    int32_t block_size;         /* threads per block (64..1024, a multiple of 64); 0 = tuned default */
+    int32_t max_blocks;         /* persistent grid size; 0 = fill the device            */
+    int32_t flags;              /* reserved, 0                                          */
+    void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned */
+    int64_t workspace_bytes;
+} bt_trace2d_args;
+
+/* Library version string. */
+const char* bt_version(void);
+/* Description of the last error on the calling thread ("" if none). */
+const char* bt_last_error(void);
+/* sizeof(bt_trace2d_args), for binding checks. */
+int64_t bt_trace_args_size(void);
+
+/* The fixed-point exponent e (LSB = 2^e) for deposits whose typical magnitude is
+ * `scale` (e.g. max|dq| times a typical step, or max|q|): 52 bits below it. */
+int32_t bt_deposit_exponent(double scale);
+
+/* Device scratch needed by a trace or a deposit onto `n_nodes` nodes. */
+int64_t bt_trace_workspace_bytes(int64_t n_nodes);
+
+/* trace_trajectories_2d: replaces the kernel call of solver/tracer.py:1237-1247. */
+int bt_trace_trajectories_2d(const bt_trace2d_args* args);
+int bt_trace_trajectories_2d_device(const bt_trace2d_args* args, void* stream);
+
+/* staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey):
+ * fx (nx + 1, ny), fy (nx, ny + 1); replaces staggered_field_2d_pyccel.py:13-55 as called
+ * from solver/fields.py:162-165. */
+int bt_staggered_field_2d(const double* px, const double* py, int64_t n,
+                          const double* fx, const double* fy, int64_t nx, int64_t ny,
+                          double origin_x, double origin_y, double h,
+                          double* ex, double* ey);
+int bt_staggered_field_2d_device(const double* px, const double* py, int64_t n,
+                                 const double* fx, const double* fy, int64_t nx, int64_t ny,
+                                 double origin_x, double origin_y, double h,
+                                 double* ex, double* ey, void* stream);
+
+/* deposit_trajectories_2d(px, py, q, origin_x, origin_y, h, cylindrical, charge):
+ * charge (nx, ny) += the area-weighted deposit; replaces
+ * deposit_trajectories_2d_pyccel.py:13-59 as called from solver/tracer.py:812-820. */
+int bt_deposit_trajectories_2d(const double* px, const double* py, const double* q,
+                               int64_t n, double origin_x, double origin_y, double h,
+                               int64_t cylindrical, double* charge, int64_t nx, int64_t ny,
+                               int32_t deposit_exponent);
+int bt_deposit_trajectories_2d_device(const double* px, const double* py, const double* q,
+                                      int64_t n, double origin_x, double origin_y, double h,
+                                      int64_t cylindrical, double* charge, int64_t nx,
+                                      int64_t ny, int32_t deposit_exponent,
+                                      void* workspace, int64_t workspace_bytes,
+                                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAMTRACER_HIP_H */

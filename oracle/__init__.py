@@ -1,0 +1,171 @@
+"""ORACLE -- test infrastructure only.
+
+CPU restatements of beamtracer's particle-stage kernels, used as the checker by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``. The product
+package ``beamtracer_amd`` never imports this module.
+
+* ``oracle.c`` (built into ``oracle/liboracle.so`` by :func:`build`) follows the Pyccel
+  kernels statement by statement: ``staggered_field_2d_pyccel.py:13-55``,
+  ``deposit_trajectories_2d_pyccel.py:13-59`` and
+  ``trace_trajectories_2d_pyccel.py:19-507`` (reference paths under
+  ``src/beamtracer/kernels/``).
+* ``numpy_ref.py`` restates the vectorized NumPy versions
+  (``staggered_field_2d_numpy.py:12-39``, ``deposit_trajectories_2d_numpy.py:12-42``) and
+  the step-synchronous NumPy tracer of ``solver/tracer.py:851-1054`` as an independent
+  second witness of the fused C tracer.
+
+Parity pins are in ``tests/test_oracle_*.py``: closed forms, invariants and brute force.
+Parity of the relativistic Lorentz factor and of the Boris rotation with mpiparticles (an
+un-vendored dependency) is pinned by invariants only, not bit for bit.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_ptr = ctypes.c_void_p
+
+
+def build(force: bool = True) -> Path:
+    """Compile ``oracle.c`` into ``liboracle.so`` (gcc, OpenMP, no FMA contraction)."""
+    if force or not LIB_PATH.exists():
+        cmd = [
+            "gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+            "-fno-fast-math", "-o", str(LIB_PATH), str(HERE / "oracle.c"), "-lm",
+        ]  # fmt: skip
+        subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+class TraceArgs(ctypes.Structure):
+    """ctypes mirror of ``oracle_trace_args`` in ``oracle.c``."""
+
+    _fields_ = [
+        ("active", _ptr), ("n_active", _i64),
+        ("x", _ptr), ("v", _ptr), ("n", _i64),
+        ("qm", _ptr), ("dq", _ptr),
+        ("steps", _ptr), ("status", _ptr), ("electrode", _ptr),
+        ("fxs", _ptr), ("fys", _ptr),
+        ("patch_int", _ptr), ("patch_real", _ptr), ("n_patches", _i64),
+        ("origin_x", _dbl), ("origin_y", _dbl), ("h", _dbl), ("upper_x", _dbl),
+        ("upper_y", _dbl),
+        ("cylindrical", _i64), ("integrator", _i64),
+        ("step_fraction", _dbl),
+        ("max_steps", _i64),
+        ("cells", _ptr), ("cells_cols", _i64),
+        ("subcells", _ptr), ("refine", _i64),
+        ("planes", _ptr), ("n_planes", _i64),
+        ("hits", _ptr), ("capacity", _i64),
+        ("counts", _ptr),
+        ("path_slot", _ptr), ("paths", _ptr), ("path_capacity", _i64), ("path_count", _ptr),
+        ("num_threads", _i64), ("charge", _ptr), ("charge_stride", _i64),
+        ("graded", _i64), ("x_nodes", _ptr), ("n_x_nodes", _i64),
+        ("inv_c2", _dbl),
+        ("has_b", _i64), ("b0", _ptr), ("b1", _ptr), ("b2", _ptr),
+    ]  # fmt: skip
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded oracle library (built on first use if missing)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        _LIB = ctypes.CDLL(str(LIB_PATH))
+        _LIB.oracle_trace_args_size.restype = _i64
+        if _LIB.oracle_trace_args_size() != ctypes.sizeof(TraceArgs):
+            raise RuntimeError("oracle_trace_args layout mismatch")
+    return _LIB
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_ptr)
+
+
+def _check(a: np.ndarray, dtype, name: str) -> np.ndarray:
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous {np.dtype(dtype)} array")
+    return a
+
+
+def staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey) -> None:
+    """``(E_x, E_y)`` at ``(px, py)`` from the edge fields (reference arguments)."""
+    for name, a in (("px", px), ("py", py), ("fx", fx), ("fy", fy), ("ex", ex), ("ey", ey)):
+        _check(a, np.float64, name)
+    nx, ny = fy.shape[0], fx.shape[1]
+    lib().oracle_staggered_field_2d(
+        _p(px), _p(py), _i64(px.shape[0]), _p(fx), _p(fy), _i64(nx), _i64(ny),
+        _dbl(origin_x), _dbl(origin_y), _dbl(h), _p(ex), _p(ey),
+    )  # fmt: skip
+
+
+def deposit_trajectories_2d(px, py, q, origin_x, origin_y, h, cylindrical, charge) -> None:
+    """Add the charges ``q`` at ``(px, py)`` to ``charge`` (reference arguments)."""
+    for name, a in (("px", px), ("py", py), ("q", q), ("charge", charge)):
+        _check(a, np.float64, name)
+    nx, ny = charge.shape
+    lib().oracle_deposit_trajectories_2d(
+        _p(px), _p(py), _p(q), _i64(px.shape[0]), _dbl(origin_x), _dbl(origin_y),
+        _dbl(h), _i64(int(cylindrical)), _p(charge), _i64(nx), _i64(ny),
+    )  # fmt: skip
+
+
+def trace_trajectories_2d(
+    active, x, v, qm, dq, steps, status, electrode, fxs, fys, patch_int, patch_real,
+    origin_x, origin_y, h, upper_x, upper_y, cylindrical, integrator, step_fraction,
+    max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
+    path_count, num_threads, charge, graded, x_nodes, inv_c2, magnetic=None,
+) -> None:  # fmt: skip
+    """The fused tracer with the reference's 35 arguments (+ an optional node field).
+
+    ``magnetic`` is None or a tuple of three ``(nx, ny)`` node arrays (or None entries):
+    ``(B_x, B_r, B_theta)`` cylindrical, ``(B_x, B_y, B_z)`` planar.
+    """
+    i64, f64 = np.int64, np.float64
+    for name, a, dt in (
+        ("active", active, i64), ("x", x, f64), ("v", v, f64), ("qm", qm, f64),
+        ("dq", dq, f64), ("steps", steps, i64), ("status", status, i64),
+        ("electrode", electrode, i64), ("fxs", fxs, f64), ("fys", fys, f64),
+        ("patch_int", patch_int, i64), ("patch_real", patch_real, f64),
+        ("cells", cells, i64), ("subcells", subcells, i64), ("planes", planes, f64),
+        ("hits", hits, f64), ("counts", counts, i64), ("path_slot", path_slot, i64),
+        ("paths", paths, f64), ("path_count", path_count, i64), ("charge", charge, f64),
+        ("x_nodes", x_nodes, f64),
+    ):  # fmt: skip
+        _check(a, dt, name)
+    b = [None, None, None]
+    if magnetic is not None:
+        b = [None if c is None else _check(np.ascontiguousarray(c, f64), f64, "B") for c in magnetic]
+    args = TraceArgs(
+        _p(active), active.shape[0], _p(x), _p(v), x.shape[1], _p(qm), _p(dq),
+        _p(steps), _p(status), _p(electrode), _p(fxs), _p(fys), _p(patch_int),
+        _p(patch_real), patch_int.shape[0], float(origin_x), float(origin_y), float(h),
+        float(upper_x), float(upper_y), int(cylindrical), int(integrator),
+        float(step_fraction), int(max_steps), _p(cells), cells.shape[1], _p(subcells),
+        int(refine), _p(planes), planes.shape[0], _p(hits), hits.shape[2], _p(counts),
+        _p(path_slot), _p(paths), paths.shape[1], _p(path_count), int(num_threads),
+        _p(charge), charge.shape[1], int(graded), _p(x_nodes), x_nodes.shape[0],
+        float(inv_c2), int(magnetic is not None), _p(b[0]), _p(b[1]), _p(b[2]),
+    )  # fmt: skip
+    lib().oracle_trace_trajectories_2d(ctypes.byref(args))
+
+
+def default_threads() -> int:
+    """Usable host cores (``OMP_NUM_THREADS`` first), as ``solver/tracer.py:52-60``."""
+    value = os.environ.get("OMP_NUM_THREADS", "").split(",")[0].strip()
+    if value.isdigit() and int(value) > 0:
+        return int(value)
+    return len(os.sched_getaffinity(0))

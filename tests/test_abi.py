@@ -1,0 +1,59 @@
+"""The C ABI library: it loads without a GPU and exports every symbol of the header."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+
+import pytest
+
+from beamtracer_amd import _lib
+
+
+def _declared() -> set[str]:
+    text = (_lib.ROOT / "include" / "beamtracer_hip.h").read_text()
+    return set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(bt_\w+)\s*\(", text, flags=re.M))
+
+
+def test_header_declares_the_kernels():
+    names = _declared()
+    for name in ("bt_trace_trajectories_2d", "bt_staggered_field_2d", "bt_deposit_trajectories_2d"):
+        assert name in names and f"{name}_device" in names
+
+
+def test_library_exports_every_declared_symbol():
+    handle = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [name for name in _declared() if not hasattr(handle, name)]
+    assert not missing
+    assert set(_lib.SIGNATURES) == _declared()
+
+
+def test_struct_layout_and_pure_helpers():
+    L = _lib.lib()
+    assert L.bt_trace_args_size() == ctypes.sizeof(_lib.TraceArgs)
+    assert L.bt_version().startswith(b"beamtracer_hip")
+    assert L.bt_deposit_exponent(1.0) == -52
+    assert L.bt_deposit_exponent(0.75) == -53
+    assert L.bt_trace_workspace_bytes(161001) >= 16 * 161001
+
+
+def test_argument_errors_are_raised_before_any_device_call():
+    import numpy as np
+
+    from beamtracer_amd import kernels
+
+    with pytest.raises(TypeError):
+        kernels.staggered_field_2d([0.0], [0.0], np.zeros((3, 2)), np.zeros((2, 3)), 0, 0, 1, np.zeros(1), np.zeros(1))
+    with pytest.raises(ValueError):
+        kernels.staggered_field_2d(
+            np.zeros(2), np.zeros(2), np.zeros((3, 2)), np.zeros((3, 3)), 0.0, 0.0, 1.0, np.zeros(2), np.zeros(2)
+        )
+    with pytest.raises(TypeError):
+        kernels.deposit_trajectories_2d(
+            np.zeros(2, np.float32), np.zeros(2), np.zeros(2), 0.0, 0.0, 1.0, 0, np.zeros((3, 3))
+        )
+    # a C-side check (h > 0) reports through bt_last_error as ValueError, with no GPU call
+    with pytest.raises(ValueError, match="h must be positive"):
+        kernels.staggered_field_2d(
+            np.zeros(2), np.zeros(2), np.zeros((3, 2)), np.zeros((2, 3)), 0.0, 0.0, 0.0, np.zeros(2), np.zeros(2)
+        )

@@ -1,0 +1,175 @@
+"""Parity of the HIP kernels with the oracle, through the C ABI (needs an MI355X).
+
+The inputs are the reference's own test cases, regenerated (``tests/unit/test_kernels.py:26-40``,
+``tests/unit/test_compiled_tracer.py:250-312``); the tolerances are the reference's
+(``rtol = atol = 1e-13`` for the gather and deposit, ``1e-12`` for the tracer outputs, the
+charge within ``1e-12`` of its total). The deposit is exact fixed point on the GPU, so its
+result is also checked to be independent of the order of the charges and of the launch shape.
+"""
+
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from beamtracer_amd import kernels
+
+    return kernels
+
+
+def _run(fn, args, **kw):
+    out = cases.copy_args(args)
+    fn(*out, **kw)
+    return out
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_gather_matches_oracle(hip, seed):
+    args = cases.gather_case(np.random.default_rng(seed))
+    expected = _run(oracle.staggered_field_2d, args)
+    result = _run(hip.staggered_field_2d, args)
+    for i in (7, 8):
+        np.testing.assert_allclose(result[i], expected[i], rtol=1e-13, atol=1e-13)
+
+
+def test_gather_large_and_empty(hip):
+    rng = np.random.default_rng(11)
+    nx, ny, h = 801, 201, 1e-4
+    fx, fy = rng.normal(size=(nx + 1, ny)), rng.normal(size=(nx, ny + 1))
+    n = 1_000_003
+    px = rng.uniform(-0.01, 0.09, n)
+    py = rng.uniform(-0.005, 0.025, n)
+    args = [px, py, fx, fy, 0.0, 0.0, h, np.zeros(n), np.zeros(n)]
+    expected = _run(oracle.staggered_field_2d, args)
+    result = _run(hip.staggered_field_2d, args)
+    np.testing.assert_allclose(result[7], expected[7], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(result[8], expected[8], rtol=1e-13, atol=1e-13)
+    empty = [np.zeros(0), np.zeros(0), fx, fy, 0.0, 0.0, h, np.zeros(0), np.zeros(0)]
+    hip.staggered_field_2d(*empty)
+
+
+@pytest.mark.parametrize("cylindrical", [0, 1])
+@pytest.mark.parametrize("seed", [4, 5])
+def test_deposit_matches_oracle(hip, seed, cylindrical):
+    args = cases.deposit_case(np.random.default_rng(seed), cylindrical)
+    expected = _run(oracle.deposit_trajectories_2d, args)
+    result = _run(hip.deposit_trajectories_2d, args)
+    np.testing.assert_allclose(result[7], expected[7], rtol=1e-13, atol=1e-13)
+
+
+def test_deposit_conserves_charge_and_adds(hip):
+    args = cases.deposit_case(np.random.default_rng(6))
+    args[7] = np.full((9, 7), 0.5)  # the deposit adds to what is there
+    result = _run(hip.deposit_trajectories_2d, args)
+    assert result[7].sum() - 0.5 * 63 == pytest.approx(args[2].sum(), abs=1e-12)
+
+
+def test_deposit_is_independent_of_order(hip):
+    rng = np.random.default_rng(12)
+    n, nx, ny, h = 400_000, 201, 51, 1e-3
+    px = rng.uniform(-0.01, 0.21, n)
+    py = rng.uniform(0.0, 0.052, n)
+    q = rng.normal(size=n) * 1e-9
+    base = np.zeros((nx, ny))
+    first = base.copy()
+    hip.deposit_trajectories_2d(px, py, q, 0.0, 0.0, h, 1, first)
+    perm = rng.permutation(n)
+    second = base.copy()
+    hip.deposit_trajectories_2d(
+        np.ascontiguousarray(px[perm]), np.ascontiguousarray(py[perm]),
+        np.ascontiguousarray(q[perm]), 0.0, 0.0, h, 1, second,
+    )  # fmt: skip
+    assert np.array_equal(first, second)  # bit for bit
+    expected = base.copy()
+    oracle.deposit_trajectories_2d(px, py, q, 0.0, 0.0, h, 1, expected)
+    total = np.abs(q).sum()
+    assert np.abs(first - expected).max() <= 1e-12 * total
+
+
+KERNEL_CASES = list(itertools.product([0, 1, 2], [0, 1], [False, True], [False, True], [False, True]))
+
+
+@pytest.mark.parametrize("cylindrical,integrator,refined,graded,relativistic", KERNEL_CASES)
+def test_trace_matches_oracle(hip, cylindrical, integrator, refined, graded, relativistic):
+    args = cases.kernel_case(cylindrical, integrator, refined, graded, relativistic)
+    expected = _run(oracle.trace_trajectories_2d, args)
+    assert set(np.unique(expected[6])) >= {-2, 0}
+    assert expected[29].max() > 30
+    gpu_args = cases.copy_args(args)
+    gpu_args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])  # one slab, as the CUDA kernel
+    hip.trace_trajectories_2d(*gpu_args)
+    for i in cases.OUTPUTS:
+        np.testing.assert_allclose(gpu_args[i], expected[i], rtol=1e-12, atol=1e-12)
+    want = expected[cases.CHARGE].sum(0)
+    got = gpu_args[cases.CHARGE][0]
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-18)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).sum()
+
+
+def _magnetic(args, scale=0.3):
+    pi = args[10]
+    nx, ny = int(pi[-1, 0]), int(pi[-1, 1])
+    i = np.arange(nx)[:, None] / (nx - 1)
+    j = np.arange(ny)[None, :] / (ny - 1)
+    return (
+        scale * np.exp(-((i - 0.4) / 0.2) ** 2) * np.ones_like(j),
+        -0.2 * scale * j * (i - 0.4),
+        0.05 * scale * np.cos(3 * i) * j,
+    )
+
+
+@pytest.mark.parametrize("cylindrical,integrator,general", list(itertools.product([0, 1, 2], [0, 1], [False, True])))
+def test_trace_with_magnetic_field_matches_oracle(hip, cylindrical, integrator, general):
+    args = cases.kernel_case(cylindrical, integrator, refined=general, graded=general, relativistic=general)
+    b = _magnetic(args)
+    expected = cases.copy_args(args)
+    oracle.trace_trajectories_2d(*expected, magnetic=b)
+    gpu_args = cases.copy_args(args)
+    gpu_args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    hip.trace_trajectories_2d(*gpu_args, magnetic=b)
+    for i in cases.OUTPUTS:
+        np.testing.assert_allclose(gpu_args[i], expected[i], rtol=1e-12, atol=1e-12)
+    want = expected[cases.CHARGE].sum(0)
+    assert np.abs(gpu_args[cases.CHARGE][0] - want).max() <= 1e-12 * np.abs(want).sum()
+
+
+def test_trace_device_form_and_launch_shapes_agree(hip):
+    import torch
+
+    args = cases.kernel_case(1, 0, refined=True)
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    host = _run(hip.trace_trajectories_2d, args)
+    for block, blocks in ((64, 0), (128, 1), (256, 3)):
+        dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+        hip.trace_trajectories_2d(*dev, block_size=block, max_blocks=blocks)
+        torch.cuda.synchronize()
+        for i in (*cases.OUTPUTS, cases.CHARGE):
+            assert np.array_equal(dev[i].cpu().numpy(), host[i]), (block, blocks, i)
+
+
+def test_trace_without_active_and_at_max_steps(hip):
+    args = cases.kernel_case(1, 0)
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    none = cases.copy_args(args)
+    none[0] = np.zeros(0, dtype=np.int64)
+    hip.trace_trajectories_2d(*none)
+    for i in cases.OUTPUTS:
+        assert np.array_equal(none[i], args[i])
+    done = cases.copy_args(args)
+    done[5][:] = 400  # already at max_steps: no step, status 4
+    expected = cases.copy_args(done)
+    oracle.trace_trajectories_2d(*expected)
+    hip.trace_trajectories_2d(*done)
+    assert (done[6] == 4).all()
+    for i in cases.OUTPUTS:
+        assert np.array_equal(done[i], expected[i])
