@@ -90,7 +90,9 @@ int ensure(Buffer& b, size_t bytes) {
     return BT_OK;
 }
 
-size_t workspace_bytes(i64 nodes) { return round256((size_t)(16 * nodes)) + 256 + kTableBytes; }
+// The node words of a deposit: 2 words of 8 bytes per node.
+size_t node_bytes(i64 nodes) { return round256((size_t)(16 * nodes)); }
+size_t workspace_bytes(i64 nodes) { return node_bytes(nodes) + 256 + kTableBytes; }
 
 // The fixed launch-independent checks of a trace (host-readable tables only).
 int check_trace(const bt_trace2d_args* a) {
@@ -173,8 +175,8 @@ int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t
     const i64 nodes = trace_nodes(a);
     char* base = static_cast<char*>(ws);
     p.acc = reinterpret_cast<u64*>(base);
-    p.queue = reinterpret_cast<u64*>(base + round256((size_t)(16 * nodes)));
-    char* tables = base + round256((size_t)(16 * nodes)) + 256;
+    p.queue = reinterpret_cast<u64*>(base + node_bytes(nodes));
+    char* tables = base + node_bytes(nodes) + 256;
     const size_t pi_bytes = (size_t)a->n_patches * 6 * 8, pr_bytes = (size_t)a->n_patches * 7 * 8;
     i64* d_pi = reinterpret_cast<i64*>(tables);
     double* d_pr = reinterpret_cast<double*>(tables + pi_bytes);
@@ -190,7 +192,7 @@ int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const bool has_b = a->has_b != 0;
     const bool general = a->n_patches > 1 || a->graded != 0 || a->inv_c2 != 0.0;
-    int block = a->block_size > 0 ? a->block_size : 64;
+    int block = a->block_size > 0 ? a->block_size : 256;
     if (block % 64 != 0 || block > 256) return fail(BT_E_ARG, "block_size must be a multiple of 64, at most 256");
     int grid = 0;
     if (a->n_active > 0) {

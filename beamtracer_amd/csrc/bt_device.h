@@ -66,28 +66,42 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
     return low + u * (high - low);
 }
 
-// ---- 128-bit fixed point ------------------------------------------------------------
-// A node holds {u64 lo, i64 hi}: the value (hi * 2^64 + lo) * 2^e in two's complement.
+// ---- fixed point ---------------------------------------------------------------------
+// A deposit is rounded once to an integer multiple q of 2^e (|q| < 2^94).  A node holds two
+// u64 words: w0 sums the low 32 bits of each q (as an unsigned number below 2^32) and w1
+// sums q >> 32 (arithmetic); the node's value is (w1 2^32 + w0) 2^e, exact in any order of
+// the additions, which are no-return integer atomics.  w0 stays exact for up to 2^31
+// deposits into one node.
+constexpr int kWords = 2;
 
-__device__ __forceinline__ void fixed_add(u64* __restrict__ acc, i64 node, double value, int e) {
+// The two words of q = round(value 2^-e); false for a zero, non-finite or out-of-range
+// deposit (dropped).
+__device__ __forceinline__ bool to_fixed(double value, int e, u64& w0, u64& w1) {
     const double s = rint(ldexp(value, -e));
-    if (!(fabs(s) < 0x1p126)) return;  // zero is handled below; NaN/overflow are dropped
-    u64 lo;
-    i64 hi;
+    if (!(fabs(s) < 0x1p94)) return false;
     if (fabs(s) < 0x1p63) {
         const i64 q = (i64)s;
-        if (q == 0) return;
-        lo = (u64)q;
-        hi = q < 0 ? -1 : 0;
-    } else {
-        const double hd = floor(s * 0x1p-64);
-        hi = (i64)hd;
-        lo = (u64)(s - hd * 0x1p64);  // exact: 0 <= s - hd 2^64 < 2^64
+        if (q == 0) return false;
+        w0 = (u64)q & 0xffffffffull;
+        w1 = (u64)(q >> 32);
+        return true;
     }
-    u64* p = acc + 2 * node;
-    const u64 old = atomicAdd(p, lo);
-    const i64 dh = hi + ((old + lo) < old ? 1 : 0);
-    if (dh != 0) atomicAdd(p + 1, (u64)dh);
+    const double hd = floor(s * 0x1p-32);  // exact: s is a multiple of 2^11 here
+    w1 = (u64)(i64)hd;
+    w0 = (u64)(s - hd * 0x1p32);           // exact, in [0, 2^32)
+    return true;
+}
+
+// Adds the words to a node's words in global memory (no-return atomics).
+__device__ __forceinline__ void add_words(u64* __restrict__ w, u64 w0, u64 w1) {
+    if (w0) __hip_atomic_fetch_add(w, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w1) __hip_atomic_fetch_add(w + 1, w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One deposit of `value` into `node` of the word array `acc`.
+__device__ __forceinline__ void fixed_add(u64* __restrict__ acc, i64 node, double value, int e) {
+    u64 w0, w1;
+    if (to_fixed(value, e, w0, w1)) add_words(acc + kWords * node, w0, w1);
 }
 
 // The node value as a double, rounded once (correctly) from the 128-bit integer.
@@ -115,6 +129,14 @@ __device__ __forceinline__ double fixed_value(u64 lo, i64 hi_signed, int e) {
         m = ldexp((double)(top | sticky), bits);
     }
     return ldexp(neg ? -m : m, e);
+}
+
+// The double of a node's two words: (w1 2^32 + w0) 2^e, rounded once.
+__device__ __forceinline__ double words_value(u64 w0, u64 w1, int e) {
+    const u64 lo = w0 + (w1 << 32);
+    const u64 carry = lo < w0 ? 1ull : 0ull;
+    const i64 hi = ((i64)w1 >> 32) + (i64)carry;
+    return fixed_value(lo, hi, e);
 }
 
 }  // namespace bt

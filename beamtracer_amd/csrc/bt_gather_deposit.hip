@@ -79,9 +79,8 @@ __global__ void zero_words(u64* __restrict__ acc, i64 words) {
 __global__ void add_fixed(const u64* __restrict__ acc, i64 nodes, int e, double* __restrict__ charge) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
     for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < nodes; c += stride) {
-        const u64 lo = acc[2 * c];
-        const i64 hi = (i64)acc[2 * c + 1];
-        if (lo != 0 || hi != 0) charge[c] += fixed_value(lo, hi, e);
+        const u64 w0 = acc[kWords * c], w1 = acc[kWords * c + 1];
+        if (w0 != 0 || w1 != 0) charge[c] += words_value(w0, w1, e);
     }
 }
 
@@ -105,8 +104,8 @@ int launch_deposit(const double* px, const double* py, const double* q, i64 n, d
                    double oy, double h, i64 cylindrical, double* charge, i64 nx, i64 ny, int e,
                    u64* acc, hipStream_t stream) {
     const i64 nodes = nx * ny;
-    hipLaunchKernelGGL(zero_words, dim3(grid_for(2 * nodes, 256, 4096)), dim3(256), 0, stream, acc,
-                       2 * nodes);
+    hipLaunchKernelGGL(zero_words, dim3(grid_for(kWords * nodes, 256, 4096)), dim3(256), 0, stream, acc,
+                       kWords * nodes);
     if (n > 0)
         hipLaunchKernelGGL(deposit_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, stream, px,
                            py, q, n, ox, oy, h, cylindrical, nx, ny, e, acc);

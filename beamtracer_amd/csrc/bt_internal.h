@@ -33,7 +33,7 @@ struct TraceParams {
     i64 bofx, bofy, bofn;
     double box, boy, bh;
     // deposit accumulator and work queue (workspace)
-    u64* acc; int deposit_exponent; u64* queue;
+    u64* acc; int deposit_exponent; u64* queue;   // node words, fixed-point exponent, work queue
     double* charge;
 };
 

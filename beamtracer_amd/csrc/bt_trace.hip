@@ -18,8 +18,12 @@
 // * state in registers (x, v component rows are coalesced loads / stores);
 // * the FP64 edge fields (2.6 MB for an 801 x 201 mesh) stay resident in each XCD's 4 MB
 //   L2 and are read with plain loads;
-// * the deposit goes into a 128-bit fixed-point node array with integer atomics
-//   (deterministic), converted and added to charge row 0 by a finalize kernel;
+// * the deposit is exact fixed point (deterministic): each workgroup sums its deposits in
+//   an LDS window of node sums that follows its beam (integer LDS atomics) and adds a
+//   node's sum to global memory once when the window leaves it.  Global integer atomics
+//   on this GPU are executed at the memory side, where same-node adds from every wave
+//   serialize; the window turns one atomic per deposit into one per workgroup and node.
+//   A finalize kernel converts and adds to charge row 0;
 // * compile-time variants on integrator, geometry, magnetic field and the general
 //   features (refined patches, graded axis, relativistic motion), so the common case
 //   carries no dead branches.
@@ -159,9 +163,131 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, double q0, doub
     ey = clamped_bilinear(p.fys + v.ofy, v.ny + 1, sx, sy + 0.5, (int)v.nx - 2, (int)v.ny - 1);
 }
 
+// ---- the deposit window ---------------------------------------------------------------
+// The workgroup's deposits go into a window of node sums in LDS that follows its beam: a
+// ring of kWinCols column slots (column i in slot i mod kWinCols), each holding kWinRows
+// rows from a base row.  Lanes add with LDS integer atomics; a deposit outside the window
+// goes to global memory directly.  Between steps the workgroup flushes the slots of
+// columns it has left (one global atomic per word of each node) and claims the columns
+// around its current cells.  All waves of the workgroup step in lockstep: deposits and
+// flushes are separated by barriers, and the slot tables are double-buffered (deposits of
+// step k read table k mod 2 while the maintenance after step k writes table k + 1 mod 2),
+// so no deposit can land in a slot that is being flushed.
+
+constexpr int kWinCols = 16;
+constexpr int kWinRows = 64;
+
+struct DepositWindow {
+    i64 tag[2][kWinCols];             // the column held by a slot, -1 for none
+    int base[2][kWinCols];            // its first row
+    int bounds[2][4];                 // min/max column and row of the step's deposits
+    u64 word[kWinCols][kWinRows][kWords];
+};
+
+__device__ __forceinline__ void window_reset_bounds(DepositWindow& w, int t) {
+    w.bounds[t][0] = 0x7fffffff;
+    w.bounds[t][1] = -1;
+    w.bounds[t][2] = 0x7fffffff;
+    w.bounds[t][3] = -1;
+}
+
+__device__ __forceinline__ void window_init(DepositWindow& w) {
+    for (int k = threadIdx.x; k < kWinCols * kWinRows * kWords; k += blockDim.x)
+        (&w.word[0][0][0])[k] = 0ull;
+    if (threadIdx.x < kWinCols) {
+        w.tag[0][threadIdx.x] = -1;
+        w.tag[1][threadIdx.x] = -1;
+        w.base[0][threadIdx.x] = 0;
+        w.base[1][threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) {
+        window_reset_bounds(w, 0);
+        window_reset_bounds(w, 1);
+    }
+}
+
+// The 4-node deposit of one step: nodes (i, j), (i, j + 1), (i + 1, j), (i + 1, j + 1) of a
+// patch whose nodes start at `ofn`; the window serves the base patch only (in_base).
+__device__ __forceinline__ void window_deposit(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn,
+                                               i64 ny, bool in_base, i64 i, i64 j,
+                                               const double (&value)[4], int e) {
+    if (in_base) {
+        atomicMin(&win.bounds[t][0], (int)i);
+        atomicMax(&win.bounds[t][1], (int)i);
+        atomicMin(&win.bounds[t][2], (int)j);
+        atomicMax(&win.bounds[t][3], (int)j);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        u64 w0, w1;
+        if (!to_fixed(value[q], e, w0, w1)) continue;
+        const i64 ii = i + (q >> 1), jj = j + (q & 1);
+        if (in_base) {
+            const int slot = (int)(ii & (kWinCols - 1));
+            const i64 row = jj - win.base[t][slot];
+            if (win.tag[t][slot] == ii && row >= 0 && row < kWinRows) {
+                if (w0) atomicAdd(&win.word[slot][row][0], w0);
+                if (w1) atomicAdd(&win.word[slot][row][1], w1);
+                continue;
+            }
+        }
+        add_words(acc + kWords * (ofn + ii * ny + jj), w0, w1);
+    }
+}
+
+// After the deposits of step k (table t = k mod 2): move the window to the columns around
+// the step's cells, flushing the slots that change, and write table 1 - t.
+__device__ __forceinline__ void window_maintain(DepositWindow& win, int t, u64* __restrict__ acc,
+                                                i64 ofn, i64 nx, i64 ny) {
+    const int n = 1 - t;
+    const int cmin = win.bounds[t][0], cmax = win.bounds[t][1];
+    const int rmin = win.bounds[t][2];
+    const bool moved = cmin <= cmax;
+    int lo = cmin - 1, hi = cmax + 2;
+    if (lo < 0) lo = 0;
+    if (hi > (int)nx - 1) hi = (int)nx - 1;
+    if (hi > lo + kWinCols - 1) hi = lo + kWinCols - 1;
+    int nbase = rmin - 4;
+    if (nbase > (int)ny - kWinRows) nbase = (int)ny - kWinRows;
+    if (nbase < 0) nbase = 0;
+    for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
+        const int slot = k / kWinRows, row = k % kWinRows;
+        const i64 old = win.tag[t][slot];
+        i64 want = old;
+        if (moved) {
+            const int col = lo + ((slot - lo) & (kWinCols - 1));
+            if (col <= hi) want = col;
+        }
+        if (want != old) {
+            const u64 w0 = win.word[slot][row][0], w1 = win.word[slot][row][1];
+            if (old >= 0 && (w0 | w1) != 0) {
+                const i64 jj = win.base[t][slot] + row;
+                add_words(acc + kWords * (ofn + old * ny + jj), w0, w1);
+            }
+            win.word[slot][row][0] = 0ull;
+            win.word[slot][row][1] = 0ull;
+        }
+        if (row == 0) {
+            win.tag[n][slot] = want;
+            win.base[n][slot] = want != old ? nbase : win.base[t][slot];
+        }
+    }
+    if (threadIdx.x == 0) window_reset_bounds(win, n);
+}
+
+// Every node sum still in the window (table t) to global memory.
+__device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn, i64 ny) {
+    for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
+        const int slot = k / kWinRows, row = k % kWinRows;
+        const i64 old = win.tag[t][slot];
+        const u64 w0 = win.word[slot][row][0], w1 = win.word[slot][row][1];
+        if (old >= 0 && (w0 | w1) != 0) add_words(acc + kWords * (ofn + old * ny + win.base[t][slot] + row), w0, w1);
+    }
+}
+
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
-__device__ __forceinline__ int step(const TraceParams& p, LaneState& s) {
+__device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositWindow& win, int t) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -328,12 +454,10 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s) {
         else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = pv.ny - 2; }
         else j = (i64)fj;
         const double deposit = s.dqk * dt;
-        const i64 c = pv.ofn + i * pv.ny + j;
-        const int e = p.deposit_exponent;
-        fixed_add(p.acc, c, (1.0 - u) * (1.0 - w) * deposit, e);
-        fixed_add(p.acc, c + 1, (1.0 - u) * w * deposit, e);
-        fixed_add(p.acc, c + pv.ny, u * (1.0 - w) * deposit, e);
-        fixed_add(p.acc, c + pv.ny + 1, u * w * deposit, e);
+        const double value[4] = {(1.0 - u) * (1.0 - w) * deposit, (1.0 - u) * w * deposit,
+                                 u * (1.0 - w) * deposit, u * w * deposit};
+        window_deposit(win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, i, j, value,
+                       p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
     s.ns = ns;
@@ -425,11 +549,14 @@ __device__ __forceinline__ void store_end(const TraceParams& p, const LaneState&
 
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
 __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
+    __shared__ DepositWindow win;
+    window_init(win);
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     LaneState s;
     s.k = -1;
     bool exhausted = false;
+    int t = 0;  // the window table of this step
     for (;;) {
         const bool want = s.k < 0 && !exhausted;
         const u64 need = __ballot(want);
@@ -467,12 +594,10 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
                 }
             }
         }
-        if (__ballot(s.k >= 0) == 0) {
-            if (__ballot(!exhausted) == 0) break;
-            continue;
-        }
+        // the whole workgroup goes on while any lane has or may get a trajectory
+        if (!__syncthreads_or(s.k >= 0 || !exhausted)) break;
         if (s.k >= 0) {
-            const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s);
+            const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s, win, t);
             if (st != -1) {
                 store_end(p, s, st);
                 s.k = -1;
@@ -481,7 +606,11 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
                 s.k = -1;
             }
         }
+        __syncthreads();
+        window_maintain(win, t, p.acc, p.bofn, p.nx, p.ny);
+        t = 1 - t;
     }
+    window_flush(win, t, p.acc, p.bofn, p.ny);
 }
 
 // zero the fixed-point accumulator and the queue head
@@ -492,17 +621,13 @@ __global__ void prepare_kernel(u64* __restrict__ acc, i64 words, u64* __restrict
 }
 
 // charge[c] += the fixed-point total of node c
-__global__ void finalize_kernel(const u64* __restrict__ acc, i64 nodes, int e,
-                                double* __restrict__ charge) {
+__global__ void finalize_kernel(const u64* __restrict__ acc, i64 nodes, int e, double* __restrict__ charge) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
     for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < nodes; c += stride) {
-        const u64 lo = acc[2 * c];
-        const i64 hi = (i64)acc[2 * c + 1];
-        if (lo != 0 || hi != 0) charge[c] += fixed_value(lo, hi, e);
+        const u64 w0 = acc[kWords * c], w1 = acc[kWords * c + 1];
+        if (w0 != 0 || w1 != 0) charge[c] += words_value(w0, w1, e);
     }
 }
-
-typedef void (*TraceKernel)(TraceParams);
 
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
 static TraceKernel kernel_ptr() {
@@ -525,7 +650,7 @@ int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, 
                  bool has_b, bool general, i64 nodes) {
     TraceKernel kern = select_trace_kernel(integ, geom, has_b, general);
     if (kern == nullptr) return -1;
-    const i64 words = 2 * nodes;
+    const i64 words = kWords * nodes;
     int zgrid = (int)((words + 255) / 256);
     if (zgrid > 4096) zgrid = 4096;
     if (zgrid < 1) zgrid = 1;

@@ -121,7 +121,8 @@ def _solenoid_case(integrator):
     n = 20
     x = np.stack([np.zeros(n), np.linspace(1e-3, 8e-3, n), np.zeros(n)])
     v = np.stack([np.full(n, 1e6), np.zeros(n), np.zeros(n)])
-    args = make_case(fx, fy, (0.0, 0.0), h, 0, integrator, x, v, qm=1.602e-19 / 1.67e-27, planes=[0.19],
+    # the reference's mesh takes IonOpticsMesh's default geometry: cylindrical
+    args = make_case(fx, fy, (0.0, 0.0), h, 1, integrator, x, v, qm=1.602e-19 / 1.67e-27, planes=[0.19],
                      step_fraction=0.25)  # fmt: skip
     return run(args, magnetic=magnetic), x
 
