@@ -118,6 +118,7 @@ int check_trace(const bt_trace2d_args* a) {
         if (nx < 2 || ny < 2) return fail(BT_E_ARG, "a patch has fewer than 2 nodes per axis");
         if (pi[2] < 0 || pi[2] + (nx + 1) * ny > a->n_fxs) return fail(BT_E_ARG, "fx offset beyond fxs");
         if (pi[3] < 0 || pi[3] + nx * (ny + 1) > a->n_fys) return fail(BT_E_ARG, "fy offset beyond fys");
+        if ((nx + 1) * (ny + 1) >= ((i64)1 << 29)) return fail(BT_E_ARG, "a patch holds 2^29 nodes or more");
         if (pi[5] < 0 || pi[5] + nx * ny > a->charge_cols)
             return fail(BT_E_ARG, "node offset beyond charge");
         if (pi[5] + nx * ny > nodes) nodes = pi[5] + nx * ny;
@@ -158,6 +159,7 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     p.bofx = base[2]; p.bofy = base[3]; p.bofn = base[5];
     p.box = basr[0]; p.boy = basr[1]; p.bh = basr[2];
     p.deposit_exponent = a->deposit_exponent;
+    p.reuse = a->n_patches == 1 && basr[0] == a->origin_x && basr[1] == a->origin_y && basr[2] == a->h;
     return p;
 }
 
@@ -249,7 +251,7 @@ int64_t bt_trace_args_size(void) { return (int64_t)sizeof(bt_trace2d_args); }
 
 int32_t bt_deposit_exponent(double scale) {
     if (!(scale > 0.0) || !std::isfinite(scale)) return -1074 + 60;
-    return (int32_t)std::ilogb(scale) - 52;
+    return (int32_t)std::ilogb(scale) - 48;
 }
 
 int64_t bt_trace_workspace_bytes(int64_t n_nodes) { return (int64_t)workspace_bytes(n_nodes); }

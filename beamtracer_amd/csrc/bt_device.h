@@ -59,8 +59,10 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
     const int j = clamped_floor(t, 0, jmax);
     const double u = s - (double)i;
     const double w = t - (double)j;
-    const i64 c = (i64)i * cols + j;
-    const double f00 = f[c], f01 = f[c + 1], f10 = f[c + cols], f11 = f[c + cols + 1];
+    // 32-bit offsets (the arrays hold fewer than 2^29 values): base + offset addressing
+    const unsigned c = (unsigned)i * (unsigned)cols + (unsigned)j;
+    const unsigned d = c + (unsigned)cols;
+    const double f00 = f[c], f01 = f[c + 1], f10 = f[d], f11 = f[d + 1];
     const double low = f00 + w * (f01 - f00);
     const double high = f10 + w * (f11 - f10);
     return low + u * (high - low);
@@ -74,10 +76,18 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
 // deposits into one node.
 constexpr int kWords = 2;
 
-// The two words of q = round(value 2^-e); false for a zero, non-finite or out-of-range
-// deposit (dropped).
+// The two words of q = round(value 2^-e) (to nearest, ties to even); false for a zero,
+// non-finite or out-of-range deposit (dropped).  Below 2^51 the rounding is that of adding
+// 1.5 2^52, whose bit pattern then holds q exactly.
 __device__ __forceinline__ bool to_fixed(double value, int e, u64& w0, u64& w1) {
-    const double s = rint(ldexp(value, -e));
+    const double t = ldexp(value, -e);
+    if (fabs(t) < 0x1p51) {
+        const i64 q = __double_as_longlong(t + 0x1.8p52) - 0x4338000000000000ll;
+        w0 = (u64)q & 0xffffffffull;
+        w1 = (u64)(q >> 32);
+        return q != 0;
+    }
+    const double s = rint(t);
     if (!(fabs(s) < 0x1p94)) return false;
     if (fabs(s) < 0x1p63) {
         const i64 q = (i64)s;

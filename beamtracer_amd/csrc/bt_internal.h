@@ -26,6 +26,7 @@ struct TraceParams {
     double* hits; i64 capacity; i64* counts;
     const i64* path_slot; double* paths; i64 n_paths; i64 path_capacity; i64* path_count;
     int graded; const double* x_nodes; i64 n_x_nodes;
+    int reuse;   // one patch with the mesh's origin and spacing: end-test values carry over
     double inv_c2;
     const double* b0; const double* b1; const double* b2;
     // base (last) patch = the mesh; filled on the host

@@ -45,6 +45,10 @@ struct LaneState {
     double ce0, ce1, ce2, cb0, cb1, cb2;  // with B: the last end's E and B
     i64 k, slot, ns;
     int have;
+    // the mesh coordinate across the axis and the cell coordinates of the current point,
+    // as the last step's end test computed them (valid when `point` is set)
+    double pt_y, pt_sx, pt_sy;
+    int point;
 };
 
 // Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
@@ -141,24 +145,30 @@ __device__ __forceinline__ double lorentz(double u0, double u1, double u2, doubl
 
 // E_x, E_y of the finest patch at the Cartesian point q, with the radial direction and
 // the mesh coordinates (pyccel.py:196-269).
+// With `known` (one patch whose origin and spacing are the mesh's), qy, sx and sy are the
+// values the end test of the last step computed for this same point: the same operations on
+// the same operands, so the same bits.  Otherwise they are computed and returned.
 template <int GEOM, bool GENERAL>
 __device__ __forceinline__ void edge_field(const TraceParams& p, double q0, double q1, double q2,
                                            double& ex, double& ey, double& cy, double& cz,
-                                           double& lq, double& qy) {
-    qy = q1; cy = 0.0; cz = 0.0;
+                                           double& lq, double& qy, bool known, double& sx, double& sy) {
+    cy = 0.0; cz = 0.0;
+    if (!known) qy = q1;
     if (GEOM == 1) {
-        const double r = hypot(q1, q2);
+        const double r = known ? qy : hypot(q1, q2);
         if (r > 0.0) { cy = q1 / r; cz = q2 / r; }
         qy = r;
     } else if (GEOM == 2) {
         cy = 1.0;
         if (q1 < p.origin_y) cy = -1.0;
-        qy = p.origin_y + fabs(q1 - p.origin_y);
+        if (!known) qy = p.origin_y + fabs(q1 - p.origin_y);
     }
     lq = locate_x<GENERAL>(p, q0);
     const PatchView v = patch_view<GENERAL>(p, patch_of<GENERAL>(p, lq, qy));
-    const double sx = (lq - v.ox) / v.h;
-    const double sy = (qy - v.oy) / v.h;
+    if (!known) {
+        sx = (lq - v.ox) / v.h;
+        sy = (qy - v.oy) / v.h;
+    }
     ex = clamped_bilinear(p.fxs + v.ofx, v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
     ey = clamped_bilinear(p.fys + v.ofy, v.ny + 1, sx, sy + 0.5, (int)v.nx - 2, (int)v.ny - 1);
 }
@@ -211,7 +221,11 @@ __device__ __forceinline__ void window_init(DepositWindow& w) {
 __device__ __forceinline__ void window_deposit(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn,
                                                i64 ny, bool in_base, i64 i, i64 j,
                                                const double (&value)[4], int e) {
-    if (in_base) {
+    // the window follows the cells of the first and last such lane of each wave: the bounds
+    // only place the window (a deposit outside it goes to global memory)
+    const u64 lanes = __ballot(in_base);
+    const int me = (int)__lane_id();
+    if (in_base && (me == __ffsll((long long)lanes) - 1 || me == 63 - __clzll((long long)lanes))) {
         atomicMin(&win.bounds[t][0], (int)i);
         atomicMax(&win.bounds[t][1], (int)i);
         atomicMin(&win.bounds[t][2], (int)j);
@@ -315,6 +329,7 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
     double w0 = 0.0, w1 = 0.0, w2 = 0.0, y0, y1, y2;
     double dt = 0.0, hdt = 0.0, dt6 = 0.0;
     double e0 = 0.0, e1 = 0.0, e2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    double end_y = 0.0, end_sx = 0.0, end_sy = 0.0;  // Boris: the end point's, from stage 1
     constexpr int NSTAGE = INTEG == 0 ? 4 : 2;
 #pragma unroll
     for (int stage = 0; stage < NSTAGE; ++stage) {
@@ -331,8 +346,11 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
                 g0 = s.ca0; g1 = s.ca1; g2 = s.ca2;
             }
         } else {
-            double ex, ey, cy, cz, lq, qy;
-            edge_field<GEOM, GENERAL>(p, q0, q1, q2, ex, ey, cy, cz, lq, qy);
+            double ex, ey, cy, cz, lq, qy, psx, psy;
+            const bool known = !GENERAL && p.reuse && stage == 0 && s.point != 0;
+            if (known) { qy = s.pt_y; psx = s.pt_sx; psy = s.pt_sy; }
+            edge_field<GEOM, GENERAL>(p, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy);
+            if (INTEG != 0 && stage == 1) { end_y = qy; end_sx = psx; end_sy = psy; }
             if (GEOM != 0) {
                 g0 = qmk * ex;
                 g1 = qmk * (ey * cy);
@@ -490,9 +508,12 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
     s.x0 = x0; s.x1 = x1; s.x2 = x2; s.v0 = v0; s.v1 = v1; s.v2 = v2;
 
     // the end of the step: off the mesh or in an electrode (pyccel.py:450-497)
+    const bool end_known = !GENERAL && p.reuse && INTEG != 0 && s.have != 0;
     double px = x0, py = x1;
-    if (GEOM == 1) py = hypot(x1, x2);
+    if (end_known) py = end_y;
+    else if (GEOM == 1) py = hypot(x1, x2);
     else if (GEOM == 2) py = p.origin_y + fabs(x1 - p.origin_y);
+    s.point = 0;
     if (s.slot >= 0) {
         const i64 c = p.path_count[s.slot];
         if (c < p.path_capacity) {
@@ -510,8 +531,9 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
     } else if (py > p.upper_y || (GEOM == 0 && py < p.origin_y)) {
         st = 2;
     } else {
-        const double ci = (px - p.origin_x) / p.h;
-        const double cj = (py - p.origin_y) / p.h;
+        const double ci = end_known ? end_sx : (px - p.origin_x) / p.h;
+        const double cj = end_known ? end_sy : (py - p.origin_y) / p.h;
+        s.pt_y = py; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
         // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341
         i64 i = (i64)fmax(ci, 0.0);
         if (i > p.nx - 2) i = p.nx - 2;
@@ -582,6 +604,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
                     if (s.slot >= p.n_paths) s.slot = -1;
                     s.ns = p.steps[k];
                     s.have = 0;
+                    s.point = 0;
                     s.ca0 = s.ca1 = s.ca2 = 0.0;
                     s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
                     if (s.ns >= p.max_steps) {  // no step: the loop of the reference ends at once

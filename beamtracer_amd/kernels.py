@@ -135,7 +135,7 @@ def deposit_trajectories_2d(
 
     Bilinear in planar geometry, linear in ``r**2`` along the radius in cylindrical geometry;
     charge beyond the mesh goes to its boundary nodes. The sum is exact in 128-bit fixed
-    point with LSB ``2**exponent`` (default: 52 bits below ``max|q|``), so it does not depend
+    point with LSB ``2**exponent`` (default: 48 bits below ``max|q|``), so it does not depend
     on the order of the charges.
     """
     L = _lib.lib()
@@ -166,7 +166,7 @@ def deposit_trajectories_2d(
 
 
 def trace_exponent(active, v, dq, qm, fxs, fys, patch_real, step_fraction, inv_c2=0.0) -> int:
-    """The deposit exponent of a trace: 52 bits below ``max|dq|`` times a typical step.
+    """The deposit exponent of a trace: 48 bits below ``max|dq|`` times a typical step.
 
     The step is ``step_fraction * min(h / v_max, sqrt(2 h / a_max))`` with the finest
     spacing, the fastest start and the strongest field, as the step rule of

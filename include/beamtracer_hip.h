@@ -18,9 +18,9 @@
  * Arrays are C-contiguous; int64_t where the reference has Pyccel `int`, double where it
  * has `float`.  Outputs are mutated in place exactly as the reference kernels do, except
  * that `charge` receives the whole deposit in its row 0 (as the reference's CUDA kernel,
- * kernels/trace_trajectories_2d/trace_trajectories_2d_cuda.cu:1-4), summed in 128-bit
- * fixed point with least significant bit 2^deposit_exponent, so the node charges do not
- * depend on the order of the deposits.
+ * kernels/trace_trajectories_2d/trace_trajectories_2d_cuda.cu:1-4), summed exactly in
+ * fixed point with least significant bit 2^deposit_exponent (each deposit rounded once to
+ * that grid), so the node charges do not depend on the order of the deposits.
  *
  * Return value: 0 on success, a negative BT_E_* code otherwise (never aborts);
  * bt_last_error() describes the last failure of the calling thread.
@@ -75,7 +75,7 @@ typedef struct bt_trace2d_args {
     int64_t has_b; const double* b0; const double* b1; const double* b2;
     /* controls */
     int32_t deposit_exponent;   /* fixed-point LSB 2^e of the deposit; see bt_deposit_exponent */
-    int32_t block_size;         /* threads per block (64..1024, a multiple of 64); 0 = tuned default */
+    int32_t block_size;         /* threads per block (64..256, a multiple of 64); 0 = default 256 */
     int32_t max_blocks;         /* persistent grid size; 0 = fill the device            */
     int32_t flags;              /* reserved, 0                                          */
     void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned */
@@ -90,7 +90,7 @@ const char* bt_last_error(void);
 int64_t bt_trace_args_size(void);
 
 /* The fixed-point exponent e (LSB = 2^e) for deposits whose typical magnitude is
- * `scale` (e.g. max|dq| times a typical step, or max|q|): 52 bits below it. */
+ * `scale` (e.g. max|dq| times a typical step, or max|q|): 48 bits below it. */
 int32_t bt_deposit_exponent(double scale);
 
 /* Device scratch needed by a trace or a deposit onto `n_nodes` nodes. */
