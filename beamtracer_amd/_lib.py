@@ -22,8 +22,8 @@ HEADERS = [ROOT / "include" / "beamtracer_hip.h"] + sorted((PACKAGE / "csrc").gl
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # wave-uniform atomics are combined across the wave with DPP (log-depth), not a lane loop
-    "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
+    # the kernels' atomics are lane-divergent or issued by one lane on purpose
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
 ]  # fmt: skip
 
 BT_OK, BT_E_ARG, BT_E_NOMEM, BT_E_LAUNCH, BT_E_DEVICE, BT_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5

@@ -251,7 +251,7 @@ int64_t bt_trace_args_size(void) { return (int64_t)sizeof(bt_trace2d_args); }
 
 int32_t bt_deposit_exponent(double scale) {
     if (!(scale > 0.0) || !std::isfinite(scale)) return -1074 + 60;
-    return (int32_t)std::ilogb(scale) - 48;
+    return (int32_t)std::ilogb(scale) - 40;
 }
 
 int64_t bt_trace_workspace_bytes(int64_t n_nodes) { return (int64_t)workspace_bytes(n_nodes); }

@@ -186,13 +186,24 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, double q0, doub
 
 constexpr int kWinCols = 16;
 constexpr int kWinRows = 64;
+constexpr int kRoundSteps = 1;        // steps between window moves
+constexpr int kFlushRounds = 32;      // every slot is emptied at least this often
 
+// A slot sums a node's deposits in one signed 64-bit word.  Only deposits below 2^47 units
+// enter the window (larger ones go to the node words directly), a slot receives at most
+// one deposit per lane and step (blockDim * kRoundSteps per round, at most 1024), and every
+// slot is flushed at least every kFlushRounds rounds: |sum| < 32 * 1024 * 2^47 = 2^62.
 struct DepositWindow {
     i64 tag[2][kWinCols];             // the column held by a slot, -1 for none
     int base[2][kWinCols];            // its first row
-    int bounds[2][4];                 // min/max column and row of the step's deposits
-    u64 word[kWinCols][kWinRows][kWords];
+    int bounds[2][4];                 // min/max column and row of the round's deposits
+    u64 word[kWinCols][kWinRows];
 };
+
+// Splits a slot's signed sum q into the two node words (low 32 bits, q >> 32).
+__device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
+    add_words(w, q & 0xffffffffull, (u64)((i64)q >> 32));
+}
 
 __device__ __forceinline__ void window_reset_bounds(DepositWindow& w, int t) {
     w.bounds[t][0] = 0x7fffffff;
@@ -202,8 +213,7 @@ __device__ __forceinline__ void window_reset_bounds(DepositWindow& w, int t) {
 }
 
 __device__ __forceinline__ void window_init(DepositWindow& w) {
-    for (int k = threadIdx.x; k < kWinCols * kWinRows * kWords; k += blockDim.x)
-        (&w.word[0][0][0])[k] = 0ull;
+    for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) (&w.word[0][0])[k] = 0ull;
     if (threadIdx.x < kWinCols) {
         w.tag[0][threadIdx.x] = -1;
         w.tag[1][threadIdx.x] = -1;
@@ -239,9 +249,9 @@ __device__ __forceinline__ void window_deposit(DepositWindow& win, int t, u64* _
         if (in_base) {
             const int slot = (int)(ii & (kWinCols - 1));
             const i64 row = jj - win.base[t][slot];
-            if (win.tag[t][slot] == ii && row >= 0 && row < kWinRows) {
-                if (w0) atomicAdd(&win.word[slot][row][0], w0);
-                if (w1) atomicAdd(&win.word[slot][row][1], w1);
+            const bool small = (i64)w1 >= -(1ll << 15) && (i64)w1 < (1ll << 15);  // |q| < 2^47
+            if (small && win.tag[t][slot] == ii && row >= 0 && row < kWinRows) {
+                atomicAdd(&win.word[slot][row], (w1 << 32) + w0);  // q itself, two's complement
                 continue;
             }
         }
@@ -249,15 +259,16 @@ __device__ __forceinline__ void window_deposit(DepositWindow& win, int t, u64* _
     }
 }
 
-// After the deposits of step k (table t = k mod 2): move the window to the columns around
-// the step's cells, flushing the slots that change, and write table 1 - t.
+// After the deposits of round k (table t = k mod 2): move the window to the columns around
+// the round's cells (with room for kRoundSteps steps of travel either way), flushing the
+// slots that change, and write table 1 - t.
 __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, u64* __restrict__ acc,
-                                                i64 ofn, i64 nx, i64 ny) {
+                                                i64 ofn, i64 nx, i64 ny, bool empty_all) {
     const int n = 1 - t;
     const int cmin = win.bounds[t][0], cmax = win.bounds[t][1];
     const int rmin = win.bounds[t][2];
     const bool moved = cmin <= cmax;
-    int lo = cmin - 1, hi = cmax + 2;
+    int lo = cmin - 1 - kRoundSteps, hi = cmax + 2 + kRoundSteps;
     if (lo < 0) lo = 0;
     if (hi > (int)nx - 1) hi = (int)nx - 1;
     if (hi > lo + kWinCols - 1) hi = lo + kWinCols - 1;
@@ -272,14 +283,10 @@ __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, u64* 
             const int col = lo + ((slot - lo) & (kWinCols - 1));
             if (col <= hi) want = col;
         }
-        if (want != old) {
-            const u64 w0 = win.word[slot][row][0], w1 = win.word[slot][row][1];
-            if (old >= 0 && (w0 | w1) != 0) {
-                const i64 jj = win.base[t][slot] + row;
-                add_words(acc + kWords * (ofn + old * ny + jj), w0, w1);
-            }
-            win.word[slot][row][0] = 0ull;
-            win.word[slot][row][1] = 0ull;
+        if (want != old || empty_all) {
+            const u64 q = win.word[slot][row];
+            if (old >= 0 && q != 0) add_sum(acc + kWords * (ofn + old * ny + win.base[t][slot] + row), q);
+            win.word[slot][row] = 0ull;
         }
         if (row == 0) {
             win.tag[n][slot] = want;
@@ -294,8 +301,8 @@ __device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __r
     for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
         const int slot = k / kWinRows, row = k % kWinRows;
         const i64 old = win.tag[t][slot];
-        const u64 w0 = win.word[slot][row][0], w1 = win.word[slot][row][1];
-        if (old >= 0 && (w0 | w1) != 0) add_words(acc + kWords * (ofn + old * ny + win.base[t][slot] + row), w0, w1);
+        const u64 q = win.word[slot][row];
+        if (old >= 0 && q != 0) add_sum(acc + kWords * (ofn + old * ny + win.base[t][slot] + row), q);
     }
 }
 
@@ -578,7 +585,8 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
     LaneState s;
     s.k = -1;
     bool exhausted = false;
-    int t = 0;  // the window table of this step
+    int t = 0;       // the window table of this round
+    int rounds = 0;
     for (;;) {
         const bool want = s.k < 0 && !exhausted;
         const u64 need = __ballot(want);
@@ -619,18 +627,21 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
         }
         // the whole workgroup goes on while any lane has or may get a trajectory
         if (!__syncthreads_or(s.k >= 0 || !exhausted)) break;
-        if (s.k >= 0) {
-            const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s, win, t);
-            if (st != -1) {
-                store_end(p, s, st);
-                s.k = -1;
-            } else if (s.ns >= p.max_steps) {
-                store_end(p, s, 4);
-                s.k = -1;
+        // a round of steps; a lane whose trajectory ends waits for the next round's refill
+        for (int r = 0; r < kRoundSteps; ++r) {
+            if (s.k >= 0) {
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s, win, t);
+                if (st != -1) {
+                    store_end(p, s, st);
+                    s.k = -1;
+                } else if (s.ns >= p.max_steps) {
+                    store_end(p, s, 4);
+                    s.k = -1;
+                }
             }
         }
         __syncthreads();
-        window_maintain(win, t, p.acc, p.bofn, p.nx, p.ny);
+        window_maintain(win, t, p.acc, p.bofn, p.nx, p.ny, ++rounds % kFlushRounds == 0);
         t = 1 - t;
     }
     window_flush(win, t, p.acc, p.bofn, p.ny);

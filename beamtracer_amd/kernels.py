@@ -134,9 +134,9 @@ def deposit_trajectories_2d(
     """Add the charges ``q`` at ``(px, py)`` to the node charges ``charge``, area-weighted.
 
     Bilinear in planar geometry, linear in ``r**2`` along the radius in cylindrical geometry;
-    charge beyond the mesh goes to its boundary nodes. The sum is exact in 128-bit fixed
-    point with LSB ``2**exponent`` (default: 48 bits below ``max|q|``), so it does not depend
-    on the order of the charges.
+    charge beyond the mesh goes to its boundary nodes. The sum is exact in fixed point with
+    LSB ``2**exponent`` (default: 50 bits below ``max|q|``, each charge rounded once to that
+    grid), so it does not depend on the order of the charges.
     """
     L = _lib.lib()
     if _is_torch(px):
@@ -144,7 +144,7 @@ def deposit_trajectories_2d(
         if px.dim() != 1 or py.shape != px.shape or q.shape != px.shape or charge.dim() != 2:
             raise ValueError("px, py, q must be 1D of one length and charge 2D")
         if exponent is None:
-            exponent = deposit_exponent(float(q.abs().max().item()) if q.numel() else 1.0)
+            exponent = deposit_exponent(float(q.abs().max().item()) if q.numel() else 1.0) - 10
         nx, ny = charge.shape
         rc = L.bt_deposit_trajectories_2d_device(
             _ptr(px), _ptr(py), _ptr(q), px.shape[0], float(origin_x), float(origin_y), float(h),
@@ -156,7 +156,7 @@ def deposit_trajectories_2d(
         if px.ndim != 1 or py.shape != px.shape or q.shape != px.shape or charge.ndim != 2:
             raise ValueError("px, py, q must be 1D of one length and charge 2D")
         if exponent is None:
-            exponent = deposit_exponent(float(np.abs(q).max()) if q.size else 1.0)
+            exponent = deposit_exponent(float(np.abs(q).max()) if q.size else 1.0) - 10
         nx, ny = charge.shape
         rc = L.bt_deposit_trajectories_2d(
             _ptr(px), _ptr(py), _ptr(q), px.shape[0], float(origin_x), float(origin_y), float(h),
@@ -166,7 +166,7 @@ def deposit_trajectories_2d(
 
 
 def trace_exponent(active, v, dq, qm, fxs, fys, patch_real, step_fraction, inv_c2=0.0) -> int:
-    """The deposit exponent of a trace: 48 bits below ``max|dq|`` times a typical step.
+    """The deposit exponent of a trace: 40 bits below ``max|dq|`` times a typical step.
 
     The step is ``step_fraction * min(h / v_max, sqrt(2 h / a_max))`` with the finest
     spacing, the fastest start and the strongest field, as the step rule of

@@ -90,7 +90,7 @@ const char* bt_last_error(void);
 int64_t bt_trace_args_size(void);
 
 /* The fixed-point exponent e (LSB = 2^e) for deposits whose typical magnitude is
- * `scale` (e.g. max|dq| times a typical step, or max|q|): 48 bits below it. */
+ * `scale` (e.g. max|dq| times a typical step, or max|q|): 40 bits below it. */
 int32_t bt_deposit_exponent(double scale);
 
 /* Device scratch needed by a trace or a deposit onto `n_nodes` nodes. */

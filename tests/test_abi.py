@@ -32,8 +32,8 @@ def test_struct_layout_and_pure_helpers():
     L = _lib.lib()
     assert L.bt_trace_args_size() == ctypes.sizeof(_lib.TraceArgs)
     assert L.bt_version().startswith(b"beamtracer_hip")
-    assert L.bt_deposit_exponent(1.0) == -48
-    assert L.bt_deposit_exponent(0.75) == -49
+    assert L.bt_deposit_exponent(1.0) == -40
+    assert L.bt_deposit_exponent(0.75) == -41
     assert L.bt_trace_workspace_bytes(161001) >= 16 * 161001
 
 
