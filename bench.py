@@ -136,6 +136,8 @@ def main() -> int:
     parser.add_argument("--block-size", type=int, default=0)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
+    parser.add_argument("--calibrate-copy", action="store_true",
+                        help="only copy 1 GiB device to device (calibrates the HBM byte counters)")
     opts = parser.parse_args()
     if opts.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -147,6 +149,15 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if opts.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}))
+        return 0
     log("building the einzel-lens workload (mesh, host Poisson, edge fields, emitter)")
     design, tracer = build_workload(opts.particles)
     host = kernel_arguments(design, tracer)
@@ -189,6 +200,10 @@ def main() -> int:
     log(f"timed: {opts.steps} traces in {wall:.3f} s; launch avg {avg_launch_s * 1e3:.3f} ms")
 
     achieved = FLOP_PER_STEP_RK4_CYL * steps_per_trace / avg_launch_s / 1e12
+    traffic = None
+    traffic_file = ROOT / "profiles" / "trace_kernel_einzel_traffic.json"
+    if traffic_file.exists():
+        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": value,
@@ -217,7 +232,7 @@ def main() -> int:
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "flop_per_step": FLOP_PER_STEP_RK4_CYL,
             "launch_ms": avg_launch_s * 1e3,
         },
