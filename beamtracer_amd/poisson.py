@@ -66,11 +66,14 @@ class BoltzmannIons:
 class IonOpticsPoisson:
     """Poisson's equation on an :class:`IonOpticsMesh`, assembled once and factorized."""
 
-    def __init__(self, mesh: IonOpticsMesh):
+    def __init__(self, mesh: IonOpticsMesh, fixed=None, fixed_value=None):
         self.mesh = mesh
+        self.fixed = mesh.fixed if fixed is None else np.asarray(fixed, dtype=bool)
+        self.fixed_value = mesh.fixed_value if fixed_value is None else np.asarray(fixed_value, dtype=float)
         self._assemble()
         self._lu = None
         self._lu_reaction = None
+        self._region = None
         self.newton_iterations = 0
         self.factorisations = 0
 
@@ -78,8 +81,8 @@ class IonOpticsPoisson:
         mesh = self.mesh
         nx, ny = mesh.shape
         h = mesh.h
-        fixed = mesh.fixed
-        value = mesh.fixed_value
+        fixed = self.fixed
+        value = self.fixed_value
         solid = mesh.solid
         potentials = np.array([e.potential for e in mesh.electrodes] + [0.0])
         index = -np.ones((nx, ny), dtype=np.int64)
@@ -93,13 +96,6 @@ class IonOpticsPoisson:
         ii, jj = np.nonzero(free)
         dx = mesh.dx / h  # edges along x in units of h
         faces = mesh.faces
-
-        def neighbour(i, j, di, dj):
-            """(exists, index, fixed?, value, distance-fraction) of a neighbour."""
-            ni, nj = i + di, j + dj
-            inside = (ni >= 0) & (ni < nx) & (nj >= 0) & (nj < ny)
-            ci, cj = np.clip(ni, 0, nx - 1), np.clip(nj, 0, ny - 1)
-            return inside, ci, cj
 
         for axis in (0, 1):
             lo_face, hi_face = ("x_min", "x_max") if axis == 0 else ("y_min", "y_max")
@@ -144,11 +140,9 @@ class IonOpticsPoisson:
                 c_hi = np.where(cyl_axis, 4.0 / (a_hi * a_hi), c_hi)
                 c_lo = np.where(cyl_axis, 0.0, c_lo)
             np.add.at(diag, (ii, jj), c_lo + c_hi)
-            # low side
-            g_lo = faces[lo_face]
-            for side, has, c, ni_, nj_, solid_side, a_arm, face_name, sign in (
-                ("lo", has_lo, c_lo, lo_i, lo_j, lo_solid, a_lo, lo_face, -1),
-                ("hi", has_hi, c_hi, hi_i, hi_j, hi_solid, a_hi, hi_face, 1),
+            for has, c, ni_, nj_, solid_side, a_arm, face_name, sign in (
+                (has_lo, c_lo, lo_i, lo_j, lo_solid, a_lo, lo_face, -1),
+                (has_hi, c_hi, hi_i, hi_j, hi_solid, a_hi, hi_face, 1),
             ):
                 # existing neighbour: free -> matrix; fixed -> constant
                 nb_fixed = has & fixed[ni_, nj_]
@@ -197,10 +191,25 @@ class IonOpticsPoisson:
         self._lu_reaction = reaction
         return self._lu
 
-    def solve(self, rho=None, *, plasma=None, initial=None, tolerance: float = 1e-6, max_newton: int = 100) -> np.ndarray:
-        """The potential ``(nx, ny)`` for the node charge density ``rho`` [C/m^3]."""
+    def solve(self, rho=None, *, plasma=None, initial=None, tolerance: float = 1e-6, max_newton: int = 100,
+              fixed_region=None, fixed_potential: float = 0.0) -> np.ndarray:
+        """The potential ``(nx, ny)`` for the node charge density ``rho`` [C/m^3].
+
+        ``fixed_region`` (a node mask, or a function of the node coordinates) holds its free
+        nodes at ``fixed_potential``: the reference's initial plasma (``poisson.py:809-860``).
+        """
         mesh = self.mesh
-        phi = np.array(mesh.fixed_value, dtype=float)
+        if fixed_region is not None:
+            region = fixed_region(*mesh.nodes()) if callable(fixed_region) else np.asarray(fixed_region, dtype=bool)
+            region = region & ~self.fixed
+            key = (region.tobytes(), float(fixed_potential))
+            if self._region is None or self._region[0] != key:
+                solver = IonOpticsPoisson(mesh, self.fixed | region,
+                                          np.where(region, float(fixed_potential), self.fixed_value))
+                self._region = (key, solver)
+            return self._region[1].solve(rho, plasma=plasma, initial=initial, tolerance=tolerance,
+                                         max_newton=max_newton)
+        phi = np.array(self.fixed_value, dtype=float)
         if self.n == 0:
             return phi
         base = self.constant.copy()
@@ -219,7 +228,7 @@ class IonOpticsPoisson:
         temp = float(plasma.temperature)
         up = float(plasma.plasma_potential)
         s = plasma.sign
-        prescribed = mesh.fixed_value[mesh.fixed]
+        prescribed = self.fixed_value[self.fixed]
         scale = max(float(np.ptp(prescribed)) if prescribed.size else 0.0, temp, 1.0)
         stop = tolerance * scale * 0.1
         refresh, fresh, previous = True, False, np.inf

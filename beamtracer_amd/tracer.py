@@ -272,10 +272,7 @@ class HipBeamTracer:
                     SUBCELLS, planes, hits, counts, path_slot, paths, path_count, 1, charge,
                     int(mesh.graded), x_nodes, self._inv_c2(),
                 ]  # fmt: skip
-                if self.device:
-                    self._call_device(call, magnetic, exponent)
-                else:
-                    kernels.trace_trajectories_2d(*call, magnetic=magnetic, exponent=exponent, block_size=self.block_size)
+                self._launch(call, magnetic, exponent)
                 paused = active[status[active] == _PAUSED]
                 if not paused.size:
                     break
@@ -318,6 +315,13 @@ class HipBeamTracer:
             steps=int(steps.max()) if n else 0,
             particle_steps=int(steps.sum()),
         )
+
+    def _launch(self, call: list, magnetic, exponent: int) -> None:
+        """One ``trace_trajectories_2d`` call with the 35 arguments of ``call``."""
+        if self.device:
+            self._call_device(call, magnetic, exponent)
+        else:
+            kernels.trace_trajectories_2d(*call, magnetic=magnetic, exponent=exponent, block_size=self.block_size)
 
     def _call_device(self, call: list, magnetic, exponent: int) -> None:
         """One kernel call with the arrays on the GPU (copied back into ``call`` after)."""
