@@ -1,0 +1,147 @@
+"""The caller protocol and the Vlasov loop on the GPU, against the same protocol on the oracle.
+
+``OracleTracer`` runs ``HipBeamTracer``'s restatement of ``BeamTracer._trace_compiled``
+(``solver/tracer.py:1185-1304``) with the oracle's kernel in place of the hip one, so every
+difference is the kernel's. The bars are the north star's: trajectory end states and
+diagnostic-plane moments within 1e-10 relative, the deposited charge within 1e-12 of its
+total, and the converged potential of the Vlasov loop within 1e-9 relative.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from beamtracer_amd import designs, diagnostics, fields, mesh as meshmod, poisson
+from beamtracer_amd.tracer import HipBeamTracer
+from beamtracer_amd.vlasov import vlasov_iterate
+
+pytestmark = pytest.mark.gpu
+
+
+class OracleTracer(HipBeamTracer):
+    """The same caller, with the oracle's fused tracer (4 OpenMP slabs) as the kernel."""
+
+    def _launch(self, call, magnetic, exponent):
+        args = list(call)
+        slabs = np.zeros((4, call[31].shape[1]))
+        args[30], args[31] = 4, slabs
+        oracle.trace_trajectories_2d(*args, magnetic=magnetic)
+        call[31][0] += slabs.sum(0)
+
+
+def _field(design):
+    phi = poisson.IonOpticsPoisson(design.mesh).solve()
+    return fields.StaggeredElectricField(design.mesh, phi)
+
+
+def _pair(design, field, cls_kwargs=None, **kw):
+    kw = dict(planes=design.planes, step_fraction=design.step_fraction, integrator=design.integrator, **kw)
+    return HipBeamTracer(design.mesh, field, **kw), OracleTracer(design.mesh, field, **kw)
+
+
+def _assert_same_trace(got, want, cylindrical, mass):
+    np.testing.assert_array_equal(got.status, want.status)
+    np.testing.assert_array_equal(got.electrode, want.electrode)
+    scale = np.abs(want.charge_density).sum()
+    assert np.abs(got.charge_density - want.charge_density).max() <= 1e-12 * scale
+    for a, b in zip(got.planes, want.planes):
+        assert a.point.shape == b.point.shape
+        np.testing.assert_allclose(a.point, b.point, rtol=1e-10, atol=1e-15)
+        np.testing.assert_allclose(a.velocity, b.velocity, rtol=1e-10, atol=1e-9)
+        ma = diagnostics.plane_moments(a, cylindrical=cylindrical, mass=mass)
+        mb = diagnostics.plane_moments(b, cylindrical=cylindrical, mass=mass)
+        for name in ("current", "rms_size", "rms_angle", "emittance", "mean_energy"):
+            assert getattr(ma, name) == pytest.approx(getattr(mb, name), rel=1e-10), name
+
+
+def test_einzel_trace_matches_the_oracle_caller():
+    design = designs.einzel_lens(particles=3000)
+    field = _field(design)
+    hip, ref = _pair(design, field)
+    got, want = hip.trace(design.particles), ref.trace(design.particles)
+    assert got.particle_steps == want.particle_steps > 3000 * 1500
+    _assert_same_trace(got, want, True, design.particles.mass[0])
+
+
+def test_device_mode_equals_host_mode():
+    design = designs.einzel_lens(particles=1000)
+    field = _field(design)
+    host = HipBeamTracer(design.mesh, field, planes=design.planes)
+    dev = HipBeamTracer(design.mesh, field, planes=design.planes, device=True)
+    a, b = host.trace(design.particles), dev.trace(design.particles)
+    np.testing.assert_array_equal(a.charge_density, b.charge_density)
+    np.testing.assert_array_equal(a.planes[0].point, b.planes[0].point)
+
+
+def _aperture_design(n=400):
+    """A planar beam onto an electrode with a round hole: trajectories stop on cut sub-cells."""
+
+    def disk(x, y):
+        x = np.asarray(x)
+        y = np.asarray(y)
+        return ((x - 6e-3) ** 2 + (y - 2.5e-3) ** 2 < (1.3e-3) ** 2) | ((x > 9e-3) & (x < 9.6e-3) & (y > 3.2e-3))
+
+    faces = {"x_min": meshmod.FaceCondition("dirichlet", 0.0), "x_max": meshmod.FaceCondition("dirichlet", -2000.0)}
+    electrode = meshmod.Electrode(disk, -500.0)
+    mesh = meshmod.IonOpticsMesh((61, 31), (0.0, 0.0), 0.2e-3, "planar", (electrode,), faces)
+    from beamtracer_amd.beam import beam_with_energy
+
+    particles, jv = beam_with_energy(n, 10.0, 1.0, 1.007276, 100.0, 0.0, 0.5, (0.0, 0.2e-3), (0.0, 5.8e-3),
+                                     cylindrical=False)  # fmt: skip
+    return designs.Design(mesh, particles, jv, (11e-3,))
+
+
+def test_pause_and_resume_on_cut_subcells_match_the_oracle_caller():
+    design = _aperture_design()
+    field = _field(design)
+    hip, ref = _pair(design, field)
+    _, subcells = hip.electrode_lookup()
+    assert (subcells == -2).any()
+    got, want = hip.trace(design.particles), ref.trace(design.particles)
+    assert (want.status == 3).sum() > 10 and (want.status == 0).sum() > 10
+    _assert_same_trace(got, want, False, design.particles.mass[0])
+
+
+def test_buffer_overflow_retrace_is_identical():
+    design = _aperture_design(200)
+    field = _field(design)
+    big = HipBeamTracer(design.mesh, field, planes=(4e-3, 11e-3), save_trajectories=7)
+    small = HipBeamTracer(design.mesh, field, planes=(4e-3, 11e-3), save_trajectories=7)
+    small._capacity = (1, 8)
+    a, b = big.trace(design.particles), small.trace(design.particles)
+    assert small._capacity[1] > 8  # the paths outgrew their buffer and the trace was repeated
+    np.testing.assert_array_equal(a.status, b.status)
+    np.testing.assert_array_equal(a.charge_density, b.charge_density)
+    for p, q in zip(a.planes, b.planes):
+        np.testing.assert_array_equal(p.point, q.point)
+    for p, q in zip(a.trajectories, b.trajectories):
+        np.testing.assert_array_equal(p, q)
+
+
+def test_diode_vlasov_loop_matches_the_oracle_loop():
+    design = designs.planar_diode(particles=40)
+    results = []
+    for cls in (HipBeamTracer, OracleTracer):
+        tracer = cls(design.mesh, None, planes=design.planes)
+        state = vlasov_iterate(design.mesh, lambda: (design.particles, design.j_over_v), tracer, iterations=8)
+        results.append(state)
+    got, want = results
+    span = np.ptp(want.potential)
+    assert np.abs(got.potential - want.potential).max() <= 1e-9 * span
+    _assert_same_trace(got.trace, want.trace, False, design.particles.mass[0])
+
+
+def test_einzel_vlasov_loop_with_relaxation_matches_the_oracle_loop():
+    design = designs.einzel_lens(particles=2000, current_density=2000.0)  # strong space charge
+    results = []
+    for cls in (HipBeamTracer, OracleTracer):
+        tracer = cls(design.mesh, None, planes=design.planes)
+        state = vlasov_iterate(design.mesh, lambda: (design.particles, design.j_over_v), tracer,
+                               iterations=3, relaxation=0.5)  # fmt: skip
+        results.append(state)
+    got, want = results
+    span = np.ptp(want.potential)
+    assert np.abs(got.potential - want.potential).max() <= 1e-9 * span
+    assert np.abs(got.charge_density).max() > 0.0
