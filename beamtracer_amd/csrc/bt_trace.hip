@@ -49,7 +49,15 @@ struct LaneState {
     // as the last step's end test computed them (valid when `point` is set)
     double pt_y, pt_sx, pt_sy;
     int point;
+    // the lane's deposit cache: exact fixed-point sums (units 2^e, |sum| < 2^46) for the four
+    // nodes of base-mesh cell (cci, ccj); a sum leaves the lane when the trajectory moves to
+    // a cell without that node, when it would reach 2^46, or when the trajectory ends
+    int cci, ccj;
+    i64 cq[4];
 };
+
+constexpr int kNoCell = -(1 << 30);
+constexpr i64 kCacheLimit = 1ll << 46;
 
 // Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
 __device__ __forceinline__ double node_bilinear(const double* __restrict__ f, double sx, double sy,
@@ -189,10 +197,11 @@ constexpr int kWinRows = 64;
 constexpr int kRoundSteps = 1;        // steps between window moves
 constexpr int kFlushRounds = 32;      // every slot is emptied at least this often
 
-// A slot sums a node's deposits in one signed 64-bit word.  Only deposits below 2^47 units
-// enter the window (larger ones go to the node words directly), a slot receives at most
-// one deposit per lane and step (blockDim * kRoundSteps per round, at most 1024), and every
-// slot is flushed at least every kFlushRounds rounds: |sum| < 32 * 1024 * 2^47 = 2^62.
+// A slot sums a node's deposits in one signed 64-bit word.  What enters the window are the
+// lanes' cached sums (each below 2^46 units in magnitude); a lane adds at most two of them to
+// a node per step (one when its sum would overflow, one when it leaves the node), a round is
+// kRoundSteps steps of at most 1024 lanes, and every slot is flushed at least every
+// kFlushRounds rounds: |sum| < 32 * 1 * 1024 * 2 * 2^46 = 2^62.
 struct DepositWindow {
     i64 tag[2][kWinCols];             // the column held by a slot, -1 for none
     int base[2][kWinCols];            // its first row
@@ -226,13 +235,10 @@ __device__ __forceinline__ void window_init(DepositWindow& w) {
     }
 }
 
-// The 4-node deposit of one step: nodes (i, j), (i, j + 1), (i + 1, j), (i + 1, j + 1) of a
-// patch whose nodes start at `ofn`; the window serves the base patch only (in_base).
-__device__ __forceinline__ void window_deposit(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn,
-                                               i64 ny, bool in_base, i64 i, i64 j,
-                                               const double (&value)[4], int e) {
-    // the window follows the cells of the first and last such lane of each wave: the bounds
-    // only place the window (a deposit outside it goes to global memory)
+// Records the base-mesh cell (i, j) of a lane's deposit for the window's next placement:
+// the first and last such lane of each wave report (the bounds only place the window; a
+// sum outside it goes to global memory).
+__device__ __forceinline__ void window_note(DepositWindow& win, int t, bool in_base, i64 i, i64 j) {
     const u64 lanes = __ballot(in_base);
     const int me = (int)__lane_id();
     if (in_base && (me == __ffsll((long long)lanes) - 1 || me == 63 - __clzll((long long)lanes))) {
@@ -241,19 +247,85 @@ __device__ __forceinline__ void window_deposit(DepositWindow& win, int t, u64* _
         atomicMin(&win.bounds[t][2], (int)j);
         atomicMax(&win.bounds[t][3], (int)j);
     }
+}
+
+// Adds q units (|q| < 2^46) to base-mesh node (ii, jj): into the window slot that holds the
+// node, else to the node's words in global memory.
+__device__ __forceinline__ void node_add(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn, i64 ny,
+                                         i64 ii, i64 jj, i64 q) {
+    const int slot = (int)(ii & (kWinCols - 1));
+    const i64 row = jj - win.base[t][slot];
+    if (win.tag[t][slot] == ii && row >= 0 && row < kWinRows) {
+        atomicAdd(&win.word[slot][row], (u64)q);  // two's complement
+        return;
+    }
+    add_sum(acc + kWords * (ofn + ii * ny + jj), (u64)q);
+}
+
+// Moves the lane's cache to cell (i, j): the sums of nodes that both cells share move to
+// their new places, the others leave the lane.
+__device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
+                                           i64 ofn, i64 ny, int i, int j) {
+    if (i == s.cci && j == s.ccj) return;
+    const int di = i - s.cci, dj = j - s.ccj;
+    i64 moved[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {  // new place r takes old place (ra + di, rb + dj)
+        const int sa = (r >> 1) + di, sb = (r & 1) + dj;
+        i64 v = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v = (sa == (q >> 1) && sb == (q & 1)) ? s.cq[q] : v;
+        moved[r] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int na = (q >> 1) - di, nb = (q & 1) - dj;
+        const bool kept = na >= 0 && na <= 1 && nb >= 0 && nb <= 1;
+        if (!kept && s.cq[q] != 0) node_add(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s.cq[r] = moved[r];
+    s.cci = i;
+    s.ccj = j;
+}
+
+// Empties the lane's cache (the trajectory ended).
+__device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
+                                            i64 ofn, i64 ny) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (s.cq[q] != 0) node_add(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
+        s.cq[q] = 0;
+    }
+    s.cci = kNoCell;
+    s.ccj = kNoCell;
+}
+
+// The 4-node deposit of one step: nodes (i, j), (i, j + 1), (i + 1, j), (i + 1, j + 1) of a
+// patch whose nodes start at `ofn`.  Each value is rounded once to units of 2^e; on the base
+// mesh (in_base) the units gather in the lane's cache, elsewhere (or when a single deposit
+// reaches 2^46 units) they go to the node words in global memory.  The node sums are exact
+// integers, so the grouping does not change them.
+__device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
+                                         i64 ofn, i64 ny, bool in_base, i64 i, i64 j,
+                                         const double (&value)[4], int e) {
+    window_note(win, t, in_base, i, j);
+    if (in_base) cache_move(s, win, t, acc, ofn, ny, (int)i, (int)j);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         u64 w0, w1;
         if (!to_fixed(value[q], e, w0, w1)) continue;
         const i64 ii = i + (q >> 1), jj = j + (q & 1);
-        if (in_base) {
-            const int slot = (int)(ii & (kWinCols - 1));
-            const i64 row = jj - win.base[t][slot];
-            const bool small = (i64)w1 >= -(1ll << 15) && (i64)w1 < (1ll << 15);  // |q| < 2^47
-            if (small && win.tag[t][slot] == ii && row >= 0 && row < kWinRows) {
-                atomicAdd(&win.word[slot][row], (w1 << 32) + w0);  // q itself, two's complement
-                continue;
+        if (in_base && (i64)w1 >= -(1ll << 14) && (i64)w1 < (1ll << 14)) {  // |units| < 2^46
+            const i64 units = (i64)((w1 << 32) + w0);
+            const i64 sum = s.cq[q] + units;
+            if (sum > -kCacheLimit && sum < kCacheLimit) {
+                s.cq[q] = sum;
+            } else {
+                node_add(win, t, acc, ofn, ny, ii, jj, s.cq[q]);
+                s.cq[q] = units;
             }
+            continue;
         }
         add_words(acc + kWords * (ofn + ii * ny + jj), w0, w1);
     }
@@ -481,8 +553,8 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
         const double deposit = s.dqk * dt;
         const double value[4] = {(1.0 - u) * (1.0 - w) * deposit, (1.0 - u) * w * deposit,
                                  u * (1.0 - w) * deposit, u * w * deposit};
-        window_deposit(win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, i, j, value,
-                       p.deposit_exponent);
+        deposit4(s, win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, i, j, value,
+                 p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
     s.ns = ns;
@@ -584,6 +656,8 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     LaneState s;
     s.k = -1;
+    s.cci = s.ccj = kNoCell;
+    s.cq[0] = s.cq[1] = s.cq[2] = s.cq[3] = 0;
     bool exhausted = false;
     int t = 0;       // the window table of this round
     int rounds = 0;
@@ -631,12 +705,10 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
         for (int r = 0; r < kRoundSteps; ++r) {
             if (s.k >= 0) {
                 const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s, win, t);
-                if (st != -1) {
-                    store_end(p, s, st);
+                if (st != -1 || s.ns >= p.max_steps) {
+                    store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;
-                } else if (s.ns >= p.max_steps) {
-                    store_end(p, s, 4);
-                    s.k = -1;
+                    cache_flush(s, win, t, p.acc, p.bofn, p.ny);
                 }
             }
         }

@@ -1,4 +1,4 @@
-"""Geometries of the measured workloads: the cylindrical einzel lens and the planar diode.
+"""Geometries of the measured workloads: the einzel lens, the planar diode, the extraction.
 
 Both are built from :mod:`beamtracer_amd.mesh` primitives (the reference builds electrode
 shapes with the absent simgeometry package, ``sections.py:595-624``):
@@ -9,6 +9,12 @@ shapes with the absent simgeometry package, ``sections.py:595-624``):
   the ``x`` faces are Neumann 0 and the wall Dirichlet 0.
 * :func:`planar_diode` -- the planar diode of the reference README (``README.md:63-94``):
   Dirichlet 0 V at ``x_min`` and ``-1 kV`` at ``x_max``, Neumann 0 on ``y``.
+* :func:`extraction` -- positive-ion extraction from a Boltzmann-electron plasma through a
+  round aperture (the reference's ``_extraction`` test geometry,
+  ``tests/unit/test_beam_optics.py:211-261``, scaled 7.5x in ``x`` and 5x in ``r``) with a
+  30 kV gap, H+ and H2+ in a 0.8:0.2 current ratio, and the field of a current loop of
+  0.2 T peak on the axis after the gap, sampled onto the nodes through the paraxial series
+  of :class:`~beamtracer_amd.fields.AxisymmetricMagneticField`; traced with Boris.
 """
 
 from __future__ import annotations
@@ -18,7 +24,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from beamtracer_amd.beam import BeamParticles, beam_with_energy
+from beamtracer_amd.fields import AxisymmetricMagneticField, current_loop_on_axis, node_magnetic_field
 from beamtracer_amd.mesh import Electrode, FaceCondition, IonOpticsMesh, box
+from beamtracer_amd.vlasov import PlasmaModel
 
 PROTON_MASS_U = 1.007276
 H2_MASS_U = 2.014552
@@ -35,6 +43,9 @@ class Design:
     integrator: str = "rk4"
     step_fraction: float = 0.5
     max_steps: int = 100_000
+    #: node magnetic field ``(B_x, B_r, B_theta)`` on the mesh nodes, or None
+    magnetic: tuple | None = None
+    plasma: PlasmaModel | None = None
 
 
 def einzel_lens(
@@ -80,6 +91,65 @@ def planar_diode(particles: int = 40, plane: float = 9.9e-3) -> Design:
         particles, 10.0, 1.0, PROTON_MASS_U, 0.01, 0.0, 0.5, (0.0, 0.0), (0.0, 5e-3), cylindrical=False,
     )  # fmt: skip
     return Design(mesh, beam, jv, (plane,))
+
+
+def extraction(
+    nx: int = 1025,
+    ny: int = 257,
+    length_x: float = 60e-3,
+    voltage: float = 30e3,
+    current_density: float = 50.0,
+    particles: int = 1_000_000,
+    h2_share: float = 0.2,
+    energy: float = 0.5,
+    transverse_temperature: float = 0.5,
+    electron_temperature: float = 5.0,
+    electron_density: float = 1e17,
+    plasma_potential: float = 5.0,
+    loop_peak: float = 0.2,
+    loop_centre: float = 45e-3,
+    loop_radius: float = 30e-3,
+    field_order: int = 3,
+    plane: float = 58e-3,
+) -> Design:
+    """Positive-ion extraction with a current-loop lens: 1025 x 257 nodes over 60 x 15 mm."""
+    from scipy import constants
+
+    h = length_x / (nx - 1)
+    radius = (ny - 1) * h
+    sx, sr = length_x / 8e-3, radius / 3e-3  # the reference geometry spans 8 x 3 mm
+    electrodes = (
+        Electrode(box(0.5e-3 * sx, 1.0e-3 * sr, 1.0e-3 * sx, 3.1e-3 * sr), 0.0, "plasma"),
+        Electrode(box(3.0e-3 * sx, 1.2e-3 * sr, 4.0e-3 * sx, 3.1e-3 * sr), -float(voltage), "extraction"),
+    )
+    faces = {
+        "x_min": FaceCondition("dirichlet", float(plasma_potential)),
+        "x_max": FaceCondition("dirichlet", -float(voltage)),
+    }
+    mesh = IonOpticsMesh((nx, ny), (0.0, 0.0), h, "cylindrical", electrodes, faces)
+    beam_end = (0.0, 0.9e-3 * sr)
+    n2 = int(round(particles * h2_share))
+    n1 = particles - n2
+    parts = []
+    jv = 0.0
+    for k, (count, share, mass) in enumerate(((n1, 1.0 - h2_share, PROTON_MASS_U), (n2, h2_share, H2_MASS_U))):
+        if count == 0:
+            continue
+        b, species_jv = beam_with_energy(
+            count, current_density * share, 1.0, mass, energy, 0.0, transverse_temperature,
+            (0.0, 0.0), beam_end, cylindrical=True, species=k,
+        )  # fmt: skip
+        parts.append(b)
+        jv += species_jv
+    beam = BeamParticles.concatenate(parts)
+    hz = 0.5 * h  # the on-axis profile at half the mesh spacing (model.py:179-188)
+    z = np.arange(-8, int(np.ceil(length_x / hz)) + 9) * hz
+    lens = AxisymmetricMagneticField(float(z[0]), hz, current_loop_on_axis(z, loop_centre, loop_radius, loop_peak),
+                                     order=field_order)  # fmt: skip
+    plasma = PlasmaModel(electron_temperature, float(plasma_potential), 0.6e-3 * sx,
+                         rho_e=constants.e * electron_density)  # fmt: skip
+    return Design(mesh, beam, jv, (plane,), integrator="boris", magnetic=node_magnetic_field(mesh, lens),
+                  plasma=plasma)  # fmt: skip
 
 
 def sample_particles(particles: BeamParticles, count: int) -> BeamParticles:

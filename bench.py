@@ -14,7 +14,17 @@ the launch stream) and the CPU baseline: the oracle's C restatement of the refer
 Pyccel tracer (OpenMP over the host cores) on an evenly strided sample of the same
 trajectories, sized to about ten seconds.
 
+Beside the headline the line carries, under ``extra``:
+
+* ``extraction`` -- the positive-ion extraction (1025 x 257 nodes over 60 x 15 mm, 30 kV,
+  H+ and H2+ 0.8:0.2, 1e6 trajectories) traced with Boris through the sampled field of a
+  0.2 T current loop (node B), on the field of the initial plasma: particle-steps/s and
+  FP64 FLOP/s at 257 FLOP per Boris+B step;
+* ``vlasov`` -- Vlasov iterations/s of the einzel lens (host Poisson solve, edge fields,
+  host-form trace with its copies, relaxation 0.5), with the time split.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--no-cpu]
+                       [--no-extra]
 """
 
 from __future__ import annotations
@@ -36,6 +46,7 @@ METRIC = "particle-steps/s plain vs batched at 1e-10 rel err; adjoint/forward ti
 #: Algorithmic FP64 work per RK4 step in cylindrical geometry: +, -, *, /, sqrt count 1,
 #: an fma 2; index arithmetic, compares and clamps 0 (derivation in DESIGN.md).
 FLOP_PER_STEP_RK4_CYL = 322
+FLOP_PER_STEP_BORIS_B_CYL = 257
 PEAK_FP64_TFLOPS = 78.6
 
 
@@ -127,6 +138,107 @@ def cpu_baseline(args_host, target_seconds: float) -> dict:
     }
 
 
+def device_arguments(host, magnetic=None):
+    """The kernel arguments on the GPU (patch tables and planes stay on the host)."""
+    import torch
+
+    dev = [torch.from_numpy(a).cuda() if isinstance(a, np.ndarray) else a for a in host]
+    for i in (10, 11, 24):
+        dev[i] = host[i]
+    mag = None
+    if magnetic is not None:
+        mag = tuple(None if c is None else torch.from_numpy(np.ascontiguousarray(c)).cuda() for c in magnetic)
+    return dev, mag
+
+
+def time_traces(host, magnetic, steps: int, warmup: int, block_size: int) -> dict:
+    """Trace every trajectory ``steps`` times (after ``warmup``), states reset before each."""
+    import torch
+
+    from beamtracer_amd import kernels
+
+    dev, mag = device_arguments(host, magnetic)
+    initial = {i: dev[i].clone() for i in (1, 2, 5, 6, 7, 25, 26, 31)}
+    exponent = kernels.trace_exponent(host[0], host[2], host[4], host[3], host[8], host[9], host[11], host[19])
+    stream = torch.cuda.current_stream()
+
+    def launch(start=None, end=None) -> None:
+        for i, a in initial.items():
+            dev[i].copy_(a)
+        if start is not None:
+            start.record(stream)
+        kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=block_size, stream=stream)
+        if end is not None:
+            end.record(stream)
+
+    for _ in range(warmup):
+        launch()
+    torch.cuda.synchronize()
+    steps_per_trace = int(dev[5].sum().item())
+    status = torch.bincount(dev[6].clamp(min=-2) + 2, minlength=7).cpu().numpy()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        launch(starts[k], ends[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_s = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)])) / 1e3
+    return {"steps_per_trace": steps_per_trace, "wall": wall, "launch_s": launch_s, "status": status.tolist(),
+            "trajectories": host[1].shape[1]}
+
+
+def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
+    from beamtracer_amd import designs, fields, poisson
+    from beamtracer_amd.tracer import HipBeamTracer
+
+    design = designs.extraction(particles=particles)
+    plasma = design.plasma
+    phi = poisson.IonOpticsPoisson(design.mesh).solve(
+        None, fixed_region=lambda x, y: x < plasma.initial_plasma_x, fixed_potential=plasma.plasma_potential)
+    field = fields.StaggeredElectricField(design.mesh, phi)
+    tracer = HipBeamTracer(design.mesh, field, design.magnetic, planes=design.planes, integrator="boris")
+    host = kernel_arguments(design, tracer)
+    r = time_traces(host, design.magnetic, steps, 1, block_size)
+    rate = r["steps_per_trace"] * steps / r["wall"]
+    flops = FLOP_PER_STEP_BORIS_B_CYL * r["steps_per_trace"] / r["launch_s"] / 1e12
+    return {
+        "workload": "positive-ion extraction, cylindrical 1025x257 (60x15 mm), 30 kV, H+/H2+ 0.8:0.2, "
+        f"{r['trajectories']} trajectories, Boris with a sampled 0.2 T current-loop field, initial-plasma field",
+        "value": rate,
+        "unit": "particle-steps/s",
+        "particle_steps_per_trace": r["steps_per_trace"],
+        "launch_ms": r["launch_s"] * 1e3,
+        "status_counts_-2..4": r["status"],
+        "fp64_tflops": flops,
+        "frac_fp64_peak": flops / PEAK_FP64_TFLOPS,
+        "flop_per_step": FLOP_PER_STEP_BORIS_B_CYL,
+    }
+
+
+def vlasov_leg(design, iterations: int) -> dict:
+    from beamtracer_amd.tracer import HipBeamTracer
+    from beamtracer_amd.vlasov import vlasov_iterate
+
+    tracer = HipBeamTracer(design.mesh, None, planes=design.planes, integrator=design.integrator)
+    t0 = time.perf_counter()
+    state = vlasov_iterate(design.mesh, lambda: (design.particles, design.j_over_v), tracer,
+                           iterations=iterations, relaxation=0.5)  # fmt: skip
+    wall = time.perf_counter() - t0
+    h = state.history[1:]  # the first iteration also factorizes the Poisson matrix
+    per = float(np.mean([x["field_solve_s"] + x["trace_s"] for x in h]))
+    return {
+        "workload": f"einzel lens Vlasov loop, {len(design.particles)} trajectories, relaxation 0.5, "
+        f"{iterations} iterations (host Poisson + edge fields + host-form hip trace with its copies)",
+        "iterations_per_s": iterations / wall,
+        "steady_iterations_per_s": 1.0 / per if per > 0 else None,
+        "field_solve_s": float(np.mean([x["field_solve_s"] for x in h])),
+        "trace_s": float(np.mean([x["trace_s"] for x in h])),
+        "max_potential_change_last": state.history[-1]["max_potential_change"],
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -136,6 +248,8 @@ def main() -> int:
     parser.add_argument("--block-size", type=int, default=0)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
+    parser.add_argument("--no-extra", action="store_true", help="only the einzel-lens headline")
+    parser.add_argument("--extraction-particles", type=int, default=1_000_000)
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="only copy 1 GiB device to device (calibrates the HBM byte counters)")
     opts = parser.parse_args()
@@ -144,8 +258,6 @@ def main() -> int:
         return 2
 
     import torch
-
-    from beamtracer_amd import kernels
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -162,41 +274,11 @@ def main() -> int:
     design, tracer = build_workload(opts.particles)
     host = kernel_arguments(design, tracer)
     n = host[1].shape[1]
-    dev = [torch.from_numpy(a).cuda() if isinstance(a, np.ndarray) else a for a in host]
-    for i in (10, 11, 24):
-        dev[i] = host[i]  # patch tables and planes are read on the host
-    initial = {i: dev[i].clone() for i in (1, 2, 5, 6, 7, 25, 26, 31)}
-    exponent = kernels.trace_exponent(host[0], host[2], host[4], host[3], host[8], host[9], host[11], host[19])
-    stream = torch.cuda.current_stream()
-
-    def one_step() -> None:
-        for i, a in initial.items():
-            dev[i].copy_(a)
-        kernels.trace_trajectories_2d(*dev, exponent=exponent, block_size=opts.block_size, stream=stream)
-
-    log(f"warmup: {opts.warmup} traces of {n} trajectories")
-    for _ in range(opts.warmup):
-        one_step()
-    torch.cuda.synchronize()
-    steps_per_trace = int(dev[5].sum().item())
-    status = torch.bincount(dev[6].clamp(min=-2) + 2, minlength=7).cpu().numpy()
-    log(f"particle-steps per trace {steps_per_trace}; status counts (-2..4) {status.tolist()}")
-
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(opts.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(opts.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(opts.steps):
-        for i, a in initial.items():
-            dev[i].copy_(a)
-        starts[k].record(stream)
-        kernels.trace_trajectories_2d(*dev, exponent=exponent, block_size=opts.block_size, stream=stream)
-        ends[k].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    log(f"warmup {opts.warmup} and {opts.steps} timed traces of {n} trajectories")
+    r = time_traces(host, None, opts.steps, opts.warmup, opts.block_size)
+    steps_per_trace, wall, avg_launch_s = r["steps_per_trace"], r["wall"], r["launch_s"]
     value = steps_per_trace * opts.steps / wall
+    log(f"particle-steps per trace {steps_per_trace}; status counts (-2..4) {r['status']}")
     log(f"timed: {opts.steps} traces in {wall:.3f} s; launch avg {avg_launch_s * 1e3:.3f} ms")
 
     achieved = FLOP_PER_STEP_RK4_CYL * steps_per_trace / avg_launch_s / 1e12
@@ -237,6 +319,12 @@ def main() -> int:
             "launch_ms": avg_launch_s * 1e3,
         },
     }
+    if not opts.no_extra:
+        log("extraction: Boris + node B, 1e6 trajectories")
+        extra = {"extraction": extraction_leg(opts.extraction_particles, max(2, opts.steps // 2), opts.block_size)}
+        log("vlasov: einzel-lens iterations")
+        extra["vlasov"] = vlasov_leg(design, 3)
+        line["extra"] = extra
     if not opts.no_cpu:
         log("cpu baseline: oracle tracer on the host cores")
         line["cpu_baseline"] = cpu_baseline(host, opts.cpu_seconds)
