@@ -78,6 +78,40 @@ class TraceArgs(ctypes.Structure):
     ]  # fmt: skip
 
 
+class BatchDesign(ctypes.Structure):
+    """ctypes mirror of ``bt_batch_design``."""
+
+    _fields_ = [
+        ("nx", _i64), ("ny", _i64),
+        ("origin_x", _dbl), ("origin_y", _dbl), ("h", _dbl), ("upper_x", _dbl), ("upper_y", _dbl),
+        ("fx_offset", _i64), ("fy_offset", _i64),
+        ("cells_offset", _i64), ("cells_rows", _i64), ("cells_cols", _i64),
+        ("subcells_offset", _i64), ("n_subcells", _i64),
+        ("node_offset", _i64), ("n_nodes", _i64),
+        ("deposit_exponent", _i32), ("reserved", _i32),
+    ]  # fmt: skip
+
+
+class BatchArgs(ctypes.Structure):
+    """ctypes mirror of ``bt_batch_args``."""
+
+    _fields_ = [
+        ("active", _ptr), ("n_active", _i64),
+        ("x", _ptr), ("v", _ptr), ("n", _i64),
+        ("x0", _ptr), ("v0", _ptr),
+        ("qm", _ptr), ("dq", _ptr),
+        ("steps", _ptr), ("status", _ptr), ("electrode", _ptr),
+        ("design_of", _ptr),
+        ("fxs", _ptr), ("fys", _ptr), ("cells", _ptr), ("subcells", _ptr),
+        ("planes", _ptr), ("n_planes", _i64),
+        ("hits", _ptr), ("capacity", _i64), ("counts", _ptr),
+        ("charge", _ptr),
+        ("cylindrical", _i64), ("integrator", _i64),
+        ("step_fraction", _dbl), ("max_steps", _i64), ("refine", _i64),
+        ("block_size", _i32), ("max_blocks", _i32),
+    ]  # fmt: skip
+
+
 #: Every exported symbol of the C ABI, with (restype, argtypes).
 SIGNATURES = {
     "bt_version": (ctypes.c_char_p, []),
@@ -103,6 +137,19 @@ SIGNATURES = {
         ctypes.c_int,
         [_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _i64, _ptr, _i64, _i64, _i32, _ptr, _i64, _ptr],
     ),
+    "bt_batch_args_size": (_i64, []),
+    "bt_batch_design_size": (_i64, []),
+    "bt_batch_plan_create": (
+        ctypes.c_int, [ctypes.POINTER(BatchDesign), _i64, _i64, _i64, _i64, _i64, ctypes.POINTER(_ptr)],
+    ),
+    "bt_batch_plan_destroy": (ctypes.c_int, [_ptr]),
+    "bt_batch_plan_update": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchDesign), _i64]),
+    "bt_batch_plan_nodes": (_i64, [_ptr]),
+    "bt_batch_occupancy": (ctypes.c_int, [_i64, _i64, _i32]),
+    "bt_batch_trace_device": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchArgs), _ptr]),
+    "bt_batch_graph_create": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchArgs), ctypes.POINTER(_ptr)]),
+    "bt_graph_launch": (ctypes.c_int, [_ptr, _ptr]),
+    "bt_graph_destroy": (ctypes.c_int, [_ptr]),
 }
 
 _LIB: ctypes.CDLL | None = None

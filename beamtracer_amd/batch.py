@@ -1,0 +1,497 @@
+"""Many designs in one persistent launch: the batched form of the particle stage.
+
+The reference's design search (``optimization/search.py:200`` ``optimize``, through the
+worker processes of ``optimization/pool.py:110-160``) evaluates every candidate design in a
+process of its own, each tracing its beam on its own mesh. :class:`BatchedTracer` traces the
+beams of many designs -- meshes of different sizes, fields, electrodes and beams -- in one
+persistent launch of the batched kernel (``bt_batch_trace_device``): the trajectories of
+every design share one work queue, and a lane that finishes a trajectory of one design takes
+the next trajectory of any design. Per design the protocol is :class:`HipBeamTracer`'s
+(trajectories starting in an electrode end at once, paused ones are decided on the host and
+resumed, a buffer overflow repeats the trace, node charges become densities) and the results
+equal the plain tracer's bit for bit: each design keeps its own fixed-point exponent, and the
+node sums are exact integers.
+
+For the Vlasov iteration of every design at once, :meth:`BatchedTracer.capture` records the
+trace from the emitter (state reset, queue, trace, finalize) as one HIP graph, replayed after
+each field update by :meth:`BatchedTracer.replay`. :func:`autotune` picks the launch shape
+(threads per block, resident blocks) and the order of the queue by timing them.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from collections.abc import Sequence
+from dataclasses import dataclass
+
+import numpy as np
+
+from beamtracer_amd import _lib, kernels
+from beamtracer_amd.beam import BeamParticles
+from beamtracer_amd.tracer import (
+    _ALIVE,
+    _PAUSED,
+    ELECTRODE,
+    INTEGRATORS,
+    SUBCELLS,
+    HipBeamTracer,
+    TraceResult,
+    collect_plane_hits,
+    geometry_code,
+)
+
+ORDERS = ("design", "longest")
+
+
+@dataclass
+class LaunchShape:
+    """Threads per block (0: 256), resident blocks (0: all that fit) and queue order."""
+
+    block_size: int = 0
+    max_blocks: int = 0
+    order: str = "longest"
+
+
+class BatchedTracer:
+    """Trace the beams of many designs in one launch; one :class:`HipBeamTracer` per design.
+
+    Every design is a single uniform mesh (no refined patches, no graded axis), classical
+    and electrostatic, without saved paths; all share the integrator, geometry, step
+    fraction, step limit and number of diagnostic planes.
+    """
+
+    def __init__(self, tracers: Sequence[HipBeamTracer], shape: LaunchShape | None = None) -> None:
+        if not tracers:
+            raise ValueError("no designs")
+        t0 = tracers[0]
+        for t in tracers:
+            if t.mesh.graded or t.relativistic or t.magnetic is not None or t.save_trajectories:
+                raise ValueError("the batched form takes uniform, classical, electrostatic designs without paths")
+            if (t.integrator, geometry_code(t.mesh), t.step_fraction, t.max_steps, len(t.planes)) != (
+                t0.integrator, geometry_code(t0.mesh), t0.step_fraction, t0.max_steps, len(t0.planes)
+            ):  # fmt: skip
+                raise ValueError("the designs must share integrator, geometry, step rule and plane count")
+        self.tracers = list(tracers)
+        self.shape = shape or LaunchShape()
+        self.capacity = 4
+        self._plan = None
+        self._graph = None
+        self._table = None
+        self._dev: dict = {}
+        self._build_arenas()
+
+    # -- arenas and plan ----------------------------------------------------------------
+
+    def _build_arenas(self) -> None:
+        fxs, fys, cells, subs, planes = [], [], [], [], []
+        rows = []
+        ofx = ofy = ofc = ofs = ofn = 0
+        for t in self.tracers:
+            if t.electric is None:
+                raise RuntimeError("a design has no electric field yet")
+            fx, fy, _, patch_real = t.patch_arrays()
+            c, s = t.electrode_lookup()
+            mesh = t.mesh
+            nx, ny = mesh.shape
+            rows.append((nx, ny, *map(float, mesh.logical_origin), mesh.h, *map(float, mesh.logical_upper),
+                         ofx, ofy, ofc, c.shape[0], c.shape[1], ofs, s.size, ofn, nx * ny))  # fmt: skip
+            fxs.append(fx)
+            fys.append(fy)
+            cells.append(c.ravel())
+            subs.append(s)
+            planes.append(np.array(t.planes, dtype=float))
+            ofx += fx.size
+            ofy += fy.size
+            ofc += c.size
+            ofs += s.size
+            ofn += nx * ny
+        self._rows = rows
+        self.total_nodes = ofn
+        self._fxs = np.concatenate(fxs)
+        self._fys = np.concatenate(fys)
+        self._cells = np.concatenate(cells).astype(np.int64)
+        self._subcells = np.concatenate(subs).astype(np.int64)
+        n_planes = len(self.tracers[0].planes)
+        self._planes = np.concatenate(planes) if n_planes else np.full(len(self.tracers), np.nan)
+        self._n_planes = n_planes
+
+    def update_fields(self) -> None:
+        """Take every design's current electric field (e.g. after a field solve)."""
+        self._fxs = np.concatenate([t.patch_arrays()[0] for t in self.tracers])
+        self._fys = np.concatenate([t.patch_arrays()[1] for t in self.tracers])
+        if "fxs" in self._dev:
+            import torch
+
+            self._dev["fxs"].copy_(torch.from_numpy(self._fxs))
+            self._dev["fys"].copy_(torch.from_numpy(self._fys))
+
+    def _design_table(self, exponents: Sequence[int]):
+        table = (_lib.BatchDesign * len(self.tracers))()
+        for d, (row, e) in enumerate(zip(self._rows, exponents)):
+            table[d] = _lib.BatchDesign(*row, int(e), 0)
+        return table
+
+    def _ensure_plan(self, exponents: Sequence[int]) -> None:
+        import torch
+
+        L = _lib.lib()
+        table = self._design_table(exponents)
+        if self._plan is None:
+            plan = ctypes.c_void_p()
+            rc = L.bt_batch_plan_create(table, len(self.tracers), self._fxs.size, self._fys.size, self._cells.size,
+                                        self._subcells.size, ctypes.byref(plan))  # fmt: skip
+            _lib.check(rc, "bt_batch_plan_create")
+            self._plan = plan
+            self._dev["fxs"] = torch.from_numpy(self._fxs).cuda()
+            self._dev["fys"] = torch.from_numpy(self._fys).cuda()
+            self._dev["cells"] = torch.from_numpy(self._cells).cuda()
+            self._dev["subcells"] = torch.from_numpy(self._subcells).cuda()
+            self._dev["planes"] = torch.from_numpy(np.ascontiguousarray(self._planes)).cuda()
+            self._dev["charge"] = torch.zeros(self.total_nodes, dtype=torch.float64, device="cuda")
+        else:
+            rc = L.bt_batch_plan_update(self._plan, table, len(self.tracers))
+            _lib.check(rc, "bt_batch_plan_update")
+        self._table = list(exponents)
+
+    def close(self) -> None:
+        L = _lib.lib()
+        if self._graph is not None:
+            L.bt_graph_destroy(self._graph)
+            self._graph = None
+        if self._plan is not None:
+            L.bt_batch_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- the trace ----------------------------------------------------------------------
+
+    def _prepare(self, particles: Sequence[BeamParticles]) -> dict:
+        """Concatenated host state of every design's beam, as the plain tracer starts it."""
+        if len(particles) != len(self.tracers):
+            raise ValueError("one beam per design")
+        xs, vs, qms, dqs, currents, species, design_of, first = [], [], [], [], [], [], [], []
+        status, electrode, starts, exponents = [], [], [], []
+        n = 0
+        for d, (t, p) in enumerate(zip(self.tracers, particles)):
+            x = np.array(p.position, dtype=float, order="C")
+            v = t._state(p.velocity)
+            qm = np.ascontiguousarray(p.charge / p.mass, dtype=float)
+            cur = p.current.astype(float)
+            dq = np.ascontiguousarray(np.sign(p.charge) * np.abs(cur))
+            st = np.full(len(p), _ALIVE, dtype=np.int64)
+            el = np.full(len(p), -1, dtype=np.int64)
+            if t.mesh.electrodes and len(p):
+                hit = t.mesh.inside_solid(*t._project(x))
+                st[hit >= 0] = ELECTRODE
+                el[hit >= 0] = hit[hit >= 0]
+            start = np.flatnonzero(st == _ALIVE).astype(np.int64)
+            fx, fy, _, patch_real = t.patch_arrays()
+            exponents.append(kernels.trace_exponent(start, v, dq, qm, fx, fy, patch_real, t.step_fraction,
+                                                    t._inv_c2()))  # fmt: skip
+            xs.append(x)
+            vs.append(v)
+            qms.append(qm)
+            dqs.append(dq)
+            currents.append(cur)
+            species.append(p.species)
+            design_of.append(np.full(len(p), d, dtype=np.int32))
+            status.append(st)
+            electrode.append(el)
+            starts.append(start + n)
+            first.append(n)
+            n += len(p)
+        first.append(n)
+        return {
+            "x": np.ascontiguousarray(np.concatenate(xs, axis=1)),
+            "v": np.ascontiguousarray(np.concatenate(vs, axis=1)),
+            "qm": np.concatenate(qms), "dq": np.concatenate(dqs), "current": np.concatenate(currents),
+            "species": np.concatenate(species), "design_of": np.concatenate(design_of),
+            "status": np.concatenate(status), "electrode": np.concatenate(electrode),
+            "starts": starts, "first": first, "n": n, "exponents": exponents,
+        }  # fmt: skip
+
+    def _queue(self, starts: Sequence[np.ndarray]) -> np.ndarray:
+        """The active list: designs in queue order (``longest``: most steps first)."""
+        order = range(len(self.tracers))
+        if self.shape.order == "longest":
+            steps = [(t.mesh.upper[0] - t.mesh.origin[0]) / t.mesh.h for t in self.tracers]
+            order = sorted(order, key=lambda d: -steps[d])
+        elif self.shape.order != "design":
+            raise ValueError(f"order must be one of {ORDERS}")
+        parts = [starts[d] for d in order]
+        return np.concatenate(parts).astype(np.int64) if parts else np.zeros(0, np.int64)
+
+    def _args(self, dev: dict, active, reset: bool):
+        t0 = self.tracers[0]
+        a = _lib.BatchArgs()
+        a.active = active.data_ptr() if active.numel() else None
+        a.n_active = active.numel()
+        a.x, a.v, a.n = dev["x"].data_ptr(), dev["v"].data_ptr(), dev["x"].shape[1]
+        a.x0 = dev["x0"].data_ptr() if reset else None
+        a.v0 = dev["v0"].data_ptr() if reset else None
+        a.qm, a.dq = dev["qm"].data_ptr(), dev["dq"].data_ptr()
+        a.steps, a.status, a.electrode = dev["steps"].data_ptr(), dev["status"].data_ptr(), dev["electrode"].data_ptr()
+        a.design_of = dev["design_of"].data_ptr()
+        a.fxs, a.fys = self._dev["fxs"].data_ptr(), self._dev["fys"].data_ptr()
+        a.cells, a.subcells = self._dev["cells"].data_ptr(), self._dev["subcells"].data_ptr()
+        a.planes, a.n_planes = self._dev["planes"].data_ptr(), self._n_planes
+        a.hits, a.capacity, a.counts = dev["hits"].data_ptr(), dev["hits"].shape[2], dev["counts"].data_ptr()
+        a.charge = self._dev["charge"].data_ptr()
+        a.cylindrical = geometry_code(t0.mesh)
+        a.integrator = INTEGRATORS.index(t0.integrator)
+        a.step_fraction, a.max_steps, a.refine = t0.step_fraction, t0.max_steps, SUBCELLS
+        a.block_size, a.max_blocks = self.shape.block_size, self.shape.max_blocks
+        return a
+
+    def _upload(self, host: dict, capacity: int) -> dict:
+        import torch
+
+        n = host["n"]
+        dev = {k: torch.from_numpy(host[k]).cuda() for k in ("x", "v", "qm", "dq", "status", "electrode",
+                                                               "design_of")}  # fmt: skip
+        dev["x0"], dev["v0"] = dev["x"].clone(), dev["v"].clone()
+        dev["status0"], dev["electrode0"] = dev["status"].clone(), dev["electrode"].clone()
+        dev["steps"] = torch.zeros(n, dtype=torch.int64, device="cuda")
+        dev["hits"] = torch.zeros((n, max(self._n_planes, 1), capacity, 8), dtype=torch.float64, device="cuda")
+        dev["counts"] = torch.zeros((n, max(self._n_planes, 1)), dtype=torch.int64, device="cuda")
+        dev["queue"] = torch.from_numpy(self._queue(host["starts"])).cuda()
+        return dev
+
+    def _restart(self, dev: dict) -> None:
+        for k in ("x", "v", "status", "electrode"):
+            dev[k].copy_(dev[k + "0"])
+        dev["steps"].zero_()
+        dev["counts"].zero_()
+        self._dev["charge"].zero_()
+
+    def launch(self, dev: dict, active, reset: bool = False, stream=None) -> None:
+        """One batched kernel call over ``active`` (asynchronous on ``stream``)."""
+        import torch
+
+        st = stream if stream is not None else torch.cuda.current_stream()
+        a = self._args(dev, active, reset)
+        rc = _lib.lib().bt_batch_trace_device(self._plan, ctypes.byref(a), ctypes.c_void_p(st.cuda_stream))
+        _lib.check(rc, "bt_batch_trace_device")
+
+    def _resume_paused(self, dev: dict, host: dict) -> None:
+        """Decide paused trajectories with the electrodes' own test and resume the others,
+        design by design, until none is paused (``tracer.py:1236-1256``)."""
+        import torch
+
+        while True:
+            torch.cuda.synchronize()
+            status = dev["status"].cpu().numpy()
+            paused = np.flatnonzero(status == _PAUSED)
+            if not paused.size:
+                return
+            x = dev["x"][:, torch.from_numpy(paused).cuda()].cpu().numpy()
+            resume = []
+            electrode = dev["electrode"].cpu().numpy()
+            for d, t in enumerate(self.tracers):
+                lo, hi = host["first"][d], host["first"][d + 1]
+                sel = (paused >= lo) & (paused < hi)
+                if not sel.any():
+                    continue
+                hit = t.mesh.inside_solid(*t._project(x[:, sel]))
+                ids = paused[sel]
+                status[ids[hit >= 0]] = ELECTRODE
+                electrode[ids[hit >= 0]] = hit[hit >= 0]
+                resume.append(ids[hit < 0])
+            active = np.concatenate(resume) if resume else np.zeros(0, np.int64)
+            status[active] = _ALIVE
+            dev["status"].copy_(torch.from_numpy(status))
+            dev["electrode"].copy_(torch.from_numpy(electrode))
+            if not active.size:
+                return
+            self.launch(dev, torch.from_numpy(active.astype(np.int64)).cuda())
+
+    def _collect(self, dev: dict, host: dict) -> list[TraceResult]:
+        import torch
+
+        torch.cuda.synchronize()
+        status = dev["status"].cpu().numpy()
+        electrode = dev["electrode"].cpu().numpy()
+        dead = dev["status0"].cpu().numpy() == ELECTRODE
+        status[dead] = ELECTRODE
+        electrode[dead] = dev["electrode0"].cpu().numpy()[dead]
+        steps = dev["steps"].cpu().numpy()
+        hits = dev["hits"].cpu().numpy()
+        counts = dev["counts"].cpu().numpy()
+        charge = self._dev["charge"].cpu().numpy()
+        x = dev["x"].cpu().numpy()
+        v = dev["v"].cpu().numpy()
+        out = []
+        for d, t in enumerate(self.tracers):
+            lo, hi = host["first"][d], host["first"][d + 1]
+            row = self._rows[d]
+            nodes = charge[row[14]: row[14] + row[15]].reshape(t.mesh.shape)
+            planes = collect_plane_hits(t.planes, hits[lo:hi], counts[lo:hi], host["current"][lo:hi],
+                                        host["species"][lo:hi])  # fmt: skip
+            res = TraceResult(
+                charge_density=t.finalize(nodes.copy()),
+                status=status[lo:hi].astype(np.int8),
+                electrode=electrode[lo:hi].astype(np.int32),
+                planes=planes,
+                trajectories=[],
+                trajectory_species=np.zeros(0, dtype=np.int32),
+                steps=int(steps[lo:hi].max()) if hi > lo else 0,
+                particle_steps=int(steps[lo:hi].sum()),
+            )
+            t.last_state = (x[:, lo:hi], v[:, lo:hi], steps[lo:hi])
+            out.append(res)
+        return out
+
+    def trace(self, particles: Sequence[BeamParticles]) -> list[TraceResult]:
+        """Trace every design's beam; one :class:`TraceResult` per design."""
+        host = self._prepare(particles)
+        self._ensure_plan(host["exponents"])
+        capacity = self.capacity
+        while True:
+            dev = self._upload(host, capacity)
+            self._restart(dev)
+            self.launch(dev, dev["queue"])
+            self._resume_paused(dev, host)
+            most = int(dev["counts"].max().item()) if host["n"] else 0
+            if most <= capacity:
+                break
+            capacity = most
+        self.capacity = capacity
+        self.last_device_state = dev
+        return self._collect(dev, host)
+
+    # -- graph capture for the Vlasov iteration -----------------------------------------
+
+    def capture(self, particles: Sequence[BeamParticles]) -> None:
+        """Record the trace from the emitter (reset, queue, trace, finalize) as a HIP graph."""
+        host = self._prepare(particles)
+        self._ensure_plan(host["exponents"])
+        dev = self._upload(host, self.capacity)
+        L = _lib.lib()
+        if self._graph is not None:
+            L.bt_graph_destroy(self._graph)
+            self._graph = None
+        a = self._args(dev, dev["queue"], reset=True)
+        graph = ctypes.c_void_p()
+        import torch
+
+        torch.cuda.synchronize()
+        _lib.check(L.bt_batch_graph_create(self._plan, ctypes.byref(a), ctypes.byref(graph)), "bt_batch_graph_create")
+        self._graph = graph
+        self._graph_state = (host, dev, a)
+
+    def replay(self, refresh_exponents: bool = True) -> list[TraceResult]:
+        """Replay the captured trace on the current fields (after :meth:`update_fields`)."""
+        import torch
+
+        if self._graph is None:
+            raise RuntimeError("nothing captured")
+        host, dev, _ = self._graph_state
+        if refresh_exponents:
+            fresh = []
+            for d, t in enumerate(self.tracers):
+                lo, hi = host["first"][d], host["first"][d + 1]
+                fx, fy, _, patch_real = t.patch_arrays()
+                fresh.append(kernels.trace_exponent(host["starts"][d] - lo, host["v"][:, lo:hi], host["dq"][lo:hi],
+                                                    host["qm"][lo:hi], fx, fy, patch_real, t.step_fraction))  # fmt: skip
+            if fresh != self._table:
+                torch.cuda.synchronize()
+                self._ensure_plan(fresh)
+        st = torch.cuda.current_stream()
+        _lib.check(_lib.lib().bt_graph_launch(self._graph, ctypes.c_void_p(st.cuda_stream)), "bt_graph_launch")
+        self._resume_paused(dev, host)
+        if int(dev["counts"].max().item()) > dev["hits"].shape[2]:
+            raise RuntimeError("plane buffers overflowed in a captured trace: trace() first to size them")
+        return self._collect(dev, host)
+
+
+def _vacuum_arrays(spec: dict) -> tuple:
+    """The vacuum edge fields and the electrode lookup of the einzel design ``spec``."""
+    from beamtracer_amd import designs, fields, poisson
+
+    design = designs.einzel_lens(**spec)
+    phi = poisson.IonOpticsPoisson(design.mesh).solve()
+    field = fields.StaggeredElectricField(design.mesh, phi)
+    cells, subcells = HipBeamTracer(design.mesh, field).electrode_lookup()
+    return phi, field.fx, field.fy, cells, subcells
+
+
+def vacuum_tracers(designs: Sequence, processes: int = 0) -> list[HipBeamTracer]:
+    """One :class:`HipBeamTracer` per einzel design on its vacuum field (host Poisson solves
+    and electrode lookups, in ``processes`` worker processes when above 1)."""
+    from beamtracer_amd.fields import StaggeredElectricField
+
+    specs = [d.spec for d in designs]
+    if processes > 1 and all(s is not None for s in specs):
+        import multiprocessing
+        from concurrent.futures import ProcessPoolExecutor
+
+        with ProcessPoolExecutor(processes, mp_context=multiprocessing.get_context("spawn")) as pool:
+            arrays = list(pool.map(_vacuum_arrays, specs, chunksize=2))
+    else:
+        arrays = []
+        for d in designs:
+            from beamtracer_amd import fields, poisson
+
+            phi = poisson.IonOpticsPoisson(d.mesh).solve()
+            f = fields.StaggeredElectricField(d.mesh, phi)
+            cells, subcells = HipBeamTracer(d.mesh, f).electrode_lookup()
+            arrays.append((phi, f.fx, f.fy, cells, subcells))
+    tracers = []
+    for d, (phi, fx, fy, cells, subcells) in zip(designs, arrays):
+        t = HipBeamTracer(d.mesh, StaggeredElectricField.from_edges(d.mesh, fx, fy), planes=d.planes,
+                          step_fraction=d.step_fraction, max_steps=d.max_steps, integrator=d.integrator)  # fmt: skip
+        t._lookup = (cells, subcells)
+        t.potential = phi
+        tracers.append(t)
+    return tracers
+
+
+def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, blocks=(64, 128, 192, 256),
+             fills=(1.0, 0.5), orders=ORDERS, repeats: int = 2) -> dict:
+    """Time one batched trace per launch shape and keep the fastest in ``tracer.shape``.
+
+    The candidates are threads per block, the fraction of the resident blocks the grid
+    fills, and the queue order; each is timed (HIP events on the launch stream, best of
+    ``repeats``) on the real batch. Returns every timing in ms and the choice.
+    """
+    import torch
+
+    host = tracer._prepare(particles)
+    tracer._ensure_plan(host["exponents"])
+    t0 = tracer.tracers[0]
+    L = _lib.lib()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    timings = {}
+    best = None
+    for order in orders:
+        tracer.shape = LaunchShape(0, 0, order)
+        dev = tracer._upload(host, tracer.capacity)
+        for block in blocks:
+            per_cu = L.bt_batch_occupancy(INTEGRATORS.index(t0.integrator), geometry_code(t0.mesh), block)
+            for fill in fills:
+                grid = max(1, int(per_cu * cus * fill))
+                tracer.shape = LaunchShape(block, grid, order)
+                times = []
+                for _ in range(repeats):
+                    tracer._restart(dev)
+                    start = torch.cuda.Event(enable_timing=True)
+                    end = torch.cuda.Event(enable_timing=True)
+                    stream = torch.cuda.current_stream()
+                    start.record(stream)
+                    tracer.launch(dev, dev["queue"])
+                    end.record(stream)
+                    torch.cuda.synchronize()
+                    times.append(start.elapsed_time(end))
+                key = f"{order}/block{block}/fill{fill}"
+                timings[key] = min(times)
+                if best is None or min(times) < best[0]:
+                    best = (min(times), LaunchShape(block, grid, order))
+    assert best is not None
+    tracer.shape = best[1]
+    return {"timings_ms": timings, "choice": vars(best[1]), "best_ms": best[0], "time": time.time()}

@@ -239,6 +239,122 @@ int host_stream(DeviceState& s) {
     return BT_OK;
 }
 
+// A batch of designs: the validated design table on the device and the fixed-point
+// workspace of all their nodes (allocated once, so that launches allocate nothing and can
+// be captured in a graph).
+struct BatchPlan {
+    int device = 0;
+    int cus = 0;
+    std::vector<bt_batch_design> designs;
+    bt::BatchDesign* d_designs = nullptr;
+    void* workspace = nullptr;
+    i64 total_nodes = 0, max_nodes = 0;
+    i64 n_fxs = 0, n_fys = 0, n_cells = 0, n_subcells = 0;
+};
+
+struct GraphHandle {
+    hipGraphExec_t exec = nullptr;
+};
+
+static_assert(sizeof(bt_batch_design) == sizeof(bt::BatchDesign), "bt_batch_design layout");
+
+// Every design against the arena sizes; the node total and the largest design.
+int check_designs(const bt_batch_design* designs, i64 n_designs, i64 n_fxs, i64 n_fys, i64 n_cells,
+                  i64 n_subcells, i64* total_out, i64* most_out) {
+    if (n_designs < 1 || designs == nullptr) return fail(BT_E_ARG, "no designs");
+    if (n_designs > 65535) return fail(BT_E_ARG, "at most 65535 designs");
+    i64 total = 0, most = 0;
+    for (i64 d = 0; d < n_designs; ++d) {
+        const bt_batch_design& b = designs[d];
+        const std::string at = "design " + std::to_string(d) + ": ";
+        if (b.nx < 2 || b.ny < 2) return fail(BT_E_ARG, at + "fewer than 2 nodes per axis");
+        if ((b.nx + 1) * (b.ny + 1) >= ((i64)1 << 29)) return fail(BT_E_ARG, at + "2^29 nodes or more");
+        if (!(b.h > 0.0)) return fail(BT_E_ARG, at + "h must be positive");
+        if (b.fx_offset < 0 || b.fx_offset + (b.nx + 1) * b.ny > n_fxs) return fail(BT_E_ARG, at + "fx beyond fxs");
+        if (b.fy_offset < 0 || b.fy_offset + b.nx * (b.ny + 1) > n_fys) return fail(BT_E_ARG, at + "fy beyond fys");
+        if (b.cells_rows < b.nx - 1 || b.cells_cols < b.ny - 1)
+            return fail(BT_E_ARG, at + "cells must be at least (nx - 1, ny - 1)");
+        if (b.cells_offset < 0 || b.cells_offset + b.cells_rows * b.cells_cols > n_cells)
+            return fail(BT_E_ARG, at + "cells beyond the cell arena");
+        if (b.n_subcells < 1 || b.subcells_offset < 0 || b.subcells_offset + b.n_subcells > n_subcells)
+            return fail(BT_E_ARG, at + "subcells beyond the sub-cell arena");
+        if (b.n_nodes != b.nx * b.ny || b.node_offset < 0) return fail(BT_E_ARG, at + "n_nodes must be nx * ny");
+        if (b.node_offset + b.n_nodes > total) total = b.node_offset + b.n_nodes;
+        if (b.n_nodes > most) most = b.n_nodes;
+    }
+    *total_out = total;
+    *most_out = most;
+    return BT_OK;
+}
+
+int check_batch(const BatchPlan* plan, const bt_batch_args* a) {
+    if (plan == nullptr) return fail(BT_E_ARG, "plan is NULL");
+    if (a == nullptr) return fail(BT_E_ARG, "args is NULL");
+    if (a->n < 0 || a->n_active < 0) return fail(BT_E_ARG, "negative trajectory count");
+    if (a->n_planes < 0 || a->n_planes > kMaxPlanes) return fail(BT_E_ARG, "n_planes must be 0..256");
+    if (a->cylindrical < 0 || a->cylindrical > 2) return fail(BT_E_ARG, "cylindrical must be 0, 1 or 2");
+    if (a->integrator < 0 || a->integrator > 1) return fail(BT_E_ARG, "integrator must be 0 or 1");
+    if (a->refine < 1) return fail(BT_E_ARG, "refine must be positive");
+    if (a->capacity < 0) return fail(BT_E_ARG, "negative capacity");
+    if (a->block_size != 0 && (a->block_size % 64 != 0 || a->block_size < 64 || a->block_size > 256))
+        return fail(BT_E_ARG, "block_size must be a multiple of 64 in 64..256");
+    if ((a->x0 == nullptr) != (a->v0 == nullptr)) return fail(BT_E_ARG, "x0 and v0 go together");
+    if (a->n > 0 && (a->x == nullptr || a->v == nullptr || a->qm == nullptr || a->dq == nullptr ||
+                     a->steps == nullptr || a->status == nullptr || a->electrode == nullptr ||
+                     a->design_of == nullptr))
+        return fail(BT_E_ARG, "a trajectory array is NULL");
+    if (a->n_active > 0 && a->active == nullptr) return fail(BT_E_ARG, "active is NULL");
+    if (a->n_planes > 0 && (a->planes == nullptr || a->counts == nullptr || (a->capacity > 0 && a->hits == nullptr)))
+        return fail(BT_E_ARG, "plane arrays are NULL");
+    if (a->fxs == nullptr || a->fys == nullptr || a->cells == nullptr || a->subcells == nullptr ||
+        a->charge == nullptr)
+        return fail(BT_E_ARG, "a field, lookup or charge array is NULL");
+    return BT_OK;
+}
+
+bt::TraceParams batch_params(const BatchPlan* plan, const bt_batch_args* a) {
+    bt::TraceParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.active = a->active; p.n_active = a->n_active;
+    p.x = a->x; p.v = a->v; p.n = a->n;
+    p.qm = a->qm; p.dq = a->dq;
+    p.steps = (i64*)a->steps; p.status = (i64*)a->status; p.electrode = (i64*)a->electrode;
+    p.fxs = a->fxs; p.fys = a->fys;
+    p.n_patches = 1;
+    p.step_fraction = a->step_fraction; p.max_steps = a->max_steps;
+    p.cells = (const i64*)a->cells; p.subcells = (const i64*)a->subcells; p.refine = a->refine;
+    p.planes = a->planes; p.n_planes = (int)a->n_planes;
+    p.hits = a->hits; p.capacity = a->capacity; p.counts = (i64*)a->counts;
+    p.reuse = 1;  // every design is one uniform patch: the mesh itself
+    char* base = static_cast<char*>(plan->workspace);
+    p.acc = reinterpret_cast<u64*>(base);
+    p.queue = reinterpret_cast<u64*>(base + node_bytes(plan->total_nodes));
+    p.charge = a->charge;
+    p.designs = plan->d_designs; p.design_of = a->design_of; p.n_designs = (i64)plan->designs.size();
+    return p;
+}
+
+int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream) {
+    const bt::TraceParams p = batch_params(plan, a);
+    const int integ = (int)a->integrator, geom = (int)a->cylindrical;
+    const int block = a->block_size > 0 ? a->block_size : 256;
+    int grid = 0;
+    if (a->n_active > 0) {
+        int per_cu = bt::batched_occupancy(integ, geom, false, block);
+        if (per_cu < 1) per_cu = 1;
+        const i64 full = (i64)per_cu * plan->cus;
+        const i64 need = (a->n_active + block - 1) / block;
+        grid = (int)(need < full ? need : full);
+        if (a->max_blocks > 0 && grid > a->max_blocks) grid = a->max_blocks;
+        if (grid < 1) grid = 1;
+    }
+    const bool reset = a->x0 != nullptr;
+    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, false, plan->total_nodes,
+                           plan->max_nodes))
+        return fail(BT_E_LAUNCH, std::string("batched launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -435,6 +551,138 @@ int bt_deposit_trajectories_2d(const double* px, const double* py, const double*
     err = hipStreamSynchronize(st);
     if (err != hipSuccess) return hip_fail(err, "deposit");
     for (i64 c = 0; c < nodes; ++c) charge[c] += added[(size_t)c];
+    return BT_OK;
+}
+
+
+int64_t bt_batch_args_size(void) { return (int64_t)sizeof(bt_batch_args); }
+
+int64_t bt_batch_design_size(void) { return (int64_t)sizeof(bt_batch_design); }
+
+int bt_batch_plan_create(const bt_batch_design* designs, int64_t n_designs, int64_t n_fxs, int64_t n_fys,
+                         int64_t n_cells, int64_t n_subcells, void** plan_out) {
+    if (plan_out == nullptr) return fail(BT_E_ARG, "plan is NULL");
+    *plan_out = nullptr;
+    i64 total = 0, most = 0;
+    int rc = check_designs(designs, n_designs, n_fxs, n_fys, n_cells, n_subcells, &total, &most);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    if ((rc = device_state(&s))) return rc;
+    BatchPlan* plan = new BatchPlan();
+    plan->designs.assign(designs, designs + n_designs);
+    plan->total_nodes = total;
+    plan->max_nodes = most;
+    plan->n_fxs = n_fxs; plan->n_fys = n_fys; plan->n_cells = n_cells; plan->n_subcells = n_subcells;
+    plan->cus = s->cus;
+    (void)hipGetDevice(&plan->device);
+    hipError_t err = hipMalloc(reinterpret_cast<void**>(&plan->d_designs), sizeof(bt::BatchDesign) * (size_t)n_designs);
+    if (err == hipSuccess) err = hipMalloc(&plan->workspace, node_bytes(total) + 256);
+    if (err == hipSuccess)
+        err = hipMemcpy(plan->d_designs, designs, sizeof(bt::BatchDesign) * (size_t)n_designs, hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        if (plan->d_designs) (void)hipFree(plan->d_designs);
+        if (plan->workspace) (void)hipFree(plan->workspace);
+        delete plan;
+        return fail(BT_E_NOMEM, std::string("batch plan: ") + hipGetErrorString(err));
+    }
+    *plan_out = plan;
+    return BT_OK;
+}
+
+int bt_batch_plan_destroy(void* handle) {
+    BatchPlan* plan = static_cast<BatchPlan*>(handle);
+    if (plan == nullptr) return BT_OK;
+    (void)hipDeviceSynchronize();
+    if (plan->d_designs) (void)hipFree(plan->d_designs);
+    if (plan->workspace) (void)hipFree(plan->workspace);
+    delete plan;
+    return BT_OK;
+}
+
+int bt_batch_plan_update(void* handle, const bt_batch_design* designs, int64_t n_designs) {
+    BatchPlan* plan = static_cast<BatchPlan*>(handle);
+    if (plan == nullptr) return fail(BT_E_ARG, "plan is NULL");
+    if (n_designs != (i64)plan->designs.size()) return fail(BT_E_ARG, "the design count differs from the plan's");
+    i64 total = 0, most = 0;
+    int rc = check_designs(designs, n_designs, plan->n_fxs, plan->n_fys, plan->n_cells, plan->n_subcells,
+                           &total, &most);
+    if (rc) return rc;
+    if (total > plan->total_nodes) return fail(BT_E_ARG, "the designs hold more nodes than the plan");
+    hipError_t err = hipDeviceSynchronize();
+    if (err == hipSuccess)
+        err = hipMemcpy(plan->d_designs, designs, sizeof(bt::BatchDesign) * (size_t)n_designs, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return hip_fail(err, "bt_batch_plan_update");
+    plan->designs.assign(designs, designs + n_designs);
+    plan->max_nodes = most > plan->max_nodes ? most : plan->max_nodes;
+    return BT_OK;
+}
+
+int64_t bt_batch_plan_nodes(void* handle) {
+    const BatchPlan* plan = static_cast<const BatchPlan*>(handle);
+    return plan ? plan->total_nodes : -1;
+}
+
+int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size) {
+    if (integrator < 0 || integrator > 1 || cylindrical < 0 || cylindrical > 2) return fail(BT_E_ARG, "bad variant");
+    if (block_size % 64 != 0 || block_size < 64 || block_size > 256) return fail(BT_E_ARG, "bad block size");
+    return bt::batched_occupancy((int)integrator, (int)cylindrical, false, block_size);
+}
+
+int bt_batch_trace_device(void* handle, const bt_batch_args* a, void* stream) {
+    const BatchPlan* plan = static_cast<const BatchPlan*>(handle);
+    int rc = check_batch(plan, a);
+    if (rc) return rc;
+    return run_batch(plan, a, static_cast<hipStream_t>(stream));
+}
+
+int bt_batch_graph_create(void* handle, const bt_batch_args* a, void** graph_out) {
+    const BatchPlan* plan = static_cast<const BatchPlan*>(handle);
+    if (graph_out == nullptr) return fail(BT_E_ARG, "graph is NULL");
+    *graph_out = nullptr;
+    int rc = check_batch(plan, a);
+    if (rc) return rc;
+    hipStream_t st = nullptr;
+    hipError_t err = hipStreamCreate(&st);
+    if (err != hipSuccess) return hip_fail(err, "hipStreamCreate");
+    hipGraph_t graph = nullptr;
+    err = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) {
+        (void)hipStreamDestroy(st);
+        return hip_fail(err, "hipStreamBeginCapture");
+    }
+    rc = run_batch(plan, a, st);
+    err = hipStreamEndCapture(st, &graph);
+    (void)hipStreamDestroy(st);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
+    GraphHandle* g = new GraphHandle();
+    err = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) {
+        delete g;
+        return hip_fail(err, "hipGraphInstantiate");
+    }
+    *graph_out = g;
+    return BT_OK;
+}
+
+int bt_graph_launch(void* handle, void* stream) {
+    GraphHandle* g = static_cast<GraphHandle*>(handle);
+    if (g == nullptr || g->exec == nullptr) return fail(BT_E_ARG, "graph is NULL");
+    hipError_t err = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream));
+    if (err != hipSuccess) return hip_fail(err, "hipGraphLaunch");
+    return BT_OK;
+}
+
+int bt_graph_destroy(void* handle) {
+    GraphHandle* g = static_cast<GraphHandle*>(handle);
+    if (g == nullptr) return BT_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    delete g;
     return BT_OK;
 }
 

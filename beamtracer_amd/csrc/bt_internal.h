@@ -9,6 +9,18 @@ namespace bt {
 typedef int64_t i64;
 typedef unsigned long long u64;
 
+// One design of a batched trace: its mesh and the offsets of its arrays in the batch's
+// arenas (same layout as bt_batch_design in include/beamtracer_hip.h).
+struct BatchDesign {
+    i64 nx, ny;
+    double origin_x, origin_y, h, upper_x, upper_y;
+    i64 fx_offset, fy_offset;
+    i64 cells_offset, cells_rows, cells_cols;
+    i64 subcells_offset, n_subcells;
+    i64 node_offset, n_nodes;
+    int32_t deposit_exponent, reserved;
+};
+
 // Device pointers and scalars of one trace launch (see bt_trace2d_args in
 // include/beamtracer_hip.h for the meaning of each).
 struct TraceParams {
@@ -36,6 +48,10 @@ struct TraceParams {
     // deposit accumulator and work queue (workspace)
     u64* acc; int deposit_exponent; u64* queue;   // node words, fixed-point exponent, work queue
     double* charge;
+    // batched form: the design table and every trajectory's design (device), else null;
+    // `design` is the design whose mesh these parameters describe (0 for one mesh)
+    const BatchDesign* designs; const int32_t* design_of; i64 n_designs;
+    i64 design;
 };
 
 typedef void (*TraceKernel)(TraceParams);
@@ -44,6 +60,14 @@ TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general);
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
                  bool has_b, bool general, i64 nodes);
 int trace_occupancy(int integ, int geom, bool has_b, bool general, int block);
+
+// Batched form: one persistent launch over the trajectories of every design (single-patch,
+// uniform, classical meshes); `reset` first restores x, v from x0, v0 and clears steps,
+// status, electrode, counts and charge.
+TraceKernel select_batched_kernel(int integ, int geom, bool has_b);
+int launch_batched(const TraceParams& p, const double* x0, const double* v0, bool reset, int block, int grid,
+                   hipStream_t stream, int integ, int geom, bool has_b, i64 total_nodes, i64 max_design_nodes);
+int batched_occupancy(int integ, int geom, bool has_b, int block);
 
 int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
                   i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,

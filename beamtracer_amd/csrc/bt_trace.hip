@@ -184,18 +184,21 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, double q0, doub
 // ---- the deposit window ---------------------------------------------------------------
 // The workgroup's deposits go into a window of node sums in LDS that follows its beam: a
 // ring of kWinCols column slots (column i in slot i mod kWinCols), each holding kWinRows
-// rows from a base row.  Lanes add with LDS integer atomics; a deposit outside the window
-// goes to global memory directly.  Between steps the workgroup flushes the slots of
-// columns it has left (one global atomic per word of each node) and claims the columns
-// around its current cells.  All waves of the workgroup step in lockstep: deposits and
-// flushes are separated by barriers, and the slot tables are double-buffered (deposits of
-// step k read table k mod 2 while the maintenance after step k writes table k + 1 mod 2),
-// so no deposit can land in a slot that is being flushed.
+// rows from a base row.  A slot's tag is the accumulator index of its column's row 0
+// (the patch's node offset + i ny), which also names the design in a batched launch, so
+// lanes of other meshes never match it.  Lanes add with LDS integer atomics; a sum outside
+// the window goes to global memory directly.  Between steps the workgroup flushes the
+// slots of columns it has left (one global atomic per word of each node) and claims the
+// columns around the current cells of its lowest design.  All waves of the workgroup step
+// in lockstep: deposits and flushes are separated by barriers, and the slot tables are
+// double-buffered (deposits of step k read table k mod 2 while the maintenance after step
+// k writes table k + 1 mod 2), so no deposit can land in a slot that is being flushed.
 
 constexpr int kWinCols = 16;
 constexpr int kWinRows = 64;
 constexpr int kRoundSteps = 1;        // steps between window moves
 constexpr int kFlushRounds = 32;      // every slot is emptied at least this often
+constexpr int kColumnBits = 20;       // a placement key is (design << 20) | column
 
 // A slot sums a node's deposits in one signed 64-bit word.  What enters the window are the
 // lanes' cached sums (each below 2^46 units in magnitude); a lane adds at most two of them to
@@ -203,9 +206,10 @@ constexpr int kFlushRounds = 32;      // every slot is emptied at least this oft
 // kRoundSteps steps of at most 1024 lanes, and every slot is flushed at least every
 // kFlushRounds rounds: |sum| < 32 * 1 * 1024 * 2 * 2^46 = 2^62.
 struct DepositWindow {
-    i64 tag[2][kWinCols];             // the column held by a slot, -1 for none
+    i64 tag[2][kWinCols];             // accumulator index of the slot's column at row 0, -1: none
     int base[2][kWinCols];            // its first row
-    int bounds[2][4];                 // min/max column and row of the round's deposits
+    long long key[2][2];              // min / max placement key of the round's deposits
+    int rows[2][2];                   // min / max row of the round's deposits
     u64 word[kWinCols][kWinRows];
 };
 
@@ -215,10 +219,10 @@ __device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
 }
 
 __device__ __forceinline__ void window_reset_bounds(DepositWindow& w, int t) {
-    w.bounds[t][0] = 0x7fffffff;
-    w.bounds[t][1] = -1;
-    w.bounds[t][2] = 0x7fffffff;
-    w.bounds[t][3] = -1;
+    w.key[t][0] = 0x7fffffffffffffffll;
+    w.key[t][1] = -1;
+    w.rows[t][0] = 0x7fffffff;
+    w.rows[t][1] = -1;
 }
 
 __device__ __forceinline__ void window_init(DepositWindow& w) {
@@ -238,32 +242,37 @@ __device__ __forceinline__ void window_init(DepositWindow& w) {
 // Records the base-mesh cell (i, j) of a lane's deposit for the window's next placement:
 // the first and last such lane of each wave report (the bounds only place the window; a
 // sum outside it goes to global memory).
-__device__ __forceinline__ void window_note(DepositWindow& win, int t, bool in_base, i64 i, i64 j) {
+__device__ __forceinline__ void window_note(DepositWindow& win, int t, bool in_base, i64 design, i64 i, i64 j) {
     const u64 lanes = __ballot(in_base);
     const int me = (int)__lane_id();
     if (in_base && (me == __ffsll((long long)lanes) - 1 || me == 63 - __clzll((long long)lanes))) {
-        atomicMin(&win.bounds[t][0], (int)i);
-        atomicMax(&win.bounds[t][1], (int)i);
-        atomicMin(&win.bounds[t][2], (int)j);
-        atomicMax(&win.bounds[t][3], (int)j);
+        const long long key = (long long)((design << kColumnBits) | i);
+        atomicMin(&win.key[t][0], key);
+        atomicMax(&win.key[t][1], key);
+        atomicMin(&win.rows[t][0], (int)j);
+        atomicMax(&win.rows[t][1], (int)j);
     }
 }
 
 // Adds q units (|q| < 2^46) to base-mesh node (ii, jj): into the window slot that holds the
-// node, else to the node's words in global memory.
+// node (USE_WIN), else to the node's words in global memory.
+template <bool USE_WIN>
 __device__ __forceinline__ void node_add(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn, i64 ny,
                                          i64 ii, i64 jj, i64 q) {
-    const int slot = (int)(ii & (kWinCols - 1));
-    const i64 row = jj - win.base[t][slot];
-    if (win.tag[t][slot] == ii && row >= 0 && row < kWinRows) {
-        atomicAdd(&win.word[slot][row], (u64)q);  // two's complement
-        return;
+    if (USE_WIN) {
+        const int slot = (int)(ii & (kWinCols - 1));
+        const i64 row = jj - win.base[t][slot];
+        if (win.tag[t][slot] == ofn + ii * ny && row >= 0 && row < kWinRows) {
+            atomicAdd(&win.word[slot][row], (u64)q);  // two's complement
+            return;
+        }
     }
     add_sum(acc + kWords * (ofn + ii * ny + jj), (u64)q);
 }
 
 // Moves the lane's cache to cell (i, j): the sums of nodes that both cells share move to
 // their new places, the others leave the lane.
+template <bool USE_WIN>
 __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
                                            i64 ofn, i64 ny, int i, int j) {
     if (i == s.cci && j == s.ccj) return;
@@ -281,7 +290,8 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int
     for (int q = 0; q < 4; ++q) {
         const int na = (q >> 1) - di, nb = (q & 1) - dj;
         const bool kept = na >= 0 && na <= 1 && nb >= 0 && nb <= 1;
-        if (!kept && s.cq[q] != 0) node_add(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
+        if (!kept && s.cq[q] != 0)
+            node_add<USE_WIN>(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) s.cq[r] = moved[r];
@@ -290,11 +300,12 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int
 }
 
 // Empties the lane's cache (the trajectory ended).
+template <bool USE_WIN>
 __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
                                             i64 ofn, i64 ny) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (s.cq[q] != 0) node_add(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
+        if (s.cq[q] != 0) node_add<USE_WIN>(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
         s.cq[q] = 0;
     }
     s.cci = kNoCell;
@@ -306,11 +317,12 @@ __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, in
 // mesh (in_base) the units gather in the lane's cache, elsewhere (or when a single deposit
 // reaches 2^46 units) they go to the node words in global memory.  The node sums are exact
 // integers, so the grouping does not change them.
+template <bool USE_WIN>
 __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
-                                         i64 ofn, i64 ny, bool in_base, i64 i, i64 j,
+                                         i64 ofn, i64 ny, bool in_base, i64 design, i64 i, i64 j,
                                          const double (&value)[4], int e) {
-    window_note(win, t, in_base, i, j);
-    if (in_base) cache_move(s, win, t, acc, ofn, ny, (int)i, (int)j);
+    if (USE_WIN) window_note(win, t, in_base, design, i, j);
+    if (in_base) cache_move<USE_WIN>(s, win, t, acc, ofn, ny, (int)i, (int)j);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         u64 w0, w1;
@@ -322,7 +334,7 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t
             if (sum > -kCacheLimit && sum < kCacheLimit) {
                 s.cq[q] = sum;
             } else {
-                node_add(win, t, acc, ofn, ny, ii, jj, s.cq[q]);
+                node_add<USE_WIN>(win, t, acc, ofn, ny, ii, jj, s.cq[q]);
                 s.cq[q] = units;
             }
             continue;
@@ -332,14 +344,24 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t
 }
 
 // After the deposits of round k (table t = k mod 2): move the window to the columns around
-// the round's cells (with room for kRoundSteps steps of travel either way), flushing the
-// slots that change, and write table 1 - t.
-__device__ __forceinline__ void window_maintain(DepositWindow& win, int t, u64* __restrict__ acc,
-                                                i64 ofn, i64 nx, i64 ny, bool empty_all) {
+// the round's cells of its lowest design (with room for kRoundSteps steps of travel either
+// way), flushing the slots that change, and write table 1 - t.  `p` describes the mesh of
+// a one-mesh launch; a batched launch reads the chosen design's mesh from its table.
+template <bool BATCHED>
+__device__ __forceinline__ void window_maintain(DepositWindow& win, int t, const TraceParams& p, bool empty_all) {
     const int n = 1 - t;
-    const int cmin = win.bounds[t][0], cmax = win.bounds[t][1];
-    const int rmin = win.bounds[t][2];
-    const bool moved = cmin <= cmax;
+    const i64 kmin = win.key[t][0], kmax = win.key[t][1];
+    const bool moved = kmin <= kmax;
+    const i64 design = moved ? kmin >> kColumnBits : 0;
+    i64 ofn = p.bofn, nx = p.nx, ny = p.ny;
+    if (BATCHED && moved) {
+        const BatchDesign& b = p.designs[design];
+        ofn = b.node_offset; nx = b.nx; ny = b.ny;
+    }
+    const int cmin = (int)(kmin & ((1 << kColumnBits) - 1));
+    // the highest column of the same design (another design's lanes leave the span open)
+    const int cmax = (kmax >> kColumnBits) == design ? (int)(kmax & ((1 << kColumnBits) - 1)) : (int)nx - 1;
+    const int rmin = win.rows[t][0];
     int lo = cmin - 1 - kRoundSteps, hi = cmax + 2 + kRoundSteps;
     if (lo < 0) lo = 0;
     if (hi > (int)nx - 1) hi = (int)nx - 1;
@@ -353,11 +375,11 @@ __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, u64* 
         i64 want = old;
         if (moved) {
             const int col = lo + ((slot - lo) & (kWinCols - 1));
-            if (col <= hi) want = col;
+            if (col <= hi) want = ofn + (i64)col * ny;
         }
         if (want != old || empty_all) {
             const u64 q = win.word[slot][row];
-            if (old >= 0 && q != 0) add_sum(acc + kWords * (ofn + old * ny + win.base[t][slot] + row), q);
+            if (old >= 0 && q != 0) add_sum(p.acc + kWords * (old + win.base[t][slot] + row), q);
             win.word[slot][row] = 0ull;
         }
         if (row == 0) {
@@ -369,17 +391,17 @@ __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, u64* 
 }
 
 // Every node sum still in the window (table t) to global memory.
-__device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn, i64 ny) {
+__device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __restrict__ acc) {
     for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
         const int slot = k / kWinRows, row = k % kWinRows;
         const i64 old = win.tag[t][slot];
         const u64 q = win.word[slot][row];
-        if (old >= 0 && q != 0) add_sum(acc + kWords * (ofn + old * ny + win.base[t][slot] + row), q);
+        if (old >= 0 && q != 0) add_sum(acc + kWords * (old + win.base[t][slot] + row), q);
     }
 }
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN>
 __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositWindow& win, int t) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
@@ -553,8 +575,8 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
         const double deposit = s.dqk * dt;
         const double value[4] = {(1.0 - u) * (1.0 - w) * deposit, (1.0 - u) * w * deposit,
                                  u * (1.0 - w) * deposit, u * w * deposit};
-        deposit4(s, win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, i, j, value,
-                 p.deposit_exponent);
+        deposit4<USE_WIN>(s, win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, value,
+                          p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
     s.ns = ns;
@@ -648,12 +670,37 @@ __device__ __forceinline__ void store_end(const TraceParams& p, const LaneState&
     p.status[k] = st;
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
-__global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
+// ---- batched form -------------------------------------------------------------------
+// The lane's mesh: design d's table entry over the batch's shared fields (the arenas, the
+// accumulator and the trajectory arrays stay those of the launch; the design's nodes start
+// at its node offset in the accumulator).
+__device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& p, int d) {
+    const BatchDesign& b = p.designs[d];
+    lp.nx = b.nx; lp.ny = b.ny;
+    lp.origin_x = b.origin_x; lp.origin_y = b.origin_y; lp.h = b.h;
+    lp.upper_x = b.upper_x; lp.upper_y = b.upper_y;
+    lp.box = b.origin_x; lp.boy = b.origin_y; lp.bh = b.h;
+    lp.bofx = 0; lp.bofy = 0; lp.bofn = b.node_offset;
+    lp.fxs = p.fxs + b.fx_offset; lp.fys = p.fys + b.fy_offset;
+    lp.cells = p.cells + b.cells_offset; lp.cells_rows = b.cells_rows; lp.cells_cols = b.cells_cols;
+    lp.subcells = p.subcells + b.subcells_offset; lp.n_subcells = b.n_subcells;
+    lp.planes = p.planes + (i64)d * p.n_planes;
+    lp.deposit_exponent = b.deposit_exponent;
+    lp.design = d;
+}
+
+// The persistent trace loop.  One mesh (BATCHED false): every lane uses the launch's
+// parameters.  BATCHED: every design's trajectories share the queue, and a lane takes the
+// mesh of each new trajectory's design; the deposit window follows the lowest design of
+// the workgroup's lanes, and lanes of other designs add to global memory.
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED>
+__device__ __forceinline__ void trace_loop(const TraceParams& p) {
     __shared__ DepositWindow win;
     window_init(win);
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    TraceParams lp = p;
+    const TraceParams& q = BATCHED ? lp : p;  // the lane's mesh
     LaneState s;
     s.k = -1;
     s.cci = s.ccj = kNoCell;
@@ -675,14 +722,21 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
                 if (m < (u64)p.n_active) {
                     const i64 k = p.active[m];
                     const i64 N = p.n;
-                    // an index that is not a trajectory is skipped (the lane asks again)
-                    if (k >= 0 && k < N) {
+                    bool valid = k >= 0 && k < N;
+                    if (BATCHED && valid) {
+                        const int d = p.design_of[k];
+                        valid = d >= 0 && d < p.n_designs;
+                        if (valid) load_design(lp, p, d);
+                    }
+                    // an index that is not a trajectory (or of no design) is skipped: the
+                    // lane asks again
+                    if (valid) {
                     s.k = k;
                     s.x0 = p.x[k]; s.x1 = p.x[N + k]; s.x2 = p.x[2 * N + k];
                     s.v0 = p.v[k]; s.v1 = p.v[N + k]; s.v2 = p.v[2 * N + k];
                     s.qmk = p.qm[k];
                     s.dqk = p.dq[k];
-                    s.slot = p.path_slot[k];
+                    s.slot = (!BATCHED && p.path_slot != nullptr) ? p.path_slot[k] : -1;
                     if (s.slot >= p.n_paths) s.slot = -1;
                     s.ns = p.steps[k];
                     s.have = 0;
@@ -704,19 +758,62 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
             if (s.k >= 0) {
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL>(p, s, win, t);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, s, win, t);
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush(s, win, t, p.acc, p.bofn, p.ny);
+                    cache_flush<true>(s, win, t, p.acc, q.bofn, q.ny);
                 }
             }
         }
         __syncthreads();
-        window_maintain(win, t, p.acc, p.bofn, p.nx, p.ny, ++rounds % kFlushRounds == 0);
+        window_maintain<BATCHED>(win, t, p, ++rounds % kFlushRounds == 0);
         t = 1 - t;
     }
-    window_flush(win, t, p.acc, p.bofn, p.ny);
+    window_flush(win, t, p.acc);
+}
+
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+__global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
+    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false>(p);
+}
+
+// Every design's trajectories in one persistent launch.
+template <int INTEG, int GEOM, bool HAS_B>
+__global__ __launch_bounds__(256) void batched_kernel(TraceParams p) {
+    trace_loop<INTEG, GEOM, HAS_B, false, true>(p);
+}
+
+// The start of a batched trace from the emitter: x, v from x0, v0; steps 0; status and
+// electrode -1; plane counts 0; the node charges 0 (a kernel, so that a captured graph
+// holds no memset node).
+__global__ void batched_reset_kernel(TraceParams p, const double* __restrict__ x0, const double* __restrict__ v0,
+                                     i64 total_nodes) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    const i64 first = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    for (i64 t = first; t < 3 * p.n; t += stride) {
+        p.x[t] = x0[t];
+        p.v[t] = v0[t];
+    }
+    for (i64 t = first; t < p.n; t += stride) {
+        p.steps[t] = 0;
+        p.status[t] = -1;
+        p.electrode[t] = -1;
+    }
+    for (i64 t = first; t < p.n * p.n_planes; t += stride) p.counts[t] = 0;
+    for (i64 t = first; t < total_nodes; t += stride) p.charge[t] = 0.0;
+}
+
+// charge[design d's node c] += its fixed-point total, with d's exponent (blockIdx.y = d)
+__global__ void batched_finalize_kernel(const u64* __restrict__ acc, const BatchDesign* __restrict__ designs,
+                                        double* __restrict__ charge) {
+    const BatchDesign& b = designs[blockIdx.y];
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < b.n_nodes; c += stride) {
+        const i64 node = b.node_offset + c;
+        const u64 w0 = acc[kWords * node], w1 = acc[kWords * node + 1];
+        if (w0 != 0 || w1 != 0) charge[node] += words_value(w0, w1, b.deposit_exponent);
+    }
 }
 
 // zero the fixed-point accumulator and the queue head
@@ -769,6 +866,52 @@ int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, 
     hipLaunchKernelGGL(finalize_kernel, dim3(fgrid), dim3(256), 0, stream, p.acc, nodes,
                        p.deposit_exponent, p.charge);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+TraceKernel select_batched_kernel(int integ, int geom, bool has_b) {
+#define BT_SELB(I, G, B) \
+    if (integ == I && geom == G && has_b == B) return batched_kernel<I, G, B>;
+    BT_SELB(0, 0, false) BT_SELB(0, 1, false) BT_SELB(0, 2, false)
+    BT_SELB(1, 0, false) BT_SELB(1, 1, false) BT_SELB(1, 2, false)
+#undef BT_SELB
+    return nullptr;
+}
+
+int launch_batched(const TraceParams& p, const double* x0, const double* v0, bool reset, int block, int grid,
+                   hipStream_t stream, int integ, int geom, bool has_b, i64 total_nodes, i64 max_design_nodes) {
+    TraceKernel kern = select_batched_kernel(integ, geom, has_b);
+    if (kern == nullptr) return -1;
+    if (reset) {
+        i64 work = 3 * p.n > total_nodes ? 3 * p.n : total_nodes;
+        int rgrid = (int)((work + 255) / 256);
+        if (rgrid > 8192) rgrid = 8192;
+        if (rgrid < 1) rgrid = 1;
+        hipLaunchKernelGGL(batched_reset_kernel, dim3(rgrid), dim3(256), 0, stream, p, x0, v0, total_nodes);
+    }
+    const i64 words = kWords * total_nodes;
+    int zgrid = (int)((words + 255) / 256);
+    if (zgrid > 8192) zgrid = 8192;
+    if (zgrid < 1) zgrid = 1;
+    hipLaunchKernelGGL(prepare_kernel, dim3(zgrid), dim3(256), 0, stream, p.acc, words, p.queue);
+    if (p.n_active > 0 && grid > 0)
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, p);
+    int fgrid = (int)((max_design_nodes + 255) / 256);
+    if (fgrid > 1024) fgrid = 1024;
+    if (fgrid < 1) fgrid = 1;
+    if (p.n_designs > 0)
+        hipLaunchKernelGGL(batched_finalize_kernel, dim3(fgrid, (unsigned)p.n_designs), dim3(256), 0, stream,
+                           p.acc, p.designs, p.charge);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int batched_occupancy(int integ, int geom, bool has_b, int block) {
+    TraceKernel kern = select_batched_kernel(integ, geom, has_b);
+    if (kern == nullptr) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),
+                                                     block, 0) != hipSuccess)
+        return 0;
+    return blocks;
 }
 
 int trace_occupancy(int integ, int geom, bool has_b, bool general, int block) {

@@ -46,6 +46,8 @@ class Design:
     #: node magnetic field ``(B_x, B_r, B_theta)`` on the mesh nodes, or None
     magnetic: tuple | None = None
     plasma: PlasmaModel | None = None
+    #: the keyword arguments that rebuild this design (``einzel_lens(**spec)``), if any
+    spec: dict | None = None
 
 
 def einzel_lens(
@@ -150,6 +152,44 @@ def extraction(
                          rho_e=constants.e * electron_density)  # fmt: skip
     return Design(mesh, beam, jv, (plane,), integrator="boris", magnetic=node_magnetic_field(mesh, lens),
                   plasma=plasma)  # fmt: skip
+
+
+def einzel_batch(
+    count: int = 256,
+    seed: int = 0,
+    nx_range: tuple[int, int] = (201, 1025),
+    ny_range: tuple[int, int] = (51, 257),
+    particles_range: tuple[float, float] = (1e4, 2e5),
+    voltage_range: tuple[float, float] = (2e3, 10e3),
+    gap_range: tuple[float, float] = (2e-3, 8e-3),
+    energy_range: tuple[float, float] = (5e3, 20e3),
+    length_x: float = 80e-3,
+) -> list[Design]:
+    """``count`` einzel-lens candidates drawn from one seed, per design in the order nx, ny,
+    particles (log-uniform), middle-tube voltage, gap, beam energy.
+
+    Every mesh spans ``length_x`` with ``h = length_x / (nx - 1)`` and the radius
+    ``(ny - 1) h``; the bore is 10 mm or half the radius if that is smaller, the beam radius
+    3 mm or 0.3 of the bore if that is smaller, and the tubes 24 mm long.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        n = int(round(np.exp(rng.uniform(np.log(particles_range[0]), np.log(particles_range[1])))))
+        voltage = float(rng.uniform(*voltage_range))
+        gap = float(rng.uniform(*gap_range))
+        energy = float(rng.uniform(*energy_range))
+        h = length_x / (nx - 1)
+        radius = (ny - 1) * h
+        bore = min(10e-3, 0.5 * radius)
+        spec = dict(nx=nx, ny=ny, length_x=length_x, bore=bore, gap=gap, voltage=voltage, energy=energy,
+                    particles=n, beam_radius=min(3e-3, 0.3 * bore), plane=min(75e-3, length_x - h))
+        design = einzel_lens(**spec)
+        design.spec = spec
+        out.append(design)
+    return out
 
 
 def sample_particles(particles: BeamParticles, count: int) -> BeamParticles:

@@ -85,6 +85,19 @@ class StaggeredElectricField:
             edge_field(phi.T, solid.T, near.T, mesh.distance[2].T, mesh.distance[3].T, mesh.h, self.extrapolation[2:4]).T
         )
 
+    @classmethod
+    def from_edges(cls, mesh: IonOpticsMesh, fx: np.ndarray, fy: np.ndarray) -> StaggeredElectricField:
+        """The field of edge values computed elsewhere (``fx`` (nx + 1, ny), ``fy`` (nx, ny + 1))."""
+        nx, ny = mesh.shape
+        if fx.shape != (nx + 1, ny) or fy.shape != (nx, ny + 1):
+            raise ValueError("fx and fy must be (nx + 1, ny) and (nx, ny + 1)")
+        field = cls.__new__(cls)
+        field.mesh = mesh
+        field.extrapolation = ()
+        field.fx = np.ascontiguousarray(fx, dtype=float)
+        field.fy = np.ascontiguousarray(fy, dtype=float)
+        return field
+
     def __call__(self, x: np.ndarray, y: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         """``(E_x, E_y)`` [V/m] at the points ``(x, y)``, gathered on the GPU."""
         from beamtracer_amd.kernels import staggered_field_2d

@@ -73,6 +73,27 @@ class TraceResult:
     particle_steps: int = 0
 
 
+def collect_plane_hits(planes, hits, counts, current, species) -> list[PlaneHits]:
+    """The recorded crossings of every plane, in the order of the steps, then of the
+    trajectories (``tracer.py:1278-1293``); ``hits`` ``(n, n_planes, capacity, 8)``."""
+    capacity = hits.shape[2]
+    slots = np.arange(capacity)
+    out = []
+    for p, position in enumerate(planes):
+        k, c = np.nonzero(slots[None, :] < np.minimum(counts[:, p], capacity)[:, None])
+        record = hits[k, p, c]
+        order = np.lexsort((k, record[:, 7]))
+        k, record = k[order], record[order]
+        plane = PlaneHits(position)
+        plane.point = record[:, 0:3].T.copy()
+        plane.velocity = record[:, 3:6].T.copy()
+        plane.current = current[k]
+        plane.species = species[k].astype(np.int32)
+        plane.direction = record[:, 6].astype(np.int8)
+        out.append(plane)
+    return out
+
+
 class HipBeamTracer:
     """Trace particles through a static staggered E field (and a node B) on the GPU."""
 
@@ -290,20 +311,7 @@ class HipBeamTracer:
             x, v, status, electrode = (a.copy() for a in initial)
             steps[:] = 0
         charge = charge.sum(axis=0).reshape(mesh.shape)
-        hit_planes = []
-        slots = np.arange(capacity)
-        for p, position in enumerate(self.planes):
-            k, c = np.nonzero(slots[None, :] < counts[:, p, None])
-            record = hits[k, p, c]
-            order = np.lexsort((k, record[:, 7]))
-            k, record = k[order], record[order]
-            plane = PlaneHits(position)
-            plane.point = record[:, 0:3].T.copy()
-            plane.velocity = record[:, 3:6].T.copy()
-            plane.current = current[k]
-            plane.species = species[k].astype(np.int32)
-            plane.direction = record[:, 6].astype(np.int8)
-            hit_planes.append(plane)
+        hit_planes = collect_plane_hits(self.planes, hits, counts, current, species)
         self.last_state = (x, v, steps)
         return TraceResult(
             charge_density=charge,

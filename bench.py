@@ -21,7 +21,10 @@ Beside the headline the line carries, under ``extra``:
   0.2 T current loop (node B), on the field of the initial plasma: particle-steps/s and
   FP64 FLOP/s at 257 FLOP per Boris+B step;
 * ``vlasov`` -- Vlasov iterations/s of the einzel lens (host Poisson solve, edge fields,
-  host-form trace with its copies, relaxation 0.5), with the time split.
+  host-form trace with its copies, relaxation 0.5), with the time split;
+* ``batched`` -- 256 einzel candidates (mesh, voltage, gap, energy and beam size drawn from
+  one seed) traced in one batched launch, against one plain launch per design: particle-
+  steps/s of each and their ratio, and the rate of the captured HIP graph.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--no-cpu]
                        [--no-extra]
@@ -239,6 +242,83 @@ def vlasov_leg(design, iterations: int) -> dict:
     }
 
 
+def batched_leg(count: int, seed: int, processes: int, repeats: int) -> dict:
+    """256 einzel candidates: one batched launch against one plain launch per design."""
+    import ctypes
+
+    import torch
+
+    from beamtracer_amd import _lib, batch, designs, kernels
+
+    t0 = time.perf_counter()
+    ds = designs.einzel_batch(count, seed=seed)
+    tracers = batch.vacuum_tracers(ds, processes=processes)
+    setup = time.perf_counter() - t0
+    beams = [d.particles for d in ds]
+    bt = batch.BatchedTracer(tracers)
+    tune = batch.autotune(bt, beams, blocks=(128, 256), fills=(1.0,), repeats=1)
+    log(f"batched autotune: {tune['choice']} ({tune['best_ms']:.1f} ms)")
+    host = bt._prepare(beams)
+    dev = bt._upload(host, bt.capacity)
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(repeats):
+        bt._restart(dev)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        bt.launch(dev, dev["queue"])
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) / 1e3)
+    batched_steps = int(dev["steps"].sum().item())
+    batched_s = float(np.median(times))
+    # the same traces as one plain launch per design (inputs resident on the GPU)
+    plain_s = 0.0
+    plain_steps = 0
+    for d, t in zip(ds, tracers):
+        args = kernel_arguments(d, t)
+        dv, _ = device_arguments(args)
+        e = kernels.trace_exponent(args[0], args[2], args[4], args[3], args[8], args[9], args[11], args[19])
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        kernels.trace_trajectories_2d(*dv, exponent=e, stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        plain_s += a.elapsed_time(b) / 1e3
+        plain_steps += int(dv[5].sum().item())
+    # the captured graph of the whole batched trace (reset, queue, trace, finalize)
+    bt.capture(beams)
+    gtimes = []
+    for _ in range(repeats):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _lib.check(_lib.lib().bt_graph_launch(bt._graph, ctypes.c_void_p(stream.cuda_stream)), "graph")
+        b.record(stream)
+        torch.cuda.synchronize()
+        gtimes.append(a.elapsed_time(b) / 1e3)
+    graph_steps = int(bt._graph_state[1]["steps"].sum().item())
+    bt.close()
+    return {
+        "workload": f"{count} einzel candidates from seed {seed}: nx in [201,1025], ny in [51,257], "
+        "N log-uniform in [1e4,2e5], V in [2,10] kV, gap in [2,8] mm, E in [5,20] keV; RK4, vacuum fields",
+        "trajectories": int(host["n"]),
+        "particle_steps": batched_steps,
+        "batched_value": batched_steps / batched_s,
+        "batched_ms": batched_s * 1e3,
+        "graph_value": graph_steps / float(np.median(gtimes)),
+        "plain_value": plain_steps / plain_s,
+        "plain_ms_sum": plain_s * 1e3,
+        "plain_particle_steps": plain_steps,
+        "speedup_vs_plain": (batched_steps / batched_s) / (plain_steps / plain_s),
+        "unit": "particle-steps/s",
+        "launch_shape": tune["choice"],
+        "autotune_ms": tune["timings_ms"],
+        "setup_s": setup,
+        "note": "one launch (queue over every design) vs one launch per design, both with inputs resident in "
+        "HBM, HIP events; per-design results are bit-identical (tests/test_gpu_batch.py)",
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -250,6 +330,7 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--no-extra", action="store_true", help="only the einzel-lens headline")
     parser.add_argument("--extraction-particles", type=int, default=1_000_000)
+    parser.add_argument("--batch-designs", type=int, default=256)
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="only copy 1 GiB device to device (calibrates the HBM byte counters)")
     opts = parser.parse_args()
@@ -324,6 +405,9 @@ def main() -> int:
         extra = {"extraction": extraction_leg(opts.extraction_particles, max(2, opts.steps // 2), opts.block_size)}
         log("vlasov: einzel-lens iterations")
         extra["vlasov"] = vlasov_leg(design, 3)
+        log(f"batched: {opts.batch_designs} einzel candidates")
+        processes = min(16, len(os.sched_getaffinity(0)))
+        extra["batched"] = batched_leg(opts.batch_designs, 0, processes, 3)
         line["extra"] = extra
     if not opts.no_cpu:
         log("cpu baseline: oracle tracer on the host cores")

@@ -126,6 +126,67 @@ int bt_deposit_trajectories_2d_device(const double* px, const double* py, const 
                                       void* workspace, int64_t workspace_bytes,
                                       void* stream);
 
+/* ---- batched form ---------------------------------------------------------------------
+ * Many designs (meshes, fields, beams) traced in one persistent launch: the reference
+ * evaluates each candidate of beamtracer.optimization in a separate process
+ * (optimization/search.py:200 optimize, through the ProcessPoolExecutor of
+ * optimization/pool.py:110-160); here every design's trajectories share one work queue.
+ * Each design is one uniform patch (no refinement, no graded axis), classical
+ * (inv_c2 = 0) and electrostatic; its arrays sit at offsets in shared arenas.  Per design,
+ * the results equal those of bt_trace_trajectories_2d on the same inputs (same exponent),
+ * bit for bit. */
+typedef struct bt_batch_design {
+    int64_t nx, ny;                       /* nodes of the design's mesh                    */
+    double origin_x, origin_y, h, upper_x, upper_y;
+    int64_t fx_offset, fy_offset;         /* its fx (nx+1, ny) in fxs, fy (nx, ny+1) in fys */
+    int64_t cells_offset, cells_rows, cells_cols;   /* its cells (>= nx-1, >= ny-1)        */
+    int64_t subcells_offset, n_subcells;  /* its sub-cell electrode table                  */
+    int64_t node_offset, n_nodes;         /* its nx * ny node charges in charge            */
+    int32_t deposit_exponent;             /* its fixed-point exponent                      */
+    int32_t reserved;
+} bt_batch_design;
+
+/* The arrays of a batched trace (device memory), as bt_trace2d_args for one design, over
+ * the concatenated trajectories of every design; design_of[k] is trajectory k's design. */
+typedef struct bt_batch_args {
+    const int64_t* active; int64_t n_active;
+    double* x; double* v; int64_t n;              /* (3, n)                                 */
+    const double* x0; const double* v0;           /* non-NULL: restore x, v from them and clear
+                                                     steps, status, electrode, counts and
+                                                     charge before the trace                */
+    const double* qm; const double* dq;
+    int64_t* steps; int64_t* status; int64_t* electrode;
+    const int32_t* design_of;                     /* (n,)                                   */
+    const double* fxs; const double* fys; const int64_t* cells; const int64_t* subcells;
+    const double* planes; int64_t n_planes;       /* (n_designs, n_planes)                  */
+    double* hits; int64_t capacity; int64_t* counts;   /* (n, n_planes, capacity, 8), (n, n_planes) */
+    double* charge;                               /* (bt_batch_plan_nodes(),) += deposits   */
+    int64_t cylindrical, integrator;
+    double step_fraction; int64_t max_steps; int64_t refine;
+    int32_t block_size, max_blocks;               /* 0: 256 threads, all resident blocks    */
+} bt_batch_args;
+
+int64_t bt_batch_args_size(void);
+int64_t bt_batch_design_size(void);
+/* Validates the design table (host memory) against the arena sizes, copies it to the
+ * device and allocates the fixed-point workspace of every node. */
+int bt_batch_plan_create(const bt_batch_design* designs, int64_t n_designs, int64_t n_fxs,
+                         int64_t n_fys, int64_t n_cells, int64_t n_subcells, void** plan);
+int bt_batch_plan_destroy(void* plan);
+/* Replaces the plan's design table (same count, arenas and node total; e.g. new deposit
+ * exponents after a field update); waits for the device first. */
+int bt_batch_plan_update(void* plan, const bt_batch_design* designs, int64_t n_designs);
+/* Total nodes of the plan's designs (the length of `charge`). */
+int64_t bt_batch_plan_nodes(void* plan);
+/* Resident blocks per CU of the batched kernel (for launch-shape tuning). */
+int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size);
+/* One batched trace, asynchronous on `stream`; allocates nothing. */
+int bt_batch_trace_device(void* plan, const bt_batch_args* args, void* stream);
+/* Captures bt_batch_trace_device(plan, args) as a HIP graph (kernel nodes only). */
+int bt_batch_graph_create(void* plan, const bt_batch_args* args, void** graph);
+int bt_graph_launch(void* graph, void* stream);
+int bt_graph_destroy(void* graph);
+
 #ifdef __cplusplus
 }
 #endif

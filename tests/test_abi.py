@@ -35,6 +35,23 @@ def test_struct_layout_and_pure_helpers():
     assert L.bt_deposit_exponent(1.0) == -40
     assert L.bt_deposit_exponent(0.75) == -41
     assert L.bt_trace_workspace_bytes(161001) >= 16 * 161001
+    assert L.bt_batch_args_size() == ctypes.sizeof(_lib.BatchArgs)
+    assert L.bt_batch_design_size() == ctypes.sizeof(_lib.BatchDesign)
+
+
+def test_batch_plan_rejects_inconsistent_designs_before_any_device_call():
+    L = _lib.lib()
+    table = (_lib.BatchDesign * 2)()
+    table[0] = _lib.BatchDesign(11, 5, 0.0, 0.0, 1e-3, 10e-3, 4e-3, 0, 0, 0, 10, 4, 0, 1, 0, 55, -40, 0)
+    table[1] = _lib.BatchDesign(11, 5, 0.0, 0.0, 1e-3, 10e-3, 4e-3, 60, 66, 40, 10, 4, 1, 1, 55, 55, -40, 0)
+    plan = ctypes.c_void_p()
+    # the second design's fy runs past the fy arena (66 + 11 * 6 > 120)
+    assert L.bt_batch_plan_create(table, 2, 120, 120, 80, 2, ctypes.byref(plan)) == _lib.BT_E_ARG
+    assert b"design 1: fy beyond fys" in L.bt_last_error()
+    table[1].n_nodes = 54
+    assert L.bt_batch_plan_create(table, 2, 200, 200, 80, 2, ctypes.byref(plan)) == _lib.BT_E_ARG
+    assert b"n_nodes must be nx * ny" in L.bt_last_error()
+    assert not plan.value
 
 
 def test_argument_errors_are_raised_before_any_device_call():
