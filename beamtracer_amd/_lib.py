@@ -14,7 +14,8 @@ from pathlib import Path
 PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIB_PATH = PACKAGE / "libbeamtracer_hip.so"
-SOURCES = [PACKAGE / "csrc" / name for name in ("bt_api.hip", "bt_trace.hip", "bt_gather_deposit.hip")]
+SOURCES = [PACKAGE / "csrc" / name for name in ("bt_api.hip", "bt_trace.hip", "bt_gather_deposit.hip",
+                                                "bt_adjoint.hip")]
 HEADERS = [ROOT / "include" / "beamtracer_hip.h"] + sorted((PACKAGE / "csrc").glob("*.h"))
 
 #: hipcc flags of the library: FP64 results in the reference's operation order (no fused
@@ -112,6 +113,28 @@ class BatchArgs(ctypes.Structure):
     ]  # fmt: skip
 
 
+class AdjointArgs(ctypes.Structure):
+    """ctypes mirror of ``bt_adjoint_args``."""
+
+    _fields_ = [
+        ("active", _ptr), ("n_active", _i64),
+        ("x", _ptr), ("v", _ptr), ("n", _i64),
+        ("qm", _ptr),
+        ("fx", _ptr), ("fy", _ptr),
+        ("nx", _i64), ("ny", _i64),
+        ("origin_x", _dbl), ("origin_y", _dbl), ("h", _dbl),
+        ("cylindrical", _i64), ("integrator", _i64),
+        ("step_fraction", _dbl),
+        ("planes", _ptr), ("n_planes", _i64),
+        ("hits", _ptr), ("capacity", _i64), ("counts", _ptr),
+        ("seeds", _ptr),
+        ("max_step", _i64),
+        ("interval", _i32), ("block_size", _i32),
+        ("dfx", _ptr), ("dfy", _ptr),
+        ("dx", _ptr), ("dv", _ptr),
+    ]  # fmt: skip
+
+
 #: Every exported symbol of the C ABI, with (restype, argtypes).
 SIGNATURES = {
     "bt_version": (ctypes.c_char_p, []),
@@ -137,6 +160,8 @@ SIGNATURES = {
         ctypes.c_int,
         [_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _i64, _ptr, _i64, _i64, _i32, _ptr, _i64, _ptr],
     ),
+    "bt_adjoint_args_size": (_i64, []),
+    "bt_trace_adjoint_device": (ctypes.c_int, [ctypes.POINTER(AdjointArgs), _ptr]),
     "bt_batch_args_size": (_i64, []),
     "bt_batch_design_size": (_i64, []),
     "bt_batch_plan_create": (

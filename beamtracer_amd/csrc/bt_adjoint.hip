@@ -1,0 +1,435 @@
+// The discrete adjoint of the trace: dJ/dfx, dJ/dfy for an objective J of the plane
+// crossings, for the electrostatic, classical trace on one uniform mesh (RK4 or Boris).
+//
+// The forward map is the trace kernel's step (kernels/trace_trajectories_2d/
+// trace_trajectories_2d_pyccel.py:178-370 with gamma = 1): stage points, the field gather at
+// each (pyccel.py:196-269), the step-size rule dt = f min(h / |v|, sqrt(2 h / |a|))
+// (pyccel.py:282-295) and the crossing record (x_p, y, z, v) interpolated at
+// s = before / (before - after) (pyccel.py:426-442).  The adjoint differentiates exactly
+// that arithmetic: the chosen branch of the step rule, the gather's cell (the derivative of
+// the bilinear form inside it, including the clamped cells' extrapolation), the
+// interpolation fraction.  Given dJ/d(record) for every crossing (the seeds), one lane
+// per trajectory
+//   1. steps forward from the initial state to its last seeded step, storing the state
+//      every `interval` steps (checkpoints);
+//   2. for each segment between checkpoints, last first: steps forward again from the
+//      checkpoint storing every step's start state, then walks the segment backward,
+//      recomputing each step's stages from its start state and applying the transposed
+//      step to the adjoint state (lambda_x, lambda_v); the transposed gather scatters
+//      lambda_E times the bilinear weights into dJ/dfx and dJ/dfy.
+// The forward steps here repeat the trace kernel's operations in the same order, so they
+// reproduce its states bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "bt_device.h"
+#include "bt_internal.h"
+
+namespace bt {
+
+struct Vec3 {
+    double a, b, c;
+};
+
+__device__ __forceinline__ Vec3 vadd(Vec3 x, Vec3 y) { return {x.a + y.a, x.b + y.b, x.c + y.c}; }
+__device__ __forceinline__ Vec3 vaxpy(Vec3 x, double s, Vec3 y) { return {x.a + s * y.a, x.b + s * y.b, x.c + s * y.c}; }
+__device__ __forceinline__ Vec3 vscale(double s, Vec3 y) { return {s * y.a, s * y.b, s * y.c}; }
+__device__ __forceinline__ double vdot(Vec3 x, Vec3 y) { return x.a * y.a + x.b * y.b + x.c * y.c; }
+
+// q/m E at point q: the trace kernel's edge_field for one patch whose origin and spacing are
+// the mesh's, then the radial projection (bt_trace.hip, same operations).
+template <int GEOM>
+__device__ __forceinline__ Vec3 accel(const AdjointParams& p, double qm, Vec3 q) {
+    double cy = 0.0, cz = 0.0, qy = q.b;
+    if (GEOM == 1) {
+        const double r = hypot(q.b, q.c);
+        if (r > 0.0) { cy = q.b / r; cz = q.c / r; }
+        qy = r;
+    } else if (GEOM == 2) {
+        cy = 1.0;
+        if (q.b < p.origin_y) cy = -1.0;
+        qy = p.origin_y + fabs(q.b - p.origin_y);
+    }
+    const double sx = (q.a - p.origin_x) / p.h;
+    const double sy = (qy - p.origin_y) / p.h;
+    const double ex = clamped_bilinear(p.fxs, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const double ey = clamped_bilinear(p.fys, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    if (GEOM != 0) return {qm * ex, qm * (ey * cy), qm * (ey * cz)};
+    return {qm * ex, qm * ey, qm * 0.0};
+}
+
+// The bilinear gather of the staggered grid f (rows of `cols`) at (s, t) with its partial
+// derivatives in s and t and its cell and weights (the same clamped cell as the forward).
+struct Gathered {
+    double value, ds, dt;
+    unsigned c;
+    double u, w;
+};
+
+__device__ __forceinline__ Gathered gather_d(const double* __restrict__ f, i64 cols, double s, double t,
+                                             int imax, int jmax) {
+    const int i = clamped_floor(s, 0, imax);
+    const int j = clamped_floor(t, 0, jmax);
+    Gathered g;
+    g.u = s - (double)i;
+    g.w = t - (double)j;
+    g.c = (unsigned)i * (unsigned)cols + (unsigned)j;
+    const unsigned d = g.c + (unsigned)cols;
+    const double f00 = f[g.c], f01 = f[g.c + 1], f10 = f[d], f11 = f[d + 1];
+    const double low = f00 + g.w * (f01 - f00);
+    const double high = f10 + g.w * (f11 - f10);
+    g.value = low + g.u * (high - low);
+    g.ds = high - low;
+    g.dt = (1.0 - g.u) * (f01 - f00) + g.u * (f11 - f10);
+    return g;
+}
+
+// lambda_E times the bilinear weights of cell c into the gradient grid (rows of `cols`).
+__device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, const Gathered& g, double lam) {
+    if (lam == 0.0) return;
+    const unsigned d = g.c + (unsigned)cols;
+    unsafeAtomicAdd(grad + g.c, (1.0 - g.u) * (1.0 - g.w) * lam);
+    unsafeAtomicAdd(grad + g.c + 1, (1.0 - g.u) * g.w * lam);
+    unsafeAtomicAdd(grad + d, g.u * (1.0 - g.w) * lam);
+    unsafeAtomicAdd(grad + d + 1, g.u * g.w * lam);
+}
+
+// The transposed field evaluation: given lambda_g for g = q/m E(q), scatter into dJ/dfx,
+// dJ/dfy and return lambda_q.
+template <int GEOM>
+__device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, double qm, Vec3 q, Vec3 lg) {
+    double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
+    if (GEOM == 1) {
+        r = hypot(q.b, q.c);
+        if (r > 0.0) { cy = q.b / r; cz = q.c / r; }
+        qy = r;
+    } else if (GEOM == 2) {
+        cy = 1.0;
+        if (q.b < p.origin_y) cy = -1.0;
+        qy = p.origin_y + fabs(q.b - p.origin_y);
+    }
+    const double sx = (q.a - p.origin_x) / p.h;
+    const double sy = (qy - p.origin_y) / p.h;
+    const Gathered gx = gather_d(p.fxs, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const Gathered gy = gather_d(p.fys, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    const double lex = qm * lg.a;
+    double ley, lcy = 0.0, lcz = 0.0;
+    if (GEOM == 1) {
+        ley = qm * (lg.b * cy + lg.c * cz);
+        lcy = qm * lg.b * gy.value;
+        lcz = qm * lg.c * gy.value;
+    } else if (GEOM == 2) {
+        ley = qm * lg.b * cy;
+    } else {
+        ley = qm * lg.b;
+    }
+    scatter(p.dfx, p.ny, gx, lex);
+    scatter(p.dfy, p.ny + 1, gy, ley);
+    const double lsx = lex * gx.ds + ley * gy.ds;
+    const double lsy = lex * gx.dt + ley * gy.dt;
+    Vec3 lq = {lsx / p.h, 0.0, 0.0};
+    const double lqy = lsy / p.h;
+    if (GEOM == 1) {
+        if (r > 0.0) {
+            // r = |(y, z)|, cy = y / r, cz = z / r
+            lq.b = lqy * cy + (lcy * cz * cz - lcz * cy * cz) / r;
+            lq.c = lqy * cz + (lcz * cy * cy - lcy * cy * cz) / r;
+        }
+    } else if (GEOM == 2) {
+        lq.b = lqy * cy;
+    } else {
+        lq.b = lqy;
+    }
+    return lq;
+}
+
+// dt of the step rule and its branch: 1 = h / |v|, 2 = sqrt(2 h / |a|), 0 = h.
+__device__ __forceinline__ double step_size(const AdjointParams& p, Vec3 v, Vec3 a, int& branch, double& speed,
+                                            double& acc) {
+    speed = 1.0 * sqrt(v.a * v.a + v.b * v.b + v.c * v.c);
+    acc = sqrt(a.a * a.a + a.b * a.b + a.c * a.c);
+    double dt = -1.0;
+    branch = 0;
+    if (speed > 0.0) { dt = p.h / speed; branch = 1; }
+    if (acc > 0.0) {
+        const double d2 = sqrt(2.0 * p.h / acc);
+        if (dt < 0.0 || d2 < dt) { dt = d2; branch = 2; }
+    }
+    if (dt < 0.0) dt = p.h;
+    return p.step_fraction * dt;
+}
+
+// One forward step (the trace kernel's arithmetic).
+template <int INTEG, int GEOM>
+__device__ __forceinline__ void forward_step(const AdjointParams& p, double qm, Vec3& x, Vec3& v) {
+    const Vec3 a = accel<GEOM>(p, qm, x);
+    int branch;
+    double speed, acc;
+    const double dt = step_size(p, v, a, branch, speed, acc);
+    const double hdt = 0.5 * dt;
+    if (INTEG == 0) {
+        const double dt6 = dt / 6.0;
+        Vec3 sx = v, sv = a;
+        Vec3 q = vaxpy(x, hdt, v), k = vaxpy(v, hdt, a);
+        Vec3 g = accel<GEOM>(p, qm, q);
+        sx = vaxpy(sx, 2.0, k); sv = vaxpy(sv, 2.0, g);
+        q = vaxpy(x, hdt, k); k = vaxpy(v, hdt, g);
+        g = accel<GEOM>(p, qm, q);
+        sx = vaxpy(sx, 2.0, k); sv = vaxpy(sv, 2.0, g);
+        q = vaxpy(x, dt, k); k = vaxpy(v, dt, g);
+        g = accel<GEOM>(p, qm, q);
+        sx = vadd(sx, k); sv = vadd(sv, g);
+        x = vaxpy(x, dt6, sx);
+        v = vaxpy(v, dt6, sv);
+    } else {
+        const Vec3 k = vaxpy(v, hdt, a);
+        const Vec3 q = vaxpy(x, dt, k);
+        const Vec3 g = accel<GEOM>(p, qm, q);
+        x = q;
+        v = vaxpy(k, hdt, g);
+    }
+}
+
+// The seeds of the records of step m (1-based) of trajectory k: the crossing's adjoint
+// split into the step's start (ls_x, ls_v) and end (le_x, le_v) terms.
+__device__ __forceinline__ void crossing_seeds(const AdjointParams& p, i64 k, i64 m, Vec3 x, Vec3 v, Vec3 y, Vec3 w,
+                                               Vec3& lsx, Vec3& lsv, Vec3& lex, Vec3& lev) {
+    for (int pl = 0; pl < p.n_planes; ++pl) {
+        const i64 idx = k * p.n_planes + pl;
+        i64 cnt = p.counts[idx];
+        if (cnt > p.capacity) cnt = p.capacity;
+        for (i64 c = 0; c < cnt; ++c) {
+            const double* rec = p.hits + (idx * p.capacity + c) * 8;
+            if ((i64)rec[7] != m) continue;
+            const double* sd = p.seeds + (idx * p.capacity + c) * 5;
+            const double xp = p.planes[pl];
+            const double before = x.a - xp;
+            const double after = y.a - xp;
+            const double sfrac = before / (before - after);
+            const double ls = sd[0] * (y.b - x.b) + sd[1] * (y.c - x.c) + sd[2] * (w.a - v.a) + sd[3] * (w.b - v.b) +
+                              sd[4] * (w.c - v.c);
+            const double den = before - after;
+            lsx.a += ls * (-after) / (den * den);
+            lex.a += ls * before / (den * den);
+            lsx.b += sd[0] * (1.0 - sfrac); lsx.c += sd[1] * (1.0 - sfrac);
+            lex.b += sd[0] * sfrac; lex.c += sd[1] * sfrac;
+            lsv.a += sd[2] * (1.0 - sfrac); lsv.b += sd[3] * (1.0 - sfrac); lsv.c += sd[4] * (1.0 - sfrac);
+            lev.a += sd[2] * sfrac; lev.b += sd[3] * sfrac; lev.c += sd[4] * sfrac;
+        }
+    }
+}
+
+// The largest seeded step below `below` (0 if none).
+__device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 below) {
+    i64 best = 0;
+    for (int pl = 0; pl < p.n_planes; ++pl) {
+        const i64 idx = k * p.n_planes + pl;
+        i64 cnt = p.counts[idx];
+        if (cnt > p.capacity) cnt = p.capacity;
+        for (i64 c = 0; c < cnt; ++c) {
+            const double* sd = p.seeds + (idx * p.capacity + c) * 5;
+            if (sd[0] == 0.0 && sd[1] == 0.0 && sd[2] == 0.0 && sd[3] == 0.0 && sd[4] == 0.0) continue;
+            const i64 m = (i64)p.hits[(idx * p.capacity + c) * 8 + 7];
+            if (m < below && m > best) best = m;
+        }
+    }
+    return best;
+}
+
+// The transposed step m (start state x, v): lambda of the end (lx, lv) -> of the start.
+template <int INTEG, int GEOM>
+__device__ __forceinline__ void reverse_step(const AdjointParams& p, i64 k, i64 m, bool seeded, double qm, Vec3 x,
+                                             Vec3 v, Vec3& lx, Vec3& lv) {
+    const Vec3 a = accel<GEOM>(p, qm, x);
+    int branch;
+    double speed, acc;
+    const double dt = step_size(p, v, a, branch, speed, acc);
+    const double hdt = 0.5 * dt;
+    Vec3 La = {0.0, 0.0, 0.0}, Lx, Lv;
+    double Ldt = 0.0, Lhdt = 0.0;
+    if (INTEG == 0) {
+        const double dt6 = dt / 6.0;
+        const Vec3 q1 = vaxpy(x, hdt, v), k1 = vaxpy(v, hdt, a);
+        const Vec3 g1 = accel<GEOM>(p, qm, q1);
+        const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g1);
+        const Vec3 g2 = accel<GEOM>(p, qm, q2);
+        const Vec3 q3 = vaxpy(x, dt, k2), k3 = vaxpy(v, dt, g2);
+        const Vec3 g3 = accel<GEOM>(p, qm, q3);
+        const Vec3 sx = vadd(vaxpy(vaxpy(v, 2.0, k1), 2.0, k2), k3);
+        const Vec3 sv = vadd(vaxpy(vaxpy(a, 2.0, g1), 2.0, g2), g3);
+        Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
+        if (seeded) {
+            const Vec3 y = vaxpy(x, dt6, sx), w = vaxpy(v, dt6, sv);
+            crossing_seeds(p, k, m, x, v, y, w, xs, vs, lx, lv);
+        }
+        Lx = lx;
+        Lv = lv;
+        const double Ldt6 = vdot(lx, sx) + vdot(lv, sv);
+        const Vec3 Lsx = vscale(dt6, lx), Lsv = vscale(dt6, lv);
+        Lv = vadd(Lv, Lsx);
+        Vec3 Lk1 = vscale(2.0, Lsx), Lk2 = vscale(2.0, Lsx);
+        const Vec3 Lk3 = Lsx;
+        La = Lsv;
+        Vec3 Lg1 = vscale(2.0, Lsv), Lg2 = vscale(2.0, Lsv);
+        const Vec3 Lg3 = Lsv;
+        const Vec3 Lq3 = accel_adjoint<GEOM>(p, qm, q3, Lg3);
+        Lv = vadd(Lv, Lk3); Ldt += vdot(Lk3, g2); Lg2 = vaxpy(Lg2, dt, Lk3);
+        Lx = vadd(Lx, Lq3); Ldt += vdot(Lq3, k2); Lk2 = vaxpy(Lk2, dt, Lq3);
+        const Vec3 Lq2 = accel_adjoint<GEOM>(p, qm, q2, Lg2);
+        Lv = vadd(Lv, Lk2); Lhdt += vdot(Lk2, g1); Lg1 = vaxpy(Lg1, hdt, Lk2);
+        Lx = vadd(Lx, Lq2); Lhdt += vdot(Lq2, k1); Lk1 = vaxpy(Lk1, hdt, Lq2);
+        const Vec3 Lq1 = accel_adjoint<GEOM>(p, qm, q1, Lg1);
+        Lv = vadd(Lv, Lk1); Lhdt += vdot(Lk1, a); La = vaxpy(La, hdt, Lk1);
+        Lx = vadd(Lx, Lq1); Lhdt += vdot(Lq1, v); Lv = vaxpy(Lv, hdt, Lq1);
+        Ldt += Ldt6 / 6.0;
+        Lx = vadd(Lx, xs);
+        Lv = vadd(Lv, vs);
+    } else {
+        const Vec3 kk = vaxpy(v, hdt, a);
+        const Vec3 q = vaxpy(x, dt, kk);
+        const Vec3 g1 = accel<GEOM>(p, qm, q);
+        Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
+        if (seeded) {
+            const Vec3 w = vaxpy(kk, hdt, g1);
+            crossing_seeds(p, k, m, x, v, q, w, xs, vs, lx, lv);
+        }
+        Vec3 Lk = lv;
+        Lhdt += vdot(lv, g1);
+        const Vec3 Lg1 = vscale(hdt, lv);
+        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, qm, q, Lg1));
+        Lx = Lq;
+        Ldt += vdot(Lq, kk);
+        Lk = vaxpy(Lk, dt, Lq);
+        Lv = Lk;
+        Lhdt += vdot(Lk, a);
+        La = vscale(hdt, Lk);
+        Lx = vadd(Lx, xs);
+        Lv = vadd(Lv, vs);
+    }
+    Ldt += 0.5 * Lhdt;
+    if (branch == 1) {
+        Lv = vaxpy(Lv, -Ldt * dt / (speed * speed), v);
+    } else if (branch == 2) {
+        La = vaxpy(La, -0.5 * Ldt * dt / (acc * acc), a);
+    }
+    Lx = vadd(Lx, accel_adjoint<GEOM>(p, qm, x, La));
+    lx = Lx;
+    lv = Lv;
+}
+
+// scratch: checkpoints [n_ckpt][6][lanes], then segment states [interval][6][lanes]
+__device__ __forceinline__ void put_state(double* __restrict__ base, i64 lanes, i64 slot, i64 lane, Vec3 x, Vec3 v) {
+    double* s = base + slot * 6 * lanes + lane;
+    s[0] = x.a; s[lanes] = x.b; s[2 * lanes] = x.c;
+    s[3 * lanes] = v.a; s[4 * lanes] = v.b; s[5 * lanes] = v.c;
+}
+
+__device__ __forceinline__ void get_state(const double* __restrict__ base, i64 lanes, i64 slot, i64 lane, Vec3& x,
+                                          Vec3& v) {
+    const double* s = base + slot * 6 * lanes + lane;
+    x = {s[0], s[lanes], s[2 * lanes]};
+    v = {s[3 * lanes], s[4 * lanes], s[5 * lanes]};
+}
+
+template <int INTEG, int GEOM>
+__global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
+    const unsigned lane = __lane_id();
+    const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const i64 gl = (i64)blockIdx.x * blockDim.x + threadIdx.x;  // this lane's scratch column
+    double* ck = p.scratch;
+    double* seg = p.scratch + (i64)p.n_ckpt * 6 * p.lanes;
+    const i64 K = p.interval;
+    bool exhausted = false;
+    for (;;) {
+        i64 k = -1;
+        const bool want = !exhausted;
+        const u64 need = __ballot(want);
+        if (need != 0) {
+            const int count = __popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            u64 base = 0;
+            if ((int)lane == leader) base = atomicAdd(p.queue, (u64)count);
+            base = __shfl(base, leader);
+            if (want) {
+                const u64 mm = base + (u64)__popcll(need & below);
+                if (mm < (u64)p.n_active) {
+                    const i64 kk = p.active[mm];
+                    if (kk >= 0 && kk < p.n) k = kk;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(k >= 0 || !exhausted) == 0) break;
+        const i64 last = k >= 0 ? seeded_step(p, k, 0x7fffffffffffffffll) : 0;
+        // a trajectory with no seed, or seeded beyond the checkpoints, adds nothing
+        if (last > 0 && (last - 1) / K < p.n_ckpt) {
+            const i64 N = p.n;
+            const double qm = p.qm[k];
+            Vec3 x = {p.x[k], p.x[N + k], p.x[2 * N + k]};
+            Vec3 v = {p.v[k], p.v[N + k], p.v[2 * N + k]};
+            // 1. forward to the last seeded step, a checkpoint every K steps
+            for (i64 m = 0; m < last; ++m) {
+                if (m % K == 0) put_state(ck, p.lanes, m / K, gl, x, v);
+                forward_step<INTEG, GEOM>(p, qm, x, v);
+            }
+            // 2. the segments backward
+            Vec3 lx = {0.0, 0.0, 0.0}, lv = {0.0, 0.0, 0.0};
+            i64 next = last;
+            for (i64 c = (last - 1) / K; c >= 0; --c) {
+                const i64 m0 = c * K;
+                const i64 m1 = m0 + K < last ? m0 + K : last;
+                get_state(ck, p.lanes, c, gl, x, v);
+                for (i64 m = m0; m < m1; ++m) {
+                    put_state(seg, p.lanes, m - m0, gl, x, v);
+                    if (m + 1 < m1) forward_step<INTEG, GEOM>(p, qm, x, v);
+                }
+                for (i64 m = m1 - 1; m >= m0; --m) {
+                    get_state(seg, p.lanes, m - m0, gl, x, v);
+                    const bool seeded = m + 1 == next;
+                    reverse_step<INTEG, GEOM>(p, k, m + 1, seeded, qm, x, v, lx, lv);
+                    if (seeded) next = seeded_step(p, k, m + 1);
+                }
+            }
+            if (p.dx != nullptr) {
+                p.dx[k] += lx.a; p.dx[N + k] += lx.b; p.dx[2 * N + k] += lx.c;
+            }
+            if (p.dv != nullptr) {
+                p.dv[k] += lv.a; p.dv[N + k] += lv.b; p.dv[2 * N + k] += lv.c;
+            }
+        }
+    }
+}
+
+__global__ void adjoint_prepare_kernel(u64* __restrict__ queue) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0ull;
+}
+
+typedef void (*AdjointKernel)(AdjointParams);
+
+AdjointKernel select_adjoint_kernel(int integ, int geom) {
+#define BT_SELA(I, G) \
+    if (integ == I && geom == G) return adjoint_kernel<I, G>;
+    BT_SELA(0, 0) BT_SELA(0, 1) BT_SELA(0, 2) BT_SELA(1, 0) BT_SELA(1, 1) BT_SELA(1, 2)
+#undef BT_SELA
+    return nullptr;
+}
+
+int adjoint_occupancy(int integ, int geom, int block) {
+    AdjointKernel kern = select_adjoint_kernel(integ, geom);
+    if (kern == nullptr) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern), block, 0) !=
+        hipSuccess)
+        return 0;
+    return blocks;
+}
+
+int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom) {
+    AdjointKernel kern = select_adjoint_kernel(integ, geom);
+    if (kern == nullptr) return -1;
+    hipLaunchKernelGGL(adjoint_prepare_kernel, dim3(1), dim3(64), 0, stream, p.queue);
+    if (p.n_active > 0 && grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, p);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace bt

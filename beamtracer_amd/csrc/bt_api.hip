@@ -48,6 +48,7 @@ struct Buffer {
 struct DeviceState {
     std::vector<Buffer> buffers;   // host-form staging, by slot
     Buffer workspace;              // library-owned workspace
+    Buffer adjoint;                // library-owned adjoint scratch
     hipStream_t stream = nullptr;  // host-form stream
     int cus = 0;
 };
@@ -554,6 +555,62 @@ int bt_deposit_trajectories_2d(const double* px, const double* py, const double*
     return BT_OK;
 }
 
+
+int64_t bt_adjoint_args_size(void) { return (int64_t)sizeof(bt_adjoint_args); }
+
+int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
+    if (a == nullptr) return fail(BT_E_ARG, "args is NULL");
+    if (a->n < 0 || a->n_active < 0) return fail(BT_E_ARG, "negative trajectory count");
+    if (a->nx < 2 || a->ny < 2) return fail(BT_E_ARG, "fewer than 2 nodes per axis");
+    if ((a->nx + 1) * (a->ny + 1) >= ((i64)1 << 29)) return fail(BT_E_ARG, "2^29 nodes or more");
+    if (!(a->h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    if (a->cylindrical < 0 || a->cylindrical > 2) return fail(BT_E_ARG, "cylindrical must be 0, 1 or 2");
+    if (a->integrator < 0 || a->integrator > 1) return fail(BT_E_ARG, "integrator must be 0 or 1");
+    if (a->n_planes < 1 || a->n_planes > kMaxPlanes) return fail(BT_E_ARG, "n_planes must be 1..256");
+    if (a->capacity < 1) return fail(BT_E_ARG, "capacity must be positive");
+    if (a->interval < 1) return fail(BT_E_ARG, "interval must be positive");
+    if (a->max_step < 0) return fail(BT_E_ARG, "negative max_step");
+    if (a->block_size != 0 && (a->block_size % 64 != 0 || a->block_size < 64 || a->block_size > 256))
+        return fail(BT_E_ARG, "block_size must be a multiple of 64 in 64..256");
+    if (a->n_active > 0 && (a->active == nullptr || a->x == nullptr || a->v == nullptr || a->qm == nullptr ||
+                            a->fx == nullptr || a->fy == nullptr || a->planes == nullptr || a->hits == nullptr ||
+                            a->counts == nullptr || a->seeds == nullptr || a->dfx == nullptr || a->dfy == nullptr))
+        return fail(BT_E_ARG, "an array is NULL");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    int rc = device_state(&s);
+    if (rc) return rc;
+    const int integ = (int)a->integrator, geom = (int)a->cylindrical;
+    const int block = a->block_size > 0 ? a->block_size : 256;
+    int grid = 0;
+    if (a->n_active > 0) {
+        int per_cu = bt::adjoint_occupancy(integ, geom, block);
+        if (per_cu < 1) per_cu = 1;
+        const i64 full = (i64)per_cu * s->cus;
+        const i64 need = (a->n_active + block - 1) / block;
+        grid = (int)(need < full ? need : full);
+        if (grid < 1) grid = 1;
+    }
+    const i64 lanes = (i64)grid * block;
+    const i64 n_ckpt = a->max_step > 0 ? (a->max_step - 1) / a->interval + 1 : 1;
+    const size_t bytes = 256 + (size_t)(n_ckpt + a->interval) * 6 * 8 * (size_t)(lanes > 0 ? lanes : 1);
+    if ((rc = ensure(s->adjoint, bytes))) return rc;
+    bt::AdjointParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.active = (const i64*)a->active; p.n_active = a->n_active; p.n = a->n;
+    p.x = a->x; p.v = a->v; p.qm = a->qm;
+    p.fxs = a->fx; p.fys = a->fy; p.nx = a->nx; p.ny = a->ny;
+    p.origin_x = a->origin_x; p.origin_y = a->origin_y; p.h = a->h; p.step_fraction = a->step_fraction;
+    p.planes = a->planes; p.n_planes = (int)a->n_planes;
+    p.hits = a->hits; p.capacity = a->capacity; p.counts = (const i64*)a->counts; p.seeds = a->seeds;
+    p.interval = a->interval; p.n_ckpt = n_ckpt; p.lanes = lanes;
+    p.queue = static_cast<u64*>(s->adjoint.ptr);
+    p.scratch = reinterpret_cast<double*>(static_cast<char*>(s->adjoint.ptr) + 256);
+    p.dfx = a->dfx; p.dfy = a->dfy; p.dx = a->dx; p.dv = a->dv;
+    if (bt::launch_adjoint(p, block, grid, static_cast<hipStream_t>(stream), integ, geom))
+        return fail(BT_E_LAUNCH, std::string("adjoint launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
 
 int64_t bt_batch_args_size(void) { return (int64_t)sizeof(bt_batch_args); }
 

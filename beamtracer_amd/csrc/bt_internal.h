@@ -69,6 +69,24 @@ int launch_batched(const TraceParams& p, const double* x0, const double* v0, boo
                    hipStream_t stream, int integ, int geom, bool has_b, i64 total_nodes, i64 max_design_nodes);
 int batched_occupancy(int integ, int geom, bool has_b, int block);
 
+// The adjoint of a trace (bt_adjoint.hip): seeds dJ/d(record) of the plane crossings ->
+// dJ/dfx, dJ/dfy (and the initial states' adjoints), one uniform mesh, classical, no B.
+struct AdjointParams {
+    const i64* active; i64 n_active; i64 n;
+    const double* x; const double* v; const double* qm;
+    const double* fxs; const double* fys; i64 nx, ny;
+    double origin_x, origin_y, h, step_fraction;
+    const double* planes; int n_planes;
+    const double* hits; i64 capacity; const i64* counts; const double* seeds;
+    i64 interval, n_ckpt, lanes;
+    double* scratch;
+    double* dfx; double* dfy; double* dx; double* dv;
+    u64* queue;
+};
+
+int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom);
+int adjoint_occupancy(int integ, int geom, int block);
+
 int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
                   i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,
                   hipStream_t stream);

@@ -66,10 +66,17 @@ class BoltzmannIons:
 class IonOpticsPoisson:
     """Poisson's equation on an :class:`IonOpticsMesh`, assembled once and factorized."""
 
-    def __init__(self, mesh: IonOpticsMesh, fixed=None, fixed_value=None):
+    def __init__(self, mesh: IonOpticsMesh, fixed=None, fixed_value=None, potentials=None):
+        """``potentials`` replaces the electrodes' potentials (one per electrode) in the
+        surface rows; ``fixed_value`` should then hold them on the electrodes' nodes."""
         self.mesh = mesh
         self.fixed = mesh.fixed if fixed is None else np.asarray(fixed, dtype=bool)
         self.fixed_value = mesh.fixed_value if fixed_value is None else np.asarray(fixed_value, dtype=float)
+        if potentials is None:
+            potentials = [e.potential for e in mesh.electrodes]
+        if len(potentials) != len(mesh.electrodes):
+            raise ValueError("one potential per electrode")
+        self.potentials = np.array(list(potentials) + [0.0], dtype=float)
         self._assemble()
         self._lu = None
         self._lu_reaction = None
@@ -84,7 +91,7 @@ class IonOpticsPoisson:
         fixed = self.fixed
         value = self.fixed_value
         solid = mesh.solid
-        potentials = np.array([e.potential for e in mesh.electrodes] + [0.0])
+        potentials = self.potentials
         index = -np.ones((nx, ny), dtype=np.int64)
         free = ~fixed
         index[free] = np.arange(int(free.sum()))
@@ -205,7 +212,8 @@ class IonOpticsPoisson:
             key = (region.tobytes(), float(fixed_potential))
             if self._region is None or self._region[0] != key:
                 solver = IonOpticsPoisson(mesh, self.fixed | region,
-                                          np.where(region, float(fixed_potential), self.fixed_value))
+                                          np.where(region, float(fixed_potential), self.fixed_value),
+                                          self.potentials[:-1])  # fmt: skip
                 self._region = (key, solver)
             return self._region[1].solve(rho, plasma=plasma, initial=initial, tolerance=tolerance,
                                          max_newton=max_newton)

@@ -313,6 +313,7 @@ class HipBeamTracer:
         charge = charge.sum(axis=0).reshape(mesh.shape)
         hit_planes = collect_plane_hits(self.planes, hits, counts, current, species)
         self.last_state = (x, v, steps)
+        self.last_records = (hits, counts)
         return TraceResult(
             charge_density=charge,
             status=status.astype(np.int8),

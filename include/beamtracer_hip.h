@@ -187,6 +187,38 @@ int bt_batch_graph_create(void* plan, const bt_batch_args* args, void** graph);
 int bt_graph_launch(void* graph, void* stream);
 int bt_graph_destroy(void* graph);
 
+/* ---- adjoint ---------------------------------------------------------------------------
+ * The discrete adjoint of bt_trace_trajectories_2d for an objective J of the plane
+ * crossings (no counterpart in the reference, which has no gradients): given the forward
+ * trace's records (hits, counts) and seeds dJ/d(y, z, vx, vy, vz) of every record, adds
+ * dJ/dfx and dJ/dfy of the edge fields (and dJ/dx, dJ/dv of the initial states) for the
+ * electrostatic, classical trace on one uniform mesh.  Each trajectory is stepped again
+ * from its initial state to its last seeded step with a checkpoint every `interval`
+ * steps, then walked backward segment by segment (recomputed from its checkpoint).
+ * Device memory, asynchronous on `stream`; the library-owned scratch is
+ * (max_step / interval + 1 + interval) * 48 bytes per lane of the resident grid. */
+typedef struct bt_adjoint_args {
+    const int64_t* active; int64_t n_active;      /* the trajectories with seeded records  */
+    const double* x; const double* v; int64_t n;  /* initial states (3, n)                 */
+    const double* qm;                             /* (n,)                                  */
+    const double* fx; const double* fy;           /* (nx + 1, ny), (nx, ny + 1)             */
+    int64_t nx, ny;
+    double origin_x, origin_y, h;
+    int64_t cylindrical, integrator;
+    double step_fraction;
+    const double* planes; int64_t n_planes;
+    const double* hits; int64_t capacity; const int64_t* counts;  /* the forward trace's   */
+    const double* seeds;                          /* (n, n_planes, capacity, 5)             */
+    int64_t max_step;                             /* largest seeded record step            */
+    int32_t interval;                             /* steps between checkpoints (>= 1)      */
+    int32_t block_size;                           /* 0: 256                                 */
+    double* dfx; double* dfy;                     /* += dJ/dfx, dJ/dfy                      */
+    double* dx; double* dv;                       /* += dJ/dx, dJ/dv (3, n), or NULL        */
+} bt_adjoint_args;
+
+int64_t bt_adjoint_args_size(void);
+int bt_trace_adjoint_device(const bt_adjoint_args* args, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,8 @@ package ``beamtracer_amd`` never imports this module.
   ``deposit_trajectories_2d_pyccel.py:13-59`` and
   ``trace_trajectories_2d_pyccel.py:19-507`` (reference paths under
   ``src/beamtracer/kernels/``).
+* ``tangent.c`` (in the same library) is the forward-mode tangent of the C tracer along a
+  direction of the edge fields: the FP64 CPU tangent that the GPU adjoint must transpose.
 * ``numpy_ref.py`` restates the vectorized NumPy versions
   (``staggered_field_2d_numpy.py:12-39``, ``deposit_trajectories_2d_numpy.py:12-42``) and
   the step-synchronous NumPy tracer of ``solver/tracer.py:851-1054`` as an independent
@@ -37,11 +39,12 @@ _ptr = ctypes.c_void_p
 
 
 def build(force: bool = True) -> Path:
-    """Compile ``oracle.c`` into ``liboracle.so`` (gcc, OpenMP, no FMA contraction)."""
+    """Compile ``oracle.c`` and ``tangent.c`` into ``liboracle.so`` (gcc, OpenMP, no FMA
+    contraction)."""
     if force or not LIB_PATH.exists():
         cmd = [
             "gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
-            "-fno-fast-math", "-o", str(LIB_PATH), str(HERE / "oracle.c"), "-lm",
+            "-fno-fast-math", "-o", str(LIB_PATH), str(HERE / "oracle.c"), str(HERE / "tangent.c"), "-lm",
         ]  # fmt: skip
         subprocess.run(cmd, check=True)
     return LIB_PATH
@@ -123,17 +126,13 @@ def deposit_trajectories_2d(px, py, q, origin_x, origin_y, h, cylindrical, charg
     )  # fmt: skip
 
 
-def trace_trajectories_2d(
+def _trace_args(
     active, x, v, qm, dq, steps, status, electrode, fxs, fys, patch_int, patch_real,
     origin_x, origin_y, h, upper_x, upper_y, cylindrical, integrator, step_fraction,
     max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
     path_count, num_threads, charge, graded, x_nodes, inv_c2, magnetic=None,
-) -> None:  # fmt: skip
-    """The fused tracer with the reference's 35 arguments (+ an optional node field).
-
-    ``magnetic`` is None or a tuple of three ``(nx, ny)`` node arrays (or None entries):
-    ``(B_x, B_r, B_theta)`` cylindrical, ``(B_x, B_y, B_z)`` planar.
-    """
+) -> TraceArgs:  # fmt: skip
+    """The checked C argument struct of the reference's 35 arguments (+ a node field)."""
     i64, f64 = np.int64, np.float64
     for name, a, dt in (
         ("active", active, i64), ("x", x, f64), ("v", v, f64), ("qm", qm, f64),
@@ -149,7 +148,8 @@ def trace_trajectories_2d(
     b = [None, None, None]
     if magnetic is not None:
         b = [None if c is None else _check(np.ascontiguousarray(c, f64), f64, "B") for c in magnetic]
-    args = TraceArgs(
+    _last_b[:] = b  # kept alive for the call
+    return TraceArgs(
         _p(active), active.shape[0], _p(x), _p(v), x.shape[1], _p(qm), _p(dq),
         _p(steps), _p(status), _p(electrode), _p(fxs), _p(fys), _p(patch_int),
         _p(patch_real), patch_int.shape[0], float(origin_x), float(origin_y), float(h),
@@ -160,7 +160,35 @@ def trace_trajectories_2d(
         _p(charge), charge.shape[1], int(graded), _p(x_nodes), x_nodes.shape[0],
         float(inv_c2), int(magnetic is not None), _p(b[0]), _p(b[1]), _p(b[2]),
     )  # fmt: skip
-    lib().oracle_trace_trajectories_2d(ctypes.byref(args))
+
+
+_last_b: list = [None, None, None]
+
+
+def trace_trajectories_2d(*args, magnetic=None) -> None:
+    """The fused tracer with the reference's 35 arguments (+ an optional node field).
+
+    ``magnetic`` is None or a tuple of three ``(nx, ny)`` node arrays (or None entries):
+    ``(B_x, B_r, B_theta)`` cylindrical, ``(B_x, B_y, B_z)`` planar.
+    """
+    a = _trace_args(*args, magnetic=magnetic)
+    lib().oracle_trace_trajectories_2d(ctypes.byref(a))
+
+
+def trace_tangent(*args, dfx, dfy) -> np.ndarray:
+    """The trace (35 reference arguments; one uniform patch, classical, no B) with the
+    tangents of its crossing records (y, z, vx, vy, vz) along the edge-field direction
+    (``dfx``, ``dfy``, laid out as ``fxs``, ``fys``): ``(n, n_planes, capacity, 5)``."""
+    a = _trace_args(*args)
+    dfx = _check(np.ascontiguousarray(dfx, np.float64), np.float64, "dfx")
+    dfy = _check(np.ascontiguousarray(dfy, np.float64), np.float64, "dfy")
+    hits = args[25]
+    rec_d = np.zeros(hits.shape[:3] + (5,))
+    L = lib()
+    L.oracle_trace_tangent.restype = ctypes.c_int
+    if L.oracle_trace_tangent(ctypes.byref(a), _p(dfx), _p(dfy), _p(rec_d)) != 0:
+        raise ValueError("the tangent takes one uniform patch, classical, without a magnetic field")
+    return rec_d
 
 
 def default_threads() -> int:
