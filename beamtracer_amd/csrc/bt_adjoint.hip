@@ -16,7 +16,9 @@
 //      checkpoint storing every step's start state, then walks the segment backward,
 //      recomputing each step's stages from its start state and applying the transposed
 //      step to the adjoint state (lambda_x, lambda_v); the transposed gather scatters
-//      lambda_E times the bilinear weights into dJ/dfx and dJ/dfy.
+//      lambda_E times the bilinear weights into dJ/dfx and dJ/dfy, through a window of
+//      node sums in LDS that each wave moves with its lanes (the segments run in step
+//      across the wave).
 // The forward steps here repeat the trace kernel's operations in the same order, so they
 // reproduce its states bit for bit.
 #include <hip/hip_runtime.h>
@@ -62,6 +64,7 @@ __device__ __forceinline__ Vec3 accel(const AdjointParams& p, double qm, Vec3 q)
 struct Gathered {
     double value, ds, dt;
     unsigned c;
+    int i, j;
     double u, w;
 };
 
@@ -70,6 +73,8 @@ __device__ __forceinline__ Gathered gather_d(const double* __restrict__ f, i64 c
     const int i = clamped_floor(s, 0, imax);
     const int j = clamped_floor(t, 0, jmax);
     Gathered g;
+    g.i = i;
+    g.j = j;
     g.u = s - (double)i;
     g.w = t - (double)j;
     g.c = (unsigned)i * (unsigned)cols + (unsigned)j;
@@ -83,20 +88,101 @@ __device__ __forceinline__ Gathered gather_d(const double* __restrict__ f, i64 c
     return g;
 }
 
-// lambda_E times the bilinear weights of cell c into the gradient grid (rows of `cols`).
-__device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, const Gathered& g, double lam) {
+// ---- the wave's gradient window ------------------------------------------------------
+// Each wave sums its lanes' dJ/dfx and dJ/dfy contributions in LDS for kWinCols columns x
+// kWinRows rows of both grids (column i in slot i mod kWinCols) that follow its lanes'
+// cells; a column leaving the window is added to global memory once.  The lanes of a wave
+// trace neighbouring trajectories backward in step, so most contributions stay in LDS.
+constexpr int kWinCols = 16;
+constexpr int kWinRows = 64;
+
+// per wave of a block of at most 256 threads: [0] dJ/dfx, [1] dJ/dfy sums
+__shared__ double adjoint_windows[4][2][kWinCols * kWinRows];
+
+struct GradWindow {
+    int wave;    // this wave's windows in adjoint_windows
+    int lo;      // first column held, -1: none
+    int base;    // first row held
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <int A>
+__device__ __forceinline__ void add_node(double* __restrict__ grad, i64 cols, const GradWindow& W, int ii, int jj,
+                                         double v) {
+    if (W.lo >= 0 && ii >= W.lo && ii < W.lo + kWinCols && jj >= W.base && jj < W.base + kWinRows) {
+        atomicAdd(&adjoint_windows[W.wave][A][(ii & (kWinCols - 1)) * kWinRows + (jj - W.base)], v);
+        return;
+    }
+    unsafeAtomicAdd(grad + (i64)ii * cols + jj, v);
+}
+
+// lambda_E times the bilinear weights of the gathered cell into the gradient of grid A
+// (0: fx, 1: fy; rows of `cols`).
+template <int A>
+__device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, const GradWindow& W, const Gathered& g,
+                                        double lam) {
     if (lam == 0.0) return;
-    const unsigned d = g.c + (unsigned)cols;
-    unsafeAtomicAdd(grad + g.c, (1.0 - g.u) * (1.0 - g.w) * lam);
-    unsafeAtomicAdd(grad + g.c + 1, (1.0 - g.u) * g.w * lam);
-    unsafeAtomicAdd(grad + d, g.u * (1.0 - g.w) * lam);
-    unsafeAtomicAdd(grad + d + 1, g.u * g.w * lam);
+    add_node<A>(grad, cols, W, g.i, g.j, (1.0 - g.u) * (1.0 - g.w) * lam);
+    add_node<A>(grad, cols, W, g.i, g.j + 1, (1.0 - g.u) * g.w * lam);
+    add_node<A>(grad, cols, W, g.i + 1, g.j, g.u * (1.0 - g.w) * lam);
+    add_node<A>(grad, cols, W, g.i + 1, g.j + 1, g.u * g.w * lam);
+}
+
+// Column `col` of the window (slot s) to global memory; lane r takes row base + r.
+__device__ __forceinline__ void flush_column(const AdjointParams& p, const GradWindow& W, int s, int col) {
+    const int r = (int)__lane_id();
+    const int row = W.base + r;
+    const double vx = __longlong_as_double((long long)atomicExch(
+        reinterpret_cast<unsigned long long*>(&adjoint_windows[W.wave][0][s * kWinRows + r]), 0ull));
+    const double vy = __longlong_as_double((long long)atomicExch(
+        reinterpret_cast<unsigned long long*>(&adjoint_windows[W.wave][1][s * kWinRows + r]), 0ull));
+    if (vx != 0.0 && col <= p.nx && row < p.ny) unsafeAtomicAdd(p.dfx + (i64)col * p.ny + row, vx);
+    if (vy != 0.0 && col < p.nx && row <= p.ny) unsafeAtomicAdd(p.dfy + (i64)col * (p.ny + 1) + row, vy);
+}
+
+__device__ __forceinline__ void window_flush_all(const AdjointParams& p, GradWindow& W) {
+    if (W.lo < 0) return;
+    for (int s = 0; s < kWinCols; ++s) flush_column(p, W, s, W.lo + ((s - W.lo) & (kWinCols - 1)));
+    W.lo = -1;
+}
+
+// Moves the window (wave-uniform; every lane of the wave calls it) to hold the cells
+// (ci, cj) of the active lanes with room below them, the direction of the backward walk.
+__device__ __forceinline__ void window_follow(const AdjointParams& p, GradWindow& W, bool act, int ci, int cj) {
+    const int imin = wave_min(act ? ci : 0x7fffffff), imax = wave_max(act ? ci : -1);
+    const int jmin = wave_min(act ? cj : 0x7fffffff), jmax = wave_max(act ? cj : -1);
+    if (imax < 0) return;
+    const bool rows_ok = W.lo >= 0 && jmin - 1 >= W.base && jmax + 2 < W.base + kWinRows;
+    const bool cols_ok = W.lo >= 0 && imin - 2 >= W.lo && imax + 2 <= W.lo + kWinCols - 1;
+    if (rows_ok && cols_ok) return;
+    int nlo = imax + 2 - (kWinCols - 1);
+    if (nlo > imin - 2) nlo = imin - 2;
+    if (nlo < 0) nlo = 0;
+    const int nbase = rows_ok ? W.base : (jmin - 2 > 0 ? jmin - 2 : 0);
+    if (W.lo >= 0) {
+        for (int s = 0; s < kWinCols; ++s) {
+            const int col = W.lo + ((s - W.lo) & (kWinCols - 1));
+            if (!(rows_ok && col >= nlo && col < nlo + kWinCols)) flush_column(p, W, s, col);
+        }
+    }
+    W.lo = nlo;
+    W.base = nbase;
 }
 
 // The transposed field evaluation: given lambda_g for g = q/m E(q), scatter into dJ/dfx,
 // dJ/dfy and return lambda_q.
 template <int GEOM>
-__device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, double qm, Vec3 q, Vec3 lg) {
+__device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const GradWindow& W, double qm, Vec3 q,
+                                              Vec3 lg) {
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
     if (GEOM == 1) {
         r = hypot(q.b, q.c);
@@ -122,8 +208,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, double qm,
     } else {
         ley = qm * lg.b;
     }
-    scatter(p.dfx, p.ny, gx, lex);
-    scatter(p.dfy, p.ny + 1, gy, ley);
+    scatter<0>(p.dfx, p.ny, W, gx, lex);
+    scatter<1>(p.dfy, p.ny + 1, W, gy, ley);
     const double lsx = lex * gx.ds + ley * gy.ds;
     const double lsy = lex * gx.dt + ley * gy.dt;
     Vec3 lq = {lsx / p.h, 0.0, 0.0};
@@ -237,8 +323,8 @@ __device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 be
 
 // The transposed step m (start state x, v): lambda of the end (lx, lv) -> of the start.
 template <int INTEG, int GEOM>
-__device__ __forceinline__ void reverse_step(const AdjointParams& p, i64 k, i64 m, bool seeded, double qm, Vec3 x,
-                                             Vec3 v, Vec3& lx, Vec3& lv) {
+__device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradWindow& W, i64 k, i64 m, bool seeded,
+                                             double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv) {
     const Vec3 a = accel<GEOM>(p, qm, x);
     int branch;
     double speed, acc;
@@ -271,13 +357,13 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, i64 k, i64 
         La = Lsv;
         Vec3 Lg1 = vscale(2.0, Lsv), Lg2 = vscale(2.0, Lsv);
         const Vec3 Lg3 = Lsv;
-        const Vec3 Lq3 = accel_adjoint<GEOM>(p, qm, q3, Lg3);
+        const Vec3 Lq3 = accel_adjoint<GEOM>(p, W, qm, q3, Lg3);
         Lv = vadd(Lv, Lk3); Ldt += vdot(Lk3, g2); Lg2 = vaxpy(Lg2, dt, Lk3);
         Lx = vadd(Lx, Lq3); Ldt += vdot(Lq3, k2); Lk2 = vaxpy(Lk2, dt, Lq3);
-        const Vec3 Lq2 = accel_adjoint<GEOM>(p, qm, q2, Lg2);
+        const Vec3 Lq2 = accel_adjoint<GEOM>(p, W, qm, q2, Lg2);
         Lv = vadd(Lv, Lk2); Lhdt += vdot(Lk2, g1); Lg1 = vaxpy(Lg1, hdt, Lk2);
         Lx = vadd(Lx, Lq2); Lhdt += vdot(Lq2, k1); Lk1 = vaxpy(Lk1, hdt, Lq2);
-        const Vec3 Lq1 = accel_adjoint<GEOM>(p, qm, q1, Lg1);
+        const Vec3 Lq1 = accel_adjoint<GEOM>(p, W, qm, q1, Lg1);
         Lv = vadd(Lv, Lk1); Lhdt += vdot(Lk1, a); La = vaxpy(La, hdt, Lk1);
         Lx = vadd(Lx, Lq1); Lhdt += vdot(Lq1, v); Lv = vaxpy(Lv, hdt, Lq1);
         Ldt += Ldt6 / 6.0;
@@ -295,7 +381,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, i64 k, i64 
         Vec3 Lk = lv;
         Lhdt += vdot(lv, g1);
         const Vec3 Lg1 = vscale(hdt, lv);
-        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, qm, q, Lg1));
+        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, W, qm, q, Lg1));
         Lx = Lq;
         Ldt += vdot(Lq, kk);
         Lk = vaxpy(Lk, dt, Lq);
@@ -311,7 +397,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, i64 k, i64 
     } else if (branch == 2) {
         La = vaxpy(La, -0.5 * Ldt * dt / (acc * acc), a);
     }
-    Lx = vadd(Lx, accel_adjoint<GEOM>(p, qm, x, La));
+    Lx = vadd(Lx, accel_adjoint<GEOM>(p, W, qm, x, La));
     lx = Lx;
     lv = Lv;
 }
@@ -330,10 +416,28 @@ __device__ __forceinline__ void get_state(const double* __restrict__ base, i64 l
     v = {s[3 * lanes], s[4 * lanes], s[5 * lanes]};
 }
 
+// The mesh cell of point x (the window's key), clamped to the mesh.
+template <int GEOM>
+__device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, int& ci, int& cj) {
+    double qy = x.b;
+    if (GEOM == 1) qy = hypot(x.b, x.c);
+    else if (GEOM == 2) qy = p.origin_y + fabs(x.b - p.origin_y);
+    ci = clamped_floor((x.a - p.origin_x) / p.h, 0, (int)p.nx - 1);
+    cj = clamped_floor((qy - p.origin_y) / p.h, 0, (int)p.ny - 1);
+}
+
 template <int INTEG, int GEOM>
 __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    GradWindow W;
+    W.wave = (int)(threadIdx.x >> 6);
+    W.lo = -1;
+    W.base = 0;
+    for (int t = (int)lane; t < kWinCols * kWinRows; t += 64) {
+        adjoint_windows[W.wave][0][t] = 0.0;
+        adjoint_windows[W.wave][1][t] = 0.0;
+    }
     const i64 gl = (i64)blockIdx.x * blockDim.x + threadIdx.x;  // this lane's scratch column
     double* ck = p.scratch;
     double* seg = p.scratch + (i64)p.n_ckpt * 6 * p.lanes;
@@ -360,44 +464,61 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
             }
         }
         if (__ballot(k >= 0 || !exhausted) == 0) break;
-        const i64 last = k >= 0 ? seeded_step(p, k, 0x7fffffffffffffffll) : 0;
+        i64 last = k >= 0 ? seeded_step(p, k, 0x7fffffffffffffffll) : 0;
         // a trajectory with no seed, or seeded beyond the checkpoints, adds nothing
-        if (last > 0 && (last - 1) / K < p.n_ckpt) {
-            const i64 N = p.n;
-            const double qm = p.qm[k];
-            Vec3 x = {p.x[k], p.x[N + k], p.x[2 * N + k]};
-            Vec3 v = {p.v[k], p.v[N + k], p.v[2 * N + k]};
+        const bool valid = last > 0 && (last - 1) / K < p.n_ckpt;
+        if (!valid) last = 0;
+        const i64 N = p.n;
+        const double qm = valid ? p.qm[k] : 0.0;
+        Vec3 x = {0.0, 0.0, 0.0}, v = {0.0, 0.0, 0.0};
+        if (valid) {
+            x = {p.x[k], p.x[N + k], p.x[2 * N + k]};
+            v = {p.v[k], p.v[N + k], p.v[2 * N + k]};
             // 1. forward to the last seeded step, a checkpoint every K steps
             for (i64 m = 0; m < last; ++m) {
                 if (m % K == 0) put_state(ck, p.lanes, m / K, gl, x, v);
                 forward_step<INTEG, GEOM>(p, qm, x, v);
             }
-            // 2. the segments backward
-            Vec3 lx = {0.0, 0.0, 0.0}, lv = {0.0, 0.0, 0.0};
-            i64 next = last;
-            for (i64 c = (last - 1) / K; c >= 0; --c) {
-                const i64 m0 = c * K;
-                const i64 m1 = m0 + K < last ? m0 + K : last;
+        }
+        // 2. the segments backward, in step across the wave (the window moves with them)
+        Vec3 lx = {0.0, 0.0, 0.0}, lv = {0.0, 0.0, 0.0};
+        i64 next = last;
+        const int top = wave_max(valid ? (int)((last - 1) / K) : -1);
+        for (int c = top; c >= 0; --c) {
+            const i64 m0 = (i64)c * K;
+            const bool has = valid && m0 < last;
+            const i64 m1 = has ? (m0 + K < last ? m0 + K : last) : m0;
+            if (has) {
                 get_state(ck, p.lanes, c, gl, x, v);
                 for (i64 m = m0; m < m1; ++m) {
                     put_state(seg, p.lanes, m - m0, gl, x, v);
                     if (m + 1 < m1) forward_step<INTEG, GEOM>(p, qm, x, v);
                 }
-                for (i64 m = m1 - 1; m >= m0; --m) {
+            }
+            for (i64 r = K - 1; r >= 0; --r) {
+                const i64 m = m0 + r;
+                const bool act = has && m < m1;
+                int ci = 0, cj = 0;
+                if (act) {
                     get_state(seg, p.lanes, m - m0, gl, x, v);
+                    cell_of<GEOM>(p, x, ci, cj);
+                }
+                window_follow(p, W, act, ci, cj);
+                if (act) {
                     const bool seeded = m + 1 == next;
-                    reverse_step<INTEG, GEOM>(p, k, m + 1, seeded, qm, x, v, lx, lv);
+                    reverse_step<INTEG, GEOM>(p, W, k, m + 1, seeded, qm, x, v, lx, lv);
                     if (seeded) next = seeded_step(p, k, m + 1);
                 }
             }
-            if (p.dx != nullptr) {
-                p.dx[k] += lx.a; p.dx[N + k] += lx.b; p.dx[2 * N + k] += lx.c;
-            }
-            if (p.dv != nullptr) {
-                p.dv[k] += lv.a; p.dv[N + k] += lv.b; p.dv[2 * N + k] += lv.c;
-            }
+        }
+        if (valid && p.dx != nullptr) {
+            p.dx[k] += lx.a; p.dx[N + k] += lx.b; p.dx[2 * N + k] += lx.c;
+        }
+        if (valid && p.dv != nullptr) {
+            p.dv[k] += lv.a; p.dv[N + k] += lv.b; p.dv[2 * N + k] += lv.c;
         }
     }
+    window_flush_all(p, W);
 }
 
 __global__ void adjoint_prepare_kernel(u64* __restrict__ queue) {

@@ -24,7 +24,10 @@ Beside the headline the line carries, under ``extra``:
   host-form trace with its copies, relaxation 0.5), with the time split;
 * ``batched`` -- 256 einzel candidates (mesh, voltage, gap, energy and beam size drawn from
   one seed) traced in one batched launch, against one plain launch per design: particle-
-  steps/s of each and their ratio, and the rate of the captured HIP graph.
+  steps/s of each and their ratio, and the rate of the captured HIP graph;
+* ``adjoint`` -- one adjoint gradient of ``rms_angle + emittance`` at 75 mm (einzel lens,
+  2e5 trajectories, checkpoints every 64 steps) against one forward trace: both times,
+  their ratio, and dJ/dV of the middle tube against a central difference.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--no-cpu]
                        [--no-extra]
@@ -319,6 +322,77 @@ def batched_leg(count: int, seed: int, processes: int, repeats: int) -> dict:
     }
 
 
+def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
+    """One adjoint gradient of J = rms_angle + emittance at 75 mm against one forward trace."""
+    import torch
+
+    from beamtracer_amd import adjoint, kernels
+
+    design, tracer = build_workload(particles)
+    fxk, fyk, fx0, fy0 = adjoint.unit_fields(design.mesh)
+    volts = np.array([e.potential for e in design.mesh.electrodes])
+    fx = torch.from_numpy(np.tensordot(volts, fxk, 1) + fx0).cuda()
+    fy = torch.from_numpy(np.tensordot(volts, fyk, 1) + fy0).cuda()
+    problem = adjoint.TraceAdjoint(tracer, design.particles, interval=interval)
+    value = problem.objective(fx, fy)
+    hits, counts, seeds = problem._records
+    stream = torch.cuda.current_stream()
+    # forward: the plain trace of every trajectory (inputs in HBM, as the headline)
+    host = kernel_arguments(design, tracer)
+    fwd = time_traces(host, None, repeats, 1, 0)
+    # adjoint: the kernel alone on device-resident records and seeds
+    p = design.particles
+    dev = torch.device("cuda")
+    x0 = torch.from_numpy(np.ascontiguousarray(p.position)).to(dev)
+    v0 = torch.from_numpy(np.ascontiguousarray(p.velocity)).to(dev)
+    qm = torch.from_numpy(np.ascontiguousarray(p.charge / p.mass)).to(dev)
+    planes = torch.tensor(tracer.planes, dtype=torch.float64, device=dev)
+    hits_d = torch.from_numpy(hits).to(dev)
+    counts_d = torch.from_numpy(counts).to(dev)
+    seeds_d = torch.from_numpy(seeds).to(dev)
+    seeded = np.flatnonzero(np.abs(seeds).sum(axis=(1, 2, 3)) > 0)
+    active = torch.from_numpy(seeded.astype(np.int64)).to(dev)
+    mask = np.abs(seeds).sum(-1) > 0
+    max_step = int(hits[..., 7][mask].max())
+    times = []
+    for r in range(repeats + 1):
+        dfx, dfy = torch.zeros_like(fx), torch.zeros_like(fy)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        problem.launch(active, x0, v0, qm, fx, fy, planes, hits_d, counts_d, seeds_d, max_step, dfx, dfy)
+        b.record(stream)
+        torch.cuda.synchronize()
+        if r:
+            times.append(a.elapsed_time(b) / 1e3)
+    adj_s = float(np.median(times))
+    # dJ/dV through the unit fields against a central difference of the middle tube
+    grad_v = np.array([float((dfx * torch.from_numpy(fxk[k]).cuda()).sum() + (dfy * torch.from_numpy(fyk[k]).cuda()).sum())
+                       for k in range(volts.size)])  # fmt: skip
+    step = 1e-4 * np.abs(volts).max()
+    fd = []
+    for sign in (1.0, -1.0):
+        vv = volts.copy()
+        vv[1] += sign * step
+        fd.append(problem.objective(torch.from_numpy(np.tensordot(vv, fxk, 1) + fx0).cuda(),
+                                    torch.from_numpy(np.tensordot(vv, fyk, 1) + fy0).cuda()))  # fmt: skip
+    fd_v = (fd[0] - fd[1]) / (2 * step)
+    return {
+        "workload": f"einzel lens 801x201, {len(p)} H+, RK4; J = rms_angle + emittance at 75 mm; checkpoints every "
+        f"{interval} steps; gradient w.r.t. fx (802x201), fy (801x202) and the 3 electrode voltages",
+        "objective": value,
+        "forward_ms": fwd["launch_s"] * 1e3,
+        "adjoint_ms": adj_s * 1e3,
+        "ratio": adj_s / fwd["launch_s"],
+        "seeded_trajectories": int(seeded.size),
+        "max_seeded_step": max_step,
+        "dJ_dV": grad_v.tolist(),
+        "fd_dJ_dV_middle": fd_v,
+        "fd_rel_err_middle": abs(grad_v[1] - fd_v) / abs(fd_v),
+        "note": "ratio = adjoint kernel (checkpointed forward, segment recompute, transposed steps, gradient "
+        "scatter) / one forward trace launch, HIP events; FD relative step 1e-4",
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -408,6 +482,8 @@ def main() -> int:
         log(f"batched: {opts.batch_designs} einzel candidates")
         processes = min(16, len(os.sched_getaffinity(0)))
         extra["batched"] = batched_leg(opts.batch_designs, 0, processes, 3)
+        log("adjoint: einzel lens, 2e5 trajectories")
+        extra["adjoint"] = adjoint_leg(200_000, 3)
         line["extra"] = extra
     if not opts.no_cpu:
         log("cpu baseline: oracle tracer on the host cores")
