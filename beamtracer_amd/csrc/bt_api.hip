@@ -112,6 +112,8 @@ int check_trace(const bt_trace2d_args* a) {
         return fail(BT_E_ARG, "patch tables are NULL");
     if (a->n_planes > 0 && a->planes == nullptr) return fail(BT_E_ARG, "planes is NULL");
     if (a->graded != 0 && a->n_x_nodes < 2) return fail(BT_E_ARG, "graded axis needs 2+ nodes");
+    if (a->n_fxs < 1 || a->n_fys < 1 || a->n_fxs >= ((i64)1 << 28) || a->n_fys >= ((i64)1 << 28))
+        return fail(BT_E_ARG, "fxs and fys must hold 1 to 2^28 - 1 values");
     i64 nodes = 0;
     for (i64 q = 0; q < a->n_patches; ++q) {
         const int64_t* pi = a->patch_int + q * 6;
@@ -141,6 +143,7 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     p.qm = a->qm; p.dq = a->dq;
     p.steps = (i64*)a->steps; p.status = (i64*)a->status; p.electrode = (i64*)a->electrode;
     p.fxs = a->fxs; p.fys = a->fys;
+    p.n_fxs = a->n_fxs; p.n_fys = a->n_fys;
     p.n_patches = (int)a->n_patches;
     p.origin_x = a->origin_x; p.origin_y = a->origin_y; p.h = a->h;
     p.upper_x = a->upper_x; p.upper_y = a->upper_y;
@@ -263,6 +266,7 @@ static_assert(sizeof(bt_batch_design) == sizeof(bt::BatchDesign), "bt_batch_desi
 int check_designs(const bt_batch_design* designs, i64 n_designs, i64 n_fxs, i64 n_fys, i64 n_cells,
                   i64 n_subcells, i64* total_out, i64* most_out) {
     if (n_designs < 1 || designs == nullptr) return fail(BT_E_ARG, "no designs");
+    if (n_fxs >= ((i64)1 << 28) || n_fys >= ((i64)1 << 28)) return fail(BT_E_ARG, "field arenas of 2^28 values or more");
     if (n_designs > 65535) return fail(BT_E_ARG, "at most 65535 designs");
     i64 total = 0, most = 0;
     for (i64 d = 0; d < n_designs; ++d) {
@@ -321,6 +325,7 @@ bt::TraceParams batch_params(const BatchPlan* plan, const bt_batch_args* a) {
     p.qm = a->qm; p.dq = a->dq;
     p.steps = (i64*)a->steps; p.status = (i64*)a->status; p.electrode = (i64*)a->electrode;
     p.fxs = a->fxs; p.fys = a->fys;
+    p.n_fxs = plan->n_fxs; p.n_fys = plan->n_fys;
     p.n_patches = 1;
     p.step_fraction = a->step_fraction; p.max_steps = a->max_steps;
     p.cells = (const i64*)a->cells; p.subcells = (const i64*)a->subcells; p.refine = a->refine;

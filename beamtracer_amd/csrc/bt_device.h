@@ -68,6 +68,38 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
     return low + u * (high - low);
 }
 
+// ---- buffer gathers -------------------------------------------------------------------
+// A field array read through a buffer resource: 32-bit byte offsets (no 64-bit address
+// arithmetic per load) and both values of a row pair f[c], f[c + 1] in one 16-byte load.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+// The raw buffer of `count` doubles at `base` (count < 2^28; gfx9 resource word 3).
+__device__ __forceinline__ Rsrc field_rsrc(const double* base, i64 count) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, (int)(count * 8), 0x00020000);
+}
+
+struct Pair {
+    double a, b;
+};
+
+__device__ __forceinline__ Pair load_pair(Rsrc r, unsigned elem) {
+    return __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 8u, 0, 0));
+}
+
+// clamped_bilinear of the grid that starts at element `base` of buffer r.
+__device__ __forceinline__ double clamped_bilinear_buf(Rsrc r, unsigned base, i64 cols, double s, double t,
+                                                       int imax, int jmax) {
+    const int i = clamped_floor(s, 0, imax);
+    const int j = clamped_floor(t, 0, jmax);
+    const double u = s - (double)i;
+    const double w = t - (double)j;
+    const unsigned c = base + (unsigned)i * (unsigned)cols + (unsigned)j;
+    const Pair lo = load_pair(r, c), hi = load_pair(r, c + (unsigned)cols);
+    const double low = lo.a + w * (lo.b - lo.a);
+    const double high = hi.a + w * (hi.b - hi.a);
+    return low + u * (high - low);
+}
+
 // ---- fixed point ---------------------------------------------------------------------
 // A deposit is rounded once to an integer multiple q of 2^e (|q| < 2^94).  A node holds two
 // u64 words: w0 sums the low 32 bits of each q (as an unsigned number below 2^32) and w1

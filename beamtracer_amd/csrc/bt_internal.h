@@ -29,6 +29,8 @@ struct TraceParams {
     const double* qm; const double* dq;
     i64* steps; i64* status; i64* electrode;
     const double* fxs; const double* fys;
+    i64 n_fxs, n_fys;      // doubles in fxs, fys (read through 32-bit buffer offsets)
+    i64 fx_base, fy_base;  // added to every patch offset (a batched lane's design arenas)
     const i64* patch_int; const double* patch_real; int n_patches;
     double origin_x, origin_y, h, upper_x, upper_y;
     double step_fraction; i64 max_steps;

@@ -151,13 +151,18 @@ __device__ __forceinline__ double lorentz(double u0, double u1, double u2, doubl
     return lorentz_factor(u0, u1, u2, inv_c2);
 }
 
+// The two edge-field buffers of a launch (uniform).
+struct Fields {
+    Rsrc fx, fy;
+};
+
 // E_x, E_y of the finest patch at the Cartesian point q, with the radial direction and
 // the mesh coordinates (pyccel.py:196-269).
 // With `known` (one patch whose origin and spacing are the mesh's), qy, sx and sy are the
 // values the end test of the last step computed for this same point: the same operations on
 // the same operands, so the same bits.  Otherwise they are computed and returned.
 template <int GEOM, bool GENERAL>
-__device__ __forceinline__ void edge_field(const TraceParams& p, double q0, double q1, double q2,
+__device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F, double q0, double q1, double q2,
                                            double& ex, double& ey, double& cy, double& cz,
                                            double& lq, double& qy, bool known, double& sx, double& sy) {
     cy = 0.0; cz = 0.0;
@@ -177,8 +182,9 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, double q0, doub
         sx = (lq - v.ox) / v.h;
         sy = (qy - v.oy) / v.h;
     }
-    ex = clamped_bilinear(p.fxs + v.ofx, v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
-    ey = clamped_bilinear(p.fys + v.ofy, v.ny + 1, sx, sy + 0.5, (int)v.nx - 2, (int)v.ny - 1);
+    ex = clamped_bilinear_buf(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
+    ey = clamped_bilinear_buf(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
+                              (int)v.ny - 1);
 }
 
 // ---- the deposit window ---------------------------------------------------------------
@@ -196,15 +202,16 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, double q0, doub
 
 constexpr int kWinCols = 16;
 constexpr int kWinRows = 64;
-constexpr int kRoundSteps = 1;        // steps between window moves
-constexpr int kFlushRounds = 32;      // every slot is emptied at least this often
+constexpr int kRoundSteps = 2;        // steps between window moves
+constexpr int kFlushRounds = 16;      // every slot is emptied at least this often
 constexpr int kColumnBits = 20;       // a placement key is (design << 20) | column
+static_assert(kFlushRounds * kRoundSteps <= 32, "window slots must stay below 2^62");
 
 // A slot sums a node's deposits in one signed 64-bit word.  What enters the window are the
 // lanes' cached sums (each below 2^46 units in magnitude); a lane adds at most two of them to
 // a node per step (one when its sum would overflow, one when it leaves the node), a round is
 // kRoundSteps steps of at most 1024 lanes, and every slot is flushed at least every
-// kFlushRounds rounds: |sum| < 32 * 1 * 1024 * 2 * 2^46 = 2^62.
+// kFlushRounds rounds: |sum| < kFlushRounds * kRoundSteps * 1024 * 2 * 2^46 = 2^62.
 struct DepositWindow {
     i64 tag[2][kWinCols];             // accumulator index of the slot's column at row 0, -1: none
     int base[2][kWinCols];            // its first row
@@ -402,7 +409,7 @@ __device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __r
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN>
-__device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositWindow& win, int t) {
+__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, DepositWindow& win, int t) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -450,7 +457,7 @@ __device__ __forceinline__ int step(const TraceParams& p, LaneState& s, DepositW
             double ex, ey, cy, cz, lq, qy, psx, psy;
             const bool known = !GENERAL && p.reuse && stage == 0 && s.point != 0;
             if (known) { qy = s.pt_y; psx = s.pt_sx; psy = s.pt_sy; }
-            edge_field<GEOM, GENERAL>(p, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy);
+            edge_field<GEOM, GENERAL>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy);
             if (INTEG != 0 && stage == 1) { end_y = qy; end_sx = psx; end_sy = psy; }
             if (GEOM != 0) {
                 g0 = qmk * ex;
@@ -681,7 +688,7 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
     lp.upper_x = b.upper_x; lp.upper_y = b.upper_y;
     lp.box = b.origin_x; lp.boy = b.origin_y; lp.bh = b.h;
     lp.bofx = 0; lp.bofy = 0; lp.bofn = b.node_offset;
-    lp.fxs = p.fxs + b.fx_offset; lp.fys = p.fys + b.fy_offset;
+    lp.fx_base = b.fx_offset; lp.fy_base = b.fy_offset;
     lp.cells = p.cells + b.cells_offset; lp.cells_rows = b.cells_rows; lp.cells_cols = b.cells_cols;
     lp.subcells = p.subcells + b.subcells_offset; lp.n_subcells = b.n_subcells;
     lp.planes = p.planes + (i64)d * p.n_planes;
@@ -701,6 +708,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     TraceParams lp = p;
     const TraceParams& q = BATCHED ? lp : p;  // the lane's mesh
+    const Fields F = {field_rsrc(p.fxs, p.n_fxs), field_rsrc(p.fys, p.n_fys)};
     LaneState s;
     s.k = -1;
     s.cci = s.ccj = kNoCell;
@@ -758,7 +766,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
             if (s.k >= 0) {
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, s, win, t);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, F, s, win, t);
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;
