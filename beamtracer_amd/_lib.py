@@ -28,6 +28,7 @@ HIPCC_FLAGS = [
 ]  # fmt: skip
 
 BT_OK, BT_E_ARG, BT_E_NOMEM, BT_E_LAUNCH, BT_E_DEVICE, BT_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+BT_REFILL_BLOCK = 1
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32
@@ -161,6 +162,7 @@ SIGNATURES = {
         [_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _i64, _ptr, _i64, _i64, _i32, _ptr, _i64, _ptr],
     ),
     "bt_adjoint_args_size": (_i64, []),
+    "bt_division_selftest": (ctypes.c_int, [_ptr, _ptr, _i64, _ptr, _ptr]),
     "bt_trace_adjoint_device": (ctypes.c_int, [ctypes.POINTER(AdjointArgs), _ptr]),
     "bt_batch_args_size": (_i64, []),
     "bt_batch_design_size": (_i64, []),

@@ -162,8 +162,11 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     p.nx = base[0]; p.ny = base[1];
     p.bofx = base[2]; p.bofy = base[3]; p.bofn = base[5];
     p.box = basr[0]; p.boy = basr[1]; p.bh = basr[2];
+    p.inv_bh = 1.0 / basr[2];
+    p.inv_h = 1.0 / a->h;
     p.deposit_exponent = a->deposit_exponent;
     p.reuse = a->n_patches == 1 && basr[0] == a->origin_x && basr[1] == a->origin_y && basr[2] == a->h;
+    p.refill_block = (a->flags & BT_REFILL_BLOCK) != 0;
     return p;
 }
 
@@ -560,6 +563,15 @@ int bt_deposit_trajectories_2d(const double* px, const double* py, const double*
     return BT_OK;
 }
 
+
+int bt_division_selftest(const double* a, const double* b, int64_t n, uint64_t* count, void* stream) {
+    if (n < 0 || (n > 0 && (a == nullptr || b == nullptr)) || count == nullptr) return fail(BT_E_ARG, "bad arguments");
+    if (n == 0) return BT_OK;
+    if (bt::launch_division_selftest(a, b, n, reinterpret_cast<unsigned long long*>(count),
+                                     static_cast<hipStream_t>(stream)))
+        return fail(BT_E_LAUNCH, std::string("selftest launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
 
 int64_t bt_adjoint_args_size(void) { return (int64_t)sizeof(bt_adjoint_args); }
 

@@ -68,6 +68,21 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
     return low + u * (high - low);
 }
 
+// a / b rounded to nearest, from y = RN(1/b) computed once: two corrections
+// q <- q + (a - b q) y with the residual from a fused multiply-add (P. Markstein, IBM J.
+// Res. Dev. 34 (1990) 111-119; J.-M. Muller et al., Handbook of Floating-Point Arithmetic).
+// The second correction starts within one ulp of a/b, so its residual is exact and its
+// result is the correctly rounded quotient: the bits of IEEE a / b for finite operands
+// whose quotient neither overflows nor underflows (mesh coordinates and step sizes; a
+// zero quotient may differ only in its sign).  bt_division_selftest checks it on the GPU.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+    double q = a * y;
+    double r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, y, q);
+    r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
+
 // ---- buffer gathers -------------------------------------------------------------------
 // A field array read through a buffer resource: 32-bit byte offsets (no 64-bit address
 // arithmetic per load) and both values of a row pair f[c], f[c + 1] in one 16-byte load.

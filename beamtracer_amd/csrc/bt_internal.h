@@ -41,12 +41,14 @@ struct TraceParams {
     const i64* path_slot; double* paths; i64 n_paths; i64 path_capacity; i64* path_count;
     int graded; const double* x_nodes; i64 n_x_nodes;
     int reuse;   // one patch with the mesh's origin and spacing: end-test values carry over
+    int refill_block;  // refill a workgroup's lanes together (BT_REFILL_BLOCK)
     double inv_c2;
     const double* b0; const double* b1; const double* b2;
     // base (last) patch = the mesh; filled on the host
     i64 nx, ny;
     i64 bofx, bofy, bofn;
     double box, boy, bh;
+    double inv_bh, inv_h;  // RN(1 / bh), RN(1 / h): divisions by the spacings via div_rn
     // deposit accumulator and work queue (workspace)
     u64* acc; int deposit_exponent; u64* queue;   // node words, fixed-point exponent, work queue
     double* charge;
@@ -88,6 +90,9 @@ struct AdjointParams {
 
 int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom);
 int adjoint_occupancy(int integ, int geom, int block);
+
+int launch_division_selftest(const double* a, const double* b, i64 n, unsigned long long* count,
+                             hipStream_t stream);
 
 int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
                   i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,

@@ -169,7 +169,11 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
     if (!known) qy = q1;
     if (GEOM == 1) {
         const double r = known ? qy : hypot(q1, q2);
-        if (r > 0.0) { cy = q1 / r; cz = q2 / r; }
+        if (r > 0.0) {
+            const double ir = 1.0 / r;
+            cy = div_rn(q1, r, ir);
+            cz = div_rn(q2, r, ir);
+        }
         qy = r;
     } else if (GEOM == 2) {
         cy = 1.0;
@@ -179,8 +183,13 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
     lq = locate_x<GENERAL>(p, q0);
     const PatchView v = patch_view<GENERAL>(p, patch_of<GENERAL>(p, lq, qy));
     if (!known) {
-        sx = (lq - v.ox) / v.h;
-        sy = (qy - v.oy) / v.h;
+        if (GENERAL) {
+            sx = (lq - v.ox) / v.h;
+            sy = (qy - v.oy) / v.h;
+        } else {
+            sx = div_rn(lq - v.ox, v.h, p.inv_bh);
+            sy = div_rn(qy - v.oy, v.h, p.inv_bh);
+        }
     }
     ex = clamped_bilinear_buf(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
     ey = clamped_bilinear_buf(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
@@ -492,7 +501,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
             if (dt < 0.0) dt = hloc;
             dt = p.step_fraction * dt;
             hdt = 0.5 * dt;
-            dt6 = dt / 6.0;
+            dt6 = div_rn(dt, 6.0, 1.0 / 6.0);
             if (INTEG != 0) {
                 if (HAS_B) {
                     kx0 = v0; kx1 = v1; kx2 = v2;
@@ -560,16 +569,19 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     m0 = locate_x<GENERAL>(p, m0);
     {
         const PatchView pv = patch_view<GENERAL>(p, patch_of<GENERAL>(p, m0, m1));
-        const double fi = floor((m0 - pv.ox) / pv.h);
-        const double fj = floor((m1 - pv.oy) / pv.h);
-        double u = (m0 - (pv.ox + pv.h * fi)) / pv.h;
+        // (x / h of the base patch through div_rn: the same bits as the division)
+        const double fi = floor(GENERAL ? (m0 - pv.ox) / pv.h : div_rn(m0 - pv.ox, pv.h, p.inv_bh));
+        const double fj = floor(GENERAL ? (m1 - pv.oy) / pv.h : div_rn(m1 - pv.oy, pv.h, p.inv_bh));
+        const double du = m0 - (pv.ox + pv.h * fi);
+        double u = GENERAL ? du / pv.h : div_rn(du, pv.h, p.inv_bh);
         double w;
         if (GEOM == 1) {
             const double low = pv.oy + pv.h * fj;
             const double high = low + pv.h;
             w = (m1 * m1 - low * low) / (high * high - low * low);
         } else {
-            w = (m1 - (pv.oy + pv.h * fj)) / pv.h;
+            const double dw = m1 - (pv.oy + pv.h * fj);
+            w = GENERAL ? dw / pv.h : div_rn(dw, pv.h, p.inv_bh);
         }
         i64 i, j;
         // !(f >= 0) also sends a NaN midpoint to the first node (its deposit is dropped)
@@ -639,8 +651,10 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     } else if (py > p.upper_y || (GEOM == 0 && py < p.origin_y)) {
         st = 2;
     } else {
-        const double ci = end_known ? end_sx : (px - p.origin_x) / p.h;
-        const double cj = end_known ? end_sy : (py - p.origin_y) / p.h;
+        const double ci = end_known ? end_sx
+                                    : (GENERAL ? (px - p.origin_x) / p.h : div_rn(px - p.origin_x, p.h, p.inv_h));
+        const double cj = end_known ? end_sy
+                                    : (GENERAL ? (py - p.origin_y) / p.h : div_rn(py - p.origin_y, p.h, p.inv_h));
         s.pt_y = py; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
         // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341
         i64 i = (i64)fmax(ci, 0.0);
@@ -687,6 +701,7 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
     lp.origin_x = b.origin_x; lp.origin_y = b.origin_y; lp.h = b.h;
     lp.upper_x = b.upper_x; lp.upper_y = b.upper_y;
     lp.box = b.origin_x; lp.boy = b.origin_y; lp.bh = b.h;
+    lp.inv_bh = 1.0 / b.h; lp.inv_h = lp.inv_bh;
     lp.bofx = 0; lp.bofy = 0; lp.bofn = b.node_offset;
     lp.fx_base = b.fx_offset; lp.fy_base = b.fy_offset;
     lp.cells = p.cells + b.cells_offset; lp.cells_rows = b.cells_rows; lp.cells_cols = b.cells_cols;
@@ -714,10 +729,11 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     s.cci = s.ccj = kNoCell;
     s.cq[0] = s.cq[1] = s.cq[2] = s.cq[3] = 0;
     bool exhausted = false;
-    int t = 0;       // the window table of this round
+    bool refill = true;  // with refill_block: only when every lane of the workgroup is free
+    int t = 0;           // the window table of this round
     int rounds = 0;
     for (;;) {
-        const bool want = s.k < 0 && !exhausted;
+        const bool want = s.k < 0 && !exhausted && refill;
         const u64 need = __ballot(want);
         if (need != 0) {
             const int count = __popcll(need);
@@ -774,7 +790,8 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                 }
             }
         }
-        __syncthreads();
+        const bool busy = __syncthreads_or(s.k >= 0);
+        refill = !p.refill_block || !busy;
         window_maintain<BATCHED>(win, t, p, ++rounds % kFlushRounds == 0);
         t = 1 - t;
     }
@@ -920,6 +937,29 @@ int batched_occupancy(int integ, int geom, bool has_b, int block) {
                                                      block, 0) != hipSuccess)
         return 0;
     return blocks;
+}
+
+// count[0] += the number of k with div_rn(a[k], b[k], RN(1 / b[k])) != a[k] / b[k] (bits)
+__global__ void division_selftest_kernel(const double* __restrict__ a, const double* __restrict__ b, i64 n,
+                                         unsigned long long* __restrict__ count) {
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    unsigned long long bad = 0;
+    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const double y = 1.0 / b[k];
+        const double fast = div_rn(a[k], b[k], y);
+        const double ieee = a[k] / b[k];
+        if (__double_as_longlong(fast) != __double_as_longlong(ieee)) ++bad;
+    }
+    if (bad) atomicAdd(count, bad);
+}
+
+int launch_division_selftest(const double* a, const double* b, i64 n, unsigned long long* count,
+                             hipStream_t stream) {
+    int grid = (int)((n + 255) / 256);
+    if (grid > 8192) grid = 8192;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(division_selftest_kernel, dim3(grid), dim3(256), 0, stream, a, b, n, count);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int trace_occupancy(int integ, int geom, bool has_b, bool general, int block) {

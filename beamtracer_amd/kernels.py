@@ -217,6 +217,7 @@ def trace_trajectories_2d(
     max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
     path_count, num_threads, charge, graded, x_nodes, inv_c2,
     *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
+    refill_block=False,
 ) -> None:  # fmt: skip
     """Trace the ``active`` trajectories until they end, or reach a surface.
 
@@ -226,7 +227,8 @@ def trace_trajectories_2d(
     the base mesh, ``(B_x, B_r, B_theta)`` cylindrical or ``(B_x, B_y, B_z)`` planar
     (None entries are zero). ``block_size`` and ``max_blocks`` set the persistent launch
     (0: defaults); ``exponent`` the fixed-point LSB ``2**exponent`` (default
-    :func:`trace_exponent`).
+    :func:`trace_exponent`); ``refill_block`` refills a workgroup's lanes only when all are
+    free (``BT_REFILL_BLOCK``: coherent lanes, whose deposits add up in LDS).
     """
     L = _lib.lib()
     device = _is_torch(x)
@@ -289,7 +291,7 @@ def trace_trajectories_2d(
         int(num_threads), _ptr(charge), charge.shape[0], charge.shape[1],
         int(graded), _ptr(x_nodes), x_nodes.shape[0], float(inv_c2),
         int(magnetic is not None), _ptr(bx), _ptr(by), _ptr(bz),
-        int(exponent), int(block_size), int(max_blocks), 0,
+        int(exponent), int(block_size), int(max_blocks), _lib.BT_REFILL_BLOCK if refill_block else 0,
         _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
     )  # fmt: skip
     if device:

@@ -41,6 +41,12 @@ extern "C" {
 #define BT_E_DEVICE (-4)     /* no usable HIP device                                   */
 #define BT_E_UNSUPPORTED (-5)
 
+/* flags of a trace: refill a workgroup's lanes only when all of them are free, so that
+ * the lanes of a workgroup start their trajectories together and stay in the same cells
+ * (their deposits then add up in LDS; the lanes of trajectories that end early wait).
+ * Results do not depend on it. */
+#define BT_REFILL_BLOCK 1
+
 /* Every argument of trace_trajectories_2d (trace_trajectories_2d_pyccel.py:42-78) as a
  * pointer with its extents, then the node magnetic field that the reference's compiled
  * kernel lacks (its NumPy path takes B, solver/tracer.py:519-529, 605-650), then launch
@@ -77,7 +83,7 @@ typedef struct bt_trace2d_args {
     int32_t deposit_exponent;   /* fixed-point LSB 2^e of the deposit; see bt_deposit_exponent */
     int32_t block_size;         /* threads per block (64..256, a multiple of 64); 0 = default 256 */
     int32_t max_blocks;         /* persistent grid size; 0 = fill the device            */
-    int32_t flags;              /* reserved, 0                                          */
+    int32_t flags;              /* BT_REFILL_BLOCK or 0                                 */
     void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned */
     int64_t workspace_bytes;
 } bt_trace2d_args;
@@ -215,6 +221,11 @@ typedef struct bt_adjoint_args {
     double* dfx; double* dfy;                     /* += dJ/dfx, dJ/dfy                      */
     double* dx; double* dv;                       /* += dJ/dx, dJ/dv (3, n), or NULL        */
 } bt_adjoint_args;
+
+/* Self-test of the kernels' division by a known reciprocal (no counterpart in the
+ * reference): count[0] += the number of k (device arrays of n doubles) where it differs
+ * in any bit from IEEE a[k] / b[k].  0 is the kernels' premise. */
+int bt_division_selftest(const double* a, const double* b, int64_t n, uint64_t* count, void* stream);
 
 int64_t bt_adjoint_args_size(void);
 int bt_trace_adjoint_device(const bt_adjoint_args* args, void* stream);

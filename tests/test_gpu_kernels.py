@@ -173,3 +173,31 @@ def test_trace_without_active_and_at_max_steps(hip):
     assert (done[6] == 4).all()
     for i in cases.OUTPUTS:
         assert np.array_equal(done[i], expected[i])
+
+
+def test_division_by_a_known_reciprocal_is_the_ieee_division():
+    """The kernels divide mesh coordinates by the spacings (and the stage direction by r)
+    through div_rn with RN(1/b): every bit must be IEEE's (nonzero finite quotients)."""
+    import ctypes
+
+    import torch
+
+    from beamtracer_amd import _lib
+
+    rng = np.random.default_rng(17)
+    n = 1 << 22
+    spacings = [1e-4, 5.859375e-5, 2.5e-4, 6.0] + [80e-3 / (k - 1) for k in range(201, 1026, 7)]
+    b = np.concatenate([rng.choice(spacings, n // 2), 10.0 ** rng.uniform(-7, -1.5, n // 2)])
+    a = rng.uniform(-0.08, 0.08, n)
+    # numerators that land next to cell boundaries: k h (1 + tiny)
+    near = n // 4
+    k = rng.integers(-1000, 1000, near).astype(float)
+    a[:near] = k * b[:near] * (1.0 + rng.normal(size=near) * 1e-15)
+    a[a == 0.0] = 1e-3
+    da = torch.from_numpy(a).cuda()
+    db = torch.from_numpy(b).cuda()
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    rc = _lib.lib().bt_division_selftest(da.data_ptr(), db.data_ptr(), n, count.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert int(count.item()) == 0
