@@ -77,6 +77,7 @@ class TraceArgs(ctypes.Structure):
         ("has_b", _i64), ("b0", _ptr), ("b1", _ptr), ("b2", _ptr),
         ("deposit_exponent", _i32), ("block_size", _i32), ("max_blocks", _i32), ("flags", _i32),
         ("workspace", _ptr), ("workspace_bytes", _i64),
+        ("checkpoints", _ptr), ("n_checkpoints", _i64), ("checkpoint_interval", _i64),
     ]  # fmt: skip
 
 
@@ -130,6 +131,7 @@ class AdjointArgs(ctypes.Structure):
         ("hits", _ptr), ("capacity", _i64), ("counts", _ptr),
         ("seeds", _ptr),
         ("max_step", _i64),
+        ("checkpoints", _ptr), ("n_checkpoints", _i64),
         ("interval", _i32), ("block_size", _i32),
         ("dfx", _ptr), ("dfy", _ptr),
         ("dx", _ptr), ("dv", _ptr),

@@ -107,6 +107,17 @@ class TraceAdjoint:
         self.interval = int(interval)
         self.cylindrical = mesh.cylindrical
         self._records = None
+        # the forward trace keeps its states every `interval` steps on the GPU (device form)
+        # for twice the steps of a straight path through the mesh at the step fraction;
+        # a trajectory that runs longer is stepped again by the adjoint itself
+        import torch
+
+        length = (mesh.upper[0] - mesh.origin[0]) + 2.0 * (mesh.upper[1] - mesh.origin[1])
+        steps = 2.0 * length / (tracer.step_fraction * mesh.h)
+        count = int(min(steps, tracer.max_steps) // self.interval) + 2
+        tracer.device = True
+        tracer.checkpoints = (torch.zeros((count, 6, len(particles)), dtype=torch.float64, device="cuda"),
+                              self.interval)  # fmt: skip
 
     def objective(self, fx, fy) -> float:
         """J for the fields (torch tensors or arrays); keeps the records for :meth:`gradient`."""
@@ -156,8 +167,9 @@ class TraceAdjoint:
         return dfx, dfy
 
     def launch(self, active, x0, v0, qm, fx, fy, planes, hits, counts, seeds, max_step, dfx, dfy, dx=None,
-               dv=None, stream=None) -> None:  # fmt: skip
-        """One adjoint kernel call on device tensors (asynchronous on ``stream``)."""
+               dv=None, stream=None, checkpoints=True) -> None:  # fmt: skip
+        """One adjoint kernel call on device tensors (asynchronous on ``stream``); with
+        ``checkpoints`` it starts from the forward trace's stored states."""
         import torch
 
         t = self.tracer
@@ -178,6 +190,9 @@ class TraceAdjoint:
         a.hits, a.capacity, a.counts = hits.data_ptr(), hits.shape[2], counts.data_ptr()
         a.seeds = seeds.data_ptr()
         a.max_step = int(max_step)
+        if checkpoints and t.checkpoints is not None:
+            a.checkpoints = t.checkpoints[0].data_ptr()
+            a.n_checkpoints = t.checkpoints[0].shape[0]
         a.interval = self.interval
         a.block_size = 0
         a.dfx, a.dfy = dfx.data_ptr(), dfy.data_ptr()

@@ -39,22 +39,31 @@ __device__ __forceinline__ double vdot(Vec3 x, Vec3 y) { return x.a * y.a + x.b 
 
 // q/m E at point q: the trace kernel's edge_field for one patch whose origin and spacing are
 // the mesh's, then the radial projection (bt_trace.hip, same operations).
+// The two edge-field buffers (uniform).
+struct AdjFields {
+    Rsrc fx, fy;
+};
+
 template <int GEOM>
-__device__ __forceinline__ Vec3 accel(const AdjointParams& p, double qm, Vec3 q) {
+__device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F, double qm, Vec3 q) {
     double cy = 0.0, cz = 0.0, qy = q.b;
     if (GEOM == 1) {
         const double r = hypot(q.b, q.c);
-        if (r > 0.0) { cy = q.b / r; cz = q.c / r; }
+        if (r > 0.0) {
+            const double ir = 1.0 / r;
+            cy = div_rn(q.b, r, ir);
+            cz = div_rn(q.c, r, ir);
+        }
         qy = r;
     } else if (GEOM == 2) {
         cy = 1.0;
         if (q.b < p.origin_y) cy = -1.0;
         qy = p.origin_y + fabs(q.b - p.origin_y);
     }
-    const double sx = (q.a - p.origin_x) / p.h;
-    const double sy = (qy - p.origin_y) / p.h;
-    const double ex = clamped_bilinear(p.fxs, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
-    const double ey = clamped_bilinear(p.fys, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    const double sx = div_rn(q.a - p.origin_x, p.h, p.inv_h);
+    const double sy = div_rn(qy - p.origin_y, p.h, p.inv_h);
+    const double ex = clamped_bilinear_buf(F.fx, 0u, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const double ey = clamped_bilinear_buf(F.fy, 0u, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
     if (GEOM != 0) return {qm * ex, qm * (ey * cy), qm * (ey * cz)};
     return {qm * ex, qm * ey, qm * 0.0};
 }
@@ -68,8 +77,7 @@ struct Gathered {
     double u, w;
 };
 
-__device__ __forceinline__ Gathered gather_d(const double* __restrict__ f, i64 cols, double s, double t,
-                                             int imax, int jmax) {
+__device__ __forceinline__ Gathered gather_d(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
     const int i = clamped_floor(s, 0, imax);
     const int j = clamped_floor(t, 0, jmax);
     Gathered g;
@@ -78,8 +86,8 @@ __device__ __forceinline__ Gathered gather_d(const double* __restrict__ f, i64 c
     g.u = s - (double)i;
     g.w = t - (double)j;
     g.c = (unsigned)i * (unsigned)cols + (unsigned)j;
-    const unsigned d = g.c + (unsigned)cols;
-    const double f00 = f[g.c], f01 = f[g.c + 1], f10 = f[d], f11 = f[d + 1];
+    const Pair lo = load_pair(f, g.c), hi = load_pair(f, g.c + (unsigned)cols);
+    const double f00 = lo.a, f01 = lo.b, f10 = hi.a, f11 = hi.b;
     const double low = f00 + g.w * (f01 - f00);
     const double high = f10 + g.w * (f11 - f10);
     g.value = low + g.u * (high - low);
@@ -181,22 +189,26 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, GradWindow
 // The transposed field evaluation: given lambda_g for g = q/m E(q), scatter into dJ/dfx,
 // dJ/dfy and return lambda_q.
 template <int GEOM>
-__device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const GradWindow& W, double qm, Vec3 q,
-                                              Vec3 lg) {
+__device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
+                                              double qm, Vec3 q, Vec3 lg) {
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
     if (GEOM == 1) {
         r = hypot(q.b, q.c);
-        if (r > 0.0) { cy = q.b / r; cz = q.c / r; }
+        if (r > 0.0) {
+            const double ir = 1.0 / r;
+            cy = div_rn(q.b, r, ir);
+            cz = div_rn(q.c, r, ir);
+        }
         qy = r;
     } else if (GEOM == 2) {
         cy = 1.0;
         if (q.b < p.origin_y) cy = -1.0;
         qy = p.origin_y + fabs(q.b - p.origin_y);
     }
-    const double sx = (q.a - p.origin_x) / p.h;
-    const double sy = (qy - p.origin_y) / p.h;
-    const Gathered gx = gather_d(p.fxs, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
-    const Gathered gy = gather_d(p.fys, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    const double sx = div_rn(q.a - p.origin_x, p.h, p.inv_h);
+    const double sy = div_rn(qy - p.origin_y, p.h, p.inv_h);
+    const Gathered gx = gather_d(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const Gathered gy = gather_d(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
     const double lex = qm * lg.a;
     double ley, lcy = 0.0, lcz = 0.0;
     if (GEOM == 1) {
@@ -246,30 +258,31 @@ __device__ __forceinline__ double step_size(const AdjointParams& p, Vec3 v, Vec3
 
 // One forward step (the trace kernel's arithmetic).
 template <int INTEG, int GEOM>
-__device__ __forceinline__ void forward_step(const AdjointParams& p, double qm, Vec3& x, Vec3& v) {
-    const Vec3 a = accel<GEOM>(p, qm, x);
+__device__ __forceinline__ void forward_step(const AdjointParams& p, const AdjFields& F, double qm, Vec3& x,
+                                             Vec3& v) {
+    const Vec3 a = accel<GEOM>(p, F, qm, x);
     int branch;
     double speed, acc;
     const double dt = step_size(p, v, a, branch, speed, acc);
     const double hdt = 0.5 * dt;
     if (INTEG == 0) {
-        const double dt6 = dt / 6.0;
+        const double dt6 = div_rn(dt, 6.0, 1.0 / 6.0);
         Vec3 sx = v, sv = a;
         Vec3 q = vaxpy(x, hdt, v), k = vaxpy(v, hdt, a);
-        Vec3 g = accel<GEOM>(p, qm, q);
+        Vec3 g = accel<GEOM>(p, F, qm, q);
         sx = vaxpy(sx, 2.0, k); sv = vaxpy(sv, 2.0, g);
         q = vaxpy(x, hdt, k); k = vaxpy(v, hdt, g);
-        g = accel<GEOM>(p, qm, q);
+        g = accel<GEOM>(p, F, qm, q);
         sx = vaxpy(sx, 2.0, k); sv = vaxpy(sv, 2.0, g);
         q = vaxpy(x, dt, k); k = vaxpy(v, dt, g);
-        g = accel<GEOM>(p, qm, q);
+        g = accel<GEOM>(p, F, qm, q);
         sx = vadd(sx, k); sv = vadd(sv, g);
         x = vaxpy(x, dt6, sx);
         v = vaxpy(v, dt6, sv);
     } else {
         const Vec3 k = vaxpy(v, hdt, a);
         const Vec3 q = vaxpy(x, dt, k);
-        const Vec3 g = accel<GEOM>(p, qm, q);
+        const Vec3 g = accel<GEOM>(p, F, qm, q);
         x = q;
         v = vaxpy(k, hdt, g);
     }
@@ -323,9 +336,9 @@ __device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 be
 
 // The transposed step m (start state x, v): lambda of the end (lx, lv) -> of the start.
 template <int INTEG, int GEOM>
-__device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradWindow& W, i64 k, i64 m, bool seeded,
-                                             double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv) {
-    const Vec3 a = accel<GEOM>(p, qm, x);
+__device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFields& F, const GradWindow& W, i64 k,
+                                             i64 m, bool seeded, double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv) {
+    const Vec3 a = accel<GEOM>(p, F, qm, x);
     int branch;
     double speed, acc;
     const double dt = step_size(p, v, a, branch, speed, acc);
@@ -333,13 +346,13 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradW
     Vec3 La = {0.0, 0.0, 0.0}, Lx, Lv;
     double Ldt = 0.0, Lhdt = 0.0;
     if (INTEG == 0) {
-        const double dt6 = dt / 6.0;
+        const double dt6 = div_rn(dt, 6.0, 1.0 / 6.0);
         const Vec3 q1 = vaxpy(x, hdt, v), k1 = vaxpy(v, hdt, a);
-        const Vec3 g1 = accel<GEOM>(p, qm, q1);
+        const Vec3 g1 = accel<GEOM>(p, F, qm, q1);
         const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g1);
-        const Vec3 g2 = accel<GEOM>(p, qm, q2);
+        const Vec3 g2 = accel<GEOM>(p, F, qm, q2);
         const Vec3 q3 = vaxpy(x, dt, k2), k3 = vaxpy(v, dt, g2);
-        const Vec3 g3 = accel<GEOM>(p, qm, q3);
+        const Vec3 g3 = accel<GEOM>(p, F, qm, q3);
         const Vec3 sx = vadd(vaxpy(vaxpy(v, 2.0, k1), 2.0, k2), k3);
         const Vec3 sv = vadd(vaxpy(vaxpy(a, 2.0, g1), 2.0, g2), g3);
         Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
@@ -357,13 +370,13 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradW
         La = Lsv;
         Vec3 Lg1 = vscale(2.0, Lsv), Lg2 = vscale(2.0, Lsv);
         const Vec3 Lg3 = Lsv;
-        const Vec3 Lq3 = accel_adjoint<GEOM>(p, W, qm, q3, Lg3);
+        const Vec3 Lq3 = accel_adjoint<GEOM>(p, F, W, qm, q3, Lg3);
         Lv = vadd(Lv, Lk3); Ldt += vdot(Lk3, g2); Lg2 = vaxpy(Lg2, dt, Lk3);
         Lx = vadd(Lx, Lq3); Ldt += vdot(Lq3, k2); Lk2 = vaxpy(Lk2, dt, Lq3);
-        const Vec3 Lq2 = accel_adjoint<GEOM>(p, W, qm, q2, Lg2);
+        const Vec3 Lq2 = accel_adjoint<GEOM>(p, F, W, qm, q2, Lg2);
         Lv = vadd(Lv, Lk2); Lhdt += vdot(Lk2, g1); Lg1 = vaxpy(Lg1, hdt, Lk2);
         Lx = vadd(Lx, Lq2); Lhdt += vdot(Lq2, k1); Lk1 = vaxpy(Lk1, hdt, Lq2);
-        const Vec3 Lq1 = accel_adjoint<GEOM>(p, W, qm, q1, Lg1);
+        const Vec3 Lq1 = accel_adjoint<GEOM>(p, F, W, qm, q1, Lg1);
         Lv = vadd(Lv, Lk1); Lhdt += vdot(Lk1, a); La = vaxpy(La, hdt, Lk1);
         Lx = vadd(Lx, Lq1); Lhdt += vdot(Lq1, v); Lv = vaxpy(Lv, hdt, Lq1);
         Ldt += Ldt6 / 6.0;
@@ -372,7 +385,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradW
     } else {
         const Vec3 kk = vaxpy(v, hdt, a);
         const Vec3 q = vaxpy(x, dt, kk);
-        const Vec3 g1 = accel<GEOM>(p, qm, q);
+        const Vec3 g1 = accel<GEOM>(p, F, qm, q);
         Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
         if (seeded) {
             const Vec3 w = vaxpy(kk, hdt, g1);
@@ -381,7 +394,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradW
         Vec3 Lk = lv;
         Lhdt += vdot(lv, g1);
         const Vec3 Lg1 = vscale(hdt, lv);
-        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, W, qm, q, Lg1));
+        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, F, W, qm, q, Lg1));
         Lx = Lq;
         Ldt += vdot(Lq, kk);
         Lk = vaxpy(Lk, dt, Lq);
@@ -397,7 +410,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const GradW
     } else if (branch == 2) {
         La = vaxpy(La, -0.5 * Ldt * dt / (acc * acc), a);
     }
-    Lx = vadd(Lx, accel_adjoint<GEOM>(p, W, qm, x, La));
+    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, qm, x, La));
     lx = Lx;
     lv = Lv;
 }
@@ -430,6 +443,7 @@ template <int INTEG, int GEOM>
 __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const AdjFields F = {field_rsrc(p.fxs, (p.nx + 1) * p.ny), field_rsrc(p.fys, p.nx * (p.ny + 1))};
     GradWindow W;
     W.wave = (int)(threadIdx.x >> 6);
     W.lo = -1;
@@ -471,13 +485,15 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
         const i64 N = p.n;
         const double qm = valid ? p.qm[k] : 0.0;
         Vec3 x = {0.0, 0.0, 0.0}, v = {0.0, 0.0, 0.0};
-        if (valid) {
+        // the forward trace's checkpoints when they reach the last seeded step
+        const bool stored = valid && p.fwd_ckpt != nullptr && (last - 1) / K < p.fwd_n_ckpt;
+        if (valid && !stored) {
             x = {p.x[k], p.x[N + k], p.x[2 * N + k]};
             v = {p.v[k], p.v[N + k], p.v[2 * N + k]};
             // 1. forward to the last seeded step, a checkpoint every K steps
             for (i64 m = 0; m < last; ++m) {
                 if (m % K == 0) put_state(ck, p.lanes, m / K, gl, x, v);
-                forward_step<INTEG, GEOM>(p, qm, x, v);
+                forward_step<INTEG, GEOM>(p, F, qm, x, v);
             }
         }
         // 2. the segments backward, in step across the wave (the window moves with them)
@@ -489,10 +505,16 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
             const bool has = valid && m0 < last;
             const i64 m1 = has ? (m0 + K < last ? m0 + K : last) : m0;
             if (has) {
-                get_state(ck, p.lanes, c, gl, x, v);
+                if (stored) {
+                    const double* cp = p.fwd_ckpt + (i64)c * 6 * N + k;
+                    x = {cp[0], cp[N], cp[2 * N]};
+                    v = {cp[3 * N], cp[4 * N], cp[5 * N]};
+                } else {
+                    get_state(ck, p.lanes, c, gl, x, v);
+                }
                 for (i64 m = m0; m < m1; ++m) {
                     put_state(seg, p.lanes, m - m0, gl, x, v);
-                    if (m + 1 < m1) forward_step<INTEG, GEOM>(p, qm, x, v);
+                    if (m + 1 < m1) forward_step<INTEG, GEOM>(p, F, qm, x, v);
                 }
             }
             for (i64 r = K - 1; r >= 0; --r) {
@@ -506,7 +528,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
                 window_follow(p, W, act, ci, cj);
                 if (act) {
                     const bool seeded = m + 1 == next;
-                    reverse_step<INTEG, GEOM>(p, W, k, m + 1, seeded, qm, x, v, lx, lv);
+                    reverse_step<INTEG, GEOM>(p, F, W, k, m + 1, seeded, qm, x, v, lx, lv);
                     if (seeded) next = seeded_step(p, k, m + 1);
                 }
             }

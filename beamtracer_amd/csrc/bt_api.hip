@@ -167,6 +167,9 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     p.deposit_exponent = a->deposit_exponent;
     p.reuse = a->n_patches == 1 && basr[0] == a->origin_x && basr[1] == a->origin_y && basr[2] == a->h;
     p.refill_block = (a->flags & BT_REFILL_BLOCK) != 0;
+    if (a->checkpoints != nullptr && a->n_checkpoints > 0 && a->checkpoint_interval > 0) {
+        p.ckpt = a->checkpoints; p.n_ckpt = a->n_checkpoints; p.ckpt_interval = a->checkpoint_interval;
+    }
     return p;
 }
 
@@ -401,6 +404,7 @@ int bt_trace_trajectories_2d_device(const bt_trace2d_args* a, void* stream) {
 int bt_trace_trajectories_2d(const bt_trace2d_args* a) {
     int rc = check_trace(a);
     if (rc) return rc;
+    if (a->checkpoints != nullptr) return fail(BT_E_UNSUPPORTED, "checkpoints need the device form");
     std::lock_guard<std::mutex> lock(g_mutex);
     DeviceState* s = nullptr;
     if ((rc = device_state(&s))) return rc;
@@ -618,9 +622,14 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     p.x = a->x; p.v = a->v; p.qm = a->qm;
     p.fxs = a->fx; p.fys = a->fy; p.nx = a->nx; p.ny = a->ny;
     p.origin_x = a->origin_x; p.origin_y = a->origin_y; p.h = a->h; p.step_fraction = a->step_fraction;
+    p.inv_h = 1.0 / a->h;
     p.planes = a->planes; p.n_planes = (int)a->n_planes;
     p.hits = a->hits; p.capacity = a->capacity; p.counts = (const i64*)a->counts; p.seeds = a->seeds;
     p.interval = a->interval; p.n_ckpt = n_ckpt; p.lanes = lanes;
+    if (a->checkpoints != nullptr && a->n_checkpoints > 0) {
+        p.fwd_ckpt = a->checkpoints;
+        p.fwd_n_ckpt = a->n_checkpoints;
+    }
     p.queue = static_cast<u64*>(s->adjoint.ptr);
     p.scratch = reinterpret_cast<double*>(static_cast<char*>(s->adjoint.ptr) + 256);
     p.dfx = a->dfx; p.dfy = a->dfy; p.dx = a->dx; p.dv = a->dv;

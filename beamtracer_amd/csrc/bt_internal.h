@@ -42,6 +42,7 @@ struct TraceParams {
     int graded; const double* x_nodes; i64 n_x_nodes;
     int reuse;   // one patch with the mesh's origin and spacing: end-test values carry over
     int refill_block;  // refill a workgroup's lanes together (BT_REFILL_BLOCK)
+    double* ckpt; i64 n_ckpt, ckpt_interval;  // checkpoints [c][6][n] of the states, or null
     double inv_c2;
     const double* b0; const double* b1; const double* b2;
     // base (last) patch = the mesh; filled on the host
@@ -80,9 +81,11 @@ struct AdjointParams {
     const double* x; const double* v; const double* qm;
     const double* fxs; const double* fys; i64 nx, ny;
     double origin_x, origin_y, h, step_fraction;
+    double inv_h;  // RN(1 / h)
     const double* planes; int n_planes;
     const double* hits; i64 capacity; const i64* counts; const double* seeds;
     i64 interval, n_ckpt, lanes;
+    const double* fwd_ckpt; i64 fwd_n_ckpt;  // the forward trace's checkpoints [c][6][n], or null
     double* scratch;
     double* dfx; double* dfy; double* dx; double* dv;
     u64* queue;

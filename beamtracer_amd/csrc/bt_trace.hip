@@ -54,6 +54,7 @@ struct LaneState {
     // a cell without that node, when it would reach 2^46, or when the trajectory ends
     int cci, ccj;
     i64 cq[4];
+    i64 ck_next, ck_index;  // the step count of the next checkpoint and its slot
 };
 
 constexpr int kNoCell = -(1 << 30);
@@ -767,6 +768,10 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.point = 0;
                     s.ca0 = s.ca1 = s.ca2 = 0.0;
                     s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
+                    if (!BATCHED && p.ckpt != nullptr) {  // the first multiple of the interval >= ns
+                        s.ck_index = (s.ns + p.ckpt_interval - 1) / p.ckpt_interval;
+                        s.ck_next = s.ck_index * p.ckpt_interval;
+                    }
                     if (s.ns >= p.max_steps) {  // no step: the loop of the reference ends at once
                         store_end(p, s, 4);
                         s.k = -1;
@@ -782,6 +787,15 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
             if (s.k >= 0) {
+                if (!BATCHED && p.ckpt != nullptr && s.ns == s.ck_next) {
+                    if (s.ck_index < p.n_ckpt) {
+                        double* c = p.ckpt + s.ck_index * 6 * p.n + s.k;
+                        c[0] = s.x0; c[p.n] = s.x1; c[2 * p.n] = s.x2;
+                        c[3 * p.n] = s.v0; c[4 * p.n] = s.v1; c[5 * p.n] = s.v2;
+                    }
+                    s.ck_index += 1;
+                    s.ck_next += p.ckpt_interval;
+                }
                 const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, F, s, win, t);
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);

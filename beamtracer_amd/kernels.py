@@ -217,7 +217,7 @@ def trace_trajectories_2d(
     max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
     path_count, num_threads, charge, graded, x_nodes, inv_c2,
     *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
-    refill_block=False,
+    refill_block=False, checkpoints=None, checkpoint_interval=64,
 ) -> None:  # fmt: skip
     """Trace the ``active`` trajectories until they end, or reach a surface.
 
@@ -228,7 +228,9 @@ def trace_trajectories_2d(
     (None entries are zero). ``block_size`` and ``max_blocks`` set the persistent launch
     (0: defaults); ``exponent`` the fixed-point LSB ``2**exponent`` (default
     :func:`trace_exponent`); ``refill_block`` refills a workgroup's lanes only when all are
-    free (``BT_REFILL_BLOCK``: coherent lanes, whose deposits add up in LDS).
+    free (``BT_REFILL_BLOCK``: coherent lanes, whose deposits add up in LDS);
+    ``checkpoints`` (device form: a float64 tensor ``(C, 6, N)``) receives ``(x, v)`` of every
+    trajectory before each step whose count is a multiple of ``checkpoint_interval``.
     """
     L = _lib.lib()
     device = _is_torch(x)
@@ -293,7 +295,19 @@ def trace_trajectories_2d(
         int(magnetic is not None), _ptr(bx), _ptr(by), _ptr(bz),
         int(exponent), int(block_size), int(max_blocks), _lib.BT_REFILL_BLOCK if refill_block else 0,
         _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
+        None, 0, 0,
     )  # fmt: skip
+    if checkpoints is not None:
+        if not device:
+            raise ValueError("checkpoints need the device form (torch tensors)")
+        check(checkpoints, _F64, "checkpoints")
+        if checkpoints.ndim != 3 or tuple(checkpoints.shape[1:]) != (6, n):
+            raise ValueError(f"checkpoints must be (C, 6, {n})")
+        if int(checkpoint_interval) < 1:
+            raise ValueError("checkpoint_interval must be positive")
+        args.checkpoints = _ptr(checkpoints)
+        args.n_checkpoints = checkpoints.shape[0]
+        args.checkpoint_interval = int(checkpoint_interval)
     if device:
         rc = L.bt_trace_trajectories_2d_device(ctypes.byref(args), _stream(stream))
     else:

@@ -132,6 +132,9 @@ class HipBeamTracer:
         self._lookup: tuple[np.ndarray, np.ndarray] | None = None
         self._capacity = (4, 1024)
         self.reduce = None
+        #: device form only: (tensor (C, 6, N), interval) receiving the states every
+        #: `interval` steps (the adjoint's checkpoints), or None
+        self.checkpoints = None
         touch = (mesh.solid >= 0) | mesh.near_solid
         self._electrode_cells = touch[:-1, :-1] | touch[1:, :-1] | touch[:-1, 1:] | touch[1:, 1:]
 
@@ -340,7 +343,9 @@ class HipBeamTracer:
         for i in (10, 11, 24):  # patch tables and planes stay on the host
             dev[i] = call[i]
         mag = None if magnetic is None else tuple(None if c is None else torch.from_numpy(c).cuda() for c in magnetic)
-        kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size)
+        ck, interval = self.checkpoints if self.checkpoints is not None else (None, 64)
+        kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size,
+                                      checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
         torch.cuda.synchronize()
         for i in (1, 2, 5, 6, 7, 25, 26, 28, 29, 31):
             call[i][...] = dev[i].cpu().numpy()

@@ -157,8 +157,9 @@ def device_arguments(host, magnetic=None):
     return dev, mag
 
 
-def time_traces(host, magnetic, steps: int, warmup: int, block_size: int) -> dict:
-    """Trace every trajectory ``steps`` times (after ``warmup``), states reset before each."""
+def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkpoints=None) -> dict:
+    """Trace every trajectory ``steps`` times (after ``warmup``), states reset before each
+    (``checkpoints``: (tensor, interval) receiving the states, as for the adjoint)."""
     import torch
 
     from beamtracer_amd import kernels
@@ -173,7 +174,9 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int) -> dic
             dev[i].copy_(a)
         if start is not None:
             start.record(stream)
-        kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=block_size, stream=stream)
+        ck, interval = checkpoints if checkpoints is not None else (None, 64)
+        kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=block_size, stream=stream,
+                                      checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
         if end is not None:
             end.record(stream)
 
@@ -340,6 +343,7 @@ def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
     # forward: the plain trace of every trajectory (inputs in HBM, as the headline)
     host = kernel_arguments(design, tracer)
     fwd = time_traces(host, None, repeats, 1, 0)
+    fwd_ck = time_traces(host, None, repeats, 1, 0, checkpoints=(torch.zeros_like(tracer.checkpoints[0]), interval))
     # adjoint: the kernel alone on device-resident records and seeds
     p = design.particles
     dev = torch.device("cuda")
@@ -381,6 +385,7 @@ def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
         f"{interval} steps; gradient w.r.t. fx (802x201), fy (801x202) and the 3 electrode voltages",
         "objective": value,
         "forward_ms": fwd["launch_s"] * 1e3,
+        "forward_with_checkpoints_ms": fwd_ck["launch_s"] * 1e3,
         "adjoint_ms": adj_s * 1e3,
         "ratio": adj_s / fwd["launch_s"],
         "seeded_trajectories": int(seeded.size),
@@ -388,8 +393,9 @@ def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
         "dJ_dV": grad_v.tolist(),
         "fd_dJ_dV_middle": fd_v,
         "fd_rel_err_middle": abs(grad_v[1] - fd_v) / abs(fd_v),
-        "note": "ratio = adjoint kernel (checkpointed forward, segment recompute, transposed steps, gradient "
-        "scatter) / one forward trace launch, HIP events; FD relative step 1e-4",
+        "note": "ratio = adjoint kernel (from the forward trace's checkpoints: segment recompute, transposed "
+        "steps, gradient scatter) / one forward trace launch, HIP events; the forward that feeds the adjoint "
+        "stores its checkpoints (forward_with_checkpoints_ms); FD relative step 1e-4",
     }
 
 

@@ -86,6 +86,10 @@ typedef struct bt_trace2d_args {
     int32_t flags;              /* BT_REFILL_BLOCK or 0                                 */
     void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned */
     int64_t workspace_bytes;
+    /* extension: the state (x, v) of every trajectory before each step whose count is a
+     * multiple of checkpoint_interval, into checkpoints[c][6][n] for c < n_checkpoints
+     * (device form only; NULL: none) -- the adjoint's checkpoints */
+    double* checkpoints; int64_t n_checkpoints; int64_t checkpoint_interval;
 } bt_trace2d_args;
 
 /* Library version string. */
@@ -216,6 +220,8 @@ typedef struct bt_adjoint_args {
     const double* hits; int64_t capacity; const int64_t* counts;  /* the forward trace's   */
     const double* seeds;                          /* (n, n_planes, capacity, 5)             */
     int64_t max_step;                             /* largest seeded record step            */
+    const double* checkpoints;                    /* the forward trace's [c][6][n] at this  */
+    int64_t n_checkpoints;                        /* interval, or NULL: stepped again here  */
     int32_t interval;                             /* steps between checkpoints (>= 1)      */
     int32_t block_size;                           /* 0: 256                                 */
     double* dfx; double* dfy;                     /* += dJ/dfx, dJ/dfy                      */
