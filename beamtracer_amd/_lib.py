@@ -135,6 +135,7 @@ class AdjointArgs(ctypes.Structure):
         ("interval", _i32), ("block_size", _i32),
         ("dfx", _ptr), ("dfy", _ptr),
         ("dx", _ptr), ("dv", _ptr),
+        ("workspace", _ptr), ("workspace_bytes", _i64),
     ]  # fmt: skip
 
 
@@ -164,6 +165,7 @@ SIGNATURES = {
         [_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _i64, _ptr, _i64, _i64, _i32, _ptr, _i64, _ptr],
     ),
     "bt_adjoint_args_size": (_i64, []),
+    "bt_adjoint_workspace_bytes": (_i64, [_i64, _i64, _i32, _i64, _i64, _i32]),
     "bt_division_selftest": (ctypes.c_int, [_ptr, _ptr, _i64, _ptr, _ptr]),
     "bt_trace_adjoint_device": (ctypes.c_int, [ctypes.POINTER(AdjointArgs), _ptr]),
     "bt_batch_args_size": (_i64, []),

@@ -16,7 +16,9 @@ so J is smooth in every record.
 * :class:`PlaneObjective` -- the ``torch.autograd.Function`` ``J(fx, fy)`` whose backward is
   the adjoint, so that gradients flow to whatever made the fields;
 * :func:`unit_fields` -- the edge fields of unit electrode voltages: the vacuum field is
-  linear in the voltages, ``fx = sum_k V_k fx_k + fx_faces``, so dJ/dV_k follows by autograd.
+  linear in the voltages, ``fx = sum_k V_k fx_k + fx_faces``, so dJ/dV_k follows by autograd;
+* :func:`batch_gradients` -- the gradients of many designs, their adjoint kernels running
+  concurrently on several HIP streams.
 """
 
 from __future__ import annotations
@@ -129,11 +131,13 @@ class TraceAdjoint:
         hits, counts = t.last_records
         value, seeds = plane_objective(hits, counts, self.particles.current, self.plane, self.cylindrical)
         self._records = (hits, counts, seeds)
+        self.last_value = value
         return value
 
-    def gradient(self, fx, fy, scale: float = 1.0, stream=None):
+    def gradient(self, fx, fy, scale: float = 1.0, stream=None, own_workspace: bool = False):
         """``scale`` dJ/dfx, dJ/dfy (torch tensors) at the fields of the last
-        :meth:`objective`; also sets ``self.dx``, ``self.dv`` (dJ/d initial state)."""
+        :meth:`objective`; also sets ``self.dx``, ``self.dv`` (dJ/d initial state).
+        ``own_workspace`` gives the call its own scratch (for calls on several streams)."""
         import torch
 
         if self._records is None:
@@ -162,12 +166,18 @@ class TraceAdjoint:
         dfy = torch.zeros_like(fyd)
         self.dx = torch.zeros_like(x0)
         self.dv = torch.zeros_like(v0)
+        workspace = None
+        if own_workspace:
+            size = _lib.lib().bt_adjoint_workspace_bytes(active.size, max_step, self.interval,
+                                                         INTEGRATORS.index(t.integrator), geometry_code(mesh), 0)  # fmt: skip
+            _lib.check(0 if size > 0 else int(size), "bt_adjoint_workspace_bytes")
+            workspace = torch.empty(int(size), dtype=torch.uint8, device=dev)
         self.launch(act, x0, v0, qm, fxd, fyd, planes, hits_d, counts_d, seeds_d, max_step, dfx, dfy, self.dx,
-                    self.dv, stream)  # fmt: skip
+                    self.dv, stream, workspace=workspace)  # fmt: skip
         return dfx, dfy
 
     def launch(self, active, x0, v0, qm, fx, fy, planes, hits, counts, seeds, max_step, dfx, dfy, dx=None,
-               dv=None, stream=None, checkpoints=True) -> None:  # fmt: skip
+               dv=None, stream=None, checkpoints=True, workspace=None) -> None:  # fmt: skip
         """One adjoint kernel call on device tensors (asynchronous on ``stream``); with
         ``checkpoints`` it starts from the forward trace's stored states."""
         import torch
@@ -198,9 +208,36 @@ class TraceAdjoint:
         a.dfx, a.dfy = dfx.data_ptr(), dfy.data_ptr()
         a.dx = dx.data_ptr() if dx is not None else None
         a.dv = dv.data_ptr() if dv is not None else None
+        if workspace is not None:
+            a.workspace = workspace.data_ptr()
+            a.workspace_bytes = workspace.numel()
         st = stream if stream is not None else torch.cuda.current_stream()
         rc = _lib.lib().bt_trace_adjoint_device(ctypes.byref(a), ctypes.c_void_p(st.cuda_stream))
         _lib.check(rc, "bt_trace_adjoint_device")
+
+
+def batch_gradients(problems, fields, streams: int = 4, evaluate: bool = True) -> list:
+    """J, dJ/dfx, dJ/dfy of several designs (one :class:`TraceAdjoint` and one field pair
+    each): the forward traces one after another (``evaluate``; else the last ones), then
+    every adjoint kernel on ``streams`` HIP streams at once, each call with its own scratch,
+    so that the small designs' kernels share the GPU. Returns ``[(J, dfx, dfy), ...]`` in
+    the order of ``problems``."""
+    import torch
+
+    if evaluate:
+        values = [p.objective(fx, fy) for p, (fx, fy) in zip(problems, fields)]
+    else:
+        values = [p.last_value for p in problems]
+    pool = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]  # default priority
+    torch.cuda.synchronize()
+    out = []
+    for i, (p, (fx, fy)) in enumerate(zip(problems, fields)):
+        st = pool[i % len(pool)]
+        with torch.cuda.stream(st):
+            dfx, dfy = p.gradient(fx, fy, stream=st, own_workspace=True)
+        out.append((values[i], dfx, dfy))
+    torch.cuda.synchronize()
+    return out
 
 
 def _objective_function():

@@ -367,6 +367,25 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
     return BT_OK;
 }
 
+// The grid, checkpoint slots and scratch bytes of an adjoint call.
+size_t adjoint_shape(i64 n_active, i64 max_step, i64 interval, int integ, int geom, int block, int cus, int* grid_out,
+                     i64* n_ckpt_out) {
+    int grid = 0;
+    if (n_active > 0) {
+        int per_cu = bt::adjoint_occupancy(integ, geom, block);
+        if (per_cu < 1) per_cu = 1;
+        const i64 full = (i64)per_cu * cus;
+        const i64 need = (n_active + block - 1) / block;
+        grid = (int)(need < full ? need : full);
+        if (grid < 1) grid = 1;
+    }
+    const i64 lanes = (i64)grid * block;
+    const i64 n_ckpt = max_step > 0 ? (max_step - 1) / interval + 1 : 1;
+    *grid_out = grid;
+    *n_ckpt_out = n_ckpt;
+    return 256 + (size_t)(n_ckpt + interval) * 6 * 8 * (size_t)(lanes > 0 ? lanes : 1);
+}
+
 }  // namespace
 
 extern "C" {
@@ -579,6 +598,22 @@ int bt_division_selftest(const double* a, const double* b, int64_t n, uint64_t* 
 
 int64_t bt_adjoint_args_size(void) { return (int64_t)sizeof(bt_adjoint_args); }
 
+int64_t bt_adjoint_workspace_bytes(int64_t n_active, int64_t max_step, int32_t interval, int64_t integrator,
+                                   int64_t cylindrical, int32_t block_size) {
+    if (n_active < 0 || max_step < 0 || interval < 1 || integrator < 0 || integrator > 1 || cylindrical < 0 ||
+        cylindrical > 2)
+        return fail(BT_E_ARG, "bad arguments");
+    const int block = block_size > 0 ? block_size : 256;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    int rc = device_state(&s);
+    if (rc) return rc;
+    int grid = 0;
+    i64 n_ckpt = 1;
+    return (int64_t)adjoint_shape(n_active, max_step, interval, (int)integrator, (int)cylindrical, block, s->cus,
+                                  &grid, &n_ckpt);
+}
+
 int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     if (a == nullptr) return fail(BT_E_ARG, "args is NULL");
     if (a->n < 0 || a->n_active < 0) return fail(BT_E_ARG, "negative trajectory count");
@@ -604,18 +639,17 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const int block = a->block_size > 0 ? a->block_size : 256;
     int grid = 0;
-    if (a->n_active > 0) {
-        int per_cu = bt::adjoint_occupancy(integ, geom, block);
-        if (per_cu < 1) per_cu = 1;
-        const i64 full = (i64)per_cu * s->cus;
-        const i64 need = (a->n_active + block - 1) / block;
-        grid = (int)(need < full ? need : full);
-        if (grid < 1) grid = 1;
-    }
+    i64 n_ckpt = 1;
+    const size_t bytes = adjoint_shape(a->n_active, a->max_step, a->interval, integ, geom, block, s->cus, &grid,
+                                       &n_ckpt);
     const i64 lanes = (i64)grid * block;
-    const i64 n_ckpt = a->max_step > 0 ? (a->max_step - 1) / a->interval + 1 : 1;
-    const size_t bytes = 256 + (size_t)(n_ckpt + a->interval) * 6 * 8 * (size_t)(lanes > 0 ? lanes : 1);
-    if ((rc = ensure(s->adjoint, bytes))) return rc;
+    void* ws = a->workspace;
+    if (ws == nullptr) {
+        if ((rc = ensure(s->adjoint, bytes))) return rc;
+        ws = s->adjoint.ptr;
+    } else if (a->workspace_bytes < (int64_t)bytes) {
+        return fail(BT_E_ARG, "workspace smaller than bt_adjoint_workspace_bytes()");
+    }
     bt::AdjointParams p;
     std::memset(&p, 0, sizeof(p));
     p.active = (const i64*)a->active; p.n_active = a->n_active; p.n = a->n;
@@ -630,8 +664,8 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
         p.fwd_ckpt = a->checkpoints;
         p.fwd_n_ckpt = a->n_checkpoints;
     }
-    p.queue = static_cast<u64*>(s->adjoint.ptr);
-    p.scratch = reinterpret_cast<double*>(static_cast<char*>(s->adjoint.ptr) + 256);
+    p.queue = static_cast<u64*>(ws);
+    p.scratch = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
     p.dfx = a->dfx; p.dfy = a->dfy; p.dx = a->dx; p.dv = a->dv;
     if (bt::launch_adjoint(p, block, grid, static_cast<hipStream_t>(stream), integ, geom))
         return fail(BT_E_LAUNCH, std::string("adjoint launch: ") + hipGetErrorString(hipGetLastError()));

@@ -292,6 +292,15 @@ def batched_leg(count: int, seed: int, processes: int, repeats: int) -> dict:
         torch.cuda.synchronize()
         plain_s += a.elapsed_time(b) / 1e3
         plain_steps += int(dv[5].sum().item())
+    # the same at the caller: HipBeamTracer.trace per design (host form: copies, launches,
+    # paused trajectories, node volumes, plane hits) against one BatchedTracer.trace
+    t0 = time.perf_counter()
+    for d, t in zip(ds, tracers):
+        t.trace(d.particles)
+    caller_plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bt.trace(beams)
+    caller_batched_s = time.perf_counter() - t0
     # the captured graph of the whole batched trace (reset, queue, trace, finalize)
     bt.capture(beams)
     gtimes = []
@@ -316,12 +325,54 @@ def batched_leg(count: int, seed: int, processes: int, repeats: int) -> dict:
         "plain_ms_sum": plain_s * 1e3,
         "plain_particle_steps": plain_steps,
         "speedup_vs_plain": (batched_steps / batched_s) / (plain_steps / plain_s),
+        "caller_plain_s": caller_plain_s,
+        "caller_batched_s": caller_batched_s,
+        "caller_speedup": caller_plain_s / caller_batched_s,
         "unit": "particle-steps/s",
         "launch_shape": tune["choice"],
         "autotune_ms": tune["timings_ms"],
         "setup_s": setup,
-        "note": "one launch (queue over every design) vs one launch per design, both with inputs resident in "
-        "HBM, HIP events; per-design results are bit-identical (tests/test_gpu_batch.py)",
+        "note": "speedup_vs_plain: one launch (queue over every design) vs one launch per design, both with "
+        "inputs resident in HBM, HIP events; caller_speedup: HipBeamTracer.trace per design (host arrays) vs one "
+        "BatchedTracer.trace, wall clock; per-design results are bit-identical (tests/test_gpu_batch.py)",
+    }
+
+
+def batched_gradients_leg(count: int, seed: int, processes: int) -> dict:
+    """dJ/dfx, dJ/dfy of `count` einzel candidates: their adjoint kernels one after another
+    against all of them on 4 concurrent streams."""
+    import torch
+
+    from beamtracer_amd import adjoint, batch, designs
+
+    ds = designs.einzel_batch(count, seed=seed)
+    tracers = batch.vacuum_tracers(ds, processes=processes)
+    problems = [adjoint.TraceAdjoint(t, d.particles) for d, t in zip(ds, tracers)]
+    fields_ = [(torch.from_numpy(t.electric.fx).cuda(), torch.from_numpy(t.electric.fy).cuda()) for t in tracers]
+    t0 = time.perf_counter()
+    for p, (fx, fy) in zip(problems, fields_):
+        p.objective(fx, fy)
+    forward_s = time.perf_counter() - t0
+    adjoint.batch_gradients(problems, fields_, streams=4, evaluate=False)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, (fx, fy) in zip(problems, fields_):
+        p.gradient(fx, fy)
+        torch.cuda.synchronize()
+    sequential_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    adjoint.batch_gradients(problems, fields_, streams=4, evaluate=False)
+    concurrent_s = time.perf_counter() - t0
+    return {
+        "workload": f"{count} einzel candidates from seed {seed} (as the batched leg), J = rms_angle + emittance at "
+        "the last plane, gradients w.r.t. each design's fx, fy",
+        "trajectories": int(sum(len(d.particles) for d in ds)),
+        "forward_traces_s": forward_s,
+        "gradients_sequential_s": sequential_s,
+        "gradients_concurrent_s": concurrent_s,
+        "speedup": sequential_s / concurrent_s,
+        "note": "gradients: host seeds + uploads + adjoint kernels (from the forward traces' checkpoints), "
+        "one design at a time vs 4 HIP streams with a scratch each; wall clock",
     }
 
 
@@ -490,6 +541,8 @@ def main() -> int:
         extra["batched"] = batched_leg(opts.batch_designs, 0, processes, 3)
         log("adjoint: einzel lens, 2e5 trajectories")
         extra["adjoint"] = adjoint_leg(200_000, 3)
+        log("adjoint: 32 einzel candidates")
+        extra["adjoint"]["batched_gradients"] = batched_gradients_leg(32, 1, processes)
         line["extra"] = extra
     if not opts.no_cpu:
         log("cpu baseline: oracle tracer on the host cores")

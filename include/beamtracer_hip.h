@@ -226,6 +226,9 @@ typedef struct bt_adjoint_args {
     int32_t block_size;                           /* 0: 256                                 */
     double* dfx; double* dfy;                     /* += dJ/dfx, dJ/dfy                      */
     double* dx; double* dv;                       /* += dJ/dx, dJ/dv (3, n), or NULL        */
+    void* workspace; int64_t workspace_bytes;     /* device scratch of bt_adjoint_workspace_bytes()
+                                                     (calls on several streams need their own);
+                                                     NULL: library-owned, one call at a time */
 } bt_adjoint_args;
 
 /* Self-test of the kernels' division by a known reciprocal (no counterpart in the
@@ -234,6 +237,9 @@ typedef struct bt_adjoint_args {
 int bt_division_selftest(const double* a, const double* b, int64_t n, uint64_t* count, void* stream);
 
 int64_t bt_adjoint_args_size(void);
+/* Scratch of an adjoint call: its queue head and checkpoint/segment states per lane. */
+int64_t bt_adjoint_workspace_bytes(int64_t n_active, int64_t max_step, int32_t interval, int64_t integrator,
+                                   int64_t cylindrical, int32_t block_size);
 int bt_trace_adjoint_device(const bt_adjoint_args* args, void* stream);
 
 #ifdef __cplusplus

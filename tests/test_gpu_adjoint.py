@@ -7,6 +7,7 @@
 * The checkpoint interval changes nothing but the order of the atomic adds.
 * The electrode-voltage gradient through the unit-voltage fields and autograd equals
   central differences of J with a relative voltage step of 1e-4.
+* Gradients of several designs on concurrent streams equal the one-at-a-time gradients.
 """
 
 from __future__ import annotations
@@ -108,3 +109,22 @@ def test_voltage_gradient_matches_central_differences():
         fd = (float(objective(torch.tensor(up, device="cuda"))) - float(objective(torch.tensor(down, device="cuda"))))
         fd /= 2 * step
         assert grad[k] == pytest.approx(fd, rel=1e-6, abs=1e-6 * np.abs(grad).max())
+
+
+def test_batch_gradients_equal_single_gradients():
+    import torch
+
+    ds = designs.einzel_batch(6, seed=9, nx_range=(121, 201), ny_range=(31, 51), particles_range=(200, 600))
+    from beamtracer_amd import batch
+
+    tracers = batch.vacuum_tracers(ds)
+    problems = [adjoint.TraceAdjoint(t, d.particles) for d, t in zip(ds, tracers)]
+    fields_ = [(torch.from_numpy(t.electric.fx).cuda(), torch.from_numpy(t.electric.fy).cuda()) for t in tracers]
+    together = adjoint.batch_gradients(problems, fields_, streams=3)
+    for p, (fx, fy), (value, gx, gy) in zip(problems, fields_, together):
+        assert p.objective(fx, fy) == value
+        sx, sy = p.gradient(fx, fy)
+        torch.cuda.synchronize()
+        for a, b in ((gx, sx), (gy, sy)):
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13 * np.abs(b).max())
