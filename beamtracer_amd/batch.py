@@ -36,8 +36,8 @@ from beamtracer_amd.tracer import (
     INTEGRATORS,
     SUBCELLS,
     HipBeamTracer,
+    PlaneHits,
     TraceResult,
-    collect_plane_hits,
     geometry_code,
 )
 
@@ -281,71 +281,113 @@ class BatchedTracer:
 
     def _resume_paused(self, dev: dict, host: dict) -> None:
         """Decide paused trajectories with the electrodes' own test and resume the others,
-        design by design, until none is paused (``tracer.py:1236-1256``)."""
+        design by design, until none is paused (``tracer.py:1236-1256``).
+
+        Only the paused trajectories cross to the host: their indices and positions go
+        down, their new status and electrode go back up by index.
+        """
         import torch
 
+        first = np.asarray(host["first"])
         while True:
-            torch.cuda.synchronize()
-            status = dev["status"].cpu().numpy()
-            paused = np.flatnonzero(status == _PAUSED)
-            if not paused.size:
+            paused_dev = torch.nonzero(dev["status"] == _PAUSED).squeeze(1)
+            if not paused_dev.numel():
                 return
-            x = dev["x"][:, torch.from_numpy(paused).cuda()].cpu().numpy()
-            resume = []
-            electrode = dev["electrode"].cpu().numpy()
-            for d, t in enumerate(self.tracers):
-                lo, hi = host["first"][d], host["first"][d + 1]
-                sel = (paused >= lo) & (paused < hi)
-                if not sel.any():
-                    continue
-                hit = t.mesh.inside_solid(*t._project(x[:, sel]))
-                ids = paused[sel]
-                status[ids[hit >= 0]] = ELECTRODE
-                electrode[ids[hit >= 0]] = hit[hit >= 0]
-                resume.append(ids[hit < 0])
-            active = np.concatenate(resume) if resume else np.zeros(0, np.int64)
-            status[active] = _ALIVE
-            dev["status"].copy_(torch.from_numpy(status))
-            dev["electrode"].copy_(torch.from_numpy(electrode))
+            paused = paused_dev.cpu().numpy()
+            x = dev["x"][:, paused_dev].cpu().numpy()
+            design = np.searchsorted(first, paused, side="right") - 1
+            hit = np.full(paused.size, -1, dtype=np.int64)
+            for d in np.unique(design):
+                sel = design == d
+                t = self.tracers[d]
+                hit[sel] = t.mesh.inside_solid(*t._project(x[:, sel]))
+            inside = hit >= 0
+            if inside.any():
+                ids = torch.from_numpy(paused[inside]).cuda()
+                dev["status"][ids] = ELECTRODE
+                dev["electrode"][ids] = torch.from_numpy(hit[inside]).cuda()
+            active = paused[~inside]
             if not active.size:
                 return
-            self.launch(dev, torch.from_numpy(active.astype(np.int64)).cuda())
+            active_dev = torch.from_numpy(active.astype(np.int64)).cuda()
+            dev["status"][active_dev] = _ALIVE
+            self.launch(dev, active_dev)
 
-    def _collect(self, dev: dict, host: dict) -> list[TraceResult]:
+    def _plane_records(self, dev: dict) -> list[tuple[np.ndarray, np.ndarray]]:
+        """Per plane, the recorded crossings of every design as ``(trajectory, record)``:
+        compacted on the device to the filled slots, ordered by design, then by step, then
+        by trajectory (the order of ``collect_plane_hits``)."""
         import torch
 
-        torch.cuda.synchronize()
-        status = dev["status"].cpu().numpy()
-        electrode = dev["electrode"].cpu().numpy()
-        dead = dev["status0"].cpu().numpy() == ELECTRODE
-        status[dead] = ELECTRODE
-        electrode[dead] = dev["electrode0"].cpu().numpy()[dead]
-        steps = dev["steps"].cpu().numpy()
-        hits = dev["hits"].cpu().numpy()
-        counts = dev["counts"].cpu().numpy()
+        hits, counts = dev["hits"], dev["counts"]
+        capacity = hits.shape[2]
+        slots = torch.arange(capacity, device=hits.device)
+        design_of = dev["design_of"].to(torch.int64)
+        out = []
+        for p in range(len(self.tracers[0].planes)):
+            k, c = torch.nonzero(slots[None, :] < counts[:, p, None], as_tuple=True)
+            record = hits[k, p, c]
+            # k ascends in nonzero's output; a stable sort on (design, step) keeps it
+            # ascending among equal keys
+            key = design_of[k] * (self.tracers[0].max_steps + 2) + record[:, 7].to(torch.int64)
+            order = torch.sort(key, stable=True).indices
+            out.append((k[order].cpu().numpy(), record[order].cpu().numpy()))
+        return out
+
+    def _collect(self, dev: dict, host: dict) -> list[TraceResult]:
+        """Results per design; only the per-trajectory outcomes, the filled plane slots and
+        the node charges are copied to the host (the end states stay in
+        ``last_device_state``, see :meth:`end_state`)."""
+        import torch
+
+        dead = dev["status0"] == ELECTRODE
+        status = torch.where(dead, dev["status0"], dev["status"]).to(torch.int8).cpu().numpy()
+        electrode = torch.where(dead, dev["electrode0"], dev["electrode"]).to(torch.int32).cpu().numpy()
+        design_of = dev["design_of"].to(torch.int64)
+        n_designs = len(self.tracers)
+        steps = dev["steps"]
+        longest = torch.zeros(n_designs, dtype=torch.int64, device=steps.device)
+        longest.scatter_reduce_(0, design_of, steps, "amax")
+        total = torch.zeros(n_designs, dtype=torch.int64, device=steps.device).index_add_(0, design_of, steps)
+        longest, total = longest.cpu().numpy(), total.cpu().numpy()
         charge = self._dev["charge"].cpu().numpy()
-        x = dev["x"].cpu().numpy()
-        v = dev["v"].cpu().numpy()
+        records = self._plane_records(dev)
+        first = np.asarray(host["first"])
+        bounds = [np.searchsorted(k, first) for k, _ in records]
         out = []
         for d, t in enumerate(self.tracers):
             lo, hi = host["first"][d], host["first"][d + 1]
             row = self._rows[d]
             nodes = charge[row[14]: row[14] + row[15]].reshape(t.mesh.shape)
-            planes = collect_plane_hits(t.planes, hits[lo:hi], counts[lo:hi], host["current"][lo:hi],
-                                        host["species"][lo:hi])  # fmt: skip
-            res = TraceResult(
+            planes = []
+            for position, (k, record), b in zip(t.planes, records, bounds):
+                kd, rd = k[b[d]: b[d + 1]], record[b[d]: b[d + 1]]
+                plane = PlaneHits(position)
+                plane.point = rd[:, 0:3].T.copy()
+                plane.velocity = rd[:, 3:6].T.copy()
+                plane.current = host["current"][kd]
+                plane.species = host["species"][kd].astype(np.int32)
+                plane.direction = rd[:, 6].astype(np.int8)
+                planes.append(plane)
+            out.append(TraceResult(
                 charge_density=t.finalize(nodes.copy()),
-                status=status[lo:hi].astype(np.int8),
-                electrode=electrode[lo:hi].astype(np.int32),
+                status=status[lo:hi],
+                electrode=electrode[lo:hi],
                 planes=planes,
                 trajectories=[],
                 trajectory_species=np.zeros(0, dtype=np.int32),
-                steps=int(steps[lo:hi].max()) if hi > lo else 0,
-                particle_steps=int(steps[lo:hi].sum()),
-            )
-            t.last_state = (x[:, lo:hi], v[:, lo:hi], steps[lo:hi])
-            out.append(res)
+                steps=int(longest[d]),
+                particle_steps=int(total[d]),
+            ))  # fmt: skip
+        self._first = host["first"]
         return out
+
+    def end_state(self, design: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Positions, velocities and step counts of one design's trajectories at the end of
+        the last trace (what :class:`HipBeamTracer` keeps as ``last_state``)."""
+        dev = self.last_device_state
+        lo, hi = self._first[design], self._first[design + 1]
+        return dev["x"][:, lo:hi].cpu().numpy(), dev["v"][:, lo:hi].cpu().numpy(), dev["steps"][lo:hi].cpu().numpy()
 
     def trace(self, particles: Sequence[BeamParticles]) -> list[TraceResult]:
         """Trace every design's beam; one :class:`TraceResult` per design."""
@@ -407,6 +449,7 @@ class BatchedTracer:
         self._resume_paused(dev, host)
         if int(dev["counts"].max().item()) > dev["hits"].shape[2]:
             raise RuntimeError("plane buffers overflowed in a captured trace: trace() first to size them")
+        self.last_device_state = dev
         return self._collect(dev, host)
 
 

@@ -2,7 +2,7 @@
 
 Six einzel candidates of different mesh sizes, voltages, gaps, energies and beam sizes (the
 batch generator's draws, at small sizes) are traced in one batched launch and, design by
-design, by the plain hip caller; every output must agree bit for bit (the same arithmetic,
+design, by the plain hip caller; every output and end state must agree bit for bit (the same arithmetic,
 each design's own fixed-point exponent, exact integer node sums). The captured HIP graph is
 replayed several times with eager work between the replays, and after a field update.
 """
@@ -43,8 +43,10 @@ def test_batched_trace_equals_plain_traces_per_design():
     outcomes = np.concatenate([g.status for g in got])
     # transmitted beams, and ions turned back by a middle tube above their energy
     assert (outcomes == 0).sum() > 100 and (outcomes == 1).sum() > 100
-    for d, t, g in zip(ds, tracers, got):
+    for k, (d, t, g) in enumerate(zip(ds, tracers, got)):
         _assert_identical(g, t.trace(d.particles))
+        for a, b in zip(bt.end_state(k), t.last_state):  # the end states too
+            np.testing.assert_array_equal(a, b)
 
 
 def test_queue_orders_and_launch_shapes_give_the_same_bits():
