@@ -48,20 +48,15 @@ template <int GEOM>
 __device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F, double qm, Vec3 q) {
     double cy = 0.0, cz = 0.0, qy = q.b;
     if (GEOM == 1) {
-        const double r = hypot(q.b, q.c);
-        if (r > 0.0) {
-            const double ir = 1.0 / r;
-            cy = div_rn(q.b, r, ir);
-            cz = div_rn(q.c, r, ir);
-        }
-        qy = r;
+        const Radial rd = radial(q.b, q.c);
+        qy = rd.r; cy = rd.cy; cz = rd.cz;
     } else if (GEOM == 2) {
         cy = 1.0;
         if (q.b < p.origin_y) cy = -1.0;
         qy = p.origin_y + fabs(q.b - p.origin_y);
     }
-    const double sx = div_rn(q.a - p.origin_x, p.h, p.inv_h);
-    const double sy = div_rn(qy - p.origin_y, p.h, p.inv_h);
+    const double sx = (q.a - p.origin_x) * p.inv_h;
+    const double sy = (qy - p.origin_y) * p.inv_h;
     const double ex = clamped_bilinear_buf(F.fx, 0u, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
     const double ey = clamped_bilinear_buf(F.fy, 0u, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
     if (GEOM != 0) return {qm * ex, qm * (ey * cy), qm * (ey * cz)};
@@ -193,20 +188,15 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
                                               double qm, Vec3 q, Vec3 lg) {
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
     if (GEOM == 1) {
-        r = hypot(q.b, q.c);
-        if (r > 0.0) {
-            const double ir = 1.0 / r;
-            cy = div_rn(q.b, r, ir);
-            cz = div_rn(q.c, r, ir);
-        }
-        qy = r;
+        const Radial rd = radial(q.b, q.c);
+        r = rd.r; qy = r; cy = rd.cy; cz = rd.cz;
     } else if (GEOM == 2) {
         cy = 1.0;
         if (q.b < p.origin_y) cy = -1.0;
         qy = p.origin_y + fabs(q.b - p.origin_y);
     }
-    const double sx = div_rn(q.a - p.origin_x, p.h, p.inv_h);
-    const double sy = div_rn(qy - p.origin_y, p.h, p.inv_h);
+    const double sx = (q.a - p.origin_x) * p.inv_h;
+    const double sy = (qy - p.origin_y) * p.inv_h;
     const Gathered gx = gather_d(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
     const Gathered gy = gather_d(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
     const double lex = qm * lg.a;
@@ -224,8 +214,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     scatter<1>(p.dfy, p.ny + 1, W, gy, ley);
     const double lsx = lex * gx.ds + ley * gy.ds;
     const double lsy = lex * gx.dt + ley * gy.dt;
-    Vec3 lq = {lsx / p.h, 0.0, 0.0};
-    const double lqy = lsy / p.h;
+    Vec3 lq = {lsx * p.inv_h, 0.0, 0.0};
+    const double lqy = lsy * p.inv_h;
     if (GEOM == 1) {
         if (r > 0.0) {
             // r = |(y, z)|, cy = y / r, cz = z / r
@@ -240,20 +230,16 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     return lq;
 }
 
-// dt of the step rule and its branch: 1 = h / |v|, 2 = sqrt(2 h / |a|), 0 = h.
+// dt of the step rule and its branch: 1 = h / |v|, 2 = sqrt(2 h / |a|), 0 = h (the trace
+// kernel's step_rule on the same operands).
 __device__ __forceinline__ double step_size(const AdjointParams& p, Vec3 v, Vec3 a, int& branch, double& speed,
                                             double& acc) {
-    speed = 1.0 * sqrt(v.a * v.a + v.b * v.b + v.c * v.c);
-    acc = sqrt(a.a * a.a + a.b * a.b + a.c * a.c);
-    double dt = -1.0;
-    branch = 0;
-    if (speed > 0.0) { dt = p.h / speed; branch = 1; }
-    if (acc > 0.0) {
-        const double d2 = sqrt(2.0 * p.h / acc);
-        if (dt < 0.0 || d2 < dt) { dt = d2; branch = 2; }
-    }
-    if (dt < 0.0) dt = p.h;
-    return p.step_fraction * dt;
+    const StepRule r = step_rule(v.a * v.a + v.b * v.b + v.c * v.c, a.a * a.a + a.b * a.b + a.c * a.c, p.h,
+                                 p.step_fraction, 1.0);
+    branch = r.branch;
+    speed = r.speed;
+    acc = r.accel;
+    return r.dt;
 }
 
 // One forward step (the trace kernel's arithmetic).
@@ -266,7 +252,7 @@ __device__ __forceinline__ void forward_step(const AdjointParams& p, const AdjFi
     const double dt = step_size(p, v, a, branch, speed, acc);
     const double hdt = 0.5 * dt;
     if (INTEG == 0) {
-        const double dt6 = div_rn(dt, 6.0, 1.0 / 6.0);
+        const double dt6 = dt * (1.0 / 6.0);
         Vec3 sx = v, sv = a;
         Vec3 q = vaxpy(x, hdt, v), k = vaxpy(v, hdt, a);
         Vec3 g = accel<GEOM>(p, F, qm, q);
@@ -346,7 +332,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
     Vec3 La = {0.0, 0.0, 0.0}, Lx, Lv;
     double Ldt = 0.0, Lhdt = 0.0;
     if (INTEG == 0) {
-        const double dt6 = div_rn(dt, 6.0, 1.0 / 6.0);
+        const double dt6 = dt * (1.0 / 6.0);
         const Vec3 q1 = vaxpy(x, hdt, v), k1 = vaxpy(v, hdt, a);
         const Vec3 g1 = accel<GEOM>(p, F, qm, q1);
         const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g1);
@@ -379,7 +365,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         const Vec3 Lq1 = accel_adjoint<GEOM>(p, F, W, qm, q1, Lg1);
         Lv = vadd(Lv, Lk1); Lhdt += vdot(Lk1, a); La = vaxpy(La, hdt, Lk1);
         Lx = vadd(Lx, Lq1); Lhdt += vdot(Lq1, v); Lv = vaxpy(Lv, hdt, Lq1);
-        Ldt += Ldt6 / 6.0;
+        Ldt += Ldt6 * (1.0 / 6.0);
         Lx = vadd(Lx, xs);
         Lv = vadd(Lv, vs);
     } else {
@@ -433,10 +419,10 @@ __device__ __forceinline__ void get_state(const double* __restrict__ base, i64 l
 template <int GEOM>
 __device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, int& ci, int& cj) {
     double qy = x.b;
-    if (GEOM == 1) qy = hypot(x.b, x.c);
+    if (GEOM == 1) qy = radial(x.b, x.c).r;
     else if (GEOM == 2) qy = p.origin_y + fabs(x.b - p.origin_y);
-    ci = clamped_floor((x.a - p.origin_x) / p.h, 0, (int)p.nx - 1);
-    cj = clamped_floor((qy - p.origin_y) / p.h, 0, (int)p.ny - 1);
+    ci = clamped_floor((x.a - p.origin_x) * p.inv_h, 0, (int)p.nx - 1);
+    cj = clamped_floor((qy - p.origin_y) * p.inv_h, 0, (int)p.ny - 1);
 }
 
 template <int INTEG, int GEOM>

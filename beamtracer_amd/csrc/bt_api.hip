@@ -7,6 +7,7 @@
 // caller's stream; only the small patch and plane tables of the trace are read on the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -130,6 +131,7 @@ int check_trace(const bt_trace2d_args* a) {
     const int64_t* base = a->patch_int + (a->n_patches - 1) * 6;
     if (a->cells_rows < base[0] - 1 || a->cells_cols < base[1] - 1)
         return fail(BT_E_ARG, "cells must be at least (nx - 1, ny - 1)");
+    if (a->cells_rows * a->cells_cols >= ((i64)1 << 31)) return fail(BT_E_ARG, "cells holds 2^31 entries or more");
     if (a->n_subcells < 1) return fail(BT_E_ARG, "subcells is empty");
     if (a->charge_rows < 1) return fail(BT_E_ARG, "charge has no rows");
     return BT_OK;
@@ -165,6 +167,8 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     p.inv_bh = 1.0 / basr[2];
     p.inv_h = 1.0 / a->h;
     p.deposit_exponent = a->deposit_exponent;
+    p.cell_imax = (int)std::min(p.nx - 2, p.cells_rows - 1);
+    p.cell_jmax = (int)std::min(p.ny - 2, p.cells_cols - 1);
     p.reuse = a->n_patches == 1 && basr[0] == a->origin_x && basr[1] == a->origin_y && basr[2] == a->h;
     p.refill_block = (a->flags & BT_REFILL_BLOCK) != 0;
     if (a->checkpoints != nullptr && a->n_checkpoints > 0 && a->checkpoint_interval > 0) {
@@ -203,7 +207,8 @@ int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t
 
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const bool has_b = a->has_b != 0;
-    const bool general = a->n_patches > 1 || a->graded != 0 || a->inv_c2 != 0.0;
+    // the uniform variant takes one patch that is the mesh itself (same origin and spacing)
+    const bool general = a->n_patches > 1 || a->graded != 0 || a->inv_c2 != 0.0 || p.reuse == 0;
     int block = a->block_size > 0 ? a->block_size : 256;
     if (block % 64 != 0 || block > 256) return fail(BT_E_ARG, "block_size must be a multiple of 64, at most 256");
     int grid = 0;
@@ -287,6 +292,7 @@ int check_designs(const bt_batch_design* designs, i64 n_designs, i64 n_fxs, i64 
             return fail(BT_E_ARG, at + "cells must be at least (nx - 1, ny - 1)");
         if (b.cells_offset < 0 || b.cells_offset + b.cells_rows * b.cells_cols > n_cells)
             return fail(BT_E_ARG, at + "cells beyond the cell arena");
+        if (b.cells_rows * b.cells_cols >= ((i64)1 << 31)) return fail(BT_E_ARG, at + "2^31 cells or more");
         if (b.n_subcells < 1 || b.subcells_offset < 0 || b.subcells_offset + b.n_subcells > n_subcells)
             return fail(BT_E_ARG, at + "subcells beyond the sub-cell arena");
         if (b.n_nodes != b.nx * b.ny || b.node_offset < 0) return fail(BT_E_ARG, at + "n_nodes must be nx * ny");
@@ -587,11 +593,13 @@ int bt_deposit_trajectories_2d(const double* px, const double* py, const double*
 }
 
 
-int bt_division_selftest(const double* a, const double* b, int64_t n, uint64_t* count, void* stream) {
-    if (n < 0 || (n > 0 && (a == nullptr || b == nullptr)) || count == nullptr) return fail(BT_E_ARG, "bad arguments");
+int bt_arith_selftest(const double* y, const double* z, int64_t n, double* r, double* cy, double* cz, double* rs,
+                      void* stream) {
+    if (n < 0 || (n > 0 && (y == nullptr || z == nullptr || r == nullptr || cy == nullptr || cz == nullptr ||
+                            rs == nullptr)))
+        return fail(BT_E_ARG, "bad arguments");
     if (n == 0) return BT_OK;
-    if (bt::launch_division_selftest(a, b, n, reinterpret_cast<unsigned long long*>(count),
-                                     static_cast<hipStream_t>(stream)))
+    if (bt::launch_arith_selftest(y, z, n, r, cy, cz, rs, static_cast<hipStream_t>(stream)))
         return fail(BT_E_LAUNCH, std::string("selftest launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;
 }

@@ -68,19 +68,94 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
     return low + u * (high - low);
 }
 
-// a / b rounded to nearest, from y = RN(1/b) computed once: two corrections
-// q <- q + (a - b q) y with the residual from a fused multiply-add (P. Markstein, IBM J.
-// Res. Dev. 34 (1990) 111-119; J.-M. Muller et al., Handbook of Floating-Point Arithmetic).
-// The second correction starts within one ulp of a/b, so its residual is exact and its
-// result is the correctly rounded quotient: the bits of IEEE a / b for finite operands
-// whose quotient neither overflows nor underflows (mesh coordinates and step sizes; a
-// zero quotient may differ only in its sign).  bt_division_selftest checks it on the GPU.
-__device__ __forceinline__ double div_rn(double a, double b, double y) {
-    double q = a * y;
-    double r = __builtin_fma(-b, q, a);
-    q = __builtin_fma(r, y, q);
-    r = __builtin_fma(-b, q, a);
-    return __builtin_fma(r, y, q);
+// ---- square roots through the hardware reciprocal square root -------------------------
+// The reference's divisions by |v|, r and sqrt(|a|) and its hypot become products with
+// 1 / sqrt(x): the hardware estimate y (v_rsq_f64) and two Newton steps
+// y <- y + y (1/2 - x y^2 / 2), each squaring the relative error, leave a few units in the
+// last place -- far inside the 1e-10 relative bar of the trace; bt_arith_selftest measures
+// it on the GPU.
+
+// 1 / sqrt(x) for a positive normal x of at least 2^-1000.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = __builtin_fma(-g, h, 0.5);
+    y = __builtin_fma(y, r, y);
+    g = x * y;
+    h = 0.5 * y;
+    r = __builtin_fma(-g, h, 0.5);
+    return __builtin_fma(y, r, y);
+}
+
+// 1 / sqrt(x) for any positive finite x (tiny ones scaled by an exact power of two).
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    if (x >= 0x1p-1000) return rsqrt_nr(x);
+    return rsqrt_nr(x * 0x1p600) * 0x1p300;
+}
+
+// The distance r = |(y, z)| from the axis and the direction (y / r, z / r); (0, 0) on the
+// axis, where r = 0 (pyccel.py:196-205: r = hypot(y, z), then y / r, z / r when r > 0).
+struct Radial {
+    double r, cy, cz;
+};
+
+__device__ __forceinline__ Radial radial(double y, double z) {
+    const double s2 = __builtin_fma(y, y, z * z);
+    Radial o;
+    if (s2 >= 0x1p-1000) {
+        const double ir = rsqrt_nr(s2);
+        o.r = s2 * ir;
+        o.cy = y * ir;
+        o.cz = z * ir;
+    } else if (s2 > 0.0) {  // within 1e-150 of the axis: the same with y, z scaled by 2^600
+        const double ys = y * 0x1p600, zs = z * 0x1p600;
+        const double s2s = __builtin_fma(ys, ys, zs * zs);
+        const double ir = rsqrt_nr(s2s);
+        o.r = (s2s * ir) * 0x1p-600;
+        o.cy = ys * ir;
+        o.cz = zs * ir;
+    } else {  // on the axis (or a NaN coordinate, which stays NaN in r)
+        o.r = s2;
+        o.cy = 0.0;
+        o.cz = 0.0;
+    }
+    return o;
+}
+
+// The step-size rule (pyccel.py:282-295): dt = f min(h / |v|, sqrt(2 h / |a|)) over the
+// terms that exist, h when neither does; |v| = gv sqrt(v2), |a| = sqrt(a2) (gv = 1 / gamma,
+// exactly 1 for classical motion).  `branch`: 1 = h / |v|, 2 = sqrt(2 h / |a|), 0 = h;
+// `speed`, `accel` = |v|, |a| where they exist (0 otherwise).
+struct StepRule {
+    double dt, speed, accel;
+    int branch;
+};
+
+__device__ __forceinline__ StepRule step_rule(double v2, double a2, double h, double f, double gv) {
+    StepRule o;
+    double dt = -1.0;
+    o.branch = 0;
+    o.speed = 0.0;
+    o.accel = 0.0;
+    if (v2 > 0.0) {
+        const double iv = rsqrt_pos(v2);
+        o.speed = gv * (v2 * iv);
+        dt = (h * iv) / gv;
+        o.branch = 1;
+    }
+    if (a2 > 0.0) {
+        const double ia = rsqrt_pos(a2);
+        o.accel = a2 * ia;
+        const double q = (2.0 * h) * ia;
+        const double d2 = q * rsqrt_pos(q);
+        if (dt < 0.0 || d2 < dt) {
+            dt = d2;
+            o.branch = 2;
+        }
+    }
+    if (dt < 0.0) dt = h;
+    o.dt = f * dt;
+    return o;
 }
 
 // ---- buffer gathers -------------------------------------------------------------------

@@ -35,6 +35,7 @@ struct TraceParams {
     double origin_x, origin_y, h, upper_x, upper_y;
     double step_fraction; i64 max_steps;
     const i64* cells; i64 cells_rows, cells_cols;
+    int cell_imax, cell_jmax;  // min(nx - 2, cells_rows - 1), min(ny - 2, cells_cols - 1)
     const i64* subcells; i64 n_subcells; i64 refine;
     const double* planes; int n_planes;
     double* hits; i64 capacity; i64* counts;
@@ -49,7 +50,7 @@ struct TraceParams {
     i64 nx, ny;
     i64 bofx, bofy, bofn;
     double box, boy, bh;
-    double inv_bh, inv_h;  // RN(1 / bh), RN(1 / h): divisions by the spacings via div_rn
+    double inv_bh, inv_h;  // RN(1 / bh), RN(1 / h): divisions by the spacings as products
     // deposit accumulator and work queue (workspace)
     u64* acc; int deposit_exponent; u64* queue;   // node words, fixed-point exponent, work queue
     double* charge;
@@ -81,7 +82,7 @@ struct AdjointParams {
     const double* x; const double* v; const double* qm;
     const double* fxs; const double* fys; i64 nx, ny;
     double origin_x, origin_y, h, step_fraction;
-    double inv_h;  // RN(1 / h)
+    double inv_h;  // RN(1 / h): divisions by the spacing as products
     const double* planes; int n_planes;
     const double* hits; i64 capacity; const i64* counts; const double* seeds;
     i64 interval, n_ckpt, lanes;
@@ -94,8 +95,8 @@ struct AdjointParams {
 int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom);
 int adjoint_occupancy(int integ, int geom, int block);
 
-int launch_division_selftest(const double* a, const double* b, i64 n, unsigned long long* count,
-                             hipStream_t stream);
+int launch_arith_selftest(const double* y, const double* z, i64 n, double* r, double* cy, double* cz, double* rs,
+                          hipStream_t stream);
 
 int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
                   i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,

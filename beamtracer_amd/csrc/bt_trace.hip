@@ -8,8 +8,10 @@
 // evaluations) or Boris's kick--drift--kick (1, one evaluation: the end field is the next
 // start's), deposits dq dt at the Hermite midpoint, records plane crossings and path
 // points, and tests the end point.  The arithmetic follows the reference statement by
-// statement in the same order; the library is built with -ffp-contract=off, so the
-// results are those of the CPU restatement bit for bit where the math library agrees.
+// statement in the same order (the library is built with -ffp-contract=off), except that
+// hypot, the square roots of the step-size rule and the divisions by r, |v|, |a| and the
+// mesh spacing become products with reciprocal square roots and reciprocals (bt_device.h):
+// a few units in the last place per operation, against the CPU restatement's 1e-10 bar.
 //
 // MI355X design:
 // * persistent grid, one trajectory per lane, refilled from a device work queue: a
@@ -45,20 +47,22 @@ struct LaneState {
     double ce0, ce1, ce2, cb0, cb1, cb2;  // with B: the last end's E and B
     i64 k, slot, ns;
     int have;
-    // the mesh coordinate across the axis and the cell coordinates of the current point,
-    // as the last step's end test computed them (valid when `point` is set)
-    double pt_y, pt_sx, pt_sy;
+    // the mesh coordinate across the axis, the radial direction and the cell coordinates of
+    // the current point, as the last step's end test computed them (valid when `point` is set)
+    double pt_y, pt_cy, pt_cz, pt_sx, pt_sy;
     int point;
     // the lane's deposit cache: exact fixed-point sums (units 2^e, |sum| < 2^46) for the four
-    // nodes of base-mesh cell (cci, ccj); a sum leaves the lane when the trajectory moves to
-    // a cell without that node, when it would reach 2^46, or when the trajectory ends
+    // nodes of base-mesh cell (cci, ccj), held as integer-valued doubles (exact below 2^53);
+    // slot (a << 1) | b holds the cell's node whose row and column have parities a and b, so a
+    // node the next cell shares keeps its slot.  A sum leaves the lane when the trajectory
+    // moves to a cell without that node, when it would reach 2^46, or when the trajectory ends
     int cci, ccj;
-    i64 cq[4];
+    double cq[4];
     i64 ck_next, ck_index;  // the step count of the next checkpoint and its slot
 };
 
 constexpr int kNoCell = -(1 << 30);
-constexpr i64 kCacheLimit = 1ll << 46;
+constexpr double kCacheLimit = 0x1p46;
 
 // Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
 __device__ __forceinline__ double node_bilinear(const double* __restrict__ f, double sx, double sy,
@@ -128,9 +132,9 @@ struct PatchView {
 template <bool GENERAL>
 __device__ __forceinline__ PatchView patch_view(const TraceParams& p, int pp) {
     PatchView v;
-    if (!GENERAL) {
+    if (!GENERAL) {  // the one patch is the mesh (same origin and spacing)
         v.nx = p.nx; v.ny = p.ny; v.ofx = p.bofx; v.ofy = p.bofy; v.ofn = p.bofn;
-        v.ox = p.box; v.oy = p.boy; v.h = p.bh;
+        v.ox = p.origin_x; v.oy = p.origin_y; v.h = p.h;
         return v;
     }
     const i64* pi = p.patch_int + (i64)pp * 6;
@@ -159,25 +163,22 @@ struct Fields {
 
 // E_x, E_y of the finest patch at the Cartesian point q, with the radial direction and
 // the mesh coordinates (pyccel.py:196-269).
-// With `known` (one patch whose origin and spacing are the mesh's), qy, sx and sy are the
-// values the end test of the last step computed for this same point: the same operations on
-// the same operands, so the same bits.  Otherwise they are computed and returned.
+// With `known` (one patch that is the mesh), qy, cy, cz, sx and sy are the values the end
+// test of the last step computed for this same point: the same operations on the same
+// operands, so the same bits.  Otherwise they are computed and returned.
 template <int GEOM, bool GENERAL>
 __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F, double q0, double q1, double q2,
                                            double& ex, double& ey, double& cy, double& cz,
                                            double& lq, double& qy, bool known, double& sx, double& sy) {
-    cy = 0.0; cz = 0.0;
-    if (!known) qy = q1;
-    if (GEOM == 1) {
-        const double r = known ? qy : hypot(q1, q2);
-        if (r > 0.0) {
-            const double ir = 1.0 / r;
-            cy = div_rn(q1, r, ir);
-            cz = div_rn(q2, r, ir);
+    if (!known) {
+        cy = 0.0; cz = 0.0; qy = q1;
+        if (GEOM == 1) {
+            const Radial rd = radial(q1, q2);
+            qy = rd.r; cy = rd.cy; cz = rd.cz;
         }
-        qy = r;
-    } else if (GEOM == 2) {
-        cy = 1.0;
+    }
+    if (GEOM == 2) {
+        cy = 1.0; cz = 0.0;
         if (q1 < p.origin_y) cy = -1.0;
         if (!known) qy = p.origin_y + fabs(q1 - p.origin_y);
     }
@@ -188,8 +189,8 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
             sx = (lq - v.ox) / v.h;
             sy = (qy - v.oy) / v.h;
         } else {
-            sx = div_rn(lq - v.ox, v.h, p.inv_bh);
-            sy = div_rn(qy - v.oy, v.h, p.inv_bh);
+            sx = (lq - v.ox) * p.inv_h;
+            sy = (qy - v.oy) * p.inv_h;
         }
     }
     ex = clamped_bilinear_buf(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
@@ -287,31 +288,28 @@ __device__ __forceinline__ void node_add(DepositWindow& win, int t, u64* __restr
     add_sum(acc + kWords * (ofn + ii * ny + jj), (u64)q);
 }
 
-// Moves the lane's cache to cell (i, j): the sums of nodes that both cells share move to
-// their new places, the others leave the lane.
+// The node of cell c (a row or column coordinate) with the given parity: c or c + 1.
+__device__ __forceinline__ int parity_node(int c, int parity) { return c + ((c ^ parity) & 1); }
+
+// An integer-valued double below 2^51 in magnitude as a signed 64-bit integer.
+__device__ __forceinline__ i64 exact_i64(double q) {
+    return __double_as_longlong(q + 0x1.8p52) - 0x4338000000000000ll;
+}
+
+// Moves the lane's cache to cell (i, j): the sums of nodes that both cells share stay in
+// their slots, the others leave the lane.
 template <bool USE_WIN>
 __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
                                            i64 ofn, i64 ny, int i, int j) {
     if (i == s.cci && j == s.ccj) return;
-    const int di = i - s.cci, dj = j - s.ccj;
-    i64 moved[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {  // new place r takes old place (ra + di, rb + dj)
-        const int sa = (r >> 1) + di, sb = (r & 1) + dj;
-        i64 v = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v = (sa == (q >> 1) && sb == (q & 1)) ? s.cq[q] : v;
-        moved[r] = v;
-    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int na = (q >> 1) - di, nb = (q & 1) - dj;
-        const bool kept = na >= 0 && na <= 1 && nb >= 0 && nb <= 1;
-        if (!kept && s.cq[q] != 0)
-            node_add<USE_WIN>(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
+        const int oi = parity_node(s.cci, q >> 1), oj = parity_node(s.ccj, q & 1);
+        if (oi != parity_node(i, q >> 1) || oj != parity_node(j, q & 1)) {
+            if (s.cq[q] != 0.0) node_add<USE_WIN>(win, t, acc, ofn, ny, oi, oj, exact_i64(s.cq[q]));
+            s.cq[q] = 0.0;
+        }
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) s.cq[r] = moved[r];
     s.cci = i;
     s.ccj = j;
 }
@@ -322,41 +320,48 @@ __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, in
                                             i64 ofn, i64 ny) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (s.cq[q] != 0) node_add<USE_WIN>(win, t, acc, ofn, ny, s.cci + (q >> 1), s.ccj + (q & 1), s.cq[q]);
-        s.cq[q] = 0;
+        if (s.cq[q] != 0.0)
+            node_add<USE_WIN>(win, t, acc, ofn, ny, parity_node(s.cci, q >> 1), parity_node(s.ccj, q & 1),
+                              exact_i64(s.cq[q]));
+        s.cq[q] = 0.0;
     }
     s.cci = kNoCell;
     s.ccj = kNoCell;
 }
 
-// The 4-node deposit of one step: nodes (i, j), (i, j + 1), (i + 1, j), (i + 1, j + 1) of a
-// patch whose nodes start at `ofn`.  Each value is rounded once to units of 2^e; on the base
-// mesh (in_base) the units gather in the lane's cache, elsewhere (or when a single deposit
-// reaches 2^46 units) they go to the node words in global memory.  The node sums are exact
-// integers, so the grouping does not change them.
+// The 4-node deposit of one step into cell (i, j) of a patch whose nodes start at `ofn`:
+// node (i + a, j + b) receives ((a ? u : 1 - u) (b ? w : 1 - w)) deposit, evaluated in that
+// order (pyccel.py:407-423).  Each value is rounded once to units of 2^e (ties to even); on
+// the base mesh (in_base) units below 2^45 gather in the lane's cache, elsewhere (or for a
+// larger single deposit) they go to the node words in global memory.  The node sums are
+// exact integers, so the grouping does not change them.
 template <bool USE_WIN>
 __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
-                                         i64 ofn, i64 ny, bool in_base, i64 design, i64 i, i64 j,
-                                         const double (&value)[4], int e) {
+                                         i64 ofn, i64 ny, bool in_base, i64 design, int i, int j,
+                                         double u, double w, double deposit, int e) {
     if (USE_WIN) window_note(win, t, in_base, design, i, j);
-    if (in_base) cache_move<USE_WIN>(s, win, t, acc, ofn, ny, (int)i, (int)j);
+    if (in_base) cache_move<USE_WIN>(s, win, t, acc, ofn, ny, i, j);
+    const double u0 = 1.0 - u, w0 = 1.0 - w;
+    const bool io = (i & 1) != 0, jo = (j & 1) != 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        u64 w0, w1;
-        if (!to_fixed(value[q], e, w0, w1)) continue;
-        const i64 ii = i + (q >> 1), jj = j + (q & 1);
-        if (in_base && (i64)w1 >= -(1ll << 14) && (i64)w1 < (1ll << 14)) {  // |units| < 2^46
-            const i64 units = (i64)((w1 << 32) + w0);
-            const i64 sum = s.cq[q] + units;
-            if (sum > -kCacheLimit && sum < kCacheLimit) {
+    for (int q = 0; q < 4; ++q) {  // slot q: the node of parities (q >> 1, q & 1)
+        const bool ia = io != ((q >> 1) != 0);  // the node is i + 1
+        const bool jb = jo != ((q & 1) != 0);   // the node is j + 1
+        const double value = ((ia ? u : u0) * (jb ? w : w0)) * deposit;
+        const double units = ldexp(value, -e);
+        if (in_base && fabs(units) < 0x1p45) {
+            const double rounded = (units + 0x1.8p52) - 0x1.8p52;  // to nearest, ties to even
+            const double sum = s.cq[q] + rounded;                   // exact: below 2^47
+            if (fabs(sum) < kCacheLimit) {
                 s.cq[q] = sum;
             } else {
-                node_add<USE_WIN>(win, t, acc, ofn, ny, ii, jj, s.cq[q]);
-                s.cq[q] = units;
+                node_add<USE_WIN>(win, t, acc, ofn, ny, i + ia, j + jb, exact_i64(s.cq[q]));
+                s.cq[q] = rounded;
             }
             continue;
         }
-        add_words(acc + kWords * (ofn + ii * ny + jj), w0, w1);
+        u64 lo, hi;
+        if (to_fixed(value, e, lo, hi)) add_words(acc + kWords * (ofn + (i + ia) * ny + (j + jb)), lo, hi);
     }
 }
 
@@ -427,7 +432,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     double hloc = p.h;
     if (GENERAL) {
         double rr = x1;
-        if (GEOM == 1) rr = hypot(x1, x2);
+        if (GEOM == 1) rr = radial(x1, x2).r;
         else if (GEOM == 2) rr = p.origin_y + fabs(x1 - p.origin_y);
         const double lx = locate_x<GENERAL>(p, x0);
         for (int qq = 0; qq < p.n_patches - 1; ++qq) {
@@ -447,7 +452,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     double w0 = 0.0, w1 = 0.0, w2 = 0.0, y0, y1, y2;
     double dt = 0.0, hdt = 0.0, dt6 = 0.0;
     double e0 = 0.0, e1 = 0.0, e2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    double end_y = 0.0, end_sx = 0.0, end_sy = 0.0;  // Boris: the end point's, from stage 1
+    double end_y = 0.0, end_cy = 0.0, end_cz = 0.0, end_sx = 0.0, end_sy = 0.0;  // Boris: the end point's
     constexpr int NSTAGE = INTEG == 0 ? 4 : 2;
 #pragma unroll
     for (int stage = 0; stage < NSTAGE; ++stage) {
@@ -465,10 +470,10 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
             }
         } else {
             double ex, ey, cy, cz, lq, qy, psx, psy;
-            const bool known = !GENERAL && p.reuse && stage == 0 && s.point != 0;
-            if (known) { qy = s.pt_y; psx = s.pt_sx; psy = s.pt_sy; }
+            const bool known = !GENERAL && stage == 0 && s.point != 0;
+            if (known) { qy = s.pt_y; cy = s.pt_cy; cz = s.pt_cz; psx = s.pt_sx; psy = s.pt_sy; }
             edge_field<GEOM, GENERAL>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy);
-            if (INTEG != 0 && stage == 1) { end_y = qy; end_sx = psx; end_sy = psy; }
+            if (INTEG != 0 && stage == 1) { end_y = qy; end_cy = cy; end_cz = cz; end_sx = psx; end_sy = psy; }
             if (GEOM != 0) {
                 g0 = qmk * ex;
                 g1 = qmk * (ey * cy);
@@ -491,18 +496,9 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         }
         if (stage == 0) {
             const double a0 = g0, a1 = g1, a2 = g2;
-            const double speed = gv * sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-            const double accel = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
-            dt = -1.0;
-            if (speed > 0.0) dt = hloc / speed;
-            if (accel > 0.0) {
-                const double d2 = sqrt(2.0 * hloc / accel);
-                if (dt < 0.0 || d2 < dt) dt = d2;
-            }
-            if (dt < 0.0) dt = hloc;
-            dt = p.step_fraction * dt;
+            dt = step_rule(v0 * v0 + v1 * v1 + v2 * v2, a0 * a0 + a1 * a1 + a2 * a2, hloc, p.step_fraction, gv).dt;
             hdt = 0.5 * dt;
-            dt6 = div_rn(dt, 6.0, 1.0 / 6.0);
+            dt6 = dt * (1.0 / 6.0);
             if (INTEG != 0) {
                 if (HAS_B) {
                     kx0 = v0; kx1 = v1; kx2 = v2;
@@ -565,16 +561,15 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     double m0 = 0.5 * (x0 + y0) + dt * (gv * v0 - gw * w0) / 8.0;
     double m1 = 0.5 * (x1 + y1) + dt * (gv * v1 - gw * w1) / 8.0;
     const double m2 = 0.5 * (x2 + y2) + dt * (gv * v2 - gw * w2) / 8.0;
-    if (GEOM == 1) m1 = hypot(m1, m2);
+    if (GEOM == 1) m1 = radial(m1, m2).r;
     else if (GEOM == 2) m1 = p.origin_y + fabs(m1 - p.origin_y);
     m0 = locate_x<GENERAL>(p, m0);
     {
         const PatchView pv = patch_view<GENERAL>(p, patch_of<GENERAL>(p, m0, m1));
-        // (x / h of the base patch through div_rn: the same bits as the division)
-        const double fi = floor(GENERAL ? (m0 - pv.ox) / pv.h : div_rn(m0 - pv.ox, pv.h, p.inv_bh));
-        const double fj = floor(GENERAL ? (m1 - pv.oy) / pv.h : div_rn(m1 - pv.oy, pv.h, p.inv_bh));
+        const double fi = floor(GENERAL ? (m0 - pv.ox) / pv.h : (m0 - pv.ox) * p.inv_h);
+        const double fj = floor(GENERAL ? (m1 - pv.oy) / pv.h : (m1 - pv.oy) * p.inv_h);
         const double du = m0 - (pv.ox + pv.h * fi);
-        double u = GENERAL ? du / pv.h : div_rn(du, pv.h, p.inv_bh);
+        double u = GENERAL ? du / pv.h : du * p.inv_h;
         double w;
         if (GEOM == 1) {
             const double low = pv.oy + pv.h * fj;
@@ -582,21 +577,18 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
             w = (m1 * m1 - low * low) / (high * high - low * low);
         } else {
             const double dw = m1 - (pv.oy + pv.h * fj);
-            w = GENERAL ? dw / pv.h : div_rn(dw, pv.h, p.inv_bh);
+            w = GENERAL ? dw / pv.h : dw * p.inv_h;
         }
-        i64 i, j;
+        int i, j;  // (a patch has fewer than 2^29 nodes)
         // !(f >= 0) also sends a NaN midpoint to the first node (its deposit is dropped)
         if (!(fi >= 0.0)) { u = 0.0; i = 0; }
-        else if (fi >= (double)(pv.nx - 1)) { u = 1.0; i = pv.nx - 2; }
-        else i = (i64)fi;
+        else if (fi >= (double)(pv.nx - 1)) { u = 1.0; i = (int)pv.nx - 2; }
+        else i = (int)fi;
         if (!(fj >= 0.0)) { w = 0.0; j = 0; }
-        else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = pv.ny - 2; }
-        else j = (i64)fj;
-        const double deposit = s.dqk * dt;
-        const double value[4] = {(1.0 - u) * (1.0 - w) * deposit, (1.0 - u) * w * deposit,
-                                 u * (1.0 - w) * deposit, u * w * deposit};
-        deposit4<USE_WIN>(s, win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, value,
-                          p.deposit_exponent);
+        else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = (int)pv.ny - 2; }
+        else j = (int)fj;
+        deposit4<USE_WIN>(s, win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
+                          s.dqk * dt, p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
     s.ns = ns;
@@ -629,11 +621,16 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     s.x0 = x0; s.x1 = x1; s.x2 = x2; s.v0 = v0; s.v1 = v1; s.v2 = v2;
 
     // the end of the step: off the mesh or in an electrode (pyccel.py:450-497)
-    const bool end_known = !GENERAL && p.reuse && INTEG != 0 && s.have != 0;
-    double px = x0, py = x1;
-    if (end_known) py = end_y;
-    else if (GEOM == 1) py = hypot(x1, x2);
-    else if (GEOM == 2) py = p.origin_y + fabs(x1 - p.origin_y);
+    const bool end_known = !GENERAL && INTEG != 0 && s.have != 0;
+    double px = x0, py = x1, pcy = 0.0, pcz = 0.0;
+    if (end_known) {
+        py = end_y; pcy = end_cy; pcz = end_cz;
+    } else if (GEOM == 1) {
+        const Radial rd = radial(x1, x2);
+        py = rd.r; pcy = rd.cy; pcz = rd.cz;
+    } else if (GEOM == 2) {
+        py = p.origin_y + fabs(x1 - p.origin_y);
+    }
     s.point = 0;
     if (s.slot >= 0) {
         const i64 c = p.path_count[s.slot];
@@ -652,26 +649,20 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     } else if (py > p.upper_y || (GEOM == 0 && py < p.origin_y)) {
         st = 2;
     } else {
-        const double ci = end_known ? end_sx
-                                    : (GENERAL ? (px - p.origin_x) / p.h : div_rn(px - p.origin_x, p.h, p.inv_h));
-        const double cj = end_known ? end_sy
-                                    : (GENERAL ? (py - p.origin_y) / p.h : div_rn(py - p.origin_y, p.h, p.inv_h));
-        s.pt_y = py; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
-        // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341
-        i64 i = (i64)fmax(ci, 0.0);
-        if (i > p.nx - 2) i = p.nx - 2;
-        if (i > p.cells_rows - 1) i = p.cells_rows - 1;
-        i64 j = (i64)fmax(cj, 0.0);
-        if (j > p.ny - 2) j = p.ny - 2;
-        if (j > p.cells_cols - 1) j = p.cells_cols - 1;
-        const i64 cell = p.cells[i * p.cells_cols + j];
+        const double ci = end_known ? end_sx : (GENERAL ? (px - p.origin_x) / p.h : (px - p.origin_x) * p.inv_h);
+        const double cj = end_known ? end_sy : (GENERAL ? (py - p.origin_y) / p.h : (py - p.origin_y) * p.inv_h);
+        s.pt_y = py; s.pt_cy = pcy; s.pt_cz = pcz; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
+        // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341: min(trunc(max(c, 0)),
+        // n - 2, cells - 1), taken as trunc(min(max(c, 0), that bound)) (the same integer for
+        // an integer bound, and a 32-bit conversion)
+        const int i = (int)fmin(fmax(ci, 0.0), (double)p.cell_imax);
+        const int j = (int)fmin(fmax(cj, 0.0), (double)p.cell_jmax);
+        const i64 cell = p.cells[(unsigned)i * (unsigned)p.cells_cols + (unsigned)j];
         if (cell >= 0) {
-            const i64 R = p.refine;
-            i64 si = (i64)((ci - (double)i) * (double)R);
-            if (si < 0) si = 0; else if (si > R - 1) si = R - 1;
-            i64 sj = (i64)((cj - (double)j) * (double)R);
-            if (sj < 0) sj = 0; else if (sj > R - 1) sj = R - 1;
-            const i64 at = (cell * R + si) * R + sj;
+            const double R = (double)p.refine;
+            const int si = (int)fmin(fmax((ci - (double)i) * R, 0.0), R - 1.0);
+            const int sj = (int)fmin(fmax((cj - (double)j) * R, 0.0), R - 1.0);
+            const i64 at = (cell * p.refine + si) * p.refine + sj;
             const i64 e = at < p.n_subcells ? p.subcells[at] : -1;
             if (e >= 0) {
                 st = 3;
@@ -706,6 +697,8 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
     lp.bofx = 0; lp.bofy = 0; lp.bofn = b.node_offset;
     lp.fx_base = b.fx_offset; lp.fy_base = b.fy_offset;
     lp.cells = p.cells + b.cells_offset; lp.cells_rows = b.cells_rows; lp.cells_cols = b.cells_cols;
+    lp.cell_imax = (int)(b.nx - 2 < b.cells_rows - 1 ? b.nx - 2 : b.cells_rows - 1);
+    lp.cell_jmax = (int)(b.ny - 2 < b.cells_cols - 1 ? b.ny - 2 : b.cells_cols - 1);
     lp.subcells = p.subcells + b.subcells_offset; lp.n_subcells = b.n_subcells;
     lp.planes = p.planes + (i64)d * p.n_planes;
     lp.deposit_exponent = b.deposit_exponent;
@@ -728,7 +721,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     LaneState s;
     s.k = -1;
     s.cci = s.ccj = kNoCell;
-    s.cq[0] = s.cq[1] = s.cq[2] = s.cq[3] = 0;
+    s.cq[0] = s.cq[1] = s.cq[2] = s.cq[3] = 0.0;
     bool exhausted = false;
     bool refill = true;  // with refill_block: only when every lane of the workgroup is free
     int t = 0;           // the window table of this round
@@ -953,26 +946,27 @@ int batched_occupancy(int integ, int geom, bool has_b, int block) {
     return blocks;
 }
 
-// count[0] += the number of k with div_rn(a[k], b[k], RN(1 / b[k])) != a[k] / b[k] (bits)
-__global__ void division_selftest_kernel(const double* __restrict__ a, const double* __restrict__ b, i64 n,
-                                         unsigned long long* __restrict__ count) {
+// The kernels' square-root arithmetic on arrays: radial(y[k], z[k]) -> r, cy, cz and
+// rs[k] = 1 / sqrt(y[k]) for y[k] > 0 (0 otherwise).
+__global__ void arith_selftest_kernel(const double* __restrict__ y, const double* __restrict__ z, i64 n,
+                                      double* __restrict__ r, double* __restrict__ cy, double* __restrict__ cz,
+                                      double* __restrict__ rs) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
-    unsigned long long bad = 0;
     for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const double y = 1.0 / b[k];
-        const double fast = div_rn(a[k], b[k], y);
-        const double ieee = a[k] / b[k];
-        if (__double_as_longlong(fast) != __double_as_longlong(ieee)) ++bad;
+        const Radial rd = radial(y[k], z[k]);
+        r[k] = rd.r;
+        cy[k] = rd.cy;
+        cz[k] = rd.cz;
+        rs[k] = y[k] > 0.0 ? rsqrt_pos(y[k]) : 0.0;
     }
-    if (bad) atomicAdd(count, bad);
 }
 
-int launch_division_selftest(const double* a, const double* b, i64 n, unsigned long long* count,
-                             hipStream_t stream) {
+int launch_arith_selftest(const double* y, const double* z, i64 n, double* r, double* cy, double* cz, double* rs,
+                          hipStream_t stream) {
     int grid = (int)((n + 255) / 256);
     if (grid > 8192) grid = 8192;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(division_selftest_kernel, dim3(grid), dim3(256), 0, stream, a, b, n, count);
+    hipLaunchKernelGGL(arith_selftest_kernel, dim3(grid), dim3(256), 0, stream, y, z, n, r, cy, cz, rs);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -231,10 +231,13 @@ typedef struct bt_adjoint_args {
                                                      NULL: library-owned, one call at a time */
 } bt_adjoint_args;
 
-/* Self-test of the kernels' division by a known reciprocal (no counterpart in the
- * reference): count[0] += the number of k (device arrays of n doubles) where it differs
- * in any bit from IEEE a[k] / b[k].  0 is the kernels' premise. */
-int bt_division_selftest(const double* a, const double* b, int64_t n, uint64_t* count, void* stream);
+/* Self-test of the kernels' square-root arithmetic (no counterpart in the reference): for
+ * device arrays of n doubles, the radial distance r[k] = |(y[k], z[k])| and direction
+ * cy[k] = y[k] / r[k], cz[k] = z[k] / r[k] (0 on the axis) as the tracer computes them from
+ * one reciprocal square root, and rs[k] = 1 / sqrt(y[k]) for y[k] > 0 (0 otherwise).  The
+ * tests bound their distance from the correctly rounded values in units in the last place. */
+int bt_arith_selftest(const double* y, const double* z, int64_t n, double* r, double* cy, double* cz, double* rs,
+                      void* stream);
 
 int64_t bt_adjoint_args_size(void);
 /* Scratch of an adjoint call: its queue head and checkpoint/segment states per lane. */

@@ -175,29 +175,43 @@ def test_trace_without_active_and_at_max_steps(hip):
         assert np.array_equal(done[i], expected[i])
 
 
-def test_division_by_a_known_reciprocal_is_the_ieee_division():
-    """The kernels divide mesh coordinates by the spacings (and the stage direction by r)
-    through div_rn with RN(1/b): every bit must be IEEE's (nonzero finite quotients)."""
-    import ctypes
+def _ulps(got, ref):
+    """|got - ref| in units in the last place of ref rounded to double (ref in long double)."""
+    spacing = np.spacing(np.abs(ref.astype(np.float64)))
+    spacing[spacing == 0] = np.finfo(np.float64).tiny
+    return (np.abs(got.astype(np.longdouble) - ref) / spacing.astype(np.longdouble)).astype(np.float64)
 
+
+def test_square_root_arithmetic_is_within_a_few_ulps():
+    """The tracer's hypot, y / r, z / r and 1 / sqrt through the hardware reciprocal square root
+    and two Newton steps: within 3 (r, y / r, z / r) and 2 (1 / sqrt) units in the last place
+    of the correctly rounded values (computed in long double), over mesh-scale coordinates, the
+    axis and points within 1e-150 of it."""
     import torch
 
     from beamtracer_amd import _lib
 
-    rng = np.random.default_rng(17)
-    n = 1 << 22
-    spacings = [1e-4, 5.859375e-5, 2.5e-4, 6.0] + [80e-3 / (k - 1) for k in range(201, 1026, 7)]
-    b = np.concatenate([rng.choice(spacings, n // 2), 10.0 ** rng.uniform(-7, -1.5, n // 2)])
-    a = rng.uniform(-0.08, 0.08, n)
-    # numerators that land next to cell boundaries: k h (1 + tiny)
-    near = n // 4
-    k = rng.integers(-1000, 1000, near).astype(float)
-    a[:near] = k * b[:near] * (1.0 + rng.normal(size=near) * 1e-15)
-    a[a == 0.0] = 1e-3
-    da = torch.from_numpy(a).cuda()
-    db = torch.from_numpy(b).cuda()
-    count = torch.zeros(1, dtype=torch.int64, device="cuda")
-    rc = _lib.lib().bt_division_selftest(da.data_ptr(), db.data_ptr(), n, count.data_ptr(), None)
+    rng = np.random.default_rng(23)
+    n = 1 << 20
+    y = rng.uniform(-0.1, 0.1, n) * 10.0 ** rng.uniform(-6, 3, n)
+    z = rng.uniform(-0.1, 0.1, n) * 10.0 ** rng.uniform(-6, 3, n)
+    y[:1000], z[:1000] = 0.0, 0.0                   # on the axis
+    y[1000:2000] = rng.uniform(-1, 1, 1000) * 1e-160  # next to it
+    z[1000:2000] = rng.uniform(-1, 1, 1000) * 1e-160
+    z[2000:3000] = 0.0
+    dy, dz = torch.from_numpy(y).cuda(), torch.from_numpy(z).cuda()
+    out = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(4)]
+    rc = _lib.lib().bt_arith_selftest(dy.data_ptr(), dz.data_ptr(), n, *(o.data_ptr() for o in out), None)
     assert rc == 0
-    torch.cuda.synchronize()
-    assert int(count.item()) == 0
+    r, cy, cz, rs = (o.cpu().numpy() for o in out)
+    yl, zl = y.astype(np.longdouble), z.astype(np.longdouble)
+    rl = np.sqrt(yl * yl + zl * zl)
+    axis = rl == 0
+    assert (r[axis] == 0).all() and (cy[axis] == 0).all() and (cz[axis] == 0).all()
+    off = ~axis
+    assert _ulps(r[off], rl[off]).max() <= 3
+    assert _ulps(cy[off], yl[off] / rl[off]).max() <= 3
+    assert _ulps(cz[off], zl[off] / rl[off]).max() <= 3
+    pos = y > 0
+    assert _ulps(rs[pos], 1 / np.sqrt(yl[pos])).max() <= 2
+    assert (rs[~pos] == 0).all()
