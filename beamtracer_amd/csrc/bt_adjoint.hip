@@ -33,7 +33,10 @@ struct Vec3 {
 };
 
 __device__ __forceinline__ Vec3 vadd(Vec3 x, Vec3 y) { return {x.a + y.a, x.b + y.b, x.c + y.c}; }
-__device__ __forceinline__ Vec3 vaxpy(Vec3 x, double s, Vec3 y) { return {x.a + s * y.a, x.b + s * y.b, x.c + s * y.c}; }
+// x + s y, fused (the trace kernel's stage updates)
+__device__ __forceinline__ Vec3 vaxpy(Vec3 x, double s, Vec3 y) {
+    return {__builtin_fma(s, y.a, x.a), __builtin_fma(s, y.b, x.b), __builtin_fma(s, y.c, x.c)};
+}
 __device__ __forceinline__ Vec3 vscale(double s, Vec3 y) { return {s * y.a, s * y.b, s * y.c}; }
 __device__ __forceinline__ double vdot(Vec3 x, Vec3 y) { return x.a * y.a + x.b * y.b + x.c * y.c; }
 
@@ -73,19 +76,20 @@ struct Gathered {
 };
 
 __device__ __forceinline__ Gathered gather_d(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
-    const int i = clamped_floor(s, 0, imax);
-    const int j = clamped_floor(t, 0, jmax);
+    const double fi = fmin(fmax(floor(s), 0.0), (double)imax);
+    const double fj = fmin(fmax(floor(t), 0.0), (double)jmax);
     Gathered g;
-    g.i = i;
-    g.j = j;
-    g.u = s - (double)i;
-    g.w = t - (double)j;
-    g.c = (unsigned)i * (unsigned)cols + (unsigned)j;
+    g.i = (int)fi;
+    g.j = (int)fj;
+    g.u = s - fi;
+    g.w = t - fj;
+    g.c = (unsigned)g.i * (unsigned)cols + (unsigned)g.j;
     const Pair lo = load_pair(f, g.c), hi = load_pair(f, g.c + (unsigned)cols);
     const double f00 = lo.a, f01 = lo.b, f10 = hi.a, f11 = hi.b;
-    const double low = f00 + g.w * (f01 - f00);
-    const double high = f10 + g.w * (f11 - f10);
-    g.value = low + g.u * (high - low);
+    // (clamped_bilinear_buf's arithmetic)
+    const double low = __builtin_fma(g.w, f01 - f00, f00);
+    const double high = __builtin_fma(g.w, f11 - f10, f10);
+    g.value = __builtin_fma(g.u, high - low, low);
     g.ds = high - low;
     g.dt = (1.0 - g.u) * (f01 - f00) + g.u * (f11 - f10);
     return g;

@@ -504,63 +504,62 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
                     kx0 = v0; kx1 = v1; kx2 = v2;
                     boris_kick(kx0, kx1, kx2, e0, e1, e2, b0, b1, b2, hdt, qmk, GENERAL ? inv_c2 : 0.0);
                 } else {
-                    kx0 = v0 + hdt * a0;
-                    kx1 = v1 + hdt * a1;
-                    kx2 = v2 + hdt * a2;
+                    kx0 = __builtin_fma(hdt, a0, v0);
+                    kx1 = __builtin_fma(hdt, a1, v1);
+                    kx2 = __builtin_fma(hdt, a2, v2);
                 }
                 gk = 1.0 / lorentz<GENERAL>(kx0, kx1, kx2, inv_c2);
-                q0 = x0 + dt * gk * kx0;
-                q1 = x1 + dt * gk * kx1;
-                q2 = x2 + dt * gk * kx2;
+                q0 = __builtin_fma(dt * gk, kx0, x0);
+                q1 = __builtin_fma(dt * gk, kx1, x1);
+                q2 = __builtin_fma(dt * gk, kx2, x2);
                 continue;
             }
+            // (the stage updates are fused multiply-adds, as the adjoint's recompute)
             sx0 = gv * v0; sx1 = gv * v1; sx2 = gv * v2;
             sv0 = a0; sv1 = a1; sv2 = a2;
-            q0 = x0 + hdt * gv * v0;
-            q1 = x1 + hdt * gv * v1;
-            q2 = x2 + hdt * gv * v2;
-            kx0 = v0 + hdt * a0;
-            kx1 = v1 + hdt * a1;
-            kx2 = v2 + hdt * a2;
+            q0 = __builtin_fma(hdt * gv, v0, x0);
+            q1 = __builtin_fma(hdt * gv, v1, x1);
+            q2 = __builtin_fma(hdt * gv, v2, x2);
+            kx0 = __builtin_fma(hdt, a0, v0);
+            kx1 = __builtin_fma(hdt, a1, v1);
+            kx2 = __builtin_fma(hdt, a2, v2);
         } else if (INTEG != 0) {
             if (HAS_B) {
                 w0 = kx0; w1 = kx1; w2 = kx2;
                 boris_kick(w0, w1, w2, e0, e1, e2, b0, b1, b2, hdt, qmk, GENERAL ? inv_c2 : 0.0);
                 s.ce0 = e0; s.ce1 = e1; s.ce2 = e2; s.cb0 = b0; s.cb1 = b1; s.cb2 = b2;
             } else {
-                w0 = kx0 + hdt * g0;
-                w1 = kx1 + hdt * g1;
-                w2 = kx2 + hdt * g2;
+                w0 = __builtin_fma(hdt, g0, kx0);
+                w1 = __builtin_fma(hdt, g1, kx1);
+                w2 = __builtin_fma(hdt, g2, kx2);
                 s.ca0 = g0; s.ca1 = g1; s.ca2 = g2;
             }
             s.have = 1;
-        } else if (stage == 1) {
-            sx0 = sx0 + 2.0 * gk * kx0; sx1 = sx1 + 2.0 * gk * kx1; sx2 = sx2 + 2.0 * gk * kx2;
-            sv0 = sv0 + 2.0 * g0; sv1 = sv1 + 2.0 * g1; sv2 = sv2 + 2.0 * g2;
-            q0 = x0 + hdt * gk * kx0; q1 = x1 + hdt * gk * kx1; q2 = x2 + hdt * gk * kx2;
-            kx0 = v0 + hdt * g0; kx1 = v1 + hdt * g1; kx2 = v2 + hdt * g2;
-        } else if (stage == 2) {
-            sx0 = sx0 + 2.0 * gk * kx0; sx1 = sx1 + 2.0 * gk * kx1; sx2 = sx2 + 2.0 * gk * kx2;
-            sv0 = sv0 + 2.0 * g0; sv1 = sv1 + 2.0 * g1; sv2 = sv2 + 2.0 * g2;
-            q0 = x0 + dt * gk * kx0; q1 = x1 + dt * gk * kx1; q2 = x2 + dt * gk * kx2;
-            kx0 = v0 + dt * g0; kx1 = v1 + dt * g1; kx2 = v2 + dt * g2;
+        } else if (stage == 1 || stage == 2) {
+            const double c = stage == 1 ? hdt : dt;
+            sx0 = __builtin_fma(2.0 * gk, kx0, sx0); sx1 = __builtin_fma(2.0 * gk, kx1, sx1);
+            sx2 = __builtin_fma(2.0 * gk, kx2, sx2);
+            sv0 = __builtin_fma(2.0, g0, sv0); sv1 = __builtin_fma(2.0, g1, sv1); sv2 = __builtin_fma(2.0, g2, sv2);
+            q0 = __builtin_fma(c * gk, kx0, x0); q1 = __builtin_fma(c * gk, kx1, x1); q2 = __builtin_fma(c * gk, kx2, x2);
+            kx0 = __builtin_fma(c, g0, v0); kx1 = __builtin_fma(c, g1, v1); kx2 = __builtin_fma(c, g2, v2);
         } else {
-            sx0 = sx0 + gk * kx0; sx1 = sx1 + gk * kx1; sx2 = sx2 + gk * kx2;
+            sx0 = __builtin_fma(gk, kx0, sx0); sx1 = __builtin_fma(gk, kx1, sx1); sx2 = __builtin_fma(gk, kx2, sx2);
             sv0 = sv0 + g0; sv1 = sv1 + g1; sv2 = sv2 + g2;
         }
     }
     if (INTEG == 0) {
-        y0 = x0 + dt6 * sx0; y1 = x1 + dt6 * sx1; y2 = x2 + dt6 * sx2;
-        w0 = v0 + dt6 * sv0; w1 = v1 + dt6 * sv1; w2 = v2 + dt6 * sv2;
+        y0 = __builtin_fma(dt6, sx0, x0); y1 = __builtin_fma(dt6, sx1, x1); y2 = __builtin_fma(dt6, sx2, x2);
+        w0 = __builtin_fma(dt6, sv0, v0); w1 = __builtin_fma(dt6, sv1, v1); w2 = __builtin_fma(dt6, sv2, v2);
     } else {
         y0 = q0; y1 = q1; y2 = q2;
     }
 
     // space charge: dq dt at the Hermite midpoint, area-weighted (pyccel.py:372-423)
     const double gw = 1.0 / lorentz<GENERAL>(w0, w1, w2, inv_c2);
-    double m0 = 0.5 * (x0 + y0) + dt * (gv * v0 - gw * w0) / 8.0;
-    double m1 = 0.5 * (x1 + y1) + dt * (gv * v1 - gw * w1) / 8.0;
-    const double m2 = 0.5 * (x2 + y2) + dt * (gv * v2 - gw * w2) / 8.0;
+    const double dt8 = dt * 0.125;
+    double m0 = __builtin_fma(dt8, gv * v0 - gw * w0, 0.5 * (x0 + y0));
+    double m1 = __builtin_fma(dt8, gv * v1 - gw * w1, 0.5 * (x1 + y1));
+    const double m2 = __builtin_fma(dt8, gv * v2 - gw * w2, 0.5 * (x2 + y2));
     if (GEOM == 1) m1 = radial(m1, m2).r;
     else if (GEOM == 2) m1 = p.origin_y + fabs(m1 - p.origin_y);
     m0 = locate_x<GENERAL>(p, m0);
