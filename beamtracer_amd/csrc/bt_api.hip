@@ -37,6 +37,7 @@ int hip_fail(hipError_t err, const char* what) {
 
 constexpr int kMaxPatches = 16;
 constexpr int kMaxPlanes = 256;
+constexpr i64 kMax32 = (i64)1 << 31;  // bound of the 32-bit launch parameters
 constexpr size_t kTableBytes = 4096;
 
 size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -109,6 +110,11 @@ int check_trace(const bt_trace2d_args* a) {
     if (a->refine < 1) return fail(BT_E_ARG, "refine must be positive");
     if (a->capacity < 0 || a->path_capacity < 0 || a->n_paths < 0)
         return fail(BT_E_ARG, "negative buffer size");
+    if (a->n >= kMax32 || a->capacity >= kMax32 || a->path_capacity >= kMax32 || a->n_paths >= kMax32 ||
+        a->refine >= (1 << 15) || a->n_subcells >= kMax32 || a->charge_cols >= kMax32)
+        return fail(BT_E_ARG, "trajectory, buffer, sub-cell and node counts must stay below 2^31");
+    if (a->checkpoints != nullptr && (a->n_checkpoints >= kMax32 || a->checkpoint_interval >= kMax32))
+        return fail(BT_E_ARG, "checkpoint counts must stay below 2^31");
     if (a->patch_int == nullptr || a->patch_real == nullptr)
         return fail(BT_E_ARG, "patch tables are NULL");
     if (a->n_planes > 0 && a->planes == nullptr) return fail(BT_E_ARG, "planes is NULL");
@@ -141,7 +147,7 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     bt::TraceParams p;
     std::memset(&p, 0, sizeof(p));
     p.active = a->active; p.n_active = a->n_active;
-    p.x = a->x; p.v = a->v; p.n = a->n;
+    p.x = a->x; p.v = a->v; p.n = (int)a->n;
     p.qm = a->qm; p.dq = a->dq;
     p.steps = (i64*)a->steps; p.status = (i64*)a->status; p.electrode = (i64*)a->electrode;
     p.fxs = a->fxs; p.fys = a->fys;
@@ -150,29 +156,29 @@ bt::TraceParams make_params(const bt_trace2d_args* a) {
     p.origin_x = a->origin_x; p.origin_y = a->origin_y; p.h = a->h;
     p.upper_x = a->upper_x; p.upper_y = a->upper_y;
     p.step_fraction = a->step_fraction; p.max_steps = a->max_steps;
-    p.cells = (const i64*)a->cells; p.cells_rows = a->cells_rows; p.cells_cols = a->cells_cols;
-    p.subcells = (const i64*)a->subcells; p.n_subcells = a->n_subcells; p.refine = a->refine;
+    p.cells = (const i64*)a->cells; p.cells_rows = (int)a->cells_rows; p.cells_cols = (int)a->cells_cols;
+    p.subcells = (const i64*)a->subcells; p.n_subcells = (int)a->n_subcells; p.refine = (int)a->refine;
     p.n_planes = (int)a->n_planes;
-    p.hits = a->hits; p.capacity = a->capacity; p.counts = (i64*)a->counts;
-    p.path_slot = (const i64*)a->path_slot; p.paths = a->paths; p.n_paths = a->n_paths;
-    p.path_capacity = a->path_capacity; p.path_count = (i64*)a->path_count;
+    p.hits = a->hits; p.capacity = (int)a->capacity; p.counts = (i64*)a->counts;
+    p.path_slot = (const i64*)a->path_slot; p.paths = a->paths; p.n_paths = (int)a->n_paths;
+    p.path_capacity = (int)a->path_capacity; p.path_count = (i64*)a->path_count;
     p.graded = (int)a->graded; p.x_nodes = a->x_nodes; p.n_x_nodes = a->n_x_nodes;
     p.inv_c2 = a->inv_c2;
     if (a->has_b) { p.b0 = a->b0; p.b1 = a->b1; p.b2 = a->b2; }
     const int64_t* base = a->patch_int + (a->n_patches - 1) * 6;
     const double* basr = a->patch_real + (a->n_patches - 1) * 7;
-    p.nx = base[0]; p.ny = base[1];
-    p.bofx = base[2]; p.bofy = base[3]; p.bofn = base[5];
+    p.nx = (int)base[0]; p.ny = (int)base[1];
+    p.bofx = (int)base[2]; p.bofy = (int)base[3]; p.bofn = (int)base[5];
     p.box = basr[0]; p.boy = basr[1]; p.bh = basr[2];
     p.inv_bh = 1.0 / basr[2];
     p.inv_h = 1.0 / a->h;
     p.deposit_exponent = a->deposit_exponent;
-    p.cell_imax = (int)std::min(p.nx - 2, p.cells_rows - 1);
-    p.cell_jmax = (int)std::min(p.ny - 2, p.cells_cols - 1);
+    p.cell_imax = std::min(p.nx - 2, p.cells_rows - 1);
+    p.cell_jmax = std::min(p.ny - 2, p.cells_cols - 1);
     p.reuse = a->n_patches == 1 && basr[0] == a->origin_x && basr[1] == a->origin_y && basr[2] == a->h;
     p.refill_block = (a->flags & BT_REFILL_BLOCK) != 0;
     if (a->checkpoints != nullptr && a->n_checkpoints > 0 && a->checkpoint_interval > 0) {
-        p.ckpt = a->checkpoints; p.n_ckpt = a->n_checkpoints; p.ckpt_interval = a->checkpoint_interval;
+        p.ckpt = a->checkpoints; p.n_ckpt = (int)a->n_checkpoints; p.ckpt_interval = (int)a->checkpoint_interval;
     }
     return p;
 }
@@ -296,6 +302,8 @@ int check_designs(const bt_batch_design* designs, i64 n_designs, i64 n_fxs, i64 
         if (b.n_subcells < 1 || b.subcells_offset < 0 || b.subcells_offset + b.n_subcells > n_subcells)
             return fail(BT_E_ARG, at + "subcells beyond the sub-cell arena");
         if (b.n_nodes != b.nx * b.ny || b.node_offset < 0) return fail(BT_E_ARG, at + "n_nodes must be nx * ny");
+        if (b.node_offset + b.n_nodes >= kMax32 || b.n_subcells >= kMax32)
+            return fail(BT_E_ARG, at + "node and sub-cell counts must stay below 2^31");
         if (b.node_offset + b.n_nodes > total) total = b.node_offset + b.n_nodes;
         if (b.n_nodes > most) most = b.n_nodes;
     }
@@ -313,6 +321,8 @@ int check_batch(const BatchPlan* plan, const bt_batch_args* a) {
     if (a->integrator < 0 || a->integrator > 1) return fail(BT_E_ARG, "integrator must be 0 or 1");
     if (a->refine < 1) return fail(BT_E_ARG, "refine must be positive");
     if (a->capacity < 0) return fail(BT_E_ARG, "negative capacity");
+    if (a->n >= kMax32 || a->capacity >= kMax32 || a->refine >= (1 << 15))
+        return fail(BT_E_ARG, "trajectory and buffer counts must stay below 2^31");
     if (a->block_size != 0 && (a->block_size % 64 != 0 || a->block_size < 64 || a->block_size > 256))
         return fail(BT_E_ARG, "block_size must be a multiple of 64 in 64..256");
     if ((a->x0 == nullptr) != (a->v0 == nullptr)) return fail(BT_E_ARG, "x0 and v0 go together");
@@ -333,22 +343,22 @@ bt::TraceParams batch_params(const BatchPlan* plan, const bt_batch_args* a) {
     bt::TraceParams p;
     std::memset(&p, 0, sizeof(p));
     p.active = a->active; p.n_active = a->n_active;
-    p.x = a->x; p.v = a->v; p.n = a->n;
+    p.x = a->x; p.v = a->v; p.n = (int)a->n;
     p.qm = a->qm; p.dq = a->dq;
     p.steps = (i64*)a->steps; p.status = (i64*)a->status; p.electrode = (i64*)a->electrode;
     p.fxs = a->fxs; p.fys = a->fys;
     p.n_fxs = plan->n_fxs; p.n_fys = plan->n_fys;
     p.n_patches = 1;
     p.step_fraction = a->step_fraction; p.max_steps = a->max_steps;
-    p.cells = (const i64*)a->cells; p.subcells = (const i64*)a->subcells; p.refine = a->refine;
+    p.cells = (const i64*)a->cells; p.subcells = (const i64*)a->subcells; p.refine = (int)a->refine;
     p.planes = a->planes; p.n_planes = (int)a->n_planes;
-    p.hits = a->hits; p.capacity = a->capacity; p.counts = (i64*)a->counts;
+    p.hits = a->hits; p.capacity = (int)a->capacity; p.counts = (i64*)a->counts;
     p.reuse = 1;  // every design is one uniform patch: the mesh itself
     char* base = static_cast<char*>(plan->workspace);
     p.acc = reinterpret_cast<u64*>(base);
     p.queue = reinterpret_cast<u64*>(base + node_bytes(plan->total_nodes));
     p.charge = a->charge;
-    p.designs = plan->d_designs; p.design_of = a->design_of; p.n_designs = (i64)plan->designs.size();
+    p.designs = plan->d_designs; p.design_of = a->design_of; p.n_designs = (int)plan->designs.size();
     return p;
 }
 

@@ -22,33 +22,34 @@ struct BatchDesign {
 };
 
 // Device pointers and scalars of one trace launch (see bt_trace2d_args in
-// include/beamtracer_hip.h for the meaning of each).
+// include/beamtracer_hip.h for the meaning of each).  Counts, sizes and offsets that the
+// entry points bound below 2^31 are 32-bit: the step loop holds them in scalar registers.
 struct TraceParams {
     const i64* active; i64 n_active;
-    double* x; double* v; i64 n;
+    double* x; double* v; int n;
     const double* qm; const double* dq;
     i64* steps; i64* status; i64* electrode;
     const double* fxs; const double* fys;
     i64 n_fxs, n_fys;      // doubles in fxs, fys (read through 32-bit buffer offsets)
-    i64 fx_base, fy_base;  // added to every patch offset (a batched lane's design arenas)
+    int fx_base, fy_base;  // added to every patch offset (a batched lane's design arenas)
     const i64* patch_int; const double* patch_real; int n_patches;
     double origin_x, origin_y, h, upper_x, upper_y;
     double step_fraction; i64 max_steps;
-    const i64* cells; i64 cells_rows, cells_cols;
+    const i64* cells; int cells_rows, cells_cols;
     int cell_imax, cell_jmax;  // min(nx - 2, cells_rows - 1), min(ny - 2, cells_cols - 1)
-    const i64* subcells; i64 n_subcells; i64 refine;
+    const i64* subcells; int n_subcells; int refine;
     const double* planes; int n_planes;
-    double* hits; i64 capacity; i64* counts;
-    const i64* path_slot; double* paths; i64 n_paths; i64 path_capacity; i64* path_count;
+    double* hits; int capacity; i64* counts;
+    const i64* path_slot; double* paths; int n_paths; int path_capacity; i64* path_count;
     int graded; const double* x_nodes; i64 n_x_nodes;
     int reuse;   // one patch with the mesh's origin and spacing: end-test values carry over
     int refill_block;  // refill a workgroup's lanes together (BT_REFILL_BLOCK)
-    double* ckpt; i64 n_ckpt, ckpt_interval;  // checkpoints [c][6][n] of the states, or null
+    double* ckpt; int n_ckpt, ckpt_interval;  // checkpoints [c][6][n] of the states, or null
     double inv_c2;
     const double* b0; const double* b1; const double* b2;
     // base (last) patch = the mesh; filled on the host
-    i64 nx, ny;
-    i64 bofx, bofy, bofn;
+    int nx, ny;
+    int bofx, bofy, bofn;
     double box, boy, bh;
     double inv_bh, inv_h;  // RN(1 / bh), RN(1 / h): divisions by the spacings as products
     // deposit accumulator and work queue (workspace)
@@ -56,8 +57,8 @@ struct TraceParams {
     double* charge;
     // batched form: the design table and every trajectory's design (device), else null;
     // `design` is the design whose mesh these parameters describe (0 for one mesh)
-    const BatchDesign* designs; const int32_t* design_of; i64 n_designs;
-    i64 design;
+    const BatchDesign* designs; const int32_t* design_of; int n_designs;
+    int design;
 };
 
 typedef void (*TraceKernel)(TraceParams);

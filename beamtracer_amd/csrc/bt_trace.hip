@@ -688,17 +688,17 @@ __device__ __forceinline__ void store_end(const TraceParams& p, const LaneState&
 // at its node offset in the accumulator).
 __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& p, int d) {
     const BatchDesign& b = p.designs[d];
-    lp.nx = b.nx; lp.ny = b.ny;
+    lp.nx = (int)b.nx; lp.ny = (int)b.ny;
     lp.origin_x = b.origin_x; lp.origin_y = b.origin_y; lp.h = b.h;
     lp.upper_x = b.upper_x; lp.upper_y = b.upper_y;
     lp.box = b.origin_x; lp.boy = b.origin_y; lp.bh = b.h;
     lp.inv_bh = 1.0 / b.h; lp.inv_h = lp.inv_bh;
-    lp.bofx = 0; lp.bofy = 0; lp.bofn = b.node_offset;
-    lp.fx_base = b.fx_offset; lp.fy_base = b.fy_offset;
-    lp.cells = p.cells + b.cells_offset; lp.cells_rows = b.cells_rows; lp.cells_cols = b.cells_cols;
+    lp.bofx = 0; lp.bofy = 0; lp.bofn = (int)b.node_offset;
+    lp.fx_base = (int)b.fx_offset; lp.fy_base = (int)b.fy_offset;
+    lp.cells = p.cells + b.cells_offset; lp.cells_rows = (int)b.cells_rows; lp.cells_cols = (int)b.cells_cols;
     lp.cell_imax = (int)(b.nx - 2 < b.cells_rows - 1 ? b.nx - 2 : b.cells_rows - 1);
     lp.cell_jmax = (int)(b.ny - 2 < b.cells_cols - 1 ? b.ny - 2 : b.cells_cols - 1);
-    lp.subcells = p.subcells + b.subcells_offset; lp.n_subcells = b.n_subcells;
+    lp.subcells = p.subcells + b.subcells_offset; lp.n_subcells = (int)b.n_subcells;
     lp.planes = p.planes + (i64)d * p.n_planes;
     lp.deposit_exponent = b.deposit_exponent;
     lp.design = d;
