@@ -199,17 +199,19 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 }
 
 // ---- the deposit window ---------------------------------------------------------------
-// The workgroup's deposits go into a window of node sums in LDS that follows its beam: a
-// ring of kWinCols column slots (column i in slot i mod kWinCols), each holding kWinRows
-// rows from a base row.  A slot's tag is the accumulator index of its column's row 0
-// (the patch's node offset + i ny), which also names the design in a batched launch, so
-// lanes of other meshes never match it.  Lanes add with LDS integer atomics; a sum outside
-// the window goes to global memory directly.  Between steps the workgroup flushes the
-// slots of columns it has left (one global atomic per word of each node) and claims the
-// columns around the current cells of its lowest design.  All waves of the workgroup step
-// in lockstep: deposits and flushes are separated by barriers, and the slot tables are
-// double-buffered (deposits of step k read table k mod 2 while the maintenance after step
-// k writes table k + 1 mod 2), so no deposit can land in a slot that is being flushed.
+// The workgroup's deposits go into a window of node sums in LDS that follows its beam:
+// columns lo..hi (at most kWinCols; column i in slot i mod kWinCols) of one design's mesh,
+// rows base..base + kWinRows - 1.  The placement is workgroup-uniform for a round and held
+// in registers, so a lane tests a node against it with compares and adds with an LDS
+// integer atomic; a node outside goes to global memory directly.  At the end of each round
+// of kRoundSteps steps every wave notes the cells its lanes' caches hold, and the workgroup
+// places the window around the noted cells of its lowest design (with room for kRoundSteps
+// steps of travel), flushes the slots of the columns it leaves (one global atomic per word
+// of each node; every slot when the base row or the design changes, which a margin keeps
+// rare) and writes the next round's placement.  Deposits and the maintenance are separated
+// by barriers and the placement is double-buffered (round k reads table k mod 2, the
+// maintenance after round k writes table k + 1 mod 2), so no deposit lands in a slot that
+// is being flushed.
 
 constexpr int kWinCols = 16;
 constexpr int kWinRows = 64;
@@ -224,72 +226,73 @@ static_assert(kFlushRounds * kRoundSteps <= 32, "window slots must stay below 2^
 // kRoundSteps steps of at most 1024 lanes, and every slot is flushed at least every
 // kFlushRounds rounds: |sum| < kFlushRounds * kRoundSteps * 1024 * 2 * 2^46 = 2^62.
 struct DepositWindow {
-    i64 tag[2][kWinCols];             // accumulator index of the slot's column at row 0, -1: none
-    int base[2][kWinCols];            // its first row
-    long long key[2][2];              // min / max placement key of the round's deposits
-    int rows[2][2];                   // min / max row of the round's deposits
+    int lo[2], hi[2], base[2], design[2];  // the placement of table t (hi < lo: no columns)
+    i64 ofn[2];                            // its design's node offset and rows (flush addresses)
+    int ny[2];
+    long long key[2][2];                   // min / max placement key noted in the round
+    int rows[2][2];                        // min / max row noted in the round
     u64 word[kWinCols][kWinRows];
 };
+
+// A round's placement, read once into registers.
+struct WinView {
+    int lo, hi, base, design;
+};
+
+__device__ __forceinline__ WinView window_view(const DepositWindow& w, int t) {
+    return {w.lo[t], w.hi[t], w.base[t], w.design[t]};
+}
 
 // Splits a slot's signed sum q into the two node words (low 32 bits, q >> 32).
 __device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
     add_words(w, q & 0xffffffffull, (u64)((i64)q >> 32));
 }
 
-__device__ __forceinline__ void window_reset_bounds(DepositWindow& w, int t) {
+__device__ __forceinline__ void window_reset_notes(DepositWindow& w, int t) {
     w.key[t][0] = 0x7fffffffffffffffll;
     w.key[t][1] = -1;
     w.rows[t][0] = 0x7fffffff;
     w.rows[t][1] = -1;
 }
 
-__device__ __forceinline__ void window_init(DepositWindow& w) {
+__device__ __forceinline__ void window_init(DepositWindow& w, const TraceParams& p) {
     for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) (&w.word[0][0])[k] = 0ull;
-    if (threadIdx.x < kWinCols) {
-        w.tag[0][threadIdx.x] = -1;
-        w.tag[1][threadIdx.x] = -1;
-        w.base[0][threadIdx.x] = 0;
-        w.base[1][threadIdx.x] = 0;
-    }
-    if (threadIdx.x == 0) {
-        window_reset_bounds(w, 0);
-        window_reset_bounds(w, 1);
+    if (threadIdx.x < 2) {
+        const int t = threadIdx.x;
+        w.lo[t] = 0; w.hi[t] = -1; w.base[t] = 0; w.design[t] = 0;
+        w.ofn[t] = p.bofn; w.ny[t] = p.ny;
+        window_reset_notes(w, t);
     }
 }
 
-// Records the base-mesh cell (i, j) of a lane's deposit for the window's next placement:
-// the first and last such lane of each wave report (the bounds only place the window; a
-// sum outside it goes to global memory).
-__device__ __forceinline__ void window_note(DepositWindow& win, int t, bool in_base, i64 design, i64 i, i64 j) {
-    const u64 lanes = __ballot(in_base);
+// The end of a round: the first and last lane of each wave whose cache holds a cell report
+// it (design, column, row) for the next placement (the notes only place the window; a sum
+// outside it goes to global memory).
+__device__ __forceinline__ void window_note(DepositWindow& win, int t, const LaneState& s, int design) {
+    const bool has = s.cci != kNoCell;
+    const u64 lanes = __ballot(has);
     const int me = (int)__lane_id();
-    if (in_base && (me == __ffsll((long long)lanes) - 1 || me == 63 - __clzll((long long)lanes))) {
-        const long long key = (long long)((design << kColumnBits) | i);
+    if (has && (me == __ffsll((long long)lanes) - 1 || me == 63 - __clzll((long long)lanes))) {
+        const long long key = ((long long)design << kColumnBits) | s.cci;
         atomicMin(&win.key[t][0], key);
         atomicMax(&win.key[t][1], key);
-        atomicMin(&win.rows[t][0], (int)j);
-        atomicMax(&win.rows[t][1], (int)j);
+        atomicMin(&win.rows[t][0], s.ccj);
+        atomicMax(&win.rows[t][1], s.ccj);
     }
 }
 
-// Adds q units (|q| < 2^46) to base-mesh node (ii, jj): into the window slot that holds the
-// node (USE_WIN), else to the node's words in global memory.
+// Adds q units (|q| < 2^46) to node (ii, jj) of the design's mesh: into its window slot when
+// the round's window holds it (USE_WIN), else to the node's words in global memory.
 template <bool USE_WIN>
-__device__ __forceinline__ void node_add(DepositWindow& win, int t, u64* __restrict__ acc, i64 ofn, i64 ny,
-                                         i64 ii, i64 jj, i64 q) {
-    if (USE_WIN) {
-        const int slot = (int)(ii & (kWinCols - 1));
-        const i64 row = jj - win.base[t][slot];
-        if (win.tag[t][slot] == ofn + ii * ny && row >= 0 && row < kWinRows) {
-            atomicAdd(&win.word[slot][row], (u64)q);  // two's complement
-            return;
-        }
+__device__ __forceinline__ void node_add(DepositWindow& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
+                                         i64 ny, int design, int ii, int jj, i64 q) {
+    if (USE_WIN && design == wv.design && ii >= wv.lo && ii <= wv.hi && jj >= wv.base &&
+        jj < wv.base + kWinRows) {
+        atomicAdd(&win.word[ii & (kWinCols - 1)][jj - wv.base], (u64)q);  // two's complement
+        return;
     }
-    add_sum(acc + kWords * (ofn + ii * ny + jj), (u64)q);
+    add_sum(acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
 }
-
-// The node of cell c (a row or column coordinate) with the given parity: c or c + 1.
-__device__ __forceinline__ int parity_node(int c, int parity) { return c + ((c ^ parity) & 1); }
 
 // An integer-valued double below 2^51 in magnitude as a signed 64-bit integer.
 __device__ __forceinline__ i64 exact_i64(double q) {
@@ -297,16 +300,25 @@ __device__ __forceinline__ i64 exact_i64(double q) {
 }
 
 // Moves the lane's cache to cell (i, j): the sums of nodes that both cells share stay in
-// their slots, the others leave the lane.
+// their slots, the others leave the lane.  The cache's cell (c, r) holds columns c, c + 1 and
+// rows r, r + 1; column c stays iff i is c - 1 or c, column c + 1 iff i is c or c + 1 (rows
+// alike); slot q holds column c when q's column parity is c's, else c + 1.
 template <bool USE_WIN>
-__device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
-                                           i64 ofn, i64 ny, int i, int j) {
+__device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, const WinView& wv,
+                                           u64* __restrict__ acc, i64 ofn, i64 ny, int design, int i, int j) {
     if (i == s.cci && j == s.ccj) return;
+    const int c = s.cci, r = s.ccj;
+    const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
+    const bool keep_r = (unsigned)(r - j) <= 1u, keep_r1 = (unsigned)(r + 1 - j) <= 1u;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int oi = parity_node(s.cci, q >> 1), oj = parity_node(s.ccj, q & 1);
-        if (oi != parity_node(i, q >> 1) || oj != parity_node(j, q & 1)) {
-            if (s.cq[q] != 0.0) node_add<USE_WIN>(win, t, acc, ofn, ny, oi, oj, exact_i64(s.cq[q]));
+        const bool at_c = ((c ^ (q >> 1)) & 1) == 0;  // slot q's column is c (else c + 1)
+        const bool at_r = ((r ^ (q & 1)) & 1) == 0;
+        const bool kept = (at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1);
+        if (!kept) {
+            if (s.cq[q] != 0.0)
+                node_add<USE_WIN>(win, wv, acc, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1,
+                                  exact_i64(s.cq[q]));
             s.cq[q] = 0.0;
         }
     }
@@ -316,12 +328,13 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, int
 
 // Empties the lane's cache (the trajectory ended).
 template <bool USE_WIN>
-__device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
-                                            i64 ofn, i64 ny) {
+__device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, const WinView& wv,
+                                            u64* __restrict__ acc, i64 ofn, i64 ny, int design) {
+    const int c = s.cci, r = s.ccj;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (s.cq[q] != 0.0)
-            node_add<USE_WIN>(win, t, acc, ofn, ny, parity_node(s.cci, q >> 1), parity_node(s.ccj, q & 1),
+            node_add<USE_WIN>(win, wv, acc, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
                               exact_i64(s.cq[q]));
         s.cq[q] = 0.0;
     }
@@ -336,11 +349,10 @@ __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, in
 // larger single deposit) they go to the node words in global memory.  The node sums are
 // exact integers, so the grouping does not change them.
 template <bool USE_WIN>
-__device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t, u64* __restrict__ acc,
-                                         i64 ofn, i64 ny, bool in_base, i64 design, int i, int j,
+__device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, const WinView& wv, u64* __restrict__ acc,
+                                         i64 ofn, i64 ny, bool in_base, int design, int i, int j,
                                          double u, double w, double deposit, int e) {
-    if (USE_WIN) window_note(win, t, in_base, design, i, j);
-    if (in_base) cache_move<USE_WIN>(s, win, t, acc, ofn, ny, i, j);
+    if (in_base) cache_move<USE_WIN>(s, win, wv, acc, ofn, ny, design, i, j);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
 #pragma unroll
@@ -355,76 +367,92 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, int t
             if (fabs(sum) < kCacheLimit) {
                 s.cq[q] = sum;
             } else {
-                node_add<USE_WIN>(win, t, acc, ofn, ny, i + ia, j + jb, exact_i64(s.cq[q]));
+                node_add<USE_WIN>(win, wv, acc, ofn, ny, design, i + ia, j + jb, exact_i64(s.cq[q]));
                 s.cq[q] = rounded;
             }
             continue;
         }
         u64 lo, hi;
-        if (to_fixed(value, e, lo, hi)) add_words(acc + kWords * (ofn + (i + ia) * ny + (j + jb)), lo, hi);
+        if (to_fixed(value, e, lo, hi)) add_words(acc + kWords * (ofn + (i64)(i + ia) * ny + (j + jb)), lo, hi);
     }
 }
 
-// After the deposits of round k (table t = k mod 2): move the window to the columns around
-// the round's cells of its lowest design (with room for kRoundSteps steps of travel either
-// way), flushing the slots that change, and write table 1 - t.  `p` describes the mesh of
-// a one-mesh launch; a batched launch reads the chosen design's mesh from its table.
+// After the deposits of round k (table t = k mod 2): place the window around the noted
+// cells of the lowest noted design (columns with room for kRoundSteps steps of travel either
+// way; the base row kept while the noted rows fit with that room), flush the slots that
+// leave (all of them with `empty_all`) and write table 1 - t.  `p` describes the mesh of a
+// one-mesh launch; a batched launch reads the chosen design's mesh from its table.
 template <bool BATCHED>
 __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, const TraceParams& p, bool empty_all) {
     const int n = 1 - t;
-    const i64 kmin = win.key[t][0], kmax = win.key[t][1];
-    const bool moved = kmin <= kmax;
-    const i64 design = moved ? kmin >> kColumnBits : 0;
-    i64 ofn = p.bofn, nx = p.nx, ny = p.ny;
-    if (BATCHED && moved) {
-        const BatchDesign& b = p.designs[design];
-        ofn = b.node_offset; nx = b.nx; ny = b.ny;
-    }
-    const int cmin = (int)(kmin & ((1 << kColumnBits) - 1));
-    // the highest column of the same design (another design's lanes leave the span open)
-    const int cmax = (kmax >> kColumnBits) == design ? (int)(kmax & ((1 << kColumnBits) - 1)) : (int)nx - 1;
-    const int rmin = win.rows[t][0];
-    int lo = cmin - 1 - kRoundSteps, hi = cmax + 2 + kRoundSteps;
-    if (lo < 0) lo = 0;
-    if (hi > (int)nx - 1) hi = (int)nx - 1;
-    if (hi > lo + kWinCols - 1) hi = lo + kWinCols - 1;
-    int nbase = rmin - 4;
-    if (nbase > (int)ny - kWinRows) nbase = (int)ny - kWinRows;
-    if (nbase < 0) nbase = 0;
-    for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
-        const int slot = k / kWinRows, row = k % kWinRows;
-        const i64 old = win.tag[t][slot];
-        i64 want = old;
-        if (moved) {
-            const int col = lo + ((slot - lo) & (kWinCols - 1));
-            if (col <= hi) want = ofn + (i64)col * ny;
+    const long long kmin = win.key[t][0], kmax = win.key[t][1];
+    const int olo = win.lo[t], ohi = win.hi[t], obase = win.base[t], odesign = win.design[t];
+    const i64 oofn = win.ofn[t];
+    const int ony = win.ny[t];
+    int nlo = olo, nhi = ohi, nbase = obase, ndesign = odesign, nny = ony;
+    i64 nofn = oofn;
+    if (kmin <= kmax) {
+        ndesign = (int)(kmin >> kColumnBits);
+        i64 ofn = p.bofn;
+        int nx = p.nx, ny = p.ny;
+        if (BATCHED) {
+            const BatchDesign& b = p.designs[ndesign];
+            ofn = b.node_offset; nx = (int)b.nx; ny = (int)b.ny;
         }
-        if (want != old || empty_all) {
+        const int cmin = (int)(kmin & ((1 << kColumnBits) - 1));
+        // the highest column of the same design (another design's lanes leave the span open)
+        const int cmax = (int)(kmax >> kColumnBits) == ndesign ? (int)(kmax & ((1 << kColumnBits) - 1)) : nx - 1;
+        nlo = cmin - 1 - kRoundSteps;
+        if (nlo < 0) nlo = 0;
+        nhi = cmax + 2 + kRoundSteps;
+        if (nhi > nx - 1) nhi = nx - 1;
+        if (nhi > nlo + kWinCols - 1) nhi = nlo + kWinCols - 1;
+        const int rmin = win.rows[t][0], rmax = win.rows[t][1];
+        if (!(ndesign == odesign && rmin - kRoundSteps >= obase && rmax + 1 + kRoundSteps < obase + kWinRows)) {
+            nbase = rmin - 4;
+            if (nbase > ny - kWinRows) nbase = ny - kWinRows;
+            if (nbase < 0) nbase = 0;
+        }
+        nofn = ofn;
+        nny = ny;
+    }
+    const bool same_rows = !empty_all && ndesign == odesign && nbase == obase;
+    if (ohi >= olo) {
+        for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
+            const int slot = k / kWinRows, row = k % kWinRows;
+            const int col = olo + ((slot - olo) & (kWinCols - 1));  // the column slot holds
+            if (col > ohi || (same_rows && col >= nlo && col <= nhi)) continue;  // empty, or stays
             const u64 q = win.word[slot][row];
-            if (old >= 0 && q != 0) add_sum(p.acc + kWords * (old + win.base[t][slot] + row), q);
-            win.word[slot][row] = 0ull;
-        }
-        if (row == 0) {
-            win.tag[n][slot] = want;
-            win.base[n][slot] = want != old ? nbase : win.base[t][slot];
+            if (q != 0) {
+                add_sum(p.acc + kWords * (oofn + (i64)col * ony + obase + row), q);
+                win.word[slot][row] = 0ull;
+            }
         }
     }
-    if (threadIdx.x == 0) window_reset_bounds(win, n);
+    if (threadIdx.x == 0) {
+        win.lo[n] = nlo; win.hi[n] = nhi; win.base[n] = nbase; win.design[n] = ndesign;
+        win.ofn[n] = nofn; win.ny[n] = nny;
+        window_reset_notes(win, n);
+    }
 }
 
 // Every node sum still in the window (table t) to global memory.
 __device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __restrict__ acc) {
+    const int lo = win.lo[t], hi = win.hi[t], base = win.base[t], ny = win.ny[t];
+    const i64 ofn = win.ofn[t];
+    if (hi < lo) return;
     for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
         const int slot = k / kWinRows, row = k % kWinRows;
-        const i64 old = win.tag[t][slot];
+        const int col = lo + ((slot - lo) & (kWinCols - 1));
         const u64 q = win.word[slot][row];
-        if (old >= 0 && q != 0) add_sum(acc + kWords * (old + win.base[t][slot] + row), q);
+        if (col <= hi && q != 0) add_sum(acc + kWords * (ofn + (i64)col * ny + base + row), q);
     }
 }
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN>
-__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, DepositWindow& win, int t) {
+__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, DepositWindow& win,
+                                    const WinView& wv) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -586,7 +614,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         if (!(fj >= 0.0)) { w = 0.0; j = 0; }
         else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = (int)pv.ny - 2; }
         else j = (int)fj;
-        deposit4<USE_WIN>(s, win, t, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
+        deposit4<USE_WIN>(s, win, wv, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
                           s.dqk * dt, p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
@@ -711,7 +739,7 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED>
 __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     __shared__ DepositWindow win;
-    window_init(win);
+    window_init(win, p);
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     TraceParams lp = p;
@@ -776,6 +804,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
         }
         // the whole workgroup goes on while any lane has or may get a trajectory
         if (!__syncthreads_or(s.k >= 0 || !exhausted)) break;
+        const WinView wv = window_view(win, t);
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
             if (s.k >= 0) {
@@ -788,14 +817,15 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.ck_index += 1;
                     s.ck_next += p.ckpt_interval;
                 }
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, F, s, win, t);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, F, s, win, wv);
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush<true>(s, win, t, p.acc, q.bofn, q.ny);
+                    cache_flush<true>(s, win, wv, p.acc, q.bofn, q.ny, q.design);
                 }
             }
         }
+        window_note(win, t, s, q.design);
         const bool busy = __syncthreads_or(s.k >= 0);
         refill = !p.refill_block || !busy;
         window_maintain<BATCHED>(win, t, p, ++rounds % kFlushRounds == 0);
