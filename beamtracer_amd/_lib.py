@@ -111,7 +111,7 @@ class BatchArgs(ctypes.Structure):
         ("charge", _ptr),
         ("cylindrical", _i64), ("integrator", _i64),
         ("step_fraction", _dbl), ("max_steps", _i64), ("refine", _i64),
-        ("block_size", _i32), ("max_blocks", _i32),
+        ("block_size", _i32), ("max_blocks", _i32), ("flags", _i32), ("reserved", _i32),
     ]  # fmt: skip
 
 

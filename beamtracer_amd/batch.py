@@ -46,11 +46,14 @@ ORDERS = ("design", "longest")
 
 @dataclass
 class LaunchShape:
-    """Threads per block (0: 256), resident blocks (0: all that fit) and queue order."""
+    """Threads per block (0: 256), resident blocks (0: all that fit), queue order, and whether
+    a workgroup takes new trajectories together (consecutive queue entries, so its lanes
+    mostly share one design and its deposit window) or each lane as it frees."""
 
     block_size: int = 0
     max_blocks: int = 0
     order: str = "longest"
+    refill_block: bool = False
 
 
 class BatchedTracer:
@@ -247,6 +250,7 @@ class BatchedTracer:
         a.integrator = INTEGRATORS.index(t0.integrator)
         a.step_fraction, a.max_steps, a.refine = t0.step_fraction, t0.max_steps, SUBCELLS
         a.block_size, a.max_blocks = self.shape.block_size, self.shape.max_blocks
+        a.flags = _lib.BT_REFILL_BLOCK if self.shape.refill_block else 0
         return a
 
     def _upload(self, host: dict, capacity: int) -> dict:
@@ -496,12 +500,13 @@ def vacuum_tracers(designs: Sequence, processes: int = 0) -> list[HipBeamTracer]
 
 
 def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, blocks=(64, 128, 192, 256),
-             fills=(1.0, 0.5), orders=ORDERS, repeats: int = 2) -> dict:
+             fills=(1.0, 0.5), orders=ORDERS, refills=(False, True), repeats: int = 2) -> dict:
     """Time one batched trace per launch shape and keep the fastest in ``tracer.shape``.
 
     The candidates are threads per block, the fraction of the resident blocks the grid
-    fills, and the queue order; each is timed (HIP events on the launch stream, best of
-    ``repeats``) on the real batch. Returns every timing in ms and the choice.
+    fills, the queue order and the refill mode (lane by lane, or a workgroup together); each
+    is timed (HIP events on the launch stream, best of ``repeats``) on the real batch.
+    Returns every timing in ms and the choice.
     """
     import torch
 
@@ -519,22 +524,23 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
             per_cu = L.bt_batch_occupancy(INTEGRATORS.index(t0.integrator), geometry_code(t0.mesh), block)
             for fill in fills:
                 grid = max(1, int(per_cu * cus * fill))
-                tracer.shape = LaunchShape(block, grid, order)
-                times = []
-                for _ in range(repeats):
-                    tracer._restart(dev)
-                    start = torch.cuda.Event(enable_timing=True)
-                    end = torch.cuda.Event(enable_timing=True)
-                    stream = torch.cuda.current_stream()
-                    start.record(stream)
-                    tracer.launch(dev, dev["queue"])
-                    end.record(stream)
-                    torch.cuda.synchronize()
-                    times.append(start.elapsed_time(end))
-                key = f"{order}/block{block}/fill{fill}"
-                timings[key] = min(times)
-                if best is None or min(times) < best[0]:
-                    best = (min(times), LaunchShape(block, grid, order))
+                for refill in refills:
+                    tracer.shape = LaunchShape(block, grid, order, refill)
+                    times = []
+                    for _ in range(repeats):
+                        tracer._restart(dev)
+                        start = torch.cuda.Event(enable_timing=True)
+                        end = torch.cuda.Event(enable_timing=True)
+                        stream = torch.cuda.current_stream()
+                        start.record(stream)
+                        tracer.launch(dev, dev["queue"])
+                        end.record(stream)
+                        torch.cuda.synchronize()
+                        times.append(start.elapsed_time(end))
+                    key = f"{order}/block{block}/fill{fill}" + ("/refill-block" if refill else "")
+                    timings[key] = min(times)
+                    if best is None or min(times) < best[0]:
+                        best = (min(times), LaunchShape(block, grid, order, refill))
     assert best is not None
     tracer.shape = best[1]
     return {"timings_ms": timings, "choice": vars(best[1]), "best_ms": best[0], "time": time.time()}

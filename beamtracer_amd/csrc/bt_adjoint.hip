@@ -112,15 +112,24 @@ struct GradWindow {
     int base;    // first row held
 };
 
-__device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
+// Wave-wide min / max with every lane of the wave active: DPP row operations (quad swaps,
+// half-row and row mirrors, then row broadcasts into rows 1, 3 and 2, 3) leave the result
+// in lane 63, read back as a uniform value.
+template <bool MAX>
+__device__ __forceinline__ int wave_reduce(int v) {
+    const auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1, 0, 3, 2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2, 3, 0, 1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
-}
+__device__ __forceinline__ int wave_min(int v) { return wave_reduce<false>(v); }
+
+__device__ __forceinline__ int wave_max(int v) { return wave_reduce<true>(v); }
 
 template <int A>
 __device__ __forceinline__ void add_node(double* __restrict__ grad, i64 cols, const GradWindow& W, int ii, int jj,

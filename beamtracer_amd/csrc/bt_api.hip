@@ -359,6 +359,7 @@ bt::TraceParams batch_params(const BatchPlan* plan, const bt_batch_args* a) {
     p.queue = reinterpret_cast<u64*>(base + node_bytes(plan->total_nodes));
     p.charge = a->charge;
     p.designs = plan->d_designs; p.design_of = a->design_of; p.n_designs = (int)plan->designs.size();
+    p.refill_block = (a->flags & BT_REFILL_BLOCK) != 0;
     return p;
 }
 

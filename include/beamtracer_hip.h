@@ -174,6 +174,9 @@ typedef struct bt_batch_args {
     int64_t cylindrical, integrator;
     double step_fraction; int64_t max_steps; int64_t refine;
     int32_t block_size, max_blocks;               /* 0: 256 threads, all resident blocks    */
+    int32_t flags, reserved;                      /* BT_REFILL_BLOCK: a workgroup takes new
+                                                     trajectories together (consecutive
+                                                     queue entries: mostly one design)      */
 } bt_batch_args;
 
 int64_t bt_batch_args_size(void);
