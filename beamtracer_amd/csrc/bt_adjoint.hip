@@ -143,14 +143,70 @@ __device__ __forceinline__ void add_node(double* __restrict__ grad, i64 cols, co
 
 // lambda_E times the bilinear weights of the gathered cell into the gradient of grid A
 // (0: fx, 1: fy; rows of `cols`).
+// A lane's running sums of its contributions to the four nodes of one cell of grid A (slot
+// q: the cell's node of column parity q >> 1 and row parity q & 1): consecutive stages and
+// steps mostly stay in one cell, so the node sums leave the lane -- into the wave's window
+// or global memory -- only when the lane moves to a cell without that node and when its
+// trajectory ends.
+constexpr int kNoCell = -(1 << 30);
+
+struct GradCache {
+    int c, r;  // the cell (kNoCell: none)
+    double q[4];
+};
+
+__device__ __forceinline__ void cache_reset(GradCache& g) {
+    g.c = kNoCell;
+    g.r = kNoCell;
+    g.q[0] = g.q[1] = g.q[2] = g.q[3] = 0.0;
+}
+
+// Moves the cache to cell (i, j): column c stays iff i is c - 1 or c, column c + 1 iff i is
+// c or c + 1 (rows alike); the other nodes' sums leave.
 template <int A>
-__device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, const GradWindow& W, const Gathered& g,
-                                        double lam) {
+__device__ __forceinline__ void cache_move(double* __restrict__ grad, i64 cols, const GradWindow& W, GradCache& g,
+                                           int i, int j) {
+    if (i == g.c && j == g.r) return;
+    const int c = g.c, r = g.r;
+    const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
+    const bool keep_r = (unsigned)(r - j) <= 1u, keep_r1 = (unsigned)(r + 1 - j) <= 1u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool at_c = ((c ^ (q >> 1)) & 1) == 0;  // slot q's column is c (else c + 1)
+        const bool at_r = ((r ^ (q & 1)) & 1) == 0;
+        if (!((at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1))) {
+            if (g.q[q] != 0.0) add_node<A>(grad, cols, W, at_c ? c : c + 1, at_r ? r : r + 1, g.q[q]);
+            g.q[q] = 0.0;
+        }
+    }
+    g.c = i;
+    g.r = j;
+}
+
+template <int A>
+__device__ __forceinline__ void cache_flush(double* __restrict__ grad, i64 cols, const GradWindow& W, GradCache& g) {
+    const int c = g.c, r = g.r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (g.q[q] != 0.0) add_node<A>(grad, cols, W, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1), g.q[q]);
+    cache_reset(g);
+}
+
+// lambda_E times the bilinear weights of the gathered cell into the lane's cache of grid A
+// (0: fx, 1: fy; rows of `cols`).
+template <int A>
+__device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, const GradWindow& W, GradCache& gc,
+                                        const Gathered& g, double lam) {
     if (lam == 0.0) return;
-    add_node<A>(grad, cols, W, g.i, g.j, (1.0 - g.u) * (1.0 - g.w) * lam);
-    add_node<A>(grad, cols, W, g.i, g.j + 1, (1.0 - g.u) * g.w * lam);
-    add_node<A>(grad, cols, W, g.i + 1, g.j, g.u * (1.0 - g.w) * lam);
-    add_node<A>(grad, cols, W, g.i + 1, g.j + 1, g.u * g.w * lam);
+    cache_move<A>(grad, cols, W, gc, g.i, g.j);
+    const double u0 = 1.0 - g.u, w0 = 1.0 - g.w;
+    const bool io = (g.i & 1) != 0, jo = (g.j & 1) != 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool ia = io != ((q >> 1) != 0);  // the node is i + 1
+        const bool jb = jo != ((q & 1) != 0);   // the node is j + 1
+        gc.q[q] += ((ia ? g.u : u0) * (jb ? g.w : w0)) * lam;
+    }
 }
 
 // Column `col` of the window (slot s) to global memory; lane r takes row base + r.
@@ -195,10 +251,11 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, GradWindow
 }
 
 // The transposed field evaluation: given lambda_g for g = q/m E(q), scatter into dJ/dfx,
-// dJ/dfy and return lambda_q.
+// dJ/dfy and return lambda_q; `g` receives g itself (accel's value, bit for bit).
 template <int GEOM>
 __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
-                                              double qm, Vec3 q, Vec3 lg) {
+                                              GradCache& fx_sums, GradCache& fy_sums, double qm, Vec3 q,
+                                              Vec3 lg, Vec3& g) {
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
     if (GEOM == 1) {
         const Radial rd = radial(q.b, q.c);
@@ -212,6 +269,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     const double sy = (qy - p.origin_y) * p.inv_h;
     const Gathered gx = gather_d(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
     const Gathered gy = gather_d(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    if (GEOM != 0) g = {qm * gx.value, qm * (gy.value * cy), qm * (gy.value * cz)};
+    else g = {qm * gx.value, qm * gy.value, qm * 0.0};
     const double lex = qm * lg.a;
     double ley, lcy = 0.0, lcz = 0.0;
     if (GEOM == 1) {
@@ -223,8 +282,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     } else {
         ley = qm * lg.b;
     }
-    scatter<0>(p.dfx, p.ny, W, gx, lex);
-    scatter<1>(p.dfy, p.ny + 1, W, gy, ley);
+    scatter<0>(p.dfx, p.ny, W, fx_sums, gx, lex);
+    scatter<1>(p.dfy, p.ny + 1, W, fy_sums, gy, ley);
     const double lsx = lex * gx.ds + ley * gy.ds;
     const double lsy = lex * gx.dt + ley * gy.dt;
     Vec3 lq = {lsx * p.inv_h, 0.0, 0.0};
@@ -335,7 +394,8 @@ __device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 be
 
 // The transposed step m (start state x, v): lambda of the end (lx, lv) -> of the start.
 template <int INTEG, int GEOM>
-__device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFields& F, const GradWindow& W, i64 k,
+__device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
+                                             GradCache& fx_sums, GradCache& fy_sums, i64 k,
                                              i64 m, bool seeded, double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv) {
     const Vec3 a = accel<GEOM>(p, F, qm, x);
     int branch;
@@ -346,14 +406,17 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
     double Ldt = 0.0, Lhdt = 0.0;
     if (INTEG == 0) {
         const double dt6 = dt * (1.0 / 6.0);
+        // (the stage fields are not kept: the transposed evaluations below return them)
         const Vec3 q1 = vaxpy(x, hdt, v), k1 = vaxpy(v, hdt, a);
-        const Vec3 g1 = accel<GEOM>(p, F, qm, q1);
-        const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g1);
-        const Vec3 g2 = accel<GEOM>(p, F, qm, q2);
-        const Vec3 q3 = vaxpy(x, dt, k2), k3 = vaxpy(v, dt, g2);
-        const Vec3 g3 = accel<GEOM>(p, F, qm, q3);
+        Vec3 g = accel<GEOM>(p, F, qm, q1);
+        Vec3 sv = vaxpy(a, 2.0, g);
+        const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g);
+        g = accel<GEOM>(p, F, qm, q2);
+        sv = vaxpy(sv, 2.0, g);
+        const Vec3 q3 = vaxpy(x, dt, k2), k3 = vaxpy(v, dt, g);
+        g = accel<GEOM>(p, F, qm, q3);
+        sv = vadd(sv, g);
         const Vec3 sx = vadd(vaxpy(vaxpy(v, 2.0, k1), 2.0, k2), k3);
-        const Vec3 sv = vadd(vaxpy(vaxpy(a, 2.0, g1), 2.0, g2), g3);
         Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
         if (seeded) {
             const Vec3 y = vaxpy(x, dt6, sx), w = vaxpy(v, dt6, sv);
@@ -369,13 +432,15 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         La = Lsv;
         Vec3 Lg1 = vscale(2.0, Lsv), Lg2 = vscale(2.0, Lsv);
         const Vec3 Lg3 = Lsv;
-        const Vec3 Lq3 = accel_adjoint<GEOM>(p, F, W, qm, q3, Lg3);
-        Lv = vadd(Lv, Lk3); Ldt += vdot(Lk3, g2); Lg2 = vaxpy(Lg2, dt, Lk3);
+        const Vec3 Lq3 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q3, Lg3, g);
+        Lv = vadd(Lv, Lk3); Lg2 = vaxpy(Lg2, dt, Lk3);
         Lx = vadd(Lx, Lq3); Ldt += vdot(Lq3, k2); Lk2 = vaxpy(Lk2, dt, Lq3);
-        const Vec3 Lq2 = accel_adjoint<GEOM>(p, F, W, qm, q2, Lg2);
-        Lv = vadd(Lv, Lk2); Lhdt += vdot(Lk2, g1); Lg1 = vaxpy(Lg1, hdt, Lk2);
+        const Vec3 Lq2 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q2, Lg2, g);
+        Ldt += vdot(Lk3, g);  // g2
+        Lv = vadd(Lv, Lk2); Lg1 = vaxpy(Lg1, hdt, Lk2);
         Lx = vadd(Lx, Lq2); Lhdt += vdot(Lq2, k1); Lk1 = vaxpy(Lk1, hdt, Lq2);
-        const Vec3 Lq1 = accel_adjoint<GEOM>(p, F, W, qm, q1, Lg1);
+        const Vec3 Lq1 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q1, Lg1, g);
+        Lhdt += vdot(Lk2, g);  // g1
         Lv = vadd(Lv, Lk1); Lhdt += vdot(Lk1, a); La = vaxpy(La, hdt, Lk1);
         Lx = vadd(Lx, Lq1); Lhdt += vdot(Lq1, v); Lv = vaxpy(Lv, hdt, Lq1);
         Ldt += Ldt6 * (1.0 / 6.0);
@@ -393,7 +458,8 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         Vec3 Lk = lv;
         Lhdt += vdot(lv, g1);
         const Vec3 Lg1 = vscale(hdt, lv);
-        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, F, W, qm, q, Lg1));
+        Vec3 gq;
+        const Vec3 Lq = vadd(lx, accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q, Lg1, gq));
         Lx = Lq;
         Ldt += vdot(Lq, kk);
         Lk = vaxpy(Lk, dt, Lq);
@@ -409,7 +475,8 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
     } else if (branch == 2) {
         La = vaxpy(La, -0.5 * Ldt * dt / (acc * acc), a);
     }
-    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, qm, x, La));
+    Vec3 ga;
+    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, x, La, ga));
     lx = Lx;
     lv = Lv;
 }
@@ -439,7 +506,7 @@ __device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, int& ci,
 }
 
 template <int INTEG, int GEOM>
-__global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
+__global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const AdjFields F = {field_rsrc(p.fxs, (p.nx + 1) * p.ny), field_rsrc(p.fys, p.nx * (p.ny + 1))};
@@ -447,6 +514,9 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
     W.wave = (int)(threadIdx.x >> 6);
     W.lo = -1;
     W.base = 0;
+    GradCache fx_sums, fy_sums;  // the lane's node sums of dJ/dfx and dJ/dfy
+    cache_reset(fx_sums);
+    cache_reset(fy_sums);
     for (int t = (int)lane; t < kWinCols * kWinRows; t += 64) {
         adjoint_windows[W.wave][0][t] = 0.0;
         adjoint_windows[W.wave][1][t] = 0.0;
@@ -527,11 +597,13 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointParams p) {
                 window_follow(p, W, act, ci, cj);
                 if (act) {
                     const bool seeded = m + 1 == next;
-                    reverse_step<INTEG, GEOM>(p, F, W, k, m + 1, seeded, qm, x, v, lx, lv);
+                    reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, x, v, lx, lv);
                     if (seeded) next = seeded_step(p, k, m + 1);
                 }
             }
         }
+        cache_flush<0>(p.dfx, p.ny, W, fx_sums);
+        cache_flush<1>(p.dfy, p.ny + 1, W, fy_sums);
         if (valid && p.dx != nullptr) {
             p.dx[k] += lx.a; p.dx[N + k] += lx.b; p.dx[2 * N + k] += lx.c;
         }
