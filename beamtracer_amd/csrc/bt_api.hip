@@ -604,6 +604,16 @@ int bt_deposit_trajectories_2d(const double* px, const double* py, const double*
 }
 
 
+int bt_l2_gather_bandwidth(const double* field, int64_t rows, int64_t cols, int32_t lookups, int32_t blocks,
+                           double* sink, void* stream) {
+    if (field == nullptr || sink == nullptr || rows < 2 || cols < 2 || lookups < 1 || blocks < 1)
+        return fail(BT_E_ARG, "bad arguments");
+    if (rows * cols >= ((i64)1 << 28)) return fail(BT_E_ARG, "the field must hold fewer than 2^28 values");
+    if (bt::launch_l2_gather(field, rows, cols, lookups, blocks, 0x5eedu, sink, static_cast<hipStream_t>(stream)))
+        return fail(BT_E_LAUNCH, std::string("gather launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
 int bt_arith_selftest(const double* y, const double* z, int64_t n, double* r, double* cy, double* cz, double* rs,
                       void* stream) {
     if (n < 0 || (n > 0 && (y == nullptr || z == nullptr || r == nullptr || cy == nullptr || cz == nullptr ||

@@ -114,4 +114,42 @@ int launch_deposit(const double* px, const double* py, const double* q, i64 n, d
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
+// ---- L2 gather bandwidth (measurement) ---------------------------------------------------
+// Random 16-byte row-pair gathers, two per lookup as the tracer's bilinear does (f[c], f[c+1]
+// and f[c+cols], f[c+cols+1]), from a field of `rows` x `cols` doubles read through a buffer
+// resource: each lane draws its cells from a 32-bit hash, so a field larger than the CU's
+// L1 serves them from L2.  The sum of what was read goes to sink[lane] (so nothing is
+// dropped); the bytes moved are 32 per lookup.
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void l2_gather_kernel(const double* field, unsigned rows, unsigned cols,
+                                                        int lookups, unsigned seed, double* __restrict__ sink) {
+    const Rsrc f = field_rsrc(field, (i64)rows * cols);
+    const unsigned lane = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned state = hash32(lane ^ seed) | 1u;  // xorshift32 needs a nonzero state
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < lookups; ++k) {
+        state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+        // row in [0, rows - 2], col in [0, cols - 2] by multiply-high range reduction
+        const unsigned row = __umulhi(state, rows - 1), col = __umulhi(state * 0x9e3779b9u, cols - 1);
+        const unsigned c = row * cols + col;
+        const Pair lo = load_pair(f, c), hi = load_pair(f, c + cols);
+        acc += (lo.a + lo.b) + (hi.a + hi.b);
+    }
+    sink[lane] = acc;
+}
+
+int launch_l2_gather(const double* f, i64 rows, i64 cols, int lookups, int blocks, unsigned seed, double* sink,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(l2_gather_kernel, dim3(blocks), dim3(256), 0, stream, f, (unsigned)rows, (unsigned)cols,
+                       lookups, seed, sink);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 }  // namespace bt

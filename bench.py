@@ -36,6 +36,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--no
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -50,8 +51,12 @@ if str(ROOT) not in sys.path:
 
 METRIC = "particle-steps/s plain vs batched at 1e-10 rel err; adjoint/forward time ratio"
 #: Algorithmic FP64 work per RK4 step in cylindrical geometry: +, -, *, /, sqrt count 1,
-#: an fma 2; index arithmetic, compares and clamps 0 (derivation in DESIGN.md).
+#: an fma 2; index arithmetic, compares and clamps 0 (counted on the reference's arithmetic,
+#: trace_trajectories_2d_cuda.cu:221-355).
 FLOP_PER_STEP_RK4_CYL = 322
+# algorithmic bytes per RK4 cylindrical particle-step: 4 gathers x 64 B, the 4-node deposit read+write 64 B,
+# the cell lookup 8 B
+BYTES_PER_STEP_RK4_CYL = 328
 FLOP_PER_STEP_BORIS_B_CYL = 257
 PEAK_FP64_TFLOPS = 78.6
 
@@ -376,6 +381,35 @@ def batched_gradients_leg(count: int, seed: int, processes: int) -> dict:
     }
 
 
+def l2_gather_bandwidth(rows: int = 1603, cols: int = 201, lookups: int = 256, repeats: int = 10) -> float:
+    """Measured L2 bandwidth (bytes/s) of the tracer's gather pattern: random bilinear lookups
+    (two 16-byte row-pair loads each) in a field of rows x cols doubles, by default the size
+    of the einzel lens's fx + fy (2.58 MB, L2-resident), from 2048 x 256 lanes; the median of
+    ``repeats`` launches timed with HIP events."""
+    import torch
+
+    from beamtracer_amd import _lib
+
+    field = torch.ones(rows * cols, dtype=torch.float64, device="cuda")
+    blocks = 2048
+    sink = torch.zeros(blocks * 256, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    L = _lib.lib()
+    times = []
+    for r in range(repeats + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        rc = L.bt_l2_gather_bandwidth(field.data_ptr(), rows, cols, lookups, blocks, sink.data_ptr(),
+                                      ctypes.c_void_p(stream.cuda_stream))  # fmt: skip
+        b.record(stream)
+        _lib.check(rc, "bt_l2_gather_bandwidth")
+        torch.cuda.synchronize()
+        if r:
+            times.append(a.elapsed_time(b) / 1e3)
+    assert float(sink.sum()) == 4.0 * blocks * 256 * lookups  # every lookup read four ones
+    return 32.0 * blocks * 256 * lookups / float(np.median(times))
+
+
 def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
     """One adjoint gradient of J = rms_angle + emittance at 75 mm against one forward trace."""
     import torch
@@ -494,6 +528,16 @@ def main() -> int:
     log(f"timed: {opts.steps} traces in {wall:.3f} s; launch avg {avg_launch_s * 1e3:.3f} ms")
 
     achieved = FLOP_PER_STEP_RK4_CYL * steps_per_trace / avg_launch_s / 1e12
+    b_l2 = l2_gather_bandwidth()
+    steps_per_s = steps_per_trace / avg_launch_s
+    # the roofline of the path in particle-steps/s: min(FP64 peak / FLOP per step, measured L2
+    # gather bandwidth / algorithmic bytes per step); when the trace moves more algorithmic bytes
+    # than random gathers can draw from L2 (its lanes reuse L1 lines), the L2 term does not bind
+    # and the FP64 roof alone is the roof
+    fp64_roof = PEAK_FP64_TFLOPS * 1e12 / FLOP_PER_STEP_RK4_CYL
+    l2_roof = b_l2 / BYTES_PER_STEP_RK4_CYL
+    roof_steps = fp64_roof if steps_per_s > l2_roof else min(fp64_roof, l2_roof)
+    log(f"L2 gather bandwidth {b_l2 / 1e9:.0f} GB/s; roof {roof_steps:.3e} steps/s")
     traffic = None
     traffic_file = ROOT / "profiles" / "trace_kernel_einzel_traffic.json"
     if traffic_file.exists():
@@ -529,6 +573,18 @@ def main() -> int:
             "traffic": traffic,
             "flop_per_step": FLOP_PER_STEP_RK4_CYL,
             "launch_ms": avg_launch_s * 1e3,
+            "l2": {
+                "achieved": steps_per_s * BYTES_PER_STEP_RK4_CYL / 1e9,
+                "peak_measured": b_l2 / 1e9,
+                "unit": "GB/s",
+                "frac": steps_per_s * BYTES_PER_STEP_RK4_CYL / b_l2,
+                "bytes_per_step": BYTES_PER_STEP_RK4_CYL,
+                "note": "algorithmic field/deposit/lookup bytes per step against the measured bandwidth of random "
+                "bilinear row-pair gathers from a 2.58 MB (L2-resident) field; frac > 1: the trace's gathers reuse "
+                "L1 lines, so the FP64 roof alone bounds it",
+            },
+            "roof_steps_per_s": roof_steps,
+            "frac_of_roof": steps_per_s / roof_steps,
         },
     }
     if not opts.no_extra:

@@ -234,6 +234,14 @@ typedef struct bt_adjoint_args {
                                                      NULL: library-owned, one call at a time */
 } bt_adjoint_args;
 
+/* Measurement helper (no counterpart in the reference): `blocks` x 256 lanes each make
+ * `lookups` random bilinear lookups (two 16-byte row-pair loads, 32 B) in the device field of
+ * rows x cols doubles (< 2^28 values) and write one sum each to sink[blocks * 256]; the
+ * caller times the launch on `stream` for the L2 gather bandwidth of the tracer's access
+ * pattern. */
+int bt_l2_gather_bandwidth(const double* field, int64_t rows, int64_t cols, int32_t lookups, int32_t blocks,
+                           double* sink, void* stream);
+
 /* Self-test of the kernels' square-root arithmetic (no counterpart in the reference): for
  * device arrays of n doubles, the radial distance r[k] = |(y[k], z[k])| and direction
  * cy[k] = y[k] / r[k], cz[k] = z[k] / r[k] (0 on the axis) as the tracer computes them from

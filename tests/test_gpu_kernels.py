@@ -215,3 +215,20 @@ def test_square_root_arithmetic_is_within_a_few_ulps():
     pos = y > 0
     assert _ulps(rs[pos], 1 / np.sqrt(yl[pos])).max() <= 2
     assert (rs[~pos] == 0).all()
+
+
+def test_l2_gather_measurement_reads_every_lookup():
+    """The L2 gather microbenchmark of the roofline reads four field values per lookup: on a
+    field of ones every lane's sum is 4 x lookups (and it refuses a field of 2^28 values)."""
+    import torch
+
+    from beamtracer_amd import _lib
+
+    rows, cols, lookups, blocks = 1603, 201, 64, 8
+    field = torch.ones(rows * cols, dtype=torch.float64, device="cuda")
+    sink = torch.zeros(blocks * 256, dtype=torch.float64, device="cuda")
+    L = _lib.lib()
+    assert L.bt_l2_gather_bandwidth(field.data_ptr(), rows, cols, lookups, blocks, sink.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert (sink.cpu().numpy() == 4.0 * lookups).all()
+    assert L.bt_l2_gather_bandwidth(field.data_ptr(), 1 << 14, 1 << 14, 1, 1, sink.data_ptr(), None) < 0

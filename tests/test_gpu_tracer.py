@@ -2,7 +2,7 @@
 
 ``OracleTracer`` runs ``HipBeamTracer``'s restatement of ``BeamTracer._trace_compiled``
 (``solver/tracer.py:1185-1304``) with the oracle's kernel in place of the hip one, so every
-difference is the kernel's. The bars are the north star's: trajectory end states and
+difference is the kernel's. The bars: trajectory end states and
 diagnostic-plane moments within 1e-10 relative, the deposited charge within 1e-12 of its
 total, and the converged potential of the Vlasov loop within 1e-9 relative.
 """
