@@ -112,6 +112,7 @@ class HipBeamTracer:
         speed_of_light: float = constants.c,
         device: bool = False,
         block_size: int = 0,
+        refill_block: bool = False,
     ) -> None:
         if not 0.0 < step_fraction <= 1.0:
             raise ValueError(f"step_fraction must be in (0, 1], got {step_fraction}")
@@ -129,6 +130,8 @@ class HipBeamTracer:
         self.save_trajectories = int(save_trajectories)
         self.device = bool(device)
         self.block_size = int(block_size)
+        #: a workgroup takes new trajectories together (see :mod:`beamtracer_amd.tuning`)
+        self.refill_block = bool(refill_block)
         self._lookup: tuple[np.ndarray, np.ndarray] | None = None
         self._capacity = (4, 1024)
         self.reduce = None
@@ -333,7 +336,8 @@ class HipBeamTracer:
         if self.device:
             self._call_device(call, magnetic, exponent)
         else:
-            kernels.trace_trajectories_2d(*call, magnetic=magnetic, exponent=exponent, block_size=self.block_size)
+            kernels.trace_trajectories_2d(*call, magnetic=magnetic, exponent=exponent, block_size=self.block_size,
+                                          refill_block=self.refill_block)  # fmt: skip
 
     def _call_device(self, call: list, magnetic, exponent: int) -> None:
         """One kernel call with the arrays on the GPU (copied back into ``call`` after)."""
@@ -345,6 +349,7 @@ class HipBeamTracer:
         mag = None if magnetic is None else tuple(None if c is None else torch.from_numpy(c).cuda() for c in magnetic)
         ck, interval = self.checkpoints if self.checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size,
+                                      refill_block=self.refill_block,
                                       checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
         torch.cuda.synchronize()
         for i in (1, 2, 5, 6, 7, 25, 26, 28, 29, 31):

@@ -1,0 +1,30 @@
+"""The launch-shape tuner of the plain trace: every shape it tries gives the same bits, and
+the arguments it leaves hold the trace of the shape it picked (needs an MI355X)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tuned_trace_equals_the_default_trace():
+    import torch
+
+    from beamtracer_amd import kernels, tuning
+
+    args = cases.kernel_case(1, 0, refined=False)
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    host = cases.copy_args(args)
+    kernels.trace_trajectories_2d(*host)
+    dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+    for i in (10, 11, 24):  # patch tables and planes stay on the host
+        dev[i] = args[i]
+    tune = tuning.tune_trace(dev, blocks=(64, 256), repeats=1)
+    assert set(tune["timings_ms"]) == {"block64", "block256", "block64/refill-block", "block256/refill-block"}
+    torch.cuda.synchronize()
+    for i in (*cases.OUTPUTS, cases.CHARGE):
+        assert np.array_equal(dev[i].cpu().numpy(), host[i]), i
