@@ -736,7 +736,10 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
 // parameters.  BATCHED: every design's trajectories share the queue, and a lane takes the
 // mesh of each new trajectory's design; the deposit window follows the lowest design of
 // the workgroup's lanes, and lanes of other designs add to global memory.
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED>
+//
+// RECORDS: the launch may record paths and checkpoints (else that code is compiled out; a
+// batched launch records neither).
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS>
 __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     __shared__ DepositWindow win;
     window_init(win, p);
@@ -781,14 +784,14 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.v0 = p.v[k]; s.v1 = p.v[N + k]; s.v2 = p.v[2 * N + k];
                     s.qmk = p.qm[k];
                     s.dqk = p.dq[k];
-                    s.slot = (!BATCHED && p.path_slot != nullptr) ? p.path_slot[k] : -1;
+                    s.slot = (RECORDS && p.path_slot != nullptr) ? p.path_slot[k] : -1;
                     if (s.slot >= p.n_paths) s.slot = -1;
                     s.ns = p.steps[k];
                     s.have = 0;
                     s.point = 0;
                     s.ca0 = s.ca1 = s.ca2 = 0.0;
                     s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
-                    if (!BATCHED && p.ckpt != nullptr) {  // the first multiple of the interval >= ns
+                    if (RECORDS && p.ckpt != nullptr) {  // the first multiple of the interval >= ns
                         s.ck_index = (s.ns + p.ckpt_interval - 1) / p.ckpt_interval;
                         s.ck_next = s.ck_index * p.ckpt_interval;
                     }
@@ -808,7 +811,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
             if (s.k >= 0) {
-                if (!BATCHED && p.ckpt != nullptr && s.ns == s.ck_next) {
+                if (RECORDS && p.ckpt != nullptr && s.ns == s.ck_next) {
                     if (s.ck_index < p.n_ckpt) {
                         double* c = p.ckpt + s.ck_index * 6 * p.n + s.k;
                         c[0] = s.x0; c[p.n] = s.x1; c[2 * p.n] = s.x2;
@@ -834,15 +837,15 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     window_flush(win, t, p.acc);
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS>
 __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
-    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false>(p);
+    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false, RECORDS>(p);
 }
 
 // Every design's trajectories in one persistent launch.
 template <int INTEG, int GEOM, bool HAS_B>
 __global__ __launch_bounds__(256) void batched_kernel(TraceParams p) {
-    trace_loop<INTEG, GEOM, HAS_B, false, true>(p);
+    trace_loop<INTEG, GEOM, HAS_B, false, true, false>(p);
 }
 
 // The start of a batched trace from the emitter: x, v from x0, v0; steps 0; status and
@@ -893,16 +896,19 @@ __global__ void finalize_kernel(const u64* __restrict__ acc, i64 nodes, int e, d
     }
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS>
 static TraceKernel kernel_ptr() {
-    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL>;
+    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL, RECORDS>;
 }
 
-TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general) {
-#define BT_SEL(I, G, B, X) \
-    if (integ == I && geom == G && has_b == B && general == X) return kernel_ptr<I, G, B, X>();
-#define BT_SEL_G(I, G) BT_SEL(I, G, false, false) BT_SEL(I, G, false, true) \
-                       BT_SEL(I, G, true, false) BT_SEL(I, G, true, true)
+// The general variants (refined patches, graded axis, relativistic motion) always record.
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records) {
+#define BT_SEL(I, G, B, X, R) \
+    if (integ == I && geom == G && has_b == B && general == X && (X || records == R)) \
+        return kernel_ptr<I, G, B, X, R>();
+#define BT_SEL_G(I, G) BT_SEL(I, G, false, false, false) BT_SEL(I, G, false, false, true) \
+                       BT_SEL(I, G, false, true, true) BT_SEL(I, G, true, false, false) \
+                       BT_SEL(I, G, true, false, true) BT_SEL(I, G, true, true, true)
     BT_SEL_G(0, 0) BT_SEL_G(0, 1) BT_SEL_G(0, 2)
     BT_SEL_G(1, 0) BT_SEL_G(1, 1) BT_SEL_G(1, 2)
 #undef BT_SEL_G
@@ -911,8 +917,8 @@ TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general) {
 }
 
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
-                 bool has_b, bool general, i64 nodes) {
-    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general);
+                 bool has_b, bool general, bool records, i64 nodes) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records);
     if (kern == nullptr) return -1;
     const i64 words = kWords * nodes;
     int zgrid = (int)((words + 255) / 256);
@@ -999,8 +1005,8 @@ int launch_arith_selftest(const double* y, const double* z, i64 n, double* r, do
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int trace_occupancy(int integ, int geom, bool has_b, bool general, int block) {
-    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general);
+int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, int block) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records);
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),

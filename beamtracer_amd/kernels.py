@@ -217,7 +217,7 @@ def trace_trajectories_2d(
     max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
     path_count, num_threads, charge, graded, x_nodes, inv_c2,
     *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
-    refill_block=False, checkpoints=None, checkpoint_interval=64,
+    refill_block=False, checkpoints=None, checkpoint_interval=64, record_paths=True,
 ) -> None:  # fmt: skip
     """Trace the ``active`` trajectories until they end, or reach a surface.
 
@@ -230,7 +230,10 @@ def trace_trajectories_2d(
     :func:`trace_exponent`); ``refill_block`` refills a workgroup's lanes only when all are
     free (``BT_REFILL_BLOCK``: coherent lanes, whose deposits add up in LDS);
     ``checkpoints`` (device form: a float64 tensor ``(C, 6, N)``) receives ``(x, v)`` of every
-    trajectory before each step whose count is a multiple of ``checkpoint_interval``.
+    trajectory before each step whose count is a multiple of ``checkpoint_interval``;
+    ``record_paths=False`` says that no trajectory records a path (``BT_NO_PATHS``: every
+    ``path_slot`` is negative, else those paths go unrecorded) and runs the trace variant
+    without the path code (the host form finds this itself).
     """
     L = _lib.lib()
     device = _is_torch(x)
@@ -293,7 +296,8 @@ def trace_trajectories_2d(
         int(num_threads), _ptr(charge), charge.shape[0], charge.shape[1],
         int(graded), _ptr(x_nodes), x_nodes.shape[0], float(inv_c2),
         int(magnetic is not None), _ptr(bx), _ptr(by), _ptr(bz),
-        int(exponent), int(block_size), int(max_blocks), _lib.BT_REFILL_BLOCK if refill_block else 0,
+        int(exponent), int(block_size), int(max_blocks),
+        (_lib.BT_REFILL_BLOCK if refill_block else 0) | (0 if record_paths else _lib.BT_NO_PATHS),
         _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
         None, 0, 0,
     )  # fmt: skip

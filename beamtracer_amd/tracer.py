@@ -349,7 +349,7 @@ class HipBeamTracer:
         mag = None if magnetic is None else tuple(None if c is None else torch.from_numpy(c).cuda() for c in magnetic)
         ck, interval = self.checkpoints if self.checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size,
-                                      refill_block=self.refill_block,
+                                      refill_block=self.refill_block, record_paths=bool((call[27] >= 0).any()),
                                       checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
         torch.cuda.synchronize()
         for i in (1, 2, 5, 6, 7, 25, 26, 28, 29, 31):

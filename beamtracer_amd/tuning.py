@@ -33,6 +33,7 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
     if exponent is None:
         host = [args[i].cpu().numpy() for i in (0, 2, 4, 3, 8, 9)]
         exponent = kernels.trace_exponent(*host, args[11], args[19], args[34])
+    record_paths = bool((args[27] >= 0).any())
     stream = torch.cuda.current_stream()
     timings = {}
     best = None
@@ -45,7 +46,8 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
                 start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 start.record(stream)
                 kernels.trace_trajectories_2d(*args, magnetic=magnetic, exponent=exponent, block_size=block,
-                                              refill_block=refill, stream=stream)  # fmt: skip
+                                              refill_block=refill, record_paths=record_paths,
+                                              stream=stream)  # fmt: skip
                 end.record(stream)
                 torch.cuda.synchronize()
                 times.append(start.elapsed_time(end))
@@ -57,5 +59,5 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
     for i, a in initial.items():
         args[i].copy_(a)
     kernels.trace_trajectories_2d(*args, magnetic=magnetic, exponent=exponent, block_size=best[1],
-                                  refill_block=best[2], stream=stream)  # fmt: skip
+                                  refill_block=best[2], record_paths=record_paths, stream=stream)  # fmt: skip
     return {"timings_ms": timings, "block_size": best[1], "refill_block": best[2], "best_ms": best[0]}

@@ -182,8 +182,8 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkp
             start.record(stream)
         ck, interval = checkpoints if checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=block_size, stream=stream,
-                                      checkpoints=ck, checkpoint_interval=interval,
-                                      refill_block=refill_block)  # fmt: skip
+                                      checkpoints=ck, checkpoint_interval=interval, refill_block=refill_block,
+                                      record_paths=bool((host[27] >= 0).any()))  # fmt: skip
         if end is not None:
             end.record(stream)
 

@@ -145,6 +145,9 @@ def _trace_args(
         ("x_nodes", x_nodes, f64),
     ):  # fmt: skip
         _check(a, dt, name)
+    if charge.ndim != 2 or charge.shape[0] < max(int(num_threads), 1):
+        # thread t deposits into charge[t] (trace_trajectories_2d_pyccel.py:129-135)
+        raise ValueError(f"charge must be (num_threads={int(num_threads)}, nodes), got {charge.shape}")
     b = [None, None, None]
     if magnetic is not None:
         b = [None if c is None else _check(np.ascontiguousarray(c, f64), f64, "B") for c in magnetic]

@@ -157,6 +157,33 @@ def test_trace_device_form_and_launch_shapes_agree(hip):
             assert np.array_equal(dev[i].cpu().numpy(), host[i]), (block, blocks, i)
 
 
+@pytest.mark.parametrize("cylindrical,integrator,magnetic", list(itertools.product([0, 1], [0, 1], [False, True])))
+def test_trace_without_paths_runs_the_lean_variant_with_the_same_results(hip, cylindrical, integrator, magnetic):
+    """No trajectory records a path: the host form picks the variant without the path code by
+    itself, the device form on ``record_paths=False``; both agree with the oracle, and bit for
+    bit with the recording variant."""
+    import torch
+
+    args = cases.kernel_case(cylindrical, integrator)
+    args[27][:] = -1  # path_slot: no paths
+    b = _magnetic(args) if magnetic else None
+    expected = cases.copy_args(args)  # one charge slab per oracle thread
+    oracle.trace_trajectories_2d(*expected, magnetic=b)
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])  # one slab, as the CUDA kernel
+    host = _run(hip.trace_trajectories_2d, args, magnetic=b)
+    for i in cases.OUTPUTS:
+        np.testing.assert_allclose(host[i], expected[i], rtol=1e-12, atol=1e-12)
+    want = expected[cases.CHARGE].sum(0)
+    assert np.abs(host[cases.CHARGE][0] - want).max() <= 1e-12 * np.abs(want).sum()
+    mag = None if b is None else tuple(torch.from_numpy(c.copy()).cuda() for c in b)
+    for record in (True, False):
+        dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+        hip.trace_trajectories_2d(*dev, magnetic=mag, record_paths=record)
+        torch.cuda.synchronize()
+        for i in (*cases.OUTPUTS, cases.CHARGE):
+            assert np.array_equal(dev[i].cpu().numpy(), host[i]), (record, i)
+
+
 def test_trace_without_active_and_at_max_steps(hip):
     args = cases.kernel_case(1, 0)
     args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
@@ -168,6 +195,7 @@ def test_trace_without_active_and_at_max_steps(hip):
     done = cases.copy_args(args)
     done[5][:] = 400  # already at max_steps: no step, status 4
     expected = cases.copy_args(done)
+    expected[30] = 1  # one oracle thread for the one charge slab
     oracle.trace_trajectories_2d(*expected)
     hip.trace_trajectories_2d(*done)
     assert (done[6] == 4).all()

@@ -214,6 +214,11 @@ def test_thread_slabs_sum_to_the_same_charge():
             np.testing.assert_array_equal(args[i], run(base)[i])
     for t in totals[1:]:
         np.testing.assert_allclose(t, totals[0], rtol=1e-12, atol=1e-12 * np.abs(totals[0]).sum())
+    short = cases.copy_args(base)
+    short[30] = 4
+    short[31] = np.zeros((3, base[31].shape[1]))  # fewer slabs than threads
+    with pytest.raises(ValueError, match="num_threads"):
+        oracle.trace_trajectories_2d(*short)
 
 
 def _random_field(rng, nx, ny):
