@@ -15,7 +15,7 @@ PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIB_PATH = PACKAGE / "libbeamtracer_hip.so"
 SOURCES = [PACKAGE / "csrc" / name for name in ("bt_api.hip", "bt_trace.hip", "bt_gather_deposit.hip",
-                                                "bt_adjoint.hip")]
+                                                "bt_adjoint.hip", "bt_fields.hip")]
 HEADERS = [ROOT / "include" / "beamtracer_hip.h"] + sorted((PACKAGE / "csrc").glob("*.h"))
 
 #: hipcc flags of the library: FP64 results in the reference's operation order (no fused
@@ -152,6 +152,14 @@ SIGNATURES = {
     "bt_staggered_field_2d": (
         ctypes.c_int,
         [_ptr, _ptr, _i64, _ptr, _ptr, _i64, _i64, _dbl, _dbl, _dbl, _ptr, _ptr],
+    ),
+    "bt_edge_field_2d": (
+        ctypes.c_int,
+        [_ptr, _ptr, _ptr, _ptr, _ptr, _dbl, _i64, _i64, _i32, _ptr, _ptr],
+    ),
+    "bt_edge_field_2d_device": (
+        ctypes.c_int,
+        [_ptr, _ptr, _ptr, _ptr, _ptr, _dbl, _i64, _i64, _i32, _ptr, _ptr, _ptr],
     ),
     "bt_staggered_field_2d_device": (
         ctypes.c_int,

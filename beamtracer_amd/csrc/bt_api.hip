@@ -552,6 +552,55 @@ int bt_staggered_field_2d(const double* px, const double* py, int64_t n, const d
     return BT_OK;
 }
 
+static int check_edge_field(const double* phi, const int32_t* solid, const uint8_t* near, const double* distance,
+                            double h, int64_t nx, int64_t ny, const double* fx, const double* fy) {
+    if (nx < 2 || ny < 2) return fail(BT_E_ARG, "the mesh needs 2+ nodes per axis");
+    if (nx > (1ll << 28) || ny > (1ll << 28) || nx * ny > (1ll << 40)) return fail(BT_E_ARG, "mesh too large");
+    if (!(h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    if (!phi || !solid || !near || !distance || !fx || !fy) return fail(BT_E_ARG, "null array");
+    return BT_OK;
+}
+
+int bt_edge_field_2d_device(const double* phi, const int32_t* solid, const uint8_t* near,
+                            const double* distance, const double* dx, double h, int64_t nx, int64_t ny,
+                            int32_t faces, double* fx, double* fy, void* stream) {
+    int rc = check_edge_field(phi, solid, near, distance, h, nx, ny, fx, fy);
+    if (rc) return rc;
+    if (bt::launch_edge_field(phi, solid, near, distance, dx, h, nx, ny, faces, fx, fy,
+                              static_cast<hipStream_t>(stream)))
+        return fail(BT_E_LAUNCH, std::string("edge field launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
+int bt_edge_field_2d(const double* phi, const int32_t* solid, const uint8_t* near, const double* distance,
+                     const double* dx, double h, int64_t nx, int64_t ny, int32_t faces, double* fx, double* fy) {
+    int rc = check_edge_field(phi, solid, near, distance, h, nx, ny, fx, fy);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    if ((rc = device_state(&s))) return rc;
+    if ((rc = host_stream(*s))) return rc;
+    hipStream_t st = s->stream;
+    const size_t nodes = (size_t)(nx * ny);
+    double *dphi, *ddist, *ddx = nullptr, *dfx, *dfy;
+    int32_t* dsolid;
+    uint8_t* dnear;
+    if ((rc = upload(*s, 0, phi, nodes, &dphi, st))) return rc;
+    if ((rc = upload(*s, 1, solid, nodes, &dsolid, st))) return rc;
+    if ((rc = upload(*s, 2, near, nodes, &dnear, st))) return rc;
+    if ((rc = upload(*s, 3, distance, 4 * nodes, &ddist, st))) return rc;
+    if (dx != nullptr && (rc = upload(*s, 4, dx, (size_t)(nx - 1), &ddx, st))) return rc;
+    if ((rc = upload(*s, 5, (const double*)nullptr, (size_t)((nx + 1) * ny), &dfx, st))) return rc;
+    if ((rc = upload(*s, 6, (const double*)nullptr, (size_t)(nx * (ny + 1)), &dfy, st))) return rc;
+    if (bt::launch_edge_field(dphi, dsolid, dnear, ddist, ddx, h, nx, ny, faces, dfx, dfy, st))
+        return fail(BT_E_LAUNCH, std::string("edge field launch: ") + hipGetErrorString(hipGetLastError()));
+    if ((rc = download(fx, dfx, (size_t)((nx + 1) * ny), st))) return rc;
+    if ((rc = download(fy, dfy, (size_t)(nx * (ny + 1)), st))) return rc;
+    hipError_t err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return hip_fail(err, "edge field");
+    return BT_OK;
+}
+
 int bt_deposit_trajectories_2d_device(const double* px, const double* py, const double* q, int64_t n,
                                       double origin_x, double origin_y, double h,
                                       int64_t cylindrical, double* charge, int64_t nx, int64_t ny,

@@ -100,6 +100,9 @@ int adjoint_occupancy(int integ, int geom, int block);
 int launch_arith_selftest(const double* y, const double* z, i64 n, double* r, double* cy, double* cz, double* rs,
                           hipStream_t stream);
 
+int launch_edge_field(const double* phi, const int32_t* solid, const uint8_t* near, const double* distance,
+                      const double* dx, double h, i64 nx, i64 ny, int faces, double* fx, double* fy,
+                      hipStream_t stream);
 int launch_gather(const double* px, const double* py, i64 n, const double* fx, const double* fy,
                   i64 nx, i64 ny, double ox, double oy, double h, double* ex, double* ey,
                   hipStream_t stream);

@@ -4,6 +4,8 @@
   edges along one axis, taken over the bracketed surface distance next to an electrode,
   extrapolated from the one adjacent surface on edges inside an electrode, and mirrored
   (``"symmetric"``) or linearly extrapolated on the two end faces.
+* :class:`DeviceEdgeField` computes the same edge fields on the GPU (the hip
+  ``edge_field_2d`` kernel) for a Vlasov iteration that keeps them in HBM.
 * :class:`StaggeredElectricField` restates ``solver/fields.py:105-166``: ``fx`` ``(nx + 1, ny)``
   and ``fy`` ``(nx, ny + 1)``; calling it gathers ``(E_x, E_y)`` with the hip
   ``staggered_field_2d`` kernel.
@@ -22,67 +24,66 @@ import numpy as np
 from beamtracer_amd.mesh import IonOpticsMesh
 
 
-def edge_field(phi, solid, near, dist_lo, dist_hi, h, extrapolation) -> np.ndarray:
-    """The ``(n + 1, ...)`` edge field ``(phi[i-1] - phi[i]) / h`` along axis 0."""
+def edge_field(phi, solid, near, dist_lo, dist_hi, lengths, mirror) -> np.ndarray:
+    """The ``n + 1`` edge values along axis 0 of the node potential ``phi`` ``(n, ...)``.
+
+    Edge ``k + 1`` joins nodes ``k`` and ``k + 1``: the potential drop over the spacing
+    ``lengths[k]``, or over the bracketed surface distance times the spacing where the edge
+    crosses from a free node into an electrode or out of one (``dist_hi`` of the free node
+    before the electrode, ``dist_lo`` of the one after; the latter wins). An edge with both
+    nodes in an electrode takes the value of its one neighbouring crossing edge, if only
+    one side has one, else 0. The end edges mirror (``mirror[0]``, ``mirror[1]`` for the
+    lower and upper end) or extrapolate linearly (``solver/fields.py:18-67``). ``near``
+    marks free nodes only (``IonOpticsMesh.near_solid``).
+    """
     n = phi.shape[0]
-    out = np.zeros((n + 1, *phi.shape[1:]))
-    column = (-1,) + (1,) * (phi.ndim - 1)
-    lengths = np.broadcast_to(np.asarray(h, dtype=float), (n - 1,)).reshape(column)
-    h = lengths
-    h_before = np.ones_like(lengths)
-    h_before[1:] = lengths[:-1]
-    h_after = np.ones_like(lengths)
-    h_after[:-1] = lengths[1:]
-    a, b = phi[:-1], phi[1:]
-    sa, sb = solid[:-1], solid[1:]
-    na, nb = near[:-1], near[1:]
-    field = (a - b) / h
-    case = na & sb
-    field = np.where(case, (a - b) / (dist_hi[:-1] * h), field)
-    case = nb & sa
-    field = np.where(case, (a - b) / (dist_lo[1:] * h), field)
-    both = sa & sb
-    near0 = np.zeros_like(na)
-    near0[1:] = near[:-2]
-    near3 = np.zeros_like(na)
-    near3[:-1] = near[2:]
-    phi0 = np.zeros_like(a)
-    phi0[1:] = phi[:-2]
-    phi3 = np.zeros_like(a)
-    phi3[:-1] = phi[2:]
-    d0 = np.ones_like(a)
-    d0[1:] = dist_hi[:-2]
-    d3 = np.ones_like(a)
-    d3[:-1] = dist_lo[2:]
-    inside = np.zeros_like(field)
-    inside = np.where(near0 & ~near3, (phi0 - a) / (d0 * h_before), inside)
-    inside = np.where(near3 & ~near0, (b - phi3) / (d3 * h_after), inside)
-    field = np.where(both, inside, field)
-    out[1:-1] = field
-    for end, (edge, first, second) in enumerate(((0, 1, 2), (n, n - 1, n - 2))):
-        if extrapolation[end] == "symmetric":
-            out[edge] = -out[first]
-        else:
-            out[edge] = 2.0 * out[first] - out[second]
+    span = np.broadcast_to(np.asarray(lengths, dtype=float), (n - 1,)).reshape((n - 1,) + (1,) * (phi.ndim - 1))
+    drop = phi[:-1] - phi[1:]
+    into = near[:-1] & solid[1:]  # a free node, then an electrode node
+    out_of = solid[:-1] & near[1:]  # an electrode node, then a free node
+    crossing = np.where(into, drop / (dist_hi[:-1] * span), drop / span)
+    crossing = np.where(out_of, drop / (dist_lo[1:] * span), crossing)
+    buried = solid[:-1] & solid[1:]
+    from_below = np.zeros_like(buried)
+    from_below[1:] = into[:-1]  # the previous edge crosses into this electrode
+    from_above = np.zeros_like(buried)
+    from_above[:-1] = out_of[1:]  # the next edge crosses out of it
+    before = np.zeros_like(crossing)
+    before[1:] = crossing[:-1]
+    after = np.zeros_like(crossing)
+    after[:-1] = crossing[1:]
+    inside = np.where(from_below & ~from_above, before, np.where(from_above & ~from_below, after, 0.0))
+    out = np.empty((n + 1,) + phi.shape[1:])
+    out[1:n] = np.where(buried, inside, crossing)
+    # the lower end first: with two nodes, the upper end reads it (and it reads 0 there)
+    out[0] = -out[1] if mirror[0] else 2.0 * out[1] - (out[2] if n > 2 else 0.0)
+    out[n] = -out[n - 1] if mirror[1] else 2.0 * out[n - 1] - out[n - 2]
     return out
+
+
+def face_mirrors(mesh: IonOpticsMesh, extrapolation: Sequence[str] | None = None) -> tuple[bool, bool, bool, bool]:
+    """Which faces (x_min, x_max, y_min, y_max) mirror the edge field: by default the axis of
+    a cylindrical mesh and a symmetry plane (``solver/fields.py:123-126``)."""
+    if extrapolation is None:
+        extrapolation = ["extrapolate"] * 4
+        if mesh.cylindrical or mesh.symmetry_plane:
+            extrapolation[2] = "symmetric"
+    return tuple(e == "symmetric" for e in extrapolation)  # type: ignore[return-value]
 
 
 class StaggeredElectricField:
     """The electric field of a potential on the edges between nodes (``fx``, ``fy``)."""
 
     def __init__(self, mesh: IonOpticsMesh, potential: np.ndarray, extrapolation: Sequence[str] | None = None):
-        if extrapolation is None:
-            extrapolation = ["extrapolate"] * 4
-            if mesh.cylindrical or mesh.symmetry_plane:
-                extrapolation[2] = "symmetric"
         self.mesh = mesh
-        self.extrapolation = tuple(extrapolation)
+        self.mirrors = face_mirrors(mesh, extrapolation)
+        self.extrapolation = tuple("symmetric" if m else "extrapolate" for m in self.mirrors)
         phi = np.asarray(potential, dtype=float)
         solid = mesh.solid >= 0
         near = mesh.near_solid
-        self.fx = edge_field(phi, solid, near, mesh.distance[0], mesh.distance[1], mesh.dx, self.extrapolation[0:2])
+        self.fx = edge_field(phi, solid, near, mesh.distance[0], mesh.distance[1], mesh.dx, self.mirrors[0:2])
         self.fy = np.ascontiguousarray(
-            edge_field(phi.T, solid.T, near.T, mesh.distance[2].T, mesh.distance[3].T, mesh.h, self.extrapolation[2:4]).T
+            edge_field(phi.T, solid.T, near.T, mesh.distance[2].T, mesh.distance[3].T, mesh.h, self.mirrors[2:4]).T
         )
 
     @classmethod
@@ -94,6 +95,7 @@ class StaggeredElectricField:
         field = cls.__new__(cls)
         field.mesh = mesh
         field.extrapolation = ()
+        field.mirrors = ()
         field.fx = np.ascontiguousarray(fx, dtype=float)
         field.fy = np.ascontiguousarray(fy, dtype=float)
         return field
@@ -112,6 +114,46 @@ class StaggeredElectricField:
                 *map(float, mesh.logical_origin), mesh.h, ex, ey,
             )  # fmt: skip
         return ex, ey
+
+
+class DeviceEdgeField:
+    """The edge fields of potentials on one mesh, computed on the GPU.
+
+    The mesh's node classes (``solid``, ``near_solid``, ``distance``, the x spacings of a
+    graded axis) go to HBM once; each call turns a potential (a NumPy array or a GPU tensor
+    ``(nx, ny)``) into ``fx`` ``(nx + 1, ny)`` and ``fy`` ``(nx, ny + 1)`` float64 GPU
+    tensors with the hip edge-field kernel, bit-identical to
+    :class:`StaggeredElectricField`'s (the tensors are reused by the next call).
+    """
+
+    def __init__(self, mesh: IonOpticsMesh, extrapolation: Sequence[str] | None = None):
+        import torch
+
+        self.mesh = mesh
+        self.mirrors = face_mirrors(mesh, extrapolation)
+        self.faces = sum(1 << k for k, m in enumerate(self.mirrors) if m)
+        nx, ny = mesh.shape
+        self.solid = torch.from_numpy(np.ascontiguousarray(mesh.solid, dtype=np.int32)).cuda()
+        self.near = torch.from_numpy(np.ascontiguousarray(mesh.near_solid, dtype=np.uint8)).cuda()
+        self.distance = torch.from_numpy(np.ascontiguousarray(mesh.distance, dtype=float)).cuda()
+        self.dx = torch.from_numpy(np.ascontiguousarray(mesh.dx, dtype=float)).cuda() if mesh.graded else None
+        self.fx = torch.empty((nx + 1, ny), dtype=torch.float64, device="cuda")
+        self.fy = torch.empty((nx, ny + 1), dtype=torch.float64, device="cuda")
+
+    def __call__(self, potential, stream=None):
+        import torch
+
+        from beamtracer_amd.kernels import edge_field_2d
+
+        if isinstance(potential, torch.Tensor):
+            phi = potential.to(device="cuda", dtype=torch.float64).contiguous()
+        else:
+            phi = torch.from_numpy(np.ascontiguousarray(potential, dtype=float)).cuda()
+        if tuple(phi.shape) != self.mesh.shape:
+            raise ValueError(f"the potential must be {self.mesh.shape}")
+        edge_field_2d(phi, self.solid, self.near, self.distance, self.dx, self.mesh.h, self.faces, self.fx, self.fy,
+                      stream=stream)  # fmt: skip
+        return self.fx, self.fy
 
 
 class AxisymmetricMagneticField:

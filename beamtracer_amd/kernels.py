@@ -52,7 +52,8 @@ def _dev(a: Any, dtype, name: str):
     """A contiguous CUDA (HIP) torch tensor of ``dtype``."""
     import torch
 
-    want = {np.dtype(np.int64): torch.int64, np.dtype(np.float64): torch.float64}[np.dtype(dtype)]
+    want = {np.dtype(np.int64): torch.int64, np.dtype(np.float64): torch.float64, np.dtype(np.int32): torch.int32,
+            np.dtype(np.uint8): torch.uint8}[np.dtype(dtype)]
     if not (_is_torch(a) and a.is_cuda):
         raise TypeError(f"{name} must be a GPU torch tensor like the other arrays")
     if a.dtype != want:
@@ -116,6 +117,41 @@ def staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey, *, stream=
             float(origin_x), float(origin_y), float(h), _ptr(ex), _ptr(ey),
         )  # fmt: skip
     _lib.check(rc, "staggered_field_2d")
+
+
+def edge_field_2d(phi, solid, near, distance, dx, h, faces, fx, fy, *, stream=None) -> None:
+    """The staggered edge fields of the node potential ``phi`` into ``fx`` ``(nx + 1, ny)`` and
+    ``fy`` ``(nx, ny + 1)`` (``solver/fields.py:18-67``, as ``StaggeredElectricField`` builds
+    them, ``fields.py:116-152``).
+
+    ``solid`` (int32, electrode index or -1), ``near`` (uint8, a free node next to an
+    electrode) and ``distance`` ``(4, nx, ny)`` are the mesh's node classes
+    (``IonOpticsMesh.solid``, ``near_solid``, ``distance``); ``dx`` the ``nx - 1`` spacings
+    along x or None (uniform ``h``); ``faces`` a bit per face (x_min, x_max, y_min, y_max)
+    that mirrors the field there (else linear extrapolation). NumPy arrays run the host
+    form, GPU tensors the device form on ``stream``.
+    """
+    L = _lib.lib()
+    device = _is_torch(phi)
+    check = _dev if device else _host
+    for a, dt, n in ((phi, _F64, "phi"), (solid, np.int32, "solid"), (near, np.uint8, "near"),
+                     (distance, _F64, "distance"), (fx, _F64, "fx"), (fy, _F64, "fy")):  # fmt: skip
+        check(a, dt, n)
+    if dx is not None:
+        check(dx, _F64, "dx")
+    if len(phi.shape) != 2:
+        raise ValueError("phi must be (nx, ny)")
+    nx, ny = (int(d) for d in phi.shape)
+    if tuple(solid.shape) != (nx, ny) or tuple(near.shape) != (nx, ny) or tuple(distance.shape) != (4, nx, ny):
+        raise ValueError("solid and near must be (nx, ny) and distance (4, nx, ny)")
+    if tuple(fx.shape) != (nx + 1, ny) or tuple(fy.shape) != (nx, ny + 1):
+        raise ValueError(f"fx and fy must be ({nx + 1}, {ny}) and ({nx}, {ny + 1})")
+    if dx is not None and tuple(dx.shape) != (nx - 1,):
+        raise ValueError(f"dx must hold the {nx - 1} spacings along x")
+    args = (_ptr(phi), _ptr(solid), _ptr(near), _ptr(distance), _ptr(dx), float(h), nx, ny, int(faces),
+            _ptr(fx), _ptr(fy))  # fmt: skip
+    rc = L.bt_edge_field_2d_device(*args, _stream(stream)) if device else L.bt_edge_field_2d(*args)
+    _lib.check(rc, "edge_field_2d")
 
 
 def _shape_gather(spx, spy, sfx, sfy, sex, sey) -> None:
@@ -199,7 +235,12 @@ def trace_exponent(active, v, dq, qm, fxs, fys, patch_real, step_fraction, inv_c
         dqmax = float(np.abs(dq[active]).max())
         qmax = float(np.abs(qm[active]).max())
         emax = max(float(np.abs(fxs).max()) if fxs.size else 0.0, float(np.abs(fys).max()) if fys.size else 0.0)
-    hmin = float(np.min(np.asarray(patch_real)[:, 2]))
+    return exponent_of(vmax, dqmax, qmax, emax, float(np.min(np.asarray(patch_real)[:, 2])), step_fraction)
+
+
+def exponent_of(vmax: float, dqmax: float, qmax: float, emax: float, hmin: float, step_fraction: float) -> int:
+    """:func:`trace_exponent` from its maxima: the fastest start ``vmax``, the largest
+    ``|dq|`` and ``|q/m|``, the strongest edge field ``emax`` and the finest spacing."""
     dt = math.inf
     if vmax > 0.0:
         dt = hmin / vmax

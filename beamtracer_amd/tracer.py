@@ -193,15 +193,20 @@ class HipBeamTracer:
             self._lookup = (cells, self._subcells(ci, cj).ravel())
         return self._lookup
 
-    def patch_arrays(self) -> tuple:
-        """``fxs``, ``fys``, ``patch_int`` (P, 6), ``patch_real`` (P, 7) of the field (one patch)."""
+    def patch_tables(self) -> tuple[np.ndarray, np.ndarray]:
+        """``patch_int`` (P, 6) and ``patch_real`` (P, 7) of the mesh (one patch)."""
         mesh = self.mesh
-        f = self.electric
         (bx0, by0), (bx1, by1) = mesh.origin, mesh.upper
         box = (float(mesh.logical_x(bx0)), by0, float(mesh.logical_x(bx1)), by1)
         nxp, nyp = mesh.shape
         patch_int = np.array([[nxp, nyp, 0, 0, nxp * nyp, 0]], dtype=np.int64)
         patch_real = np.array([[*mesh.logical_origin, mesh.h, *box]], dtype=float)
+        return patch_int, patch_real
+
+    def patch_arrays(self) -> tuple:
+        """``fxs``, ``fys``, ``patch_int`` (P, 6), ``patch_real`` (P, 7) of the field (one patch)."""
+        f = self.electric
+        patch_int, patch_real = self.patch_tables()
         fxs = np.ascontiguousarray(f.fx, dtype=float).ravel()
         fys = np.ascontiguousarray(f.fy, dtype=float).ravel()
         return fxs, fys, patch_int, patch_real

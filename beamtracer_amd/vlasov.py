@@ -7,6 +7,12 @@ initial plasma held at the plasma potential before), builds the staggered edge f
 traces the same beam again on the GPU and mixes the new charge density with the old one,
 ``rho <- (1 - relaxation) rho + relaxation rho_new`` (the first iteration takes the new one).
 It stops after ``iterations`` or when the largest potential change is below ``tolerance``.
+
+``device=True`` keeps the particle stage in HBM: the edge fields come from the hip
+edge-field kernel (:class:`~beamtracer_amd.fields.DeviceEdgeField`) and the beam is traced
+by a :class:`~beamtracer_amd.resident.ResidentTrace`, so an iteration moves only the
+potential up and the node charges down. Every potential, charge density and record is
+bit-identical to the host-array loop's.
 """
 
 from __future__ import annotations
@@ -18,7 +24,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from beamtracer_amd.beam import BeamParticles
-from beamtracer_amd.fields import StaggeredElectricField
+from beamtracer_amd.fields import DeviceEdgeField, StaggeredElectricField
 from beamtracer_amd.mesh import IonOpticsMesh
 from beamtracer_amd.poisson import BoltzmannElectrons, IonOpticsPoisson
 from beamtracer_amd.tracer import HipBeamTracer, TraceResult
@@ -41,12 +47,14 @@ class PlasmaModel:
 
 @dataclass
 class VlasovState:
-    """The state after the last iteration (``solver/vlasov.py:218-238``)."""
+    """The state after the last iteration (``solver/vlasov.py:218-238``). With
+    ``device=True`` the states passed to the callback during the loop carry no ``electric``
+    and ``trace`` (None: they stay on the GPU); the returned state carries both."""
 
     potential: np.ndarray
     charge_density: np.ndarray
-    electric: StaggeredElectricField
-    trace: TraceResult
+    electric: StaggeredElectricField | None
+    trace: TraceResult | None
     history: list[dict] = field(default_factory=list)
 
 
@@ -61,10 +69,14 @@ def vlasov_iterate(
     relaxation: float = 1.0,
     poisson: IonOpticsPoisson | None = None,
     callback: Callable[[int, VlasovState], None] | None = None,
+    device: bool = False,
 ) -> VlasovState:
     """Run ``iterations`` Vlasov iterations; return the last :class:`VlasovState`."""
     if not 0.0 < relaxation <= 1.0:
         raise ValueError(f"relaxation must be in (0, 1], got {relaxation}")
+    edges = DeviceEdgeField(mesh) if device else None
+    resident = None
+    fields_ = None
     poisson = poisson if poisson is not None else IonOpticsPoisson(mesh)
     rho = np.zeros(mesh.shape)
     potential = None
@@ -85,13 +97,26 @@ def vlasov_iterate(
         t1 = time.perf_counter()
         change = float("inf") if potential is None else float(np.max(np.abs(new - potential)))
         potential = new
-        efield = StaggeredElectricField(mesh, potential)
-        tracer.electric = efield
         particles, j_over_v = make_particles()
-        t2 = time.perf_counter()
-        trace = tracer.trace(particles)
-        t3 = time.perf_counter()
-        deposited = trace.charge_density
+        if edges is not None:
+            from beamtracer_amd.resident import ResidentTrace
+
+            fields_ = edges(potential)
+            efield = trace = None
+            if resident is None or resident.particles is not particles:
+                resident = ResidentTrace(tracer, particles)
+            t2 = time.perf_counter()
+            deposited = resident.trace(*fields_)
+            t3 = time.perf_counter()
+            steps, particle_steps = resident.steps, resident.particle_steps
+        else:
+            efield = StaggeredElectricField(mesh, potential)
+            tracer.electric = efield
+            t2 = time.perf_counter()
+            trace = tracer.trace(particles)
+            t3 = time.perf_counter()
+            deposited = trace.charge_density
+            steps, particle_steps = trace.steps, trace.particle_steps
         if iteration == 0:
             rho = deposited
         else:
@@ -100,10 +125,11 @@ def vlasov_iterate(
             "iteration": iteration,
             "max_potential_change": change,
             "newton_iterations": poisson.newton_iterations,
-            "steps": trace.steps,
-            "particle_steps": trace.particle_steps,
+            "steps": steps,
+            "particle_steps": particle_steps,
             "trajectories": len(particles),
             "field_solve_s": t1 - t0,
+            "edge_field_s": t2 - t1,
             "trace_s": t3 - t2,
         })
         state = VlasovState(potential, rho, efield, trace, history)
@@ -112,4 +138,8 @@ def vlasov_iterate(
         if change < tolerance:
             break
     assert state is not None
+    if resident is not None and fields_ is not None:
+        state.electric = StaggeredElectricField.from_edges(mesh, fields_[0].cpu().numpy(), fields_[1].cpu().numpy())
+        tracer.electric = state.electric
+        state.trace = resident.result(deposited)
     return state

@@ -20,8 +20,9 @@ Beside the headline the line carries, under ``extra``:
   H+ and H2+ 0.8:0.2, 1e6 trajectories) traced with Boris through the sampled field of a
   0.2 T current loop (node B), on the field of the initial plasma: particle-steps/s and
   FP64 FLOP/s at 257 FLOP per Boris+B step;
-* ``vlasov`` -- Vlasov iterations/s of the einzel lens (host Poisson solve, edge fields,
-  host-form trace with its copies, relaxation 0.5), with the time split;
+* ``vlasov`` -- Vlasov iterations/s of the einzel lens (host Poisson solve, relaxation 0.5)
+  with host arrays (NumPy edge fields, host-form trace with its copies) and device-resident
+  (hip edge fields, the beam in HBM), with the time split;
 * ``batched`` -- 256 einzel candidates (mesh, voltage, gap, energy and beam size drawn from
   one seed) traced in one batched launch, against one plain launch per design: particle-
   steps/s of each and their ratio, and the rate of the captured HIP graph;
@@ -243,24 +244,41 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
 
 
 def vlasov_leg(design, iterations: int) -> dict:
+    """The einzel lens's Vlasov loop twice: host arrays (NumPy edge fields, the host-form
+    trace with its copies) and device-resident (hip edge fields, the beam in HBM); the
+    potentials of the two are checked bit for bit."""
     from beamtracer_amd.tracer import HipBeamTracer
     from beamtracer_amd.vlasov import vlasov_iterate
 
-    tracer = HipBeamTracer(design.mesh, None, planes=design.planes, integrator=design.integrator)
-    t0 = time.perf_counter()
-    state = vlasov_iterate(design.mesh, lambda: (design.particles, design.j_over_v), tracer,
-                           iterations=iterations, relaxation=0.5)  # fmt: skip
-    wall = time.perf_counter() - t0
-    h = state.history[1:]  # the first iteration also factorizes the Poisson matrix
-    per = float(np.mean([x["field_solve_s"] + x["trace_s"] for x in h]))
+    out = {}
+    states = {}
+    for mode in ("host", "device"):
+        tracer = HipBeamTracer(design.mesh, None, planes=design.planes, integrator=design.integrator)
+        t0 = time.perf_counter()
+        state = vlasov_iterate(design.mesh, lambda: (design.particles, design.j_over_v), tracer,
+                               iterations=iterations, relaxation=0.5, device=mode == "device")  # fmt: skip
+        wall = time.perf_counter() - t0
+        h = state.history[1:]  # the first iteration also factorizes and builds the lookups
+        per = float(np.mean([x["field_solve_s"] + x["edge_field_s"] + x["trace_s"] for x in h]))
+        out[mode] = {
+            "iterations_per_s": iterations / wall,
+            "steady_iterations_per_s": 1.0 / per if per > 0 else None,
+            "field_solve_s": float(np.mean([x["field_solve_s"] for x in h])),
+            "edge_field_s": float(np.mean([x["edge_field_s"] for x in h])),
+            "trace_s": float(np.mean([x["trace_s"] for x in h])),
+        }
+        states[mode] = state
+    same = bool(np.array_equal(states["host"].potential, states["device"].potential)
+                and np.array_equal(states["host"].charge_density, states["device"].charge_density))  # fmt: skip
     return {
         "workload": f"einzel lens Vlasov loop, {len(design.particles)} trajectories, relaxation 0.5, "
-        f"{iterations} iterations (host Poisson + edge fields + host-form hip trace with its copies)",
-        "iterations_per_s": iterations / wall,
-        "steady_iterations_per_s": 1.0 / per if per > 0 else None,
-        "field_solve_s": float(np.mean([x["field_solve_s"] for x in h])),
-        "trace_s": float(np.mean([x["trace_s"] for x in h])),
-        "max_potential_change_last": state.history[-1]["max_potential_change"],
+        f"{iterations} iterations: host Poisson, then edge fields + trace from host arrays ('host') or "
+        "hip edge fields + the beam resident in HBM ('device')",
+        **out["host"],
+        "device": out["device"],
+        "device_speedup": out["device"]["steady_iterations_per_s"] / out["host"]["steady_iterations_per_s"],
+        "device_bit_identical": same,
+        "max_potential_change_last": states["device"].history[-1]["max_potential_change"],
     }
 
 
@@ -602,7 +620,7 @@ def main() -> int:
         log("extraction: Boris + node B, 1e6 trajectories")
         extra = {"extraction": extraction_leg(opts.extraction_particles, max(2, opts.steps // 2), opts.block_size)}
         log("vlasov: einzel-lens iterations")
-        extra["vlasov"] = vlasov_leg(design, 3)
+        extra["vlasov"] = vlasov_leg(design, 4)
         log(f"batched: {opts.batch_designs} einzel candidates")
         processes = min(16, len(os.sched_getaffinity(0)))
         extra["batched"] = batched_leg(opts.batch_designs, 0, processes, 3)

@@ -9,6 +9,10 @@
  *   bt_trace_trajectories_2d   kernels/trace_trajectories_2d/trace_trajectories_2d_pyccel.py:42-78
  *                              (called from solver/tracer.py:1237-1247)
  *
+ * and, on the field side of the trace, the edge fields of a potential:
+ *
+ *   bt_edge_field_2d           solver/fields.py:18-67 (StaggeredElectricField, fields.py:116-152)
+ *
  * Every entry point exists in two forms:
  *   bt_<name>          host arrays (NumPy memory), synchronous: the drop-in for the Pyccel
  *                      kernel.  Arrays are copied to the device and the outputs back.
@@ -139,6 +143,26 @@ int bt_deposit_trajectories_2d_device(const double* px, const double* py, const 
                                       int64_t ny, int32_t deposit_exponent,
                                       void* workspace, int64_t workspace_bytes,
                                       void* stream);
+
+/* ---- the trace's input: edge fields of a potential -----------------------------------
+ * The staggered edge fields of the node potential phi (nx, ny): fx (nx + 1, ny) along x and
+ * fy (nx, ny + 1) along r.  Replaces the host computation of solver/fields.py:18-67
+ * (_edge_field) as StaggeredElectricField builds them (solver/fields.py:116-152): a
+ * Vlasov iteration can then keep the fields in HBM from the potential to the trace.
+ *   solid     (nx, ny) electrode index of each node, or -1 (IonOpticsMesh.solid);
+ *   near      (nx, ny) 1 for a free node with an electrode neighbour, else 0 (never 1 on an
+ *             electrode node: IonOpticsMesh.near_solid);
+ *   distance  (4, nx, ny) the bracketed surface distance in units of the edge towards -x,
+ *             +x, -r, +r (1 where no surface; IonOpticsMesh.distance);
+ *   dx        the nx - 1 spacings along x (a graded axis), or NULL for h;
+ *   faces     bit k set: face k (x_min, x_max, y_min, y_max) mirrors the field (the axis or
+ *             a symmetry plane), else the end edge extrapolates linearly.
+ * Bit-identical to the reference's NumPy arithmetic (IEEE quotients). */
+int bt_edge_field_2d(const double* phi, const int32_t* solid, const uint8_t* near, const double* distance,
+                     const double* dx, double h, int64_t nx, int64_t ny, int32_t faces, double* fx, double* fy);
+int bt_edge_field_2d_device(const double* phi, const int32_t* solid, const uint8_t* near,
+                            const double* distance, const double* dx, double h, int64_t nx, int64_t ny,
+                            int32_t faces, double* fx, double* fy, void* stream);
 
 /* ---- batched form ---------------------------------------------------------------------
  * Many designs (meshes, fields, beams) traced in one persistent launch: the reference

@@ -11,6 +11,8 @@ package ``beamtracer_amd`` never imports this module.
   ``src/beamtracer/kernels/``).
 * ``tangent.c`` (in the same library) is the forward-mode tangent of the C tracer along a
   direction of the edge fields: the FP64 CPU tangent that the GPU adjoint must transpose.
+* ``oracle.c`` also restates ``_edge_field`` (``solver/fields.py:18-67``), the staggered
+  edge fields of a potential: the checker of the hip edge-field kernel.
 * ``numpy_ref.py`` restates the vectorized NumPy versions
   (``staggered_field_2d_numpy.py:12-39``, ``deposit_trajectories_2d_numpy.py:12-42``) and
   the step-synchronous NumPy tracer of ``solver/tracer.py:851-1054`` as an independent
@@ -192,6 +194,37 @@ def trace_tangent(*args, dfx, dfy) -> np.ndarray:
     if L.oracle_trace_tangent(ctypes.byref(a), _p(dfx), _p(dfy), _p(rec_d)) != 0:
         raise ValueError("the tangent takes one uniform patch, classical, without a magnetic field")
     return rec_d
+
+
+def edge_fields(phi, solid, near, distance, dx, h, faces) -> tuple[np.ndarray, np.ndarray]:
+    """``fx`` ``(nx + 1, ny)`` and ``fy`` ``(nx, ny + 1)`` of the node potential ``phi``: the
+    reference's ``_edge_field`` (``solver/fields.py:18-67``) along x with the spacings ``dx``
+    (``nx - 1``) and along r with ``h``, as ``StaggeredElectricField`` calls it
+    (``fields.py:116-152``). ``solid`` holds electrode indices (or -1), ``near`` the free
+    nodes next to an electrode, ``distance`` ``(4, nx, ny)`` the surface distances towards
+    -x, +x, -r, +r; bit k of ``faces`` mirrors the field at face k (x_min, x_max, y_min,
+    y_max), else the end edge extrapolates linearly."""
+    phi = _check(np.ascontiguousarray(phi, np.float64), np.float64, "phi")
+    nx, ny = phi.shape
+    solid = np.ascontiguousarray(solid, np.int32)
+    near = np.ascontiguousarray(near, np.uint8)
+    distance = _check(np.ascontiguousarray(distance, np.float64), np.float64, "distance")
+    dx = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, np.float64), (nx - 1,)))
+    if solid.shape != (nx, ny) or near.shape != (nx, ny) or distance.shape != (4, nx, ny):
+        raise ValueError("solid, near must be (nx, ny) and distance (4, nx, ny)")
+    fx = np.empty((nx + 1, ny))
+    fy = np.empty((nx, ny + 1))
+    ry = np.full(ny - 1, float(h))
+    L = lib()
+    L.oracle_edge_field.restype = None
+    L.oracle_edge_field.argtypes = [_ptr] * 6 + [_i64] * 6 + [_ptr, _i64, _i64]
+    nodes = nx * ny
+    d = distance.reshape(4, nodes)
+    L.oracle_edge_field(_p(phi), _p(solid), _p(near), _p(d[0]), _p(d[1]), _p(dx),
+                        nx, ny, ny, 1, faces & 1, (faces >> 1) & 1, _p(fx), ny, 1)  # fmt: skip
+    L.oracle_edge_field(_p(phi), _p(solid), _p(near), _p(d[2]), _p(d[3]), _p(ry),
+                        ny, nx, 1, ny, (faces >> 2) & 1, (faces >> 3) & 1, _p(fy), 1, ny + 1)  # fmt: skip
+    return fx, fy
 
 
 def default_threads() -> int:

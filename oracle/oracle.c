@@ -11,6 +11,8 @@
  *   deposit_trajectories_2d  kernels/deposit_trajectories_2d/deposit_trajectories_2d_pyccel.py:13-59
  *   trace_trajectories_2d    kernels/trace_trajectories_2d/trace_trajectories_2d_pyccel.py:19-507
  *                            (second witness: trace_trajectories_2d_cuda.cu:12-362)
+ *   edge_field               solver/fields.py:18-67 (_edge_field: the staggered edge fields of a
+ *                            potential, the input of the trace)
  *
  * Extension beyond the reference's compiled kernel (which refuses a magnetic field,
  * solver/tracer.py:1060-1090): a node magnetic field (three components on the base mesh's
@@ -540,3 +542,55 @@ void oracle_trace_trajectories_2d(const oracle_trace_args* a) {
 
 /* The size of the argument struct, checked by the Python side against its ctypes mirror. */
 i64 oracle_trace_args_size(void) { return (i64)sizeof(oracle_trace_args); }
+
+/* solver/fields.py:18-67 (_edge_field), one edge of one line at a time.  Along an axis of
+ * n nodes (node t of line c at phi[t * st + c * sc]) the n + 1 edges of line c go to
+ * out[e * ot + c * oc]; lengths[t] is the spacing from node t to node t + 1.  The selection
+ * order is the reference's: the plain difference, replaced by the surface-distance form
+ * for a near node before an electrode node, then for an electrode node before a near node,
+ * then, on an edge with both nodes in an electrode, by the surface edge of the one side
+ * that has one (else 0). */
+static double oracle_edge_interior(const double* phi, const int32_t* solid, const uint8_t* near,
+                                   const double* dist_lo, const double* dist_hi, const double* lengths,
+                                   i64 n, i64 st, i64 c, i64 e) {
+    const i64 ta = e - 1, tb = e;                     /* the edge's nodes */
+    const i64 ia = ta * st + c, ib = tb * st + c;
+    const double a = phi[ia], b = phi[ib];
+    const double len = lengths[ta];
+    const double len_before = ta >= 1 ? lengths[ta - 1] : 1.0;
+    const double len_after = ta + 1 <= n - 2 ? lengths[ta + 1] : 1.0;
+    const int sa = solid[ia] >= 0, sb = solid[ib] >= 0;
+    double field = (a - b) / len;
+    if (near[ia] && sb) field = (a - b) / (dist_hi[ia] * len);
+    if (near[ib] && sa) field = (a - b) / (dist_lo[ib] * len);
+    if (sa && sb) {
+        const int near0 = ta >= 1 ? near[ia - st] != 0 : 0;
+        const int near3 = tb + 1 <= n - 1 ? near[ib + st] != 0 : 0;
+        const double phi0 = ta >= 1 ? phi[ia - st] : 0.0;
+        const double phi3 = tb + 1 <= n - 1 ? phi[ib + st] : 0.0;
+        const double d0 = ta >= 1 ? dist_hi[ia - st] : 1.0;
+        const double d3 = tb + 1 <= n - 1 ? dist_lo[ib + st] : 1.0;
+        double inside = 0.0;
+        if (near0 && !near3) inside = (phi0 - a) / (d0 * len_before);
+        if (near3 && !near0) inside = (b - phi3) / (d3 * len_after);
+        field = inside;
+    }
+    return field;
+}
+
+void oracle_edge_field(const double* phi, const int32_t* solid, const uint8_t* near,
+                       const double* dist_lo, const double* dist_hi, const double* lengths,
+                       i64 n, i64 m, i64 st, i64 sc, i64 symmetric_lo, i64 symmetric_hi,
+                       double* out, i64 ot, i64 oc) {
+    for (i64 c = 0; c < m; ++c) {
+        const i64 node0 = c * sc, edge0 = c * oc;
+        for (i64 e = 0; e <= n; ++e) out[e * ot + edge0] = 0.0;
+        for (i64 e = 1; e <= n - 1; ++e)
+            out[e * ot + edge0] = oracle_edge_interior(phi + node0, solid + node0, near + node0, dist_lo + node0,
+                                                       dist_hi + node0, lengths, n, st, 0, e);
+        /* the two end faces, in this order (the second may read the first when n == 2) */
+        out[edge0] = symmetric_lo ? -out[ot + edge0] : 2.0 * out[ot + edge0] - out[2 * ot + edge0];
+        out[n * ot + edge0] = symmetric_hi ? -out[(n - 1) * ot + edge0]
+                                           : 2.0 * out[(n - 1) * ot + edge0] - out[(n - 2) * ot + edge0];
+    }
+}
