@@ -124,3 +124,55 @@ def test_device_vlasov_loop_equals_the_host_loop_on_the_extraction_with_plasma()
     got, want = states[1], states[0]
     assert want.history[1]["newton_iterations"] > 1
     _assert_identical_states(got, want)
+
+
+def _aperture_design(n=300):
+    """A planar beam onto an electrode with a round hole and a slab: the beam's sub-cells
+    are cut by the surfaces, so trajectories pause and are decided on the host."""
+    from beamtracer_amd import mesh as meshmod
+    from beamtracer_amd.beam import beam_with_energy
+
+    def disk(x, y):
+        x, y = np.asarray(x), np.asarray(y)
+        return ((x - 6e-3) ** 2 + (y - 2.5e-3) ** 2 < (1.3e-3) ** 2) | ((x > 9e-3) & (x < 9.6e-3) & (y > 3.2e-3))
+
+    faces = {"x_min": meshmod.FaceCondition("dirichlet", 0.0), "x_max": meshmod.FaceCondition("dirichlet", -2000.0)}
+    mesh = meshmod.IonOpticsMesh((61, 31), (0.0, 0.0), 0.2e-3, "planar", (meshmod.Electrode(disk, -500.0),), faces)
+    particles, jv = beam_with_energy(n, 10.0, 1.0, 1.007276, 100.0, 0.0, 0.5, (0.0, 0.2e-3), (0.0, 5.8e-3),
+                                     cylindrical=False)  # fmt: skip
+    return designs.Design(mesh, particles, jv, (4e-3, 11e-3))
+
+
+def test_resident_trace_pauses_and_retraces_like_the_caller():
+    from beamtracer_amd.resident import ResidentTrace
+
+    design = _aperture_design()
+    phi = poisson.IonOpticsPoisson(design.mesh).solve()
+    field = fields.StaggeredElectricField(design.mesh, phi)
+    caller = HipBeamTracer(design.mesh, field, planes=design.planes, save_trajectories=7)
+    caller._capacity = (1, 8)  # the plane and path buffers overflow: the trace repeats
+    want = caller.trace(design.particles)
+    assert caller._capacity[1] > 8
+    tracer = HipBeamTracer(design.mesh, None, planes=design.planes, save_trajectories=7)
+    tracer._capacity = (1, 8)
+    resident = ResidentTrace(tracer, design.particles)
+    decided = []
+    inside = design.mesh.inside_solid
+    design.mesh.inside_solid = lambda x, y: decided.append(np.size(x)) or inside(x, y)  # count the paused
+    try:
+        density = resident.trace(*fields.DeviceEdgeField(design.mesh)(phi))
+    finally:
+        del design.mesh.inside_solid
+    assert sum(decided) > 0  # trajectories paused on cut sub-cells and were decided on the host
+    assert tracer._capacity == caller._capacity
+    got = resident.result(density)
+    np.testing.assert_array_equal(got.charge_density, want.charge_density)
+    np.testing.assert_array_equal(got.status, want.status)
+    np.testing.assert_array_equal(got.electrode, want.electrode)
+    assert (want.status == 3).sum() > 10
+    for p, q in zip(got.planes, want.planes):
+        np.testing.assert_array_equal(p.point, q.point)
+        np.testing.assert_array_equal(p.velocity, q.velocity)
+    assert len(got.trajectories) == len(want.trajectories) > 10
+    for p, q in zip(got.trajectories, want.trajectories):
+        np.testing.assert_array_equal(p, q)
