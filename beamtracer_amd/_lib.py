@@ -15,7 +15,7 @@ PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIB_PATH = PACKAGE / "libbeamtracer_hip.so"
 SOURCES = [PACKAGE / "csrc" / name for name in ("bt_api.hip", "bt_trace.hip", "bt_gather_deposit.hip",
-                                                "bt_adjoint.hip", "bt_fields.hip")]
+                                                "bt_adjoint.hip", "bt_fields.hip", "bt_trace3d.hip")]
 HEADERS = [ROOT / "include" / "beamtracer_hip.h"] + sorted((PACKAGE / "csrc").glob("*.h"))
 
 #: hipcc flags of the library: FP64 results in the reference's operation order (no fused
@@ -79,6 +79,35 @@ class TraceArgs(ctypes.Structure):
         ("deposit_exponent", _i32), ("block_size", _i32), ("max_blocks", _i32), ("flags", _i32),
         ("workspace", _ptr), ("workspace_bytes", _i64),
         ("checkpoints", _ptr), ("n_checkpoints", _i64), ("checkpoint_interval", _i64),
+    ]  # fmt: skip
+
+
+class Trace3DArgs(ctypes.Structure):
+    """ctypes mirror of ``bt_trace3d_args``."""
+
+    _fields_ = [
+        ("active", _ptr), ("n_active", _i64),
+        ("x", _ptr), ("v", _ptr), ("n", _i64),
+        ("qm", _ptr), ("dq", _ptr),
+        ("steps", _ptr), ("status", _ptr), ("electrode", _ptr),
+        ("fxs", _ptr), ("n_fxs", _i64), ("fys", _ptr), ("n_fys", _i64), ("fzs", _ptr), ("n_fzs", _i64),
+        ("patch_int", _ptr), ("patch_real", _ptr), ("n_patches", _i64),
+        ("bn", _ptr), ("has_b", _i64),
+        ("origin_x", _dbl), ("origin_y", _dbl), ("origin_z", _dbl), ("h", _dbl),
+        ("upper_x", _dbl), ("upper_y", _dbl), ("upper_z", _dbl),
+        ("mirror_y", _i64), ("mirror_z", _i64), ("integrator", _i64),
+        ("step_fraction", _dbl), ("max_steps", _i64),
+        ("cells", _ptr), ("cells_n0", _i64), ("cells_n1", _i64), ("cells_n2", _i64),
+        ("subcells", _ptr), ("n_subcells", _i64), ("refine", _i64),
+        ("planes", _ptr), ("n_planes", _i64),
+        ("hits", _ptr), ("capacity", _i64), ("counts", _ptr),
+        ("path_slot", _ptr), ("paths", _ptr), ("n_paths", _i64), ("path_capacity", _i64), ("path_count", _ptr),
+        ("num_threads", _i64),
+        ("charge", _ptr), ("charge_rows", _i64), ("charge_cols", _i64),
+        ("has_j", _i64), ("current", _ptr), ("current_rows", _i64), ("current_cols", _i64),
+        ("inv_c2", _dbl),
+        ("deposit_exponent", _i32), ("current_exponent", _i32),
+        ("workspace", _ptr), ("workspace_bytes", _i64),
     ]  # fmt: skip
 
 
@@ -149,6 +178,10 @@ SIGNATURES = {
     "bt_trace_workspace_bytes": (_i64, [_i64]),
     "bt_trace_trajectories_2d": (ctypes.c_int, [ctypes.POINTER(TraceArgs)]),
     "bt_trace_trajectories_2d_device": (ctypes.c_int, [ctypes.POINTER(TraceArgs), _ptr]),
+    "bt_trace3d_args_size": (_i64, []),
+    "bt_trace3d_workspace_bytes": (_i64, [ctypes.POINTER(Trace3DArgs)]),
+    "bt_trace_trajectories_3d": (ctypes.c_int, [ctypes.POINTER(Trace3DArgs)]),
+    "bt_trace_trajectories_3d_device": (ctypes.c_int, [ctypes.POINTER(Trace3DArgs), _ptr]),
     "bt_staggered_field_2d": (
         ctypes.c_int,
         [_ptr, _ptr, _i64, _ptr, _ptr, _i64, _i64, _dbl, _dbl, _dbl, _ptr, _ptr],
@@ -217,6 +250,8 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = argtypes
         if handle.bt_trace_args_size() != ctypes.sizeof(TraceArgs):
             raise HipLibraryError("bt_trace2d_args layout differs from the ctypes mirror")
+        if handle.bt_trace3d_args_size() != ctypes.sizeof(Trace3DArgs):
+            raise HipLibraryError("bt_trace3d_args layout differs from the ctypes mirror")
         _LIB = handle
     return _LIB
 

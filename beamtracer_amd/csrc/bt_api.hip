@@ -552,6 +552,189 @@ int bt_staggered_field_2d(const double* px, const double* py, int64_t n, const d
     return BT_OK;
 }
 
+// ---- the 3D tracer ---------------------------------------------------------------------
+
+int64_t bt_trace3d_args_size(void) { return (int64_t)sizeof(bt_trace3d_args); }
+
+// The launch-independent checks of a 3D trace (host-readable tables only): every patch's
+// fields and nodes within their arrays, the lookup the base mesh's shape.
+static int check_trace3d(const bt_trace3d_args* a) {
+    if (a == nullptr) return fail(BT_E_ARG, "args is NULL");
+    if (a->n < 0 || a->n_active < 0) return fail(BT_E_ARG, "negative trajectory count");
+    if (a->n >= kMax32 || a->n_active >= kMax32) return fail(BT_E_ARG, "2^31 or more trajectories");
+    if (a->n_patches < 1 || a->n_patches > kMaxPatches) return fail(BT_E_ARG, "n_patches must be 1..16");
+    if (a->patch_int == nullptr || a->patch_real == nullptr) return fail(BT_E_ARG, "patch tables are NULL");
+    if (a->n_planes < 0 || a->n_planes > kMaxPlanes) return fail(BT_E_ARG, "n_planes must be 0..256");
+    if (a->n_planes > 0 && a->planes == nullptr) return fail(BT_E_ARG, "planes is NULL");
+    if (a->integrator < 0 || a->integrator > 1) return fail(BT_E_ARG, "integrator must be 0 or 1");
+    if (!(a->h > 0.0)) return fail(BT_E_ARG, "h must be positive");
+    if (a->refine < 1 || a->refine > 1024) return fail(BT_E_ARG, "refine must be 1..1024");
+    if (a->capacity < 0 || a->path_capacity < 0 || a->n_paths < 0) return fail(BT_E_ARG, "negative buffer size");
+    if (a->charge == nullptr || a->charge_rows < 1) return fail(BT_E_ARG, "charge needs a row");
+    for (i64 q = 0; q < a->n_patches; ++q) {
+        const int64_t* pi = a->patch_int + q * 8;
+        const double ph = a->patch_real[q * 10 + 3];
+        if (pi[0] < 2 || pi[1] < 2 || pi[2] < 2 || pi[0] > 65536 || pi[1] > 65536 || pi[2] > 65536)
+            return fail(BT_E_ARG, "a patch needs 2..65536 nodes per axis");
+        if (!(ph > 0.0)) return fail(BT_E_ARG, "a patch spacing must be positive");
+        const i64 cells = pi[0] * pi[1] * pi[2];
+        if (cells >= kMax32) return fail(BT_E_ARG, "a patch of 2^31 or more nodes");
+        if (pi[3] < 0 || pi[3] + (pi[0] + 1) * pi[1] * pi[2] > a->n_fxs || pi[4] < 0 ||
+            pi[4] + pi[0] * (pi[1] + 1) * pi[2] > a->n_fys || pi[5] < 0 || pi[5] + pi[0] * pi[1] * (pi[2] + 1) > a->n_fzs)
+            return fail(BT_E_ARG, "a patch's edge fields exceed fxs, fys or fzs");
+        if (pi[7] < 0 || pi[7] + cells > a->charge_cols) return fail(BT_E_ARG, "a patch's nodes exceed charge");
+    }
+    const int64_t* base = a->patch_int + (a->n_patches - 1) * 8;
+    if (a->cells == nullptr || a->cells_n0 != base[0] - 1 || a->cells_n1 != base[1] - 1 || a->cells_n2 != base[2] - 1)
+        return fail(BT_E_ARG, "cells must be (nx - 1, ny - 1, nz - 1) of the base mesh");
+    if (a->subcells == nullptr || a->n_subcells < 1) return fail(BT_E_ARG, "subcells is empty");
+    if (a->has_b != 0 && a->bn == nullptr) return fail(BT_E_ARG, "has_b without bn");
+    const i64 nb = base[0] * base[1] * base[2];
+    if (a->has_j != 0 && (a->current == nullptr || a->current_rows < 1 || a->current_cols < 3 * nb))
+        return fail(BT_E_ARG, "has_j needs current of 3 nx ny nz columns");
+    if (a->n > 0 && (a->x == nullptr || a->v == nullptr || a->qm == nullptr || a->dq == nullptr || a->steps == nullptr ||
+                     a->status == nullptr || a->electrode == nullptr || a->path_slot == nullptr))
+        return fail(BT_E_ARG, "null trajectory array");
+    if (a->n_active > 0 && a->active == nullptr) return fail(BT_E_ARG, "active is NULL");
+    return BT_OK;
+}
+
+static size_t trace3d_bytes(const bt_trace3d_args* a) {
+    const i64 current_nodes = a->has_j != 0 ? a->current_cols : 0;
+    return node_bytes(a->charge_cols) + node_bytes(current_nodes) + round256((size_t)a->n_patches * 8 * 8) +
+           round256((size_t)a->n_patches * 10 * 8) + round256((size_t)(a->n_planes + 1) * 8);
+}
+
+int64_t bt_trace3d_workspace_bytes(const bt_trace3d_args* a) {
+    return check_trace3d(a) ? -1 : (int64_t)trace3d_bytes(a);
+}
+
+// Place the words and tables in the workspace, upload the tables, launch.
+static int run_trace3d(const bt_trace3d_args* a, void* ws, hipStream_t st) {
+    char* w = static_cast<char*>(ws);
+    const i64 current_nodes = a->has_j != 0 ? a->current_cols : 0;
+    u64* acc_q = reinterpret_cast<u64*>(w);
+    w += node_bytes(a->charge_cols);
+    u64* acc_j = reinterpret_cast<u64*>(w);
+    w += node_bytes(current_nodes);
+    i64* d_pi = reinterpret_cast<i64*>(w);
+    w += round256((size_t)a->n_patches * 8 * 8);
+    double* d_pr = reinterpret_cast<double*>(w);
+    w += round256((size_t)a->n_patches * 10 * 8);
+    double* d_pl = reinterpret_cast<double*>(w);
+    hipError_t err = hipMemcpyAsync(d_pi, a->patch_int, (size_t)a->n_patches * 64, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = hipMemcpyAsync(d_pr, a->patch_real, (size_t)a->n_patches * 80, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess && a->n_planes > 0)
+        err = hipMemcpyAsync(d_pl, a->planes, (size_t)a->n_planes * 8, hipMemcpyHostToDevice, st);
+    if (err != hipSuccess) return hip_fail(err, "table upload");
+    const int64_t* base = a->patch_int + (a->n_patches - 1) * 8;
+    bt::Trace3DParams p;
+    p.active = a->active; p.n_active = a->n_active; p.n = a->n;
+    p.x = a->x; p.v = a->v; p.qm = a->qm; p.dq = a->dq;
+    p.steps = a->steps; p.status = a->status; p.electrode = a->electrode;
+    p.fxs = a->fxs; p.fys = a->fys; p.fzs = a->fzs;
+    p.patch_int = d_pi; p.patch_real = d_pr; p.n_patches = (int)a->n_patches;
+    p.bn = a->bn; p.has_b = a->has_b != 0;
+    p.ox = a->origin_x; p.oy = a->origin_y; p.oz = a->origin_z; p.h = a->h;
+    p.ux = a->upper_x; p.uy = a->upper_y; p.uz = a->upper_z;
+    p.mirror_y = a->mirror_y != 0; p.mirror_z = a->mirror_z != 0;
+    p.step_fraction = a->step_fraction; p.max_steps = a->max_steps;
+    p.cells = a->cells; p.cells_n1 = (int)a->cells_n1; p.cells_n2 = (int)a->cells_n2;
+    p.subcells = a->subcells; p.n_subcells = a->n_subcells; p.refine = (int)a->refine;
+    p.planes = d_pl; p.n_planes = (int)a->n_planes;
+    p.hits = a->hits; p.capacity = (int)(a->capacity < kMax32 ? a->capacity : kMax32 - 1); p.counts = a->counts;
+    p.path_slot = a->path_slot; p.paths = a->paths; p.n_paths = a->n_paths;
+    p.path_capacity = (int)(a->path_capacity < kMax32 ? a->path_capacity : kMax32 - 1); p.path_count = a->path_count;
+    p.has_j = a->has_j != 0; p.inv_c2 = a->inv_c2;
+    p.nx = (int)base[0]; p.ny = (int)base[1]; p.nz = (int)base[2];
+    p.acc_q = acc_q; p.acc_j = acc_j; p.e_q = a->deposit_exponent; p.e_j = a->current_exponent;
+    if (bt::launch_trace3d(p, (int)a->integrator, a->charge_cols, a->charge, current_nodes, a->current, st))
+        return fail(BT_E_LAUNCH, std::string("3D trace launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
+int bt_trace_trajectories_3d_device(const bt_trace3d_args* a, void* stream) {
+    int rc = check_trace3d(a);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    if ((rc = device_state(&s))) return rc;
+    void* ws = a->workspace;
+    if (ws == nullptr) {
+        if ((rc = ensure(s->workspace, trace3d_bytes(a)))) return rc;
+        ws = s->workspace.ptr;
+    } else if (a->workspace_bytes < (int64_t)trace3d_bytes(a)) {
+        return fail(BT_E_ARG, "workspace smaller than bt_trace3d_workspace_bytes()");
+    }
+    return run_trace3d(a, ws, static_cast<hipStream_t>(stream));
+}
+
+int bt_trace_trajectories_3d(const bt_trace3d_args* a) {
+    int rc = check_trace3d(a);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    if ((rc = device_state(&s))) return rc;
+    if ((rc = host_stream(*s))) return rc;
+    hipStream_t st = s->stream;
+    const i64 n = a->n;
+    const int64_t* base = a->patch_int + (a->n_patches - 1) * 8;
+    const size_t nb = (size_t)(base[0] * base[1] * base[2]);
+    const size_t nh = (size_t)n * a->n_planes * a->capacity * 8, np = (size_t)a->n_paths * a->path_capacity * 3;
+    const i64 current_nodes = a->has_j != 0 ? a->current_cols : 0;
+    bt_trace3d_args d = *a;
+    int64_t *active, *steps, *status, *electrode, *cells, *subcells, *counts, *slot, *pcount;
+    double *x, *v, *qm, *dq, *fxs, *fys, *fzs, *bn = nullptr, *hits, *paths, *charge, *current = nullptr;
+    if ((rc = upload(*s, 0, a->active, (size_t)a->n_active, &active, st))) return rc;
+    if ((rc = upload(*s, 1, a->x, (size_t)(3 * n), &x, st))) return rc;
+    if ((rc = upload(*s, 2, a->v, (size_t)(3 * n), &v, st))) return rc;
+    if ((rc = upload(*s, 3, a->qm, (size_t)n, &qm, st))) return rc;
+    if ((rc = upload(*s, 4, a->dq, (size_t)n, &dq, st))) return rc;
+    if ((rc = upload(*s, 5, a->steps, (size_t)n, &steps, st))) return rc;
+    if ((rc = upload(*s, 6, a->status, (size_t)n, &status, st))) return rc;
+    if ((rc = upload(*s, 7, a->electrode, (size_t)n, &electrode, st))) return rc;
+    if ((rc = upload(*s, 8, a->fxs, (size_t)a->n_fxs, &fxs, st))) return rc;
+    if ((rc = upload(*s, 9, a->fys, (size_t)a->n_fys, &fys, st))) return rc;
+    if ((rc = upload(*s, 10, a->fzs, (size_t)a->n_fzs, &fzs, st))) return rc;
+    if (a->has_b != 0 && (rc = upload(*s, 11, a->bn, 3 * nb, &bn, st))) return rc;
+    if ((rc = upload(*s, 12, a->cells, (size_t)(a->cells_n0 * a->cells_n1 * a->cells_n2), &cells, st))) return rc;
+    if ((rc = upload(*s, 13, a->subcells, (size_t)a->n_subcells, &subcells, st))) return rc;
+    if ((rc = upload(*s, 14, a->hits, nh, &hits, st))) return rc;
+    if ((rc = upload(*s, 15, a->counts, (size_t)n * a->n_planes, &counts, st))) return rc;
+    if ((rc = upload(*s, 16, a->path_slot, (size_t)n, &slot, st))) return rc;
+    if ((rc = upload(*s, 17, a->paths, np, &paths, st))) return rc;
+    if ((rc = upload(*s, 18, a->path_count, (size_t)a->n_paths, &pcount, st))) return rc;
+    if ((rc = upload(*s, 19, (const double*)nullptr, (size_t)a->charge_cols, &charge, st))) return rc;
+    if (current_nodes > 0 && (rc = upload(*s, 20, (const double*)nullptr, (size_t)current_nodes, &current, st)))
+        return rc;
+    hipError_t err = hipMemsetAsync(charge, 0, (size_t)a->charge_cols * 8, st);
+    if (err == hipSuccess && current_nodes > 0) err = hipMemsetAsync(current, 0, (size_t)current_nodes * 8, st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    d.active = active; d.x = x; d.v = v; d.qm = qm; d.dq = dq; d.steps = steps; d.status = status;
+    d.electrode = electrode; d.fxs = fxs; d.fys = fys; d.fzs = fzs; d.bn = bn; d.cells = cells;
+    d.subcells = subcells; d.hits = hits; d.counts = counts; d.path_slot = slot; d.paths = paths;
+    d.path_count = pcount; d.charge = charge; d.current = current;
+    if ((rc = ensure(s->workspace, trace3d_bytes(&d)))) return rc;
+    if ((rc = run_trace3d(&d, s->workspace.ptr, st))) return rc;
+    std::vector<double> added_q((size_t)a->charge_cols), added_j((size_t)current_nodes);
+    if ((rc = download(a->x, x, (size_t)(3 * n), st))) return rc;
+    if ((rc = download(a->v, v, (size_t)(3 * n), st))) return rc;
+    if ((rc = download(a->steps, steps, (size_t)n, st))) return rc;
+    if ((rc = download(a->status, status, (size_t)n, st))) return rc;
+    if ((rc = download(a->electrode, electrode, (size_t)n, st))) return rc;
+    if ((rc = download(a->hits, hits, nh, st))) return rc;
+    if ((rc = download(a->counts, counts, (size_t)n * a->n_planes, st))) return rc;
+    if ((rc = download(a->paths, paths, np, st))) return rc;
+    if ((rc = download(a->path_count, pcount, (size_t)a->n_paths, st))) return rc;
+    if ((rc = download(added_q.data(), charge, (size_t)a->charge_cols, st))) return rc;
+    if (current_nodes > 0 && (rc = download(added_j.data(), current, (size_t)current_nodes, st))) return rc;
+    err = hipStreamSynchronize(st);
+    if (err != hipSuccess) return hip_fail(err, "3D trace");
+    for (i64 c = 0; c < a->charge_cols; ++c) a->charge[c] += added_q[(size_t)c];
+    for (i64 c = 0; c < current_nodes; ++c) a->current[c] += added_j[(size_t)c];
+    return BT_OK;
+}
+
 static int check_edge_field(const double* phi, const int32_t* solid, const uint8_t* near, const double* distance,
                             double h, int64_t nx, int64_t ny, const double* fx, const double* fy) {
     if (nx < 2 || ny < 2) return fail(BT_E_ARG, "the mesh needs 2+ nodes per axis");

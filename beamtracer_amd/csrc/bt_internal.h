@@ -112,4 +112,31 @@ int launch_deposit(const double* px, const double* py, const double* q, i64 n, d
                    double oy, double h, i64 cylindrical, double* charge, i64 nx, i64 ny, int e,
                    u64* acc, hipStream_t stream);
 
+// Device pointers and scalars of one 3D trace launch (bt_trace3d_args in
+// include/beamtracer_hip.h gives the meaning of each; the tables are device copies).
+struct Trace3DParams {
+    const i64* active; i64 n_active, n;
+    double* x; double* v; const double* qm; const double* dq;
+    i64* steps; i64* status; i64* electrode;
+    const double* fxs; const double* fys; const double* fzs;
+    const i64* patch_int; const double* patch_real; int n_patches;
+    const double* bn; int has_b;
+    double ox, oy, oz, h, ux, uy, uz;
+    int mirror_y, mirror_z;
+    double step_fraction; i64 max_steps;
+    const i64* cells; int cells_n1, cells_n2;
+    const i64* subcells; i64 n_subcells; int refine;
+    const double* planes; int n_planes;
+    double* hits; int capacity; i64* counts;
+    const i64* path_slot; double* paths; i64 n_paths; int path_capacity; i64* path_count;
+    int has_j; double inv_c2;
+    int nx, ny, nz;          // the base mesh
+    u64* acc_q; u64* acc_j;  // fixed-point words of the charge and current nodes
+    int e_q, e_j;            // their exponents
+};
+
+// The 3D trace: zero the words, trace, add the words to charge and current (row 0).
+int launch_trace3d(const Trace3DParams& p, int integrator, i64 charge_nodes, double* charge, i64 current_nodes,
+                   double* current, hipStream_t stream);
+
 }  // namespace bt

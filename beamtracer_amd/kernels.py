@@ -119,6 +119,101 @@ def staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey, *, stream=
     _lib.check(rc, "staggered_field_2d")
 
 
+def trace3d_exponents(active, v, dq, qm, fxs, fys, fzs, patch_real, step_fraction, inv_c2=0.0) -> tuple[int, int]:
+    """The fixed-point exponents of a 3D trace's charge and current deposits.
+
+    The charge as :func:`trace_exponent` (the finest spacing, the fastest start, the
+    strongest field); the current element ``dq (x' - x)`` of a step is at most about
+    ``(f + f^2) h`` times ``|dq|`` for the step fraction ``f`` and the coarsest spacing
+    ``h``, so its grid is set 40 bits below that.
+    """
+    fx_all = np.concatenate([np.ravel(fxs), np.ravel(fzs)]) if not _is_torch(fxs) else None
+    if _is_torch(fxs):
+        import torch
+
+        fx_all = torch.cat([fxs.reshape(-1), fzs.reshape(-1)])
+    pr = np.asarray(patch_real)
+    # the 2D rule over (fx and fz together, fy): the strongest of the three components
+    e_q = trace_exponent(active, v, dq, qm, fx_all, fys, pr[:, [0, 1, 3]], step_fraction, inv_c2)
+    if _is_torch(dq):
+        dqmax = float(dq[active.long()].abs().max().item()) if active.numel() else 0.0
+    else:
+        dqmax = float(np.abs(dq[active]).max()) if active.size else 0.0
+    scale = dqmax * (step_fraction + step_fraction**2) * float(np.max(pr[:, 3]))
+    return e_q, deposit_exponent(scale if scale > 0.0 and math.isfinite(scale) else 1.0)
+
+
+def trace_trajectories_3d(
+    active, x, v, qm, dq, steps, status, electrode, fxs, fys, fzs, patch_int, patch_real, bn, has_b,
+    origin_x, origin_y, origin_z, h, upper_x, upper_y, upper_z, mirror_y, mirror_z, integrator,
+    step_fraction, max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
+    path_count, num_threads, charge, has_j, current, inv_c2,
+    *, exponents=None, workspace=None, stream=None,
+) -> None:  # fmt: skip
+    """Trace the ``active`` trajectories of a 3D mesh until they end, or reach a surface.
+
+    The reference's 41 arguments with its meaning (``trace_trajectories_3d_pyccel.py:18-60``);
+    the charge goes to ``charge[0]`` and the current to ``current[0]`` (as the reference's
+    CUDA kernel, ``trace_trajectories_3d_cuda.cu``), each exact in fixed point.
+    ``exponents`` = (charge, current) fixed-point exponents (default
+    :func:`trace3d_exponents`). NumPy arrays run the host form, GPU tensors the device form
+    on ``stream`` (``patch_int``, ``patch_real`` and ``planes`` are read on the host).
+    """
+    L = _lib.lib()
+    device = _is_torch(x)
+    check = _dev if device else _host
+    pi = _table(patch_int, _I64, "patch_int")
+    pr = _table(patch_real, _F64, "patch_real")
+    pl = _table(planes, _F64, "planes")
+    if pi.ndim != 2 or pi.shape[1] != 8 or pr.shape != (pi.shape[0], 10):
+        raise ValueError("patch_int must be (P, 8) and patch_real (P, 10)")
+    for name, a in (("active", active), ("steps", steps), ("status", status), ("electrode", electrode),
+                    ("cells", cells), ("subcells", subcells), ("counts", counts), ("path_slot", path_slot),
+                    ("path_count", path_count)):  # fmt: skip
+        check(a, _I64, name)
+    for name, a in (("x", x), ("v", v), ("qm", qm), ("dq", dq), ("fxs", fxs), ("fys", fys), ("fzs", fzs),
+                    ("bn", bn), ("hits", hits), ("paths", paths), ("charge", charge), ("current", current)):  # fmt: skip
+        check(a, _F64, name)
+    n = x.shape[1]
+    if tuple(x.shape) != (3, n) or tuple(v.shape) != (3, n):
+        raise ValueError("x and v must be (3, N)")
+    for a, nm in ((qm, "qm"), (dq, "dq"), (steps, "steps"), (status, "status"), (electrode, "electrode"),
+                  (path_slot, "path_slot")):  # fmt: skip
+        if tuple(a.shape) != (n,):
+            raise ValueError(f"{nm} must be ({n},)")
+    if hits.ndim != 4 or tuple(hits.shape[:2]) != (n, pl.shape[0]) or hits.shape[3] != 8:
+        raise ValueError("hits must be (N, n_planes, capacity, 8)")
+    if tuple(counts.shape) != (n, pl.shape[0]):
+        raise ValueError("counts must be (N, n_planes)")
+    if paths.ndim != 3 or paths.shape[2] != 3 or tuple(path_count.shape) != (paths.shape[0],):
+        raise ValueError("paths must be (S, capacity, 3) and path_count (S,)")
+    if cells.ndim != 3 or charge.ndim != 2 or current.ndim != 2 or active.ndim != 1:
+        raise ValueError("cells must be 3D, charge and current 2D, active 1D")
+    nx, ny, nz = (int(d) for d in pi[-1, :3])
+    if has_b and tuple(bn.shape) != (3, nx, ny, nz):
+        raise ValueError(f"bn must be (3, {nx}, {ny}, {nz})")
+    if exponents is None:
+        exponents = trace3d_exponents(active, v, dq, qm, fxs, fys, fzs, pr, float(step_fraction), float(inv_c2))
+    args = _lib.Trace3DArgs(
+        _ptr(active), active.shape[0], _ptr(x), _ptr(v), n, _ptr(qm), _ptr(dq), _ptr(steps), _ptr(status),
+        _ptr(electrode), _ptr(fxs), fxs.shape[0], _ptr(fys), fys.shape[0], _ptr(fzs), fzs.shape[0],
+        pi.ctypes.data, pr.ctypes.data, pi.shape[0], _ptr(bn) if has_b else None, int(has_b),
+        float(origin_x), float(origin_y), float(origin_z), float(h), float(upper_x), float(upper_y), float(upper_z),
+        int(mirror_y), int(mirror_z), int(integrator), float(step_fraction), int(max_steps),
+        _ptr(cells), cells.shape[0], cells.shape[1], cells.shape[2], _ptr(subcells), subcells.shape[0], int(refine),
+        pl.ctypes.data if pl.size else None, pl.shape[0], _ptr(hits), hits.shape[2], _ptr(counts),
+        _ptr(path_slot), _ptr(paths), paths.shape[0], paths.shape[1], _ptr(path_count), int(num_threads),
+        _ptr(charge), charge.shape[0], charge.shape[1], int(has_j), _ptr(current) if has_j else None,
+        current.shape[0], current.shape[1], float(inv_c2), int(exponents[0]), int(exponents[1]),
+        _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
+    )  # fmt: skip
+    if device:
+        rc = L.bt_trace_trajectories_3d_device(ctypes.byref(args), _stream(stream))
+    else:
+        rc = L.bt_trace_trajectories_3d(ctypes.byref(args))
+    _lib.check(rc, "trace_trajectories_3d")
+
+
 def edge_field_2d(phi, solid, near, distance, dx, h, faces, fx, fy, *, stream=None) -> None:
     """The staggered edge fields of the node potential ``phi`` into ``fx`` ``(nx + 1, ny)`` and
     ``fy`` ``(nx, ny + 1)`` (``solver/fields.py:18-67``, as ``StaggeredElectricField`` builds

@@ -13,6 +13,11 @@
  *
  *   bt_edge_field_2d           solver/fields.py:18-67 (StaggeredElectricField, fields.py:116-152)
  *
+ * and the 3D tracer of BeamTracer3D:
+ *
+ *   bt_trace_trajectories_3d   kernels/trace_trajectories_3d/trace_trajectories_3d_pyccel.py:18-60
+ *                              (called from solver/tracer3d.py:405)
+ *
  * Every entry point exists in two forms:
  *   bt_<name>          host arrays (NumPy memory), synchronous: the drop-in for the Pyccel
  *                      kernel.  Arrays are copied to the device and the outputs back.
@@ -143,6 +148,53 @@ int bt_deposit_trajectories_2d_device(const double* px, const double* py, const 
                                       int64_t ny, int32_t deposit_exponent,
                                       void* workspace, int64_t workspace_bytes,
                                       void* stream);
+
+/* ---- the 3D tracer ----------------------------------------------------------------------
+ * Every argument of trace_trajectories_3d (trace_trajectories_3d_pyccel.py:18-60, called
+ * from solver/tracer3d.py:405) as a pointer with its extents, in the reference's order and
+ * meaning: states x, v (3, n); edge fields of the patches flattened one after the other in
+ * fxs, fys, fzs; patch_int (P, 8) = node counts, the three field offsets, node count, node
+ * offset; patch_real (P, 10) = origin, spacing, box (finest first, the mesh last); bn
+ * (3, nx, ny, nz) the base mesh's node B (has_b); cells (nx - 1, ny - 1, nz - 1) and
+ * subcells (refine^3 per cell); hits (n, n_planes, capacity, 8); paths (n_paths,
+ * path_capacity, 3); charge (rows, nodes) and, with has_j, current (rows, 3 nx ny nz).
+ * patch_int, patch_real and planes are read on the host (the library uploads them).  The
+ * charge and the current go to row 0, each summed exactly in fixed point with the least
+ * significant bits 2^deposit_exponent and 2^current_exponent. */
+typedef struct bt_trace3d_args {
+    const int64_t* active; int64_t n_active;
+    double* x; double* v; int64_t n;
+    const double* qm; const double* dq;
+    int64_t* steps; int64_t* status; int64_t* electrode;
+    const double* fxs; int64_t n_fxs;
+    const double* fys; int64_t n_fys;
+    const double* fzs; int64_t n_fzs;
+    const int64_t* patch_int; const double* patch_real; int64_t n_patches;
+    const double* bn; int64_t has_b;
+    double origin_x, origin_y, origin_z, h, upper_x, upper_y, upper_z;
+    int64_t mirror_y, mirror_z, integrator;
+    double step_fraction; int64_t max_steps;
+    const int64_t* cells; int64_t cells_n0, cells_n1, cells_n2;
+    const int64_t* subcells; int64_t n_subcells; int64_t refine;
+    const double* planes; int64_t n_planes;
+    double* hits; int64_t capacity; int64_t* counts;
+    const int64_t* path_slot; double* paths; int64_t n_paths, path_capacity; int64_t* path_count;
+    int64_t num_threads;
+    double* charge; int64_t charge_rows, charge_cols;
+    int64_t has_j; double* current; int64_t current_rows, current_cols;
+    double inv_c2;
+    int32_t deposit_exponent, current_exponent;
+    void* workspace; int64_t workspace_bytes;  /* device scratch (NULL: the library's own) */
+} bt_trace3d_args;
+
+/* sizeof(bt_trace3d_args), for binding checks. */
+int64_t bt_trace3d_args_size(void);
+/* Device scratch of a 3D trace (the node words of charge and current, the tables); -1 for
+ * arguments that fail the checks. */
+int64_t bt_trace3d_workspace_bytes(const bt_trace3d_args* args);
+/* trace_trajectories_3d: replaces the kernel call of solver/tracer3d.py:405-430. */
+int bt_trace_trajectories_3d(const bt_trace3d_args* args);
+int bt_trace_trajectories_3d_device(const bt_trace3d_args* args, void* stream);
 
 /* ---- the trace's input: edge fields of a potential -----------------------------------
  * The staggered edge fields of the node potential phi (nx, ny): fx (nx + 1, ny) along x and

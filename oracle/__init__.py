@@ -11,6 +11,8 @@ package ``beamtracer_amd`` never imports this module.
   ``src/beamtracer/kernels/``).
 * ``tangent.c`` (in the same library) is the forward-mode tangent of the C tracer along a
   direction of the edge fields: the FP64 CPU tangent that the GPU adjoint must transpose.
+* ``oracle.c`` also restates the 3D tracer ``trace_trajectories_3d_pyccel.py:18-657``
+  (trilinear fields on patches, node B, symmetry planes, the charge and current deposits).
 * ``oracle.c`` also restates ``_edge_field`` (``solver/fields.py:18-67``), the staggered
   edge fields of a potential: the checker of the hip edge-field kernel.
 * ``numpy_ref.py`` restates the vectorized NumPy versions
@@ -194,6 +196,73 @@ def trace_tangent(*args, dfx, dfy) -> np.ndarray:
     if L.oracle_trace_tangent(ctypes.byref(a), _p(dfx), _p(dfy), _p(rec_d)) != 0:
         raise ValueError("the tangent takes one uniform patch, classical, without a magnetic field")
     return rec_d
+
+
+class Trace3DArgs(ctypes.Structure):
+    """ctypes mirror of ``oracle_trace3d_args`` in ``oracle.c``."""
+
+    _fields_ = [
+        ("active", _ptr), ("n_active", _i64),
+        ("x", _ptr), ("v", _ptr), ("n", _i64),
+        ("qm", _ptr), ("dq", _ptr),
+        ("steps", _ptr), ("status", _ptr), ("electrode", _ptr),
+        ("fxs", _ptr), ("fys", _ptr), ("fzs", _ptr),
+        ("patch_int", _ptr), ("patch_real", _ptr), ("n_patches", _i64),
+        ("bn", _ptr), ("has_b", _i64),
+        ("origin_x", _dbl), ("origin_y", _dbl), ("origin_z", _dbl), ("h", _dbl),
+        ("upper_x", _dbl), ("upper_y", _dbl), ("upper_z", _dbl),
+        ("mirror_y", _i64), ("mirror_z", _i64), ("integrator", _i64),
+        ("step_fraction", _dbl),
+        ("max_steps", _i64),
+        ("cells", _ptr), ("cells_n1", _i64), ("cells_n2", _i64),
+        ("subcells", _ptr), ("refine", _i64),
+        ("planes", _ptr), ("n_planes", _i64),
+        ("hits", _ptr), ("capacity", _i64), ("counts", _ptr),
+        ("path_slot", _ptr), ("paths", _ptr), ("path_capacity", _i64), ("path_count", _ptr),
+        ("num_threads", _i64), ("charge", _ptr), ("charge_stride", _i64),
+        ("has_j", _i64), ("current", _ptr), ("current_stride", _i64),
+        ("inv_c2", _dbl),
+    ]  # fmt: skip
+
+
+def trace_trajectories_3d(
+    active, x, v, qm, dq, steps, status, electrode, fxs, fys, fzs, patch_int, patch_real, bn, has_b,
+    origin_x, origin_y, origin_z, h, upper_x, upper_y, upper_z, mirror_y, mirror_z, integrator,
+    step_fraction, max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
+    path_count, num_threads, charge, has_j, current, inv_c2,
+) -> None:  # fmt: skip
+    """The 3D tracer with the reference's 41 arguments (``trace_trajectories_3d_pyccel.py:18-60``):
+    thread ``t < num_threads`` deposits into ``charge[t]`` and ``current[t]``."""
+    i64, f64 = np.int64, np.float64
+    for name, arr, dt in (
+        ("active", active, i64), ("x", x, f64), ("v", v, f64), ("qm", qm, f64), ("dq", dq, f64),
+        ("steps", steps, i64), ("status", status, i64), ("electrode", electrode, i64),
+        ("fxs", fxs, f64), ("fys", fys, f64), ("fzs", fzs, f64), ("patch_int", patch_int, i64),
+        ("patch_real", patch_real, f64), ("bn", bn, f64), ("cells", cells, i64), ("subcells", subcells, i64),
+        ("planes", planes, f64), ("hits", hits, f64), ("counts", counts, i64), ("path_slot", path_slot, i64),
+        ("paths", paths, f64), ("path_count", path_count, i64), ("charge", charge, f64), ("current", current, f64),
+    ):  # fmt: skip
+        _check(arr, dt, name)
+    threads = max(int(num_threads), 1)
+    if charge.ndim != 2 or charge.shape[0] < threads or (has_j and (current.ndim != 2 or current.shape[0] < threads)):
+        raise ValueError(f"charge and current need num_threads={threads} slabs")
+    if patch_int.shape[1:] != (8,) or patch_real.shape != (patch_int.shape[0], 10) or cells.ndim != 3:
+        raise ValueError("patch_int (P, 8), patch_real (P, 10) and cells (nx-1, ny-1, nz-1)")
+    a = Trace3DArgs(
+        _p(active), active.shape[0], _p(x), _p(v), x.shape[1], _p(qm), _p(dq), _p(steps), _p(status),
+        _p(electrode), _p(fxs), _p(fys), _p(fzs), _p(patch_int), _p(patch_real), patch_int.shape[0],
+        _p(bn), int(has_b), float(origin_x), float(origin_y), float(origin_z), float(h), float(upper_x),
+        float(upper_y), float(upper_z), int(mirror_y), int(mirror_z), int(integrator), float(step_fraction),
+        int(max_steps), _p(cells), cells.shape[1], cells.shape[2], _p(subcells), int(refine), _p(planes),
+        planes.shape[0], _p(hits), hits.shape[2], _p(counts), _p(path_slot), _p(paths), paths.shape[1],
+        _p(path_count), threads, _p(charge), charge.shape[1], int(has_j), _p(current), current.shape[1],
+        float(inv_c2),
+    )  # fmt: skip
+    L = lib()
+    L.oracle_trace3d_args_size.restype = _i64
+    if L.oracle_trace3d_args_size() != ctypes.sizeof(Trace3DArgs):
+        raise RuntimeError("oracle_trace3d_args layout mismatch")
+    L.oracle_trace_trajectories_3d(ctypes.byref(a))
 
 
 def edge_fields(phi, solid, near, distance, dx, h, faces) -> tuple[np.ndarray, np.ndarray]:

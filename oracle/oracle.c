@@ -594,3 +594,361 @@ void oracle_edge_field(const double* phi, const int32_t* solid, const uint8_t* n
                                            : 2.0 * out[(n - 1) * ot + edge0] - out[(n - 2) * ot + edge0];
     }
 }
+
+/* ---- the 3D tracer (trace_trajectories_3d_pyccel.py:18-657) -------------------------- */
+
+/* Every argument of trace_trajectories_3d_pyccel.py:18-60, flattened to pointers and
+ * extents. */
+typedef struct {
+    const i64* active; i64 n_active;
+    double* x; double* v; i64 n;
+    const double* qm; const double* dq;
+    i64* steps; i64* status; i64* electrode;
+    const double* fxs; const double* fys; const double* fzs;
+    const i64* patch_int; const double* patch_real; i64 n_patches;
+    const double* bn; i64 has_b;
+    double origin_x, origin_y, origin_z, h, upper_x, upper_y, upper_z;
+    i64 mirror_y, mirror_z, integrator;
+    double step_fraction;
+    i64 max_steps;
+    const i64* cells; i64 cells_n1, cells_n2;
+    const i64* subcells; i64 refine;
+    const double* planes; i64 n_planes;
+    double* hits; i64 capacity; i64* counts;
+    const i64* path_slot; double* paths; i64 path_capacity; i64* path_count;
+    i64 num_threads; double* charge; i64 charge_stride;
+    i64 has_j; double* current; i64 current_stride;
+    double inv_c2;
+} oracle_trace3d_args;
+
+i64 oracle_trace3d_args_size(void) { return (i64)sizeof(oracle_trace3d_args); }
+
+/* the index floor(s) clamped to [lo, hi] (pyccel.py:225-268) */
+static i64 clamp_floor3(double s, i64 lo, i64 hi) {
+    i64 i = (i64)floor(s);
+    if (i < lo) i = lo; else if (i > hi) i = hi;
+    return i;
+}
+
+/* the trilinear sum of pyccel.py:237-241: f at ci and its +1, +dj, +di neighbours */
+static double trilinear3(const double* f, i64 ci, i64 di, i64 dj, double a, double b, double c) {
+    return (1.0 - a) * ((1.0 - b) * ((1.0 - c) * f[ci] + c * f[ci + 1])
+                        + b * ((1.0 - c) * f[ci + dj] + c * f[ci + dj + 1]))
+         + a * ((1.0 - b) * ((1.0 - c) * f[ci + di] + c * f[ci + di + 1])
+                + b * ((1.0 - c) * f[ci + di + dj] + c * f[ci + di + dj + 1]));
+}
+
+/* the finest patch containing (p0, p1, p2), else the base mesh (pyccel.py:196-206) */
+static i64 patch3(const oracle_trace3d_args* a, double p0, double p1, double p2) {
+    for (i64 qq = 0; qq < a->n_patches - 1; ++qq) {
+        const double* r = a->patch_real + qq * 10;
+        if (p0 >= r[4] && p0 <= r[7] && p1 >= r[5] && p1 <= r[8] && p2 >= r[6] && p2 <= r[9]) return qq;
+    }
+    return a->n_patches - 1;
+}
+
+/* E (and B) at stage point q, mirrored below a symmetry plane (pyccel.py:182-318) */
+static void fields3(const oracle_trace3d_args* a, double q0, double q1, double q2, double* e, double* b) {
+    double p0 = q0, p1 = q1, p2 = q2, s1 = 1.0, s2 = 1.0;
+    if (a->mirror_y != 0 && p1 < a->origin_y) { p1 = 2.0 * a->origin_y - p1; s1 = -1.0; }
+    if (a->mirror_z != 0 && p2 < a->origin_z) { p2 = 2.0 * a->origin_z - p2; s2 = -1.0; }
+    const i64 pp = patch3(a, p0, p1, p2);
+    const i64* pi = a->patch_int + pp * 8;
+    const double* pr = a->patch_real + pp * 10;
+    const i64 pnx = pi[0], pny = pi[1], pnz = pi[2];
+    const double phh = pr[3];
+    const double sx = (p0 - pr[0]) / phh, sy = (p1 - pr[1]) / phh, sz = (p2 - pr[2]) / phh;
+    /* E_x on (pnx + 1, pny, pnz) at (sx + 1/2, sy, sz) */
+    double s = sx + 0.5;
+    i64 i = clamp_floor3(s, 0, pnx - 1), j = clamp_floor3(sy, 0, pny - 2), kz = clamp_floor3(sz, 0, pnz - 2);
+    double wa = s - (double)i, wb = sy - (double)j, wc = sz - (double)kz;
+    i64 dj = pnz, di = pny * pnz;
+    e[0] = trilinear3(a->fxs, pi[3] + i * di + j * dj + kz, di, dj, wa, wb, wc);
+    /* E_y on (pnx, pny + 1, pnz) at (sx, sy + 1/2, sz): kz and c as for E_x */
+    i = clamp_floor3(sx, 0, pnx - 2);
+    s = sy + 0.5;
+    j = clamp_floor3(s, 0, pny - 1);
+    wa = sx - (double)i;
+    wb = s - (double)j;
+    dj = pnz;
+    di = (pny + 1) * pnz;
+    e[1] = s1 * trilinear3(a->fys, pi[4] + i * di + j * dj + kz, di, dj, wa, wb, wc);
+    /* E_z on (pnx, pny, pnz + 1) at (sx, sy, sz + 1/2): i and a as for E_y */
+    j = clamp_floor3(sy, 0, pny - 2);
+    s = sz + 0.5;
+    kz = clamp_floor3(s, 0, pnz - 1);
+    wb = sy - (double)j;
+    wc = s - (double)kz;
+    dj = pnz + 1;
+    di = pny * (pnz + 1);
+    e[2] = s2 * trilinear3(a->fzs, pi[5] + i * di + j * dj + kz, di, dj, wa, wb, wc);
+    b[0] = b[1] = b[2] = 0.0;
+    if (a->has_b != 0) {  /* B on the base mesh's nodes (3, nx, ny, nz) */
+        const i64* bi = a->patch_int + (a->n_patches - 1) * 8;
+        const i64 nx = bi[0], ny = bi[1], nz = bi[2];
+        const double bx = (p0 - a->origin_x) / a->h, by = (p1 - a->origin_y) / a->h, bz = (p2 - a->origin_z) / a->h;
+        i = clamp_floor3(bx, 0, nx - 2);
+        j = clamp_floor3(by, 0, ny - 2);
+        kz = clamp_floor3(bz, 0, nz - 2);
+        wa = bx - (double)i;
+        wb = by - (double)j;
+        wc = bz - (double)kz;
+        const double w000 = (1.0 - wa) * (1.0 - wb) * (1.0 - wc), w001 = (1.0 - wa) * (1.0 - wb) * wc;
+        const double w010 = (1.0 - wa) * wb * (1.0 - wc), w011 = (1.0 - wa) * wb * wc;
+        const double w100 = wa * (1.0 - wb) * (1.0 - wc), w101 = wa * (1.0 - wb) * wc;
+        const double w110 = wa * wb * (1.0 - wc), w111 = wa * wb * wc;
+        const i64 nodes = nx * ny * nz, si = ny * nz, sj = nz;
+        for (int n0 = 0; n0 < 3; ++n0) {
+            const double* f = a->bn + n0 * nodes + i * si + j * sj + kz;
+            const double v = w000 * f[0] + w001 * f[1] + w010 * f[sj] + w011 * f[sj + 1]
+                           + w100 * f[si] + w101 * f[si + 1] + w110 * f[si + sj] + w111 * f[si + sj + 1];
+            if (n0 == 0) b[0] = s1 * s2 * v;
+            else if (n0 == 1) b[1] = s2 * v;
+            else b[2] = s1 * v;
+        }
+    }
+}
+
+/* a node index floor(s) with its fraction, clamped for a deposit (pyccel.py:453-468) */
+static void deposit_cell3(double s, i64 n, i64* i, double* w) {
+    i64 k = (i64)floor(s);
+    double f = s - (double)k;
+    if (k < 0) { f = 0.0; k = 0; }
+    else if (k >= n - 1) { f = 1.0; k = n - 2; }
+    *i = k;
+    *w = f;
+}
+
+/* the 8 trilinear node shares of value at node cc (pyccel.py:472-480) */
+static void deposit8(double* row, i64 cc, i64 di, i64 dj, double a, double b, double c, double value) {
+    row[cc] += (1.0 - a) * (1.0 - b) * (1.0 - c) * value;
+    row[cc + 1] += (1.0 - a) * (1.0 - b) * c * value;
+    row[cc + dj] += (1.0 - a) * b * (1.0 - c) * value;
+    row[cc + dj + 1] += (1.0 - a) * b * c * value;
+    row[cc + di] += a * (1.0 - b) * (1.0 - c) * value;
+    row[cc + di + 1] += a * (1.0 - b) * c * value;
+    row[cc + di + dj] += a * b * (1.0 - c) * value;
+    row[cc + di + dj + 1] += a * b * c * value;
+}
+
+static void trace3d_one(const oracle_trace3d_args* a, i64 k, double* charge_row, double* current_row) {
+    const i64 N = a->n;
+    const i64* base = a->patch_int + (a->n_patches - 1) * 8;
+    const i64 nx = base[0], ny = base[1], nz = base[2];
+    double x0 = a->x[k], x1 = a->x[N + k], x2 = a->x[2 * N + k];
+    double v0 = a->v[k], v1 = a->v[N + k], v2 = a->v[2 * N + k];
+    const double qmk = a->qm[k], inv_c2 = a->inv_c2;
+    const i64 slot = a->path_slot[k];
+    i64 ns = a->steps[k], st = -1;
+    double dt = 0.0, hdt = 0.0, dt6 = 0.0, tau = 0.0;
+    const int nstage = a->integrator == 0 ? 4 : 2;
+    int have = 0;
+    double ce[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
+    while (ns < a->max_steps) {
+        /* the spacing of the finest patch at the step's start (pyccel.py:149-163) */
+        double p1 = x1, p2 = x2;
+        if (a->mirror_y != 0 && p1 < a->origin_y) p1 = 2.0 * a->origin_y - p1;
+        if (a->mirror_z != 0 && p2 < a->origin_z) p2 = 2.0 * a->origin_z - p2;
+        double hloc = a->h;
+        for (i64 qq = 0; qq < a->n_patches - 1; ++qq) {
+            const double* r = a->patch_real + qq * 10;
+            if (x0 >= r[4] && x0 <= r[7] && p1 >= r[5] && p1 <= r[8] && p2 >= r[6] && p2 <= r[9]) {
+                hloc = r[3];
+                break;
+            }
+        }
+        const double gv = 1.0 / lorentz_factor(v0, v1, v2, inv_c2);
+        double q0 = x0, q1 = x1, q2 = x2, kx0 = v0, kx1 = v1, kx2 = v2;
+        double sx0 = 0.0, sx1 = 0.0, sx2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
+        double w0 = 0.0, w1 = 0.0, w2 = 0.0, y0, y1, y2;
+        for (int stage = 0; stage < nstage; ++stage) {
+            double e[3], b[3];
+            if (stage == 0 && have != 0) {
+                e[0] = ce[0]; e[1] = ce[1]; e[2] = ce[2];
+                b[0] = cb[0]; b[1] = cb[1]; b[2] = cb[2];
+            } else {
+                fields3(a, q0, q1, q2, e, b);
+            }
+            /* pyccel.py:320-324 */
+            double gk = 1.0 / lorentz_factor(kx0, kx1, kx2, inv_c2);
+            const double g0 = qmk * (e[0] + gk * kx1 * b[2] - gk * kx2 * b[1]);
+            const double g1 = qmk * (e[1] + gk * kx2 * b[0] - gk * kx0 * b[2]);
+            const double g2 = qmk * (e[2] + gk * kx0 * b[1] - gk * kx1 * b[0]);
+            if (stage == 0) {
+                const double speed = gv * sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                const double accel = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+                dt = -1.0;
+                if (speed > 0.0) dt = hloc / speed;
+                if (accel > 0.0) {
+                    const double d2 = sqrt(2.0 * hloc / accel);
+                    if (dt < 0.0 || d2 < dt) dt = d2;
+                }
+                if (dt < 0.0) dt = hloc;
+                dt = a->step_fraction * dt;
+                hdt = 0.5 * dt;
+                dt6 = dt / 6.0;
+                if (a->integrator != 0) {  /* Boris: a half step's kick-rotate-kick, the drift */
+                    tau = 0.5 * hdt * qmk;
+                    kx0 = v0; kx1 = v1; kx2 = v2;
+                    boris_kick(&kx0, &kx1, &kx2, e[0], e[1], e[2], b[0], b[1], b[2], hdt, qmk, inv_c2);
+                    gk = 1.0 / lorentz_factor(kx0, kx1, kx2, inv_c2);
+                    q0 = x0 + dt * gk * kx0;
+                    q1 = x1 + dt * gk * kx1;
+                    q2 = x2 + dt * gk * kx2;
+                    continue;
+                }
+                sx0 = gv * v0; sx1 = gv * v1; sx2 = gv * v2;
+                sv0 = g0; sv1 = g1; sv2 = g2;
+                q0 = x0 + hdt * gv * v0;
+                q1 = x1 + hdt * gv * v1;
+                q2 = x2 + hdt * gv * v2;
+                kx0 = v0 + hdt * g0;
+                kx1 = v1 + hdt * g1;
+                kx2 = v2 + hdt * g2;
+            } else if (a->integrator != 0) {  /* the second half step's kick at the end */
+                w0 = kx0; w1 = kx1; w2 = kx2;
+                boris_kick(&w0, &w1, &w2, e[0], e[1], e[2], b[0], b[1], b[2], hdt, qmk, inv_c2);
+                ce[0] = e[0]; ce[1] = e[1]; ce[2] = e[2];
+                cb[0] = b[0]; cb[1] = b[1]; cb[2] = b[2];
+                have = 1;
+            } else if (stage == 1 || stage == 2) {
+                const double c = stage == 1 ? hdt : dt;
+                sx0 = sx0 + 2.0 * gk * kx0; sx1 = sx1 + 2.0 * gk * kx1; sx2 = sx2 + 2.0 * gk * kx2;
+                sv0 = sv0 + 2.0 * g0; sv1 = sv1 + 2.0 * g1; sv2 = sv2 + 2.0 * g2;
+                q0 = x0 + c * gk * kx0; q1 = x1 + c * gk * kx1; q2 = x2 + c * gk * kx2;
+                kx0 = v0 + c * g0; kx1 = v1 + c * g1; kx2 = v2 + c * g2;
+            } else {
+                sx0 = sx0 + gk * kx0; sx1 = sx1 + gk * kx1; sx2 = sx2 + gk * kx2;
+                sv0 = sv0 + g0; sv1 = sv1 + g1; sv2 = sv2 + g2;
+            }
+        }
+        (void)tau;
+        if (a->integrator == 0) {
+            y0 = x0 + dt6 * sx0; y1 = x1 + dt6 * sx1; y2 = x2 + dt6 * sx2;
+            w0 = v0 + dt6 * sv0; w1 = v1 + dt6 * sv1; w2 = v2 + dt6 * sv2;
+        } else {
+            y0 = q0; y1 = q1; y2 = q2;
+        }
+        /* space charge: dq dt at the Hermite midpoint, trilinear (pyccel.py:432-480) */
+        const double gw = 1.0 / lorentz_factor(w0, w1, w2, inv_c2);
+        const double m0 = 0.5 * (x0 + y0) + dt * (gv * v0 - gw * w0) / 8.0;
+        double m1 = 0.5 * (x1 + y1) + dt * (gv * v1 - gw * w1) / 8.0;
+        double m2 = 0.5 * (x2 + y2) + dt * (gv * v2 - gw * w2) / 8.0;
+        if (a->mirror_y != 0 && m1 < a->origin_y) m1 = 2.0 * a->origin_y - m1;
+        if (a->mirror_z != 0 && m2 < a->origin_z) m2 = 2.0 * a->origin_z - m2;
+        {
+            const i64 pp = patch3(a, m0, m1, m2);
+            const i64* pi = a->patch_int + pp * 8;
+            const double* pr = a->patch_real + pp * 10;
+            const double phh = pr[3];
+            i64 i, j, kz;
+            double wa, wb, wc;
+            deposit_cell3((m0 - pr[0]) / phh, pi[0], &i, &wa);
+            deposit_cell3((m1 - pr[1]) / phh, pi[1], &j, &wb);
+            deposit_cell3((m2 - pr[2]) / phh, pi[2], &kz, &wc);
+            const i64 dj = pi[2], di = pi[1] * pi[2];
+            deposit8(charge_row, pi[7] + i * di + j * dj + kz, di, dj, wa, wb, wc, a->dq[k] * dt);
+        }
+        if (a->has_j != 0) {  /* the current element dq (x' - x) on the base mesh (pyccel.py:481-540) */
+            double jd[3] = {a->dq[k] * (y0 - x0), a->dq[k] * (y1 - x1), a->dq[k] * (y2 - x2)};
+            if (a->mirror_y != 0 && 0.5 * (x1 + y1) < a->origin_y) jd[1] = -jd[1];
+            if (a->mirror_z != 0 && 0.5 * (x2 + y2) < a->origin_z) jd[2] = -jd[2];
+            i64 bi, bj, bk;
+            double ba, bb, bc;
+            deposit_cell3((m0 - a->origin_x) / a->h, nx, &bi, &ba);
+            deposit_cell3((m1 - a->origin_y) / a->h, ny, &bj, &bb);
+            deposit_cell3((m2 - a->origin_z) / a->h, nz, &bk, &bc);
+            const i64 nb = nx * ny * nz, dj = nz, di = ny * nz;
+            for (int jc = 0; jc < 3; ++jc)
+                deposit8(current_row, jc * nb + bi * di + bj * dj + bk, di, dj, ba, bb, bc, jd[jc]);
+        }
+        ns = ns + 1;
+        /* plane crossings (pyccel.py:543-559) */
+        for (i64 p = 0; p < a->n_planes; ++p) {
+            const double xp = a->planes[p];
+            const double before = x0 - xp, after = y0 - xp;
+            if ((before < 0.0) != (after < 0.0)) {
+                const double s = before / (before - after);
+                const i64 idx = k * a->n_planes + p;
+                const i64 cc = a->counts[idx];
+                if (cc < a->capacity) {
+                    double* rec = a->hits + (idx * a->capacity + cc) * 8;
+                    rec[0] = xp;
+                    rec[1] = x1 + s * (y1 - x1);
+                    rec[2] = x2 + s * (y2 - x2);
+                    rec[3] = gv * v0 + s * (gw * w0 - gv * v0);
+                    rec[4] = gv * v1 + s * (gw * w1 - gv * v1);
+                    rec[5] = gv * v2 + s * (gw * w2 - gv * v2);
+                    rec[6] = after >= 0.0 ? 1.0 : -1.0;
+                    rec[7] = 1.0 * (double)ns;
+                }
+                a->counts[idx] = cc + 1;
+            }
+        }
+        x0 = y0; x1 = y1; x2 = y2;
+        v0 = w0; v1 = w1; v2 = w2;
+        /* the end of the step: off the mesh or in an electrode (pyccel.py:567-624) */
+        const double px = x0;
+        double py = x1, pz = x2;
+        if (a->mirror_y != 0 && py < a->origin_y) py = 2.0 * a->origin_y - py;
+        if (a->mirror_z != 0 && pz < a->origin_z) pz = 2.0 * a->origin_z - pz;
+        if (slot >= 0) {
+            const i64 cc = a->path_count[slot];
+            if (cc < a->path_capacity) {
+                double* rec = a->paths + (slot * a->path_capacity + cc) * 3;
+                rec[0] = px; rec[1] = py; rec[2] = pz;
+            }
+            a->path_count[slot] = cc + 1;
+        }
+        if (px >= a->upper_x) {
+            st = 0;
+        } else if (px < a->origin_x) {
+            st = 1;
+        } else if (py > a->upper_y || py < a->origin_y || pz > a->upper_z || pz < a->origin_z) {
+            st = 2;
+        } else {
+            i64 i = (i64)((px - a->origin_x) / a->h);
+            if (i > nx - 2) i = nx - 2;
+            i64 j = (i64)((py - a->origin_y) / a->h);
+            if (j > ny - 2) j = ny - 2;
+            i64 kz = (i64)((pz - a->origin_z) / a->h);
+            if (kz > nz - 2) kz = nz - 2;
+            const i64 cell = a->cells[(i * a->cells_n1 + j) * a->cells_n2 + kz];
+            if (cell >= 0) {
+                const i64 R = a->refine;
+                i64 si = (i64)(((px - a->origin_x) / a->h - (double)i) * (double)R);
+                if (si < 0) si = 0; else if (si > R - 1) si = R - 1;
+                i64 sj = (i64)(((py - a->origin_y) / a->h - (double)j) * (double)R);
+                if (sj < 0) sj = 0; else if (sj > R - 1) sj = R - 1;
+                i64 sk = (i64)(((pz - a->origin_z) / a->h - (double)kz) * (double)R);
+                if (sk < 0) sk = 0; else if (sk > R - 1) sk = R - 1;
+                const i64 e = a->subcells[((cell * R + si) * R + sj) * R + sk];
+                if (e >= 0) {
+                    st = 3;
+                    a->electrode[k] = e;
+                } else if (e == -2) {
+                    st = -2;
+                }
+            }
+        }
+        if (st != -1) break;
+    }
+    if (st == -1) st = 4;
+    a->x[k] = x0; a->x[N + k] = x1; a->x[2 * N + k] = x2;
+    a->v[k] = v0; a->v[N + k] = v1; a->v[2 * N + k] = v2;
+    a->steps[k] = ns;
+    a->status[k] = st;
+}
+
+/* pyccel.py:128-135: thread t traces every num_threads-th active trajectory into its own
+ * slabs charge[t] and current[t]. */
+void oracle_trace_trajectories_3d(const oracle_trace3d_args* a) {
+    const i64 T = a->num_threads > 0 ? a->num_threads : 1;
+#pragma omp parallel for schedule(static, 1) num_threads(T)
+    for (i64 t = 0; t < T; ++t) {
+        for (i64 m = t; m < a->n_active; m += T) {
+            trace3d_one(a, a->active[m], a->charge + t * a->charge_stride,
+                        a->current + (a->has_j != 0 ? t * a->current_stride : 0));
+        }
+    }
+}

@@ -74,3 +74,30 @@ def test_argument_errors_are_raised_before_any_device_call():
         kernels.staggered_field_2d(
             np.zeros(2), np.zeros(2), np.zeros((3, 2)), np.zeros((2, 3)), 0.0, 0.0, 0.0, np.zeros(2), np.zeros(2)
         )
+
+
+def test_the_3d_tracer_checks_its_arguments_before_any_device_call():
+    import numpy as np
+
+    from beamtracer_amd import kernels
+    from oracle import cases
+
+    L = _lib.lib()
+    assert L.bt_trace3d_args_size() == ctypes.sizeof(_lib.Trace3DArgs)
+    args = cases.kernel3d_case(0, True, False, True)
+    bad = cases.copy_args(args)
+    bad[11] = bad[11][:, :7]  # patch_int (P, 7)
+    with pytest.raises(ValueError, match="patch_int"):
+        kernels.trace_trajectories_3d(*bad)
+    bad = cases.copy_args(args)
+    bad[11][0, 3] = bad[8].size  # the refined patch's E_x beyond fxs: the C-side check
+    with pytest.raises(ValueError, match="edge fields exceed"):
+        kernels.trace_trajectories_3d(*bad)
+    bad = cases.copy_args(args)
+    bad[27] = np.ascontiguousarray(bad[27][:, :-1])  # cells one short in y
+    with pytest.raises(ValueError, match="cells must be"):
+        kernels.trace_trajectories_3d(*bad)
+    bad = cases.copy_args(args)
+    bad[39] = np.zeros((1, 5))  # has_j without room for the current
+    with pytest.raises(ValueError, match="current"):
+        kernels.trace_trajectories_3d(*bad)
