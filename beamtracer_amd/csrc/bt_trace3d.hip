@@ -9,7 +9,8 @@
 // reaches max_steps.  Every step deposits dq dt at its Hermite midpoint on the 8 nodes of
 // its patch cell and, with has_j, the current element dq (x' - x) on the base mesh's nodes;
 // both as 128-bit fixed-point words (bt_device.h: to_fixed / add_words), so the node sums do
-// not depend on the order of the lanes.  The step's arithmetic is the reference's in its
+// not depend on the order of the lanes.  A lane keeps its cell's node sums in registers and
+// moves a node's sum to memory only when the node leaves its cell.  The step's arithmetic is the reference's in its
 // order, with no fused multiply-add, so a trajectory's states, records and paths are
 // bit-identical to the reference arithmetic; indices are clamped in floating point before
 // they become integers, so a non-finite point addresses the mesh's first cell.
@@ -134,17 +135,103 @@ __device__ __forceinline__ void deposit_axis(double s, int n, int& i, double& w)
     else i = 0;
 }
 
-// The 8 node shares of `value` at node cc (pyccel.py:472-480) into the words `acc`.
-__device__ __forceinline__ void deposit8(u64* __restrict__ acc, i64 cc, i64 di, i64 dj, double a, double b, double c,
-                                         double value, int e) {
-    fixed_add(acc, cc, (1.0 - a) * (1.0 - b) * (1.0 - c) * value, e);
-    fixed_add(acc, cc + 1, (1.0 - a) * (1.0 - b) * c * value, e);
-    fixed_add(acc, cc + dj, (1.0 - a) * b * (1.0 - c) * value, e);
-    fixed_add(acc, cc + dj + 1, (1.0 - a) * b * c * value, e);
-    fixed_add(acc, cc + di, a * (1.0 - b) * (1.0 - c) * value, e);
-    fixed_add(acc, cc + di + 1, a * (1.0 - b) * c * value, e);
-    fixed_add(acc, cc + di + dj, a * b * (1.0 - c) * value, e);
-    fixed_add(acc, cc + di + dj + 1, a * b * c * value, e);
+// The node sums of a lane's current cell, NQ quantities: slot q holds the node whose x, y, z
+// parities are the bits of q (so the nodes that two neighbouring cells share keep their
+// slots), as integer-valued doubles in units of 2^e (exact below 2^53).  A deposit's shares
+// are rounded once to that grid, as to_fixed rounds them, so the node totals equal a
+// deposit of every share straight into the words; the cache only moves a node's sum to
+// memory when the node leaves the lane's cell (or the trajectory ends).
+template <int NQ>
+struct NodeCache {
+    int i, j, k;         // the cell (i < 0: empty)
+    i64 origin, di, dj;  // its mesh's first node and strides in the word array
+    i64 qstride;         // the node count between two quantities' grids
+    double s[NQ][8];
+};
+
+template <int NQ>
+__device__ __forceinline__ void cache_init(NodeCache<NQ>& c, i64 qstride) {
+    c.i = c.j = c.k = -1;
+    c.origin = c.di = c.dj = 0;
+    c.qstride = qstride;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c.s[m][q] = 0.0;
+}
+
+// The node of parity `bit` among cell index a's two nodes a, a + 1.
+__device__ __forceinline__ int node_of(int a, int bit) { return a + ((a ^ bit) & 1); }
+
+__device__ __forceinline__ void flush_units(u64* __restrict__ acc, i64 node, double units) {
+    if (units != 0.0) {
+        const i64 q = (i64)units;
+        add_words(acc + kWords * node, (u64)q & 0xffffffffull, (u64)(q >> 32));
+    }
+}
+
+// Moves the cache to cell (i, j, k) of the mesh at `origin` with strides di, dj: the sums of
+// the nodes that the new cell does not hold go to memory (all of them on another mesh).
+template <int NQ>
+__device__ __forceinline__ void cache_move(NodeCache<NQ>& c, u64* __restrict__ acc, int i, int j, int k, i64 origin,
+                                          i64 di, i64 dj) {
+    if (c.i == i && c.j == j && c.k == k && c.origin == origin) return;
+    if (c.i >= 0) {
+        const bool same = c.origin == origin;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int ni = node_of(c.i, q >> 2), nj = node_of(c.j, (q >> 1) & 1), nk = node_of(c.k, q & 1);
+            const bool kept = same && (unsigned)(ni - i) <= 1u && (unsigned)(nj - j) <= 1u && (unsigned)(nk - k) <= 1u;
+            if (!kept) {
+                const i64 node = c.origin + ni * c.di + nj * c.dj + nk;
+#pragma unroll
+                for (int m = 0; m < NQ; ++m) {
+                    flush_units(acc, node + m * c.qstride, c.s[m][q]);
+                    c.s[m][q] = 0.0;
+                }
+            }
+        }
+    }
+    c.i = i; c.j = j; c.k = k;
+    c.origin = origin; c.di = di; c.dj = dj;
+}
+
+template <int NQ>
+__device__ __forceinline__ void cache_flush(NodeCache<NQ>& c, u64* __restrict__ acc) {
+    cache_move(c, acc, -1, -1, -1, -1, 0, 0);
+    c.i = -1;
+}
+
+// The trilinear shares of value[m] (pyccel.py:472-480: ((1 - a)(1 - b))(1 - c) value for
+// node (i, j, k), and alike) into the cache's cell, which must be the deposit's cell.
+template <int NQ>
+__device__ __forceinline__ void cache_add(NodeCache<NQ>& c, u64* __restrict__ acc, double a, double b, double cz,
+                                         const double (&value)[NQ], int e) {
+    const double a0 = 1.0 - a, b0 = 1.0 - b, c0 = 1.0 - cz;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        // slot q holds the node (i + da, j + db, k + dc) of parities q
+        const int da = (c.i ^ (q >> 2)) & 1, db = (c.j ^ (q >> 1)) & 1, dc = (c.k ^ q) & 1;
+        const double w = ((da ? a : a0) * (db ? b : b0)) * (dc ? cz : c0);
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            const double share = w * value[m];
+            const double t = ldexp(share, -e);
+            if (fabs(t) < 0x1p50) {
+                const double rounded = (t + 0x1.8p52) - 0x1.8p52;  // to nearest, ties to even
+                const double sum = c.s[m][q] + rounded;               // exact: below 2^51
+                if (fabs(sum) < 0x1p51) {
+                    c.s[m][q] = sum;
+                    continue;
+                }
+                flush_units(acc, c.origin + m * c.qstride + (c.i + da) * c.di + (c.j + db) * c.dj + (c.k + dc),
+                            c.s[m][q]);
+                c.s[m][q] = rounded;
+                continue;
+            }
+            fixed_add(acc, c.origin + m * c.qstride + (c.i + da) * c.di + (c.j + db) * c.dj + (c.k + dc), share, e);
+        }
+    }
 }
 
 template <int INTEG, bool HAS_B>
@@ -159,6 +246,10 @@ __device__ void trace_one(const Trace3DParams& p, i64 k) {
     int st = -1;
     int have = 0;
     double ce[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
+    NodeCache<1> charge;
+    NodeCache<3> current;
+    cache_init(charge, 0);
+    cache_init(current, (i64)p.nx * p.ny * p.nz);
     constexpr int NSTAGE = INTEG == 0 ? 4 : 2;
     while (ns < p.max_steps) {
         // the spacing of the finest patch at the step's start (pyccel.py:149-163)
@@ -256,8 +347,9 @@ __device__ void trace_one(const Trace3DParams& p, i64 k) {
             deposit_axis((m0 - pr[0]) / phh, (int)pi[0], i, a);
             deposit_axis((m1 - pr[1]) / phh, (int)pi[1], j, b);
             deposit_axis((m2 - pr[2]) / phh, (int)pi[2], kz, c);
-            const i64 dj = pi[2], di = pi[1] * pi[2];
-            deposit8(p.acc_q, pi[7] + i * di + j * dj + kz, di, dj, a, b, c, dqk * dt, p.e_q);
+            cache_move(charge, p.acc_q, i, j, kz, pi[7], pi[1] * pi[2], pi[2]);
+            const double value[1] = {dqk * dt};
+            cache_add(charge, p.acc_q, a, b, c, value, p.e_q);
         }
         if (p.has_j != 0) {  // the current element dq (x' - x) on the base mesh (pyccel.py:481-540)
             double jd[3] = {dqk * (y0 - x0), dqk * (y1 - x1), dqk * (y2 - x2)};
@@ -268,9 +360,8 @@ __device__ void trace_one(const Trace3DParams& p, i64 k) {
             deposit_axis((m0 - p.ox) / p.h, p.nx, bi, ba);
             deposit_axis((m1 - p.oy) / p.h, p.ny, bj, bb);
             deposit_axis((m2 - p.oz) / p.h, p.nz, bk, bc);
-            const i64 nb = (i64)p.nx * p.ny * p.nz, dj = p.nz, di = (i64)p.ny * p.nz;
-            for (int jc = 0; jc < 3; ++jc)
-                deposit8(p.acc_j, jc * nb + bi * di + bj * dj + bk, di, dj, ba, bb, bc, jd[jc], p.e_j);
+            cache_move(current, p.acc_j, bi, bj, bk, 0, (i64)p.ny * p.nz, p.nz);
+            cache_add(current, p.acc_j, ba, bb, bc, jd, p.e_j);
         }
         ns = ns + 1;
         for (int pl = 0; pl < p.n_planes; ++pl) {  // plane crossings (pyccel.py:543-559)
@@ -341,6 +432,8 @@ __device__ void trace_one(const Trace3DParams& p, i64 k) {
         if (st != -1) break;
     }
     if (st == -1) st = 4;
+    cache_flush(charge, p.acc_q);
+    if (p.has_j != 0) cache_flush(current, p.acc_j);
     p.x[k] = x0; p.x[N + k] = x1; p.x[2 * N + k] = x2;
     p.v[k] = v0; p.v[N + k] = v1; p.v[2 * N + k] = v2;
     p.steps[k] = ns;
@@ -349,8 +442,10 @@ __device__ void trace_one(const Trace3DParams& p, i64 k) {
 
 }  // namespace
 
+// Two waves per SIMD without a magnetic field (the node caches fit 256 registers); with one,
+// the registers the B gather needs.
 template <int INTEG, bool HAS_B>
-__global__ __launch_bounds__(256) void trace3d_kernel(Trace3DParams p) {
+__global__ __launch_bounds__(256, HAS_B ? 1 : 2) void trace3d_kernel(Trace3DParams p) {
     const i64 stride = (i64)gridDim.x * blockDim.x;
     for (i64 m = (i64)blockIdx.x * blockDim.x + threadIdx.x; m < p.n_active; m += stride) {
         const i64 k = p.active[m];

@@ -23,6 +23,8 @@ Beside the headline the line carries, under ``extra``:
 * ``vlasov`` -- Vlasov iterations/s of the einzel lens (host Poisson solve, relaxation 0.5)
   with host arrays (NumPy edge fields, host-form trace with its copies) and device-resident
   (hip edge fields, the beam in HBM), with the time split;
+* ``trace3d`` -- the 3D tracer on a quarter of a focusing lens (symmetry planes), 1e5
+  protons, RK4, charge and current deposits;
 * ``batched`` -- 256 einzel candidates (mesh, voltage, gap, energy and beam size drawn from
   one seed) traced in one batched launch, against one plain launch per design: particle-
   steps/s of each and their ratio, and the rate of the captured HIP graph;
@@ -204,6 +206,101 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkp
     launch_s = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)])) / 1e3
     return {"steps_per_trace": steps_per_trace, "wall": wall, "launch_s": launch_s, "status": status.tolist(),
             "trajectories": host[1].shape[1]}
+
+
+def _edges3(phi: np.ndarray, axis: int, h: float, mirror_low: bool) -> np.ndarray:
+    """The edge field along ``axis`` of a potential with no electrode: differences over h,
+    the end edges linear (the lower one mirrored on a symmetry plane)."""
+    phi = np.moveaxis(phi, axis, 0)
+    inner = (phi[:-1] - phi[1:]) / h
+    out = np.empty((phi.shape[0] + 1, *phi.shape[1:]))
+    out[1:-1] = inner
+    out[0] = -inner[0] if mirror_low else 2.0 * inner[0] - inner[1]
+    out[-1] = 2.0 * inner[-1] - inner[-2]
+    return np.ascontiguousarray(np.moveaxis(out, 0, axis))
+
+
+def trace3d_arguments(particles: int, seed: int = 0) -> list:
+    """The 41 arguments of a 3D trace: a quarter of a focusing lens (symmetry planes at y = 0
+    and z = 0), 161 x 41 x 41 nodes over 20 x 5 x 5 mm, the potential
+    ``-1e5 x + 4e7 (y^2 + z^2 / 2) sin^2(pi x / 20 mm)``; protons at 200 km/s along x from
+    r < 3 mm; RK4, one plane at 19.5 mm, the charge and the current deposited."""
+    nx, ny, nz, h = 161, 41, 41, 0.125e-3
+    x, y, z = np.arange(nx) * h, np.arange(ny) * h, np.arange(nz) * h
+    phi = -1e5 * x[:, None, None] + 4e7 * (y[None, :, None] ** 2 + 0.5 * z[None, None, :] ** 2) * (
+        np.sin(np.pi * x / 0.02) ** 2)[:, None, None]  # fmt: skip
+    fx, fy, fz = _edges3(phi, 0, h, False), _edges3(phi, 1, h, True), _edges3(phi, 2, h, True)
+    rng = np.random.default_rng(seed)
+    n = particles
+    position = np.stack([np.zeros(n), rng.uniform(0.05e-3, 3e-3, n), rng.uniform(0.05e-3, 3e-3, n)])
+    velocity = np.stack([np.full(n, 2e5), np.zeros(n), np.zeros(n)])
+    nodes = nx * ny * nz
+    return [
+        np.arange(n, dtype=np.int64), position, velocity, np.full(n, 9.58e7), np.full(n, 1e-6 / n),
+        np.zeros(n, np.int64), np.full(n, -1, np.int64), np.full(n, -1, np.int64), fx.ravel(), fy.ravel(),
+        fz.ravel(), np.array([[nx, ny, nz, 0, 0, 0, nodes, 0]], np.int64),
+        np.array([[0.0, 0.0, 0.0, h, 0.0, 0.0, 0.0, x[-1], y[-1], z[-1]]]), np.zeros((3, 2, 2, 2)), 0,
+        0.0, 0.0, 0.0, h, x[-1], y[-1], z[-1], 1, 1, 0, 0.5, 100_000,
+        np.full((nx - 1, ny - 1, nz - 1), -1, np.int64), np.full(1, -1, np.int64), 4, np.array([0.0195]),
+        np.zeros((n, 1, 2, 8)), np.zeros((n, 1), np.int64), np.full(n, -1, np.int64), np.zeros((1, 1, 3)),
+        np.zeros(1, np.int64), 1, np.zeros((1, nodes)), 1, np.zeros((1, 3 * nodes)), 0.0,
+    ]  # fmt: skip
+
+
+def trace3d_leg(particles: int, steps: int, cpu_sample: int) -> dict:
+    """The 3D tracer on :func:`trace3d_arguments`: traces timed with HIP events (states
+    reset before each), and the oracle's OpenMP tracer on a sample of the same beam."""
+    import torch
+
+    from beamtracer_amd import kernels
+
+    host = trace3d_arguments(particles)
+    dev = [torch.from_numpy(a).cuda() if isinstance(a, np.ndarray) else a for a in host]
+    for i in (11, 12, 30):  # tables read on the host
+        dev[i] = host[i]
+    outputs = (1, 2, 5, 6, 7, 31, 32, 37, 39)
+    initial = {i: dev[i].clone() for i in outputs}
+    exps = kernels.trace3d_exponents(host[0], host[2], host[4], host[3], host[8], host[9], host[10], host[12], 0.5)
+    stream = torch.cuda.current_stream()
+    times = []
+    for rep in range(steps + 1):
+        for i, a in initial.items():
+            dev[i].copy_(a)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        kernels.trace_trajectories_3d(*dev, exponents=exps, stream=stream)
+        end.record(stream)
+        torch.cuda.synchronize()
+        if rep:
+            times.append(start.elapsed_time(end) / 1e3)
+    particle_steps = int(dev[5].sum().item())
+    status = torch.bincount(dev[6].clamp(min=-2) + 2, minlength=7).cpu().numpy()
+    gpu_s = float(np.median(times))
+    out = {
+        "workload": f"3D quarter focusing lens 161x41x41 (h = 0.125 mm, symmetry planes y = 0 and z = 0), "
+        f"{particles} protons at 200 km/s, RK4, charge + current deposits, one plane",
+        "particle_steps": particle_steps,
+        "ms_per_trace": gpu_s * 1e3,
+        "value": particle_steps / gpu_s,
+        "unit": "particle-steps/s",
+        "status_counts": {str(k - 2): int(c) for k, c in enumerate(status) if c},
+    }
+    if cpu_sample:
+        import oracle
+        from oracle import cases
+
+        threads = len(os.sched_getaffinity(0))
+        sample = cases.copy_args(host)
+        index = np.linspace(0, particles - 1, min(cpu_sample, particles)).round().astype(np.int64)
+        sample[0] = index
+        sample[36], sample[37], sample[39] = threads, np.zeros((threads, host[37].shape[1])), np.zeros(
+            (threads, host[39].shape[1]))  # fmt: skip
+        t0 = time.perf_counter()
+        oracle.trace_trajectories_3d(*sample)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_oracle"] = {"value": int(sample[5][index].sum()) / cpu_s, "cores": threads,
+                             "sample": f"{index.size} of the trajectories"}  # fmt: skip
+    return out
 
 
 def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
@@ -624,6 +721,8 @@ def main() -> int:
         log(f"batched: {opts.batch_designs} einzel candidates")
         processes = min(16, len(os.sched_getaffinity(0)))
         extra["batched"] = batched_leg(opts.batch_designs, 0, processes, 3)
+        log("trace3d: 3D quarter lens, 1e5 trajectories")
+        extra["trace3d"] = trace3d_leg(100_000, 3, 0 if opts.no_cpu else 4000)
         log("adjoint: einzel lens, 2e5 trajectories")
         extra["adjoint"] = adjoint_leg(200_000, 3)
         log("adjoint: 32 einzel candidates")
