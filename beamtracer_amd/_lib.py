@@ -30,6 +30,7 @@ HIPCC_FLAGS = [
 BT_OK, BT_E_ARG, BT_E_NOMEM, BT_E_LAUNCH, BT_E_DEVICE, BT_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 BT_REFILL_BLOCK = 1
 BT_NO_PATHS = 2
+BT_WIDE_WINDOW = 4
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32

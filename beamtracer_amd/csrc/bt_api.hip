@@ -216,11 +216,12 @@ int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t
     // the uniform variant takes one patch that is the mesh itself (same origin and spacing)
     const bool general = a->n_patches > 1 || a->graded != 0 || a->inv_c2 != 0.0 || p.reuse == 0;
     const bool records = (a->flags & BT_NO_PATHS) == 0 || p.ckpt != nullptr;
+    const bool wide = (a->flags & BT_WIDE_WINDOW) != 0;
     int block = a->block_size > 0 ? a->block_size : 256;
     if (block % 64 != 0 || block > 256) return fail(BT_E_ARG, "block_size must be a multiple of 64, at most 256");
     int grid = 0;
     if (a->n_active > 0) {
-        int per_cu = bt::trace_occupancy(integ, geom, has_b, general, records, block);
+        int per_cu = bt::trace_occupancy(integ, geom, has_b, general, records, wide, block);
         if (per_cu < 1) per_cu = 1;
         i64 full = (i64)per_cu * cus;
         i64 need = (a->n_active + block - 1) / block;
@@ -229,7 +230,7 @@ int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t
         if (grid < 1) grid = 1;
     }
     p.charge = a->charge;  // row 0
-    int rc = bt::launch_trace(p, block, grid, stream, integ, geom, has_b, general, records, nodes);
+    int rc = bt::launch_trace(p, block, grid, stream, integ, geom, has_b, general, records, wide, nodes);
     if (rc) return fail(BT_E_LAUNCH, std::string("trace launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;
 }

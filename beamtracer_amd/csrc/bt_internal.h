@@ -64,10 +64,11 @@ struct TraceParams {
 typedef void (*TraceKernel)(TraceParams);
 
 // `records`: paths or checkpoints may be written (else a variant without that code runs)
-TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records);
+// `wide`: the deposit window spans 64 columns instead of 16 (beams whose lanes spread along x)
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide);
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
-                 bool has_b, bool general, bool records, i64 nodes);
-int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, int block);
+                 bool has_b, bool general, bool records, bool wide, i64 nodes);
+int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, bool wide, int block);
 
 // Batched form: one persistent launch over the trajectories of every design (single-patch,
 // uniform, classical meshes); `reset` first restores x, v from x0, v0 and clears steps,

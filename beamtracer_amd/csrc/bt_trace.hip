@@ -51,18 +51,19 @@ struct LaneState {
     // the current point, as the last step's end test computed them (valid when `point` is set)
     double pt_y, pt_cy, pt_cz, pt_sx, pt_sy;
     int point;
-    // the lane's deposit cache: exact fixed-point sums (units 2^e, |sum| < 2^46) for the four
+    // the lane's deposit cache: exact fixed-point sums (units 2^e, |sum| < 2^50) for the four
     // nodes of base-mesh cell (cci, ccj), held as integer-valued doubles (exact below 2^53);
     // slot (a << 1) | b holds the cell's node whose row and column have parities a and b, so a
     // node the next cell shares keeps its slot.  A sum leaves the lane when the trajectory
-    // moves to a cell without that node, when it would reach 2^46, or when the trajectory ends
+    // moves to a cell without that node, when it would reach 2^50, or when the trajectory ends
     int cci, ccj;
     double cq[4];
     i64 ck_next, ck_index;  // the step count of the next checkpoint and its slot
 };
 
 constexpr int kNoCell = -(1 << 30);
-constexpr double kCacheLimit = 0x1p46;
+constexpr double kCacheLimit = 0x1p50;  // a cached sum (exact_i64 needs |q| < 2^51)
+constexpr i64 kWindowLimit = (i64)1 << 46;  // a sum added to a window slot (the slots' bound)
 
 // Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
 __device__ __forceinline__ double node_bilinear(const double* __restrict__ f, double sx, double sy,
@@ -200,7 +201,7 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 
 // ---- the deposit window ---------------------------------------------------------------
 // The workgroup's deposits go into a window of node sums in LDS that follows its beam:
-// columns lo..hi (at most kWinCols; column i in slot i mod kWinCols) of one design's mesh,
+// columns lo..hi (at most WC, 16 or 64; column i in slot i mod WC) of one design's mesh,
 // rows base..base + kWinRows - 1.  The placement is workgroup-uniform for a round and held
 // in registers, so a lane tests a node against it with compares and adds with an LDS
 // integer atomic; a node outside goes to global memory directly.  At the end of each round
@@ -213,7 +214,8 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 // maintenance after round k writes table k + 1 mod 2), so no deposit lands in a slot that
 // is being flushed.
 
-constexpr int kWinCols = 16;
+constexpr int kWinCols = 16;   // the columns of a window (a wide window: kWideCols)
+constexpr int kWideCols = 64;
 constexpr int kWinRows = 64;
 constexpr int kRoundSteps = 2;        // steps between window moves
 constexpr int kFlushRounds = 16;      // every slot is emptied at least this often
@@ -221,17 +223,19 @@ constexpr int kColumnBits = 20;       // a placement key is (design << 20) | col
 static_assert(kFlushRounds * kRoundSteps <= 32, "window slots must stay below 2^62");
 
 // A slot sums a node's deposits in one signed 64-bit word.  What enters the window are the
-// lanes' cached sums (each below 2^46 units in magnitude); a lane adds at most two of them to
+// lanes' cached sums below 2^46 units in magnitude (larger ones go to the node's words in
+// global memory); a lane adds at most two of them to
 // a node per step (one when its sum would overflow, one when it leaves the node), a round is
 // kRoundSteps steps of at most 1024 lanes, and every slot is flushed at least every
 // kFlushRounds rounds: |sum| < kFlushRounds * kRoundSteps * 1024 * 2 * 2^46 = 2^62.
+template <int WC>
 struct DepositWindow {
     int lo[2], hi[2], base[2], design[2];  // the placement of table t (hi < lo: no columns)
     i64 ofn[2];                            // its design's node offset and rows (flush addresses)
     int ny[2];
     long long key[2][2];                   // min / max placement key noted in the round
     int rows[2][2];                        // min / max row noted in the round
-    u64 word[kWinCols][kWinRows];
+    u64 word[WC][kWinRows];
 };
 
 // A round's placement, read once into registers.
@@ -239,7 +243,8 @@ struct WinView {
     int lo, hi, base, design;
 };
 
-__device__ __forceinline__ WinView window_view(const DepositWindow& w, int t) {
+template <int WC>
+__device__ __forceinline__ WinView window_view(const DepositWindow<WC>& w, int t) {
     return {w.lo[t], w.hi[t], w.base[t], w.design[t]};
 }
 
@@ -248,15 +253,17 @@ __device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
     add_words(w, q & 0xffffffffull, (u64)((i64)q >> 32));
 }
 
-__device__ __forceinline__ void window_reset_notes(DepositWindow& w, int t) {
+template <int WC>
+__device__ __forceinline__ void window_reset_notes(DepositWindow<WC>& w, int t) {
     w.key[t][0] = 0x7fffffffffffffffll;
     w.key[t][1] = -1;
     w.rows[t][0] = 0x7fffffff;
     w.rows[t][1] = -1;
 }
 
-__device__ __forceinline__ void window_init(DepositWindow& w, const TraceParams& p) {
-    for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) (&w.word[0][0])[k] = 0ull;
+template <int WC>
+__device__ __forceinline__ void window_init(DepositWindow<WC>& w, const TraceParams& p) {
+    for (int k = threadIdx.x; k < WC * kWinRows; k += blockDim.x) (&w.word[0][0])[k] = 0ull;
     if (threadIdx.x < 2) {
         const int t = threadIdx.x;
         w.lo[t] = 0; w.hi[t] = -1; w.base[t] = 0; w.design[t] = 0;
@@ -268,7 +275,8 @@ __device__ __forceinline__ void window_init(DepositWindow& w, const TraceParams&
 // The end of a round: the first and last lane of each wave whose cache holds a cell report
 // it (design, column, row) for the next placement (the notes only place the window; a sum
 // outside it goes to global memory).
-__device__ __forceinline__ void window_note(DepositWindow& win, int t, const LaneState& s, int design) {
+template <int WC>
+__device__ __forceinline__ void window_note(DepositWindow<WC>& win, int t, const LaneState& s, int design) {
     const bool has = s.cci != kNoCell;
     const u64 lanes = __ballot(has);
     const int me = (int)__lane_id();
@@ -281,14 +289,15 @@ __device__ __forceinline__ void window_note(DepositWindow& win, int t, const Lan
     }
 }
 
-// Adds q units (|q| < 2^46) to node (ii, jj) of the design's mesh: into its window slot when
-// the round's window holds it (USE_WIN), else to the node's words in global memory.
-template <bool USE_WIN>
-__device__ __forceinline__ void node_add(DepositWindow& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
+// Adds q units (|q| < 2^50) to node (ii, jj) of the design's mesh: into its window slot when
+// the round's window holds it (USE_WIN) and |q| < 2^46, else to the node's words in global
+// memory.
+template <bool USE_WIN, int WC>
+__device__ __forceinline__ void node_add(DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
                                          i64 ny, int design, int ii, int jj, i64 q) {
-    if (USE_WIN && design == wv.design && ii >= wv.lo && ii <= wv.hi && jj >= wv.base &&
-        jj < wv.base + kWinRows) {
-        atomicAdd(&win.word[ii & (kWinCols - 1)][jj - wv.base], (u64)q);  // two's complement
+    if (USE_WIN && q < kWindowLimit && q > -kWindowLimit && design == wv.design && ii >= wv.lo && ii <= wv.hi &&
+        jj >= wv.base && jj < wv.base + kWinRows) {
+        atomicAdd(&win.word[ii & (WC - 1)][jj - wv.base], (u64)q);  // two's complement
         return;
     }
     add_sum(acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
@@ -303,8 +312,8 @@ __device__ __forceinline__ i64 exact_i64(double q) {
 // their slots, the others leave the lane.  The cache's cell (c, r) holds columns c, c + 1 and
 // rows r, r + 1; column c stays iff i is c - 1 or c, column c + 1 iff i is c or c + 1 (rows
 // alike); slot q holds column c when q's column parity is c's, else c + 1.
-template <bool USE_WIN>
-__device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, const WinView& wv,
+template <bool USE_WIN, int WC>
+__device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win, const WinView& wv,
                                            u64* __restrict__ acc, i64 ofn, i64 ny, int design, int i, int j) {
     if (i == s.cci && j == s.ccj) return;
     const int c = s.cci, r = s.ccj;
@@ -327,8 +336,8 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow& win, con
 }
 
 // Empties the lane's cache (the trajectory ended).
-template <bool USE_WIN>
-__device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, const WinView& wv,
+template <bool USE_WIN, int WC>
+__device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow<WC>& win, const WinView& wv,
                                             u64* __restrict__ acc, i64 ofn, i64 ny, int design) {
     const int c = s.cci, r = s.ccj;
 #pragma unroll
@@ -345,11 +354,11 @@ __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow& win, co
 // The 4-node deposit of one step into cell (i, j) of a patch whose nodes start at `ofn`:
 // node (i + a, j + b) receives ((a ? u : 1 - u) (b ? w : 1 - w)) deposit, evaluated in that
 // order (pyccel.py:407-423).  Each value is rounded once to units of 2^e (ties to even); on
-// the base mesh (in_base) units below 2^45 gather in the lane's cache, elsewhere (or for a
+// the base mesh (in_base) units below 2^50 gather in the lane's cache, elsewhere (or for a
 // larger single deposit) they go to the node words in global memory.  The node sums are
 // exact integers, so the grouping does not change them.
-template <bool USE_WIN>
-__device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, const WinView& wv, u64* __restrict__ acc,
+template <bool USE_WIN, int WC>
+__device__ __forceinline__ void deposit4(LaneState& s, DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc,
                                          i64 ofn, i64 ny, bool in_base, int design, int i, int j,
                                          double u, double w, double deposit, int e) {
     if (in_base) cache_move<USE_WIN>(s, win, wv, acc, ofn, ny, design, i, j);
@@ -361,9 +370,9 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, const
         const bool jb = jo != ((q & 1) != 0);   // the node is j + 1
         const double value = ((ia ? u : u0) * (jb ? w : w0)) * deposit;
         const double units = ldexp(value, -e);
-        if (in_base && fabs(units) < 0x1p45) {
+        if (in_base && fabs(units) < kCacheLimit) {
             const double rounded = (units + 0x1.8p52) - 0x1.8p52;  // to nearest, ties to even
-            const double sum = s.cq[q] + rounded;                   // exact: below 2^47
+            const double sum = s.cq[q] + rounded;                   // exact: below 2^51
             if (fabs(sum) < kCacheLimit) {
                 s.cq[q] = sum;
             } else {
@@ -382,8 +391,8 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow& win, const
 // way; the base row kept while the noted rows fit with that room), flush the slots that
 // leave (all of them with `empty_all`) and write table 1 - t.  `p` describes the mesh of a
 // one-mesh launch; a batched launch reads the chosen design's mesh from its table.
-template <bool BATCHED>
-__device__ __forceinline__ void window_maintain(DepositWindow& win, int t, const TraceParams& p, bool empty_all) {
+template <bool BATCHED, int WC>
+__device__ __forceinline__ void window_maintain(DepositWindow<WC>& win, int t, const TraceParams& p, bool empty_all) {
     const int n = 1 - t;
     const long long kmin = win.key[t][0], kmax = win.key[t][1];
     const int olo = win.lo[t], ohi = win.hi[t], obase = win.base[t], odesign = win.design[t];
@@ -406,7 +415,7 @@ __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, const
         if (nlo < 0) nlo = 0;
         nhi = cmax + 2 + kRoundSteps;
         if (nhi > nx - 1) nhi = nx - 1;
-        if (nhi > nlo + kWinCols - 1) nhi = nlo + kWinCols - 1;
+        if (nhi > nlo + WC - 1) nhi = nlo + WC - 1;
         const int rmin = win.rows[t][0], rmax = win.rows[t][1];
         if (!(ndesign == odesign && rmin - kRoundSteps >= obase && rmax + 1 + kRoundSteps < obase + kWinRows)) {
             nbase = rmin - 4;
@@ -418,9 +427,9 @@ __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, const
     }
     const bool same_rows = !empty_all && ndesign == odesign && nbase == obase;
     if (ohi >= olo) {
-        for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
+        for (int k = threadIdx.x; k < WC * kWinRows; k += blockDim.x) {
             const int slot = k / kWinRows, row = k % kWinRows;
-            const int col = olo + ((slot - olo) & (kWinCols - 1));  // the column slot holds
+            const int col = olo + ((slot - olo) & (WC - 1));  // the column slot holds
             if (col > ohi || (same_rows && col >= nlo && col <= nhi)) continue;  // empty, or stays
             const u64 q = win.word[slot][row];
             if (q != 0) {
@@ -437,21 +446,22 @@ __device__ __forceinline__ void window_maintain(DepositWindow& win, int t, const
 }
 
 // Every node sum still in the window (table t) to global memory.
-__device__ __forceinline__ void window_flush(DepositWindow& win, int t, u64* __restrict__ acc) {
+template <int WC>
+__device__ __forceinline__ void window_flush(DepositWindow<WC>& win, int t, u64* __restrict__ acc) {
     const int lo = win.lo[t], hi = win.hi[t], base = win.base[t], ny = win.ny[t];
     const i64 ofn = win.ofn[t];
     if (hi < lo) return;
-    for (int k = threadIdx.x; k < kWinCols * kWinRows; k += blockDim.x) {
+    for (int k = threadIdx.x; k < WC * kWinRows; k += blockDim.x) {
         const int slot = k / kWinRows, row = k % kWinRows;
-        const int col = lo + ((slot - lo) & (kWinCols - 1));
+        const int col = lo + ((slot - lo) & (WC - 1));
         const u64 q = win.word[slot][row];
         if (col <= hi && q != 0) add_sum(acc + kWords * (ofn + (i64)col * ny + base + row), q);
     }
 }
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN>
-__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, DepositWindow& win,
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN, int WC>
+__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, DepositWindow<WC>& win,
                                     const WinView& wv) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
@@ -739,9 +749,9 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
 //
 // RECORDS: the launch may record paths and checkpoints (else that code is compiled out; a
 // batched launch records neither).
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS, int WC = kWinCols>
 __device__ __forceinline__ void trace_loop(const TraceParams& p) {
-    __shared__ DepositWindow win;
+    __shared__ DepositWindow<WC> win;
     window_init(win, p);
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -837,9 +847,9 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     window_flush(win, t, p.acc);
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
 __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
-    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false, RECORDS>(p);
+    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false, RECORDS, WC>(p);
 }
 
 // Every design's trajectories in one persistent launch.
@@ -896,29 +906,32 @@ __global__ void finalize_kernel(const u64* __restrict__ acc, i64 nodes, int e, d
     }
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
 static TraceKernel kernel_ptr() {
-    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL, RECORDS>;
+    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL, RECORDS, WC>;
 }
 
-// The general variants (refined patches, graded axis, relativistic motion) always record.
-TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records) {
-#define BT_SEL(I, G, B, X, R) \
-    if (integ == I && geom == G && has_b == B && general == X && (X || records == R)) \
-        return kernel_ptr<I, G, B, X, R>();
-#define BT_SEL_G(I, G) BT_SEL(I, G, false, false, false) BT_SEL(I, G, false, false, true) \
-                       BT_SEL(I, G, false, true, true) BT_SEL(I, G, true, false, false) \
-                       BT_SEL(I, G, true, false, true) BT_SEL(I, G, true, true, true)
+// The general variants (refined patches, graded axis, relativistic motion) always record and
+// use the narrow window; the others come with either window width.
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide) {
+#define BT_SEL(I, G, B, X, R, W) \
+    if (integ == I && geom == G && has_b == B && general == X && (X || records == R) && (X || wide == W)) \
+        return kernel_ptr<I, G, B, X, R, W ? kWideCols : kWinCols>();
+#define BT_SEL_B(I, G, B) BT_SEL(I, G, B, false, false, false) BT_SEL(I, G, B, false, true, false) \
+                          BT_SEL(I, G, B, false, false, true) BT_SEL(I, G, B, false, true, true) \
+                          BT_SEL(I, G, B, true, true, false)
+#define BT_SEL_G(I, G) BT_SEL_B(I, G, false) BT_SEL_B(I, G, true)
     BT_SEL_G(0, 0) BT_SEL_G(0, 1) BT_SEL_G(0, 2)
     BT_SEL_G(1, 0) BT_SEL_G(1, 1) BT_SEL_G(1, 2)
 #undef BT_SEL_G
+#undef BT_SEL_B
 #undef BT_SEL
     return nullptr;
 }
 
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
-                 bool has_b, bool general, bool records, i64 nodes) {
-    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records);
+                 bool has_b, bool general, bool records, bool wide, i64 nodes) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records, wide);
     if (kern == nullptr) return -1;
     const i64 words = kWords * nodes;
     int zgrid = (int)((words + 255) / 256);
@@ -1005,8 +1018,8 @@ int launch_arith_selftest(const double* y, const double* z, i64 n, double* r, do
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, int block) {
-    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records);
+int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, bool wide, int block) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records, wide);
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),

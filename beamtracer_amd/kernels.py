@@ -353,7 +353,7 @@ def trace_trajectories_2d(
     max_steps, cells, subcells, refine, planes, hits, counts, path_slot, paths,
     path_count, num_threads, charge, graded, x_nodes, inv_c2,
     *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
-    refill_block=False, checkpoints=None, checkpoint_interval=64, record_paths=True,
+    refill_block=False, checkpoints=None, checkpoint_interval=64, record_paths=True, wide_window=False,
 ) -> None:  # fmt: skip
     """Trace the ``active`` trajectories until they end, or reach a surface.
 
@@ -369,7 +369,9 @@ def trace_trajectories_2d(
     trajectory before each step whose count is a multiple of ``checkpoint_interval``;
     ``record_paths=False`` says that no trajectory records a path (``BT_NO_PATHS``: every
     ``path_slot`` is negative, else those paths go unrecorded) and runs the trace variant
-    without the path code (the host form finds this itself).
+    without the path code (the host form finds this itself); ``wide_window`` gives the deposit
+    window in LDS 64 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes spread
+    along x). None of these changes a result.
     """
     L = _lib.lib()
     device = _is_torch(x)
@@ -433,7 +435,8 @@ def trace_trajectories_2d(
         int(graded), _ptr(x_nodes), x_nodes.shape[0], float(inv_c2),
         int(magnetic is not None), _ptr(bx), _ptr(by), _ptr(bz),
         int(exponent), int(block_size), int(max_blocks),
-        (_lib.BT_REFILL_BLOCK if refill_block else 0) | (0 if record_paths else _lib.BT_NO_PATHS),
+        (_lib.BT_REFILL_BLOCK if refill_block else 0) | (0 if record_paths else _lib.BT_NO_PATHS)
+        | (_lib.BT_WIDE_WINDOW if wide_window else 0),
         _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
         None, 0, 0,
     )  # fmt: skip

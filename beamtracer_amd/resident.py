@@ -150,7 +150,7 @@ class ResidentTrace:
                 ]  # fmt: skip
                 kernels.trace_trajectories_2d(*call, magnetic=self._magnetic, exponent=exponent,
                                               block_size=tracer.block_size, refill_block=tracer.refill_block,
-                                              record_paths=self._records)  # fmt: skip
+                                              record_paths=self._records, wide_window=tracer.wide_window)  # fmt: skip
                 paused = active[s["status"][active] == _PAUSED]
                 if not paused.numel():
                     break

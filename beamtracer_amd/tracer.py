@@ -113,6 +113,7 @@ class HipBeamTracer:
         device: bool = False,
         block_size: int = 0,
         refill_block: bool = False,
+        wide_window: bool = False,
     ) -> None:
         if not 0.0 < step_fraction <= 1.0:
             raise ValueError(f"step_fraction must be in (0, 1], got {step_fraction}")
@@ -132,6 +133,8 @@ class HipBeamTracer:
         self.block_size = int(block_size)
         #: a workgroup takes new trajectories together (see :mod:`beamtracer_amd.tuning`)
         self.refill_block = bool(refill_block)
+        #: a deposit window of 64 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
+        self.wide_window = bool(wide_window)
         self._lookup: tuple[np.ndarray, np.ndarray] | None = None
         self._capacity = (4, 1024)
         self.reduce = None
@@ -342,7 +345,7 @@ class HipBeamTracer:
             self._call_device(call, magnetic, exponent)
         else:
             kernels.trace_trajectories_2d(*call, magnetic=magnetic, exponent=exponent, block_size=self.block_size,
-                                          refill_block=self.refill_block)  # fmt: skip
+                                          refill_block=self.refill_block, wide_window=self.wide_window)  # fmt: skip
 
     def _call_device(self, call: list, magnetic, exponent: int) -> None:
         """One kernel call with the arrays on the GPU (copied back into ``call`` after)."""
@@ -355,6 +358,7 @@ class HipBeamTracer:
         ck, interval = self.checkpoints if self.checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size,
                                       refill_block=self.refill_block, record_paths=bool((call[27] >= 0).any()),
+                                      wide_window=self.wide_window,
                                       checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
         torch.cuda.synchronize()
         for i in (1, 2, 5, 6, 7, 25, 26, 28, 29, 31):

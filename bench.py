@@ -166,7 +166,7 @@ def device_arguments(host, magnetic=None):
 
 
 def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkpoints=None,
-                refill_block: bool = False) -> dict:
+                refill_block: bool = False, wide_window: bool = False) -> dict:
     """Trace every trajectory ``steps`` times (after ``warmup``), states reset before each
     (``checkpoints``: (tensor, interval) receiving the states, as for the adjoint)."""
     import torch
@@ -186,7 +186,7 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkp
         ck, interval = checkpoints if checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=block_size, stream=stream,
                                       checkpoints=ck, checkpoint_interval=interval, refill_block=refill_block,
-                                      record_paths=bool((host[27] >= 0).any()))  # fmt: skip
+                                      record_paths=bool((host[27] >= 0).any()), wide_window=wide_window)  # fmt: skip
         if end is not None:
             end.record(stream)
 
@@ -315,12 +315,14 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
     tracer = HipBeamTracer(design.mesh, field, design.magnetic, planes=design.planes, integrator="boris")
     host = kernel_arguments(design, tracer)
     default = time_traces(host, design.magnetic, steps, 1, block_size)
-    # the launch shape by the tuner (threads per workgroup x refill mode), then the timed traces
+    # the launch shape by the tuner (threads per workgroup x refill mode x window width), then the timed traces
     dev, mag = device_arguments(host, design.magnetic)
     tune = tuning.tune_trace(dev, magnetic=mag, repeats=1)
-    log(f"extraction tuned: block {tune['block_size']} refill_block {tune['refill_block']} ({tune['best_ms']:.1f} ms)")
+    log(f"extraction tuned: block {tune['block_size']} refill_block {tune['refill_block']} "
+        f"wide_window {tune['wide_window']} ({tune['best_ms']:.1f} ms)")
     del dev, mag
-    r = time_traces(host, design.magnetic, steps, 1, tune["block_size"], refill_block=tune["refill_block"])
+    r = time_traces(host, design.magnetic, steps, 1, tune["block_size"], refill_block=tune["refill_block"],
+                    wide_window=tune["wide_window"])  # fmt: skip
     rate = r["steps_per_trace"] * steps / r["wall"]
     flops = FLOP_PER_STEP_BORIS_B_CYL * r["steps_per_trace"] / r["launch_s"] / 1e12
     return {
@@ -334,7 +336,8 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
         "fp64_tflops": flops,
         "frac_fp64_peak": flops / PEAK_FP64_TFLOPS,
         "flop_per_step": FLOP_PER_STEP_BORIS_B_CYL,
-        "launch_shape": {"block_size": tune["block_size"], "refill_block": tune["refill_block"]},
+        "launch_shape": {"block_size": tune["block_size"], "refill_block": tune["refill_block"],
+                         "wide_window": tune["wide_window"]},
         "tuning_ms": tune["timings_ms"],
         "default_shape_launch_ms": default["launch_s"] * 1e3,
     }

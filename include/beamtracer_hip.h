@@ -59,6 +59,10 @@ extern "C" {
  * then runs a variant without the path code (the host form finds this itself).  A path slot
  * that is not negative is ignored under this flag. */
 #define BT_NO_PATHS 2
+/* flags of a trace: the deposit window in LDS spans 64 mesh columns instead of 16 (faster for
+ * beams whose lanes spread along x; results unchanged; ignored with refined patches, a
+ * graded axis or relativistic motion). */
+#define BT_WIDE_WINDOW 4
 
 /* Every argument of trace_trajectories_2d (trace_trajectories_2d_pyccel.py:42-78) as a
  * pointer with its extents, then the node magnetic field that the reference's compiled
@@ -96,7 +100,7 @@ typedef struct bt_trace2d_args {
     int32_t deposit_exponent;   /* fixed-point LSB 2^e of the deposit; see bt_deposit_exponent */
     int32_t block_size;         /* threads per block (64..256, a multiple of 64); 0 = default 256 */
     int32_t max_blocks;         /* persistent grid size; 0 = fill the device            */
-    int32_t flags;              /* BT_REFILL_BLOCK | BT_NO_PATHS, or 0                  */
+    int32_t flags;              /* BT_REFILL_BLOCK | BT_NO_PATHS | BT_WIDE_WINDOW, or 0 */
     void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned */
     int64_t workspace_bytes;
     /* extension: the state (x, v) of every trajectory before each step whose count is a

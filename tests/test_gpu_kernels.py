@@ -184,6 +184,41 @@ def test_trace_without_paths_runs_the_lean_variant_with_the_same_results(hip, cy
             assert np.array_equal(dev[i].cpu().numpy(), host[i]), (record, i)
 
 
+@pytest.mark.parametrize("cylindrical,integrator,magnetic", list(itertools.product([0, 1, 2], [0, 1], [False, True])))
+def test_wide_deposit_window_gives_the_same_bits(hip, cylindrical, integrator, magnetic):
+    """A deposit window of 64 columns (``wide_window``) and the default 16 give the same trace
+    and the same charge, workgroup or lane refill."""
+    import torch
+
+    args = cases.kernel_case(cylindrical, integrator)
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    b = _magnetic(args) if magnetic else None
+    narrow = _run(hip.trace_trajectories_2d, args, magnetic=b)
+    mag = None if b is None else tuple(torch.from_numpy(c.copy()).cuda() for c in b)
+    for refill in (False, True):
+        dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+        hip.trace_trajectories_2d(*dev, magnetic=mag, wide_window=True, refill_block=refill, block_size=64)
+        torch.cuda.synchronize()
+        for i in (*cases.OUTPUTS, cases.CHARGE):
+            assert np.array_equal(dev[i].cpu().numpy(), narrow[i]), (refill, i)
+
+
+def test_wide_deposit_window_on_a_beam_gives_the_same_bits(hip):
+    """3000 einzel-lens trajectories: the window moves along the lens and wraps its slots."""
+    import bench
+
+    design, tracer = bench.build_workload(3000)
+    host = bench.kernel_arguments(design, tracer)
+    results = []
+    for wide in (False, True):
+        args = cases.copy_args(host)
+        hip.trace_trajectories_2d(*args, wide_window=wide)
+        results.append(args)
+    assert (results[0][6] == 0).sum() > 1000
+    for i in (*cases.OUTPUTS, cases.CHARGE):
+        assert np.array_equal(results[1][i], results[0][i]), i
+
+
 def test_trace_without_active_and_at_max_steps(hip):
     args = cases.kernel_case(1, 0)
     args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
