@@ -201,7 +201,7 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 
 // ---- the deposit window ---------------------------------------------------------------
 // The workgroup's deposits go into a window of node sums in LDS that follows its beam:
-// columns lo..hi (at most WC, 16 or 64; column i in slot i mod WC) of one design's mesh,
+// columns lo..hi (at most WC, 16 or 128; column i in slot i mod WC) of one design's mesh,
 // rows base..base + kWinRows - 1.  The placement is workgroup-uniform for a round and held
 // in registers, so a lane tests a node against it with compares and adds with an LDS
 // integer atomic; a node outside goes to global memory directly.  At the end of each round
@@ -215,7 +215,7 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 // is being flushed.
 
 constexpr int kWinCols = 16;   // the columns of a window (a wide window: kWideCols)
-constexpr int kWideCols = 64;
+constexpr int kWideCols = 128;
 constexpr int kWinRows = 64;
 constexpr int kRoundSteps = 2;        // steps between window moves
 constexpr int kFlushRounds = 16;      // every slot is emptied at least this often

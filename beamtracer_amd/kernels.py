@@ -370,7 +370,7 @@ def trace_trajectories_2d(
     ``record_paths=False`` says that no trajectory records a path (``BT_NO_PATHS``: every
     ``path_slot`` is negative, else those paths go unrecorded) and runs the trace variant
     without the path code (the host form finds this itself); ``wide_window`` gives the deposit
-    window in LDS 64 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes spread
+    window in LDS 128 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes spread
     along x). None of these changes a result.
     """
     L = _lib.lib()

@@ -133,7 +133,7 @@ class HipBeamTracer:
         self.block_size = int(block_size)
         #: a workgroup takes new trajectories together (see :mod:`beamtracer_amd.tuning`)
         self.refill_block = bool(refill_block)
-        #: a deposit window of 64 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
+        #: a deposit window of 128 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
         self.wide_window = bool(wide_window)
         self._lookup: tuple[np.ndarray, np.ndarray] | None = None
         self._capacity = (4, 1024)

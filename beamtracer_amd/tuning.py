@@ -5,7 +5,7 @@ node sums are exact integers and every lane steps its own trajectory): workgroup
 threads suit coherent beams whose lanes stay together, while beams whose trajectories end at
 scattered times (electrodes, reflections) keep their workgroups together longer with a
 workgroup-wide refill (``BT_REFILL_BLOCK``) and smaller workgroups, and beams whose lanes
-spread along x (a fine mesh, a thermal start) deposit faster into a deposit window of 64
+spread along x (a fine mesh, a thermal start) deposit faster into a deposit window of 128
 columns (``BT_WIDE_WINDOW``) than into the 16 that suit a coherent beam. :func:`tune_trace`
 times the candidates on the real arguments and returns the fastest; :class:`HipBeamTracer`
 takes the choice as ``block_size``, ``refill_block`` and ``wide_window``. (The batched form has

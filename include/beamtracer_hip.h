@@ -59,7 +59,7 @@ extern "C" {
  * then runs a variant without the path code (the host form finds this itself).  A path slot
  * that is not negative is ignored under this flag. */
 #define BT_NO_PATHS 2
-/* flags of a trace: the deposit window in LDS spans 64 mesh columns instead of 16 (faster for
+/* flags of a trace: the deposit window in LDS spans 128 mesh columns instead of 16 (faster for
  * beams whose lanes spread along x; results unchanged; ignored with refined patches, a
  * graded axis or relativistic motion). */
 #define BT_WIDE_WINDOW 4
