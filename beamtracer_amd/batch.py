@@ -38,6 +38,7 @@ from beamtracer_amd.tracer import (
     HipBeamTracer,
     PlaneHits,
     TraceResult,
+    beam_start,
     geometry_code,
 )
 
@@ -178,45 +179,26 @@ class BatchedTracer:
         """Concatenated host state of every design's beam, as the plain tracer starts it."""
         if len(particles) != len(self.tracers):
             raise ValueError("one beam per design")
-        xs, vs, qms, dqs, currents, species, design_of, first = [], [], [], [], [], [], [], []
-        status, electrode, starts, exponents = [], [], [], []
-        n = 0
-        for d, (t, p) in enumerate(zip(self.tracers, particles)):
-            x = np.array(p.position, dtype=float, order="C")
-            v = t._state(p.velocity)
-            qm = np.ascontiguousarray(p.charge / p.mass, dtype=float)
-            cur = p.current.astype(float)
-            dq = np.ascontiguousarray(np.sign(p.charge) * np.abs(cur))
-            st = np.full(len(p), _ALIVE, dtype=np.int64)
-            el = np.full(len(p), -1, dtype=np.int64)
-            if t.mesh.electrodes and len(p):
-                hit = t.mesh.inside_solid(*t._project(x))
-                st[hit >= 0] = ELECTRODE
-                el[hit >= 0] = hit[hit >= 0]
-            start = np.flatnonzero(st == _ALIVE).astype(np.int64)
+        beams = [beam_start(t, p) for t, p in zip(self.tracers, particles)]
+        exponents = []
+        for t, b in zip(self.tracers, beams):
             fx, fy, _, patch_real = t.patch_arrays()
-            exponents.append(kernels.trace_exponent(start, v, dq, qm, fx, fy, patch_real, t.step_fraction,
+            exponents.append(kernels.trace_exponent(b.start, b.v, b.dq, b.qm, fx, fy, patch_real, t.step_fraction,
                                                     t._inv_c2()))  # fmt: skip
-            xs.append(x)
-            vs.append(v)
-            qms.append(qm)
-            dqs.append(dq)
-            currents.append(cur)
-            species.append(p.species)
-            design_of.append(np.full(len(p), d, dtype=np.int32))
-            status.append(st)
-            electrode.append(el)
-            starts.append(start + n)
-            first.append(n)
-            n += len(p)
-        first.append(n)
+        sizes = [b.x.shape[1] for b in beams]
+        first = [0, *np.cumsum(sizes).tolist()]
+        n = first[-1]
+
+        def joined(name, axis=0):
+            return np.ascontiguousarray(np.concatenate([getattr(b, name) for b in beams], axis=axis))
+
         return {
-            "x": np.ascontiguousarray(np.concatenate(xs, axis=1)),
-            "v": np.ascontiguousarray(np.concatenate(vs, axis=1)),
-            "qm": np.concatenate(qms), "dq": np.concatenate(dqs), "current": np.concatenate(currents),
-            "species": np.concatenate(species), "design_of": np.concatenate(design_of),
-            "status": np.concatenate(status), "electrode": np.concatenate(electrode),
-            "starts": starts, "first": first, "n": n, "exponents": exponents,
+            "x": joined("x", 1), "v": joined("v", 1), "qm": joined("qm"), "dq": joined("dq"),
+            "current": joined("current"), "species": joined("species"),
+            "design_of": np.repeat(np.arange(len(beams), dtype=np.int32), sizes),
+            "status": joined("status"), "electrode": joined("electrode"),
+            "starts": [b.start + offset for b, offset in zip(beams, first)], "first": first, "n": n,
+            "exponents": exponents,
         }  # fmt: skip
 
     def _queue(self, starts: Sequence[np.ndarray]) -> np.ndarray:

@@ -1,22 +1,24 @@
 """The caller of the hip tracer: ``BeamTracer``'s compiled-trace protocol on MI355X.
 
-:class:`HipBeamTracer` restates the boundary of the reference's fused kernel,
-``BeamTracer._trace_compiled`` (``solver/tracer.py:1185-1304``) with its helpers, around the
-hip ``trace_trajectories_2d``:
+:class:`HipBeamTracer` keeps the boundary of the reference's fused kernel,
+``BeamTracer._trace_compiled`` (``solver/tracer.py:1185-1304``), around the hip
+``trace_trajectories_2d``; what a caller sees is the reference's:
 
 * trajectories starting inside an electrode end at once (``1206-1210``);
 * trajectories stopped in a sub-cell that an electrode surface cuts (status -2) are decided
   by the electrodes' own test and the others resume (``1236-1256``);
-* a trace whose plane or path buffers overflowed is repeated from the saved initial state
-  with larger buffers, which are kept (``1226-1263``);
+* a trace whose plane or path buffers overflowed is repeated from the initial state with
+  larger buffers, which are kept (``1226-1263``);
 * plane crossings are returned in the order of the steps (``1278-1293``) and the node
   charges divided by the node volumes (``_finalize_base``, ``678-698``).
 
-The electrode lookup (``_electrode_lookup``/``_subcells``, ``1092-1139``) samples every
-sub-cell of the cells that touch an electrode at its corners and centre. Unlike the
-reference's compiled path this tracer also takes a magnetic field (node arrays, gathered
-bilinearly by the kernel). ``device=True`` keeps the arrays on the GPU as torch tensors
-(no host copies inside the trace loop).
+The pieces of that protocol are module functions (:func:`beam_start`,
+:func:`resume_paused`, :func:`outgrown`, :func:`collect_plane_hits`), shared with the
+resident trace of the Vlasov loop and with the batched tracer. The electrode lookup
+(``_electrode_lookup``/``_subcells``, ``1092-1139``) gives every sub-cell of the cells that
+touch an electrode the electrode holding its corners and centre. Unlike the reference's
+compiled path this tracer also takes a magnetic field (node arrays, gathered bilinearly by
+the kernel). ``device=True`` runs each launch on GPU copies of the arrays.
 """
 
 from __future__ import annotations
@@ -32,6 +34,7 @@ from beamtracer_amd.beam import BeamParticles
 from beamtracer_amd.fields import StaggeredElectricField
 from beamtracer_amd.mesh import IonOpticsMesh
 
+# the status codes the kernel writes (``tracer.py:35-48``)
 TRANSMITTED, BACKWARD, WALL, ELECTRODE, MAX_STEPS = 0, 1, 2, 3, 4
 STATUS_NAMES = ("transmitted", "backward", "wall", "electrode", "max_steps")
 _ALIVE, _PAUSED = -1, -2
@@ -40,10 +43,32 @@ INTEGRATORS = ("rk4", "boris")
 
 
 def geometry_code(mesh: IonOpticsMesh) -> int:
-    """0 planar, 1 cylindrical, 2 planar with a symmetry plane (``tracer.py:63-67``)."""
-    if mesh.symmetry_plane:
-        return 2
-    return int(mesh.cylindrical)
+    """The kernel's geometry code: 0 planar, 1 cylindrical, 2 planar with a symmetry plane
+    (``tracer.py:63-67``)."""
+    return 2 if mesh.symmetry_plane else (1 if mesh.cylindrical else 0)
+
+
+def node_volumes(mesh: IonOpticsMesh) -> np.ndarray:
+    """The volume ``(nx, ny)`` that divides each node's charge (``tracer.py:678-698``).
+
+    Along x a node owns half of each edge beside it. Across, a node owns the band of
+    half a spacing either side -- in the cylindrical form the ring between those radii, and
+    the first node the disc of radius ``h`` past the origin. The nodes of the two
+    y faces count half of that.
+    """
+    h, base = mesh.h, mesh.origin[1]
+    half = 0.5 * mesh.dx
+    along = np.pad(half, (1, 0)) + np.pad(half, (0, 1))
+    ny = mesh.shape[1]
+    if mesh.cylindrical:
+        radius = (np.arange(ny + 1) - 0.5) * h + base  # the band edges
+        across = np.pi * (radius[1:] ** 2 - radius[:-1] ** 2)
+        across[0] = np.pi * (h + base) ** 2
+    else:
+        across = np.full(ny, h)
+    volume = along[:, None] * across[None, :]
+    volume[:, [0, -1]] *= 0.5
+    return volume
 
 
 @dataclass
@@ -73,25 +98,123 @@ class TraceResult:
     particle_steps: int = 0
 
 
+@dataclass
+class BeamStart:
+    """A beam in the kernel's arrays as a trace starts it (``tracer.py:1195-1217``)."""
+
+    x: np.ndarray  #: (3, n) positions
+    v: np.ndarray  #: (3, n) velocities (momenta per mass when relativistic)
+    qm: np.ndarray  #: charge over mass
+    dq: np.ndarray  #: the current each trajectory deposits, signed as its charge
+    current: np.ndarray
+    species: np.ndarray
+    status: np.ndarray  #: _ALIVE, or ELECTRODE for a start inside an electrode
+    electrode: np.ndarray  #: that electrode, -1 for the others
+    start: np.ndarray  #: the trajectories the kernel traces
+    saved: np.ndarray  #: the trajectories whose paths are kept
+    path_slot: np.ndarray  #: (n,) each trajectory's path slot, -1 if not kept
+    first: np.ndarray  #: (saved, 2) the kept paths' first points, in mesh coordinates
+
+
+def beam_start(tracer: HipBeamTracer, particles: BeamParticles) -> BeamStart:
+    """The start of ``particles`` for ``tracer``: every ``save_trajectories``-th path kept."""
+    n = len(particles)
+    x = np.array(particles.position, dtype=float, order="C")
+    current = particles.current.astype(float)
+    owner = np.full(n, -1, dtype=np.int64)
+    if tracer.mesh.electrodes and n:
+        owner[:] = tracer.mesh.inside_solid(*tracer._project(x))
+    index = np.arange(n, dtype=np.int64)
+    every = tracer.save_trajectories
+    path_slot = np.where(index % every == 0, index // every, -1) if every > 0 else np.full(n, -1, dtype=np.int64)
+    saved = index[path_slot >= 0]
+    return BeamStart(
+        x=x,
+        v=tracer._state(particles.velocity),
+        qm=np.ascontiguousarray(particles.charge / particles.mass, dtype=float),
+        dq=np.ascontiguousarray(np.sign(particles.charge) * np.abs(current)),
+        current=current,
+        species=particles.species,
+        status=np.where(owner >= 0, ELECTRODE, _ALIVE).astype(np.int64),
+        electrode=owner,
+        start=np.flatnonzero(owner < 0).astype(np.int64),
+        saved=saved,
+        path_slot=path_slot.astype(np.int64),
+        first=np.ascontiguousarray(np.stack(tracer._project(x[:, saved]), axis=1)),
+    )
+
+
+def _host(a):
+    return a.cpu().numpy() if hasattr(a, "cpu") else a
+
+
+def resume_paused(tracer: HipBeamTracer, active, x, status, electrode):
+    """Settle the trajectories of ``active`` that the kernel paused in a cut sub-cell
+    (``tracer.py:1236-1256``): those inside an electrode by its own test end there, the
+    others are marked alive again and returned, to be traced on. Works in place on numpy
+    arrays or on GPU tensors (then only the paused positions cross to the host)."""
+    paused = active[status[active] == _PAUSED]
+    if len(paused):
+        owner = np.asarray(tracer.mesh.inside_solid(*tracer._project(_host(x[:, paused]))), dtype=np.int64)
+        if hasattr(paused, "cpu"):
+            import torch
+
+            owner = torch.from_numpy(owner).to(paused.device)
+        struck = owner >= 0
+        status[paused] = _ALIVE
+        status[paused[struck]] = ELECTRODE
+        electrode[paused[struck]] = owner[struck]
+        paused = paused[~struck]
+    return paused
+
+
+def outgrown(capacity: tuple[int, int], counts, path_count) -> tuple[int, int] | None:
+    """The plane-record and path capacities that a repeat of an overflowed trace needs, or
+    None when the trace fit (``tracer.py:1257-1263``): the kernel counts every crossing
+    and every path point, written or not."""
+    most = int(counts.max()) if len(counts) else 0
+    need = (max(capacity[0], most), max(capacity[1], int(path_count.max())))
+    return None if need == tuple(capacity) else need
+
+
 def collect_plane_hits(planes, hits, counts, current, species) -> list[PlaneHits]:
     """The recorded crossings of every plane, in the order of the steps, then of the
-    trajectories (``tracer.py:1278-1293``); ``hits`` ``(n, n_planes, capacity, 8)``."""
+    trajectories (``tracer.py:1278-1293``); ``hits`` ``(n, n_planes, capacity, 8)``, each
+    record (point, velocity, direction, step)."""
     capacity = hits.shape[2]
-    slots = np.arange(capacity)
+    filled = np.arange(capacity) < np.minimum(counts, capacity)[:, :, None]
     out = []
     for p, position in enumerate(planes):
-        k, c = np.nonzero(slots[None, :] < np.minimum(counts[:, p], capacity)[:, None])
-        record = hits[k, p, c]
-        order = np.lexsort((k, record[:, 7]))
-        k, record = k[order], record[order]
-        plane = PlaneHits(position)
-        plane.point = record[:, 0:3].T.copy()
-        plane.velocity = record[:, 3:6].T.copy()
-        plane.current = current[k]
-        plane.species = species[k].astype(np.int32)
-        plane.direction = record[:, 6].astype(np.int8)
-        out.append(plane)
+        owner = np.nonzero(filled[:, p])[0]  # trajectory-major, as the records below
+        record = hits[:, p][filled[:, p]]
+        by_step = np.argsort(record[:, 7], kind="stable")
+        owner, record = owner[by_step], record[by_step]
+        out.append(PlaneHits(
+            position,
+            point=record[:, 0:3].T.copy(),
+            velocity=record[:, 3:6].T.copy(),
+            current=current[owner],
+            species=species[owner].astype(np.int32),
+            direction=record[:, 6].astype(np.int8),
+        ))  # fmt: skip
     return out
+
+
+def trace_result(beam: BeamStart, charge_density, status, electrode, steps, records, planes) -> TraceResult:
+    """The :class:`TraceResult` of a finished trace; ``records`` the kernel's
+    ``(hits, counts, paths, path_count)`` as host arrays."""
+    hits, counts, paths, path_count = records
+    kept = path_count[: beam.saved.size]
+    return TraceResult(
+        charge_density=charge_density,
+        status=status.astype(np.int8),
+        electrode=electrode.astype(np.int32),
+        planes=collect_plane_hits(planes, hits, counts, beam.current, beam.species),
+        trajectories=[paths[slot, :count].copy() for slot, count in enumerate(kept)],
+        trajectory_species=beam.species[beam.saved],
+        steps=int(steps.max()) if steps.size else 0,
+        particle_steps=int(steps.sum()),
+    )
 
 
 class HipBeamTracer:
@@ -115,8 +238,8 @@ class HipBeamTracer:
         refill_block: bool = False,
         wide_window: bool = False,
     ) -> None:
-        if not 0.0 < step_fraction <= 1.0:
-            raise ValueError(f"step_fraction must be in (0, 1], got {step_fraction}")
+        if not (0.0 < step_fraction <= 1.0):
+            raise ValueError(f"step_fraction must lie in (0, 1], got {step_fraction}")
         if integrator not in INTEGRATORS:
             raise ValueError(f"integrator must be one of {INTEGRATORS}, got {integrator!r}")
         self.mesh = mesh
@@ -127,7 +250,7 @@ class HipBeamTracer:
         self.relativistic = bool(relativistic)
         self.speed_of_light = float(speed_of_light)
         self.max_steps = int(max_steps)
-        self.planes = tuple(float(p) for p in planes)
+        self.planes = tuple(map(float, planes))
         self.save_trajectories = int(save_trajectories)
         self.device = bool(device)
         self.block_size = int(block_size)
@@ -136,83 +259,89 @@ class HipBeamTracer:
         #: a deposit window of 128 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
         self.wide_window = bool(wide_window)
         self._lookup: tuple[np.ndarray, np.ndarray] | None = None
+        #: (plane records per trajectory and plane, points per kept path); grown on overflow
         self._capacity = (4, 1024)
+        #: a hook summing the node charges of several processes, or None
         self.reduce = None
         #: device form only: (tensor (C, 6, N), interval) receiving the states every
         #: `interval` steps (the adjoint's checkpoints), or None
         self.checkpoints = None
-        touch = (mesh.solid >= 0) | mesh.near_solid
-        self._electrode_cells = touch[:-1, :-1] | touch[1:, :-1] | touch[:-1, 1:] | touch[1:, 1:]
 
     # -- geometry helpers --------------------------------------------------------------
 
     def _project(self, x: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-        if self.mesh.cylindrical:
+        """Mesh coordinates (x, y) or (x, r) of Cartesian points ``(3, n)``; below a
+        symmetry plane y is folded onto the mesh."""
+        mesh = self.mesh
+        if mesh.cylindrical:
             return x[0], np.hypot(x[1], x[2])
-        if self.mesh.symmetry_plane:
-            y0 = self.mesh.origin[1]
-            return x[0], y0 + np.abs(x[1] - y0)
+        if mesh.symmetry_plane:
+            fold = mesh.origin[1]
+            return x[0], fold + np.abs(x[1] - fold)
         return x[0], x[1]
 
     def _state(self, velocity: np.ndarray) -> np.ndarray:
+        """The kernel's velocity state: the velocity, or the momentum per mass
+        ``v / sqrt(1 - v^2/c^2)`` of a relativistic trace."""
         state = np.array(velocity, dtype=float, order="C")
-        if self.relativistic:
-            c = self.speed_of_light
-            state = np.ascontiguousarray(state / np.sqrt(1.0 - np.sum(state * state, axis=0) / (c * c)))
-        return state
+        if not self.relativistic:
+            return state
+        c2 = self.speed_of_light * self.speed_of_light
+        return np.ascontiguousarray(state / np.sqrt(1.0 - (state * state).sum(axis=0) / c2))
 
     def _inv_c2(self) -> float:
         return 1.0 / self.speed_of_light**2 if self.relativistic else 0.0
 
     def _subcells(self, ci: np.ndarray, cj: np.ndarray) -> np.ndarray:
-        """The electrode of every sub-cell (corners and centre agree), -1, or -2 if cut."""
-        mesh = self.mesh
-
-        def sample(fraction: np.ndarray) -> np.ndarray:
-            px = mesh.position_x(ci[:, None, None] + fraction[None, :, None])
-            py = mesh.origin[1] + mesh.h * (cj[:, None, None] + fraction[None, None, :])
-            px, py = np.broadcast_arrays(px, py)
-            return mesh.inside_solid(px.ravel(), py.ravel()).reshape(px.shape)
-
-        corner = sample(np.arange(SUBCELLS + 1) / SUBCELLS)
-        centre = sample((np.arange(SUBCELLS) + 0.5) / SUBCELLS)
-        same = (
-            (corner[:, :-1, :-1] == centre)
-            & (corner[:, 1:, :-1] == centre)
-            & (corner[:, :-1, 1:] == centre)
-            & (corner[:, 1:, 1:] == centre)
+        """The owner of each sub-cell ``(len(ci), SUBCELLS, SUBCELLS)`` of cells ``(ci, cj)``:
+        the electrode holding its four corners and its centre, -1 for free space, or -2
+        when they disagree, a surface cutting it (``tracer.py:1092-1115``). One electrode
+        test takes the ``(S+1)^2`` corners and ``S^2`` centres of every cell."""
+        mesh, s = self.mesh, SUBCELLS
+        corners, centres = np.arange(s + 1) / s, (np.arange(s) + 0.5) / s
+        fx = np.concatenate([np.repeat(corners, s + 1), np.repeat(centres, s)])
+        fy = np.concatenate([np.tile(corners, s + 1), np.tile(centres, s)])
+        owner = mesh.inside_solid(
+            mesh.position_x(ci[:, None] + fx[None, :]), mesh.origin[1] + mesh.h * (cj[:, None] + fy[None, :])
         )
-        return np.where(same, centre, -2).astype(np.int64)
+        split = (s + 1) ** 2
+        corner = owner[:, :split].reshape(-1, s + 1, s + 1)
+        centre = owner[:, split:].reshape(-1, s, s)
+        five = np.stack([centre, corner[:, :-1, :-1], corner[:, 1:, :-1], corner[:, :-1, 1:], corner[:, 1:, 1:]])
+        return np.where(five.min(axis=0) == five.max(axis=0), centre, -2).astype(np.int64)
 
     def electrode_lookup(self) -> tuple[np.ndarray, np.ndarray]:
-        """``cells`` ``(nx - 1, ny - 1)`` and the flat ``subcells`` (built once per mesh)."""
+        """``cells`` ``(nx - 1, ny - 1)``: the number in ``subcells`` of every cell touching an
+        electrode (a node of it inside one or next to a surface), -1 for the others; the
+        flat ``subcells``: each numbered cell's ``SUBCELLS**2`` owners (``tracer.py:1117-1139``).
+        Built once per mesh."""
         if self._lookup is None:
-            cells = np.full(self._electrode_cells.shape, -1, dtype=np.int64)
-            ci, cj = np.nonzero(self._electrode_cells)
-            if not self.mesh.electrodes or not ci.size:
-                self._lookup = (cells, np.full(1, -1, dtype=np.int64))
-                return self._lookup
-            cells[ci, cj] = np.arange(ci.size, dtype=np.int64)
-            self._lookup = (cells, self._subcells(ci, cj).ravel())
+            mesh = self.mesh
+            touch = (mesh.solid >= 0) | mesh.near_solid
+            cell = touch[:-1, :-1] | touch[1:, :-1] | touch[:-1, 1:] | touch[1:, 1:]
+            if not mesh.electrodes:
+                cell[:] = False
+            number = np.cumsum(cell.ravel()).reshape(cell.shape) - 1
+            cells = np.where(cell, number, -1).astype(np.int64)
+            ci, cj = np.nonzero(cell)
+            subcells = self._subcells(ci, cj).ravel() if ci.size else np.full(1, -1, dtype=np.int64)
+            self._lookup = (cells, subcells)
         return self._lookup
 
     def patch_tables(self) -> tuple[np.ndarray, np.ndarray]:
         """``patch_int`` (P, 6) and ``patch_real`` (P, 7) of the mesh (one patch)."""
         mesh = self.mesh
-        (bx0, by0), (bx1, by1) = mesh.origin, mesh.upper
-        box = (float(mesh.logical_x(bx0)), by0, float(mesh.logical_x(bx1)), by1)
         nxp, nyp = mesh.shape
+        box = (float(mesh.logical_x(mesh.origin[0])), mesh.origin[1], float(mesh.logical_x(mesh.upper[0])), mesh.upper[1])
         patch_int = np.array([[nxp, nyp, 0, 0, nxp * nyp, 0]], dtype=np.int64)
         patch_real = np.array([[*mesh.logical_origin, mesh.h, *box]], dtype=float)
         return patch_int, patch_real
 
     def patch_arrays(self) -> tuple:
         """``fxs``, ``fys``, ``patch_int`` (P, 6), ``patch_real`` (P, 7) of the field (one patch)."""
-        f = self.electric
         patch_int, patch_real = self.patch_tables()
-        fxs = np.ascontiguousarray(f.fx, dtype=float).ravel()
-        fys = np.ascontiguousarray(f.fy, dtype=float).ravel()
-        return fxs, fys, patch_int, patch_real
+        flat = (np.ascontiguousarray(c, dtype=float).ravel() for c in (self.electric.fx, self.electric.fy))
+        return (*flat, patch_int, patch_real)
 
     def magnetic_nodes(self):
         """The node magnetic field: three ``(nx, ny)`` arrays (or None entries), or None."""
@@ -225,25 +354,8 @@ class HipBeamTracer:
         return tuple(None if c is None else np.ascontiguousarray(c, dtype=float) for c in self.magnetic)
 
     def finalize(self, charge: np.ndarray) -> np.ndarray:
-        """Node charges over node volumes, faces halved (``tracer.py:678-698``)."""
-        mesh = self.mesh
-        h = mesh.h
-        density = charge.copy()
-        length = np.zeros(mesh.shape[0])
-        length[:-1] += 0.5 * mesh.dx
-        length[1:] += 0.5 * mesh.dx
-        if mesh.cylindrical:
-            j = np.arange(mesh.shape[1])
-            r1 = (j - 0.5) * h + mesh.origin[1]
-            r2 = (j + 0.5) * h + mesh.origin[1]
-            ring = np.pi * (r2 * r2 - r1 * r1)
-            ring[0] = np.pi * (h + mesh.origin[1]) ** 2
-            density /= length[:, None] * ring[None, :]
-        else:
-            density /= length[:, None] * h
-        density[:, 0] *= 2.0
-        density[:, -1] *= 2.0
-        return density
+        """The charge density: node charges over :func:`node_volumes`."""
+        return charge / node_volumes(self.mesh)
 
     # -- the trace ---------------------------------------------------------------------
 
@@ -251,93 +363,61 @@ class HipBeamTracer:
         """Trace ``particles`` until they leave the mesh or hit an electrode."""
         if self.electric is None:
             raise RuntimeError("the tracer has no electric field yet")
-        result = self._trace_kernel(particles)
-        charge = result.charge_density
+        result = self._trace_beam(particles)
         if self.reduce is not None:
-            charge = self.reduce(charge)
-        result.charge_density = self.finalize(charge)
+            result.charge_density = self.reduce(result.charge_density)
+        result.charge_density = self.finalize(result.charge_density)
         return result
 
-    def _trace_kernel(self, particles: BeamParticles) -> TraceResult:
+    def kernel_scalars(self) -> tuple:
+        """The scalar arguments of ``trace_trajectories_2d`` between the patch tables and the
+        electrode lookup: logical origin, spacing, logical upper corner, geometry,
+        integrator, step fraction, step limit."""
         mesh = self.mesh
+        return (
+            *map(float, mesh.logical_origin), mesh.h, *map(float, mesh.logical_upper), geometry_code(mesh),
+            INTEGRATORS.index(self.integrator), self.step_fraction, self.max_steps,
+        )  # fmt: skip
+
+    def _trace_beam(self, particles: BeamParticles) -> TraceResult:
+        """The node charges (before :meth:`finalize`) and records of a whole trace."""
+        mesh = self.mesh
+        beam = beam_start(self, particles)
         n = len(particles)
-        x = np.array(particles.position, dtype=float, order="C")
-        v = self._state(particles.velocity)
-        qm = np.ascontiguousarray(particles.charge / particles.mass, dtype=float)
-        current = particles.current.astype(float)
-        dq = np.ascontiguousarray(np.sign(particles.charge) * np.abs(current))
-        species = particles.species
-        status = np.full(n, _ALIVE, dtype=np.int64)
-        electrode = np.full(n, -1, dtype=np.int64)
-        steps = np.zeros(n, dtype=np.int64)
-        if mesh.electrodes and n:
-            hit = mesh.inside_solid(*self._project(x))
-            dead = hit >= 0
-            status[dead] = ELECTRODE
-            electrode[dead] = hit[dead]
-        start = np.flatnonzero(status == _ALIVE).astype(np.int64)
-        save = self.save_trajectories
-        saved_ids = np.arange(0, n, save) if save else np.zeros(0, int)
-        path_slot = np.full(n, -1, dtype=np.int64)
-        path_slot[saved_ids] = np.arange(saved_ids.size)
-        first = np.stack(self._project(x[:, saved_ids]))
         fxs, fys, patch_int, patch_real = self.patch_arrays()
-        x_nodes = np.ascontiguousarray(mesh.x if mesh.graded else np.zeros(2))
         cells, subcells = self.electrode_lookup()
-        planes = np.array(self.planes or (np.nan,), dtype=float)
+        planes = np.array(self.planes or (np.nan,), dtype=float)  # a plane never crossed
         magnetic = self.magnetic_nodes()
-        geometry = geometry_code(mesh)
-        integrator = INTEGRATORS.index(self.integrator)
-        initial = (x.copy(), v.copy(), status.copy(), electrode.copy())
-        exponent = kernels.trace_exponent(start, v, dq, qm, fxs, fys, patch_real, self.step_fraction, self._inv_c2())
-        while True:
-            capacity, path_capacity = self._capacity
+        inv_c2 = self._inv_c2()
+        x_nodes = np.ascontiguousarray(mesh.x if mesh.graded else np.zeros(2))
+        exponent = kernels.trace_exponent(beam.start, beam.v, beam.dq, beam.qm, fxs, fys, patch_real,
+                                          self.step_fraction, inv_c2)  # fmt: skip
+        slots = max(beam.saved.size, 1)
+        grown: tuple[int, int] | None = self._capacity
+        while grown is not None:
+            self._capacity = capacity, path_capacity = grown
+            x, v, status, electrode = (a.copy() for a in (beam.x, beam.v, beam.status, beam.electrode))
+            steps = np.zeros(n, dtype=np.int64)
             charge = np.zeros((1, int(patch_int[:, 4].sum())))
             hits = np.zeros((n, planes.size, capacity, 8))
             counts = np.zeros((n, planes.size), dtype=np.int64)
-            paths = np.zeros((max(saved_ids.size, 1), path_capacity, 2))
-            path_count = np.ones(max(saved_ids.size, 1), dtype=np.int64)
-            paths[: saved_ids.size, 0] = first.T
-            active = start
+            paths = np.zeros((slots, path_capacity, 2))
+            paths[: beam.saved.size, 0] = beam.first
+            path_count = np.ones(slots, dtype=np.int64)
+            active = beam.start
             while active.size:
                 call = [
-                    active, x, v, qm, dq, steps, status, electrode, fxs, fys, patch_int, patch_real,
-                    *map(float, mesh.logical_origin), mesh.h, *map(float, mesh.logical_upper),
-                    geometry, integrator, self.step_fraction, self.max_steps, cells, subcells,
-                    SUBCELLS, planes, hits, counts, path_slot, paths, path_count, 1, charge,
-                    int(mesh.graded), x_nodes, self._inv_c2(),
+                    active, x, v, beam.qm, beam.dq, steps, status, electrode, fxs, fys, patch_int, patch_real,
+                    *self.kernel_scalars(), cells, subcells, SUBCELLS, planes, hits, counts, beam.path_slot,
+                    paths, path_count, 1, charge, int(mesh.graded), x_nodes, inv_c2,
                 ]  # fmt: skip
                 self._launch(call, magnetic, exponent)
-                paused = active[status[active] == _PAUSED]
-                if not paused.size:
-                    break
-                hit = mesh.inside_solid(*self._project(x[:, paused]))
-                inside = hit >= 0
-                status[paused[inside]] = ELECTRODE
-                electrode[paused[inside]] = hit[inside]
-                active = paused[~inside]
-                status[active] = _ALIVE
-            most = int(counts.max()) if counts.size else 0
-            longest = int(path_count.max())
-            if most <= capacity and longest <= path_capacity:
-                break
-            self._capacity = (max(capacity, most), max(path_capacity, longest))
-            x, v, status, electrode = (a.copy() for a in initial)
-            steps[:] = 0
-        charge = charge.sum(axis=0).reshape(mesh.shape)
-        hit_planes = collect_plane_hits(self.planes, hits, counts, current, species)
+                active = resume_paused(self, active, x, status, electrode)
+            grown = outgrown(self._capacity, counts, path_count)
         self.last_state = (x, v, steps)
         self.last_records = (hits, counts)
-        return TraceResult(
-            charge_density=charge,
-            status=status.astype(np.int8),
-            electrode=electrode.astype(np.int32),
-            planes=hit_planes,
-            trajectories=[paths[i, : path_count[i]].copy() for i in range(saved_ids.size)],
-            trajectory_species=species[saved_ids],
-            steps=int(steps.max()) if n else 0,
-            particle_steps=int(steps.sum()),
-        )
+        records = (hits, counts, paths, path_count)
+        return trace_result(beam, charge[0].reshape(mesh.shape), status, electrode, steps, records, self.planes)
 
     def _launch(self, call: list, magnetic, exponent: int) -> None:
         """One ``trace_trajectories_2d`` call with the 35 arguments of ``call``."""
