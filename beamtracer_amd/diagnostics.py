@@ -39,6 +39,12 @@ class BeamMoments:
         return dict(self.__dict__)
 
 
+def _moments(coords: np.ndarray, weight: np.ndarray, total: float) -> np.ndarray:
+    """The weighted second-moment matrix ``sum_k w_k c_k c_k^T / total`` of coordinate
+    columns ``coords`` ``(d, k)``."""
+    return (coords * weight) @ coords.T / total
+
+
 def plane_moments(
     hits: PlaneHits,
     *,
@@ -48,64 +54,60 @@ def plane_moments(
     forward_only: bool = True,
     axis: str = "y",
 ) -> BeamMoments:
-    """The :class:`BeamMoments` of the trajectories crossing a plane."""
-    keep = np.ones(hits.current.size, dtype=bool)
-    if species is not None:
-        keep &= hits.species == species
+    """The :class:`BeamMoments` of the trajectories crossing a plane (``diagnostics.py:56-135``).
+
+    ``mass`` [kg] is one value per species (indexed by ``hits.species``) or one value for
+    all; only crossings of ``species`` (all when None) count, and with ``forward_only``
+    only those towards +x. ``axis`` picks the planar coordinate (``"y"`` or ``"z"``).
+    """
+    if not cylindrical and axis not in ("y", "z"):
+        raise ValueError(f"axis must be 'y' or 'z', got {axis!r}")
+    chosen = np.ones(hits.current.size, dtype=bool) if species is None else hits.species == species
     if forward_only:
-        keep &= hits.direction > 0
-    w = np.abs(hits.current[keep])
-    point = hits.point[:, keep]
-    vel = hits.velocity[:, keep]
-    masses = np.asarray(mass, dtype=float)
-    m = masses[hits.species[keep]] if masses.ndim else np.full(w.size, float(masses))
-    total = float(w.sum())
-    if w.size == 0 or total == 0.0:
+        chosen = chosen & (hits.direction > 0)
+    weight = np.abs(hits.current[chosen])
+    total = float(weight.sum())
+    if total == 0.0:  # no crossings, or no current
         nan = float("nan")
-        return BeamMoments(0.0, nan, nan, nan, nan, nan, nan, nan, nan, nan, 0)
-    vx = vel[0]
+        return BeamMoments(0.0, *(nan,) * 9, 0)
+    point, velocity = hits.point[:, chosen], hits.velocity[:, chosen]
+    slope = velocity[1:] / velocity[0]  # dy/dx, dz/dx
     if cylindrical:
-        r = np.hypot(point[1], point[2])
+        # the ring's radius and its slopes along and around the radius, projected on one
+        # transverse plane: every second moment halves and the cross term r a' averages out
+        radius = np.hypot(point[1], point[2])
         with np.errstate(invalid="ignore", divide="ignore"):
-            cy = np.where(r > 0, point[1] / r, 1.0)
-            cz = np.where(r > 0, point[2] / r, 0.0)
-        vr = vel[1] * cy + vel[2] * cz
-        vt = vel[2] * cy - vel[1] * cz
-        rp = vr / vx
-        ap = vt / vx
-        xx = np.sum(w * r * r) / (2.0 * total)
-        pp = np.sum(w * (rp * rp + ap * ap)) / (2.0 * total)
-        xp = np.sum(w * r * rp) / (2.0 * total)
+            ey = np.where(radius > 0, point[1] / radius, 1.0)
+            ez = np.where(radius > 0, point[2] / radius, 0.0)
+        along = slope[0] * ey + slope[1] * ez
+        around = slope[1] * ey - slope[0] * ez
+        spread = 0.5 * _moments(np.stack([radius, along]), weight, total)
+        spread[1, 1] += 0.5 * float(np.sum(weight * around * around) / total)
     else:
-        if axis not in ("y", "z"):
-            raise ValueError(f"axis must be 'y' or 'z', got {axis!r}")
-        a = 1 if axis == "y" else 2
-        y = point[a]
-        yp = vel[a] / vx
-        my = np.sum(w * y) / total
-        myp = np.sum(w * yp) / total
-        xx = np.sum(w * (y - my) ** 2) / total
-        pp = np.sum(w * (yp - myp) ** 2) / total
-        xp = np.sum(w * (y - my) * (yp - myp)) / total
+        k = 1 if axis == "y" else 2
+        coords = np.stack([point[k], slope[k - 1]])
+        coords = coords - (coords @ weight / total)[:, None]
+        spread = _moments(coords, weight, total)
+    xx, xp, pp = float(spread[0, 0]), float(spread[0, 1]), float(spread[1, 1])
     emittance = float(np.sqrt(max(xx * pp - xp * xp, 0.0)))
-    speed2 = np.sum(vel * vel, axis=0)
+    speed2 = (velocity * velocity).sum(axis=0)
     with np.errstate(invalid="ignore", divide="ignore"):
-        lorentz = 1.0 / np.sqrt(1.0 - speed2 / constants.c**2)
-    beta_gamma = float(np.sum(w * lorentz * vx) / total) / constants.c
-    energy = m * lorentz**2 * speed2 / (lorentz + 1.0) / constants.e
-    alpha = -xp / emittance if emittance > 0 else float("nan")
-    beta = xx / emittance if emittance > 0 else float("nan")
-    gamma = pp / emittance if emittance > 0 else float("nan")
+        lorentz = 1.0 / np.sqrt(1.0 - speed2 / constants.c**2)  # NaN at c or above
+    masses = np.asarray(mass, dtype=float)
+    m = masses[hits.species[chosen]] if masses.ndim else float(masses)
+    # (gamma - 1) m c^2 in the form that keeps its digits at low speed
+    kinetic = m * speed2 * lorentz * lorentz / (lorentz + 1.0) / constants.e
+    twiss = (-xp / emittance, xx / emittance, pp / emittance) if emittance > 0 else (float("nan"),) * 3
     return BeamMoments(
         current=total,
         rms_size=float(np.sqrt(xx)),
         rms_angle=float(np.sqrt(pp)),
-        correlation=float(xp),
+        correlation=xp,
         emittance=emittance,
-        normalized_emittance=emittance * beta_gamma,
-        alpha=float(alpha),
-        beta=float(beta),
-        gamma=float(gamma),
-        mean_energy=float(np.sum(w * energy) / total),
-        count=int(w.size),
+        normalized_emittance=emittance * float(np.sum(weight * lorentz * velocity[0]) / total) / constants.c,
+        alpha=twiss[0],
+        beta=twiss[1],
+        gamma=twiss[2],
+        mean_energy=float(np.sum(weight * kinetic) / total),
+        count=int(weight.size),
     )
