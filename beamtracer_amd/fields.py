@@ -19,6 +19,9 @@ from __future__ import annotations
 
 from collections.abc import Sequence
 
+from fractions import Fraction
+from math import factorial
+
 import numpy as np
 
 from beamtracer_amd.mesh import IonOpticsMesh
@@ -156,22 +159,34 @@ class DeviceEdgeField:
         return self.fx, self.fy
 
 
+def central_weights(order: int, half_width: int = 3) -> tuple[float, ...]:
+    """The weights of the central finite difference of the ``order``-th derivative on the
+    ``2 * half_width + 1`` points ``-w..w`` (unit spacing): the exact rational solution of
+    the Taylor conditions ``sum_j c_j j^m / m! = [m == order]``, ``m = 0..2w``, each
+    rounded once to the nearest double."""
+    points = range(-half_width, half_width + 1)
+    size = 2 * half_width + 1
+    rows = [[Fraction(j) ** m / factorial(m) for j in points] + [Fraction(int(m == order))] for m in range(size)]
+    for col in range(size):  # Gauss-Jordan elimination in exact arithmetic
+        pivot = next(r for r in range(col, size) if rows[r][col] != 0)
+        rows[col], rows[pivot] = rows[pivot], rows[col]
+        rows[col] = [v / rows[col][col] for v in rows[col]]
+        for r in range(size):
+            if r != col and rows[r][col] != 0:
+                rows[r] = [a - rows[r][col] * b for a, b in zip(rows[r], rows[col])]
+    return tuple(float(row[-1]) for row in rows)
+
+
 class AxisymmetricMagneticField:
-    """A static axisymmetric magnetic field from its on-axis profile ``B_z(z)``.
+    """A static axisymmetric magnetic field from its on-axis profile ``B_z(z)`` sampled at
+    ``z0 + hz * k`` (``solver/fields.py:205-279``).
 
-    The power series in ``r`` up to ``order`` (at most 6) with 7-point finite-difference
-    derivatives of the sampled profile (``solver/fields.py:205-279``).
+    Off the axis, the field is the paraxial power series in ``r`` up to ``order`` (at most
+    6), ``B_z = sum_k (-1)^k r^2k B^(2k) / (4^k k!^2)`` and
+    ``B_r = sum_k (-1)^(k+1) r^(2k+1) B^(2k+1) / (2^(2k+1) k! (k+1)!)``, with the profile's
+    derivatives from 7-point central differences at the samples (zero within three
+    samples of either end), interpolated linearly between samples.
     """
-
-    STENCILS = (
-        (0, 0, 0, 1, 0, 0, 0),
-        (-1 / 60, 9 / 60, -45 / 60, 0, 45 / 60, -9 / 60, 1 / 60),
-        (2 / 180, -27 / 180, 270 / 180, -490 / 180, 270 / 180, -27 / 180, 2 / 180),
-        (1 / 8, -8 / 8, 13 / 8, 0, -13 / 8, 8 / 8, -1 / 8),
-        (-1 / 6, 12 / 6, -39 / 6, 56 / 6, -39 / 6, 12 / 6, -1 / 6),
-        (-1 / 2, 4 / 2, -5 / 2, 0, 5 / 2, -4 / 2, 1 / 2),
-        (1, -6, 15, -20, 15, -6, 1),
-    )
 
     def __init__(self, z0: float, hz: float, bz, order: int = 6):
         samples = np.asarray(bz, dtype=float)
@@ -181,50 +196,52 @@ class AxisymmetricMagneticField:
             raise ValueError(f"order must be 0 to 6, got {order}")
         self.z0, self.hz, self.order, self.bz = float(z0), float(hz), int(order), samples
         n = samples.size
-        centre = np.arange(3, n - 3)
-        self._derivatives = np.zeros((self.order + 1, n))
-        for k, stencil in enumerate(self.STENCILS[: self.order + 1]):
-            d = sum(c * samples[centre + a - 3] for a, c in enumerate(stencil))
-            self._derivatives[k, centre] = d / self.hz**k
-        self._samples = np.ascontiguousarray(self._derivatives.T)
+        #: row k: the samples' k-th derivative, the series' terms up to ``order``
+        table = np.zeros((self.order + 1, n))
+        for k in range(self.order + 1):
+            acc = np.zeros(n - 6)
+            for offset, w in enumerate(central_weights(k)):
+                acc = acc + w * samples[offset : offset + n - 6]
+            table[k, 3 : n - 3] = acc / self.hz**k
+        self._table = np.ascontiguousarray(table.T)  # one row per sample
 
     def derivatives(self, z: np.ndarray) -> np.ndarray:
-        n = self.bz.size
+        """``d^k B_z / dz^k`` ``(order + 1, len(z))`` at ``z``: the table interpolated
+        linearly, the interval clamped to the one that holds derivatives on both sides."""
         s = (np.asarray(z, dtype=float) - self.z0) / self.hz
-        i = np.clip(np.floor(s).astype(np.int64), 3, n - 5)
-        t = (s - i)[:, None]
-        low = self._samples[i]
-        return (low + t * (self._samples[i + 1] - low)).T
+        k = np.clip(np.floor(s).astype(np.int64), 3, self.bz.size - 5)
+        left = self._table[k]
+        return (left + (s - k)[:, None] * (self._table[k + 1] - left)).T
 
     def __call__(self, z: np.ndarray, r: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """``(B_z, B_r)`` [T] at axial positions ``z`` and radii ``r``."""
         d = self.derivatives(np.ravel(z))
         r = np.ravel(np.asarray(r, dtype=float))
         r2 = r * r
-        bz_c = (1.0, -1.0 / 4.0, 1.0 / 64.0, -1.0 / 2304.0)
-        br_c = (-1.0 / 2.0, 1.0 / 16.0, -1.0 / 384.0)
-        bz = d[0].copy()
-        br = np.zeros_like(bz)
-        power = np.ones_like(r2)
-        for k in range(1, self.order // 2 + 1):
-            power = power * r2
-            bz += bz_c[k] * power * d[2 * k]
-        power = r.copy()
-        for k in range((self.order + 1) // 2):
-            br += br_c[k] * power * d[2 * k + 1]
-            power = power * r2
+        bz, br = d[0].copy(), np.zeros_like(d[0])
+        even, odd = np.ones_like(r), r.copy()  # r^2k and r^(2k+1)
+        for j in range(1, self.order + 1):
+            k = j // 2
+            if j % 2:
+                br += (-1) ** (k + 1) / (2 ** (2 * k + 1) * factorial(k) * factorial(k + 1)) * odd * d[j]
+                odd = odd * r2
+            else:
+                even = even * r2
+                bz += (-1) ** k / (4**k * factorial(k) ** 2) * even * d[j]
         return bz, br
 
 
 def solenoid_on_axis(z, centre: float, length: float, radius: float, peak: float) -> np.ndarray:
-    """On-axis ``B_z`` of a thin-walled solenoid scaled to ``peak`` at its centre (``fields.py:307-321``)."""
-    z = np.asarray(z, dtype=float)
+    """On-axis ``B_z`` of a thin-walled solenoid, scaled to ``peak`` at its centre
+    (``fields.py:307-321``): the difference of the cosines of the angles under which its
+    two ends are seen, ``u / sqrt(u^2 + a^2)`` for an end at axial distance ``u``."""
 
-    def shape(zz):
-        a = zz - centre + 0.5 * length
-        b = zz - centre - 0.5 * length
-        return a / np.hypot(a, radius) - b / np.hypot(b, radius)
+    def cosines(at):
+        u = at - centre
+        upstream, downstream = u + 0.5 * length, u - 0.5 * length
+        return upstream / np.hypot(upstream, radius) - downstream / np.hypot(downstream, radius)
 
-    return peak * shape(z) / shape(np.asarray(centre))
+    return peak * cosines(np.asarray(z, dtype=float)) / cosines(np.asarray(centre))
 
 
 def current_loop_on_axis(z, centre: float, radius: float, peak: float) -> np.ndarray:
