@@ -76,13 +76,11 @@ struct Gathered {
 };
 
 __device__ __forceinline__ Gathered gather_d(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
-    const double fi = fmin(fmax(floor(s), 0.0), (double)imax);
-    const double fj = fmin(fmax(floor(t), 0.0), (double)jmax);
     Gathered g;
-    g.i = (int)fi;
-    g.j = (int)fj;
-    g.u = s - fi;
-    g.w = t - fj;
+    g.i = clamped_floor_index(s, imax);
+    g.j = clamped_floor_index(t, jmax);
+    g.u = s - (double)g.i;
+    g.w = t - (double)g.j;
     g.c = (unsigned)g.i * (unsigned)cols + (unsigned)g.j;
     const Pair lo = load_pair(f, g.c), hi = load_pair(f, g.c + (unsigned)cols);
     const double f00 = lo.a, f01 = lo.b, f10 = hi.a, f11 = hi.b;

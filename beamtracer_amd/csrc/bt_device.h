@@ -176,16 +176,31 @@ __device__ __forceinline__ Pair load_pair(Rsrc r, unsigned elem) {
     return __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 8u, 0, 0));
 }
 
+// trunc(s) saturated to the 32-bit range, NaN to 0: the hardware conversion v_cvt_i32_f64
+// (a C++ conversion of an out-of-range double is undefined, so the instruction is named).
+__device__ __forceinline__ int trunc_sat_i32(double s) {
+    int i;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(i) : "v"(s));
+    return i;
+}
+
+// floor(s) clamped to [0, hi] (hi >= 0), and the same for a NaN s (0): trunc equals floor
+// from 0 up, and everything below 0 clamps to 0 either way -- one conversion and one
+// integer median (v_med3_i32) instead of floor, two double clamps and a conversion.
+__device__ __forceinline__ int clamped_floor_index(double s, int hi) {
+    return min(max(trunc_sat_i32(s), 0), hi);
+}
+
 // clamped_bilinear of the grid that starts at element `base` of buffer r, for the tracer:
-// the cell clamped in double (one conversion per index; the same cell), the three
-// interpolations fused (fma: one rounding each instead of two).
+// the cell clamped as an integer (the same cell and fractions as clamping floor(s) in
+// double), the three interpolations fused (fma: one rounding each instead of two).
 __device__ __forceinline__ double clamped_bilinear_buf(Rsrc r, unsigned base, i64 cols, double s, double t,
                                                        int imax, int jmax) {
-    const double fi = fmin(fmax(floor(s), 0.0), (double)imax);
-    const double fj = fmin(fmax(floor(t), 0.0), (double)jmax);
-    const double u = s - fi;
-    const double w = t - fj;
-    const unsigned c = base + (unsigned)(int)fi * (unsigned)cols + (unsigned)(int)fj;
+    const int i = clamped_floor_index(s, imax);
+    const int j = clamped_floor_index(t, jmax);
+    const double u = s - (double)i;
+    const double w = t - (double)j;
+    const unsigned c = base + (unsigned)i * (unsigned)cols + (unsigned)j;
     const Pair lo = load_pair(r, c), hi = load_pair(r, c + (unsigned)cols);
     const double low = __builtin_fma(w, lo.b - lo.a, lo.a);
     const double high = __builtin_fma(w, hi.b - hi.a, hi.a);

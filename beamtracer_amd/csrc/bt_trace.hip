@@ -51,18 +51,19 @@ struct LaneState {
     // the current point, as the last step's end test computed them (valid when `point` is set)
     double pt_y, pt_cy, pt_cz, pt_sx, pt_sy;
     int point;
-    // the lane's deposit cache: exact fixed-point sums (units 2^e, |sum| < 2^50) for the four
-    // nodes of base-mesh cell (cci, ccj), held as integer-valued doubles (exact below 2^53);
+    // the lane's deposit cache: the FP64 sums of the trajectory's deposits into the four nodes
+    // of base-mesh cell (cci, ccj) since it reached each of them, added in step order;
     // slot (a << 1) | b holds the cell's node whose row and column have parities a and b, so a
-    // node the next cell shares keeps its slot.  A sum leaves the lane when the trajectory
-    // moves to a cell without that node, when it would reach 2^50, or when the trajectory ends
+    // node the next cell shares keeps its slot.  A sum leaves the lane -- rounded once to the
+    // fixed-point units 2^e -- when the trajectory moves to a cell without that node or ends.
+    // Which deposits a sum holds and their order depend on the trajectory alone, so the node
+    // totals do not depend on the launch shape or on the order of the lanes.
     int cci, ccj;
     double cq[4];
     i64 ck_next, ck_index;  // the step count of the next checkpoint and its slot
 };
 
 constexpr int kNoCell = -(1 << 30);
-constexpr double kCacheLimit = 0x1p50;  // a cached sum (exact_i64 needs |q| < 2^51)
 constexpr i64 kWindowLimit = (i64)1 << 46;  // a sum added to a window slot (the slots' bound)
 
 // Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
@@ -224,10 +225,10 @@ static_assert(kFlushRounds * kRoundSteps <= 32, "window slots must stay below 2^
 
 // A slot sums a node's deposits in one signed 64-bit word.  What enters the window are the
 // lanes' cached sums below 2^46 units in magnitude (larger ones go to the node's words in
-// global memory); a lane adds at most two of them to
-// a node per step (one when its sum would overflow, one when it leaves the node), a round is
-// kRoundSteps steps of at most 1024 lanes, and every slot is flushed at least every
-// kFlushRounds rounds: |sum| < kFlushRounds * kRoundSteps * 1024 * 2 * 2^46 = 2^62.
+// global memory); a lane adds at most one of them to
+// a node per step (when its trajectory leaves the node or ends), a round is kRoundSteps steps
+// of at most 1024 lanes, and every slot is flushed at least every kFlushRounds rounds:
+// |sum| < kFlushRounds * kRoundSteps * 1024 * 2^46 = 2^61.
 template <int WC>
 struct DepositWindow {
     int lo[2], hi[2], base[2], design[2];  // the placement of table t (hi < lo: no columns)
@@ -303,9 +304,27 @@ __device__ __forceinline__ void node_add(DepositWindow<WC>& win, const WinView& 
     add_sum(acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
 }
 
-// An integer-valued double below 2^51 in magnitude as a signed 64-bit integer.
+// A double below 2^51 in magnitude rounded to the nearest integer (ties to even), as a signed
+// 64-bit integer: adding 1.5 2^52 rounds it, and the sum's bit pattern then holds it.
 __device__ __forceinline__ i64 exact_i64(double q) {
     return __double_as_longlong(q + 0x1.8p52) - 0x4338000000000000ll;
+}
+
+// A lane's FP64 sum of a node's deposits leaves the lane: rounded once to units 2^e and added
+// to node (ii, jj) (window slot or global words).  A sum of 2^51 units or more (a trajectory
+// that stays very long in one cell) goes to the global words through to_fixed, which drops a
+// non-finite or out-of-range sum as it drops such a single deposit.
+template <bool USE_WIN, int WC>
+__device__ __forceinline__ void sum_leaves(DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
+                                           i64 ny, int design, int ii, int jj, double sum, int e) {
+    const double units = ldexp(sum, -e);
+    if (fabs(units) < 0x1p51) {
+        const i64 q = exact_i64(units);
+        if (q != 0) node_add<USE_WIN>(win, wv, acc, ofn, ny, design, ii, jj, q);
+        return;
+    }
+    u64 lo, hi;
+    if (to_fixed(sum, e, lo, hi)) add_words(acc + kWords * (ofn + (i64)ii * ny + jj), lo, hi);
 }
 
 // Moves the lane's cache to cell (i, j): the sums of nodes that both cells share stay in
@@ -314,7 +333,7 @@ __device__ __forceinline__ i64 exact_i64(double q) {
 // alike); slot q holds column c when q's column parity is c's, else c + 1.
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win, const WinView& wv,
-                                           u64* __restrict__ acc, i64 ofn, i64 ny, int design, int i, int j) {
+                                           u64* __restrict__ acc, i64 ofn, i64 ny, int design, int i, int j, int e) {
     if (i == s.cci && j == s.ccj) return;
     const int c = s.cci, r = s.ccj;
     const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
@@ -326,8 +345,7 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win,
         const bool kept = (at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1);
         if (!kept) {
             if (s.cq[q] != 0.0)
-                node_add<USE_WIN>(win, wv, acc, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1,
-                                  exact_i64(s.cq[q]));
+                sum_leaves<USE_WIN>(win, wv, acc, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q], e);
             s.cq[q] = 0.0;
         }
     }
@@ -338,13 +356,13 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win,
 // Empties the lane's cache (the trajectory ended).
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow<WC>& win, const WinView& wv,
-                                            u64* __restrict__ acc, i64 ofn, i64 ny, int design) {
+                                            u64* __restrict__ acc, i64 ofn, i64 ny, int design, int e) {
     const int c = s.cci, r = s.ccj;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (s.cq[q] != 0.0)
-            node_add<USE_WIN>(win, wv, acc, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
-                              exact_i64(s.cq[q]));
+            sum_leaves<USE_WIN>(win, wv, acc, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
+                                s.cq[q], e);
         s.cq[q] = 0.0;
     }
     s.cci = kNoCell;
@@ -353,15 +371,17 @@ __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow<WC>& win
 
 // The 4-node deposit of one step into cell (i, j) of a patch whose nodes start at `ofn`:
 // node (i + a, j + b) receives ((a ? u : 1 - u) (b ? w : 1 - w)) deposit, evaluated in that
-// order (pyccel.py:407-423).  Each value is rounded once to units of 2^e (ties to even); on
-// the base mesh (in_base) units below 2^50 gather in the lane's cache, elsewhere (or for a
-// larger single deposit) they go to the node words in global memory.  The node sums are
-// exact integers, so the grouping does not change them.
+// order (pyccel.py:407-423).  On the base mesh (in_base) the values add to the lane's FP64
+// sums (rounded to units of 2^e when a sum leaves the lane); on a refined patch each value is
+// rounded once to units of 2^e (ties to even) and goes to the node words in global memory.
+// The node totals are exact integer sums of rounded values, so the order of the lanes does
+// not change them; a non-finite deposit is dropped.
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc,
                                          i64 ofn, i64 ny, bool in_base, int design, int i, int j,
                                          double u, double w, double deposit, int e) {
-    if (in_base) cache_move<USE_WIN>(s, win, wv, acc, ofn, ny, design, i, j);
+    if (!(fabs(deposit) < __builtin_inf())) return;  // (the four values would not be finite)
+    if (in_base) cache_move<USE_WIN>(s, win, wv, acc, ofn, ny, design, i, j, e);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
 #pragma unroll
@@ -369,16 +389,8 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow<WC>& win, c
         const bool ia = io != ((q >> 1) != 0);  // the node is i + 1
         const bool jb = jo != ((q & 1) != 0);   // the node is j + 1
         const double value = ((ia ? u : u0) * (jb ? w : w0)) * deposit;
-        const double units = ldexp(value, -e);
-        if (in_base && fabs(units) < kCacheLimit) {
-            const double rounded = (units + 0x1.8p52) - 0x1.8p52;  // to nearest, ties to even
-            const double sum = s.cq[q] + rounded;                   // exact: below 2^51
-            if (fabs(sum) < kCacheLimit) {
-                s.cq[q] = sum;
-            } else {
-                node_add<USE_WIN>(win, wv, acc, ofn, ny, design, i + ia, j + jb, exact_i64(s.cq[q]));
-                s.cq[q] = rounded;
-            }
+        if (in_base) {
+            s.cq[q] += value;
             continue;
         }
         u64 lo, hi;
@@ -690,10 +702,10 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         const double cj = end_known ? end_sy : (GENERAL ? (py - p.origin_y) / p.h : (py - p.origin_y) * p.inv_h);
         s.pt_y = py; s.pt_cy = pcy; s.pt_cz = pcz; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
         // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341: min(trunc(max(c, 0)),
-        // n - 2, cells - 1), taken as trunc(min(max(c, 0), that bound)) (the same integer for
-        // an integer bound, and a 32-bit conversion)
-        const int i = (int)fmin(fmax(ci, 0.0), (double)p.cell_imax);
-        const int j = (int)fmin(fmax(cj, 0.0), (double)p.cell_jmax);
+        // n - 2, cells - 1), taken as the saturated trunc(c) clamped to [0, that bound] (the
+        // same integer for an integer bound)
+        const int i = clamped_floor_index(ci, p.cell_imax);
+        const int j = clamped_floor_index(cj, p.cell_jmax);
         const i64 cell = p.cells[(unsigned)i * (unsigned)p.cells_cols + (unsigned)j];
         if (cell >= 0) {
             const double R = (double)p.refine;
@@ -834,7 +846,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush<true>(s, win, wv, p.acc, q.bofn, q.ny, q.design);
+                    cache_flush<true>(s, win, wv, p.acc, q.bofn, q.ny, q.design, q.deposit_exponent);
                 }
             }
         }
