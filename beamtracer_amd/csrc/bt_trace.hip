@@ -864,9 +864,12 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
     trace_loop<INTEG, GEOM, HAS_B, GENERAL, false, RECORDS, WC>(p);
 }
 
-// Every design's trajectories in one persistent launch.
+// Every design's trajectories in one persistent launch.  Held to three waves per SIMD (168
+// VGPRs): the lanes' own meshes add about 40 registers to the one-mesh loop, and a batch
+// always fills the GPU, so the third wave's latency hiding outweighs the spills of the RK4
+// cylindrical variant (256 einzel candidates: 536 -> 472 ms).
 template <int INTEG, int GEOM, bool HAS_B>
-__global__ __launch_bounds__(256) void batched_kernel(TraceParams p) {
+__global__ __launch_bounds__(256, 3) void batched_kernel(TraceParams p) {
     trace_loop<INTEG, GEOM, HAS_B, false, true, false>(p);
 }
 
