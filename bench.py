@@ -192,9 +192,6 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkp
 
     for _ in range(warmup):
         launch()
-    torch.cuda.synchronize()
-    steps_per_trace = int(dev[5].sum().item())
-    status = torch.bincount(dev[6].clamp(min=-2) + 2, minlength=7).cpu().numpy()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
     torch.cuda.synchronize()
@@ -203,6 +200,9 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkp
         launch(starts[k], ends[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # every launch starts from the same states, so the last one's counts are each trace's
+    steps_per_trace = int(dev[5].sum().item())
+    status = torch.bincount(dev[6].clamp(min=-2) + 2, minlength=7).cpu().numpy()
     launch_s = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)])) / 1e3
     return {"steps_per_trace": steps_per_trace, "wall": wall, "launch_s": launch_s, "status": status.tolist(),
             "trajectories": host[1].shape[1]}
