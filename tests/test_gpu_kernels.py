@@ -295,3 +295,33 @@ def test_l2_gather_measurement_reads_every_lookup():
     torch.cuda.synchronize()
     assert (sink.cpu().numpy() == 4.0 * lookups).all()
     assert L.bt_l2_gather_bandwidth(field.data_ptr(), 1 << 14, 1 << 14, 1, 1, sink.data_ptr(), None) < 0
+
+
+def test_trajectories_resting_in_one_cell_deposit_large_lane_sums(hip):
+    """Trajectories at rest in a zero field stay in one cell for 20,000 steps: each lane's sum
+    for a node grows to about 2^54 fixed-point units, past the 2^51 of the rounding by
+    addition, so the sums leave the lanes through the 128-bit path.  The charge equals the
+    oracle's within 1e-12 of its total, and does not depend on the launch shape."""
+    args = cases.kernel_case(1, 0)
+    args[1][0] = np.linspace(0.5e-3, 2.5e-3, args[1].shape[1])  # away from the cut column
+    args[2][:] = 0.0
+    args[8][:] = 0.0
+    args[9][:] = 0.0
+    args[20] = 20_000
+    args[27][:] = -1
+    e = hip.trace_exponent(args[0], args[2], args[4], args[3], args[8], args[9], args[11], args[19])
+    expected = cases.copy_args(args)
+    oracle.trace_trajectories_2d(*expected)
+    assert (expected[6] == 4).all() and (expected[5] == 20_000).all()
+    lane_units = np.abs(args[4]).max() * args[19] * 1e-3 * 20_000 / 4 / 2.0**e
+    assert lane_units > 2.0**52
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    want = expected[cases.CHARGE].sum(0)
+    results = []
+    for block in (256, 64):
+        got = _run(hip.trace_trajectories_2d, args, block_size=block)
+        for i in cases.OUTPUTS:
+            assert np.array_equal(got[i], expected[i]), i
+        assert np.abs(got[cases.CHARGE][0] - want).max() <= 1e-12 * np.abs(want).sum()
+        results.append(got[cases.CHARGE])
+    assert np.array_equal(results[0], results[1])
