@@ -16,6 +16,8 @@ trajectories, sized to about ten seconds.
 
 Beside the headline the line carries, under ``extra``:
 
+* ``einzel_full`` -- the same lens with 1e6 trajectories, enough to fill every SIMD (the
+  headline's 1e5 are 1.5 waves per SIMD): the kernel's rate when the GPU is full;
 * ``extraction`` -- the positive-ion extraction (1025 x 257 nodes over 60 x 15 mm, 30 kV,
   H+ and H2+ 0.8:0.2, 1e6 trajectories) traced with Boris through the sampled field of a
   0.2 T current loop (node B), on the field of the initial plasma: particle-steps/s and
@@ -301,6 +303,23 @@ def trace3d_leg(particles: int, steps: int, cpu_sample: int) -> dict:
         out["cpu_oracle"] = {"value": int(sample[5][index].sum()) / cpu_s, "cores": threads,
                              "sample": f"{index.size} of the trajectories"}  # fmt: skip
     return out
+
+
+def full_load_leg(particles: int, steps: int) -> dict:
+    """The headline's einzel lens with enough trajectories to fill every SIMD: the kernel's
+    rate when the GPU is full (1e5 trajectories are 1.5 waves per SIMD)."""
+    design, tracer = build_workload(particles)
+    r = time_traces(kernel_arguments(design, tracer), None, steps, 1, 0)
+    rate = r["steps_per_trace"] / r["launch_s"]
+    return {
+        "workload": f"einzel lens as the headline, {particles} H+ (Sobol spreads), RK4, plain hip trace",
+        "value": rate,
+        "unit": "particle-steps/s",
+        "particle_steps_per_trace": r["steps_per_trace"],
+        "launch_ms": r["launch_s"] * 1e3,
+        "fp64_tflops": FLOP_PER_STEP_RK4_CYL * rate / 1e12,
+        "frac_fp64_peak": FLOP_PER_STEP_RK4_CYL * rate / 1e12 / PEAK_FP64_TFLOPS,
+    }
 
 
 def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
@@ -717,8 +736,10 @@ def main() -> int:
         },
     }
     if not opts.no_extra:
+        log("einzel lens with 1e6 trajectories (the GPU full)")
+        extra = {"einzel_full": full_load_leg(1_000_000, max(2, opts.steps // 3))}
         log("extraction: Boris + node B, 1e6 trajectories")
-        extra = {"extraction": extraction_leg(opts.extraction_particles, max(2, opts.steps // 2), opts.block_size)}
+        extra["extraction"] = extraction_leg(opts.extraction_particles, max(2, opts.steps // 2), opts.block_size)
         log("vlasov: einzel-lens iterations")
         extra["vlasov"] = vlasov_leg(design, 4)
         log(f"batched: {opts.batch_designs} einzel candidates")

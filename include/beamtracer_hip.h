@@ -28,8 +28,10 @@
  * has `float`.  Outputs are mutated in place exactly as the reference kernels do, except
  * that `charge` receives the whole deposit in its row 0 (as the reference's CUDA kernel,
  * kernels/trace_trajectories_2d/trace_trajectories_2d_cuda.cu:1-4), summed exactly in
- * fixed point with least significant bit 2^deposit_exponent (each deposit rounded once to
- * that grid), so the node charges do not depend on the order of the deposits.
+ * fixed point with least significant bit 2^deposit_exponent (a trajectory's deposits into a
+ * node during one stay in its cells are added in FP64 in step order and rounded once to that
+ * grid), so the node charges do not depend on the order of the trajectories or the launch
+ * shape.
  *
  * Return value: 0 on success, a negative BT_E_* code otherwise (never aborts);
  * bt_last_error() describes the last failure of the calling thread.
