@@ -4,6 +4,9 @@
   the hip caller against the same caller on the oracle's kernel, run on every host core --
   the same particle-steps, statuses and electrodes, the plane records and their moments
   within 1e-10 relative, the charge density within 1e-12 of its total.
+* The positive-ion extraction (1025 x 257 nodes, 1e6 H+ and H2+, Boris through the node
+  field of a current loop, the initial-plasma potential): the same bars, with the caller's
+  pauses on cut sub-cells resolved alike.
 * Candidates of the batched sweep drawn from its full distribution (meshes of up to
   1025 x 257 nodes, 1e4 to 2e5 trajectories each): one batched launch equals one plain trace
   per design bit for bit.
@@ -35,6 +38,22 @@ class HostCoresOracleTracer(HipBeamTracer):
         call[31][0] += slabs.sum(0)
 
 
+def _assert_same_trace(got, want, mass):
+    assert got.particle_steps == want.particle_steps
+    np.testing.assert_array_equal(got.status, want.status)
+    np.testing.assert_array_equal(got.electrode, want.electrode)
+    scale = np.abs(want.charge_density).sum()
+    assert np.abs(got.charge_density - want.charge_density).max() <= 1e-12 * scale
+    for a, b in zip(got.planes, want.planes):
+        assert a.point.shape == b.point.shape
+        np.testing.assert_allclose(a.point, b.point, rtol=1e-10, atol=1e-15)
+        np.testing.assert_allclose(a.velocity, b.velocity, rtol=1e-10, atol=1e-9)
+        ma = diagnostics.plane_moments(a, cylindrical=True, mass=mass)
+        mb = diagnostics.plane_moments(b, cylindrical=True, mass=mass)
+        for name in ("current", "rms_size", "rms_angle", "emittance", "mean_energy"):
+            assert getattr(ma, name) == pytest.approx(getattr(mb, name), rel=1e-10), name
+
+
 def test_einzel_lens_full_beam_matches_the_oracle():
     design = designs.einzel_lens(particles=100_000)
     assert design.mesh.shape == (801, 201)
@@ -44,20 +63,23 @@ def test_einzel_lens_full_beam_matches_the_oracle():
               max_steps=design.max_steps)
     got = HipBeamTracer(design.mesh, field, **kw).trace(design.particles)
     want = HostCoresOracleTracer(design.mesh, field, **kw).trace(design.particles)
-    assert got.particle_steps == want.particle_steps > 1.5e8
-    np.testing.assert_array_equal(got.status, want.status)
-    np.testing.assert_array_equal(got.electrode, want.electrode)
-    scale = np.abs(want.charge_density).sum()
-    assert np.abs(got.charge_density - want.charge_density).max() <= 1e-12 * scale
-    mass = design.particles.mass[0]
-    for a, b in zip(got.planes, want.planes):
-        assert a.point.shape == b.point.shape and a.point.shape[1] > 90_000
-        np.testing.assert_allclose(a.point, b.point, rtol=1e-10, atol=1e-15)
-        np.testing.assert_allclose(a.velocity, b.velocity, rtol=1e-10, atol=1e-9)
-        ma = diagnostics.plane_moments(a, cylindrical=True, mass=mass)
-        mb = diagnostics.plane_moments(b, cylindrical=True, mass=mass)
-        for name in ("current", "rms_size", "rms_angle", "emittance", "mean_energy"):
-            assert getattr(ma, name) == pytest.approx(getattr(mb, name), rel=1e-10), name
+    assert got.particle_steps > 1.5e8 and want.planes[0].point.shape[1] > 90_000
+    _assert_same_trace(got, want, design.particles.mass[0])
+
+
+def test_extraction_full_beam_matches_the_oracle():
+    design = designs.extraction(particles=1_000_000)
+    assert design.mesh.shape == (1025, 257)
+    plasma = design.plasma
+    phi = poisson.IonOpticsPoisson(design.mesh).solve(
+        None, fixed_region=lambda x, y: x < plasma.initial_plasma_x, fixed_potential=plasma.plasma_potential)
+    field = fields.StaggeredElectricField(design.mesh, phi)
+    kw = dict(planes=design.planes, integrator="boris")
+    got = HipBeamTracer(design.mesh, field, design.magnetic, **kw).trace(design.particles)
+    want = HostCoresOracleTracer(design.mesh, field, design.magnetic, **kw).trace(design.particles)
+    assert got.particle_steps > 1e9
+    assert (want.status == 0).sum() > 500_000 and (want.status == 3).sum() > 100_000
+    _assert_same_trace(got, want, design.particles.mass[0])
 
 
 def test_batched_candidates_of_the_full_sweep_equal_plain_traces():
