@@ -306,6 +306,8 @@ int check_designs(const bt_batch_design* designs, i64 n_designs, i64 n_fxs, i64 
         if (b.n_nodes != b.nx * b.ny || b.node_offset < 0) return fail(BT_E_ARG, at + "n_nodes must be nx * ny");
         if (b.node_offset + b.n_nodes >= kMax32 || b.n_subcells >= kMax32)
             return fail(BT_E_ARG, at + "node and sub-cell counts must stay below 2^31");
+        if (b.cells_offset + b.cells_rows * b.cells_cols >= kMax32 || b.subcells_offset + b.n_subcells >= kMax32)
+            return fail(BT_E_ARG, at + "cell and sub-cell arena offsets must stay below 2^31");
         if (b.node_offset + b.n_nodes > total) total = b.node_offset + b.n_nodes;
         if (b.n_nodes > most) most = b.n_nodes;
     }

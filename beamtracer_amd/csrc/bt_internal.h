@@ -39,6 +39,10 @@ struct TraceParams {
     int cell_imax, cell_jmax;  // min(nx - 2, cells_rows - 1), min(ny - 2, cells_cols - 1)
     const i64* subcells; int n_subcells; int refine;
     const double* planes; int n_planes;
+    // element offsets of the lane's design in cells, subcells and planes (a batched launch;
+    // 0 for one mesh): 32-bit offsets from the launch's arrays instead of a 64-bit pointer
+    // per lane keep the batched loop's registers down
+    int cells_base, subcells_base, planes_base;
     double* hits; int capacity; i64* counts;
     const i64* path_slot; double* paths; int n_paths; int path_capacity; i64* path_count;
     int graded; const double* x_nodes; i64 n_x_nodes;
@@ -87,6 +91,10 @@ struct AdjointParams {
     double origin_x, origin_y, h, step_fraction;
     double inv_h;  // RN(1 / h): divisions by the spacing as products
     const double* planes; int n_planes;
+    // element offsets of the lane's design in cells, subcells and planes (a batched launch;
+    // 0 for one mesh): 32-bit offsets from the launch's arrays instead of a 64-bit pointer
+    // per lane keep the batched loop's registers down
+    int cells_base, subcells_base, planes_base;
     const double* hits; i64 capacity; const i64* counts; const double* seeds;
     i64 interval, n_ckpt, lanes;
     const double* fwd_ckpt; i64 fwd_n_ckpt;  // the forward trace's checkpoints [c][6][n], or null
@@ -128,6 +136,10 @@ struct Trace3DParams {
     const i64* cells; int cells_n1, cells_n2;
     const i64* subcells; i64 n_subcells; int refine;
     const double* planes; int n_planes;
+    // element offsets of the lane's design in cells, subcells and planes (a batched launch;
+    // 0 for one mesh): 32-bit offsets from the launch's arrays instead of a 64-bit pointer
+    // per lane keep the batched loop's registers down
+    int cells_base, subcells_base, planes_base;
     double* hits; int capacity; i64* counts;
     const i64* path_slot; double* paths; i64 n_paths; int path_capacity; i64* path_count;
     int has_j; double inv_c2;

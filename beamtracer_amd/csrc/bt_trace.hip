@@ -644,7 +644,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
 
     // plane crossings (pyccel.py:426-442)
     for (int pl = 0; pl < p.n_planes; ++pl) {
-        const double xp = p.planes[pl];
+        const double xp = p.planes[p.planes_base + pl];
         const double before = x0 - xp;
         const double after = y0 - xp;
         if ((before < 0.0) != (after < 0.0)) {
@@ -706,13 +706,13 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         // same integer for an integer bound)
         const int i = clamped_floor_index(ci, p.cell_imax);
         const int j = clamped_floor_index(cj, p.cell_jmax);
-        const i64 cell = p.cells[(unsigned)i * (unsigned)p.cells_cols + (unsigned)j];
+        const i64 cell = p.cells[(unsigned)p.cells_base + (unsigned)i * (unsigned)p.cells_cols + (unsigned)j];
         if (cell >= 0) {
             const double R = (double)p.refine;
             const int si = (int)fmin(fmax((ci - (double)i) * R, 0.0), R - 1.0);
             const int sj = (int)fmin(fmax((cj - (double)j) * R, 0.0), R - 1.0);
             const i64 at = (cell * p.refine + si) * p.refine + sj;
-            const i64 e = at < p.n_subcells ? p.subcells[at] : -1;
+            const i64 e = at < p.n_subcells ? p.subcells[p.subcells_base + at] : -1;
             if (e >= 0) {
                 st = 3;
                 p.electrode[s.k] = e;
@@ -745,11 +745,11 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
     lp.inv_bh = 1.0 / b.h; lp.inv_h = lp.inv_bh;
     lp.bofx = 0; lp.bofy = 0; lp.bofn = (int)b.node_offset;
     lp.fx_base = (int)b.fx_offset; lp.fy_base = (int)b.fy_offset;
-    lp.cells = p.cells + b.cells_offset; lp.cells_rows = (int)b.cells_rows; lp.cells_cols = (int)b.cells_cols;
+    lp.cells_base = (int)b.cells_offset; lp.cells_rows = (int)b.cells_rows; lp.cells_cols = (int)b.cells_cols;
     lp.cell_imax = (int)(b.nx - 2 < b.cells_rows - 1 ? b.nx - 2 : b.cells_rows - 1);
     lp.cell_jmax = (int)(b.ny - 2 < b.cells_cols - 1 ? b.ny - 2 : b.cells_cols - 1);
-    lp.subcells = p.subcells + b.subcells_offset; lp.n_subcells = (int)b.n_subcells;
-    lp.planes = p.planes + (i64)d * p.n_planes;
+    lp.subcells_base = (int)b.subcells_offset; lp.n_subcells = (int)b.n_subcells;
+    lp.planes_base = d * p.n_planes;
     lp.deposit_exponent = b.deposit_exponent;
     lp.design = d;
 }
