@@ -99,25 +99,28 @@ struct Radial {
     double r, cy, cz;
 };
 
+// The common case is computed without a branch (the GPU does not trap on the reciprocal
+// square root of 0 or a NaN), and one rarely taken branch replaces it near the axis.
 __device__ __forceinline__ Radial radial(double y, double z) {
     const double s2 = __builtin_fma(y, y, z * z);
     Radial o;
-    if (s2 >= 0x1p-1000) {
-        const double ir = rsqrt_nr(s2);
-        o.r = s2 * ir;
-        o.cy = y * ir;
-        o.cz = z * ir;
-    } else if (s2 > 0.0) {  // within 1e-150 of the axis: the same with y, z scaled by 2^600
-        const double ys = y * 0x1p600, zs = z * 0x1p600;
-        const double s2s = __builtin_fma(ys, ys, zs * zs);
-        const double ir = rsqrt_nr(s2s);
-        o.r = (s2s * ir) * 0x1p-600;
-        o.cy = ys * ir;
-        o.cz = zs * ir;
-    } else {  // on the axis (or a NaN coordinate, which stays NaN in r)
-        o.r = s2;
-        o.cy = 0.0;
-        o.cz = 0.0;
+    const double ir = rsqrt_nr(s2);
+    o.r = s2 * ir;
+    o.cy = y * ir;
+    o.cz = z * ir;
+    if (!(s2 >= 0x1p-1000)) {
+        if (s2 > 0.0) {  // within 1e-150 of the axis: the same with y, z scaled by 2^600
+            const double ys = y * 0x1p600, zs = z * 0x1p600;
+            const double s2s = __builtin_fma(ys, ys, zs * zs);
+            const double irs = rsqrt_nr(s2s);
+            o.r = (s2s * irs) * 0x1p-600;
+            o.cy = ys * irs;
+            o.cz = zs * irs;
+        } else {  // on the axis (or a NaN coordinate, which stays NaN in r)
+            o.r = s2;
+            o.cy = 0.0;
+            o.cz = 0.0;
+        }
     }
     return o;
 }
@@ -133,6 +136,23 @@ struct StepRule {
 
 __device__ __forceinline__ StepRule step_rule(double v2, double a2, double h, double f, double gv) {
     StepRule o;
+    // the common case (|v| and |a| of normal size) without a branch: the same operations as
+    // the general rule below on these inputs, so the same bits
+    {
+        const double iv = rsqrt_nr(v2);
+        const double ia = rsqrt_nr(a2);
+        const double q = (2.0 * h) * ia;
+        const double d2 = q * rsqrt_nr(q);
+        const double d1 = (h * iv) / gv;
+        if (v2 >= 0x1p-1000 && a2 >= 0x1p-1000 && q >= 0x1p-1000) {
+            o.speed = gv * (v2 * iv);
+            o.accel = a2 * ia;
+            const bool second = d2 < d1;
+            o.branch = second ? 2 : 1;
+            o.dt = f * (second ? d2 : d1);
+            return o;
+        }
+    }
     double dt = -1.0;
     o.branch = 0;
     o.speed = 0.0;
