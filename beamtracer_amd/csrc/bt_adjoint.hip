@@ -129,6 +129,25 @@ __device__ __forceinline__ int wave_min(int v) { return wave_reduce<false>(v); }
 
 __device__ __forceinline__ int wave_max(int v) { return wave_reduce<true>(v); }
 
+// The same over two 16-bit halves at once (v_pk_min_i16 / v_pk_max_i16): the minima or the
+// maxima of two quantities below 2^15 in one reduction.
+typedef short Short2 __attribute__((ext_vector_type(2)));
+
+template <bool MAX>
+__device__ __forceinline__ int wave_reduce_pk(int v) {
+    const auto op = [](int a, int b) {
+        const Short2 x = __builtin_bit_cast(Short2, a), y = __builtin_bit_cast(Short2, b);
+        return __builtin_bit_cast(int, MAX ? __builtin_elementwise_max(x, y) : __builtin_elementwise_min(x, y));
+    };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 template <int A>
 __device__ __forceinline__ void add_node(double* __restrict__ grad, i64 cols, const GradWindow& W, int ii, int jj,
                                          double v) {
@@ -228,8 +247,17 @@ __device__ __forceinline__ void window_flush_all(const AdjointParams& p, GradWin
 // Moves the window (wave-uniform; every lane of the wave calls it) to hold the cells
 // (ci, cj) of the active lanes with room below them, the direction of the backward walk.
 __device__ __forceinline__ void window_follow(const AdjointParams& p, GradWindow& W, bool act, int ci, int cj) {
-    const int imin = wave_min(act ? ci : 0x7fffffff), imax = wave_max(act ? ci : -1);
-    const int jmin = wave_min(act ? cj : 0x7fffffff), jmax = wave_max(act ? cj : -1);
+    int imin, imax, jmin, jmax;
+    if (p.nx < 32768 && p.ny < 32768) {  // cells below 2^15 on both axes: (ci, cj) in one word
+        const int both = (ci & 0xffff) | (cj << 16);
+        const int mn = wave_reduce_pk<false>(act ? both : 0x7fff7fff);
+        const int mx = wave_reduce_pk<true>(act ? both : -1);  // -1: (-1, -1)
+        imin = (short)(mn & 0xffff); jmin = mn >> 16;
+        imax = (short)(mx & 0xffff); jmax = mx >> 16;
+    } else {
+        imin = wave_min(act ? ci : 0x7fffffff); imax = wave_max(act ? ci : -1);
+        jmin = wave_min(act ? cj : 0x7fffffff); jmax = wave_max(act ? cj : -1);
+    }
     if (imax < 0) return;
     const bool rows_ok = W.lo >= 0 && jmin - 1 >= W.base && jmax + 2 < W.base + kWinRows;
     const bool cols_ok = W.lo >= 0 && imin - 2 >= W.lo && imax + 2 <= W.lo + kWinCols - 1;
