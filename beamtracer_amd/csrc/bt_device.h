@@ -3,9 +3,10 @@
 // * the Lorentz factor, graded-axis location and patch search of the reference tracer
 //   (kernels/trace_trajectories_2d/trace_trajectories_2d_pyccel.py:19-39, 165-171, 215-222);
 // * the clamped staggered bilinear gather (staggered_field_2d_pyccel.py:36-55);
-// * 128-bit fixed-point accumulation: a deposit is rounded once to a multiple of 2^e and
-//   added as an exact integer, so a node's total does not depend on the order of the
-//   atomic adds (the reference adds FP64 in an order set by its threads).
+// * 128-bit fixed-point accumulation: a value (a deposit, or one trajectory's FP64 sum of
+//   its deposits into a node) is rounded once to a multiple of 2^e and added as an exact
+//   integer, so a node's total does not depend on the order of the atomic adds (the
+//   reference adds FP64 in an order set by its threads).
 #pragma once
 
 #include <hip/hip_runtime.h>
