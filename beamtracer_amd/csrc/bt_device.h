@@ -109,7 +109,7 @@ __device__ __forceinline__ Radial radial(double y, double z) {
     o.r = s2 * ir;
     o.cy = y * ir;
     o.cz = z * ir;
-    if (!(s2 >= 0x1p-1000)) {
+    if (__builtin_expect(!(s2 >= 0x1p-1000), 0)) {
         if (s2 > 0.0) {  // within 1e-150 of the axis: the same with y, z scaled by 2^600
             const double ys = y * 0x1p600, zs = z * 0x1p600;
             const double s2s = __builtin_fma(ys, ys, zs * zs);
@@ -145,7 +145,7 @@ __device__ __forceinline__ StepRule step_rule(double v2, double a2, double h, do
         const double q = (2.0 * h) * ia;
         const double d2 = q * rsqrt_nr(q);
         const double d1 = (h * iv) / gv;
-        if (v2 >= 0x1p-1000 && a2 >= 0x1p-1000 && q >= 0x1p-1000) {
+        if (__builtin_expect(v2 >= 0x1p-1000 && a2 >= 0x1p-1000 && q >= 0x1p-1000, 1)) {
             o.speed = gv * (v2 * iv);
             o.accel = a2 * ia;
             const bool second = d2 < d1;

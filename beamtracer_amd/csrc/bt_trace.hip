@@ -318,7 +318,7 @@ template <bool USE_WIN, int WC>
 __device__ __forceinline__ void sum_leaves(DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
                                            i64 ny, int design, int ii, int jj, double sum, int e) {
     const double units = ldexp(sum, -e);
-    if (fabs(units) < 0x1p51) {
+    if (__builtin_expect(fabs(units) < 0x1p51, 1)) {
         const i64 q = exact_i64(units);
         if (q != 0) node_add<USE_WIN>(win, wv, acc, ofn, ny, design, ii, jj, q);
         return;
@@ -380,7 +380,7 @@ template <bool USE_WIN, int WC>
 __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc,
                                          i64 ofn, i64 ny, bool in_base, int design, int i, int j,
                                          double u, double w, double deposit, int e) {
-    if (!(fabs(deposit) < __builtin_inf())) return;  // (the four values would not be finite)
+    if (__builtin_expect(!(fabs(deposit) < __builtin_inf()), 0)) return;  // (the four values would not be finite)
     if (in_base) cache_move<USE_WIN>(s, win, wv, acc, ofn, ny, design, i, j, e);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
@@ -647,7 +647,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         const double xp = p.planes[p.planes_base + pl];
         const double before = x0 - xp;
         const double after = y0 - xp;
-        if ((before < 0.0) != (after < 0.0)) {
+        if (__builtin_expect((before < 0.0) != (after < 0.0), 0)) {
             const double sfrac = before / (before - after);
             const i64 idx = s.k * p.n_planes + pl;
             const i64 c = p.counts[idx];
