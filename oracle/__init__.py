@@ -54,6 +54,47 @@ def build(force: bool = True) -> Path:
     return LIB_PATH
 
 
+#: the mistakes that ``oracle.c`` can be built with (its ``ORACLE_MUTANT_*`` switches): each
+#: flips one sign that the pins in ``tests/test_oracle_pins.py`` must catch
+MUTANTS = {
+    "radial_b": "-DORACLE_MUTANT_BR=-1.0",
+    "azimuthal_b": "-DORACLE_MUTANT_BTHETA=-1.0",
+    "mirror_b": "-DORACLE_MUTANT_MIRROR_B=1",
+    "midpoint": "-DORACLE_MUTANT_MIDPOINT=-1.0",
+}
+
+
+def build_mutant(name: str, directory: Path) -> Path:
+    """Compile ``oracle.c`` with the mistake ``MUTANTS[name]`` into ``directory`` (tests only)."""
+    out = Path(directory) / f"liboracle_{name}.so"
+    cmd = [
+        "gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+        MUTANTS[name], "-o", str(out), str(HERE / "oracle.c"), str(HERE / "tangent.c"), "-lm",
+    ]  # fmt: skip
+    subprocess.run(cmd, check=True)
+    return out
+
+
+class using:
+    """``with oracle.using(path):`` routes the module's calls to the library at ``path``."""
+
+    def __init__(self, path: Path):
+        self.path = Path(path)
+
+    def __enter__(self):
+        global _LIB
+        self.saved = _LIB
+        mutant = ctypes.CDLL(str(self.path))
+        mutant.oracle_trace_args_size.restype = _i64
+        _LIB = mutant
+        return mutant
+
+    def __exit__(self, *exc):
+        global _LIB
+        _LIB = self.saved
+        return False
+
+
 class TraceArgs(ctypes.Structure):
     """ctypes mirror of ``oracle_trace_args`` in ``oracle.c``."""
 

@@ -24,8 +24,16 @@
  * Parity pins (tests/test_oracle_*.py): closed forms (bilinear fields reproduced exactly,
  * constant fields, charge conservation, uniform-field energy gain and flight time, RK4
  * order 4 and Boris order 2 on the harmonic well, |v| kept by Boris in a pure B,
- * Busch's theorem), brute force on tiny grids, and an independent step-synchronous
- * NumPy restatement of solver/tracer.py:851-1054 (oracle/numpy_ref.py).
+ * Busch's theorem through a solenoid, the force of a uniform radial and azimuthal B,
+ * exact mirror images below a symmetry plane, the Hermite midpoint of the deposit
+ * under uniform acceleration), brute force on tiny grids, and an independent
+ * step-synchronous NumPy restatement of solver/tracer.py:851-1054 (oracle/numpy_ref.py).
+ *
+ * Mutation switches (tests only): ORACLE_MUTANT_BR = -1 flips the radial B, _BTHETA = -1
+ * the azimuthal B, _MIRROR_B = 1 drops the sign of B_x below a symmetry plane and
+ * _MIDPOINT = -1 flips the dt (v - w) / 8 term of the deposit point.  tests/ builds such
+ * mutants to show that the pins above fail on each of these mistakes; the default build
+ * is the reference's arithmetic.
  * The relativistic Lorentz factor (mpiparticles, un-vendored) is written from its
  * equation gamma = sqrt(1 + u.u / c^2): bit-level parity with mpiparticles is unpinned;
  * for inv_c2 = 0 it is exactly 1.
@@ -38,6 +46,19 @@
 #endif
 
 typedef int64_t i64;
+
+#ifndef ORACLE_MUTANT_BR
+#define ORACLE_MUTANT_BR 1.0
+#endif
+#ifndef ORACLE_MUTANT_BTHETA
+#define ORACLE_MUTANT_BTHETA 1.0
+#endif
+#ifndef ORACLE_MUTANT_MIRROR_B
+#define ORACLE_MUTANT_MIRROR_B 0
+#endif
+#ifndef ORACLE_MUTANT_MIDPOINT
+#define ORACLE_MUTANT_MIDPOINT 1.0
+#endif
 
 /* gamma of the proper velocity u (mpiparticles lorentz_factor; equation of
  * solver/tracer.py:464-466): exactly 1 when inv_c2 == 0. */
@@ -178,10 +199,10 @@ static void magnetic_at(const oracle_trace_args* a, double lq, double qy, double
     double c2 = node_bilinear(a->b2, sx, sy, nx, ny);
     if (a->cylindrical == 1) {
         *B0 = c0;
-        *B1 = c1 * cy - c2 * cz;
-        *B2 = c1 * cz + c2 * cy;
+        *B1 = ORACLE_MUTANT_BR * c1 * cy - ORACLE_MUTANT_BTHETA * c2 * cz;
+        *B2 = ORACLE_MUTANT_BR * c1 * cz + ORACLE_MUTANT_BTHETA * c2 * cy;
     } else if (a->cylindrical == 2) {
-        *B0 = cy * c0;   /* cy is the mirror sign s */
+        *B0 = ORACLE_MUTANT_MIRROR_B ? c0 : cy * c0;   /* cy is the mirror sign s */
         *B1 = c1;
         *B2 = cy * c2;
     } else {
@@ -413,9 +434,9 @@ static void trace_one(const oracle_trace_args* a, i64 k, double* charge_row) {
 
         /* space charge at the Hermite midpoint (pyccel.py:372-423) */
         double gw = 1.0 / lorentz_factor(w0, w1, w2, inv_c2);
-        double m0 = 0.5 * (x0 + y0) + dt * (gv * v0 - gw * w0) / 8.0;
-        double m1 = 0.5 * (x1 + y1) + dt * (gv * v1 - gw * w1) / 8.0;
-        double m2 = 0.5 * (x2 + y2) + dt * (gv * v2 - gw * w2) / 8.0;
+        double m0 = 0.5 * (x0 + y0) + ORACLE_MUTANT_MIDPOINT * dt * (gv * v0 - gw * w0) / 8.0;
+        double m1 = 0.5 * (x1 + y1) + ORACLE_MUTANT_MIDPOINT * dt * (gv * v1 - gw * w1) / 8.0;
+        double m2 = 0.5 * (x2 + y2) + ORACLE_MUTANT_MIDPOINT * dt * (gv * v2 - gw * w2) / 8.0;
         if (a->cylindrical == 1) m1 = hypot(m1, m2);
         else if (a->cylindrical == 2) m1 = a->origin_y + fabs(m1 - a->origin_y);
         if (a->graded != 0) m0 = uniform_x(m0, a->x_nodes, a->n_x_nodes, a->origin_x, a->h);

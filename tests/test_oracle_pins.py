@@ -1,0 +1,210 @@
+"""Pins of the oracle's magnetic-field orientation and of its deposit point (CPU).
+
+Each pin is a closed form or an exact symmetry, and each is shown to fail on the oracle
+built with the one sign mistake it guards against (``oracle.MUTANTS``):
+
+* Busch's theorem through a solenoid (``tests/unit/test_ion_optics.py:221-252``):
+  ``r^2 (omega + q B / 2m)`` is conserved from the fringe field to the plane; a flipped
+  radial B gives the ring the opposite azimuthal velocity.
+* The force of a uniform radial or azimuthal B on an axial velocity is
+  ``q v_x B_r theta`` and ``-q v_x B_theta r`` (``theta = x cross r``,
+  ``solver/tracer.py:70-83``).
+* Below a symmetry plane the field is mirrored, ``E = (E_x, s E_y, 0)`` and
+  ``B = (s B_x, B_y, s B_z)`` (``solver/tracer.py:547-570``): a trajectory started at the
+  mirror image of another stays its exact mirror image, bit for bit.
+* Under uniform acceleration both integrators are exact and the deposit point
+  ``(x + y) / 2 + dt (v - w) / 8`` is the position at the middle of the step
+  (``solver/tracer.py:1308-1323``): the deposit's first moment along x (planar: linear
+  weights) and its ``r^2`` moment (cylindrical: area weights) equal
+  ``dq sum_k dt_k x(t_k + dt_k / 2)`` and ``dq sum_k dt_k r(t_k + dt_k / 2)^2``.
+* The off-axis series of ``AxisymmetricMagneticField`` converges with order 2 in the
+  sample spacing on the reference's own problem (``solver/fields.py:169-183``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from test_oracle_tracer import make_case, run, uniform
+
+import oracle
+from beamtracer_amd.fields import AxisymmetricMagneticField, solenoid_on_axis
+
+E = 1.602176634e-19
+M_P = 1.67262192369e-27
+
+
+@pytest.fixture(scope="module")
+def mutants(tmp_path_factory):
+    directory = tmp_path_factory.mktemp("oracle_mutants")
+    return {name: oracle.build_mutant(name, directory) for name in oracle.MUTANTS}
+
+
+def _node_field(nx, ny, h, magnet):
+    xn, yn = np.meshgrid(np.arange(nx) * h, np.arange(ny) * h, indexing="ij")
+    b_axial, b_radial = magnet(xn.ravel(), yn.ravel())
+    return b_axial.reshape(nx, ny), b_radial.reshape(nx, ny), None
+
+
+def _busch_defect(integrator: int) -> float:
+    """|canonical angular momentum at the plane - at the start| / its scale, for one proton."""
+    h, nx, ny = 1e-3, 601, 41
+    z = np.arange(-0.1, 0.7, 0.5e-3)
+    magnet = AxisymmetricMagneticField(z[0], 0.5e-3, solenoid_on_axis(z, 0.3, 0.2, 0.05, 0.1))
+    fx, fy = uniform(nx, ny)
+    v0 = np.sqrt(2 * 1e4 * E / M_P)
+    args = make_case(fx, fy, (0.0, 0.0), h, 1, integrator, [0.0, 5e-3, 0.0], [v0, 0.0, 0.0], qm=E / M_P,
+                     dq=1e-9, planes=[0.29], step_fraction=0.25)  # fmt: skip
+    out = run(args, magnetic=_node_field(nx, ny, h, magnet))
+    assert out[26][0, 0] == 1
+    rec = out[25][0, 0, 0]
+    point, vel = rec[0:3], rec[3:6]
+    r = np.hypot(point[1], point[2])
+    omega = (point[1] * vel[2] - point[2] * vel[1]) / r**2
+    b_here = magnet(np.array([0.29]), np.array([r]))[0][0]
+    canonical = r**2 * (omega + E * b_here / (2 * M_P))
+    b_start = magnet(np.array([0.0]), np.array([5e-3]))[0][0]
+    canonical_start = 25e-6 * E * b_start / (2 * M_P)
+    scale = E * b_here / (2 * M_P) * r**2
+    assert abs(omega) * r**2 > 0.5 * scale  # the ring turns: the pin is not vacuous
+    return abs(canonical - canonical_start) / scale
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_busch_theorem_through_a_solenoid(integrator, mutants):
+    assert _busch_defect(integrator) < 1e-3
+    with oracle.using(mutants["radial_b"]):
+        assert _busch_defect(integrator) > 0.5
+
+
+def _kick(integrator: int, component: int, phi: float) -> np.ndarray:
+    """The velocity change over one step, per (q/m v_x B dt), of a proton moving along x
+    at angle phi about the axis in a uniform (B_x, B_r, B_theta) with one component set."""
+    nx, ny, h = 21, 11, 1e-3
+    fx, fy = uniform(nx, ny)
+    b = [np.zeros((nx, ny)) for _ in range(3)]
+    b[component][:] = 1e-3
+    r0, vx, qm = 4e-3, 1e5, E / M_P
+    x = [5e-3, r0 * np.cos(phi), r0 * np.sin(phi)]
+    out = run(make_case(fx, fy, (0.0, 0.0), h, 1, integrator, x, [vx, 0.0, 0.0], qm=qm, max_steps=1),
+              magnetic=tuple(b))  # fmt: skip
+    assert out[5][0] == 1
+    dt = 0.5 * h / vx  # the first step: dt = f h / |v|
+    return (out[2][:, 0] - np.array([vx, 0.0, 0.0])) / (qm * vx * 1e-3 * dt)
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+@pytest.mark.parametrize("phi", [0.0, 0.7, 2.5, -1.9])
+def test_radial_and_azimuthal_b_push_along_theta_and_minus_r(integrator, phi, mutants):
+    r_hat = np.array([0.0, np.cos(phi), np.sin(phi)])
+    theta_hat = np.array([0.0, -np.sin(phi), np.cos(phi)])  # x cross r
+    np.testing.assert_allclose(_kick(integrator, 1, phi), theta_hat, atol=2e-3)
+    np.testing.assert_allclose(_kick(integrator, 2, phi), -r_hat, atol=2e-3)
+    np.testing.assert_allclose(_kick(integrator, 0, phi), 0.0, atol=1e-12)  # B_x is parallel to v
+    with oracle.using(mutants["radial_b"]):
+        assert np.abs(_kick(integrator, 1, phi) - theta_hat).max() > 1.0
+    with oracle.using(mutants["azimuthal_b"]):
+        assert np.abs(_kick(integrator, 2, phi) + r_hat).max() > 1.0
+
+
+def _mirror_pair(integrator: int):
+    nx, ny, h = 41, 21, 1e-3
+    i = np.arange(nx + 1)[:, None]
+    j = np.arange(ny)[None, :]
+    fx = 3e5 + 2e5 * np.sin(0.3 * i + 0.2 * j)
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny + 1)[None, :]
+    fy = 2e5 * np.cos(0.21 * i) * np.sin(0.4 * j)
+    xi = np.arange(nx)[:, None] / (nx - 1)
+    yj = np.arange(ny)[None, :] / (ny - 1)
+    magnetic = (0.15 + 0.1 * np.cos(4 * xi) * yj, 0.125 * np.sin(5 * xi) * (1 + yj), -0.1 + 0.15 * xi * yj)
+    x = np.array([[1e-3, 1e-3], [4e-3, -4e-3], [1e-3, 1e-3]])
+    v = np.array([[2e5, 2e5], [3e4, -3e4], [-2e4, -2e4]])
+    slots = np.array([0, 1], dtype=np.int64)
+    args = make_case(fx, fy, (0.0, 0.0), h, 2, integrator, x, v, qm=E / M_P, dq=1e-9, planes=[20e-3],
+                     path_slot=slots, path_capacity=400)  # fmt: skip
+    out = run(args, magnetic=magnetic)
+    plain = run(make_case(fx, fy, (0.0, 0.0), h, 2, integrator, x, v, qm=E / M_P, dq=1e-9, planes=[20e-3]))
+    return out, plain
+
+
+def _mirror_defect(out) -> float:
+    x, v, hits = out[1], out[2], out[25]
+    if out[5][0] != out[5][1] or out[6][0] != out[6][1]:
+        return np.inf  # not even the same number of steps
+    sign = np.array([1.0, -1.0, 1.0])
+    worst = max(np.abs(x[:, 0] - sign * x[:, 1]).max() / 1e-3, np.abs(v[:, 0] - sign * v[:, 1]).max() / 1e5)
+    rec = hits[:, 0, 0]
+    worst = max(worst, np.abs(rec[0, 1:3] - sign[1:] * rec[1, 1:3]).max() / 1e-3)
+    n = int(min(out[29][0], 400))
+    worst = max(worst, np.abs(out[28][0, :n] - out[28][1, :n]).max() / 1e-3)  # paths record (x, |y|)
+    return worst
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_a_mirrored_trajectory_is_the_exact_mirror_image(integrator, mutants):
+    out, plain = _mirror_pair(integrator)
+    assert out[6][0] == 0 and out[26][0, 0] == 1  # transmitted through the plane
+    assert _mirror_defect(out) == 0.0  # bit for bit
+    assert np.abs(out[1][:, 0] - plain[1][:, 0]).max() > 1e-4  # the magnetic field matters
+    with oracle.using(mutants["mirror_b"]):
+        bad, _ = _mirror_pair(integrator)
+        assert _mirror_defect(bad) > 1e-3
+
+
+def _midpoint_moment(geometry: int, integrator: int) -> tuple[float, float]:
+    """(deposited moment, closed form) for a proton accelerated from rest along x (planar,
+    first moment in x) or along r (cylindrical, r^2 moment)."""
+    nx, ny, h, qm, dq = 81, 81, 1e-4, E / M_P, 1e-9
+    field = 2e4
+    a = qm * field
+    start = np.array([1.5e-4, 0.55e-3, 0.0])
+    axis = 0 if geometry == 0 else 1
+    fx, fy = uniform(nx, ny, ex=field if axis == 0 else 0.0, ey=field if axis == 1 else 0.0)
+    steps = 40
+    args = make_case(fx, fy, (0.0, 0.0), h, geometry, integrator, start, [0.0, 0.0, 0.0], qm=qm, dq=dq,
+                     max_steps=steps, path_slot=np.array([0], dtype=np.int64), path_capacity=steps + 1)  # fmt: skip
+    args[28][0, 0] = start[:2]
+    args[29][0] = 1  # the caller stores the start point
+    out = run(args)
+    assert out[6][0] == 4 and out[5][0] == steps
+    s = out[28][0, : steps + 1, axis] - start[axis]  # displacement at each step end
+    t = np.sqrt(2.0 * s / a)
+    assert np.all(np.diff(t) > 0)
+    mid = start[axis] + 0.5 * a * (0.5 * (t[:-1] + t[1:])) ** 2
+    xn, yn = np.meshgrid(np.arange(nx) * h, np.arange(ny) * h, indexing="ij")
+    charge = out[31].sum(0).reshape(nx, ny)
+    assert charge[-1].sum() == 0 and charge[:, -1].sum() == 0  # nothing clamped to the far edges
+    if geometry == 0:
+        return float((charge * xn).sum()), float(dq * (np.diff(t) * mid).sum())
+    return float((charge * yn**2).sum()), float(dq * (np.diff(t) * mid**2).sum())
+
+
+@pytest.mark.parametrize("geometry", [0, 1])
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_the_deposit_point_is_the_middle_of_the_step(geometry, integrator, mutants):
+    got, want = _midpoint_moment(geometry, integrator)
+    assert got == pytest.approx(want, rel=1e-10)
+    with oracle.using(mutants["midpoint"]):
+        got, want = _midpoint_moment(geometry, integrator)
+        assert abs(got - want) > 1e-5 * abs(want)
+
+
+def _series_error(hz: float) -> float:
+    z = np.linspace(-1.0, 1.0, 41) + 0.0123
+    r = 0.2
+    g = np.exp(-(z**2))
+    bz = g - r**2 / 4.0 * (4.0 * z**2 - 2.0) * g
+    br = -r / 2.0 * (-2.0 * z * g) + r**3 / 16.0 * (-8.0 * z**3 + 12.0 * z) * g
+    samples = np.arange(-5.0, 5.0 + 0.5 * hz, hz)
+    field = AxisymmetricMagneticField(-5.0, hz, np.exp(-(samples**2)), order=3)
+    got_z, got_r = field(z, np.full_like(z, r))
+    return float(max(np.max(np.abs(got_z - bz)), np.max(np.abs(got_r - br))))
+
+
+def test_off_axis_magnetic_series_converges_with_order_two():
+    steps = np.array([0.2, 0.1, 0.05, 0.025])
+    errors = np.array([_series_error(hz) for hz in steps])
+    order = np.polyfit(np.log(steps), np.log(errors), 1)[0]  # the least-squares slope
+    assert errors[-1] < errors[0] / 10 and errors[-1] < 1e-3
+    assert 1.8 < order < 2.4, order
