@@ -201,104 +201,56 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 }
 
 // ---- the deposit window ---------------------------------------------------------------
-// The workgroup's deposits go into a window of node sums in LDS that follows its beam:
-// columns lo..hi (at most WC, 16 or 128; column i in slot i mod WC) of one design's mesh,
-// rows base..base + kWinRows - 1.  The placement is workgroup-uniform for a round and held
-// in registers, so a lane tests a node against it with compares and adds with an LDS
-// integer atomic; a node outside goes to global memory directly.  At the end of each round
-// of kRoundSteps steps every wave notes the cells its lanes' caches hold, and the workgroup
-// places the window around the noted cells of its lowest design (with room for kRoundSteps
-// steps of travel), flushes the slots of the columns it leaves (one global atomic per word
-// of each node; every slot when the base row or the design changes, which a margin keeps
-// rare) and writes the next round's placement.  Deposits and the maintenance are separated
-// by barriers and the placement is double-buffered (round k reads table k mod 2, the
-// maintenance after round k writes table k + 1 mod 2), so no deposit lands in a slot that
-// is being flushed.
+// Each wave sums its lanes' deposits in its own window of node sums in LDS that follows it:
+// columns lo .. lo + WC - 1 (column c in slot c mod WC) and rows base .. base + 63 of one
+// design's mesh, one signed 64-bit word per node.  The placement is wave-uniform (scalar
+// registers), so a flush from a lane tests the node with two unsigned compares and adds with
+// a no-return LDS atomic; a node outside goes to global memory directly.  There is no barrier
+// anywhere in the step loop: only the wave itself reads, adds to and moves its window, and a
+// wave's LDS operations execute in program order.  After each round of kRoundSteps steps the
+// wave reads the cache cells of its first and last lanes that hold one (its lanes took
+// consecutive queue entries: neighbours at the emitter) and, when they leave the window with
+// room for a round of travel, moves it: the columns that leave are flushed (lane l adds row
+// l), and the whole window when the rows or the design change (rare: the rows sit centred on
+// the lanes) and at least every kFlushRounds rounds.
 
 constexpr int kWinCols = 16;   // the columns of a window (a wide window: kWideCols)
-constexpr int kWideCols = 128;
-constexpr int kWinRows = 64;
-constexpr int kRoundSteps = 2;        // steps between window moves
-constexpr int kFlushRounds = 16;      // every slot is emptied at least this often
-constexpr int kColumnBits = 20;       // a placement key is (design << 20) | column
-static_assert(kFlushRounds * kRoundSteps <= 32, "window slots must stay below 2^62");
-
+constexpr int kWideCols = 32;
+constexpr int kWinRows = 64;   // the wave size: in a flush lane l takes row l
+constexpr int kRoundSteps = 2;        // steps between window checks
+constexpr int kFlushRounds = 512;     // every slot is emptied at least this often
 // A slot sums a node's deposits in one signed 64-bit word.  What enters the window are the
 // lanes' cached sums below 2^46 units in magnitude (larger ones go to the node's words in
-// global memory); a lane adds at most one of them to
-// a node per step (when its trajectory leaves the node or ends), a round is kRoundSteps steps
-// of at most 1024 lanes, and every slot is flushed at least every kFlushRounds rounds:
-// |sum| < kFlushRounds * kRoundSteps * 1024 * 2^46 = 2^61.
-template <int WC>
-struct DepositWindow {
-    int lo[2], hi[2], base[2], design[2];  // the placement of table t (hi < lo: no columns)
-    i64 ofn[2];                            // its design's node offset and rows (flush addresses)
-    int ny[2];
-    long long key[2][2];                   // min / max placement key noted in the round
-    int rows[2][2];                        // min / max row noted in the round
-    u64 word[WC][kWinRows];
-};
+// global memory); a lane adds at most one of them to a node per step (when its trajectory
+// leaves the node or ends), so a slot takes at most 64 per step, and it is flushed at least
+// every kFlushRounds rounds: |sum| < kFlushRounds * kRoundSteps * 64 * 2^46 = 2^62.
+static_assert(kFlushRounds * kRoundSteps * 64 <= (1 << 16), "window slots must stay below 2^62");
 
-// A round's placement, read once into registers.
-struct WinView {
-    int lo, hi, base, design;
-};
+// The LDS of the windows: wave w of a workgroup owns words [w WC 64, (w + 1) WC 64).
+extern __shared__ u64 bt_window_words[];
 
-template <int WC>
-__device__ __forceinline__ WinView window_view(const DepositWindow<WC>& w, int t) {
-    return {w.lo[t], w.hi[t], w.base[t], w.design[t]};
-}
+// A wave's window: its placement (wave-uniform) and the mesh of its design (flush addresses).
+struct WaveWin {
+    int slot0;             // this wave's first word in bt_window_words
+    int lo, base, design;  // columns lo .. lo + WC - 1, rows base .. base + 63 of `design`
+    int ny;                // rows of that design's mesh
+    i64 ofn;               // its first node in the accumulator
+};
 
 // Splits a slot's signed sum q into the two node words (low 32 bits, q >> 32).
 __device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
     add_words(w, q & 0xffffffffull, (u64)((i64)q >> 32));
 }
 
-template <int WC>
-__device__ __forceinline__ void window_reset_notes(DepositWindow<WC>& w, int t) {
-    w.key[t][0] = 0x7fffffffffffffffll;
-    w.key[t][1] = -1;
-    w.rows[t][0] = 0x7fffffff;
-    w.rows[t][1] = -1;
-}
-
-template <int WC>
-__device__ __forceinline__ void window_init(DepositWindow<WC>& w, const TraceParams& p) {
-    for (int k = threadIdx.x; k < WC * kWinRows; k += blockDim.x) (&w.word[0][0])[k] = 0ull;
-    if (threadIdx.x < 2) {
-        const int t = threadIdx.x;
-        w.lo[t] = 0; w.hi[t] = -1; w.base[t] = 0; w.design[t] = 0;
-        w.ofn[t] = p.bofn; w.ny[t] = p.ny;
-        window_reset_notes(w, t);
-    }
-}
-
-// The end of a round: the first and last lane of each wave whose cache holds a cell report
-// it (design, column, row) for the next placement (the notes only place the window; a sum
-// outside it goes to global memory).
-template <int WC>
-__device__ __forceinline__ void window_note(DepositWindow<WC>& win, int t, const LaneState& s, int design) {
-    const bool has = s.cci != kNoCell;
-    const u64 lanes = __ballot(has);
-    const int me = (int)__lane_id();
-    if (has && (me == __ffsll((long long)lanes) - 1 || me == 63 - __clzll((long long)lanes))) {
-        const long long key = ((long long)design << kColumnBits) | s.cci;
-        atomicMin(&win.key[t][0], key);
-        atomicMax(&win.key[t][1], key);
-        atomicMin(&win.rows[t][0], s.ccj);
-        atomicMax(&win.rows[t][1], s.ccj);
-    }
-}
-
-// Adds q units (|q| < 2^50) to node (ii, jj) of the design's mesh: into its window slot when
-// the round's window holds it (USE_WIN) and |q| < 2^46, else to the node's words in global
+// Adds q units (|q| < 2^50) to node (ii, jj) of the design's mesh: into the wave's window slot
+// when the window holds it (USE_WIN) and |q| < 2^46, else to the node's words in global
 // memory.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void node_add(DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
-                                         i64 ny, int design, int ii, int jj, i64 q) {
-    if (USE_WIN && q < kWindowLimit && q > -kWindowLimit && design == wv.design && ii >= wv.lo && ii <= wv.hi &&
-        jj >= wv.base && jj < wv.base + kWinRows) {
-        atomicAdd(&win.word[ii & (WC - 1)][jj - wv.base], (u64)q);  // two's complement
+__device__ __forceinline__ void node_add(const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny, int design,
+                                         int ii, int jj, i64 q) {
+    if (USE_WIN && q < kWindowLimit && q > -kWindowLimit && design == w.design &&
+        (unsigned)(ii - w.lo) < (unsigned)WC && (unsigned)(jj - w.base) < (unsigned)kWinRows) {
+        atomicAdd(&bt_window_words[w.slot0 + (ii & (WC - 1)) * kWinRows + (jj - w.base)], (u64)q);
         return;
     }
     add_sum(acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
@@ -315,12 +267,12 @@ __device__ __forceinline__ i64 exact_i64(double q) {
 // that stays very long in one cell) goes to the global words through to_fixed, which drops a
 // non-finite or out-of-range sum as it drops such a single deposit.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void sum_leaves(DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc, i64 ofn,
-                                           i64 ny, int design, int ii, int jj, double sum, int e) {
+__device__ __forceinline__ void sum_leaves(const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny, int design,
+                                           int ii, int jj, double sum, int e) {
     const double units = ldexp(sum, -e);
     if (__builtin_expect(fabs(units) < 0x1p51, 1)) {
         const i64 q = exact_i64(units);
-        if (q != 0) node_add<USE_WIN>(win, wv, acc, ofn, ny, design, ii, jj, q);
+        if (q != 0) node_add<USE_WIN, WC>(w, acc, ofn, ny, design, ii, jj, q);
         return;
     }
     u64 lo, hi;
@@ -332,8 +284,8 @@ __device__ __forceinline__ void sum_leaves(DepositWindow<WC>& win, const WinView
 // rows r, r + 1; column c stays iff i is c - 1 or c, column c + 1 iff i is c or c + 1 (rows
 // alike); slot q holds column c when q's column parity is c's, else c + 1.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win, const WinView& wv,
-                                           u64* __restrict__ acc, i64 ofn, i64 ny, int design, int i, int j, int e) {
+__device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny,
+                                           int design, int i, int j, int e) {
     if (i == s.cci && j == s.ccj) return;
     const int c = s.cci, r = s.ccj;
     const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
@@ -345,7 +297,7 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win,
         const bool kept = (at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1);
         if (!kept) {
             if (s.cq[q] != 0.0)
-                sum_leaves<USE_WIN>(win, wv, acc, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q], e);
+                sum_leaves<USE_WIN, WC>(w, acc, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q], e);
             s.cq[q] = 0.0;
         }
     }
@@ -355,14 +307,14 @@ __device__ __forceinline__ void cache_move(LaneState& s, DepositWindow<WC>& win,
 
 // Empties the lane's cache (the trajectory ended).
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow<WC>& win, const WinView& wv,
-                                            u64* __restrict__ acc, i64 ofn, i64 ny, int design, int e) {
+__device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny,
+                                            int design, int e) {
     const int c = s.cci, r = s.ccj;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (s.cq[q] != 0.0)
-            sum_leaves<USE_WIN>(win, wv, acc, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
-                                s.cq[q], e);
+            sum_leaves<USE_WIN, WC>(w, acc, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
+                                    s.cq[q], e);
         s.cq[q] = 0.0;
     }
     s.cci = kNoCell;
@@ -377,11 +329,11 @@ __device__ __forceinline__ void cache_flush(LaneState& s, DepositWindow<WC>& win
 // The node totals are exact integer sums of rounded values, so the order of the lanes does
 // not change them; a non-finite deposit is dropped.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void deposit4(LaneState& s, DepositWindow<WC>& win, const WinView& wv, u64* __restrict__ acc,
-                                         i64 ofn, i64 ny, bool in_base, int design, int i, int j,
-                                         double u, double w, double deposit, int e) {
+__device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, u64* __restrict__ acc, i64 ofn, i64 ny,
+                                         bool in_base, int design, int i, int j, double u, double w, double deposit,
+                                         int e) {
     if (__builtin_expect(!(fabs(deposit) < __builtin_inf()), 0)) return;  // (the four values would not be finite)
-    if (in_base) cache_move<USE_WIN>(s, win, wv, acc, ofn, ny, design, i, j, e);
+    if (in_base) cache_move<USE_WIN, WC>(s, ww, acc, ofn, ny, design, i, j, e);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
 #pragma unroll
@@ -398,83 +350,81 @@ __device__ __forceinline__ void deposit4(LaneState& s, DepositWindow<WC>& win, c
     }
 }
 
-// After the deposits of round k (table t = k mod 2): place the window around the noted
-// cells of the lowest noted design (columns with room for kRoundSteps steps of travel either
-// way; the base row kept while the noted rows fit with that room), flush the slots that
-// leave (all of them with `empty_all`) and write table 1 - t.  `p` describes the mesh of a
-// one-mesh launch; a batched launch reads the chosen design's mesh from its table.
-template <bool BATCHED, int WC>
-__device__ __forceinline__ void window_maintain(DepositWindow<WC>& win, int t, const TraceParams& p, bool empty_all) {
-    const int n = 1 - t;
-    const long long kmin = win.key[t][0], kmax = win.key[t][1];
-    const int olo = win.lo[t], ohi = win.hi[t], obase = win.base[t], odesign = win.design[t];
-    const i64 oofn = win.ofn[t];
-    const int ony = win.ny[t];
-    int nlo = olo, nhi = ohi, nbase = obase, ndesign = odesign, nny = ony;
-    i64 nofn = oofn;
-    if (kmin <= kmax) {
-        ndesign = (int)(kmin >> kColumnBits);
-        i64 ofn = p.bofn;
-        int nx = p.nx, ny = p.ny;
-        if (BATCHED) {
-            const BatchDesign& b = p.designs[ndesign];
-            ofn = b.node_offset; nx = (int)b.nx; ny = (int)b.ny;
+// Columns first .. last of the wave's window (a range it holds) to global memory, slots
+// emptied: lane l takes row l of each column.  Called by the whole wave.
+template <int WC>
+__device__ __forceinline__ void window_flush_columns(const WaveWin& w, u64* __restrict__ acc, int first, int last) {
+    const int lane = (int)__lane_id();
+    for (int c = first; c <= last; ++c) {
+        const int idx = w.slot0 + (c & (WC - 1)) * kWinRows + lane;
+        const u64 q = bt_window_words[idx];
+        if (q != 0) {
+            add_sum(acc + kWords * (w.ofn + (i64)c * w.ny + w.base + lane), q);
+            bt_window_words[idx] = 0ull;
         }
-        const int cmin = (int)(kmin & ((1 << kColumnBits) - 1));
-        // the highest column of the same design (another design's lanes leave the span open)
-        const int cmax = (int)(kmax >> kColumnBits) == ndesign ? (int)(kmax & ((1 << kColumnBits) - 1)) : nx - 1;
-        nlo = cmin - 1 - kRoundSteps;
-        if (nlo < 0) nlo = 0;
-        nhi = cmax + 2 + kRoundSteps;
-        if (nhi > nx - 1) nhi = nx - 1;
-        if (nhi > nlo + WC - 1) nhi = nlo + WC - 1;
-        const int rmin = win.rows[t][0], rmax = win.rows[t][1];
-        if (!(ndesign == odesign && rmin - kRoundSteps >= obase && rmax + 1 + kRoundSteps < obase + kWinRows)) {
-            nbase = rmin - 4;
-            if (nbase > ny - kWinRows) nbase = ny - kWinRows;
-            if (nbase < 0) nbase = 0;
-        }
-        nofn = ofn;
-        nny = ny;
-    }
-    const bool same_rows = !empty_all && ndesign == odesign && nbase == obase;
-    if (ohi >= olo) {
-        for (int k = threadIdx.x; k < WC * kWinRows; k += blockDim.x) {
-            const int slot = k / kWinRows, row = k % kWinRows;
-            const int col = olo + ((slot - olo) & (WC - 1));  // the column slot holds
-            if (col > ohi || (same_rows && col >= nlo && col <= nhi)) continue;  // empty, or stays
-            const u64 q = win.word[slot][row];
-            if (q != 0) {
-                add_sum(p.acc + kWords * (oofn + (i64)col * ony + obase + row), q);
-                win.word[slot][row] = 0ull;
-            }
-        }
-    }
-    if (threadIdx.x == 0) {
-        win.lo[n] = nlo; win.hi[n] = nhi; win.base[n] = nbase; win.design[n] = ndesign;
-        win.ofn[n] = nofn; win.ny[n] = nny;
-        window_reset_notes(win, n);
     }
 }
 
-// Every node sum still in the window (table t) to global memory.
+// The whole window to global memory (the wave ends, or re-bases).
 template <int WC>
-__device__ __forceinline__ void window_flush(DepositWindow<WC>& win, int t, u64* __restrict__ acc) {
-    const int lo = win.lo[t], hi = win.hi[t], base = win.base[t], ny = win.ny[t];
-    const i64 ofn = win.ofn[t];
-    if (hi < lo) return;
-    for (int k = threadIdx.x; k < WC * kWinRows; k += blockDim.x) {
-        const int slot = k / kWinRows, row = k % kWinRows;
-        const int col = lo + ((slot - lo) & (WC - 1));
-        const u64 q = win.word[slot][row];
-        if (col <= hi && q != 0) add_sum(acc + kWords * (ofn + (i64)col * ny + base + row), q);
+__device__ __forceinline__ void window_flush_all(const WaveWin& w, u64* __restrict__ acc) {
+    if (w.design >= 0) window_flush_columns<WC>(w, acc, w.lo, w.lo + WC - 1);
+}
+
+// The wave's window at the start: empty, placed nowhere (no design).
+template <int WC>
+__device__ __forceinline__ WaveWin window_start() {
+    WaveWin w;
+    w.slot0 = (int)(threadIdx.x / 64u) * WC * kWinRows;
+    w.lo = 0; w.base = 0; w.design = -1; w.ny = 0; w.ofn = 0;
+    const int lane = (int)__lane_id();
+    for (int c = 0; c < WC; ++c) bt_window_words[w.slot0 + c * kWinRows + lane] = 0ull;
+    return w;
+}
+
+// After a round: follow the lanes.  `q` is the mesh of the lanes being stepped (for a batch,
+// that of the wave's current design) and `counts` marks them.  The first and last such lanes
+// holding a cell give the span the window must hold for another round of travel (a trajectory
+// moves at most one cell per step); when it does not, or `force`, the window moves: columns
+// placed with the trailing lane kRoundSteps inside the low end, rows centred on the lanes'
+// rows.  Wave-uniform; called by the whole wave.
+template <int WC>
+__device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, u64* __restrict__ acc,
+                                             const LaneState& s, bool counts, bool force) {
+    const u64 has = __ballot(counts && s.cci != kNoCell);
+    if (has == 0) {
+        if (force) {
+            window_flush_all<WC>(w, acc);
+            w.design = -1;
+        }
+        return;
     }
+    const int f = __ffsll((long long)has) - 1, l = 63 - __clzll((long long)has);
+    const int c0 = __builtin_amdgcn_readlane(s.cci, f), c1 = __builtin_amdgcn_readlane(s.cci, l);
+    const int r0 = __builtin_amdgcn_readlane(s.ccj, f), r1 = __builtin_amdgcn_readlane(s.ccj, l);
+    const int cmin = min(c0, c1), cmax = max(c0, c1), rmin = min(r0, r1), rmax = max(r0, r1);
+    const bool same = q.design == w.design;
+    const bool rows_ok = same && rmin - kRoundSteps >= w.base && rmax + 1 + kRoundSteps < w.base + kWinRows;
+    const bool cols_ok = cmin - kRoundSteps >= w.lo && cmax + 1 + kRoundSteps < w.lo + WC;
+    if (__builtin_expect(rows_ok && cols_ok && !force, 1)) return;
+    int nlo = cmin - kRoundSteps;
+    if (nlo < 0) nlo = 0;
+    if (rows_ok && !force) {  // only columns leave: those of [lo, lo + WC) outside [nlo, nlo + WC)
+        if (nlo > w.lo) window_flush_columns<WC>(w, acc, w.lo, min(nlo, w.lo + WC) - 1);
+        else window_flush_columns<WC>(w, acc, max(nlo + WC, w.lo), w.lo + WC - 1);
+        w.lo = nlo;
+        return;
+    }
+    window_flush_all<WC>(w, acc);
+    int nbase = ((rmin + rmax + 1) >> 1) - kWinRows / 2;
+    if (nbase > q.ny - kWinRows) nbase = q.ny - kWinRows;
+    if (nbase < 0) nbase = 0;
+    w.lo = nlo; w.base = nbase; w.design = q.design; w.ny = q.ny; w.ofn = q.bofn;
 }
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN, int WC>
-__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, DepositWindow<WC>& win,
-                                    const WinView& wv) {
+__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, const WaveWin& ww) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -636,7 +586,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         if (!(fj >= 0.0)) { w = 0.0; j = 0; }
         else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = (int)pv.ny - 2; }
         else j = (int)fj;
-        deposit4<USE_WIN>(s, win, wv, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
+        deposit4<USE_WIN, WC>(s, ww, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
                           s.dqk * dt, p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
@@ -733,9 +683,10 @@ __device__ __forceinline__ void store_end(const TraceParams& p, const LaneState&
 }
 
 // ---- batched form -------------------------------------------------------------------
-// The lane's mesh: design d's table entry over the batch's shared fields (the arenas, the
+// A design's mesh: its table entry over the batch's shared fields (the arenas, the
 // accumulator and the trajectory arrays stay those of the launch; the design's nodes start
-// at its node offset in the accumulator).
+// at its node offset in the accumulator).  Called with a wave-uniform design, so that the
+// mesh parameters stay in scalar registers.
 __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& p, int d) {
     const BatchDesign& b = p.designs[d];
     lp.nx = (int)b.nx; lp.ny = (int)b.ny;
@@ -754,85 +705,99 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
     lp.design = d;
 }
 
-// The persistent trace loop.  One mesh (BATCHED false): every lane uses the launch's
-// parameters.  BATCHED: every design's trajectories share the queue, and a lane takes the
-// mesh of each new trajectory's design; the deposit window follows the lowest design of
-// the workgroup's lanes, and lanes of other designs add to global memory.
+// The persistent trace loop of one wave.  Waves run independently (no barrier): each claims
+// trajectories for its free lanes from the launch's queue, steps them in rounds of
+// kRoundSteps steps and keeps its own deposit window.  One mesh (BATCHED false): every lane
+// uses the launch's parameters.  BATCHED: every design's trajectories share the queue; the
+// wave steps the lanes of one design at a time, with that design's mesh in wave-uniform
+// registers, and turns to the design of its first busy lane when none of the current one is
+// left (lanes of other designs wait: a queue laid out in runs of whole waves per design
+// never makes a wave wait).
 //
 // RECORDS: the launch may record paths and checkpoints (else that code is compiled out; a
 // batched launch records neither).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS, int WC = kWinCols>
 __device__ __forceinline__ void trace_loop(const TraceParams& p) {
-    __shared__ DepositWindow<WC> win;
-    window_init(win, p);
+    WaveWin ww = window_start<WC>();
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    TraceParams lp = p;
-    const TraceParams& q = BATCHED ? lp : p;  // the lane's mesh
+    TraceParams dp = p;  // BATCHED: the mesh of design wd
+    int wd = -1;
+    const TraceParams& q = BATCHED ? dp : p;  // the mesh of the lanes being stepped
     const Fields F = {field_rsrc(p.fxs, p.n_fxs), field_rsrc(p.fys, p.n_fys)};
     LaneState s;
     s.k = -1;
     s.cci = s.ccj = kNoCell;
     s.cq[0] = s.cq[1] = s.cq[2] = s.cq[3] = 0.0;
-    bool exhausted = false;
-    bool refill = true;  // with refill_block: only when every lane of the workgroup is free
-    int t = 0;           // the window table of this round
+    int my_design = 0;      // BATCHED: the design of the lane's trajectory
+    bool exhausted = false;  // wave-uniform: the queue has no entry left for this wave
+    bool refill = true;      // with refill_block: only when every lane of the wave is free
     int rounds = 0;
     for (;;) {
-        const bool want = s.k < 0 && !exhausted && refill;
-        const u64 need = __ballot(want);
-        if (need != 0) {
-            const int count = __popcll(need);
-            const int leader = __ffsll((long long)need) - 1;
-            u64 base = 0;
-            if ((int)lane == leader) base = atomicAdd(p.queue, (u64)count);
-            base = __shfl(base, leader);
-            if (want) {
-                const u64 m = base + (u64)__popcll(need & below);
-                if (m < (u64)p.n_active) {
-                    const i64 k = p.active[m];
-                    const i64 N = p.n;
-                    bool valid = k >= 0 && k < N;
-                    if (BATCHED && valid) {
-                        const int d = p.design_of[k];
-                        valid = d >= 0 && d < p.n_designs;
-                        if (valid) load_design(lp, p, d);
+        if (!exhausted && refill) {
+            const bool want = s.k < 0;
+            const u64 need = __ballot(want);
+            bool end_seen = false;
+            if (need != 0) {
+                const int count = __popcll(need);
+                const int leader = __ffsll((long long)need) - 1;
+                u64 base = 0;
+                if ((int)lane == leader) base = atomicAdd(p.queue, (u64)count);
+                base = __shfl(base, leader);
+                if (want) {
+                    const u64 m = base + (u64)__popcll(need & below);
+                    if (m < (u64)p.n_active) {
+                        const i64 k = p.active[m];
+                        const i64 N = p.n;
+                        bool valid = k >= 0 && k < N;
+                        if (BATCHED && valid) {
+                            my_design = p.design_of[k];
+                            valid = my_design >= 0 && my_design < p.n_designs;
+                        }
+                        // an index that is not a trajectory (or of no design) is skipped: the
+                        // lane asks again
+                        if (valid) {
+                            s.k = k;
+                            s.x0 = p.x[k]; s.x1 = p.x[N + k]; s.x2 = p.x[2 * N + k];
+                            s.v0 = p.v[k]; s.v1 = p.v[N + k]; s.v2 = p.v[2 * N + k];
+                            s.qmk = p.qm[k];
+                            s.dqk = p.dq[k];
+                            s.slot = (RECORDS && p.path_slot != nullptr) ? p.path_slot[k] : -1;
+                            if (s.slot >= p.n_paths) s.slot = -1;
+                            s.ns = p.steps[k];
+                            s.have = 0;
+                            s.point = 0;
+                            s.ca0 = s.ca1 = s.ca2 = 0.0;
+                            s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
+                            if (RECORDS && p.ckpt != nullptr) {  // the first multiple of the interval >= ns
+                                s.ck_index = (s.ns + p.ckpt_interval - 1) / p.ckpt_interval;
+                                s.ck_next = s.ck_index * p.ckpt_interval;
+                            }
+                            if (s.ns >= p.max_steps) {  // no step: the loop of the reference ends at once
+                                store_end(p, s, 4);
+                                s.k = -1;
+                            }
+                        }
+                    } else {
+                        end_seen = true;
                     }
-                    // an index that is not a trajectory (or of no design) is skipped: the
-                    // lane asks again
-                    if (valid) {
-                    s.k = k;
-                    s.x0 = p.x[k]; s.x1 = p.x[N + k]; s.x2 = p.x[2 * N + k];
-                    s.v0 = p.v[k]; s.v1 = p.v[N + k]; s.v2 = p.v[2 * N + k];
-                    s.qmk = p.qm[k];
-                    s.dqk = p.dq[k];
-                    s.slot = (RECORDS && p.path_slot != nullptr) ? p.path_slot[k] : -1;
-                    if (s.slot >= p.n_paths) s.slot = -1;
-                    s.ns = p.steps[k];
-                    s.have = 0;
-                    s.point = 0;
-                    s.ca0 = s.ca1 = s.ca2 = 0.0;
-                    s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
-                    if (RECORDS && p.ckpt != nullptr) {  // the first multiple of the interval >= ns
-                        s.ck_index = (s.ns + p.ckpt_interval - 1) / p.ckpt_interval;
-                        s.ck_next = s.ck_index * p.ckpt_interval;
-                    }
-                    if (s.ns >= p.max_steps) {  // no step: the loop of the reference ends at once
-                        store_end(p, s, 4);
-                        s.k = -1;
-                    }
-                    }
-                } else {
-                    exhausted = true;
                 }
             }
+            // claims are handed out in order: once one lane sees the end, later claims would too
+            if (__ballot(end_seen) != 0) exhausted = true;
         }
-        // the whole workgroup goes on while any lane has or may get a trajectory
-        if (!__syncthreads_or(s.k >= 0 || !exhausted)) break;
-        const WinView wv = window_view(win, t);
+        const u64 busy = __ballot(s.k >= 0);
+        if (busy == 0) {
+            if (exhausted) break;
+            continue;  // every claim was skipped: ask again
+        }
+        if (BATCHED && __ballot(s.k >= 0 && my_design == wd) == 0) {
+            wd = __builtin_amdgcn_readlane(my_design, __ffsll((long long)busy) - 1);
+            load_design(dp, p, wd);
+        }
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
-            if (s.k >= 0) {
+            if (s.k >= 0 && (!BATCHED || my_design == wd)) {
                 if (RECORDS && p.ckpt != nullptr && s.ns == s.ck_next) {
                     if (s.ck_index < p.n_ckpt) {
                         double* c = p.ckpt + s.ck_index * 6 * p.n + s.k;
@@ -842,21 +807,18 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.ck_index += 1;
                     s.ck_next += p.ckpt_interval;
                 }
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true>(q, F, s, win, wv);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC>(q, F, s, ww);
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush<true>(s, win, wv, p.acc, q.bofn, q.ny, q.design, q.deposit_exponent);
+                    cache_flush<true, WC>(s, ww, p.acc, q.bofn, q.ny, q.design, q.deposit_exponent);
                 }
             }
         }
-        window_note(win, t, s, q.design);
-        const bool busy = __syncthreads_or(s.k >= 0);
-        refill = !p.refill_block || !busy;
-        window_maintain<BATCHED>(win, t, p, ++rounds % kFlushRounds == 0);
-        t = 1 - t;
+        window_round<WC>(ww, q, p.acc, s, !BATCHED || my_design == wd, ++rounds % kFlushRounds == 0);
+        refill = !p.refill_block || __ballot(s.k >= 0) == 0;
     }
-    window_flush(win, t, p.acc);
+    window_flush_all<WC>(ww, p.acc);
 }
 
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
@@ -865,9 +827,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
 }
 
 // Every design's trajectories in one persistent launch.  Held to three waves per SIMD (168
-// VGPRs): the lanes' own meshes add about 40 registers to the one-mesh loop, and a batch
-// always fills the GPU, so the third wave's latency hiding outweighs the spills of the RK4
-// cylindrical variant (256 einzel candidates: 536 -> 472 ms).
+// VGPRs): a batch always fills the GPU, so the third wave's latency hiding pays.
 template <int INTEG, int GEOM, bool HAS_B>
 __global__ __launch_bounds__(256, 3) void batched_kernel(TraceParams p) {
     trace_loop<INTEG, GEOM, HAS_B, false, true, false>(p);
@@ -921,6 +881,11 @@ __global__ void finalize_kernel(const u64* __restrict__ acc, i64 nodes, int e, d
     }
 }
 
+// The LDS of a workgroup's deposit windows: one window per wave.
+static size_t window_bytes(int block, bool wide) {
+    return (size_t)(block / 64) * (wide ? kWideCols : kWinCols) * kWinRows * sizeof(u64);
+}
+
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
 static TraceKernel kernel_ptr() {
     return trace_kernel<INTEG, GEOM, HAS_B, GENERAL, RECORDS, WC>;
@@ -954,7 +919,7 @@ int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, 
     if (zgrid < 1) zgrid = 1;
     hipLaunchKernelGGL(prepare_kernel, dim3(zgrid), dim3(256), 0, stream, p.acc, words, p.queue);
     if (p.n_active > 0 && grid > 0)
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, p);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), window_bytes(block, !general && wide), stream, p);
     int fgrid = (int)((nodes + 255) / 256);
     if (fgrid > 4096) fgrid = 4096;
     if (fgrid < 1) fgrid = 1;
@@ -989,7 +954,7 @@ int launch_batched(const TraceParams& p, const double* x0, const double* v0, boo
     if (zgrid < 1) zgrid = 1;
     hipLaunchKernelGGL(prepare_kernel, dim3(zgrid), dim3(256), 0, stream, p.acc, words, p.queue);
     if (p.n_active > 0 && grid > 0)
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, p);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), window_bytes(block, false), stream, p);
     int fgrid = (int)((max_design_nodes + 255) / 256);
     if (fgrid > 1024) fgrid = 1024;
     if (fgrid < 1) fgrid = 1;
@@ -1004,7 +969,7 @@ int batched_occupancy(int integ, int geom, bool has_b, int block) {
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),
-                                                     block, 0) != hipSuccess)
+                                                     block, window_bytes(block, false)) != hipSuccess)
         return 0;
     return blocks;
 }
@@ -1038,7 +1003,7 @@ int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records,
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),
-                                                     block, 0) != hipSuccess)
+                                                     block, window_bytes(block, !general && wide)) != hipSuccess)
         return 0;
     return blocks;
 }
