@@ -138,16 +138,24 @@ struct StepRule {
 __device__ __forceinline__ StepRule step_rule(double v2, double a2, double h, double f, double gv) {
     StepRule o;
     // the common case (|v| and |a| of normal size) without a branch: the same operations as
-    // the general rule below on these inputs, so the same bits
+    // the general rule below on these inputs, so the same bits.  d2 = sqrt(q) can only be the
+    // smaller term when q < d1^2; when q exceeds d1^2 by a relative 2^-40 (far more than the
+    // few units in the last place of the computed d2 and d1^2) the speed term is chosen without
+    // the third square root, with the bits of the full comparison.
     {
         const double iv = rsqrt_nr(v2);
         const double ia = rsqrt_nr(a2);
         const double q = (2.0 * h) * ia;
-        const double d2 = q * rsqrt_nr(q);
         const double d1 = (h * iv) / gv;
         if (__builtin_expect(v2 >= 0x1p-1000 && a2 >= 0x1p-1000 && q >= 0x1p-1000, 1)) {
             o.speed = gv * (v2 * iv);
             o.accel = a2 * ia;
+            if (__builtin_expect(q > (d1 * d1) * (1.0 + 0x1p-40), 1)) {
+                o.branch = 1;
+                o.dt = f * d1;
+                return o;
+            }
+            const double d2 = q * rsqrt_nr(q);
             const bool second = d2 < d1;
             o.branch = second ? 2 : 1;
             o.dt = f * (second ? d2 : d1);

@@ -422,9 +422,36 @@ __device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, u
     w.lo = nlo; w.base = nbase; w.design = q.design; w.ny = q.ny; w.ofn = q.bofn;
 }
 
+// The record of a crossing of plane pl at xp during the step from (x, u) to (y, w)
+// (pyccel.py:426-442): the point and velocity interpolated at the crossing, the direction and
+// the step; the count goes on past the capacity (the caller then retraces).
+__device__ __forceinline__ void record_crossing(const TraceParams& p, const LaneState& s, int pl, double xp,
+                                                double x0, double x1, double x2, double y0, double y1, double y2,
+                                                double gv, double v0, double v1, double v2, double gw, double w0,
+                                                double w1, double w2, i64 ns) {
+    const double before = x0 - xp;
+    const double after = y0 - xp;
+    const double sfrac = before / (before - after);
+    const i64 idx = s.k * p.n_planes + pl;
+    const i64 c = p.counts[idx];
+    if (c < p.capacity) {
+        double* rec = p.hits + (idx * p.capacity + c) * 8;
+        rec[0] = xp;
+        rec[1] = x1 + sfrac * (y1 - x1);
+        rec[2] = x2 + sfrac * (y2 - x2);
+        rec[3] = gv * v0 + sfrac * (gw * w0 - gv * v0);
+        rec[4] = gv * v1 + sfrac * (gw * w1 - gv * v1);
+        rec[5] = gv * v2 + sfrac * (gw * w2 - gv * v2);
+        rec[6] = after >= 0.0 ? 1.0 : -1.0;
+        rec[7] = 1.0 * (double)ns;
+    }
+    p.counts[idx] = c + 1;
+}
+
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN, int WC>
-__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, const WaveWin& ww) {
+__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, const WaveWin& ww,
+                                    double plane0) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -554,6 +581,31 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         y0 = q0; y1 = q1; y2 = q2;
     }
 
+    // the end point (the next step's start): its distance from the axis, radial direction and
+    // cell coordinates, and the electrode entry of its cell, requested here so that the load
+    // overlaps the deposit below (pyccel.py:450-497 test them after the deposit; the order of
+    // these reads does not change a result).  Cell indices are clamped, so the load stays in
+    // bounds for a point off the mesh too.
+    const bool end_known = !GENERAL && INTEG != 0 && s.have != 0;
+    double py = y1, pcy = 0.0, pcz = 0.0;
+    if (end_known) {
+        py = end_y; pcy = end_cy; pcz = end_cz;
+    } else if (GEOM == 1) {
+        const Radial rd = radial(y1, y2);
+        py = rd.r; pcy = rd.cy; pcz = rd.cz;
+    } else if (GEOM == 2) {
+        py = p.origin_y + fabs(y1 - p.origin_y);
+    }
+    const double lx = locate_x<GENERAL>(p, y0);
+    const double ci = end_known ? end_sx : (GENERAL ? (lx - p.origin_x) / p.h : (lx - p.origin_x) * p.inv_h);
+    const double cj = end_known ? end_sy : (GENERAL ? (py - p.origin_y) / p.h : (py - p.origin_y) * p.inv_h);
+    // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341: min(trunc(max(c, 0)),
+    // n - 2, cells - 1), taken as the saturated trunc(c) clamped to [0, that bound] (the
+    // same integer for an integer bound)
+    const int ei = clamped_floor_index(ci, p.cell_imax);
+    const int ej = clamped_floor_index(cj, p.cell_jmax);
+    const i64 cell = p.cells[(unsigned)p.cells_base + (unsigned)ei * (unsigned)p.cells_cols + (unsigned)ej];
+
     // space charge: dq dt at the Hermite midpoint, area-weighted (pyccel.py:372-423)
     const double gw = 1.0 / lorentz<GENERAL>(w0, w1, w2, inv_c2);
     const double dt8 = dt * 0.125;
@@ -592,75 +644,44 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     const i64 ns = s.ns + 1;
     s.ns = ns;
 
-    // plane crossings (pyccel.py:426-442)
-    for (int pl = 0; pl < p.n_planes; ++pl) {
-        const double xp = p.planes[p.planes_base + pl];
-        const double before = x0 - xp;
-        const double after = y0 - xp;
-        if (__builtin_expect((before < 0.0) != (after < 0.0), 0)) {
-            const double sfrac = before / (before - after);
-            const i64 idx = s.k * p.n_planes + pl;
-            const i64 c = p.counts[idx];
-            if (c < p.capacity) {
-                double* rec = p.hits + (idx * p.capacity + c) * 8;
-                rec[0] = xp;
-                rec[1] = x1 + sfrac * (y1 - x1);
-                rec[2] = x2 + sfrac * (y2 - x2);
-                rec[3] = gv * v0 + sfrac * (gw * w0 - gv * v0);
-                rec[4] = gv * v1 + sfrac * (gw * w1 - gv * v1);
-                rec[5] = gv * v2 + sfrac * (gw * w2 - gv * v2);
-                rec[6] = after >= 0.0 ? 1.0 : -1.0;
-                rec[7] = 1.0 * (double)ns;
+    // plane crossings (pyccel.py:426-442); the first plane's position is in a register, and a
+    // record's address is formed only for a crossing (rare)
+    if (p.n_planes > 0) {
+        if (__builtin_expect((x0 - plane0 < 0.0) != (y0 - plane0 < 0.0), 0))
+            record_crossing(p, s, 0, plane0, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
+        if (__builtin_expect(p.n_planes > 1, 0)) {
+            for (int pl = 1; pl < p.n_planes; ++pl) {
+                const double xp = p.planes[p.planes_base + pl];
+                if ((x0 - xp < 0.0) != (y0 - xp < 0.0))
+                    record_crossing(p, s, pl, xp, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
             }
-            p.counts[idx] = c + 1;
         }
     }
-    x0 = y0; x1 = y1; x2 = y2;
-    v0 = w0; v1 = w1; v2 = w2;
-    s.x0 = x0; s.x1 = x1; s.x2 = x2; s.v0 = v0; s.v1 = v1; s.v2 = v2;
+    s.x0 = y0; s.x1 = y1; s.x2 = y2; s.v0 = w0; s.v1 = w1; s.v2 = w2;
 
     // the end of the step: off the mesh or in an electrode (pyccel.py:450-497)
-    const bool end_known = !GENERAL && INTEG != 0 && s.have != 0;
-    double px = x0, py = x1, pcy = 0.0, pcz = 0.0;
-    if (end_known) {
-        py = end_y; pcy = end_cy; pcz = end_cz;
-    } else if (GEOM == 1) {
-        const Radial rd = radial(x1, x2);
-        py = rd.r; pcy = rd.cy; pcz = rd.cz;
-    } else if (GEOM == 2) {
-        py = p.origin_y + fabs(x1 - p.origin_y);
-    }
     s.point = 0;
     if (s.slot >= 0) {
         const i64 c = p.path_count[s.slot];
         if (c < p.path_capacity) {
-            p.paths[(s.slot * p.path_capacity + c) * 2 + 0] = px;
+            p.paths[(s.slot * p.path_capacity + c) * 2 + 0] = y0;
             p.paths[(s.slot * p.path_capacity + c) * 2 + 1] = py;
         }
         p.path_count[s.slot] = c + 1;
     }
-    px = locate_x<GENERAL>(p, px);
     int st = -1;
-    if (px >= p.upper_x) {
+    if (lx >= p.upper_x) {
         st = 0;
-    } else if (px < p.origin_x) {
+    } else if (lx < p.origin_x) {
         st = 1;
     } else if (py > p.upper_y || (GEOM == 0 && py < p.origin_y)) {
         st = 2;
     } else {
-        const double ci = end_known ? end_sx : (GENERAL ? (px - p.origin_x) / p.h : (px - p.origin_x) * p.inv_h);
-        const double cj = end_known ? end_sy : (GENERAL ? (py - p.origin_y) / p.h : (py - p.origin_y) * p.inv_h);
         s.pt_y = py; s.pt_cy = pcy; s.pt_cz = pcz; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
-        // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341: min(trunc(max(c, 0)),
-        // n - 2, cells - 1), taken as the saturated trunc(c) clamped to [0, that bound] (the
-        // same integer for an integer bound)
-        const int i = clamped_floor_index(ci, p.cell_imax);
-        const int j = clamped_floor_index(cj, p.cell_jmax);
-        const i64 cell = p.cells[(unsigned)p.cells_base + (unsigned)i * (unsigned)p.cells_cols + (unsigned)j];
         if (cell >= 0) {
             const double R = (double)p.refine;
-            const int si = (int)fmin(fmax((ci - (double)i) * R, 0.0), R - 1.0);
-            const int sj = (int)fmin(fmax((cj - (double)j) * R, 0.0), R - 1.0);
+            const int si = (int)fmin(fmax((ci - (double)ei) * R, 0.0), R - 1.0);
+            const int sj = (int)fmin(fmax((cj - (double)ej) * R, 0.0), R - 1.0);
             const i64 at = (cell * p.refine + si) * p.refine + sj;
             const i64 e = at < p.n_subcells ? p.subcells[p.subcells_base + at] : -1;
             if (e >= 0) {
@@ -725,6 +746,8 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     int wd = -1;
     const TraceParams& q = BATCHED ? dp : p;  // the mesh of the lanes being stepped
     const Fields F = {field_rsrc(p.fxs, p.n_fxs), field_rsrc(p.fys, p.n_fys)};
+    // the first plane's position, read once (for a batch, once per design of the wave)
+    double plane0 = (!BATCHED && p.n_planes > 0) ? p.planes[p.planes_base] : 0.0;
     LaneState s;
     s.k = -1;
     s.cci = s.ccj = kNoCell;
@@ -794,6 +817,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
         if (BATCHED && __ballot(s.k >= 0 && my_design == wd) == 0) {
             wd = __builtin_amdgcn_readlane(my_design, __ffsll((long long)busy) - 1);
             load_design(dp, p, wd);
+            plane0 = p.n_planes > 0 ? p.planes[dp.planes_base] : 0.0;
         }
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
@@ -807,7 +831,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.ck_index += 1;
                     s.ck_next += p.ckpt_interval;
                 }
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC>(q, F, s, ww);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC>(q, F, s, ww, plane0);
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(p, s, st != -1 ? st : 4);
                     s.k = -1;

@@ -17,13 +17,14 @@ from __future__ import annotations
 
 import numpy as np
 
-from beamtracer_amd import kernels
+from beamtracer_amd import kernels, tuning
 from beamtracer_amd.beam import BeamParticles
 from beamtracer_amd.tracer import (
     SUBCELLS,
     HipBeamTracer,
     TraceResult,
     beam_start,
+    geometry_code,
     outgrown,
     resume_paused,
     trace_result,
@@ -62,7 +63,10 @@ class ResidentTrace:
                          "electrode": gpu(beam.electrode)}  # fmt: skip
         self._state = {k: t.clone() for k, t in self._initial.items()}
         self._steps = torch.zeros(n, dtype=torch.int64, device="cuda")
-        self._qm, self._dq, self._start = gpu(beam.qm), gpu(beam.dq), gpu(beam.start)
+        start = beam.start
+        if tracer.queue_order == "coherent":  # the queue's order, computed once for the resident beam
+            start = tuning.coherent_order(start, beam.x, beam.v, beam.qm, geometry_code(tracer.mesh))
+        self._qm, self._dq, self._start = gpu(beam.qm), gpu(beam.dq), gpu(start)
         cells, subcells = tracer.electrode_lookup()
         self._cells, self._subcells = gpu(cells), gpu(subcells)
         mesh = tracer.mesh

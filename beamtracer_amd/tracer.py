@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 import numpy as np
 from scipy import constants
 
-from beamtracer_amd import kernels
+from beamtracer_amd import kernels, tuning
 from beamtracer_amd.beam import BeamParticles
 from beamtracer_amd.fields import StaggeredElectricField
 from beamtracer_amd.mesh import IonOpticsMesh
@@ -237,6 +237,7 @@ class HipBeamTracer:
         block_size: int = 0,
         refill_block: bool = False,
         wide_window: bool = False,
+        queue_order: str = "emitter",
     ) -> None:
         if not (0.0 < step_fraction <= 1.0):
             raise ValueError(f"step_fraction must lie in (0, 1], got {step_fraction}")
@@ -256,8 +257,13 @@ class HipBeamTracer:
         self.block_size = int(block_size)
         #: a workgroup takes new trajectories together (see :mod:`beamtracer_amd.tuning`)
         self.refill_block = bool(refill_block)
-        #: a deposit window of 128 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
+        #: per-wave deposit windows of 32 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
         self.wide_window = bool(wide_window)
+        if queue_order not in tuning.QUEUE_ORDERS:
+            raise ValueError(f"queue_order must be one of {tuning.QUEUE_ORDERS}, got {queue_order!r}")
+        #: the order in which the kernel's queue hands out the trajectories (results do not
+        #: depend on it; see :func:`beamtracer_amd.tuning.coherent_order`)
+        self.queue_order = queue_order
         self._lookup: tuple[np.ndarray, np.ndarray] | None = None
         #: (plane records per trajectory and plane, points per kept path); grown on overflow
         self._capacity = (4, 1024)
@@ -405,6 +411,8 @@ class HipBeamTracer:
             paths[: beam.saved.size, 0] = beam.first
             path_count = np.ones(slots, dtype=np.int64)
             active = beam.start
+            if self.queue_order == "coherent":
+                active = tuning.coherent_order(active, beam.x, beam.v, beam.qm, geometry_code(mesh))
             while active.size:
                 call = [
                     active, x, v, beam.qm, beam.dq, steps, status, electrode, fxs, fys, patch_int, patch_real,

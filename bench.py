@@ -168,13 +168,16 @@ def device_arguments(host, magnetic=None):
 
 
 def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkpoints=None,
-                refill_block: bool = False, wide_window: bool = False) -> dict:
+                refill_block: bool = False, wide_window: bool = False, queue_order: str = "emitter") -> dict:
     """Trace every trajectory ``steps`` times (after ``warmup``), states reset before each
     (``checkpoints``: (tensor, interval) receiving the states, as for the adjoint)."""
     import torch
 
-    from beamtracer_amd import kernels
+    from beamtracer_amd import kernels, tuning
 
+    if queue_order == "coherent":
+        host = list(host)
+        host[0] = tuning.coherent_order(host[0], host[1], host[2], host[3], int(host[17]))
     dev, mag = device_arguments(host, magnetic)
     initial = {i: dev[i].clone() for i in (1, 2, 5, 6, 7, 25, 26, 31)}
     exponent = kernels.trace_exponent(host[0], host[2], host[4], host[3], host[8], host[9], host[11], host[19])
@@ -338,10 +341,10 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
     dev, mag = device_arguments(host, design.magnetic)
     tune = tuning.tune_trace(dev, magnetic=mag, repeats=1)
     log(f"extraction tuned: block {tune['block_size']} refill_block {tune['refill_block']} "
-        f"wide_window {tune['wide_window']} ({tune['best_ms']:.1f} ms)")
+        f"wide_window {tune['wide_window']} queue {tune['queue_order']} ({tune['best_ms']:.1f} ms)")
     del dev, mag
     r = time_traces(host, design.magnetic, steps, 1, tune["block_size"], refill_block=tune["refill_block"],
-                    wide_window=tune["wide_window"])  # fmt: skip
+                    wide_window=tune["wide_window"], queue_order=tune["queue_order"])  # fmt: skip
     rate = r["steps_per_trace"] * steps / r["wall"]
     flops = FLOP_PER_STEP_BORIS_B_CYL * r["steps_per_trace"] / r["launch_s"] / 1e12
     return {
@@ -356,7 +359,7 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
         "frac_fp64_peak": flops / PEAK_FP64_TFLOPS,
         "flop_per_step": FLOP_PER_STEP_BORIS_B_CYL,
         "launch_shape": {"block_size": tune["block_size"], "refill_block": tune["refill_block"],
-                         "wide_window": tune["wide_window"]},
+                         "wide_window": tune["wide_window"], "queue_order": tune["queue_order"]},
         "tuning_ms": tune["timings_ms"],
         "default_shape_launch_ms": default["launch_s"] * 1e3,
     }

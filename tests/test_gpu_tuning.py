@@ -25,7 +25,8 @@ def test_tuned_trace_equals_the_default_trace():
         dev[i] = args[i]
     tune = tuning.tune_trace(dev, blocks=(64, 256), repeats=1)
     shapes = {"block64", "block256", "block64/refill-block", "block256/refill-block"}
-    assert set(tune["timings_ms"]) == shapes | {f"{k}/wide" for k in shapes}
+    shapes = shapes | {f"{k}/wide" for k in shapes}
+    assert set(tune["timings_ms"]) == shapes | {f"coherent/{k}" for k in shapes}
     torch.cuda.synchronize()
     for i in (*cases.OUTPUTS, cases.CHARGE):
         assert np.array_equal(dev[i].cpu().numpy(), host[i]), i

@@ -148,3 +148,35 @@ def test_surface_distance_is_bracketed():
     d = m.distance[1, 3, 2]  # from node x = 3 mm towards +x: the surface at 3.4 mm
     assert abs(d - 0.4) <= 1.001 / 255.0  # one step of the bracket
     assert m.near_solid[3, :].all()
+
+
+def test_coherent_queue_order_is_a_permutation_grouped_by_species():
+    from beamtracer_amd import tuning
+
+    rng = np.random.default_rng(3)
+    n = 5000
+    x = np.stack([np.zeros(n), rng.uniform(-1e-3, 1e-3, n), rng.uniform(-1e-3, 1e-3, n)])
+    v = rng.normal(size=(3, n)) * 1e4
+    qm = np.where(np.arange(n) < 3000, 9.6e7, 4.8e7)
+    active = np.flatnonzero(np.arange(n) % 7 != 3).astype(np.int64)
+    for geometry in (0, 1, 2):
+        order = tuning.coherent_order(active, x, v, qm, geometry)
+        assert order.dtype == np.int64 and np.array_equal(np.sort(order), active)
+        # the lighter species, then the heavier (by q/m), each as one run
+        assert np.all(np.diff(qm[order]) >= 0)
+        assert np.array_equal(order, tuning.coherent_order(active, x, v, qm, geometry))  # deterministic
+    # neighbours in the order start alike: a wave's lanes are far closer than random ones
+    order = tuning.coherent_order(active, x, v, qm, 1)
+    r = np.hypot(x[1], x[2])
+    radial, azimuthal = (v[1] * x[1] + v[2] * x[2]) / r, (v[2] * x[1] - v[1] * x[2]) / r
+    # the sort's coordinates, each as its rank among the active trajectories
+    state = np.zeros((4, n))
+    for row, values in enumerate((r, v[0], radial, np.abs(azimuthal))):
+        state[row, active[np.argsort(values[active])]] = np.arange(active.size) / active.size
+
+    def step(sequence):  # mean distance between consecutive starting states of one species
+        d = np.linalg.norm(np.diff(state[:, sequence], axis=1), axis=0)
+        return d[np.diff(qm[sequence]) == 0].mean()
+
+    assert step(order) < 0.4 * step(active)
+    assert tuning.coherent_order(np.array([4], np.int64), x, v, qm, 1).tolist() == [4]
