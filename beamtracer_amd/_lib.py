@@ -179,6 +179,7 @@ SIGNATURES = {
     "bt_trace_workspace_bytes": (_i64, [_i64]),
     "bt_trace_trajectories_2d": (ctypes.c_int, [ctypes.POINTER(TraceArgs)]),
     "bt_trace_trajectories_2d_device": (ctypes.c_int, [ctypes.POINTER(TraceArgs), _ptr]),
+    "bt_trace_trajectories_2d_noalloc": (ctypes.c_int, [ctypes.POINTER(TraceArgs), _ptr]),
     "bt_trace3d_args_size": (_i64, []),
     "bt_trace3d_workspace_bytes": (_i64, [ctypes.POINTER(Trace3DArgs)]),
     "bt_trace_trajectories_3d": (ctypes.c_int, [ctypes.POINTER(Trace3DArgs)]),

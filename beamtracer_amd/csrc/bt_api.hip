@@ -97,6 +97,47 @@ int ensure(Buffer& b, size_t bytes) {
 size_t node_bytes(i64 nodes) { return round256((size_t)(16 * nodes)); }
 size_t workspace_bytes(i64 nodes) { return node_bytes(nodes) + 256 + kTableBytes; }
 
+// ---- the allocation-free trace (bt_trace_trajectories_2d_noalloc) -------------------------
+// Its workspace (node words, queue, tables) is this file-scope device array: every GPU the
+// library runs on holds one copy until the process ends, and every call on that GPU shares it
+// (calls that overlap on one GPU race; the trace's own prepare kernel zeroes the words and the
+// queue at the start of each call).  In host code `scratch` names a host-side stand-in, so the
+// device address of each GPU's copy is taken once per device with hipGetSymbolAddress and
+// used everywhere instead.
+constexpr size_t kScratchBytes = (size_t)BT_NOALLOC_SCRATCH_BYTES;
+static_assert(kScratchBytes == 16 * (size_t)BT_NOALLOC_MAX_NODES + 256 + kTableBytes,
+              "the header's node bound must match the scratch layout");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct NoAllocState {
+    void* scratch = nullptr;  // this device's copy of `scratch`
+    int cus = 0;
+};
+constexpr int kMaxDevices = 64;
+NoAllocState g_noalloc[kMaxDevices];  // fixed: setting a device up allocates nothing
+
+// The current device's scratch address and CU count, looked up on its first call (caller
+// holds g_mutex).
+int noalloc_state(NoAllocState** out) {
+    int dev = 0;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(BT_E_DEVICE, "device index beyond the allocation-free table");
+    NoAllocState& s = g_noalloc[dev];
+    if (s.scratch == nullptr) {
+        void* ptr = nullptr;
+        hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return hip_fail(err, "hipGetSymbolAddress(scratch)");
+        int cus = 0;
+        err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (err != hipSuccess) return hip_fail(err, "hipDeviceGetAttribute");
+        s.cus = cus;
+        s.scratch = ptr;
+    }
+    *out = &s;
+    return BT_OK;
+}
+
 // The fixed launch-independent checks of a trace (host-readable tables only).
 int check_trace(const bt_trace2d_args* a) {
     if (a == nullptr) return fail(BT_E_ARG, "args is NULL");
@@ -439,6 +480,21 @@ int bt_trace_trajectories_2d_device(const bt_trace2d_args* a, void* stream) {
         return fail(BT_E_ARG, "workspace smaller than bt_trace_workspace_bytes()");
     }
     return run_trace(a, make_params(a), ws, static_cast<hipStream_t>(stream), s->cus);
+}
+
+int bt_trace_trajectories_2d_noalloc(const bt_trace2d_args* a, void* stream) {
+    int rc = check_trace(a);
+    if (rc) return rc;
+    if (a->workspace != nullptr) return fail(BT_E_ARG, "the allocation-free trace uses its own scratch: workspace must be NULL");
+    const size_t need = workspace_bytes(trace_nodes(a));
+    if (need > kScratchBytes)
+        return fail(BT_E_ARG, "the allocation-free trace holds at most " + std::to_string(BT_NOALLOC_MAX_NODES) +
+                                  " mesh nodes (its scratch is " + std::to_string(kScratchBytes) + " bytes); " +
+                                  std::to_string(trace_nodes(a)) + " given");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    NoAllocState* s = nullptr;
+    if ((rc = noalloc_state(&s))) return rc;
+    return run_trace(a, make_params(a), s->scratch, static_cast<hipStream_t>(stream), s->cus);
 }
 
 int bt_trace_trajectories_2d(const bt_trace2d_args* a) {

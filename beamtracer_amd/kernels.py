@@ -354,6 +354,7 @@ def trace_trajectories_2d(
     path_count, num_threads, charge, graded, x_nodes, inv_c2,
     *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
     refill_block=False, checkpoints=None, checkpoint_interval=64, record_paths=True, wide_window=False,
+    noalloc=False,
 ) -> None:  # fmt: skip
     """Trace the ``active`` trajectories until they end, or reach a surface.
 
@@ -371,7 +372,9 @@ def trace_trajectories_2d(
     ``path_slot`` is negative, else those paths go unrecorded) and runs the trace variant
     without the path code (the host form finds this itself); ``wide_window`` gives the deposit
     window in LDS 128 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes spread
-    along x). None of these changes a result.
+    along x). ``noalloc`` (device form, no ``workspace``) calls the allocation-free entry
+    point, whose fixed scratch holds at most ``BT_NOALLOC_MAX_NODES`` mesh nodes. None of these
+    changes a result.
     """
     L = _lib.lib()
     device = _is_torch(x)
@@ -451,7 +454,11 @@ def trace_trajectories_2d(
         args.checkpoints = _ptr(checkpoints)
         args.n_checkpoints = checkpoints.shape[0]
         args.checkpoint_interval = int(checkpoint_interval)
-    if device:
+    if noalloc:
+        if not device or workspace is not None:
+            raise ValueError("noalloc takes the device form (torch tensors) without a workspace")
+        rc = L.bt_trace_trajectories_2d_noalloc(ctypes.byref(args), _stream(stream))
+    elif device:
         rc = L.bt_trace_trajectories_2d_device(ctypes.byref(args), _stream(stream))
     else:
         rc = L.bt_trace_trajectories_2d(ctypes.byref(args))

@@ -129,6 +129,17 @@ int64_t bt_trace_workspace_bytes(int64_t n_nodes);
 int bt_trace_trajectories_2d(const bt_trace2d_args* args);
 int bt_trace_trajectories_2d_device(const bt_trace2d_args* args, void* stream);
 
+/* The device form without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as bt_trace_trajectories_2d_device, with
+ * args->workspace NULL: the node words, queue and tables live in a fixed device array of
+ * BT_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the process ends, so the
+ * mesh (all patches) may hold at most BT_NOALLOC_MAX_NODES nodes (larger: BT_E_ARG, nothing
+ * launched).  Every call on one GPU shares that array: calls that overlap on one GPU race. */
+#define BT_NOALLOC_MAX_NODES 1048304
+#define BT_NOALLOC_SCRATCH_BYTES (16 * 1048304 + 256 + 4096)
+int bt_trace_trajectories_2d_noalloc(const bt_trace2d_args* args, void* stream);
+
 /* staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey):
  * fx (nx + 1, ny), fy (nx, ny + 1); replaces staggered_field_2d_pyccel.py:13-55 as called
  * from solver/fields.py:162-165. */
