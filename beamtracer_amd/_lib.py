@@ -167,6 +167,7 @@ class AdjointArgs(ctypes.Structure):
         ("dfx", _ptr), ("dfy", _ptr),
         ("dx", _ptr), ("dv", _ptr),
         ("workspace", _ptr), ("workspace_bytes", _i64),
+        ("dropped", _ptr),
     ]  # fmt: skip
 
 

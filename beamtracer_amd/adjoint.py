@@ -89,6 +89,22 @@ def plane_objective(hits: np.ndarray, counts: np.ndarray, current: np.ndarray, p
     return value, seeds
 
 
+class EvaluationRecord:
+    """What one :meth:`TraceAdjoint.objective` left for its gradient: the plane records and
+    seeds of its trace, J, and the checkpoint buffer with the stamp of the trace that filled
+    it (a later trace through the same tracer changes the stamp)."""
+
+    def __init__(self, hits, counts, seeds, value, checkpoints, stamp):
+        self.hits, self.counts, self.seeds, self.value = hits, counts, seeds, value
+        self.checkpoints, self.stamp = checkpoints, stamp
+
+    def checkpoints_current(self, tracer: HipBeamTracer, interval: int) -> bool:
+        """True while ``tracer`` still holds this evaluation's checkpoints, every ``interval``
+        steps."""
+        return (self.checkpoints is not None and tracer.checkpoints is self.checkpoints
+                and tracer.checkpoint_stamp is self.stamp and self.checkpoints[1] == interval)  # fmt: skip
+
+
 class TraceAdjoint:
     """The forward trace of a beam through edge fields and its adjoint, on the GPU.
 
@@ -122,7 +138,9 @@ class TraceAdjoint:
                               self.interval)  # fmt: skip
 
     def objective(self, fx, fy) -> float:
-        """J for the fields (torch tensors or arrays); keeps the records for :meth:`gradient`."""
+        """J for the fields (torch tensors or arrays); keeps this evaluation's records (its
+        plane hits, seeds and a stamp of the checkpoints its trace wrote) for
+        :meth:`gradient` as ``self.record``."""
         fxn = fx.detach().cpu().numpy() if hasattr(fx, "detach") else np.asarray(fx)
         fyn = fy.detach().cpu().numpy() if hasattr(fy, "detach") else np.asarray(fy)
         t = self.tracer
@@ -130,19 +148,23 @@ class TraceAdjoint:
         self.result = t.trace(self.particles)
         hits, counts = t.last_records
         value, seeds = plane_objective(hits, counts, self.particles.current, self.plane, self.cylindrical)
-        self._records = (hits, counts, seeds)
+        self._records = self.record = EvaluationRecord(hits, counts, seeds, value, t.checkpoints, t.checkpoint_stamp)
         self.last_value = value
         return value
 
-    def gradient(self, fx, fy, scale: float = 1.0, stream=None, own_workspace: bool = False):
-        """``scale`` dJ/dfx, dJ/dfy (torch tensors) at the fields of the last
-        :meth:`objective`; also sets ``self.dx``, ``self.dv`` (dJ/d initial state).
-        ``own_workspace`` gives the call its own scratch (for calls on several streams)."""
+    def gradient(self, fx, fy, scale: float = 1.0, stream=None, own_workspace: bool = False, record=None):
+        """``scale`` dJ/dfx, dJ/dfy (torch tensors) at the fields ``fx``, ``fy`` of the
+        evaluation ``record`` (default: the last :meth:`objective`); also sets ``self.dx``,
+        ``self.dv`` (dJ/d initial state). ``own_workspace`` gives the call its own scratch (for
+        calls on several streams). The adjoint starts from the checkpoints that evaluation's
+        trace stored while the tracer still holds them; after any later trace it steps every
+        trajectory again from its start (the same gradient, more work)."""
         import torch
 
-        if self._records is None:
+        record = record if record is not None else self._records
+        if record is None:
             raise RuntimeError("objective() first")
-        hits, counts, seeds = self._records
+        hits, counts, seeds = record.hits, record.counts, record.seeds
         t = self.tracer
         mesh = t.mesh
         p = self.particles
@@ -172,12 +194,24 @@ class TraceAdjoint:
                                                          INTEGRATORS.index(t.integrator), geometry_code(mesh), 0)  # fmt: skip
             _lib.check(0 if size > 0 else int(size), "bt_adjoint_workspace_bytes")
             workspace = torch.empty(int(size), dtype=torch.uint8, device=dev)
+        self.dropped = torch.zeros(1, dtype=torch.int64, device=dev)
         self.launch(act, x0, v0, qm, fxd, fyd, planes, hits_d, counts_d, seeds_d, max_step, dfx, dfy, self.dx,
-                    self.dv, stream, workspace=workspace)  # fmt: skip
+                    self.dv, stream, checkpoints=record.checkpoints_current(t, self.interval),
+                    workspace=workspace, dropped=self.dropped)  # fmt: skip
+        if stream is None:
+            self.check_dropped()
         return dfx, dfy
 
+    def check_dropped(self) -> None:
+        """Raise if the last :meth:`gradient` left out a seeded trajectory (its last seeded step
+        beyond ``max_step``: cannot happen with the step taken from the records; waits for
+        the adjoint kernel)."""
+        missing = int(self.dropped.item())
+        if missing:
+            raise RuntimeError(f"the adjoint left out {missing} seeded trajectories (max_step below their records)")
+
     def launch(self, active, x0, v0, qm, fx, fy, planes, hits, counts, seeds, max_step, dfx, dfy, dx=None,
-               dv=None, stream=None, checkpoints=True, workspace=None) -> None:  # fmt: skip
+               dv=None, stream=None, checkpoints=True, workspace=None, dropped=None) -> None:  # fmt: skip
         """One adjoint kernel call on device tensors (asynchronous on ``stream``); with
         ``checkpoints`` it starts from the forward trace's stored states."""
         import torch
@@ -211,6 +245,8 @@ class TraceAdjoint:
         if workspace is not None:
             a.workspace = workspace.data_ptr()
             a.workspace_bytes = workspace.numel()
+        if dropped is not None:
+            a.dropped = dropped.data_ptr()
         st = stream if stream is not None else torch.cuda.current_stream()
         rc = _lib.lib().bt_trace_adjoint_device(ctypes.byref(a), ctypes.c_void_p(st.cuda_stream))
         _lib.check(rc, "bt_trace_adjoint_device")
@@ -237,6 +273,8 @@ def batch_gradients(problems, fields, streams: int = 4, evaluate: bool = True) -
             dfx, dfy = p.gradient(fx, fy, stream=st, own_workspace=True)
         out.append((values[i], dfx, dfy))
     torch.cuda.synchronize()
+    for p in problems:
+        p.check_dropped()
     return out
 
 
@@ -250,13 +288,14 @@ def _objective_function():
         def forward(ctx, fx, fy, problem: TraceAdjoint):
             value = problem.objective(fx, fy)
             ctx.problem = problem
+            ctx.record = problem.record  # this evaluation's records, whatever runs later
             ctx.save_for_backward(fx, fy)
             return torch.tensor(value, dtype=torch.float64, device=fx.device)
 
         @staticmethod
         def backward(ctx, grad):
             fx, fy = ctx.saved_tensors
-            dfx, dfy = ctx.problem.gradient(fx, fy, scale=float(grad))
+            dfx, dfy = ctx.problem.gradient(fx, fy, scale=float(grad), record=ctx.record)
             return dfx, dfy, None
 
     return PlaneObjective

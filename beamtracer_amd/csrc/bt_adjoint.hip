@@ -574,8 +574,11 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
         }
         if (__ballot(k >= 0 || !exhausted) == 0) break;
         i64 last = k >= 0 ? seeded_step(p, k, 0x7fffffffffffffffll) : 0;
-        // a trajectory with no seed, or seeded beyond the checkpoints, adds nothing
+        // a trajectory with no seed, or seeded beyond the checkpoints, adds nothing; the
+        // latter (a max_step below the seeded steps) is counted for the caller
         const bool valid = last > 0 && (last - 1) / K < p.n_ckpt;
+        if (__builtin_expect(last > 0 && !valid, 0) && p.dropped != nullptr)
+            __hip_atomic_fetch_add(p.dropped, (i64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!valid) last = 0;
         const i64 N = p.n;
         const double qm = valid ? p.qm[k] : 0.0;

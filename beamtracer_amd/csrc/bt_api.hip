@@ -993,6 +993,7 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     p.queue = static_cast<u64*>(ws);
     p.scratch = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
     p.dfx = a->dfx; p.dfy = a->dfy; p.dx = a->dx; p.dv = a->dv;
+    p.dropped = a->dropped;
     if (bt::launch_adjoint(p, block, grid, static_cast<hipStream_t>(stream), integ, geom))
         return fail(BT_E_LAUNCH, std::string("adjoint launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;

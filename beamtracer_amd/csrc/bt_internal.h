@@ -101,6 +101,7 @@ struct AdjointParams {
     double* scratch;
     double* dfx; double* dfy; double* dx; double* dv;
     u64* queue;
+    i64* dropped;  // += seeded trajectories beyond the checkpoints (their gradient is missing), or null
 };
 
 int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom);

@@ -364,15 +364,15 @@ def trace_trajectories_2d(
     the base mesh, ``(B_x, B_r, B_theta)`` cylindrical or ``(B_x, B_y, B_z)`` planar
     (None entries are zero). ``block_size`` and ``max_blocks`` set the persistent launch
     (0: defaults); ``exponent`` the fixed-point LSB ``2**exponent`` (default
-    :func:`trace_exponent`); ``refill_block`` refills a workgroup's lanes only when all are
+    :func:`trace_exponent`); ``refill_block`` refills a wave's lanes only when all are
     free (``BT_REFILL_BLOCK``: coherent lanes, whose deposits add up in LDS);
     ``checkpoints`` (device form: a float64 tensor ``(C, 6, N)``) receives ``(x, v)`` of every
     trajectory before each step whose count is a multiple of ``checkpoint_interval``;
     ``record_paths=False`` says that no trajectory records a path (``BT_NO_PATHS``: every
     ``path_slot`` is negative, else those paths go unrecorded) and runs the trace variant
-    without the path code (the host form finds this itself); ``wide_window`` gives the deposit
-    window in LDS 128 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes spread
-    along x). ``noalloc`` (device form, no ``workspace``) calls the allocation-free entry
+    without the path code (the host form finds this itself); ``wide_window`` gives each wave's
+    deposit window in LDS 32 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes
+    spread along x). ``noalloc`` (device form, no ``workspace``) calls the allocation-free entry
     point, whose fixed scratch holds at most ``BT_NOALLOC_MAX_NODES`` mesh nodes. None of these
     changes a result.
     """

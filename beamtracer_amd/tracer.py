@@ -272,6 +272,9 @@ class HipBeamTracer:
         #: device form only: (tensor (C, 6, N), interval) receiving the states every
         #: `interval` steps (the adjoint's checkpoints), or None
         self.checkpoints = None
+        #: a new object for every trace that writes checkpoints: who holds the stamp of the
+        #: trace they ran knows whether the checkpoint buffer still holds its states
+        self.checkpoint_stamp = None
 
     # -- geometry helpers --------------------------------------------------------------
 
@@ -388,6 +391,8 @@ class HipBeamTracer:
     def _trace_beam(self, particles: BeamParticles) -> TraceResult:
         """The node charges (before :meth:`finalize`) and records of a whole trace."""
         mesh = self.mesh
+        if self.device and self.checkpoints is not None:
+            self.checkpoint_stamp = object()
         beam = beam_start(self, particles)
         n = len(particles)
         fxs, fys, patch_int, patch_real = self.patch_arrays()

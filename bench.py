@@ -574,7 +574,7 @@ def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
     fy = torch.from_numpy(np.tensordot(volts, fyk, 1) + fy0).cuda()
     problem = adjoint.TraceAdjoint(tracer, design.particles, interval=interval)
     value = problem.objective(fx, fy)
-    hits, counts, seeds = problem._records
+    hits, counts, seeds = problem.record.hits, problem.record.counts, problem.record.seeds
     stream = torch.cuda.current_stream()
     # forward: the plain trace of every trajectory (inputs in HBM, as the headline)
     host = kernel_arguments(design, tracer)

@@ -52,18 +52,18 @@ extern "C" {
 #define BT_E_DEVICE (-4)     /* no usable HIP device                                   */
 #define BT_E_UNSUPPORTED (-5)
 
-/* flags of a trace: refill a workgroup's lanes only when all of them are free, so that
- * the lanes of a workgroup start their trajectories together and stay in the same cells
- * (their deposits then add up in LDS; the lanes of trajectories that end early wait).
+/* flags of a trace: refill a wave's lanes only when all of them are free, so that the lanes
+ * of a wave start their trajectories together and stay in the same cells (their deposits
+ * then add up in the wave's LDS window; the lanes of trajectories that end early wait).
  * Results do not depend on it. */
 #define BT_REFILL_BLOCK 1
 /* flags of a trace: no trajectory records a path (every path_slot is negative); the library
  * then runs a variant without the path code (the host form finds this itself).  A path slot
  * that is not negative is ignored under this flag. */
 #define BT_NO_PATHS 2
-/* flags of a trace: the deposit window in LDS spans 128 mesh columns instead of 16 (faster for
- * beams whose lanes spread along x; results unchanged; ignored with refined patches, a
- * graded axis or relativistic motion). */
+/* flags of a trace: each wave's deposit window in LDS spans 32 mesh columns instead of 16
+ * (faster for beams whose lanes spread along x; results unchanged; ignored with refined
+ * patches, a graded axis or relativistic motion). */
 #define BT_WIDE_WINDOW 4
 
 /* Every argument of trace_trajectories_2d (trace_trajectories_2d_pyccel.py:42-78) as a
@@ -103,7 +103,8 @@ typedef struct bt_trace2d_args {
     int32_t block_size;         /* threads per block (64..256, a multiple of 64); 0 = default 256 */
     int32_t max_blocks;         /* persistent grid size; 0 = fill the device            */
     int32_t flags;              /* BT_REFILL_BLOCK | BT_NO_PATHS | BT_WIDE_WINDOW, or 0 */
-    void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned */
+    void* workspace;            /* device scratch of bt_trace_workspace_bytes(); NULL = library-owned,
+                                   shared by every call on the device: one call at a time */
     int64_t workspace_bytes;
     /* extension: the state (x, v) of every trajectory before each step whose count is a
      * multiple of checkpoint_interval, into checkpoints[c][6][n] for c < n_checkpoints
@@ -201,7 +202,8 @@ typedef struct bt_trace3d_args {
     int64_t has_j; double* current; int64_t current_rows, current_cols;
     double inv_c2;
     int32_t deposit_exponent, current_exponent;
-    void* workspace; int64_t workspace_bytes;  /* device scratch (NULL: the library's own) */
+    void* workspace; int64_t workspace_bytes;  /* device scratch (NULL: the library's own, shared by
+                                                  every call on the device: one call at a time) */
 } bt_trace3d_args;
 
 /* sizeof(bt_trace3d_args), for binding checks. */
@@ -329,6 +331,10 @@ typedef struct bt_adjoint_args {
     void* workspace; int64_t workspace_bytes;     /* device scratch of bt_adjoint_workspace_bytes()
                                                      (calls on several streams need their own);
                                                      NULL: library-owned, one call at a time */
+    int64_t* dropped;                             /* device, or NULL: += the number of seeded
+                                                     trajectories whose last seeded step is
+                                                     beyond max_step (their gradients are
+                                                     missing: max_step was too small) */
 } bt_adjoint_args;
 
 /* Measurement helper (no counterpart in the reference): `blocks` x 256 lanes each make

@@ -50,7 +50,7 @@ def test_adjoint_transposes_the_cpu_tangent(integrator, planar):
     assert np.isfinite(value)
     dfx, dfy = problem.gradient(fx, fy)
     torch.cuda.synchronize()
-    hits, counts, seeds = problem._records
+    hits, counts, seeds = problem.record.hits, problem.record.counts, problem.record.seeds
     assert (np.abs(seeds).sum(-1) > 0).sum() > 200
     rng = np.random.default_rng(5)
     for _ in range(2):
@@ -128,3 +128,66 @@ def test_batch_gradients_equal_single_gradients():
         for a, b in ((gx, sx), (gy, sy)):
             a, b = a.cpu().numpy(), b.cpu().numpy()
             np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13 * np.abs(b).max())
+
+
+def test_each_evaluation_keeps_its_own_records_for_its_backward():
+    """Two forwards on different fields before one backward, and another trace (with other
+    checkpoints) between an objective and its gradient: every gradient is that of its own
+    evaluation (the later traces overwrite the checkpoint buffer, and the adjoint then steps
+    the trajectories again from their start)."""
+    import torch
+
+    d, tracer, field = _lens(particles=200)
+    problem = adjoint.TraceAdjoint(tracer, d.particles)
+    f1 = (torch.from_numpy(field.fx).cuda(), torch.from_numpy(field.fy).cuda())
+    f2 = (f1[0] * 1.01, f1[1] * 0.99)
+    single = []
+    for fx, fy in (f1, f2):
+        problem.objective(fx, fy)
+        single.append(problem.gradient(fx, fy))
+    torch.cuda.synchronize()
+    # autograd: J(f1) + J(f2), one backward
+    leaves = [(fx.clone().requires_grad_(True), fy.clone().requires_grad_(True)) for fx, fy in (f1, f2)]
+    total = sum(adjoint.plane_objective_torch(fx, fy, problem) for fx, fy in leaves)
+    total.backward()
+    for (fx, fy), (gx, gy) in zip(leaves, single):
+        scale = float(gx.abs().max())
+        assert float((fx.grad - gx).abs().max()) <= 1e-12 * scale
+        assert float((fy.grad - gy).abs().max()) <= 1e-12 * float(gy.abs().max())
+    # a second problem with another interval traces through the same tracer in between
+    problem.objective(*f1)
+    record = problem.record
+    other = adjoint.TraceAdjoint(tracer, d.particles, interval=32)
+    other.objective(*f2)
+    assert not record.checkpoints_current(tracer, problem.interval)
+    gx, gy = problem.gradient(*f1, record=record)
+    torch.cuda.synchronize()
+    assert float((gx - single[0][0]).abs().max()) <= 1e-12 * float(single[0][0].abs().max())
+    assert float((gy - single[0][1]).abs().max()) <= 1e-12 * float(single[0][1].abs().max())
+
+
+def test_a_max_step_below_the_seeded_records_is_counted_not_silent():
+    import torch
+
+    d, tracer, field = _lens(particles=100)
+    problem = adjoint.TraceAdjoint(tracer, d.particles)
+    fx = torch.from_numpy(field.fx).cuda()
+    fy = torch.from_numpy(field.fy).cuda()
+    problem.objective(fx, fy)
+    problem.gradient(fx, fy)  # the step from the records: nothing left out
+    assert int(problem.dropped.item()) == 0
+    r = problem.record
+    dev = torch.device("cuda")
+    seeded = np.flatnonzero(np.abs(r.seeds).sum(axis=(1, 2, 3)) > 0)
+    p = d.particles
+    dropped = torch.zeros(1, dtype=torch.int64, device=dev)
+    dfx, dfy = torch.zeros_like(fx), torch.zeros_like(fy)
+    problem.launch(torch.from_numpy(seeded.astype(np.int64)).to(dev),
+                   torch.from_numpy(np.ascontiguousarray(p.position)).to(dev),
+                   torch.from_numpy(np.ascontiguousarray(p.velocity)).to(dev),
+                   torch.from_numpy(np.ascontiguousarray(p.charge / p.mass)).to(dev), fx, fy,
+                   torch.tensor(tracer.planes, dtype=torch.float64, device=dev), torch.from_numpy(r.hits).to(dev),
+                   torch.from_numpy(r.counts).to(dev), torch.from_numpy(r.seeds).to(dev), 10, dfx, dfy,
+                   checkpoints=False, dropped=dropped)  # fmt: skip
+    torch.cuda.synchronize()
+    assert int(dropped.item()) == seeded.size  # every record lies beyond step 10
