@@ -213,6 +213,7 @@ SIGNATURES = {
     "bt_adjoint_workspace_bytes": (_i64, [_i64, _i64, _i32, _i64, _i64, _i32]),
     "bt_arith_selftest": (ctypes.c_int, [_ptr, _ptr, _i64, _ptr, _ptr, _ptr, _ptr, _ptr]),
     "bt_l2_gather_bandwidth": (ctypes.c_int, [_ptr, _i64, _i64, _i32, _i32, _ptr, _ptr]),
+    "bt_l2_line_bandwidth": (ctypes.c_int, [_ptr, _i64, _i32, _i32, _ptr, _ptr]),
     "bt_trace_adjoint_device": (ctypes.c_int, [ctypes.POINTER(AdjointArgs), _ptr]),
     "bt_batch_args_size": (_i64, []),
     "bt_batch_design_size": (_i64, []),

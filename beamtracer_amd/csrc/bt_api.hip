@@ -911,6 +911,16 @@ int bt_l2_gather_bandwidth(const double* field, int64_t rows, int64_t cols, int3
     return BT_OK;
 }
 
+int bt_l2_line_bandwidth(const double* field, int64_t n_lines, int32_t lines, int32_t blocks, double* sink,
+                         void* stream) {
+    if (field == nullptr || sink == nullptr || n_lines < 1 || lines < 1 || blocks < 1)
+        return fail(BT_E_ARG, "bad arguments");
+    if (n_lines >= ((i64)1 << 24)) return fail(BT_E_ARG, "the field must hold fewer than 2^24 lines");
+    if (bt::launch_l2_line(field, n_lines, lines, blocks, 0x1234u, sink, static_cast<hipStream_t>(stream)))
+        return fail(BT_E_LAUNCH, std::string("line launch: ") + hipGetErrorString(hipGetLastError()));
+    return BT_OK;
+}
+
 int bt_arith_selftest(const double* y, const double* z, int64_t n, double* r, double* cy, double* cz, double* rs,
                       void* stream) {
     if (n < 0 || (n > 0 && (y == nullptr || z == nullptr || r == nullptr || cy == nullptr || cz == nullptr ||

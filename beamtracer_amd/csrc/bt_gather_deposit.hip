@@ -145,6 +145,32 @@ __global__ __launch_bounds__(256) void l2_gather_kernel(const double* field, uns
     sink[lane] = acc;
 }
 
+// Whole-line reads of a field resident in L2: each group of 8 lanes reads one 128-byte line
+// (16 bytes a lane) at a random line of the field, `lines` times, so every byte that L2 sends
+// is used: the L2 bandwidth in useful bytes for reads that reuse nothing in L1.
+__global__ __launch_bounds__(256) void l2_line_kernel(const double* field, unsigned n_lines, int lines, unsigned seed,
+                                                      double* __restrict__ sink) {
+    const Rsrc f = field_rsrc(field, (i64)n_lines * 16);
+    const unsigned lane = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned state = hash32((lane >> 3) ^ seed) | 1u;  // one stream of lines per group of 8 lanes
+    const unsigned part = (lane & 7u) * 2u;               // this lane's two doubles of the line
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < lines; ++k) {
+        state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+        const unsigned line = __umulhi(state, n_lines);
+        const Pair q = load_pair(f, line * 16u + part);
+        acc += q.a + q.b;
+    }
+    sink[lane] = acc;
+}
+
+int launch_l2_line(const double* f, i64 n_lines, int lines, int blocks, unsigned seed, double* sink,
+                   hipStream_t stream) {
+    hipLaunchKernelGGL(l2_line_kernel, dim3(blocks), dim3(256), 0, stream, f, (unsigned)n_lines, lines, seed, sink);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 int launch_l2_gather(const double* f, i64 rows, i64 cols, int lookups, int blocks, unsigned seed, double* sink,
                      hipStream_t stream) {
     hipLaunchKernelGGL(l2_gather_kernel, dim3(blocks), dim3(256), 0, stream, f, (unsigned)rows, (unsigned)cols,

@@ -118,6 +118,8 @@ int launch_gather(const double* px, const double* py, i64 n, const double* fx, c
                   hipStream_t stream);
 int launch_l2_gather(const double* f, i64 rows, i64 cols, int lookups, int blocks, unsigned seed, double* sink,
                      hipStream_t stream);
+int launch_l2_line(const double* f, i64 n_lines, int lines, int blocks, unsigned seed, double* sink,
+                   hipStream_t stream);
 int launch_deposit(const double* px, const double* py, const double* q, i64 n, double ox,
                    double oy, double h, i64 cylindrical, double* charge, i64 nx, i64 ny, int e,
                    u64* acc, hipStream_t stream);

@@ -35,7 +35,7 @@ Beside the headline the line carries, under ``extra``:
   their ratio, and dJ/dV of the middle tube against a central difference.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N] [--no-cpu]
-                       [--no-extra]
+                       [--no-extra] [--legs einzel_full,extraction,vlasov,batched,trace3d,adjoint]
 """
 
 from __future__ import annotations
@@ -64,6 +64,8 @@ FLOP_PER_STEP_RK4_CYL = 322
 BYTES_PER_STEP_RK4_CYL = 328
 FLOP_PER_STEP_BORIS_B_CYL = 257
 PEAK_FP64_TFLOPS = 78.6
+#: the legs of the line's `extra` object, in the order they run
+LEGS = ("einzel_full", "extraction", "vlasov", "batched", "trace3d", "adjoint")
 
 
 def log(message: str) -> None:
@@ -108,12 +110,13 @@ def kernel_arguments(design, tracer, capacity: int = 4):
     ]  # fmt: skip
 
 
-def cpu_baseline(args_host, target_seconds: float) -> dict:
-    """The oracle's OpenMP tracer on a strided sample, timed on this host's cores."""
+def cpu_baseline(args_host, target_seconds: float, threads: int | None = None) -> dict:
+    """The oracle's OpenMP tracer on a strided sample, timed on this host's cores (or on
+    ``threads`` of them)."""
     import oracle
     from oracle import cases
 
-    threads = len(os.sched_getaffinity(0))
+    threads = threads or len(os.sched_getaffinity(0))
     n = args_host[1].shape[1]
 
     def run(count: int) -> tuple[float, int]:
@@ -151,6 +154,41 @@ def cpu_baseline(args_host, target_seconds: float) -> dict:
         "sample": f"{count} of {n} trajectories (evenly strided), {steps} particle-steps in {seconds:.2f} s; "
         f"oracle/oracle.c (restatement of trace_trajectories_2d_pyccel.py), OpenMP static(1) on {threads} "
         f"threads of {model or 'the host CPU'}",
+    }
+
+
+def cpu_vlasov_iteration(design, iterations: int = 3) -> dict:
+    """Seconds of one Vlasov iteration of the einzel lens on the CPU: the host Poisson solve and
+    edge fields with the oracle's OpenMP tracer (every host core) as the kernel; the last of
+    ``iterations`` (the first also factorizes and builds the lookups)."""
+    import oracle
+    from beamtracer_amd.tracer import HipBeamTracer
+    from beamtracer_amd.vlasov import vlasov_iterate
+
+    threads = len(os.sched_getaffinity(0))
+
+    class OracleKernelTracer(HipBeamTracer):
+        def _launch(self, call, magnetic, exponent):
+            args = list(call)
+            slabs = np.zeros((threads, call[31].shape[1]))
+            args[30], args[31] = threads, slabs
+            oracle.trace_trajectories_2d(*args, magnetic=magnetic)
+            call[31][0] += slabs.sum(0)
+
+    tracer = OracleKernelTracer(design.mesh, None, planes=design.planes, integrator=design.integrator)
+    state = vlasov_iterate(design.mesh, lambda: (design.particles, design.j_over_v), tracer,
+                           iterations=iterations, relaxation=0.5)  # fmt: skip
+    last = state.history[-1]
+    seconds = last["field_solve_s"] + last["edge_field_s"] + last["trace_s"]
+    return {
+        "seconds": seconds,
+        "iterations_per_s": 1.0 / seconds,
+        "field_solve_s": last["field_solve_s"],
+        "edge_field_s": last["edge_field_s"],
+        "trace_s": last["trace_s"],
+        "cores": threads,
+        "workload": f"einzel lens Vlasov iteration, {len(design.particles)} trajectories: host Poisson + NumPy edge "
+        f"fields + the oracle tracer on {threads} threads (iteration {iterations} of {iterations})",
     }
 
 
@@ -561,6 +599,36 @@ def l2_gather_bandwidth(rows: int = 1603, cols: int = 201, lookups: int = 256, r
     return 32.0 * blocks * 256 * lookups / float(np.median(times))
 
 
+def l2_line_bandwidth(n_bytes: int = 2_580_000, lines: int = 256, repeats: int = 10) -> float:
+    """Bytes/s of whole 128-byte lines read at random from an L2-resident field of ``n_bytes``
+    (by default the einzel lens's fx + fy), 2048 x 256 lanes in groups of 8 (16 bytes a lane):
+    the L2 bandwidth of reads that reuse nothing in L1, the L2 term of the roofline; the median
+    of ``repeats`` launches timed with HIP events."""
+    import torch
+
+    from beamtracer_amd import _lib
+
+    n_lines = n_bytes // 128
+    field = torch.ones(n_lines * 16, dtype=torch.float64, device="cuda")
+    blocks = 2048
+    sink = torch.zeros(blocks * 256, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    L = _lib.lib()
+    times = []
+    for r in range(repeats + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        rc = L.bt_l2_line_bandwidth(field.data_ptr(), n_lines, lines, blocks, sink.data_ptr(),
+                                    ctypes.c_void_p(stream.cuda_stream))  # fmt: skip
+        b.record(stream)
+        _lib.check(rc, "bt_l2_line_bandwidth")
+        torch.cuda.synchronize()
+        if r:
+            times.append(a.elapsed_time(b) / 1e3)
+    assert float(sink.sum()) == 2.0 * blocks * 256 * lines  # every lane read two ones per line
+    return 128.0 * blocks * 256 / 8 * lines / float(np.median(times))
+
+
 def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
     """One adjoint gradient of J = rms_angle + emittance at 75 mm against one forward trace."""
     import torch
@@ -645,6 +713,8 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--no-extra", action="store_true", help="only the einzel-lens headline")
+    parser.add_argument("--legs", default=",".join(LEGS),
+                        help=f"comma-separated legs of `extra` to run (of {','.join(LEGS)})")
     parser.add_argument("--extraction-particles", type=int, default=1_000_000)
     parser.add_argument("--batch-designs", type=int, default=256)
     parser.add_argument("--calibrate-copy", action="store_true",
@@ -679,20 +749,28 @@ def main() -> int:
     log(f"timed: {opts.steps} traces in {wall:.3f} s; launch avg {avg_launch_s * 1e3:.3f} ms")
 
     achieved = FLOP_PER_STEP_RK4_CYL * steps_per_trace / avg_launch_s / 1e12
-    b_l2 = l2_gather_bandwidth()
+    b_l2 = l2_line_bandwidth()
+    b_gather = l2_gather_bandwidth()
     steps_per_s = steps_per_trace / avg_launch_s
     # the roofline of the path in particle-steps/s: min(FP64 peak / FLOP per step, measured L2
-    # gather bandwidth / algorithmic bytes per step); when the trace moves more algorithmic bytes
-    # than random gathers can draw from L2 (its lanes reuse L1 lines), the L2 term does not bind
+    # line bandwidth / algorithmic bytes per step); when the trace moves more algorithmic bytes
+    # than whole lines can come from L2 (its lanes reuse L1 lines), the L2 term does not bind
     # and the FP64 roof alone is the roof
     fp64_roof = PEAK_FP64_TFLOPS * 1e12 / FLOP_PER_STEP_RK4_CYL
     l2_roof = b_l2 / BYTES_PER_STEP_RK4_CYL
     roof_steps = fp64_roof if steps_per_s > l2_roof else min(fp64_roof, l2_roof)
-    log(f"L2 gather bandwidth {b_l2 / 1e9:.0f} GB/s; roof {roof_steps:.3e} steps/s")
-    traffic = None
+    log(f"L2 line bandwidth {b_l2 / 1e9:.0f} GB/s (random 16-byte gathers {b_gather / 1e9:.0f} GB/s); "
+        f"roof {roof_steps:.3e} steps/s")
+    # HBM bytes per launch: not measurable from inside the run (no counters without the
+    # profiler), so the committed rocprofv3 FETCH_SIZE / WRITE_SIZE measurement is reported
+    # with its provenance
+    traffic, traffic_note = None, None
     traffic_file = ROOT / "profiles" / "trace_kernel_einzel_traffic.json"
     if traffic_file.exists():
-        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
+        measured = json.loads(traffic_file.read_text())
+        traffic = measured.get("hbm_bytes_per_launch")
+        traffic_note = (f"committed rocprofv3 --pmc measurement ({traffic_file.relative_to(ROOT)}, "
+                        f"{measured.get('measured', 'undated')}), not measured in this run")
     line = {
         "metric": METRIC,
         "value": value,
@@ -715,13 +793,14 @@ def main() -> int:
             "geometry": "cylindrical",
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64",
             "pipe": "fp64 vector (no MFMA: nothing here is a contraction; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": traffic,
+            "traffic_source": traffic_note,
             "flop_per_step": FLOP_PER_STEP_RK4_CYL,
             "launch_ms": avg_launch_s * 1e3,
             "l2": {
@@ -730,34 +809,50 @@ def main() -> int:
                 "unit": "GB/s",
                 "frac": steps_per_s * BYTES_PER_STEP_RK4_CYL / b_l2,
                 "bytes_per_step": BYTES_PER_STEP_RK4_CYL,
-                "note": "algorithmic field/deposit/lookup bytes per step against the measured bandwidth of random "
-                "bilinear row-pair gathers from a 2.58 MB (L2-resident) field; frac > 1: the trace's gathers reuse "
-                "L1 lines, so the FP64 roof alone bounds it",
+                "random_gather_measured": b_gather / 1e9,
+                "note": "algorithmic field/deposit/lookup bytes per step against the measured L2 bandwidth of whole "
+                "128-byte lines read at random from a 2.58 MB (L2-resident) field; frac > 1 would mean the trace's "
+                "gathers reuse L1 lines, and the FP64 roof alone bounds it (random_gather_measured: 16-byte bilinear "
+                "row-pair gathers, each filling a whole line)",
             },
             "roof_steps_per_s": roof_steps,
             "frac_of_roof": steps_per_s / roof_steps,
         },
     }
-    if not opts.no_extra:
+    legs = [] if opts.no_extra else [leg for leg in opts.legs.split(",") if leg]
+    unknown = set(legs) - set(LEGS)
+    if unknown:
+        print(f"unknown legs {sorted(unknown)} (of {LEGS})", file=sys.stderr)
+        return 2
+    extra = {}
+    processes = min(16, len(os.sched_getaffinity(0)))
+    if "einzel_full" in legs:
         log("einzel lens with 1e6 trajectories (the GPU full)")
-        extra = {"einzel_full": full_load_leg(1_000_000, max(2, opts.steps // 3))}
+        extra["einzel_full"] = full_load_leg(1_000_000, max(2, opts.steps // 3))
+    if "extraction" in legs:
         log("extraction: Boris + node B, 1e6 trajectories")
         extra["extraction"] = extraction_leg(opts.extraction_particles, max(2, opts.steps // 2), opts.block_size)
+    if "vlasov" in legs:
         log("vlasov: einzel-lens iterations")
         extra["vlasov"] = vlasov_leg(design, 4)
+    if "batched" in legs:
         log(f"batched: {opts.batch_designs} einzel candidates")
-        processes = min(16, len(os.sched_getaffinity(0)))
         extra["batched"] = batched_leg(opts.batch_designs, 0, processes, 3)
+    if "trace3d" in legs:
         log("trace3d: 3D quarter lens, 1e5 trajectories")
         extra["trace3d"] = trace3d_leg(100_000, 3, 0 if opts.no_cpu else 4000)
+    if "adjoint" in legs:
         log("adjoint: einzel lens, 2e5 trajectories")
         extra["adjoint"] = adjoint_leg(200_000, 3)
         log("adjoint: 32 einzel candidates")
         extra["adjoint"]["batched_gradients"] = batched_gradients_leg(32, 1, processes)
+    if extra:
         line["extra"] = extra
     if not opts.no_cpu:
-        log("cpu baseline: oracle tracer on the host cores")
+        log("cpu baseline: oracle tracer on the host cores, then one core, then a CPU Vlasov iteration")
         line["cpu_baseline"] = cpu_baseline(host, opts.cpu_seconds)
+        line["cpu_baseline"]["single_core"] = cpu_baseline(host, opts.cpu_seconds / 2, threads=1)
+        line["cpu_baseline"]["vlasov_iteration"] = cpu_vlasov_iteration(design)
     print(json.dumps(line), flush=True)
     return 0
 

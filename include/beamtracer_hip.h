@@ -344,6 +344,13 @@ typedef struct bt_adjoint_args {
  * pattern. */
 int bt_l2_gather_bandwidth(const double* field, int64_t rows, int64_t cols, int32_t lookups, int32_t blocks,
                            double* sink, void* stream);
+/* Measurement helper (no counterpart in the reference): `blocks` x 256 lanes, in groups of 8,
+ * read `lines` random whole 128-byte lines each of the device field of n_lines lines
+ * (n_lines * 16 doubles, < 2^24 lines), 16 bytes a lane, and write one sum per lane to
+ * sink[blocks * 256]: timed on `stream`, the L2 bandwidth in useful bytes of reads that find
+ * nothing in L1 (the L2 term of the tracer's roofline). */
+int bt_l2_line_bandwidth(const double* field, int64_t n_lines, int32_t lines, int32_t blocks, double* sink,
+                         void* stream);
 
 /* Self-test of the kernels' square-root arithmetic (no counterpart in the reference): for
  * device arrays of n doubles, the radial distance r[k] = |(y[k], z[k])| and direction

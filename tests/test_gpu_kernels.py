@@ -297,6 +297,26 @@ def test_l2_gather_measurement_reads_every_lookup():
     assert L.bt_l2_gather_bandwidth(field.data_ptr(), 1 << 14, 1 << 14, 1, 1, sink.data_ptr(), None) < 0
 
 
+def test_l2_line_measurement_reads_whole_lines():
+    """The line-granular L2 measurement: each lane adds its two doubles of every line it reads,
+    so on a field whose value is the index within its line, the 8 lanes of a group together
+    read each line whole (the lane sums of a group add up to lines x sum(0..15))."""
+    import torch
+
+    from beamtracer_amd import _lib
+
+    n_lines, lines, blocks = 1611, 32, 8
+    field = torch.arange(16, dtype=torch.float64, device="cuda").repeat(n_lines)
+    sink = torch.zeros(blocks * 256, dtype=torch.float64, device="cuda")
+    L = _lib.lib()
+    assert L.bt_l2_line_bandwidth(field.data_ptr(), n_lines, lines, blocks, sink.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    per_lane = sink.cpu().numpy().reshape(-1, 8)
+    np.testing.assert_array_equal(per_lane.sum(1), lines * 120.0)
+    np.testing.assert_array_equal(per_lane[:, 3], lines * (6.0 + 7.0))  # lane 3: doubles 6 and 7
+    assert L.bt_l2_line_bandwidth(field.data_ptr(), 1 << 24, 1, 1, sink.data_ptr(), None) < 0
+
+
 def test_trajectories_resting_in_one_cell_deposit_large_lane_sums(hip):
     """Trajectories at rest in a zero field stay in one cell for 20,000 steps: each lane's sum
     for a node grows to about 2^54 fixed-point units, past the 2^51 of the rounding by
