@@ -60,9 +60,9 @@ __device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F
     }
     const double sx = (q.a - p.origin_x) * p.inv_h;
     const double sy = (qy - p.origin_y) * p.inv_h;
-    const double ex = clamped_bilinear_buf(F.fx, 0u, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
-    const double ey = clamped_bilinear_buf(F.fy, 0u, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
-    if (GEOM != 0) return {qm * ex, qm * (ey * cy), qm * (ey * cz)};
+    const double ex = clamped_bilinear_buf<true>(F.fx, 0u, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const double ey = clamped_bilinear_buf<true>(F.fy, 0u, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    if (GEOM != 0) return {qm * ex, (qm * ey) * cy, (qm * ey) * cz};
     return {qm * ex, qm * ey, qm * 0.0};
 }
 
@@ -77,11 +77,11 @@ struct Gathered {
 
 __device__ __forceinline__ Gathered gather_d(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
     Gathered g;
-    g.i = clamped_floor_index(s, imax);
-    g.j = clamped_floor_index(t, jmax);
+    g.i = clamped_floor_index<true>(s, imax);
+    g.j = clamped_floor_index<true>(t, jmax);
     g.u = s - (double)g.i;
     g.w = t - (double)g.j;
-    g.c = (unsigned)g.i * (unsigned)cols + (unsigned)g.j;
+    g.c = mad24((unsigned)g.i, (unsigned)cols, (unsigned)g.j);
     const Pair lo = load_pair(f, g.c), hi = load_pair(f, g.c + (unsigned)cols);
     const double f00 = lo.a, f01 = lo.b, f10 = hi.a, f11 = hi.b;
     // (clamped_bilinear_buf's arithmetic)
@@ -295,7 +295,7 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     const double sy = (qy - p.origin_y) * p.inv_h;
     const Gathered gx = gather_d(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
     const Gathered gy = gather_d(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
-    if (GEOM != 0) g = {qm * gx.value, qm * (gy.value * cy), qm * (gy.value * cz)};
+    if (GEOM != 0) g = {qm * gx.value, (qm * gy.value) * cy, (qm * gy.value) * cz};
     else g = {qm * gx.value, qm * gy.value, qm * 0.0};
     const double lex = qm * lg.a;
     double ley, lcy = 0.0, lcz = 0.0;

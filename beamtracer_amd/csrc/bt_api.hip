@@ -38,6 +38,8 @@ int hip_fail(hipError_t err, const char* what) {
 constexpr int kMaxPatches = 16;
 constexpr int kMaxPlanes = 256;
 constexpr i64 kMax32 = (i64)1 << 31;  // bound of the 32-bit launch parameters
+// bound of a mesh's nodes and cells per axis: row offsets are formed with 24-bit multiplies
+constexpr i64 kMaxAxis = ((i64)1 << 24) - 1;
 constexpr size_t kTableBytes = 4096;
 
 size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -170,6 +172,7 @@ int check_trace(const bt_trace2d_args* a) {
         if (pi[2] < 0 || pi[2] + (nx + 1) * ny > a->n_fxs) return fail(BT_E_ARG, "fx offset beyond fxs");
         if (pi[3] < 0 || pi[3] + nx * (ny + 1) > a->n_fys) return fail(BT_E_ARG, "fy offset beyond fys");
         if ((nx + 1) * (ny + 1) >= ((i64)1 << 29)) return fail(BT_E_ARG, "a patch holds 2^29 nodes or more");
+        if (nx >= kMaxAxis || ny >= kMaxAxis) return fail(BT_E_ARG, "a patch has 2^24 - 1 nodes per axis or more");
         if (pi[5] < 0 || pi[5] + nx * ny > a->charge_cols)
             return fail(BT_E_ARG, "node offset beyond charge");
         if (pi[5] + nx * ny > nodes) nodes = pi[5] + nx * ny;
@@ -179,6 +182,7 @@ int check_trace(const bt_trace2d_args* a) {
     if (a->cells_rows < base[0] - 1 || a->cells_cols < base[1] - 1)
         return fail(BT_E_ARG, "cells must be at least (nx - 1, ny - 1)");
     if (a->cells_rows * a->cells_cols >= ((i64)1 << 31)) return fail(BT_E_ARG, "cells holds 2^31 entries or more");
+    if (a->cells_cols >= kMaxAxis) return fail(BT_E_ARG, "cells has 2^24 - 1 columns or more");
     if (a->n_subcells < 1) return fail(BT_E_ARG, "subcells is empty");
     if (a->charge_rows < 1) return fail(BT_E_ARG, "charge has no rows");
     return BT_OK;
@@ -334,6 +338,8 @@ int check_designs(const bt_batch_design* designs, i64 n_designs, i64 n_fxs, i64 
         const std::string at = "design " + std::to_string(d) + ": ";
         if (b.nx < 2 || b.ny < 2) return fail(BT_E_ARG, at + "fewer than 2 nodes per axis");
         if ((b.nx + 1) * (b.ny + 1) >= ((i64)1 << 29)) return fail(BT_E_ARG, at + "2^29 nodes or more");
+        if (b.nx >= kMaxAxis || b.ny >= kMaxAxis || b.cells_cols >= kMaxAxis)
+            return fail(BT_E_ARG, at + "2^24 - 1 nodes or cells per axis or more");
         if (!(b.h > 0.0)) return fail(BT_E_ARG, at + "h must be positive");
         if (b.fx_offset < 0 || b.fx_offset + (b.nx + 1) * b.ny > n_fxs) return fail(BT_E_ARG, at + "fx beyond fxs");
         if (b.fy_offset < 0 || b.fy_offset + b.nx * (b.ny + 1) > n_fys) return fail(BT_E_ARG, at + "fy beyond fys");
@@ -955,6 +961,7 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     if (a->n < 0 || a->n_active < 0) return fail(BT_E_ARG, "negative trajectory count");
     if (a->nx < 2 || a->ny < 2) return fail(BT_E_ARG, "fewer than 2 nodes per axis");
     if ((a->nx + 1) * (a->ny + 1) >= ((i64)1 << 29)) return fail(BT_E_ARG, "2^29 nodes or more");
+    if (a->nx >= kMaxAxis || a->ny >= kMaxAxis) return fail(BT_E_ARG, "2^24 - 1 nodes per axis or more");
     if (!(a->h > 0.0)) return fail(BT_E_ARG, "h must be positive");
     if (a->cylindrical < 0 || a->cylindrical > 2) return fail(BT_E_ARG, "cylindrical must be 0, 1 or 2");
     if (a->integrator < 0 || a->integrator > 1) return fail(BT_E_ARG, "integrator must be 0 or 1");

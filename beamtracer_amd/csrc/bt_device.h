@@ -71,21 +71,26 @@ __device__ __forceinline__ double clamped_bilinear(const double* __restrict__ f,
 
 // ---- square roots through the hardware reciprocal square root -------------------------
 // The reference's divisions by |v|, r and sqrt(|a|) and its hypot become products with
-// 1 / sqrt(x): the hardware estimate y (v_rsq_f64) and two Newton steps
-// y <- y + y (1/2 - x y^2 / 2), each squaring the relative error, leave a few units in the
-// last place -- far inside the 1e-10 relative bar of the trace; bt_arith_selftest measures
-// it on the GPU.
+// 1 / sqrt(x): the hardware estimate y (v_rsq_f64, relative error below 2^-23) and one
+// third-order correction.  With e = 1 - x y^2 the exact value is y (1 - e)^(-1/2) =
+// y (1 + e/2 + 3e^2/8 + ...); the truncation leaves about 5/16 e^3 < 2^-66, and the rounding
+// of the five operations about one unit in the last place -- far inside the 1e-10 relative bar
+// of the trace; bt_arith_selftest measures it on the GPU.
 
 // 1 / sqrt(x) for a positive normal x of at least 2^-1000.
 __device__ __forceinline__ double rsqrt_nr(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = 0.5 * y;
-    double r = __builtin_fma(-g, h, 0.5);
-    y = __builtin_fma(y, r, y);
-    g = x * y;
-    h = 0.5 * y;
-    r = __builtin_fma(-g, h, 0.5);
-    return __builtin_fma(y, r, y);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = __builtin_fma(-(x * y), y, 1.0);
+    const double p = __builtin_fma(e, 0.375, 0.5);
+    return __builtin_fma(y * e, p, y);
+}
+
+// 1 / d for a positive normal d: the hardware estimate r (v_rcp_f64) and one third-order
+// correction, r (1 + e + e^2) with e = 1 - d r (truncation about e^3 < 2^-66).
+__device__ __forceinline__ double recip_nr(double d) {
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, __builtin_fma(e, e, e), r);
 }
 
 // 1 / sqrt(x) for any positive finite x (tiny ones scaled by an exact power of two).
@@ -215,21 +220,38 @@ __device__ __forceinline__ int trunc_sat_i32(double s) {
 
 // floor(s) clamped to [0, hi] (hi >= 0), and the same for a NaN s (0): trunc equals floor
 // from 0 up, and everything below 0 clamps to 0 either way -- one conversion and one
-// integer median (v_med3_i32) instead of floor, two double clamps and a conversion.
+// integer median instead of floor, two double clamps and a conversion.  UNIFORM: `hi` is the
+// same in every lane (a scalar register), and the median is the single v_med3_i32 (the
+// compiler forms it only for constant bounds).
+template <bool UNIFORM = false>
 __device__ __forceinline__ int clamped_floor_index(double s, int hi) {
-    return min(max(trunc_sat_i32(s), 0), hi);
+    const int t = trunc_sat_i32(s);
+    if (UNIFORM) {
+        int r;
+        asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(t), "s"(hi));
+        return r;
+    }
+    return min(max(t, 0), hi);
+}
+
+// a * b + c for a, b below 2^24 (v_mad_u32_u24, full rate; a 32-bit multiply is quarter rate)
+__device__ __forceinline__ unsigned mad24(unsigned a, unsigned b, unsigned c) {
+    return __umul24(a, b) + c;
 }
 
 // clamped_bilinear of the grid that starts at element `base` of buffer r, for the tracer:
 // the cell clamped as an integer (the same cell and fractions as clamping floor(s) in
-// double), the three interpolations fused (fma: one rounding each instead of two).
+// double), the three interpolations fused (fma: one rounding each instead of two).  Rows and
+// columns stay below 2^24 (the entry points check it).  UNIFORM: the bounds are the same in
+// every lane.
+template <bool UNIFORM = false>
 __device__ __forceinline__ double clamped_bilinear_buf(Rsrc r, unsigned base, i64 cols, double s, double t,
                                                        int imax, int jmax) {
-    const int i = clamped_floor_index(s, imax);
-    const int j = clamped_floor_index(t, jmax);
+    const int i = clamped_floor_index<UNIFORM>(s, imax);
+    const int j = clamped_floor_index<UNIFORM>(t, jmax);
     const double u = s - (double)i;
     const double w = t - (double)j;
-    const unsigned c = base + (unsigned)i * (unsigned)cols + (unsigned)j;
+    const unsigned c = mad24((unsigned)i, (unsigned)cols, base + (unsigned)j);
     const Pair lo = load_pair(r, c), hi = load_pair(r, c + (unsigned)cols);
     const double low = __builtin_fma(w, lo.b - lo.a, lo.a);
     const double high = __builtin_fma(w, hi.b - hi.a, hi.a);

@@ -195,8 +195,9 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
             sy = (qy - v.oy) * p.inv_h;
         }
     }
-    ex = clamped_bilinear_buf(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
-    ey = clamped_bilinear_buf(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
+    ex = clamped_bilinear_buf<!GENERAL>(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1,
+                                        (int)v.ny - 2);
+    ey = clamped_bilinear_buf<!GENERAL>(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
                               (int)v.ny - 1);
 }
 
@@ -502,9 +503,10 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
             edge_field<GEOM, GENERAL>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy);
             if (INTEG != 0 && stage == 1) { end_y = qy; end_cy = cy; end_cz = cz; end_sx = psx; end_sy = psy; }
             if (GEOM != 0) {
+                const double qe = qmk * ey;
                 g0 = qmk * ex;
-                g1 = qmk * (ey * cy);
-                g2 = qmk * (ey * cz);
+                g1 = qe * cy;
+                g2 = qe * cz;
             } else {
                 g0 = qmk * ex;
                 g1 = qmk * ey;
@@ -602,9 +604,9 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341: min(trunc(max(c, 0)),
     // n - 2, cells - 1), taken as the saturated trunc(c) clamped to [0, that bound] (the
     // same integer for an integer bound)
-    const int ei = clamped_floor_index(ci, p.cell_imax);
-    const int ej = clamped_floor_index(cj, p.cell_jmax);
-    const i64 cell = p.cells[(unsigned)p.cells_base + (unsigned)ei * (unsigned)p.cells_cols + (unsigned)ej];
+    const int ei = clamped_floor_index<true>(ci, p.cell_imax);
+    const int ej = clamped_floor_index<true>(cj, p.cell_jmax);
+    const i64 cell = p.cells[mad24((unsigned)ei, (unsigned)p.cells_cols, (unsigned)p.cells_base + (unsigned)ej)];
 
     // space charge: dq dt at the Hermite midpoint, area-weighted (pyccel.py:372-423)
     const double gw = 1.0 / lorentz<GENERAL>(w0, w1, w2, inv_c2);
@@ -625,7 +627,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         if (GEOM == 1) {
             const double low = pv.oy + pv.h * fj;
             const double high = low + pv.h;
-            w = (m1 * m1 - low * low) / (high * high - low * low);
+            w = (m1 * m1 - low * low) * recip_nr(high * high - low * low);
         } else {
             const double dw = m1 - (pv.oy + pv.h * fj);
             w = GENERAL ? dw / pv.h : dw * p.inv_h;
