@@ -163,6 +163,18 @@ struct Fields {
     Rsrc fx, fy;
 };
 
+// The launch's parameters in the kernel-argument segment (the kernels' first argument), read
+// through a pointer that the compiler cannot follow across an empty asm: fields that only rare
+// paths use (a refill, the end of a trajectory, a plane record, an electrode sub-cell) are
+// then loaded where they are used (scalar loads) instead of holding scalar registers through
+// the whole step loop, where they were spilled.
+typedef const __attribute__((address_space(4))) TraceParams* KParams;
+__device__ __forceinline__ KParams kernel_params() { return (KParams)__builtin_amdgcn_kernarg_segment_ptr(); }
+__device__ __forceinline__ KParams cold(KParams k) {
+    asm volatile("" : "+s"(k));
+    return k;
+}
+
 // E_x, E_y of the finest patch at the Cartesian point q, with the radial direction and
 // the mesh coordinates (pyccel.py:196-269).
 // With `known` (one patch that is the mesh), qy, cy, cz, sx and sy are the values the end
@@ -247,14 +259,14 @@ __device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
 // when the window holds it (USE_WIN) and |q| < 2^46, else to the node's words in global
 // memory.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void node_add(const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny, int design,
+__device__ __forceinline__ void node_add(const WaveWin& w, KParams kp, i64 ofn, i64 ny, int design,
                                          int ii, int jj, i64 q) {
     if (USE_WIN && q < kWindowLimit && q > -kWindowLimit && design == w.design &&
         (unsigned)(ii - w.lo) < (unsigned)WC && (unsigned)(jj - w.base) < (unsigned)kWinRows) {
         atomicAdd(&bt_window_words[w.slot0 + (ii & (WC - 1)) * kWinRows + (jj - w.base)], (u64)q);
         return;
     }
-    add_sum(acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
+    add_sum(cold(kp)->acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
 }
 
 // A double below 2^51 in magnitude rounded to the nearest integer (ties to even), as a signed
@@ -268,16 +280,16 @@ __device__ __forceinline__ i64 exact_i64(double q) {
 // that stays very long in one cell) goes to the global words through to_fixed, which drops a
 // non-finite or out-of-range sum as it drops such a single deposit.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void sum_leaves(const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny, int design,
+__device__ __forceinline__ void sum_leaves(const WaveWin& w, KParams kp, i64 ofn, i64 ny, int design,
                                            int ii, int jj, double sum, int e) {
     const double units = ldexp(sum, -e);
     if (__builtin_expect(fabs(units) < 0x1p51, 1)) {
         const i64 q = exact_i64(units);
-        if (q != 0) node_add<USE_WIN, WC>(w, acc, ofn, ny, design, ii, jj, q);
+        if (q != 0) node_add<USE_WIN, WC>(w, kp, ofn, ny, design, ii, jj, q);
         return;
     }
     u64 lo, hi;
-    if (to_fixed(sum, e, lo, hi)) add_words(acc + kWords * (ofn + (i64)ii * ny + jj), lo, hi);
+    if (to_fixed(sum, e, lo, hi)) add_words(cold(kp)->acc + kWords * (ofn + (i64)ii * ny + jj), lo, hi);
 }
 
 // Moves the lane's cache to cell (i, j): the sums of nodes that both cells share stay in
@@ -285,7 +297,7 @@ __device__ __forceinline__ void sum_leaves(const WaveWin& w, u64* __restrict__ a
 // rows r, r + 1; column c stays iff i is c - 1 or c, column c + 1 iff i is c or c + 1 (rows
 // alike); slot q holds column c when q's column parity is c's, else c + 1.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny,
+__device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, KParams kp, i64 ofn, i64 ny,
                                            int design, int i, int j, int e) {
     if (i == s.cci && j == s.ccj) return;
     const int c = s.cci, r = s.ccj;
@@ -298,7 +310,7 @@ __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, u64* 
         const bool kept = (at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1);
         if (!kept) {
             if (s.cq[q] != 0.0)
-                sum_leaves<USE_WIN, WC>(w, acc, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q], e);
+                sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q], e);
             s.cq[q] = 0.0;
         }
     }
@@ -308,13 +320,13 @@ __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, u64* 
 
 // Empties the lane's cache (the trajectory ended).
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, u64* __restrict__ acc, i64 ofn, i64 ny,
+__device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, KParams kp, i64 ofn, i64 ny,
                                             int design, int e) {
     const int c = s.cci, r = s.ccj;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (s.cq[q] != 0.0)
-            sum_leaves<USE_WIN, WC>(w, acc, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
+            sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
                                     s.cq[q], e);
         s.cq[q] = 0.0;
     }
@@ -330,11 +342,11 @@ __device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, u64*
 // The node totals are exact integer sums of rounded values, so the order of the lanes does
 // not change them; a non-finite deposit is dropped.
 template <bool USE_WIN, int WC>
-__device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, u64* __restrict__ acc, i64 ofn, i64 ny,
+__device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, KParams kp, i64 ofn, i64 ny,
                                          bool in_base, int design, int i, int j, double u, double w, double deposit,
                                          int e) {
     if (__builtin_expect(!(fabs(deposit) < __builtin_inf()), 0)) return;  // (the four values would not be finite)
-    if (in_base) cache_move<USE_WIN, WC>(s, ww, acc, ofn, ny, design, i, j, e);
+    if (in_base) cache_move<USE_WIN, WC>(s, ww, kp, ofn, ny, design, i, j, e);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
 #pragma unroll
@@ -347,20 +359,20 @@ __device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, u64* _
             continue;
         }
         u64 lo, hi;
-        if (to_fixed(value, e, lo, hi)) add_words(acc + kWords * (ofn + (i64)(i + ia) * ny + (j + jb)), lo, hi);
+        if (to_fixed(value, e, lo, hi)) add_words(cold(kp)->acc + kWords * (ofn + (i64)(i + ia) * ny + (j + jb)), lo, hi);
     }
 }
 
 // Columns first .. last of the wave's window (a range it holds) to global memory, slots
 // emptied: lane l takes row l of each column.  Called by the whole wave.
 template <int WC>
-__device__ __forceinline__ void window_flush_columns(const WaveWin& w, u64* __restrict__ acc, int first, int last) {
+__device__ __forceinline__ void window_flush_columns(const WaveWin& w, KParams kp, int first, int last) {
     const int lane = (int)__lane_id();
     for (int c = first; c <= last; ++c) {
         const int idx = w.slot0 + (c & (WC - 1)) * kWinRows + lane;
         const u64 q = bt_window_words[idx];
         if (q != 0) {
-            add_sum(acc + kWords * (w.ofn + (i64)c * w.ny + w.base + lane), q);
+            add_sum(cold(kp)->acc + kWords * (w.ofn + (i64)c * w.ny + w.base + lane), q);
             bt_window_words[idx] = 0ull;
         }
     }
@@ -368,8 +380,8 @@ __device__ __forceinline__ void window_flush_columns(const WaveWin& w, u64* __re
 
 // The whole window to global memory (the wave ends, or re-bases).
 template <int WC>
-__device__ __forceinline__ void window_flush_all(const WaveWin& w, u64* __restrict__ acc) {
-    if (w.design >= 0) window_flush_columns<WC>(w, acc, w.lo, w.lo + WC - 1);
+__device__ __forceinline__ void window_flush_all(const WaveWin& w, KParams kp) {
+    if (w.design >= 0) window_flush_columns<WC>(w, kp, w.lo, w.lo + WC - 1);
 }
 
 // The wave's window at the start: empty, placed nowhere (no design).
@@ -390,12 +402,12 @@ __device__ __forceinline__ WaveWin window_start() {
 // placed with the trailing lane kRoundSteps inside the low end, rows centred on the lanes'
 // rows.  Wave-uniform; called by the whole wave.
 template <int WC>
-__device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, u64* __restrict__ acc,
+__device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, KParams kp,
                                              const LaneState& s, bool counts, bool force) {
     const u64 has = __ballot(counts && s.cci != kNoCell);
     if (has == 0) {
         if (force) {
-            window_flush_all<WC>(w, acc);
+            window_flush_all<WC>(w, kp);
             w.design = -1;
         }
         return;
@@ -411,12 +423,12 @@ __device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, u
     int nlo = cmin - kRoundSteps;
     if (nlo < 0) nlo = 0;
     if (rows_ok && !force) {  // only columns leave: those of [lo, lo + WC) outside [nlo, nlo + WC)
-        if (nlo > w.lo) window_flush_columns<WC>(w, acc, w.lo, min(nlo, w.lo + WC) - 1);
-        else window_flush_columns<WC>(w, acc, max(nlo + WC, w.lo), w.lo + WC - 1);
+        if (nlo > w.lo) window_flush_columns<WC>(w, kp, w.lo, min(nlo, w.lo + WC) - 1);
+        else window_flush_columns<WC>(w, kp, max(nlo + WC, w.lo), w.lo + WC - 1);
         w.lo = nlo;
         return;
     }
-    window_flush_all<WC>(w, acc);
+    window_flush_all<WC>(w, kp);
     int nbase = ((rmin + rmax + 1) >> 1) - kWinRows / 2;
     if (nbase > q.ny - kWinRows) nbase = q.ny - kWinRows;
     if (nbase < 0) nbase = 0;
@@ -426,17 +438,18 @@ __device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, u
 // The record of a crossing of plane pl at xp during the step from (x, u) to (y, w)
 // (pyccel.py:426-442): the point and velocity interpolated at the crossing, the direction and
 // the step; the count goes on past the capacity (the caller then retraces).
-__device__ __forceinline__ void record_crossing(const TraceParams& p, const LaneState& s, int pl, double xp,
+__device__ __forceinline__ void record_crossing(KParams kp, const LaneState& s, int pl, double xp,
                                                 double x0, double x1, double x2, double y0, double y1, double y2,
                                                 double gv, double v0, double v1, double v2, double gw, double w0,
                                                 double w1, double w2, i64 ns) {
+    const KParams p = cold(kp);
     const double before = x0 - xp;
     const double after = y0 - xp;
     const double sfrac = before / (before - after);
-    const i64 idx = s.k * p.n_planes + pl;
-    const i64 c = p.counts[idx];
-    if (c < p.capacity) {
-        double* rec = p.hits + (idx * p.capacity + c) * 8;
+    const i64 idx = s.k * p->n_planes + pl;
+    const i64 c = p->counts[idx];
+    if (c < p->capacity) {
+        double* rec = p->hits + (idx * p->capacity + c) * 8;
         rec[0] = xp;
         rec[1] = x1 + sfrac * (y1 - x1);
         rec[2] = x2 + sfrac * (y2 - x2);
@@ -446,13 +459,13 @@ __device__ __forceinline__ void record_crossing(const TraceParams& p, const Lane
         rec[6] = after >= 0.0 ? 1.0 : -1.0;
         rec[7] = 1.0 * (double)ns;
     }
-    p.counts[idx] = c + 1;
+    p->counts[idx] = c + 1;
 }
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN, int WC>
-__device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneState& s, const WaveWin& ww,
-                                    double plane0) {
+__device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fields& F, LaneState& s,
+                                    const WaveWin& ww, double plane0) {
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -640,7 +653,7 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
         if (!(fj >= 0.0)) { w = 0.0; j = 0; }
         else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = (int)pv.ny - 2; }
         else j = (int)fj;
-        deposit4<USE_WIN, WC>(s, ww, p.acc, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
+        deposit4<USE_WIN, WC>(s, ww, kp, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
                           s.dqk * dt, p.deposit_exponent);
     }
     const i64 ns = s.ns + 1;
@@ -650,12 +663,12 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     // record's address is formed only for a crossing (rare)
     if (p.n_planes > 0) {
         if (__builtin_expect((x0 - plane0 < 0.0) != (y0 - plane0 < 0.0), 0))
-            record_crossing(p, s, 0, plane0, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
+            record_crossing(kp, s, 0, plane0, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
         if (__builtin_expect(p.n_planes > 1, 0)) {
             for (int pl = 1; pl < p.n_planes; ++pl) {
                 const double xp = p.planes[p.planes_base + pl];
                 if ((x0 - xp < 0.0) != (y0 - xp < 0.0))
-                    record_crossing(p, s, pl, xp, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
+                    record_crossing(kp, s, pl, xp, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
             }
         }
     }
@@ -681,14 +694,16 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     } else {
         s.pt_y = py; s.pt_cy = pcy; s.pt_cz = pcz; s.pt_sx = ci; s.pt_sy = cj; s.point = 1;
         if (cell >= 0) {
-            const double R = (double)p.refine;
+            const KParams c = cold(kp);
+            const int refine = c->refine;
+            const double R = (double)refine;
             const int si = (int)fmin(fmax((ci - (double)ei) * R, 0.0), R - 1.0);
             const int sj = (int)fmin(fmax((cj - (double)ej) * R, 0.0), R - 1.0);
-            const i64 at = (cell * p.refine + si) * p.refine + sj;
-            const i64 e = at < p.n_subcells ? p.subcells[p.subcells_base + at] : -1;
+            const i64 at = (cell * refine + si) * refine + sj;
+            const i64 e = at < p.n_subcells ? c->subcells[p.subcells_base + at] : -1;
             if (e >= 0) {
                 st = 3;
-                p.electrode[s.k] = e;
+                c->electrode[s.k] = e;
             } else if (e == -2) {
                 st = -2;
             }
@@ -697,12 +712,15 @@ __device__ __forceinline__ int step(const TraceParams& p, const Fields& F, LaneS
     return st;
 }
 
-__device__ __forceinline__ void store_end(const TraceParams& p, const LaneState& s, int st) {
-    const i64 N = p.n, k = s.k;
-    p.x[k] = s.x0; p.x[N + k] = s.x1; p.x[2 * N + k] = s.x2;
-    p.v[k] = s.v0; p.v[N + k] = s.v1; p.v[2 * N + k] = s.v2;
-    p.steps[k] = s.ns;
-    p.status[k] = st;
+__device__ __forceinline__ void store_end(KParams kp, const LaneState& s, int st) {
+    const KParams p = cold(kp);
+    const i64 N = p->n, k = s.k;
+    double* const x = p->x;
+    double* const v = p->v;
+    x[k] = s.x0; x[N + k] = s.x1; x[2 * N + k] = s.x2;
+    v[k] = s.v0; v[N + k] = s.v1; v[2 * N + k] = s.v2;
+    p->steps[k] = s.ns;
+    p->status[k] = st;
 }
 
 // ---- batched form -------------------------------------------------------------------
@@ -741,6 +759,7 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
 // batched launch records neither).
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS, int WC = kWinCols>
 __device__ __forceinline__ void trace_loop(const TraceParams& p) {
+    const KParams kp = kernel_params();  // p itself, for the rarely used fields
     WaveWin ww = window_start<WC>();
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -760,6 +779,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     int rounds = 0;
     for (;;) {
         if (!exhausted && refill) {
+            const KParams c = cold(kp);
             const bool want = s.k < 0;
             const u64 need = __ballot(want);
             bool end_seen = false;
@@ -767,39 +787,39 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                 const int count = __popcll(need);
                 const int leader = __ffsll((long long)need) - 1;
                 u64 base = 0;
-                if ((int)lane == leader) base = atomicAdd(p.queue, (u64)count);
+                if ((int)lane == leader) base = atomicAdd(c->queue, (u64)count);
                 base = __shfl(base, leader);
                 if (want) {
                     const u64 m = base + (u64)__popcll(need & below);
-                    if (m < (u64)p.n_active) {
-                        const i64 k = p.active[m];
-                        const i64 N = p.n;
+                    if (m < (u64)c->n_active) {
+                        const i64 k = c->active[m];
+                        const i64 N = c->n;
                         bool valid = k >= 0 && k < N;
                         if (BATCHED && valid) {
-                            my_design = p.design_of[k];
-                            valid = my_design >= 0 && my_design < p.n_designs;
+                            my_design = c->design_of[k];
+                            valid = my_design >= 0 && my_design < c->n_designs;
                         }
                         // an index that is not a trajectory (or of no design) is skipped: the
                         // lane asks again
                         if (valid) {
                             s.k = k;
-                            s.x0 = p.x[k]; s.x1 = p.x[N + k]; s.x2 = p.x[2 * N + k];
-                            s.v0 = p.v[k]; s.v1 = p.v[N + k]; s.v2 = p.v[2 * N + k];
-                            s.qmk = p.qm[k];
-                            s.dqk = p.dq[k];
-                            s.slot = (RECORDS && p.path_slot != nullptr) ? p.path_slot[k] : -1;
-                            if (s.slot >= p.n_paths) s.slot = -1;
-                            s.ns = p.steps[k];
+                            s.x0 = c->x[k]; s.x1 = c->x[N + k]; s.x2 = c->x[2 * N + k];
+                            s.v0 = c->v[k]; s.v1 = c->v[N + k]; s.v2 = c->v[2 * N + k];
+                            s.qmk = c->qm[k];
+                            s.dqk = c->dq[k];
+                            s.slot = (RECORDS && c->path_slot != nullptr) ? c->path_slot[k] : -1;
+                            if (s.slot >= c->n_paths) s.slot = -1;
+                            s.ns = c->steps[k];
                             s.have = 0;
                             s.point = 0;
                             s.ca0 = s.ca1 = s.ca2 = 0.0;
                             s.ce0 = s.ce1 = s.ce2 = s.cb0 = s.cb1 = s.cb2 = 0.0;
-                            if (RECORDS && p.ckpt != nullptr) {  // the first multiple of the interval >= ns
-                                s.ck_index = (s.ns + p.ckpt_interval - 1) / p.ckpt_interval;
-                                s.ck_next = s.ck_index * p.ckpt_interval;
+                            if (RECORDS && c->ckpt != nullptr) {  // the first multiple of the interval >= ns
+                                s.ck_index = (s.ns + c->ckpt_interval - 1) / c->ckpt_interval;
+                                s.ck_next = s.ck_index * c->ckpt_interval;
                             }
-                            if (s.ns >= p.max_steps) {  // no step: the loop of the reference ends at once
-                                store_end(p, s, 4);
+                            if (s.ns >= c->max_steps) {  // no step: the loop of the reference ends at once
+                                store_end(kp, s, 4);
                                 s.k = -1;
                             }
                         }
@@ -833,18 +853,18 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.ck_index += 1;
                     s.ck_next += p.ckpt_interval;
                 }
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC>(q, F, s, ww, plane0);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC>(q, kp, F, s, ww, plane0);
                 if (st != -1 || s.ns >= p.max_steps) {
-                    store_end(p, s, st != -1 ? st : 4);
+                    store_end(kp, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush<true, WC>(s, ww, p.acc, q.bofn, q.ny, q.design, q.deposit_exponent);
+                    cache_flush<true, WC>(s, ww, kp, q.bofn, q.ny, q.design, q.deposit_exponent);
                 }
             }
         }
-        window_round<WC>(ww, q, p.acc, s, !BATCHED || my_design == wd, ++rounds % kFlushRounds == 0);
+        window_round<WC>(ww, q, kp, s, !BATCHED || my_design == wd, ++rounds % kFlushRounds == 0);
         refill = !p.refill_block || __ballot(s.k >= 0) == 0;
     }
-    window_flush_all<WC>(ww, p.acc);
+    window_flush_all<WC>(ww, kp);
 }
 
 template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
