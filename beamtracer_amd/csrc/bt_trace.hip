@@ -729,7 +729,10 @@ __device__ __forceinline__ void store_end(KParams kp, const LaneState& s, int st
 // at its node offset in the accumulator).  Called with a wave-uniform design, so that the
 // mesh parameters stay in scalar registers.
 __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& p, int d) {
-    const BatchDesign& b = p.designs[d];
+    // the design table does not change during a launch: read through the constant address
+    // space, a uniform entry is a few scalar loads and stays in scalar registers
+    typedef const __attribute__((address_space(4))) BatchDesign* Table;
+    const auto& b = ((Table)p.designs)[d];
     lp.nx = (int)b.nx; lp.ny = (int)b.ny;
     lp.origin_x = b.origin_x; lp.origin_y = b.origin_y; lp.h = b.h;
     lp.upper_x = b.upper_x; lp.upper_y = b.upper_y;
@@ -839,7 +842,8 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
         if (BATCHED && __ballot(s.k >= 0 && my_design == wd) == 0) {
             wd = __builtin_amdgcn_readlane(my_design, __ffsll((long long)busy) - 1);
             load_design(dp, p, wd);
-            plane0 = p.n_planes > 0 ? p.planes[dp.planes_base] : 0.0;
+            typedef const __attribute__((address_space(4))) double* Planes;  // read-only during the launch
+            plane0 = p.n_planes > 0 ? ((Planes)p.planes)[dp.planes_base] : 0.0;
         }
         // a round of steps; a lane whose trajectory ends waits for the next round's refill
         for (int r = 0; r < kRoundSteps; ++r) {
