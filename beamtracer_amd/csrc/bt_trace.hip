@@ -255,41 +255,29 @@ __device__ __forceinline__ void add_sum(u64* __restrict__ w, u64 q) {
     add_words(w, q & 0xffffffffull, (u64)((i64)q >> 32));
 }
 
-// Adds q units (|q| < 2^50) to node (ii, jj) of the design's mesh: into the wave's window slot
-// when the window holds it (USE_WIN) and |q| < 2^46, else to the node's words in global
-// memory.
-template <bool USE_WIN, int WC>
-__device__ __forceinline__ void node_add(const WaveWin& w, KParams kp, i64 ofn, i64 ny, int design,
-                                         int ii, int jj, i64 q) {
-    if (USE_WIN && q < kWindowLimit && q > -kWindowLimit && design == w.design &&
-        (unsigned)(ii - w.lo) < (unsigned)WC && (unsigned)(jj - w.base) < (unsigned)kWinRows) {
-        atomicAdd(&bt_window_words[w.slot0 + (ii & (WC - 1)) * kWinRows + (jj - w.base)], (u64)q);
-        return;
-    }
-    add_sum(cold(kp)->acc + kWords * (ofn + (i64)ii * ny + jj), (u64)q);
-}
-
 // A double below 2^51 in magnitude rounded to the nearest integer (ties to even), as a signed
 // 64-bit integer: adding 1.5 2^52 rounds it, and the sum's bit pattern then holds it.
 __device__ __forceinline__ i64 exact_i64(double q) {
     return __double_as_longlong(q + 0x1.8p52) - 0x4338000000000000ll;
 }
 
-// A lane's FP64 sum of a node's deposits leaves the lane: rounded once to units 2^e and added
-// to node (ii, jj) (window slot or global words).  A sum of 2^51 units or more (a trajectory
-// that stays very long in one cell) goes to the global words through to_fixed, which drops a
-// non-finite or out-of-range sum as it drops such a single deposit.
+// A lane's FP64 sum of a node's deposits leaves the lane, in units of 2^e (every deposit is
+// scaled by 2^-e, exactly, before it is summed): rounded once to an integer (to nearest, ties
+// to even) and added to node (ii, jj).  The common case -- a sum below 2^46 units whose node
+// the wave's window holds -- is one no-return LDS atomic; any other sum goes to the node's
+// words in global memory through to_fixed, which takes sums of any size and drops a non-finite
+// or out-of-range one as it drops such a single deposit.
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void sum_leaves(const WaveWin& w, KParams kp, i64 ofn, i64 ny, int design,
-                                           int ii, int jj, double sum, int e) {
-    const double units = ldexp(sum, -e);
-    if (__builtin_expect(fabs(units) < 0x1p51, 1)) {
-        const i64 q = exact_i64(units);
-        if (q != 0) node_add<USE_WIN, WC>(w, kp, ofn, ny, design, ii, jj, q);
+                                           int ii, int jj, double units) {
+    if (__builtin_expect(USE_WIN && fabs(units) < 0x1p46 && design == w.design &&
+                         (unsigned)(ii - w.lo) < (unsigned)WC && (unsigned)(jj - w.base) < (unsigned)kWinRows, 1)) {
+        atomicAdd(&bt_window_words[w.slot0 + (ii & (WC - 1)) * kWinRows + (jj - w.base)],
+                  (u64)exact_i64(units));  // two's complement
         return;
     }
     u64 lo, hi;
-    if (to_fixed(sum, e, lo, hi)) add_words(cold(kp)->acc + kWords * (ofn + (i64)ii * ny + jj), lo, hi);
+    if (to_fixed(units, 0, lo, hi)) add_words(cold(kp)->acc + kWords * (ofn + (i64)ii * ny + jj), lo, hi);
 }
 
 // Moves the lane's cache to cell (i, j): the sums of nodes that both cells share stay in
@@ -298,7 +286,7 @@ __device__ __forceinline__ void sum_leaves(const WaveWin& w, KParams kp, i64 ofn
 // alike); slot q holds column c when q's column parity is c's, else c + 1.
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, KParams kp, i64 ofn, i64 ny,
-                                           int design, int i, int j, int e) {
+                                           int design, int i, int j) {
     if (i == s.cci && j == s.ccj) return;
     const int c = s.cci, r = s.ccj;
     const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
@@ -308,11 +296,9 @@ __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, KPara
         const bool at_c = ((c ^ (q >> 1)) & 1) == 0;  // slot q's column is c (else c + 1)
         const bool at_r = ((r ^ (q & 1)) & 1) == 0;
         const bool kept = (at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1);
-        if (!kept) {
-            if (s.cq[q] != 0.0)
-                sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q], e);
-            s.cq[q] = 0.0;
-        }
+        if (!kept && s.cq[q] != 0.0)
+            sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q]);
+        s.cq[q] = kept ? s.cq[q] : 0.0;
     }
     s.cci = i;
     s.ccj = j;
@@ -321,13 +307,13 @@ __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, KPara
 // Empties the lane's cache (the trajectory ended).
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, KParams kp, i64 ofn, i64 ny,
-                                            int design, int e) {
+                                            int design) {
     const int c = s.cci, r = s.ccj;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         if (s.cq[q] != 0.0)
             sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
-                                    s.cq[q], e);
+                                    s.cq[q]);
         s.cq[q] = 0.0;
     }
     s.cci = kNoCell;
@@ -336,17 +322,17 @@ __device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, KPar
 
 // The 4-node deposit of one step into cell (i, j) of a patch whose nodes start at `ofn`:
 // node (i + a, j + b) receives ((a ? u : 1 - u) (b ? w : 1 - w)) deposit, evaluated in that
-// order (pyccel.py:407-423).  On the base mesh (in_base) the values add to the lane's FP64
-// sums (rounded to units of 2^e when a sum leaves the lane); on a refined patch each value is
-// rounded once to units of 2^e (ties to even) and goes to the node words in global memory.
-// The node totals are exact integer sums of rounded values, so the order of the lanes does
-// not change them; a non-finite deposit is dropped.
+// order (pyccel.py:407-423), `deposit` in units of 2^e.  On the base mesh (in_base) the values
+// add to the lane's FP64 sums (rounded to integers when a sum leaves the lane); on a refined
+// patch each value is rounded once to an integer (ties to even) and goes to the node words in
+// global memory.  The node totals are exact integer sums of rounded values, so the order of
+// the lanes does not change them; a non-finite deposit is dropped.
 template <bool USE_WIN, int WC>
 __device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, KParams kp, i64 ofn, i64 ny,
-                                         bool in_base, int design, int i, int j, double u, double w, double deposit,
-                                         int e) {
+                                         bool in_base, int design, int i, int j, double u, double w,
+                                         double deposit) {
     if (__builtin_expect(!(fabs(deposit) < __builtin_inf()), 0)) return;  // (the four values would not be finite)
-    if (in_base) cache_move<USE_WIN, WC>(s, ww, kp, ofn, ny, design, i, j, e);
+    if (in_base) cache_move<USE_WIN, WC>(s, ww, kp, ofn, ny, design, i, j);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
 #pragma unroll
@@ -359,7 +345,7 @@ __device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, KParam
             continue;
         }
         u64 lo, hi;
-        if (to_fixed(value, e, lo, hi)) add_words(cold(kp)->acc + kWords * (ofn + (i64)(i + ia) * ny + (j + jb)), lo, hi);
+        if (to_fixed(value, 0, lo, hi)) add_words(cold(kp)->acc + kWords * (ofn + (i64)(i + ia) * ny + (j + jb)), lo, hi);
     }
 }
 
@@ -631,30 +617,27 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
     else if (GEOM == 2) m1 = p.origin_y + fabs(m1 - p.origin_y);
     m0 = locate_x<GENERAL>(p, m0);
     {
+        // the cell and the area weights of the midpoint, clamped to the patch as the reference
+        // clamps them (pyccel.py:406-417: a point below the first node deposits on it with
+        // weight 1, one beyond the last on the last; a NaN point on the first): the cell index
+        // by the saturated conversion and a median, the fractions clamped to [0, 1]
         const PatchView pv = patch_view<GENERAL>(p, patch_of<GENERAL>(p, m0, m1));
-        const double fi = floor(GENERAL ? (m0 - pv.ox) / pv.h : (m0 - pv.ox) * p.inv_h);
-        const double fj = floor(GENERAL ? (m1 - pv.oy) / pv.h : (m1 - pv.oy) * p.inv_h);
-        const double du = m0 - (pv.ox + pv.h * fi);
-        double u = GENERAL ? du / pv.h : du * p.inv_h;
+        const double sxm = GENERAL ? (m0 - pv.ox) / pv.h : (m0 - pv.ox) * p.inv_h;
+        const double sym = GENERAL ? (m1 - pv.oy) / pv.h : (m1 - pv.oy) * p.inv_h;
+        const int i = clamped_floor_index<!GENERAL>(sxm, (int)pv.nx - 2);  // (a patch has fewer than 2^29 nodes)
+        const int j = clamped_floor_index<!GENERAL>(sym, (int)pv.ny - 2);
+        const double u = fmin(fmax(sxm - (double)i, 0.0), 1.0);
         double w;
         if (GEOM == 1) {
-            const double low = pv.oy + pv.h * fj;
+            const double low = pv.oy + pv.h * (double)j;
             const double high = low + pv.h;
             w = (m1 * m1 - low * low) * recip_nr(high * high - low * low);
         } else {
-            const double dw = m1 - (pv.oy + pv.h * fj);
-            w = GENERAL ? dw / pv.h : dw * p.inv_h;
+            w = sym - (double)j;
         }
-        int i, j;  // (a patch has fewer than 2^29 nodes)
-        // !(f >= 0) also sends a NaN midpoint to the first node (its deposit is dropped)
-        if (!(fi >= 0.0)) { u = 0.0; i = 0; }
-        else if (fi >= (double)(pv.nx - 1)) { u = 1.0; i = (int)pv.nx - 2; }
-        else i = (int)fi;
-        if (!(fj >= 0.0)) { w = 0.0; j = 0; }
-        else if (fj >= (double)(pv.ny - 1)) { w = 1.0; j = (int)pv.ny - 2; }
-        else j = (int)fj;
+        w = fmin(fmax(w, 0.0), 1.0);
         deposit4<USE_WIN, WC>(s, ww, kp, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
-                          s.dqk * dt, p.deposit_exponent);
+                              s.dqk * dt);
     }
     const i64 ns = s.ns + 1;
     s.ns = ns;
@@ -809,7 +792,14 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                             s.x0 = c->x[k]; s.x1 = c->x[N + k]; s.x2 = c->x[2 * N + k];
                             s.v0 = c->v[k]; s.v1 = c->v[N + k]; s.v2 = c->v[2 * N + k];
                             s.qmk = c->qm[k];
-                            s.dqk = c->dq[k];
+                            // the trajectory's deposits in units of its design's fixed-point
+                            // grid 2^e (an exact scaling): its lane sums are then ready to round
+                            int e = c->deposit_exponent;
+                            if (BATCHED) {
+                                typedef const __attribute__((address_space(4))) BatchDesign* Table;
+                                e = ((Table)c->designs)[my_design].deposit_exponent;
+                            }
+                            s.dqk = ldexp(c->dq[k], -e);
                             s.slot = (RECORDS && c->path_slot != nullptr) ? c->path_slot[k] : -1;
                             if (s.slot >= c->n_paths) s.slot = -1;
                             s.ns = c->steps[k];
@@ -861,7 +851,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                 if (st != -1 || s.ns >= p.max_steps) {
                     store_end(kp, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush<true, WC>(s, ww, kp, q.bofn, q.ny, q.design, q.deposit_exponent);
+                    cache_flush<true, WC>(s, ww, kp, q.bofn, q.ny, q.design);
                 }
             }
         }
