@@ -47,6 +47,17 @@ struct AdjFields {
     Rsrc fx, fy;
 };
 
+// The launch's parameters in the kernel-argument segment (the kernel's only argument), read
+// through a pointer that the compiler cannot follow across an empty asm: the fields that only
+// a refill, a seeded step, a flush to global memory or the end of a trajectory use are loaded
+// where they are used instead of holding scalar registers through the backward walk.
+typedef const __attribute__((address_space(4))) AdjointParams* KAdj;
+__device__ __forceinline__ KAdj adjoint_params() { return (KAdj)__builtin_amdgcn_kernarg_segment_ptr(); }
+__device__ __forceinline__ KAdj cold(KAdj k) {
+    asm volatile("" : "+s"(k));
+    return k;
+}
+
 template <int GEOM>
 __device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F, double qm, Vec3 q) {
     double cy = 0.0, cz = 0.0, qy = q.b;
@@ -149,13 +160,13 @@ __device__ __forceinline__ int wave_reduce_pk(int v) {
 }
 
 template <int A>
-__device__ __forceinline__ void add_node(double* __restrict__ grad, i64 cols, const GradWindow& W, int ii, int jj,
-                                         double v) {
+__device__ __forceinline__ void add_node(KAdj kp, i64 cols, const GradWindow& W, int ii, int jj, double v) {
     if (W.lo >= 0 && ii >= W.lo && ii < W.lo + kWinCols && jj >= W.base && jj < W.base + kWinRows) {
         atomicAdd(&adjoint_windows[W.wave][A][(ii & (kWinCols - 1)) * kWinRows + (jj - W.base)], v);
         return;
     }
-    unsafeAtomicAdd(grad + (i64)ii * cols + jj, v);
+    const KAdj c = cold(kp);
+    unsafeAtomicAdd((A == 0 ? c->dfx : c->dfy) + (i64)ii * cols + jj, v);
 }
 
 // lambda_E times the bilinear weights of the gathered cell into the gradient of grid A
@@ -181,7 +192,7 @@ __device__ __forceinline__ void cache_reset(GradCache& g) {
 // Moves the cache to cell (i, j): column c stays iff i is c - 1 or c, column c + 1 iff i is
 // c or c + 1 (rows alike); the other nodes' sums leave.
 template <int A>
-__device__ __forceinline__ void cache_move(double* __restrict__ grad, i64 cols, const GradWindow& W, GradCache& g,
+__device__ __forceinline__ void cache_move(KAdj kp, i64 cols, const GradWindow& W, GradCache& g,
                                            int i, int j) {
     if (i == g.c && j == g.r) return;
     const int c = g.c, r = g.r;
@@ -192,7 +203,7 @@ __device__ __forceinline__ void cache_move(double* __restrict__ grad, i64 cols, 
         const bool at_c = ((c ^ (q >> 1)) & 1) == 0;  // slot q's column is c (else c + 1)
         const bool at_r = ((r ^ (q & 1)) & 1) == 0;
         if (!((at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1))) {
-            if (g.q[q] != 0.0) add_node<A>(grad, cols, W, at_c ? c : c + 1, at_r ? r : r + 1, g.q[q]);
+            if (g.q[q] != 0.0) add_node<A>(kp, cols, W, at_c ? c : c + 1, at_r ? r : r + 1, g.q[q]);
             g.q[q] = 0.0;
         }
     }
@@ -201,21 +212,21 @@ __device__ __forceinline__ void cache_move(double* __restrict__ grad, i64 cols, 
 }
 
 template <int A>
-__device__ __forceinline__ void cache_flush(double* __restrict__ grad, i64 cols, const GradWindow& W, GradCache& g) {
+__device__ __forceinline__ void cache_flush(KAdj kp, i64 cols, const GradWindow& W, GradCache& g) {
     const int c = g.c, r = g.r;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        if (g.q[q] != 0.0) add_node<A>(grad, cols, W, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1), g.q[q]);
+        if (g.q[q] != 0.0) add_node<A>(kp, cols, W, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1), g.q[q]);
     cache_reset(g);
 }
 
 // lambda_E times the bilinear weights of the gathered cell into the lane's cache of grid A
 // (0: fx, 1: fy; rows of `cols`).
 template <int A>
-__device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, const GradWindow& W, GradCache& gc,
+__device__ __forceinline__ void scatter(KAdj kp, i64 cols, const GradWindow& W, GradCache& gc,
                                         const Gathered& g, double lam) {
     if (lam == 0.0) return;
-    cache_move<A>(grad, cols, W, gc, g.i, g.j);
+    cache_move<A>(kp, cols, W, gc, g.i, g.j);
     const double u0 = 1.0 - g.u, w0 = 1.0 - g.w;
     const bool io = (g.i & 1) != 0, jo = (g.j & 1) != 0;
 #pragma unroll
@@ -227,26 +238,29 @@ __device__ __forceinline__ void scatter(double* __restrict__ grad, i64 cols, con
 }
 
 // Column `col` of the window (slot s) to global memory; lane r takes row base + r.
-__device__ __forceinline__ void flush_column(const AdjointParams& p, const GradWindow& W, int s, int col) {
+__device__ __forceinline__ void flush_column(KAdj kp, const GradWindow& W, int s, int col) {
+    const KAdj p = cold(kp);
     const int r = (int)__lane_id();
     const int row = W.base + r;
     const double vx = __longlong_as_double((long long)atomicExch(
         reinterpret_cast<unsigned long long*>(&adjoint_windows[W.wave][0][s * kWinRows + r]), 0ull));
     const double vy = __longlong_as_double((long long)atomicExch(
         reinterpret_cast<unsigned long long*>(&adjoint_windows[W.wave][1][s * kWinRows + r]), 0ull));
-    if (vx != 0.0 && col <= p.nx && row < p.ny) unsafeAtomicAdd(p.dfx + (i64)col * p.ny + row, vx);
-    if (vy != 0.0 && col < p.nx && row <= p.ny) unsafeAtomicAdd(p.dfy + (i64)col * (p.ny + 1) + row, vy);
+    const i64 nx = p->nx, ny = p->ny;
+    if (vx != 0.0 && col <= nx && row < ny) unsafeAtomicAdd(p->dfx + (i64)col * ny + row, vx);
+    if (vy != 0.0 && col < nx && row <= ny) unsafeAtomicAdd(p->dfy + (i64)col * (ny + 1) + row, vy);
 }
 
-__device__ __forceinline__ void window_flush_all(const AdjointParams& p, GradWindow& W) {
+__device__ __forceinline__ void window_flush_all(KAdj kp, GradWindow& W) {
     if (W.lo < 0) return;
-    for (int s = 0; s < kWinCols; ++s) flush_column(p, W, s, W.lo + ((s - W.lo) & (kWinCols - 1)));
+    for (int s = 0; s < kWinCols; ++s) flush_column(kp, W, s, W.lo + ((s - W.lo) & (kWinCols - 1)));
     W.lo = -1;
 }
 
 // Moves the window (wave-uniform; every lane of the wave calls it) to hold the cells
 // (ci, cj) of the active lanes with room below them, the direction of the backward walk.
-__device__ __forceinline__ void window_follow(const AdjointParams& p, GradWindow& W, bool act, int ci, int cj) {
+__device__ __forceinline__ void window_follow(const AdjointParams& p, KAdj kp, GradWindow& W, bool act, int ci,
+                                              int cj) {
     int imin, imax, jmin, jmax;
     if (p.nx < 32768 && p.ny < 32768) {  // cells below 2^15 on both axes: (ci, cj) in one word
         const int both = (ci & 0xffff) | (cj << 16);
@@ -269,7 +283,7 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, GradWindow
     if (W.lo >= 0) {
         for (int s = 0; s < kWinCols; ++s) {
             const int col = W.lo + ((s - W.lo) & (kWinCols - 1));
-            if (!(rows_ok && col >= nlo && col < nlo + kWinCols)) flush_column(p, W, s, col);
+            if (!(rows_ok && col >= nlo && col < nlo + kWinCols)) flush_column(kp, W, s, col);
         }
     }
     W.lo = nlo;
@@ -282,6 +296,7 @@ template <int GEOM>
 __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
                                               GradCache& fx_sums, GradCache& fy_sums, double qm, Vec3 q,
                                               Vec3 lg, Vec3& g) {
+    const KAdj kp = adjoint_params();
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
     if (GEOM == 1) {
         const Radial rd = radial(q.b, q.c);
@@ -308,8 +323,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     } else {
         ley = qm * lg.b;
     }
-    scatter<0>(p.dfx, p.ny, W, fx_sums, gx, lex);
-    scatter<1>(p.dfy, p.ny + 1, W, fy_sums, gy, ley);
+    scatter<0>(kp, p.ny, W, fx_sums, gx, lex);
+    scatter<1>(kp, p.ny + 1, W, fy_sums, gy, ley);
     const double lsx = lex * gx.ds + ley * gy.ds;
     const double lsy = lex * gx.dt + ley * gy.dt;
     Vec3 lq = {lsx * p.inv_h, 0.0, 0.0};
@@ -376,15 +391,16 @@ __device__ __forceinline__ void forward_step(const AdjointParams& p, const AdjFi
 // split into the step's start (ls_x, ls_v) and end (le_x, le_v) terms.
 __device__ __forceinline__ void crossing_seeds(const AdjointParams& p, i64 k, i64 m, Vec3 x, Vec3 v, Vec3 y, Vec3 w,
                                                Vec3& lsx, Vec3& lsv, Vec3& lex, Vec3& lev) {
-    for (int pl = 0; pl < p.n_planes; ++pl) {
-        const i64 idx = k * p.n_planes + pl;
-        i64 cnt = p.counts[idx];
-        if (cnt > p.capacity) cnt = p.capacity;
+    const KAdj kc = cold(adjoint_params());
+    for (int pl = 0; pl < kc->n_planes; ++pl) {
+        const i64 idx = k * kc->n_planes + pl;
+        i64 cnt = kc->counts[idx];
+        if (cnt > kc->capacity) cnt = kc->capacity;
         for (i64 c = 0; c < cnt; ++c) {
-            const double* rec = p.hits + (idx * p.capacity + c) * 8;
+            const double* rec = kc->hits + (idx * kc->capacity + c) * 8;
             if ((i64)rec[7] != m) continue;
-            const double* sd = p.seeds + (idx * p.capacity + c) * 5;
-            const double xp = p.planes[pl];
+            const double* sd = kc->seeds + (idx * kc->capacity + c) * 5;
+            const double xp = kc->planes[pl];
             const double before = x.a - xp;
             const double after = y.a - xp;
             const double sfrac = before / (before - after);
@@ -403,15 +419,16 @@ __device__ __forceinline__ void crossing_seeds(const AdjointParams& p, i64 k, i6
 
 // The largest seeded step below `below` (0 if none).
 __device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 below) {
+    const KAdj kc = cold(adjoint_params());
     i64 best = 0;
-    for (int pl = 0; pl < p.n_planes; ++pl) {
-        const i64 idx = k * p.n_planes + pl;
-        i64 cnt = p.counts[idx];
-        if (cnt > p.capacity) cnt = p.capacity;
+    for (int pl = 0; pl < kc->n_planes; ++pl) {
+        const i64 idx = k * kc->n_planes + pl;
+        i64 cnt = kc->counts[idx];
+        if (cnt > kc->capacity) cnt = kc->capacity;
         for (i64 c = 0; c < cnt; ++c) {
-            const double* sd = p.seeds + (idx * p.capacity + c) * 5;
+            const double* sd = kc->seeds + (idx * kc->capacity + c) * 5;
             if (sd[0] == 0.0 && sd[1] == 0.0 && sd[2] == 0.0 && sd[3] == 0.0 && sd[4] == 0.0) continue;
-            const i64 m = (i64)p.hits[(idx * p.capacity + c) * 8 + 7];
+            const i64 m = (i64)kc->hits[(idx * kc->capacity + c) * 8 + 7];
             if (m < below && m > best) best = m;
         }
     }
@@ -553,6 +570,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
     const i64 K = p.interval;
     bool exhausted = false;
     for (;;) {
+        const KAdj c = cold(adjoint_params());
         i64 k = -1;
         const bool want = !exhausted;
         const u64 need = __ballot(want);
@@ -560,13 +578,13 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
             const int count = __popcll(need);
             const int leader = __ffsll((long long)need) - 1;
             u64 base = 0;
-            if ((int)lane == leader) base = atomicAdd(p.queue, (u64)count);
+            if ((int)lane == leader) base = atomicAdd(c->queue, (u64)count);
             base = __shfl(base, leader);
             if (want) {
                 const u64 mm = base + (u64)__popcll(need & below);
-                if (mm < (u64)p.n_active) {
-                    const i64 kk = p.active[mm];
-                    if (kk >= 0 && kk < p.n) k = kk;
+                if (mm < (u64)c->n_active) {
+                    const i64 kk = c->active[mm];
+                    if (kk >= 0 && kk < c->n) k = kk;
                 } else {
                     exhausted = true;
                 }
@@ -577,17 +595,17 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
         // a trajectory with no seed, or seeded beyond the checkpoints, adds nothing; the
         // latter (a max_step below the seeded steps) is counted for the caller
         const bool valid = last > 0 && (last - 1) / K < p.n_ckpt;
-        if (__builtin_expect(last > 0 && !valid, 0) && p.dropped != nullptr)
-            __hip_atomic_fetch_add(p.dropped, (i64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__builtin_expect(last > 0 && !valid, 0) && c->dropped != nullptr)
+            __hip_atomic_fetch_add(c->dropped, (i64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!valid) last = 0;
-        const i64 N = p.n;
-        const double qm = valid ? p.qm[k] : 0.0;
+        const i64 N = c->n;
+        const double qm = valid ? c->qm[k] : 0.0;
         Vec3 x = {0.0, 0.0, 0.0}, v = {0.0, 0.0, 0.0};
         // the forward trace's checkpoints when they reach the last seeded step
-        const bool stored = valid && p.fwd_ckpt != nullptr && (last - 1) / K < p.fwd_n_ckpt;
+        const bool stored = valid && c->fwd_ckpt != nullptr && (last - 1) / K < c->fwd_n_ckpt;
         if (valid && !stored) {
-            x = {p.x[k], p.x[N + k], p.x[2 * N + k]};
-            v = {p.v[k], p.v[N + k], p.v[2 * N + k]};
+            x = {c->x[k], c->x[N + k], c->x[2 * N + k]};
+            v = {c->v[k], c->v[N + k], c->v[2 * N + k]};
             // 1. forward to the last seeded step, a checkpoint every K steps
             for (i64 m = 0; m < last; ++m) {
                 if (m % K == 0) put_state(ck, p.lanes, m / K, gl, x, v);
@@ -604,7 +622,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
             const i64 m1 = has ? (m0 + K < last ? m0 + K : last) : m0;
             if (has) {
                 if (stored) {
-                    const double* cp = p.fwd_ckpt + (i64)c * 6 * N + k;
+                    const double* cp = cold(adjoint_params())->fwd_ckpt + (i64)c * 6 * N + k;
                     x = {cp[0], cp[N], cp[2 * N]};
                     v = {cp[3 * N], cp[4 * N], cp[5 * N]};
                 } else {
@@ -623,7 +641,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                     get_state(seg, p.lanes, m - m0, gl, x, v);
                     cell_of<GEOM>(p, x, ci, cj);
                 }
-                window_follow(p, W, act, ci, cj);
+                window_follow(p, adjoint_params(), W, act, ci, cj);
                 if (act) {
                     const bool seeded = m + 1 == next;
                     reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, x, v, lx, lv);
@@ -631,16 +649,21 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                 }
             }
         }
-        cache_flush<0>(p.dfx, p.ny, W, fx_sums);
-        cache_flush<1>(p.dfy, p.ny + 1, W, fy_sums);
-        if (valid && p.dx != nullptr) {
-            p.dx[k] += lx.a; p.dx[N + k] += lx.b; p.dx[2 * N + k] += lx.c;
-        }
-        if (valid && p.dv != nullptr) {
-            p.dv[k] += lv.a; p.dv[N + k] += lv.b; p.dv[2 * N + k] += lv.c;
+        cache_flush<0>(adjoint_params(), p.ny, W, fx_sums);
+        cache_flush<1>(adjoint_params(), p.ny + 1, W, fy_sums);
+        {
+            const KAdj e = cold(adjoint_params());
+            double* const dx = e->dx;
+            double* const dv = e->dv;
+            if (valid && dx != nullptr) {
+                dx[k] += lx.a; dx[N + k] += lx.b; dx[2 * N + k] += lx.c;
+            }
+            if (valid && dv != nullptr) {
+                dv[k] += lv.a; dv[N + k] += lv.b; dv[2 * N + k] += lv.c;
+            }
         }
     }
-    window_flush_all(p, W);
+    window_flush_all(adjoint_params(), W);
 }
 
 __global__ void adjoint_prepare_kernel(u64* __restrict__ queue) {
