@@ -297,10 +297,10 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
                                               GradCache& fx_sums, GradCache& fy_sums, double qm, Vec3 q,
                                               Vec3 lg, Vec3& g) {
     const KAdj kp = adjoint_params();
-    double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0;
+    double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0, ir = 0.0;
     if (GEOM == 1) {
         const Radial rd = radial(q.b, q.c);
-        r = rd.r; qy = r; cy = rd.cy; cz = rd.cz;
+        r = rd.r; qy = r; cy = rd.cy; cz = rd.cz; ir = rd.ir;
     } else if (GEOM == 2) {
         cy = 1.0;
         if (q.b < p.origin_y) cy = -1.0;
@@ -332,8 +332,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     if (GEOM == 1) {
         if (r > 0.0) {
             // r = |(y, z)|, cy = y / r, cz = z / r
-            lq.b = lqy * cy + (lcy * cz * cz - lcz * cy * cz) / r;
-            lq.c = lqy * cz + (lcz * cy * cy - lcy * cy * cz) / r;
+            lq.b = lqy * cy + (lcy * cz * cz - lcz * cy * cz) * ir;
+            lq.c = lqy * cz + (lcz * cy * cy - lcy * cy * cz) * ir;
         }
     } else if (GEOM == 2) {
         lq.b = lqy * cy;
@@ -455,19 +455,19 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         Vec3 sv = vaxpy(a, 2.0, g);
         const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g);
         g = accel<GEOM>(p, F, qm, q2);
-        sv = vaxpy(sv, 2.0, g);
+        sv = vaxpy(sv, 2.0, g);  // (a + 2 g1 + 2 g2: g3 is added below)
         const Vec3 q3 = vaxpy(x, dt, k2), k3 = vaxpy(v, dt, g);
-        g = accel<GEOM>(p, F, qm, q3);
-        sv = vadd(sv, g);
         const Vec3 sx = vadd(vaxpy(vaxpy(v, 2.0, k1), 2.0, k2), k3);
         Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
-        if (seeded) {
-            const Vec3 y = vaxpy(x, dt6, sx), w = vaxpy(v, dt6, sv);
+        if (__builtin_expect(seeded, 0)) {  // the crossing's seeds need the step's end, hence g3 now
+            const Vec3 g3 = accel<GEOM>(p, F, qm, q3);
+            const Vec3 y = vaxpy(x, dt6, sx), w = vaxpy(v, dt6, vadd(sv, g3));
             crossing_seeds(p, k, m, x, v, y, w, xs, vs, lx, lv);
         }
         Lx = lx;
         Lv = lv;
-        const double Ldt6 = vdot(lx, sx) + vdot(lv, sv);
+        // the step's derivative in dt6, lx.sx + lv.sv; lv.g3 is added once g3 is known
+        double Ldt6 = vdot(lx, sx) + vdot(lv, sv);
         const Vec3 Lsx = vscale(dt6, lx), Lsv = vscale(dt6, lv);
         Lv = vadd(Lv, Lsx);
         Vec3 Lk1 = vscale(2.0, Lsx), Lk2 = vscale(2.0, Lsx);
@@ -476,6 +476,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         Vec3 Lg1 = vscale(2.0, Lsv), Lg2 = vscale(2.0, Lsv);
         const Vec3 Lg3 = Lsv;
         const Vec3 Lq3 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q3, Lg3, g);
+        Ldt6 += vdot(lv, g);  // g3, from the transposed evaluation
         Lv = vadd(Lv, Lk3); Lg2 = vaxpy(Lg2, dt, Lk3);
         Lx = vadd(Lx, Lq3); Ldt += vdot(Lq3, k2); Lk2 = vaxpy(Lk2, dt, Lq3);
         const Vec3 Lq2 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q2, Lg2, g);

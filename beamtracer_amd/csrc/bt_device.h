@@ -103,6 +103,7 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
 // axis, where r = 0 (pyccel.py:196-205: r = hypot(y, z), then y / r, z / r when r > 0).
 struct Radial {
     double r, cy, cz;
+    double ir;  // 1 / r off the axis (0 on it)
 };
 
 // The common case is computed without a branch (the GPU does not trap on the reciprocal
@@ -114,6 +115,7 @@ __device__ __forceinline__ Radial radial(double y, double z) {
     o.r = s2 * ir;
     o.cy = y * ir;
     o.cz = z * ir;
+    o.ir = ir;
     if (__builtin_expect(!(s2 >= 0x1p-1000), 0)) {
         if (s2 > 0.0) {  // within 1e-150 of the axis: the same with y, z scaled by 2^600
             const double ys = y * 0x1p600, zs = z * 0x1p600;
@@ -122,10 +124,12 @@ __device__ __forceinline__ Radial radial(double y, double z) {
             o.r = (s2s * irs) * 0x1p-600;
             o.cy = ys * irs;
             o.cz = zs * irs;
+            o.ir = irs * 0x1p600;
         } else {  // on the axis (or a NaN coordinate, which stays NaN in r)
             o.r = s2;
             o.cy = 0.0;
             o.cz = 0.0;
+            o.ir = 0.0;
         }
     }
     return o;
