@@ -168,11 +168,7 @@ class TraceAdjoint:
         t = self.tracer
         mesh = t.mesh
         p = self.particles
-        seeded = np.abs(seeds).sum(axis=(1, 2, 3)) > 0
-        active = np.flatnonzero(seeded).astype(np.int64)
-        steps = hits[..., 7]
-        mask = np.abs(seeds).sum(-1) > 0
-        max_step = int(steps[mask].max()) if mask.any() else 0
+        active, max_step = self.queue(record)
         dev = torch.device("cuda")
         x0 = torch.from_numpy(np.ascontiguousarray(p.position, dtype=float)).to(dev)
         v0 = torch.from_numpy(np.ascontiguousarray(t._state(p.velocity))).to(dev)
@@ -201,6 +197,17 @@ class TraceAdjoint:
         if stream is None:
             self.check_dropped()
         return dfx, dfy
+
+    def queue(self, record=None) -> tuple:
+        """The adjoint kernel's ``active`` (every trajectory with a seed, in index order) and
+        ``max_step`` (the last seeded step) for the evaluation ``record`` (default: the last
+        :meth:`objective`)."""
+        record = record if record is not None else self._records
+        if record is None:
+            raise RuntimeError("objective() first")
+        mask = np.abs(record.seeds).sum(-1) > 0
+        max_step = int(record.hits[..., 7][mask].max()) if mask.any() else 0
+        return np.flatnonzero(mask.reshape(len(mask), -1).any(axis=1)).astype(np.int64), max_step
 
     def check_dropped(self) -> None:
         """Raise if the last :meth:`gradient` left out a seeded trajectory (its last seeded step

@@ -658,10 +658,8 @@ def adjoint_leg(particles: int, repeats: int, interval: int = 64) -> dict:
     hits_d = torch.from_numpy(hits).to(dev)
     counts_d = torch.from_numpy(counts).to(dev)
     seeds_d = torch.from_numpy(seeds).to(dev)
-    seeded = np.flatnonzero(np.abs(seeds).sum(axis=(1, 2, 3)) > 0)
-    active = torch.from_numpy(seeded.astype(np.int64)).to(dev)
-    mask = np.abs(seeds).sum(-1) > 0
-    max_step = int(hits[..., 7][mask].max())
+    seeded, max_step = problem.queue()
+    active = torch.from_numpy(seeded).to(dev)
     times = []
     for r in range(repeats + 1):
         dfx, dfy = torch.zeros_like(fx), torch.zeros_like(fy)
