@@ -119,6 +119,7 @@ class TraceAdjoint:
             raise ValueError("the adjoint takes one uniform mesh, classical and electrostatic")
         if not tracer.planes:
             raise ValueError("the objective needs a diagnostic plane")
+        tracer.require_uniform("the adjoint")
         self.tracer = tracer
         self.particles = particles
         self.plane = int(plane)

@@ -94,6 +94,7 @@ class BatchedTracer:
         for t in self.tracers:
             if t.electric is None:
                 raise RuntimeError("a design has no electric field yet")
+            t.require_uniform("the batched form")
             fx, fy, _, patch_real = t.patch_arrays()
             c, s = t.electrode_lookup()
             mesh = t.mesh

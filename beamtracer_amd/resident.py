@@ -38,6 +38,7 @@ class ResidentTrace:
     def __init__(self, tracer: HipBeamTracer, particles: BeamParticles):
         import torch
 
+        tracer.require_uniform("the resident beam")
         self.tracer = tracer
         self.particles = particles
         self.beam = beam = beam_start(tracer, particles)

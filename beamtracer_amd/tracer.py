@@ -33,6 +33,7 @@ from beamtracer_amd import kernels, tuning
 from beamtracer_amd.beam import BeamParticles
 from beamtracer_amd.fields import StaggeredElectricField
 from beamtracer_amd.mesh import IonOpticsMesh
+from beamtracer_amd.refinement import PatchedElectricField
 
 # the status codes the kernel writes (``tracer.py:35-48``)
 TRANSMITTED, BACKWARD, WALL, ELECTRODE, MAX_STEPS = 0, 1, 2, 3, 4
@@ -337,20 +338,68 @@ class HipBeamTracer:
             self._lookup = (cells, subcells)
         return self._lookup
 
-    def patch_tables(self) -> tuple[np.ndarray, np.ndarray]:
-        """``patch_int`` (P, 6) and ``patch_real`` (P, 7) of the mesh (one patch)."""
+    @property
+    def layout(self):
+        """The composite nodes of a refined field (:class:`~beamtracer_amd.refinement.Layout`),
+        or None for a field of the mesh alone (``tracer.py:673-676``)."""
+        return self.electric.layout if isinstance(self.electric, PatchedElectricField) else None
+
+    def _patches(self) -> tuple[list, list, list]:
+        """The fields, their meshes and the boxes they serve, finest first: one patch for a
+        field of the mesh, every patch of a refined field (``tracer.py:1141-1165``)."""
+        field = self.electric
+        if isinstance(field, PatchedElectricField):
+            if field.mesh is not self.mesh:
+                raise ValueError("the refined field's base mesh is not the tracer's mesh")
+            return field.fields[::-1], field.field_meshes()[::-1], [p.bounds for p in field.patches][::-1]
         mesh = self.mesh
-        nxp, nyp = mesh.shape
-        box = (float(mesh.logical_x(mesh.origin[0])), mesh.origin[1], float(mesh.logical_x(mesh.upper[0])), mesh.upper[1])
-        patch_int = np.array([[nxp, nyp, 0, 0, nxp * nyp, 0]], dtype=np.int64)
-        patch_real = np.array([[*mesh.logical_origin, mesh.h, *box]], dtype=float)
+        return [field], [mesh], [(mesh.origin[0], mesh.origin[1], mesh.upper[0], mesh.upper[1])]
+
+    def patch_tables(self) -> tuple[np.ndarray, np.ndarray]:
+        """``patch_int`` (P, 6) and ``patch_real`` (P, 7) of the field's patches, finest
+        first: node counts, offsets of their ``fx``, ``fy`` and node charges (and the node
+        count); the logical origin, spacing and node box (``tracer.py:1166-1176``)."""
+        _, meshes, boxes = self._patches()
+        base = self.mesh
+        patch_int = np.zeros((len(meshes), 6), dtype=np.int64)
+        patch_real = np.zeros((len(meshes), 7))
+        fx_offset = fy_offset = node_offset = 0
+        for k, (m, (x0, y0, x1, y1)) in enumerate(zip(meshes, boxes, strict=True)):
+            nxp, nyp = m.shape
+            patch_int[k] = (nxp, nyp, fx_offset, fy_offset, nxp * nyp, node_offset)
+            fx_offset += (nxp + 1) * nyp
+            fy_offset += nxp * (nyp + 1)
+            node_offset += nxp * nyp
+            # the kernel works in the uniform coordinate along a graded x
+            patch_real[k] = (*m.logical_origin, m.h, float(base.logical_x(x0)), y0, float(base.logical_x(x1)), y1)
         return patch_int, patch_real
 
     def patch_arrays(self) -> tuple:
-        """``fxs``, ``fys``, ``patch_int`` (P, 6), ``patch_real`` (P, 7) of the field (one patch)."""
+        """``fxs``, ``fys`` (every patch's edge fields one after the other, finest first),
+        ``patch_int`` (P, 6), ``patch_real`` (P, 7)."""
+        fields, _, _ = self._patches()
         patch_int, patch_real = self.patch_tables()
-        flat = (np.ascontiguousarray(c, dtype=float).ravel() for c in (self.electric.fx, self.electric.fy))
-        return (*flat, patch_int, patch_real)
+        fxs = np.concatenate([np.ascontiguousarray(f.fx, dtype=float).ravel() for f in fields])
+        fys = np.concatenate([np.ascontiguousarray(f.fy, dtype=float).ravel() for f in fields])
+        return fxs, fys, patch_int, patch_real
+
+    def node_charges(self, slabs: np.ndarray) -> np.ndarray:
+        """The node charges of the kernel's flat charge slabs: ``(nx, ny)`` for a field of
+        the mesh, the composite nodes' charges (shared nodes added) for a refined field
+        (``tracer.py:1264-1276``)."""
+        _, meshes, _ = self._patches()
+        patch_int, _ = self.patch_tables()
+        per_patch = [slabs[o : o + m.shape[0] * m.shape[1]].reshape(m.shape) for m, o in zip(meshes, patch_int[:, 5])][::-1]
+        layout = self.layout
+        if layout is None:
+            return per_patch[0]
+        # the charges of the boxes, without the margins of their fields' meshes
+        return layout.deposit_total([c[p.inner] for c, p in zip(per_patch, layout.patches, strict=True)])
+
+    def require_uniform(self, what: str) -> None:
+        """Raise unless the field is one uniform patch (the forms that take no refined field)."""
+        if self.layout is not None and self.layout.refined:
+            raise ValueError(f"{what} takes a field of the mesh alone, not a refined field")
 
     def magnetic_nodes(self):
         """The node magnetic field: three ``(nx, ny)`` arrays (or None entries), or None."""
@@ -363,7 +412,11 @@ class HipBeamTracer:
         return tuple(None if c is None else np.ascontiguousarray(c, dtype=float) for c in self.magnetic)
 
     def finalize(self, charge: np.ndarray) -> np.ndarray:
-        """The charge density: node charges over :func:`node_volumes`."""
+        """The charge density: node charges over :func:`node_volumes`, or over the composite
+        nodes' control volumes for a refined field (``tracer.py:700-705``)."""
+        layout = self.layout
+        if layout is not None:
+            return charge / layout.volumes
         return charge / node_volumes(self.mesh)
 
     # -- the trace ---------------------------------------------------------------------
@@ -430,7 +483,7 @@ class HipBeamTracer:
         self.last_state = (x, v, steps)
         self.last_records = (hits, counts)
         records = (hits, counts, paths, path_count)
-        return trace_result(beam, charge[0].reshape(mesh.shape), status, electrode, steps, records, self.planes)
+        return trace_result(beam, self.node_charges(charge[0]), status, electrode, steps, records, self.planes)
 
     def _launch(self, call: list, magnetic, exponent: int) -> None:
         """One ``trace_trajectories_2d`` call with the 35 arguments of ``call``."""
