@@ -21,6 +21,7 @@ each field update by :meth:`BatchedTracer.replay`. :func:`autotune` picks the la
 from __future__ import annotations
 
 import ctypes
+import re
 import time
 from collections.abc import Sequence
 from dataclasses import dataclass
@@ -63,6 +64,14 @@ class BatchedTracer:
     Every design is a single uniform mesh (no refined patches, no graded axis), classical
     and electrostatic, without saved paths; all share the integrator, geometry, step
     fraction, step limit and number of diagnostic planes.
+
+    A design that cannot be traced -- no field yet, a refined or non-finite field, a mesh
+    beyond the kernel's limits, a non-finite beam -- is left out of the launches instead of
+    failing the batch (the batched counterpart of ``optimization/pool.py:131-158``, where
+    an evaluation that fails becomes a failed evaluation and the others go on): its result
+    carries the reason in ``failure`` (``failures`` keeps them by design), and the other
+    designs' results are those of a batch without it, bit for bit. A traced design whose
+    node charges or plane records come out non-finite gets a ``failure`` too.
     """
 
     def __init__(self, tracers: Sequence[HipBeamTracer], shape: LaunchShape | None = None) -> None:
@@ -79,6 +88,9 @@ class BatchedTracer:
         self.tracers = list(tracers)
         self.shape = shape or LaunchShape()
         self.capacity = 4
+        #: design -> why it is left out of the launches
+        self.failures: dict[int, str] = {}
+        self._live: list[int] = []
         self._plan = None
         self._graph = None
         self._table = None
@@ -87,14 +99,44 @@ class BatchedTracer:
 
     # -- arenas and plan ----------------------------------------------------------------
 
+    @staticmethod
+    def _field_problem(t: HipBeamTracer) -> str | None:
+        """Why a design's field cannot be traced, or None."""
+        if t.electric is None:
+            return "no electric field yet"
+        if t.layout is not None and t.layout.refined:
+            return "a refined field (the batched form takes a field of the mesh alone)"
+        if not (np.isfinite(t.electric.fx).all() and np.isfinite(t.electric.fy).all()):
+            return "a non-finite electric field"
+        return None
+
     def _build_arenas(self) -> None:
+        """The live designs' fields, electrode lookups and planes one after the other, and
+        the plan that checks them; a design the plan rejects is left out."""
+        for d, t in enumerate(self.tracers):
+            problem = None if d in self.failures else self._field_problem(t)
+            if problem is not None:
+                self.failures[d] = problem
+        while True:
+            self._pack_arenas()
+            if not self._live:
+                return
+            try:
+                self._create_plan()
+                return
+            except ValueError as error:  # the limits of one design: leave it out, pack again
+                found = re.search(r"design (\d+): ", str(error))
+                if found is None:
+                    raise
+                self.failures[self._live[int(found.group(1))]] = str(error)
+
+    def _pack_arenas(self) -> None:
         fxs, fys, cells, subs, planes = [], [], [], [], []
         rows = []
         ofx = ofy = ofc = ofs = ofn = 0
-        for t in self.tracers:
-            if t.electric is None:
-                raise RuntimeError("a design has no electric field yet")
-            t.require_uniform("the batched form")
+        self._live = [d for d in range(len(self.tracers)) if d not in self.failures]
+        for d in self._live:
+            t = self.tracers[d]
             fx, fy, _, patch_real = t.patch_arrays()
             c, s = t.electrode_lookup()
             mesh = t.mesh
@@ -113,18 +155,45 @@ class BatchedTracer:
             ofn += nx * ny
         self._rows = rows
         self.total_nodes = ofn
+        n_planes = len(self.tracers[0].planes)
+        self._n_planes = n_planes
+        if not rows:
+            return
         self._fxs = np.concatenate(fxs)
         self._fys = np.concatenate(fys)
         self._cells = np.concatenate(cells).astype(np.int64)
         self._subcells = np.concatenate(subs).astype(np.int64)
-        n_planes = len(self.tracers[0].planes)
-        self._planes = np.concatenate(planes) if n_planes else np.full(len(self.tracers), np.nan)
-        self._n_planes = n_planes
+        self._planes = np.concatenate(planes) if n_planes else np.full(len(rows), np.nan)
+
+    def _create_plan(self) -> None:
+        """The device plan of the live designs (the kernel's checks of every design)."""
+        L = _lib.lib()
+        plan = ctypes.c_void_p()
+        rc = L.bt_batch_plan_create(self._design_table([0] * len(self._rows)), len(self._rows), self._fxs.size,
+                                    self._fys.size, self._cells.size, self._subcells.size, ctypes.byref(plan))  # fmt: skip
+        _lib.check(rc, "bt_batch_plan_create")
+        self._plan = plan
+        self._table = None
+
+    def _rebuild(self) -> None:
+        """Pack the live designs again after a new failure (the captured graph goes too)."""
+        self.close()
+        self._dev = {}
+        self._build_arenas()
 
     def update_fields(self) -> None:
-        """Take every design's current electric field (e.g. after a field solve)."""
-        self._fxs = np.concatenate([t.patch_arrays()[0] for t in self.tracers])
-        self._fys = np.concatenate([t.patch_arrays()[1] for t in self.tracers])
+        """Take every design's current electric field (e.g. after a field solve); a design
+        whose new field cannot be traced is left out from now on (and a captured graph,
+        which holds the old designs, must be captured again)."""
+        fresh = {d: self._field_problem(self.tracers[d]) for d in self._live}
+        if any(problem is not None for problem in fresh.values()):
+            self.failures.update({d: problem for d, problem in fresh.items() if problem is not None})
+            self._rebuild()
+            return
+        if not self._live:
+            return
+        self._fxs = np.concatenate([self.tracers[d].patch_arrays()[0] for d in self._live])
+        self._fys = np.concatenate([self.tracers[d].patch_arrays()[1] for d in self._live])
         if "fxs" in self._dev:
             import torch
 
@@ -132,31 +201,26 @@ class BatchedTracer:
             self._dev["fys"].copy_(torch.from_numpy(self._fys))
 
     def _design_table(self, exponents: Sequence[int]):
-        table = (_lib.BatchDesign * len(self.tracers))()
+        table = (_lib.BatchDesign * len(self._rows))()
         for d, (row, e) in enumerate(zip(self._rows, exponents)):
             table[d] = _lib.BatchDesign(*row, int(e), 0)
         return table
 
     def _ensure_plan(self, exponents: Sequence[int]) -> None:
+        """The live designs' exponents into the plan; the arenas on the device."""
         import torch
 
-        L = _lib.lib()
-        table = self._design_table(exponents)
         if self._plan is None:
-            plan = ctypes.c_void_p()
-            rc = L.bt_batch_plan_create(table, len(self.tracers), self._fxs.size, self._fys.size, self._cells.size,
-                                        self._subcells.size, ctypes.byref(plan))  # fmt: skip
-            _lib.check(rc, "bt_batch_plan_create")
-            self._plan = plan
+            raise RuntimeError("every design is left out of the batch")
+        rc = _lib.lib().bt_batch_plan_update(self._plan, self._design_table(exponents), len(self._rows))
+        _lib.check(rc, "bt_batch_plan_update")
+        if "fxs" not in self._dev:
             self._dev["fxs"] = torch.from_numpy(self._fxs).cuda()
             self._dev["fys"] = torch.from_numpy(self._fys).cuda()
             self._dev["cells"] = torch.from_numpy(self._cells).cuda()
             self._dev["subcells"] = torch.from_numpy(self._subcells).cuda()
             self._dev["planes"] = torch.from_numpy(np.ascontiguousarray(self._planes)).cuda()
             self._dev["charge"] = torch.zeros(self.total_nodes, dtype=torch.float64, device="cuda")
-        else:
-            rc = L.bt_batch_plan_update(self._plan, table, len(self.tracers))
-            _lib.check(rc, "bt_batch_plan_update")
         self._table = list(exponents)
 
     def close(self) -> None:
@@ -177,12 +241,28 @@ class BatchedTracer:
     # -- the trace ----------------------------------------------------------------------
 
     def _prepare(self, particles: Sequence[BeamParticles]) -> dict:
-        """Concatenated host state of every design's beam, as the plain tracer starts it."""
+        """Concatenated host state of every live design's beam, as the plain tracer starts
+        it; a design whose beam cannot be traced is left out first."""
         if len(particles) != len(self.tracers):
             raise ValueError("one beam per design")
-        beams = [beam_start(t, p) for t, p in zip(self.tracers, particles)]
+        beams, bad = [], {}
+        for d in self._live:
+            p = particles[d]
+            finite = all(np.isfinite(np.asarray(a, dtype=float)).all()
+                         for a in (p.position, p.velocity, p.charge, p.mass, p.current))  # fmt: skip
+            if not finite:
+                bad[d] = "a non-finite beam (position, velocity, charge, mass or current)"
+                continue
+            try:
+                beams.append(beam_start(self.tracers[d], p))
+            except ValueError as error:
+                bad[d] = f"the beam: {error}"
+        if bad:
+            self.failures.update(bad)
+            self._rebuild()
+            beams = [beam_start(self.tracers[d], particles[d]) for d in self._live]
         exponents = []
-        for t, b in zip(self.tracers, beams):
+        for t, b in zip((self.tracers[d] for d in self._live), beams):
             fx, fy, _, patch_real = t.patch_arrays()
             exponents.append(kernels.trace_exponent(b.start, b.v, b.dq, b.qm, fx, fy, patch_real, t.step_fraction,
                                                     t._inv_c2()))  # fmt: skip
@@ -193,6 +273,8 @@ class BatchedTracer:
         def joined(name, axis=0):
             return np.ascontiguousarray(np.concatenate([getattr(b, name) for b in beams], axis=axis))
 
+        if not beams:
+            return {"n": 0, "first": [0], "exponents": [], "starts": []}
         return {
             "x": joined("x", 1), "v": joined("v", 1), "qm": joined("qm"), "dq": joined("dq"),
             "current": joined("current"), "species": joined("species"),
@@ -203,14 +285,15 @@ class BatchedTracer:
         }  # fmt: skip
 
     def _queue(self, starts: Sequence[np.ndarray]) -> np.ndarray:
-        """The active list: designs in queue order (``longest``: most steps first)."""
-        order = range(len(self.tracers))
+        """The active list: live designs in queue order (``longest``: most steps first)."""
+        order = range(len(self._live))
         if self.shape.order == "longest":
-            steps = [(t.mesh.upper[0] - t.mesh.origin[0]) / t.mesh.h for t in self.tracers]
-            order = sorted(order, key=lambda d: -steps[d])
+            meshes = [self.tracers[d].mesh for d in self._live]
+            steps = [(m.upper[0] - m.origin[0]) / m.h for m in meshes]
+            order = sorted(order, key=lambda k: -steps[k])
         elif self.shape.order != "design":
             raise ValueError(f"order must be one of {ORDERS}")
-        parts = [starts[d] for d in order]
+        parts = [starts[k] for k in order]
         return np.concatenate(parts).astype(np.int64) if parts else np.zeros(0, np.int64)
 
     def _args(self, dev: dict, active, reset: bool):
@@ -286,7 +369,7 @@ class BatchedTracer:
             hit = np.full(paused.size, -1, dtype=np.int64)
             for d in np.unique(design):
                 sel = design == d
-                t = self.tracers[d]
+                t = self.tracers[self._live[d]]
                 hit[sel] = t.mesh.inside_solid(*t._project(x[:, sel]))
             inside = hit >= 0
             if inside.any():
@@ -331,7 +414,7 @@ class BatchedTracer:
         status = torch.where(dead, dev["status0"], dev["status"]).to(torch.int8).cpu().numpy()
         electrode = torch.where(dead, dev["electrode0"], dev["electrode"]).to(torch.int32).cpu().numpy()
         design_of = dev["design_of"].to(torch.int64)
-        n_designs = len(self.tracers)
+        n_designs = len(self._live)
         steps = dev["steps"]
         longest = torch.zeros(n_designs, dtype=torch.int64, device=steps.device)
         longest.scatter_reduce_(0, design_of, steps, "amax")
@@ -341,8 +424,9 @@ class BatchedTracer:
         records = self._plane_records(dev)
         first = np.asarray(host["first"])
         bounds = [np.searchsorted(k, first) for k, _ in records]
-        out = []
-        for d, t in enumerate(self.tracers):
+        out = self._failed_results()
+        for d, design in enumerate(self._live):
+            t = self.tracers[design]
             lo, hi = host["first"][d], host["first"][d + 1]
             row = self._rows[d]
             nodes = charge[row[14]: row[14] + row[15]].reshape(t.mesh.shape)
@@ -356,7 +440,9 @@ class BatchedTracer:
                 plane.species = host["species"][kd].astype(np.int32)
                 plane.direction = rd[:, 6].astype(np.int8)
                 planes.append(plane)
-            out.append(TraceResult(
+            finite = np.isfinite(nodes).all() and all(np.isfinite(p.point).all() and np.isfinite(p.velocity).all()
+                                                      for p in planes)  # fmt: skip
+            out[design] = TraceResult(
                 charge_density=t.finalize(nodes.copy()),
                 status=status[lo:hi],
                 electrode=electrode[lo:hi],
@@ -365,20 +451,45 @@ class BatchedTracer:
                 trajectory_species=np.zeros(0, dtype=np.int32),
                 steps=int(longest[d]),
                 particle_steps=int(total[d]),
-            ))  # fmt: skip
+                failure=None if finite else "non-finite node charges or plane records",
+            )  # fmt: skip
         self._first = host["first"]
+        return out
+
+    def _failed_results(self) -> list:
+        """One result per design: for a design left out, its reason and no trace (NaN charge
+        density, no trajectories, empty planes); None for the live ones."""
+        out: list = [None] * len(self.tracers)
+        for d, reason in self.failures.items():
+            t = self.tracers[d]
+            out[d] = TraceResult(
+                charge_density=np.full(t.mesh.shape, np.nan),
+                status=np.zeros(0, dtype=np.int8),
+                electrode=np.zeros(0, dtype=np.int32),
+                planes=[PlaneHits(position) for position in t.planes],
+                trajectories=[],
+                trajectory_species=np.zeros(0, dtype=np.int32),
+                steps=0,
+                failure=reason,
+            )  # fmt: skip
         return out
 
     def end_state(self, design: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """Positions, velocities and step counts of one design's trajectories at the end of
         the last trace (what :class:`HipBeamTracer` keeps as ``last_state``)."""
+        if design in self.failures:
+            raise ValueError(f"design {design} was left out of the batch: {self.failures[design]}")
         dev = self.last_device_state
-        lo, hi = self._first[design], self._first[design + 1]
+        k = self._live.index(design)
+        lo, hi = self._first[k], self._first[k + 1]
         return dev["x"][:, lo:hi].cpu().numpy(), dev["v"][:, lo:hi].cpu().numpy(), dev["steps"][lo:hi].cpu().numpy()
 
     def trace(self, particles: Sequence[BeamParticles]) -> list[TraceResult]:
-        """Trace every design's beam; one :class:`TraceResult` per design."""
+        """Trace every design's beam; one :class:`TraceResult` per design (with the reason
+        in ``failure`` for a design left out)."""
         host = self._prepare(particles)
+        if not self._live:
+            return self._failed_results()
         self._ensure_plan(host["exponents"])
         capacity = self.capacity
         while True:
@@ -399,6 +510,8 @@ class BatchedTracer:
     def capture(self, particles: Sequence[BeamParticles]) -> None:
         """Record the trace from the emitter (reset, queue, trace, finalize) as a HIP graph."""
         host = self._prepare(particles)
+        if not self._live:
+            raise RuntimeError("every design is left out of the batch")
         self._ensure_plan(host["exponents"])
         dev = self._upload(host, self.capacity)
         L = _lib.lib()
@@ -423,7 +536,7 @@ class BatchedTracer:
         host, dev, _ = self._graph_state
         if refresh_exponents:
             fresh = []
-            for d, t in enumerate(self.tracers):
+            for d, t in enumerate(self.tracers[k] for k in self._live):
                 lo, hi = host["first"][d], host["first"][d + 1]
                 fx, fy, _, patch_real = t.patch_arrays()
                 fresh.append(kernels.trace_exponent(host["starts"][d] - lo, host["v"][:, lo:hi], host["dq"][lo:hi],

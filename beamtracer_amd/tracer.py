@@ -97,6 +97,8 @@ class TraceResult:
     steps: int
     #: the particle-steps of this trace: the sum of every trajectory's steps
     particle_steps: int = 0
+    #: why the batched form left this design out (or found its results non-finite), else None
+    failure: str | None = None
 
 
 @dataclass

@@ -81,3 +81,50 @@ def test_graph_replays_equal_eager_traces_and_follow_field_updates():
     replayed = bt.replay()
     for d, t, g in zip(ds, tracers, replayed):
         _assert_identical(g, t.trace(d.particles))
+
+
+def _nan_beam(particles):
+    import copy
+
+    bad = copy.deepcopy(particles)
+    bad.velocity[:, 3] = np.nan
+    return bad
+
+
+def test_designs_that_cannot_be_traced_are_left_out_and_the_others_are_unchanged():
+    ds, tracers = _small_batch(6, seed=11)
+    beams = [d.particles for d in ds]
+    tracers[1].electric = None  # no field yet
+    fx = tracers[2].electric.fx.copy()
+    fx[3, 4] = np.nan
+    tracers[2].electric = StaggeredElectricField.from_edges(tracers[2].mesh, fx, tracers[2].electric.fy)
+    beams[4] = _nan_beam(beams[4])
+    bt = batch.BatchedTracer(tracers)
+    got = bt.trace(beams)
+    assert set(bt.failures) == {1, 2, 4}
+    assert "no electric field" in got[1].failure and "non-finite electric field" in got[2].failure
+    assert "non-finite beam" in got[4].failure
+    for k in (1, 2, 4):
+        assert np.isnan(got[k].charge_density).all() and got[k].status.size == 0
+        with pytest.raises(ValueError, match="left out"):
+            bt.end_state(k)
+    # the traced designs: as the plain tracer and as a batch without the others, bit for bit
+    alone = batch.BatchedTracer([tracers[k] for k in (0, 3, 5)]).trace([beams[k] for k in (0, 3, 5)])
+    for k, w in zip((0, 3, 5), alone):
+        assert got[k].failure is None
+        _assert_identical(got[k], w)
+        _assert_identical(got[k], tracers[k].trace(beams[k]))
+        for a, b in zip(bt.end_state(k), tracers[k].last_state):
+            np.testing.assert_array_equal(a, b)
+    # a field update that breaks a traced design leaves it out of the next trace
+    bt.capture(beams)
+    fy = tracers[5].electric.fy.copy()
+    fy[0, 0] = np.inf
+    tracers[5].electric = StaggeredElectricField.from_edges(tracers[5].mesh, tracers[5].electric.fx, fy)
+    bt.update_fields()
+    with pytest.raises(RuntimeError, match="nothing captured"):
+        bt.replay()
+    again = bt.trace(beams)
+    assert set(bt.failures) == {1, 2, 4, 5} and again[5].failure is not None
+    _assert_identical(again[0], got[0])
+    _assert_identical(again[3], got[3])

@@ -141,3 +141,4 @@ def test_caller_packs_patches_finest_first_and_unpacks_the_charges():
     # a field of the mesh alone keeps the (nx, ny) charges
     plain = HipBeamTracer(mesh, field.fields[0])
     assert plain.node_charges(np.arange(41 * 21.0)).shape == (41, 21) and plain.layout is None
+
