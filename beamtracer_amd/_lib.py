@@ -223,7 +223,7 @@ SIGNATURES = {
     "bt_batch_plan_destroy": (ctypes.c_int, [_ptr]),
     "bt_batch_plan_update": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchDesign), _i64]),
     "bt_batch_plan_nodes": (_i64, [_ptr]),
-    "bt_batch_occupancy": (ctypes.c_int, [_i64, _i64, _i32]),
+    "bt_batch_occupancy": (ctypes.c_int, [_i64, _i64, _i32, _i32]),
     "bt_batch_trace_device": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchArgs), _ptr]),
     "bt_batch_graph_create": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchArgs), ctypes.POINTER(_ptr)]),
     "bt_graph_launch": (ctypes.c_int, [_ptr, _ptr]),

@@ -21,6 +21,7 @@ each field update by :meth:`BatchedTracer.replay`. :func:`autotune` picks the la
 from __future__ import annotations
 
 import ctypes
+import itertools
 import re
 import time
 from collections.abc import Sequence
@@ -48,14 +49,17 @@ ORDERS = ("design", "longest")
 
 @dataclass
 class LaunchShape:
-    """Threads per block (0: 256), resident blocks (0: all that fit), queue order, and whether
+    """Threads per block (0: 256), resident blocks (0: all that fit), queue order, whether
     a workgroup takes new trajectories together (consecutive queue entries, so its lanes
-    mostly share one design and its deposit window) or each lane as it frees."""
+    mostly share one design and its deposit window) or each lane as it frees, and the
+    deposit tile: each wave's window of node sums in LDS spans 16 mesh columns, or 32 with
+    ``wide_window`` (for lanes spread further along x; twice the LDS per wave)."""
 
     block_size: int = 0
     max_blocks: int = 0
     order: str = "longest"
     refill_block: bool = False
+    wide_window: bool = False
 
 
 class BatchedTracer:
@@ -316,7 +320,8 @@ class BatchedTracer:
         a.integrator = INTEGRATORS.index(t0.integrator)
         a.step_fraction, a.max_steps, a.refine = t0.step_fraction, t0.max_steps, SUBCELLS
         a.block_size, a.max_blocks = self.shape.block_size, self.shape.max_blocks
-        a.flags = _lib.BT_REFILL_BLOCK if self.shape.refill_block else 0
+        a.flags = (_lib.BT_REFILL_BLOCK if self.shape.refill_block else 0) | (
+            _lib.BT_WIDE_WINDOW if self.shape.wide_window else 0)
         return a
 
     def _upload(self, host: dict, capacity: int) -> dict:
@@ -596,13 +601,16 @@ def vacuum_tracers(designs: Sequence, processes: int = 0) -> list[HipBeamTracer]
 
 
 def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, blocks=(64, 128, 192, 256),
-             fills=(1.0, 0.5), orders=ORDERS, refills=(False, True), repeats: int = 2) -> dict:
+             fills=(1.0, 0.5), orders=ORDERS, refills=(False, True), windows=(False, True),
+             repeats: int = 2) -> dict:  # fmt: skip
     """Time one batched trace per launch shape and keep the fastest in ``tracer.shape``.
 
     The candidates are threads per block, the fraction of the resident blocks the grid
-    fills, the queue order and the refill mode (lane by lane, or a workgroup together); each
-    is timed (HIP events on the launch stream, best of ``repeats``) on the real batch.
-    Returns every timing in ms and the choice.
+    fills, the queue order, the refill mode (lane by lane, or a workgroup together) and the
+    deposit tile (windows of 16 or 32 columns); each is timed (HIP events on the launch
+    stream, best of ``repeats``) on the real batch. Every trajectory takes one lane: a
+    trajectory's steps are a chain of dependent field gathers, which more lanes would
+    duplicate rather than share. Returns every timing in ms and the choice.
     """
     import torch
 
@@ -616,12 +624,13 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
     for order in orders:
         tracer.shape = LaunchShape(0, 0, order)
         dev = tracer._upload(host, tracer.capacity)
-        for block in blocks:
-            per_cu = L.bt_batch_occupancy(INTEGRATORS.index(t0.integrator), geometry_code(t0.mesh), block)
+        for block, wide in itertools.product(blocks, windows):
+            flags = _lib.BT_WIDE_WINDOW if wide else 0
+            per_cu = L.bt_batch_occupancy(INTEGRATORS.index(t0.integrator), geometry_code(t0.mesh), block, flags)
             for fill in fills:
                 grid = max(1, int(per_cu * cus * fill))
                 for refill in refills:
-                    tracer.shape = LaunchShape(block, grid, order, refill)
+                    tracer.shape = LaunchShape(block, grid, order, refill, wide)
                     times = []
                     for _ in range(repeats):
                         tracer._restart(dev)
@@ -633,10 +642,11 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
                         end.record(stream)
                         torch.cuda.synchronize()
                         times.append(start.elapsed_time(end))
-                    key = f"{order}/block{block}/fill{fill}" + ("/refill-block" if refill else "")
+                    key = (f"{order}/block{block}/fill{fill}" + ("/refill-block" if refill else "")
+                           + ("/window32" if wide else ""))  # fmt: skip
                     timings[key] = min(times)
                     if best is None or min(times) < best[0]:
-                        best = (min(times), LaunchShape(block, grid, order, refill))
+                        best = (min(times), LaunchShape(block, grid, order, refill, wide))
     assert best is not None
     tracer.shape = best[1]
     return {"timings_ms": timings, "choice": vars(best[1]), "best_ms": best[0], "time": time.time()}

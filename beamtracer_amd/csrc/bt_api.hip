@@ -418,9 +418,10 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
     const bt::TraceParams p = batch_params(plan, a);
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const int block = a->block_size > 0 ? a->block_size : 256;
+    const bool wide = (a->flags & BT_WIDE_WINDOW) != 0;
     int grid = 0;
     if (a->n_active > 0) {
-        int per_cu = bt::batched_occupancy(integ, geom, false, block);
+        int per_cu = bt::batched_occupancy(integ, geom, false, wide, block);
         if (per_cu < 1) per_cu = 1;
         const i64 full = (i64)per_cu * plan->cus;
         const i64 need = (a->n_active + block - 1) / block;
@@ -429,7 +430,7 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
         if (grid < 1) grid = 1;
     }
     const bool reset = a->x0 != nullptr;
-    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, false, plan->total_nodes,
+    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, false, wide, plan->total_nodes,
                            plan->max_nodes))
         return fail(BT_E_LAUNCH, std::string("batched launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;
@@ -1084,10 +1085,10 @@ int64_t bt_batch_plan_nodes(void* handle) {
     return plan ? plan->total_nodes : -1;
 }
 
-int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size) {
+int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size, int32_t flags) {
     if (integrator < 0 || integrator > 1 || cylindrical < 0 || cylindrical > 2) return fail(BT_E_ARG, "bad variant");
     if (block_size % 64 != 0 || block_size < 64 || block_size > 256) return fail(BT_E_ARG, "bad block size");
-    return bt::batched_occupancy((int)integrator, (int)cylindrical, false, block_size);
+    return bt::batched_occupancy((int)integrator, (int)cylindrical, false, (flags & BT_WIDE_WINDOW) != 0, block_size);
 }
 
 int bt_batch_trace_device(void* handle, const bt_batch_args* a, void* stream) {

@@ -77,10 +77,11 @@ int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records,
 // Batched form: one persistent launch over the trajectories of every design (single-patch,
 // uniform, classical meshes); `reset` first restores x, v from x0, v0 and clears steps,
 // status, electrode, counts and charge.
-TraceKernel select_batched_kernel(int integ, int geom, bool has_b);
+TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide);
 int launch_batched(const TraceParams& p, const double* x0, const double* v0, bool reset, int block, int grid,
-                   hipStream_t stream, int integ, int geom, bool has_b, i64 total_nodes, i64 max_design_nodes);
-int batched_occupancy(int integ, int geom, bool has_b, int block);
+                   hipStream_t stream, int integ, int geom, bool has_b, bool wide, i64 total_nodes,
+                   i64 max_design_nodes);
+int batched_occupancy(int integ, int geom, bool has_b, bool wide, int block);
 
 // The adjoint of a trace (bt_adjoint.hip): seeds dJ/d(record) of the plane crossings ->
 // dJ/dfx, dJ/dfy (and the initial states' adjoints), one uniform mesh, classical, no B.

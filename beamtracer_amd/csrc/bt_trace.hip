@@ -867,10 +867,12 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
 }
 
 // Every design's trajectories in one persistent launch.  Held to three waves per SIMD (168
-// VGPRs): a batch always fills the GPU, so the third wave's latency hiding pays.
-template <int INTEG, int GEOM, bool HAS_B>
+// VGPRs): a batch always fills the GPU, so the third wave's latency hiding pays.  WC: the
+// columns of each wave's deposit window (a wide window of 32 holds lanes spread further
+// along x, at twice the LDS).
+template <int INTEG, int GEOM, bool HAS_B, int WC>
 __global__ __launch_bounds__(256, 3) void batched_kernel(TraceParams p) {
-    trace_loop<INTEG, GEOM, HAS_B, false, true, false>(p);
+    trace_loop<INTEG, GEOM, HAS_B, false, true, false, WC>(p);
 }
 
 // The start of a batched trace from the emitter: x, v from x0, v0; steps 0; status and
@@ -968,9 +970,9 @@ int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, 
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-TraceKernel select_batched_kernel(int integ, int geom, bool has_b) {
+TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide) {
 #define BT_SELB(I, G, B) \
-    if (integ == I && geom == G && has_b == B) return batched_kernel<I, G, B>;
+    if (integ == I && geom == G && has_b == B) return wide ? batched_kernel<I, G, B, kWideCols> : batched_kernel<I, G, B, kWinCols>;
     BT_SELB(0, 0, false) BT_SELB(0, 1, false) BT_SELB(0, 2, false)
     BT_SELB(1, 0, false) BT_SELB(1, 1, false) BT_SELB(1, 2, false)
 #undef BT_SELB
@@ -978,8 +980,9 @@ TraceKernel select_batched_kernel(int integ, int geom, bool has_b) {
 }
 
 int launch_batched(const TraceParams& p, const double* x0, const double* v0, bool reset, int block, int grid,
-                   hipStream_t stream, int integ, int geom, bool has_b, i64 total_nodes, i64 max_design_nodes) {
-    TraceKernel kern = select_batched_kernel(integ, geom, has_b);
+                   hipStream_t stream, int integ, int geom, bool has_b, bool wide, i64 total_nodes,
+                   i64 max_design_nodes) {
+    TraceKernel kern = select_batched_kernel(integ, geom, has_b, wide);
     if (kern == nullptr) return -1;
     if (reset) {
         i64 work = 3 * p.n > total_nodes ? 3 * p.n : total_nodes;
@@ -994,7 +997,7 @@ int launch_batched(const TraceParams& p, const double* x0, const double* v0, boo
     if (zgrid < 1) zgrid = 1;
     hipLaunchKernelGGL(prepare_kernel, dim3(zgrid), dim3(256), 0, stream, p.acc, words, p.queue);
     if (p.n_active > 0 && grid > 0)
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), window_bytes(block, false), stream, p);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), window_bytes(block, wide), stream, p);
     int fgrid = (int)((max_design_nodes + 255) / 256);
     if (fgrid > 1024) fgrid = 1024;
     if (fgrid < 1) fgrid = 1;
@@ -1004,12 +1007,12 @@ int launch_batched(const TraceParams& p, const double* x0, const double* v0, boo
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int batched_occupancy(int integ, int geom, bool has_b, int block) {
-    TraceKernel kern = select_batched_kernel(integ, geom, has_b);
+int batched_occupancy(int integ, int geom, bool has_b, bool wide, int block) {
+    TraceKernel kern = select_batched_kernel(integ, geom, has_b, wide);
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),
-                                                     block, window_bytes(block, false)) != hipSuccess)
+                                                     block, window_bytes(block, wide)) != hipSuccess)
         return 0;
     return blocks;
 }

@@ -275,7 +275,9 @@ typedef struct bt_batch_args {
     int32_t block_size, max_blocks;               /* 0: 256 threads, all resident blocks    */
     int32_t flags, reserved;                      /* BT_REFILL_BLOCK: a workgroup takes new
                                                      trajectories together (consecutive
-                                                     queue entries: mostly one design)      */
+                                                     queue entries: mostly one design);
+                                                     BT_WIDE_WINDOW: 32-column deposit
+                                                     windows                                */
 } bt_batch_args;
 
 int64_t bt_batch_args_size(void);
@@ -290,8 +292,9 @@ int bt_batch_plan_destroy(void* plan);
 int bt_batch_plan_update(void* plan, const bt_batch_design* designs, int64_t n_designs);
 /* Total nodes of the plan's designs (the length of `charge`). */
 int64_t bt_batch_plan_nodes(void* plan);
-/* Resident blocks per CU of the batched kernel (for launch-shape tuning). */
-int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size);
+/* Resident blocks per CU of the batched kernel (for launch-shape tuning); `flags` are the
+   trace's flags, of which BT_WIDE_WINDOW (the 32-column deposit windows) counts here. */
+int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size, int32_t flags);
 /* One batched trace, asynchronous on `stream`; allocates nothing. */
 int bt_batch_trace_device(void* plan, const bt_batch_args* args, void* stream);
 /* Captures bt_batch_trace_device(plan, args) as a HIP graph (kernel nodes only). */
