@@ -15,7 +15,8 @@ PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIB_PATH = PACKAGE / "libbeamtracer_hip.so"
 SOURCES = [PACKAGE / "csrc" / name for name in ("bt_api.hip", "bt_trace.hip", "bt_gather_deposit.hip",
-                                                "bt_adjoint.hip", "bt_fields.hip", "bt_trace3d.hip")]
+                                                "bt_adjoint.hip", "bt_fields.hip", "bt_trace3d.hip",
+                                                "bt_memory.hip")]
 HEADERS = [ROOT / "include" / "beamtracer_hip.h"] + sorted((PACKAGE / "csrc").glob("*.h"))
 
 #: hipcc flags of the library: FP64 results in the reference's operation order (no fused
@@ -228,6 +229,21 @@ SIGNATURES = {
     "bt_batch_graph_create": (ctypes.c_int, [_ptr, ctypes.POINTER(BatchArgs), ctypes.POINTER(_ptr)]),
     "bt_graph_launch": (ctypes.c_int, [_ptr, _ptr]),
     "bt_graph_destroy": (ctypes.c_int, [_ptr]),
+    "bt_compute_units": (ctypes.c_int, []),
+    "bt_device_alloc": (ctypes.c_int, [_i64, ctypes.POINTER(ctypes.c_void_p)]),
+    "bt_device_free": (ctypes.c_int, [_ptr]),
+    "bt_copy_to_device": (ctypes.c_int, [_ptr, _ptr, _i64, _ptr]),
+    "bt_copy_to_host": (ctypes.c_int, [_ptr, _ptr, _i64, _ptr]),
+    "bt_copy_on_device": (ctypes.c_int, [_ptr, _ptr, _i64, _ptr]),
+    "bt_fill_i64": (ctypes.c_int, [_ptr, _i64, _i64, _ptr]),
+    "bt_fill_f64": (ctypes.c_int, [_ptr, _dbl, _i64, _ptr]),
+    "bt_synchronize": (ctypes.c_int, [_ptr]),
+    "bt_abs_max_f64": (ctypes.c_int, [_ptr, _i64, ctypes.POINTER(_dbl), _ptr]),
+    "bt_max_i64": (ctypes.c_int, [_ptr, _i64, ctypes.POINTER(_i64), _ptr]),
+    "bt_select_equal_i64": (ctypes.c_int, [_ptr, _i64, _i64, _ptr, ctypes.POINTER(_i64), _ptr]),
+    "bt_gather_columns_f64": (ctypes.c_int, [_ptr, _i64, _i64, _ptr, _i64, _ptr, _ptr]),
+    "bt_scatter_i64": (ctypes.c_int, [_ptr, _i64, _ptr, _ptr, _i64, _ptr]),
+    "bt_gather_records": (ctypes.c_int, [_ptr, _i64, _i64, _i64, _i64, _ptr, _ptr, _i64, _ptr, _ptr]),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -29,8 +29,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from beamtracer_amd import _lib, kernels
+from beamtracer_amd import _lib, device, kernels
 from beamtracer_amd.beam import BeamParticles
+from beamtracer_amd.device import DeviceArray
 from beamtracer_amd.tracer import (
     _ALIVE,
     _PAUSED,
@@ -199,10 +200,8 @@ class BatchedTracer:
         self._fxs = np.concatenate([self.tracers[d].patch_arrays()[0] for d in self._live])
         self._fys = np.concatenate([self.tracers[d].patch_arrays()[1] for d in self._live])
         if "fxs" in self._dev:
-            import torch
-
-            self._dev["fxs"].copy_(torch.from_numpy(self._fxs))
-            self._dev["fys"].copy_(torch.from_numpy(self._fys))
+            self._dev["fxs"].copy_(self._fxs)
+            self._dev["fys"].copy_(self._fys)
 
     def _design_table(self, exponents: Sequence[int]):
         table = (_lib.BatchDesign * len(self._rows))()
@@ -212,19 +211,14 @@ class BatchedTracer:
 
     def _ensure_plan(self, exponents: Sequence[int]) -> None:
         """The live designs' exponents into the plan; the arenas on the device."""
-        import torch
-
         if self._plan is None:
             raise RuntimeError("every design is left out of the batch")
         rc = _lib.lib().bt_batch_plan_update(self._plan, self._design_table(exponents), len(self._rows))
         _lib.check(rc, "bt_batch_plan_update")
         if "fxs" not in self._dev:
-            self._dev["fxs"] = torch.from_numpy(self._fxs).cuda()
-            self._dev["fys"] = torch.from_numpy(self._fys).cuda()
-            self._dev["cells"] = torch.from_numpy(self._cells).cuda()
-            self._dev["subcells"] = torch.from_numpy(self._subcells).cuda()
-            self._dev["planes"] = torch.from_numpy(np.ascontiguousarray(self._planes)).cuda()
-            self._dev["charge"] = torch.zeros(self.total_nodes, dtype=torch.float64, device="cuda")
+            for name in ("fxs", "fys", "cells", "subcells", "planes"):
+                self._dev[name] = DeviceArray.from_numpy(getattr(self, "_" + name))
+            self._dev["charge"] = DeviceArray.zeros(self.total_nodes, np.float64)
         self._table = list(exponents)
 
     def close(self) -> None:
@@ -325,17 +319,18 @@ class BatchedTracer:
         return a
 
     def _upload(self, host: dict, capacity: int) -> dict:
-        import torch
-
+        """The live designs' beams in the library's device arrays (start state kept as x0,
+        v0, status0, electrode0), empty steps, records and counts, and the queue."""
         n = host["n"]
-        dev = {k: torch.from_numpy(host[k]).cuda() for k in ("x", "v", "qm", "dq", "status", "electrode",
-                                                               "design_of")}  # fmt: skip
-        dev["x0"], dev["v0"] = dev["x"].clone(), dev["v"].clone()
-        dev["status0"], dev["electrode0"] = dev["status"].clone(), dev["electrode"].clone()
-        dev["steps"] = torch.zeros(n, dtype=torch.int64, device="cuda")
-        dev["hits"] = torch.zeros((n, max(self._n_planes, 1), capacity, 8), dtype=torch.float64, device="cuda")
-        dev["counts"] = torch.zeros((n, max(self._n_planes, 1)), dtype=torch.int64, device="cuda")
-        dev["queue"] = torch.from_numpy(self._queue(host["starts"])).cuda()
+        dev = {k: DeviceArray.from_numpy(host[k]) for k in ("x", "v", "qm", "dq", "status", "electrode",
+                                                           "design_of")}  # fmt: skip
+        for k in ("x", "v", "status", "electrode"):
+            dev[k + "0"] = dev[k].clone()
+        dev["steps"] = DeviceArray.zeros(n, np.int64)
+        dev["hits"] = DeviceArray.zeros((n, max(self._n_planes, 1), capacity, 8), np.float64)
+        dev["counts"] = DeviceArray.zeros((n, max(self._n_planes, 1)), np.int64)
+        dev["queue"] = DeviceArray.from_numpy(self._queue(host["starts"]))
+        dev["scratch"] = DeviceArray(max(n, 1), np.int64)  # the paused trajectories' indices
         return dev
 
     def _restart(self, dev: dict) -> None:
@@ -346,12 +341,10 @@ class BatchedTracer:
         self._dev["charge"].zero_()
 
     def launch(self, dev: dict, active, reset: bool = False, stream=None) -> None:
-        """One batched kernel call over ``active`` (asynchronous on ``stream``)."""
-        import torch
-
-        st = stream if stream is not None else torch.cuda.current_stream()
+        """One batched kernel call over ``active`` (asynchronous on ``stream``, by default the
+        null stream)."""
         a = self._args(dev, active, reset)
-        rc = _lib.lib().bt_batch_trace_device(self._plan, ctypes.byref(a), ctypes.c_void_p(st.cuda_stream))
+        rc = _lib.lib().bt_batch_trace_device(self._plan, ctypes.byref(a), kernels._stream(stream, dev["x"]))
         _lib.check(rc, "bt_batch_trace_device")
 
     def _resume_paused(self, dev: dict, host: dict) -> None:
@@ -361,15 +354,12 @@ class BatchedTracer:
         Only the paused trajectories cross to the host: their indices and positions go
         down, their new status and electrode go back up by index.
         """
-        import torch
-
         first = np.asarray(host["first"])
         while True:
-            paused_dev = torch.nonzero(dev["status"] == _PAUSED).squeeze(1)
-            if not paused_dev.numel():
+            paused = device.select_equal(dev["status"], _PAUSED, dev["scratch"])
+            if not paused.size:
                 return
-            paused = paused_dev.cpu().numpy()
-            x = dev["x"][:, paused_dev].cpu().numpy()
+            x = device.gather_columns(dev["x"], paused)
             design = np.searchsorted(first, paused, side="right") - 1
             hit = np.full(paused.size, -1, dtype=np.int64)
             for d in np.unique(design):
@@ -377,56 +367,48 @@ class BatchedTracer:
                 t = self.tracers[self._live[d]]
                 hit[sel] = t.mesh.inside_solid(*t._project(x[:, sel]))
             inside = hit >= 0
-            if inside.any():
-                ids = torch.from_numpy(paused[inside]).cuda()
-                dev["status"][ids] = ELECTRODE
-                dev["electrode"][ids] = torch.from_numpy(hit[inside]).cuda()
+            device.scatter(dev["status"], paused, np.where(inside, ELECTRODE, _ALIVE))
+            device.scatter(dev["electrode"], paused[inside], hit[inside])
             active = paused[~inside]
             if not active.size:
                 return
-            active_dev = torch.from_numpy(active.astype(np.int64)).cuda()
-            dev["status"][active_dev] = _ALIVE
-            self.launch(dev, active_dev)
+            self.launch(dev, DeviceArray.from_numpy(active.astype(np.int64)))
 
-    def _plane_records(self, dev: dict) -> list[tuple[np.ndarray, np.ndarray]]:
+    def _plane_records(self, dev: dict, host: dict) -> list[tuple[np.ndarray, np.ndarray]]:
         """Per plane, the recorded crossings of every design as ``(trajectory, record)``:
-        compacted on the device to the filled slots, ordered by design, then by step, then
-        by trajectory (the order of ``collect_plane_hits``)."""
-        import torch
-
-        hits, counts = dev["hits"], dev["counts"]
+        only the filled slots cross to the host (the counts say which), ordered by design,
+        then by step, then by trajectory (the order of ``collect_plane_hits``)."""
+        hits = dev["hits"]
         capacity = hits.shape[2]
-        slots = torch.arange(capacity, device=hits.device)
-        design_of = dev["design_of"].to(torch.int64)
+        counts = dev["counts"].numpy()
+        design_of = host["design_of"].astype(np.int64)
         out = []
         for p in range(len(self.tracers[0].planes)):
-            k, c = torch.nonzero(slots[None, :] < counts[:, p, None], as_tuple=True)
-            record = hits[k, p, c]
+            k, c = np.nonzero(np.arange(capacity)[None, :] < np.minimum(counts[:, p], capacity)[:, None])
+            record = device.gather_records(hits, p, k, c)
             # k ascends in nonzero's output; a stable sort on (design, step) keeps it
             # ascending among equal keys
-            key = design_of[k] * (self.tracers[0].max_steps + 2) + record[:, 7].to(torch.int64)
-            order = torch.sort(key, stable=True).indices
-            out.append((k[order].cpu().numpy(), record[order].cpu().numpy()))
+            key = design_of[k] * (self.tracers[0].max_steps + 2) + record[:, 7].astype(np.int64)
+            order = np.argsort(key, kind="stable")
+            out.append((k[order], record[order]))
         return out
 
     def _collect(self, dev: dict, host: dict) -> list[TraceResult]:
         """Results per design; only the per-trajectory outcomes, the filled plane slots and
         the node charges are copied to the host (the end states stay in
         ``last_device_state``, see :meth:`end_state`)."""
-        import torch
-
-        dead = dev["status0"] == ELECTRODE
-        status = torch.where(dead, dev["status0"], dev["status"]).to(torch.int8).cpu().numpy()
-        electrode = torch.where(dead, dev["electrode0"], dev["electrode"]).to(torch.int32).cpu().numpy()
-        design_of = dev["design_of"].to(torch.int64)
+        dead = host["status"] == ELECTRODE
+        status = np.where(dead, host["status"], dev["status"].numpy()).astype(np.int8)
+        electrode = np.where(dead, host["electrode"], dev["electrode"].numpy()).astype(np.int32)
+        design_of = host["design_of"].astype(np.int64)
         n_designs = len(self._live)
-        steps = dev["steps"]
-        longest = torch.zeros(n_designs, dtype=torch.int64, device=steps.device)
-        longest.scatter_reduce_(0, design_of, steps, "amax")
-        total = torch.zeros(n_designs, dtype=torch.int64, device=steps.device).index_add_(0, design_of, steps)
-        longest, total = longest.cpu().numpy(), total.cpu().numpy()
-        charge = self._dev["charge"].cpu().numpy()
-        records = self._plane_records(dev)
+        steps = dev["steps"].numpy()
+        longest = np.zeros(n_designs, dtype=np.int64)
+        np.maximum.at(longest, design_of, steps)
+        total = np.zeros(n_designs, dtype=np.int64)
+        np.add.at(total, design_of, steps)
+        charge = self._dev["charge"].numpy()
+        records = self._plane_records(dev, host)
         first = np.asarray(host["first"])
         bounds = [np.searchsorted(k, first) for k, _ in records]
         out = self._failed_results()
@@ -487,7 +469,7 @@ class BatchedTracer:
         dev = self.last_device_state
         k = self._live.index(design)
         lo, hi = self._first[k], self._first[k + 1]
-        return dev["x"][:, lo:hi].cpu().numpy(), dev["v"][:, lo:hi].cpu().numpy(), dev["steps"][lo:hi].cpu().numpy()
+        return dev["x"].numpy()[:, lo:hi], dev["v"].numpy()[:, lo:hi], dev["steps"][lo:hi].numpy()
 
     def trace(self, particles: Sequence[BeamParticles]) -> list[TraceResult]:
         """Trace every design's beam; one :class:`TraceResult` per design (with the reason
@@ -502,7 +484,7 @@ class BatchedTracer:
             self._restart(dev)
             self.launch(dev, dev["queue"])
             self._resume_paused(dev, host)
-            most = int(dev["counts"].max().item()) if host["n"] else 0
+            most = device.max_int(dev["counts"])
             if most <= capacity:
                 break
             capacity = most
@@ -525,17 +507,13 @@ class BatchedTracer:
             self._graph = None
         a = self._args(dev, dev["queue"], reset=True)
         graph = ctypes.c_void_p()
-        import torch
-
-        torch.cuda.synchronize()
+        device.synchronize()
         _lib.check(L.bt_batch_graph_create(self._plan, ctypes.byref(a), ctypes.byref(graph)), "bt_batch_graph_create")
         self._graph = graph
         self._graph_state = (host, dev, a)
 
     def replay(self, refresh_exponents: bool = True) -> list[TraceResult]:
         """Replay the captured trace on the current fields (after :meth:`update_fields`)."""
-        import torch
-
         if self._graph is None:
             raise RuntimeError("nothing captured")
         host, dev, _ = self._graph_state
@@ -547,12 +525,11 @@ class BatchedTracer:
                 fresh.append(kernels.trace_exponent(host["starts"][d] - lo, host["v"][:, lo:hi], host["dq"][lo:hi],
                                                     host["qm"][lo:hi], fx, fy, patch_real, t.step_fraction))  # fmt: skip
             if fresh != self._table:
-                torch.cuda.synchronize()
+                device.synchronize()
                 self._ensure_plan(fresh)
-        st = torch.cuda.current_stream()
-        _lib.check(_lib.lib().bt_graph_launch(self._graph, ctypes.c_void_p(st.cuda_stream)), "bt_graph_launch")
+        _lib.check(_lib.lib().bt_graph_launch(self._graph, None), "bt_graph_launch")
         self._resume_paused(dev, host)
-        if int(dev["counts"].max().item()) > dev["hits"].shape[2]:
+        if device.max_int(dev["counts"]) > dev["hits"].shape[2]:
             raise RuntimeError("plane buffers overflowed in a captured trace: trace() first to size them")
         self.last_device_state = dev
         return self._collect(dev, host)
@@ -607,18 +584,17 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
 
     The candidates are threads per block, the fraction of the resident blocks the grid
     fills, the queue order, the refill mode (lane by lane, or a workgroup together) and the
-    deposit tile (windows of 16 or 32 columns); each is timed (HIP events on the launch
-    stream, best of ``repeats``) on the real batch. Every trajectory takes one lane: a
+    deposit tile (windows of 16 or 32 columns); each is timed (the launch between two
+    synchronizations of the null stream, best of ``repeats``) on the real batch. Every trajectory takes one lane: a
     trajectory's steps are a chain of dependent field gathers, which more lanes would
     duplicate rather than share. Returns every timing in ms and the choice.
     """
-    import torch
-
     host = tracer._prepare(particles)
     tracer._ensure_plan(host["exponents"])
     t0 = tracer.tracers[0]
     L = _lib.lib()
-    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cus = L.bt_compute_units()
+    _lib.check(min(cus, 0), "bt_compute_units")
     timings = {}
     best = None
     for order in orders:
@@ -634,14 +610,11 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
                     times = []
                     for _ in range(repeats):
                         tracer._restart(dev)
-                        start = torch.cuda.Event(enable_timing=True)
-                        end = torch.cuda.Event(enable_timing=True)
-                        stream = torch.cuda.current_stream()
-                        start.record(stream)
+                        device.synchronize()
+                        start = time.perf_counter()
                         tracer.launch(dev, dev["queue"])
-                        end.record(stream)
-                        torch.cuda.synchronize()
-                        times.append(start.elapsed_time(end))
+                        device.synchronize()
+                        times.append((time.perf_counter() - start) * 1e3)
                     key = (f"{order}/block{block}/fill{fill}" + ("/refill-block" if refill else "")
                            + ("/window32" if wide else ""))  # fmt: skip
                     timings[key] = min(times)

@@ -53,6 +53,7 @@ struct DeviceState {
     std::vector<Buffer> buffers;   // host-form staging, by slot
     Buffer workspace;              // library-owned workspace
     Buffer adjoint;                // library-owned adjoint scratch
+    Buffer counters;               // the device-array helpers' counters
     hipStream_t stream = nullptr;  // host-form stream
     int cus = 0;
 };
@@ -1146,6 +1147,216 @@ int bt_graph_destroy(void* handle) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     delete g;
     return BT_OK;
+}
+
+}  // extern "C"
+
+// ---- device arrays owned by the library -------------------------------------------------
+// The host code's device-resident state lives in memory allocated here; the helpers below
+// are the few operations it needs between traces.  Copies to and from the host wait for
+// `stream`; the helpers that report a count or a value wait too.
+
+namespace {
+
+// the per-device counter words of the helpers (zeroed before each use), g_mutex held
+int helper_counters(u64** out) {
+    DeviceState* s = nullptr;
+    int rc = device_state(&s);
+    if (rc) return rc;
+    rc = ensure(s->counters, 256);
+    if (rc) return rc;
+    *out = static_cast<u64*>(s->counters.ptr);
+    return BT_OK;
+}
+
+int read_counter(const u64* dev, u64* host, hipStream_t stream, const char* what) {
+    hipError_t err = hipMemcpyAsync(host, dev, sizeof(u64), hipMemcpyDeviceToHost, stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    return err == hipSuccess ? BT_OK : hip_fail(err, what);
+}
+
+int bad_indices(u64 bad) {
+    return bad ? fail(BT_E_ARG, std::to_string(bad) + " index(es) out of range") : BT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bt_compute_units(void) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* s = nullptr;
+    int rc = device_state(&s);
+    return rc ? rc : s->cus;
+}
+
+int bt_device_alloc(int64_t bytes, void** out) {
+    if (out == nullptr) return fail(BT_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (bytes < 0) return fail(BT_E_ARG, "negative size");
+    if (bytes == 0) return BT_OK;
+    hipError_t err = hipMalloc(out, (size_t)bytes);
+    if (err != hipSuccess) {
+        *out = nullptr;
+        return fail(BT_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(err));
+    }
+    return BT_OK;
+}
+
+int bt_device_free(void* ptr) {
+    if (ptr == nullptr) return BT_OK;
+    hipError_t err = hipFree(ptr);
+    return err == hipSuccess ? BT_OK : hip_fail(err, "hipFree");
+}
+
+int bt_copy_to_device(void* dst, const void* src, int64_t bytes, void* stream) {
+    if (bytes < 0) return fail(BT_E_ARG, "negative size");
+    if (bytes == 0) return BT_OK;
+    if (dst == nullptr || src == nullptr) return fail(BT_E_ARG, "NULL pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    return err == hipSuccess ? BT_OK : hip_fail(err, "copy to device");
+}
+
+int bt_copy_to_host(void* dst, const void* src, int64_t bytes, void* stream) {
+    if (bytes < 0) return fail(BT_E_ARG, "negative size");
+    if (bytes == 0) return BT_OK;
+    if (dst == nullptr || src == nullptr) return fail(BT_E_ARG, "NULL pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    return err == hipSuccess ? BT_OK : hip_fail(err, "copy to host");
+}
+
+int bt_copy_on_device(void* dst, const void* src, int64_t bytes, void* stream) {
+    if (bytes < 0) return fail(BT_E_ARG, "negative size");
+    if (bytes == 0) return BT_OK;
+    if (dst == nullptr || src == nullptr) return fail(BT_E_ARG, "NULL pointer");
+    hipError_t err = hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream));
+    return err == hipSuccess ? BT_OK : hip_fail(err, "copy on device");
+}
+
+int bt_fill_i64(int64_t* dst, int64_t value, int64_t n, void* stream) {
+    if (n < 0 || (n > 0 && dst == nullptr)) return fail(BT_E_ARG, "bad array");
+    return bt::launch_fill_i64(dst, value, n, static_cast<hipStream_t>(stream)) ? hip_fail(hipGetLastError(), "fill")
+                                                                                : BT_OK;
+}
+
+int bt_fill_f64(double* dst, double value, int64_t n, void* stream) {
+    if (n < 0 || (n > 0 && dst == nullptr)) return fail(BT_E_ARG, "bad array");
+    return bt::launch_fill_f64(dst, value, n, static_cast<hipStream_t>(stream)) ? hip_fail(hipGetLastError(), "fill")
+                                                                                : BT_OK;
+}
+
+int bt_synchronize(void* stream) {
+    hipError_t err = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    return err == hipSuccess ? BT_OK : hip_fail(err, "hipStreamSynchronize");
+}
+
+int bt_abs_max_f64(const double* a, int64_t n, double* out, void* stream) {
+    if (out == nullptr || n < 0 || (n > 0 && a == nullptr)) return fail(BT_E_ARG, "bad array");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    u64* counters = nullptr;
+    int rc = helper_counters(&counters);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(counters, 0, sizeof(u64), st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if (bt::launch_abs_max(a, n, counters, st)) return hip_fail(hipGetLastError(), "abs max");
+    u64 bits = 0;
+    rc = read_counter(counters, &bits, st, "abs max");
+    if (rc) return rc;
+    std::memcpy(out, &bits, sizeof(double));
+    return BT_OK;
+}
+
+int bt_max_i64(const int64_t* a, int64_t n, int64_t* out, void* stream) {
+    if (out == nullptr || n < 1 || a == nullptr) return fail(BT_E_ARG, "bad array");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    u64* counters = nullptr;
+    int rc = helper_counters(&counters);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(counters, 0, sizeof(u64), st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if (bt::launch_max_i64(a, n, counters, st)) return hip_fail(hipGetLastError(), "max");
+    u64 key = 0;
+    rc = read_counter(counters, &key, st, "max");
+    *out = (int64_t)(key ^ 0x8000000000000000ull);
+    return rc;
+}
+
+int bt_select_equal_i64(const int64_t* a, int64_t n, int64_t value, int64_t* idx, int64_t* count, void* stream) {
+    if (count == nullptr || n < 0 || (n > 0 && (a == nullptr || idx == nullptr))) return fail(BT_E_ARG, "bad array");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    u64* counters = nullptr;
+    int rc = helper_counters(&counters);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(counters, 0, sizeof(u64), st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if (bt::launch_select_equal(a, n, value, idx, counters, st)) return hip_fail(hipGetLastError(), "select");
+    u64 found = 0;
+    rc = read_counter(counters, &found, st, "select");
+    *count = (int64_t)found;
+    return rc;
+}
+
+int bt_gather_columns_f64(const double* src, int64_t rows, int64_t n, const int64_t* idx, int64_t m, double* dst,
+                          void* stream) {
+    if (rows < 0 || n < 0 || m < 0) return fail(BT_E_ARG, "negative size");
+    if (rows == 0 || m == 0) return BT_OK;
+    if (src == nullptr || idx == nullptr || dst == nullptr) return fail(BT_E_ARG, "NULL pointer");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    u64* counters = nullptr;
+    int rc = helper_counters(&counters);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(counters, 0, sizeof(u64), st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if (bt::launch_gather_columns(src, rows, n, idx, m, dst, counters, st)) return hip_fail(hipGetLastError(), "gather");
+    u64 bad = 0;
+    rc = read_counter(counters, &bad, st, "gather");
+    return rc ? rc : bad_indices(bad);
+}
+
+int bt_scatter_i64(int64_t* dst, int64_t n, const int64_t* idx, const int64_t* values, int64_t m, void* stream) {
+    if (n < 0 || m < 0) return fail(BT_E_ARG, "negative size");
+    if (m == 0) return BT_OK;
+    if (dst == nullptr || idx == nullptr || values == nullptr) return fail(BT_E_ARG, "NULL pointer");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    u64* counters = nullptr;
+    int rc = helper_counters(&counters);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(counters, 0, sizeof(u64), st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if (bt::launch_scatter(dst, n, idx, values, m, counters, st)) return hip_fail(hipGetLastError(), "scatter");
+    u64 bad = 0;
+    rc = read_counter(counters, &bad, st, "scatter");
+    return rc ? rc : bad_indices(bad);
+}
+
+int bt_gather_records(const double* hits, int64_t n, int64_t n_planes, int64_t capacity, int64_t plane,
+                      const int64_t* trajectory, const int64_t* slot, int64_t m, double* dst, void* stream) {
+    if (n < 0 || n_planes < 1 || capacity < 1 || m < 0) return fail(BT_E_ARG, "bad record shape");
+    if (plane < 0 || plane >= n_planes) return fail(BT_E_ARG, "plane out of range");
+    if (m == 0) return BT_OK;
+    if (hits == nullptr || trajectory == nullptr || slot == nullptr || dst == nullptr)
+        return fail(BT_E_ARG, "NULL pointer");
+    std::lock_guard<std::mutex> lock(g_mutex);
+    u64* counters = nullptr;
+    int rc = helper_counters(&counters);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t err = hipMemsetAsync(counters, 0, sizeof(u64), st);
+    if (err != hipSuccess) return hip_fail(err, "hipMemsetAsync");
+    if (bt::launch_gather_records(hits, n, n_planes, capacity, plane, trajectory, slot, m, dst, counters, st))
+        return hip_fail(hipGetLastError(), "gather records");
+    u64 bad = 0;
+    rc = read_counter(counters, &bad, st, "gather records");
+    return rc ? rc : bad_indices(bad);
 }
 
 }  // extern "C"

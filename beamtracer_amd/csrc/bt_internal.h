@@ -83,6 +83,18 @@ int launch_batched(const TraceParams& p, const double* x0, const double* v0, boo
                    i64 max_design_nodes);
 int batched_occupancy(int integ, int geom, bool has_b, bool wide, int block);
 
+// The device-array helpers (bt_memory.hip); `bad` and `count` are device counters.
+int launch_fill_i64(i64* dst, i64 value, i64 n, hipStream_t stream);
+int launch_fill_f64(double* dst, double value, i64 n, hipStream_t stream);
+int launch_abs_max(const double* a, i64 n, u64* out, hipStream_t stream);
+int launch_max_i64(const i64* a, i64 n, u64* out, hipStream_t stream);
+int launch_select_equal(const i64* a, i64 n, i64 value, i64* idx, u64* count, hipStream_t stream);
+int launch_gather_columns(const double* src, i64 rows, i64 n, const i64* idx, i64 m, double* dst, u64* bad,
+                          hipStream_t stream);
+int launch_scatter(i64* dst, i64 n, const i64* idx, const i64* values, i64 m, u64* bad, hipStream_t stream);
+int launch_gather_records(const double* hits, i64 n, i64 n_planes, i64 capacity, i64 plane, const i64* trajectory,
+                          const i64* slot, i64 m, double* dst, u64* bad, hipStream_t stream);
+
 // The adjoint of a trace (bt_adjoint.hip): seeds dJ/d(record) of the plane crossings ->
 // dJ/dfx, dJ/dfy (and the initial states' adjoints), one uniform mesh, classical, no B.
 struct AdjointParams {

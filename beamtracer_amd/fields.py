@@ -123,37 +123,35 @@ class DeviceEdgeField:
     """The edge fields of potentials on one mesh, computed on the GPU.
 
     The mesh's node classes (``solid``, ``near_solid``, ``distance``, the x spacings of a
-    graded axis) go to HBM once; each call turns a potential (a NumPy array or a GPU tensor
-    ``(nx, ny)``) into ``fx`` ``(nx + 1, ny)`` and ``fy`` ``(nx, ny + 1)`` float64 GPU
-    tensors with the hip edge-field kernel, bit-identical to
-    :class:`StaggeredElectricField`'s (the tensors are reused by the next call).
+    graded axis) go to HBM once, in the library's device arrays; each call turns a potential
+    (a host array or a :class:`~beamtracer_amd.device.DeviceArray` ``(nx, ny)``) into ``fx``
+    ``(nx + 1, ny)`` and ``fy`` ``(nx, ny + 1)`` float64 device arrays with the hip
+    edge-field kernel, bit-identical to :class:`StaggeredElectricField`'s (the arrays are
+    reused by the next call).
     """
 
     def __init__(self, mesh: IonOpticsMesh, extrapolation: Sequence[str] | None = None):
-        import torch
+        from beamtracer_amd.device import DeviceArray
 
         self.mesh = mesh
         self.mirrors = face_mirrors(mesh, extrapolation)
         self.faces = sum(1 << k for k, m in enumerate(self.mirrors) if m)
         nx, ny = mesh.shape
-        self.solid = torch.from_numpy(np.ascontiguousarray(mesh.solid, dtype=np.int32)).cuda()
-        self.near = torch.from_numpy(np.ascontiguousarray(mesh.near_solid, dtype=np.uint8)).cuda()
-        self.distance = torch.from_numpy(np.ascontiguousarray(mesh.distance, dtype=float)).cuda()
-        self.dx = torch.from_numpy(np.ascontiguousarray(mesh.dx, dtype=float)).cuda() if mesh.graded else None
-        self.fx = torch.empty((nx + 1, ny), dtype=torch.float64, device="cuda")
-        self.fy = torch.empty((nx, ny + 1), dtype=torch.float64, device="cuda")
+        self.solid = DeviceArray.from_numpy(np.ascontiguousarray(mesh.solid, dtype=np.int32))
+        self.near = DeviceArray.from_numpy(np.ascontiguousarray(mesh.near_solid, dtype=np.uint8))
+        self.distance = DeviceArray.from_numpy(np.ascontiguousarray(mesh.distance, dtype=float))
+        self.dx = DeviceArray.from_numpy(np.ascontiguousarray(mesh.dx, dtype=float)) if mesh.graded else None
+        self.fx = DeviceArray((nx + 1, ny), np.float64)
+        self.fy = DeviceArray((nx, ny + 1), np.float64)
+        self._phi = DeviceArray((nx, ny), np.float64)
 
     def __call__(self, potential, stream=None):
-        import torch
-
+        from beamtracer_amd.device import DeviceArray
         from beamtracer_amd.kernels import edge_field_2d
 
-        if isinstance(potential, torch.Tensor):
-            phi = potential.to(device="cuda", dtype=torch.float64).contiguous()
-        else:
-            phi = torch.from_numpy(np.ascontiguousarray(potential, dtype=float)).cuda()
-        if tuple(phi.shape) != self.mesh.shape:
+        if tuple(np.shape(potential) if not isinstance(potential, DeviceArray) else potential.shape) != self.mesh.shape:
             raise ValueError(f"the potential must be {self.mesh.shape}")
+        phi = potential if isinstance(potential, DeviceArray) else self._phi.copy_(np.asarray(potential, dtype=float))
         edge_field_2d(phi, self.solid, self.near, self.distance, self.dx, self.mesh.h, self.faces, self.fx, self.fy,
                       stream=stream)  # fmt: skip
         return self.fx, self.fy

@@ -9,10 +9,12 @@ place the same way:
 * :func:`trace_trajectories_2d` -- ``trace_trajectories_2d_pyccel.py:42-78``
 
 NumPy arrays run the synchronous host form of the C ABI (copies to and from the GPU);
-``torch`` tensors on the GPU run the device form on the current stream with no copies
-(the small ``patch_int``, ``patch_real`` and ``planes`` tables of the trace are read on
-the host and may be NumPy arrays or CPU tensors). There is no CPU fallback: without the
-built library every call raises :class:`beamtracer_amd._lib.HipLibraryError`.
+device arrays run the device form with no copies: the library's own
+:class:`~beamtracer_amd.device.DeviceArray` (on the null stream), or GPU ``torch`` tensors
+(on torch's current stream), as the autograd wrapper holds them. The small ``patch_int``,
+``patch_real`` and ``planes`` tables of the trace are read on the host. There is no CPU
+fallback: without the built library every call raises
+:class:`beamtracer_amd._lib.HipLibraryError`.
 
 ``KERNELS`` is the catalog by name, like ``kernels/catalog.py:39-44``.
 """
@@ -26,6 +28,7 @@ from typing import Any
 import numpy as np
 
 from beamtracer_amd import _lib
+from beamtracer_amd.device import DeviceArray, abs_max
 
 KERNEL_NAMES = ("staggered_field_2d", "deposit_trajectories_2d", "trace_trajectories_2d")
 
@@ -35,6 +38,11 @@ _F64 = np.float64
 
 def _is_torch(a: Any) -> bool:
     return type(a).__module__.startswith("torch")
+
+
+def _is_device(a: Any) -> bool:
+    """A device array: the library's DeviceArray or a GPU torch tensor."""
+    return isinstance(a, DeviceArray) or _is_torch(a)
 
 
 def _host(a: Any, dtype, name: str) -> np.ndarray:
@@ -49,13 +57,17 @@ def _host(a: Any, dtype, name: str) -> np.ndarray:
 
 
 def _dev(a: Any, dtype, name: str):
-    """A contiguous CUDA (HIP) torch tensor of ``dtype``."""
+    """A device array of ``dtype``: a DeviceArray, or a contiguous GPU torch tensor."""
+    if isinstance(a, DeviceArray):
+        if a.dtype != np.dtype(dtype):
+            raise TypeError(f"{name} must be {np.dtype(dtype)}, got {a.dtype}")
+        return a
     import torch
 
     want = {np.dtype(np.int64): torch.int64, np.dtype(np.float64): torch.float64, np.dtype(np.int32): torch.int32,
             np.dtype(np.uint8): torch.uint8}[np.dtype(dtype)]
     if not (_is_torch(a) and a.is_cuda):
-        raise TypeError(f"{name} must be a GPU torch tensor like the other arrays")
+        raise TypeError(f"{name} must be a device array like the other arrays")
     if a.dtype != want:
         raise TypeError(f"{name} must be {want}, got {a.dtype}")
     if not a.is_contiguous():
@@ -66,21 +78,29 @@ def _dev(a: Any, dtype, name: str):
 def _ptr(a: Any) -> int | None:
     if a is None:
         return None
+    if isinstance(a, DeviceArray):
+        return a.data_ptr() or None
     if _is_torch(a):
         return a.data_ptr() if a.numel() else None
     return a.ctypes.data if a.size else None
 
 
 def _table(a: Any, dtype, name: str) -> np.ndarray:
-    """A small table read on the host: a NumPy array or a (CPU or GPU) tensor."""
-    if _is_torch(a):
+    """A small table read on the host: a NumPy array, a DeviceArray or a tensor."""
+    if isinstance(a, DeviceArray):
+        a = a.numpy()
+    elif _is_torch(a):
         a = a.detach().cpu().numpy()
     return _host(np.ascontiguousarray(a, dtype=dtype), dtype, name)
 
 
-def _stream(stream: Any) -> int | None:
+def _stream(stream: Any, like: Any = None) -> int | None:
+    """The stream handle: ``stream``, else the null stream for the library's arrays
+    (``like``), else torch's current stream."""
     if stream is not None:
         return int(getattr(stream, "cuda_stream", stream)) or None
+    if isinstance(like, DeviceArray):
+        return None
     import torch
 
     return int(torch.cuda.current_stream().cuda_stream) or None
@@ -98,14 +118,14 @@ def staggered_field_2d(px, py, fx, fy, origin_x, origin_y, h, ex, ey, *, stream=
     the y edges; each is interpolated bilinearly with its cell clamped to the array.
     """
     L = _lib.lib()
-    if _is_torch(px):
+    if _is_device(px):
         arrays = [_dev(a, _F64, n) for a, n in ((px, "px"), (py, "py"), (fx, "fx"), (fy, "fy"), (ex, "ex"), (ey, "ey"))]
         px, py, fx, fy, ex, ey = arrays
         nx, ny = fy.shape[0], fx.shape[1]
         _shape_gather(px.shape, py.shape, tuple(fx.shape), tuple(fy.shape), ex.shape, ey.shape)
         rc = L.bt_staggered_field_2d_device(
             _ptr(px), _ptr(py), px.shape[0], _ptr(fx), _ptr(fy), nx, ny,
-            float(origin_x), float(origin_y), float(h), _ptr(ex), _ptr(ey), _stream(stream),
+            float(origin_x), float(origin_y), float(h), _ptr(ex), _ptr(ey), _stream(stream, px),
         )  # fmt: skip
     else:
         for a, n in ((px, "px"), (py, "py"), (fx, "fx"), (fy, "fy"), (ex, "ex"), (ey, "ey")):
@@ -127,11 +147,16 @@ def trace3d_exponents(active, v, dq, qm, fxs, fys, fzs, patch_real, step_fractio
     ``(f + f^2) h`` times ``|dq|`` for the step fraction ``f`` and the coarsest spacing
     ``h``, so its grid is set 40 bits below that.
     """
-    fx_all = np.concatenate([np.ravel(fxs), np.ravel(fzs)]) if not _is_torch(fxs) else None
-    if _is_torch(fxs):
+    if isinstance(fxs, DeviceArray):
+        active, v, dq, qm = (a.numpy() for a in (active, v, dq, qm))
+        fx_all = np.array([abs_max(fxs), abs_max(fzs)])
+        fys = np.array([abs_max(fys)])
+    elif _is_torch(fxs):
         import torch
 
         fx_all = torch.cat([fxs.reshape(-1), fzs.reshape(-1)])
+    else:
+        fx_all = np.concatenate([np.ravel(fxs), np.ravel(fzs)])
     pr = np.asarray(patch_real)
     # the 2D rule over (fx and fz together, fy): the strongest of the three components
     e_q = trace_exponent(active, v, dq, qm, fx_all, fys, pr[:, [0, 1, 3]], step_fraction, inv_c2)
@@ -160,7 +185,7 @@ def trace_trajectories_3d(
     on ``stream`` (``patch_int``, ``patch_real`` and ``planes`` are read on the host).
     """
     L = _lib.lib()
-    device = _is_torch(x)
+    device = _is_device(x)
     check = _dev if device else _host
     pi = _table(patch_int, _I64, "patch_int")
     pr = _table(patch_real, _F64, "patch_real")
@@ -208,7 +233,7 @@ def trace_trajectories_3d(
         _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
     )  # fmt: skip
     if device:
-        rc = L.bt_trace_trajectories_3d_device(ctypes.byref(args), _stream(stream))
+        rc = L.bt_trace_trajectories_3d_device(ctypes.byref(args), _stream(stream, x))
     else:
         rc = L.bt_trace_trajectories_3d(ctypes.byref(args))
     _lib.check(rc, "trace_trajectories_3d")
@@ -227,7 +252,7 @@ def edge_field_2d(phi, solid, near, distance, dx, h, faces, fx, fy, *, stream=No
     form, GPU tensors the device form on ``stream``.
     """
     L = _lib.lib()
-    device = _is_torch(phi)
+    device = _is_device(phi)
     check = _dev if device else _host
     for a, dt, n in ((phi, _F64, "phi"), (solid, np.int32, "solid"), (near, np.uint8, "near"),
                      (distance, _F64, "distance"), (fx, _F64, "fx"), (fy, _F64, "fy")):  # fmt: skip
@@ -245,7 +270,7 @@ def edge_field_2d(phi, solid, near, distance, dx, h, faces, fx, fy, *, stream=No
         raise ValueError(f"dx must hold the {nx - 1} spacings along x")
     args = (_ptr(phi), _ptr(solid), _ptr(near), _ptr(distance), _ptr(dx), float(h), nx, ny, int(faces),
             _ptr(fx), _ptr(fy))  # fmt: skip
-    rc = L.bt_edge_field_2d_device(*args, _stream(stream)) if device else L.bt_edge_field_2d(*args)
+    rc = L.bt_edge_field_2d_device(*args, _stream(stream, phi)) if device else L.bt_edge_field_2d(*args)
     _lib.check(rc, "edge_field_2d")
 
 
@@ -270,16 +295,17 @@ def deposit_trajectories_2d(
     grid), so it does not depend on the order of the charges.
     """
     L = _lib.lib()
-    if _is_torch(px):
+    if _is_device(px):
         px, py, q, charge = (_dev(a, _F64, n) for a, n in ((px, "px"), (py, "py"), (q, "q"), (charge, "charge")))
-        if px.dim() != 1 or py.shape != px.shape or q.shape != px.shape or charge.dim() != 2:
+        if px.dim() != 1 or tuple(py.shape) != tuple(px.shape) or tuple(q.shape) != tuple(px.shape) or charge.dim() != 2:
             raise ValueError("px, py, q must be 1D of one length and charge 2D")
         if exponent is None:
-            exponent = deposit_exponent(float(q.abs().max().item()) if q.numel() else 1.0) - 10
+            qmax = (abs_max(q) if isinstance(q, DeviceArray) else float(q.abs().max().item())) if q.numel() else 1.0
+            exponent = deposit_exponent(qmax) - 10
         nx, ny = charge.shape
         rc = L.bt_deposit_trajectories_2d_device(
             _ptr(px), _ptr(py), _ptr(q), px.shape[0], float(origin_x), float(origin_y), float(h),
-            int(cylindrical), _ptr(charge), nx, ny, int(exponent), None, 0, _stream(stream),
+            int(cylindrical), _ptr(charge), nx, ny, int(exponent), None, 0, _stream(stream, px),
         )  # fmt: skip
     else:
         for a, n in ((px, "px"), (py, "py"), (q, "q"), (charge, "charge")):
@@ -304,6 +330,10 @@ def trace_exponent(active, v, dq, qm, fxs, fys, patch_real, step_fraction, inv_c
     ``trace_trajectories_2d_pyccel.py:282-293``; it sets only the rounding grid of the
     deposit, never a result of the trace.
     """
+    if isinstance(v, DeviceArray):
+        emax = max(abs_max(fxs), abs_max(fys))
+        return trace_exponent(active.numpy(), v.numpy(), dq.numpy(), qm.numpy(), np.array([emax]), np.zeros(0),
+                              patch_real, step_fraction, inv_c2)  # fmt: skip
     if _is_torch(v):
         import torch
 
@@ -377,7 +407,7 @@ def trace_trajectories_2d(
     changes a result.
     """
     L = _lib.lib()
-    device = _is_torch(x)
+    device = _is_device(x)
     pi = _table(patch_int, _I64, "patch_int")
     pr = _table(patch_real, _F64, "patch_real")
     pl = _table(planes, _F64, "planes")
@@ -445,7 +475,7 @@ def trace_trajectories_2d(
     )  # fmt: skip
     if checkpoints is not None:
         if not device:
-            raise ValueError("checkpoints need the device form (torch tensors)")
+            raise ValueError("checkpoints need the device form (device arrays)")
         check(checkpoints, _F64, "checkpoints")
         if checkpoints.ndim != 3 or tuple(checkpoints.shape[1:]) != (6, n):
             raise ValueError(f"checkpoints must be (C, 6, {n})")
@@ -456,10 +486,10 @@ def trace_trajectories_2d(
         args.checkpoint_interval = int(checkpoint_interval)
     if noalloc:
         if not device or workspace is not None:
-            raise ValueError("noalloc takes the device form (torch tensors) without a workspace")
-        rc = L.bt_trace_trajectories_2d_noalloc(ctypes.byref(args), _stream(stream))
+            raise ValueError("noalloc takes the device form (device arrays) without a workspace")
+        rc = L.bt_trace_trajectories_2d_noalloc(ctypes.byref(args), _stream(stream, x))
     elif device:
-        rc = L.bt_trace_trajectories_2d_device(ctypes.byref(args), _stream(stream))
+        rc = L.bt_trace_trajectories_2d_device(ctypes.byref(args), _stream(stream, x))
     else:
         rc = L.bt_trace_trajectories_2d(ctypes.byref(args))
     _lib.check(rc, "trace_trajectories_2d")

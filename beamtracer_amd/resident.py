@@ -2,7 +2,8 @@
 
 A Vlasov iteration (``solver/vlasov.py:247-429``) traces the same beam through each new
 field. :class:`ResidentTrace` uploads that beam once -- its start state, the electrode
-lookup, the plane and path buffers -- and traces it through edge-field tensors already on
+lookup, the plane and path buffers, in the library's device arrays
+(:class:`~beamtracer_amd.device.DeviceArray`) -- and traces it through edge fields already on
 the GPU (:class:`~beamtracer_amd.fields.DeviceEdgeField`), so a trace moves only the node
 charges to the host (for the host Poisson solve) and a few counters. It follows the
 trace protocol of :class:`~beamtracer_amd.tracer.HipBeamTracer` (``BeamTracer._trace_compiled``,
@@ -17,8 +18,9 @@ from __future__ import annotations
 
 import numpy as np
 
-from beamtracer_amd import kernels, tuning
+from beamtracer_amd import device, kernels, tuning
 from beamtracer_amd.beam import BeamParticles
+from beamtracer_amd.device import DeviceArray
 from beamtracer_amd.tracer import (
     SUBCELLS,
     HipBeamTracer,
@@ -36,8 +38,6 @@ class ResidentTrace:
     magnetic field), resident on the GPU for repeated traces."""
 
     def __init__(self, tracer: HipBeamTracer, particles: BeamParticles):
-        import torch
-
         tracer.require_uniform("the resident beam")
         self.tracer = tracer
         self.particles = particles
@@ -57,13 +57,11 @@ class ResidentTrace:
             speed = np.sqrt(u2) / np.sqrt(1.0 + u2 * self._inv_c2) if self._inv_c2 else np.sqrt(u2)
             self._maxima = tuple(float(a.max()) for a in (speed, np.abs(beam.dq[beam.start]), np.abs(beam.qm[beam.start])))
 
-        def gpu(a):
-            return torch.from_numpy(np.ascontiguousarray(a)).cuda()
-
+        gpu = DeviceArray.from_numpy
         self._initial = {"x": gpu(beam.x), "v": gpu(beam.v), "status": gpu(beam.status),
                          "electrode": gpu(beam.electrode)}  # fmt: skip
         self._state = {k: t.clone() for k, t in self._initial.items()}
-        self._steps = torch.zeros(n, dtype=torch.int64, device="cuda")
+        self._steps = DeviceArray.zeros(n, np.int64)
         start = beam.start
         if tracer.queue_order == "coherent":  # the queue's order, computed once for the resident beam
             start = tuning.coherent_order(start, beam.x, beam.v, beam.qm, geometry_code(tracer.mesh))
@@ -75,7 +73,7 @@ class ResidentTrace:
         self._path_slot = gpu(beam.path_slot)
         magnetic = tracer.magnetic_nodes()
         self._magnetic = None if magnetic is None else tuple(None if c is None else gpu(c) for c in magnetic)
-        self._charge = torch.zeros((1, int(self.patch_int[:, 4].sum())), dtype=torch.float64, device="cuda")
+        self._charge = DeviceArray.zeros((1, int(self.patch_int[:, 4].sum())), np.float64)
         self._hits = self._counts = self._paths = self._path_count = None
         self._shape = None
         self.steps = 0
@@ -83,33 +81,33 @@ class ResidentTrace:
 
     def _buffers(self, capacity: int, path_capacity: int) -> None:
         """Plane and path buffers of the given capacities, cleared (reallocated on growth)."""
-        import torch
-
         if self._shape != (capacity, path_capacity):
             n, p, slots = self.n, self.planes.size, max(self.beam.saved.size, 1)
-            f64 = dict(dtype=torch.float64, device="cuda")
-            self._hits = torch.empty((n, p, capacity, 8), **f64)
-            self._counts = torch.empty((n, p), dtype=torch.int64, device="cuda")
-            self._paths = torch.empty((slots, path_capacity if self._records else 1, 2), **f64)
-            self._path_count = torch.empty(slots, dtype=torch.int64, device="cuda")
+            self._hits = DeviceArray((n, p, capacity, 8), np.float64)
+            self._counts = DeviceArray((n, p), np.int64)
+            self._paths = DeviceArray((slots, path_capacity if self._records else 1, 2), np.float64)
+            self._path_count = DeviceArray(slots, np.int64)
             self._shape = (capacity, path_capacity)
         self._hits.zero_()
         self._counts.zero_()
-        self._paths.zero_()
         self._path_count.fill_(1)
-        if self._records:
-            self._paths[: self.beam.saved.size, 0] = torch.from_numpy(self.beam.first).cuda()
+        if self._records:  # every path's first point is its start
+            first = np.zeros(self._paths.shape)
+            first[: self.beam.saved.size, 0] = self.beam.first
+            self._paths.copy_(first)
+        else:
+            self._paths.zero_()
 
     def trace(self, fx, fy) -> np.ndarray:
         """Trace the beam through the edge fields ``fx`` ``(nx + 1, ny)`` and ``fy``
-        ``(nx, ny + 1)`` (float64 GPU tensors) and return the node charge density (host,
+        ``(nx, ny + 1)`` (float64 device arrays) and return the node charge density (host,
         divided by the node volumes, as ``HipBeamTracer.trace``)."""
         tracer = self.tracer
-        fxs, fys = fx.contiguous().view(-1), fy.contiguous().view(-1)
+        fxs, fys = fx.reshape(-1), fy.reshape(-1)
         if self._maxima is None:
             exponent = kernels.deposit_exponent(1.0)
         else:
-            emax = max((float(f.abs().max().item()) for f in (fxs, fys) if f.numel()), default=0.0)
+            emax = max((device.abs_max(f) for f in (fxs, fys) if f.numel()), default=0.0)
             exponent = kernels.exponent_of(*self._maxima, emax, float(np.min(self.patch_real[:, 2])),
                                            tracer.step_fraction)  # fmt: skip
         s = self._state
@@ -134,9 +132,10 @@ class ResidentTrace:
                                               record_paths=self._records, wide_window=tracer.wide_window)  # fmt: skip
                 active = resume_paused(tracer, active, s["x"], s["status"], s["electrode"])
             grown = outgrown(tracer._capacity, self._counts, self._path_count)
-        self.steps = int(self._steps.max().item()) if self.n else 0
-        self.particle_steps = int(self._steps.sum().item())
-        charge = self._charge[0].cpu().numpy().reshape(tracer.mesh.shape)
+        steps = self._steps.numpy()
+        self.steps = int(steps.max()) if self.n else 0
+        self.particle_steps = int(steps.sum())
+        charge = self._charge[0].numpy().reshape(tracer.mesh.shape)
         if tracer.reduce is not None:
             charge = tracer.reduce(charge)
         return tracer.finalize(charge)
@@ -144,9 +143,9 @@ class ResidentTrace:
     def result(self, charge_density: np.ndarray) -> TraceResult:
         """The full :class:`TraceResult` of the last trace (its records cross to the host)."""
         s = self._state
-        records = tuple(t.cpu().numpy() for t in (self._hits, self._counts, self._paths, self._path_count))
-        steps = self._steps.cpu().numpy()
-        self.tracer.last_state = (s["x"].cpu().numpy(), s["v"].cpu().numpy(), steps)
+        records = tuple(t.numpy() for t in (self._hits, self._counts, self._paths, self._path_count))
+        steps = self._steps.numpy()
+        self.tracer.last_state = (s["x"].numpy(), s["v"].numpy(), steps)
         self.tracer.last_records = records[:2]
-        status, electrode = s["status"].cpu().numpy(), s["electrode"].cpu().numpy()
+        status, electrode = s["status"].numpy(), s["electrode"].numpy()
         return trace_result(self.beam, charge_density, status, electrode, steps, records, self.tracer.planes)

@@ -29,8 +29,9 @@ from dataclasses import dataclass, field
 import numpy as np
 from scipy import constants
 
-from beamtracer_amd import kernels, tuning
+from beamtracer_amd import device, kernels, tuning
 from beamtracer_amd.beam import BeamParticles
+from beamtracer_amd.device import DeviceArray
 from beamtracer_amd.fields import StaggeredElectricField
 from beamtracer_amd.mesh import IonOpticsMesh
 from beamtracer_amd.refinement import PatchedElectricField
@@ -147,22 +148,25 @@ def beam_start(tracer: HipBeamTracer, particles: BeamParticles) -> BeamStart:
     )
 
 
-def _host(a):
-    return a.cpu().numpy() if hasattr(a, "cpu") else a
-
-
 def resume_paused(tracer: HipBeamTracer, active, x, status, electrode):
     """Settle the trajectories of ``active`` that the kernel paused in a cut sub-cell
     (``tracer.py:1236-1256``): those inside an electrode by its own test end there, the
     others are marked alive again and returned, to be traced on. Works in place on numpy
-    arrays or on GPU tensors (then only the paused positions cross to the host)."""
+    arrays or on the library's device arrays (then only the paused trajectories' indices
+    and positions cross to the host, and their new states go back by index)."""
+    if isinstance(x, DeviceArray):
+        # the kernel pauses only active trajectories: the paused ones among all are these
+        paused = device.select_equal(status, _PAUSED)
+        if not paused.size:
+            return DeviceArray(0, np.int64)
+        owner = np.asarray(tracer.mesh.inside_solid(*tracer._project(device.gather_columns(x, paused))), dtype=np.int64)
+        struck = owner >= 0
+        device.scatter(status, paused, np.where(struck, ELECTRODE, _ALIVE))
+        device.scatter(electrode, paused[struck], owner[struck])
+        return DeviceArray.from_numpy(paused[~struck])
     paused = active[status[active] == _PAUSED]
     if len(paused):
-        owner = np.asarray(tracer.mesh.inside_solid(*tracer._project(_host(x[:, paused]))), dtype=np.int64)
-        if hasattr(paused, "cpu"):
-            import torch
-
-            owner = torch.from_numpy(owner).to(paused.device)
+        owner = np.asarray(tracer.mesh.inside_solid(*tracer._project(x[:, paused])), dtype=np.int64)
         struck = owner >= 0
         status[paused] = _ALIVE
         status[paused[struck]] = ELECTRODE
@@ -174,9 +178,12 @@ def resume_paused(tracer: HipBeamTracer, active, x, status, electrode):
 def outgrown(capacity: tuple[int, int], counts, path_count) -> tuple[int, int] | None:
     """The plane-record and path capacities that a repeat of an overflowed trace needs, or
     None when the trace fit (``tracer.py:1257-1263``): the kernel counts every crossing
-    and every path point, written or not."""
-    most = int(counts.max()) if len(counts) else 0
-    need = (max(capacity[0], most), max(capacity[1], int(path_count.max())))
+    and every path point, written or not (host or device arrays)."""
+    if isinstance(counts, DeviceArray):
+        most, longest = device.max_int(counts), device.max_int(path_count)
+    else:
+        most, longest = (int(counts.max()) if len(counts) else 0), int(path_count.max())
+    need = (max(capacity[0], most), max(capacity[1], longest))
     return None if need == tuple(capacity) else need
 
 
@@ -496,18 +503,16 @@ class HipBeamTracer:
                                           refill_block=self.refill_block, wide_window=self.wide_window)  # fmt: skip
 
     def _call_device(self, call: list, magnetic, exponent: int) -> None:
-        """One kernel call with the arrays on the GPU (copied back into ``call`` after)."""
-        import torch
-
-        dev = [torch.from_numpy(a).cuda() if isinstance(a, np.ndarray) else a for a in call]
+        """One kernel call with the arrays in the library's device memory (copied back into
+        ``call`` after)."""
+        dev = [DeviceArray.from_numpy(a) if isinstance(a, np.ndarray) else a for a in call]
         for i in (10, 11, 24):  # patch tables and planes stay on the host
             dev[i] = call[i]
-        mag = None if magnetic is None else tuple(None if c is None else torch.from_numpy(c).cuda() for c in magnetic)
+        mag = None if magnetic is None else tuple(None if c is None else DeviceArray.from_numpy(c) for c in magnetic)
         ck, interval = self.checkpoints if self.checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size,
                                       refill_block=self.refill_block, record_paths=bool((call[27] >= 0).any()),
                                       wide_window=self.wide_window,
                                       checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
-        torch.cuda.synchronize()
         for i in (1, 2, 5, 6, 7, 25, 26, 28, 29, 31):
-            call[i][...] = dev[i].cpu().numpy()
+            call[i][...] = dev[i].numpy()

@@ -139,7 +139,7 @@ def vlasov_iterate(
             break
     assert state is not None
     if resident is not None and fields_ is not None:
-        state.electric = StaggeredElectricField.from_edges(mesh, fields_[0].cpu().numpy(), fields_[1].cpu().numpy())
+        state.electric = StaggeredElectricField.from_edges(mesh, fields_[0].numpy(), fields_[1].numpy())
         tracer.electric = state.electric
         state.trace = resident.result(deposited)
     return state

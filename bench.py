@@ -466,11 +466,11 @@ def batched_leg(count: int, seed: int, processes: int, repeats: int) -> dict:
         bt._restart(dev)
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        bt.launch(dev, dev["queue"])
+        bt.launch(dev, dev["queue"], stream=stream)
         b.record(stream)
         torch.cuda.synchronize()
         times.append(a.elapsed_time(b) / 1e3)
-    batched_steps = int(dev["steps"].sum().item())
+    batched_steps = int(dev["steps"].numpy().sum())
     batched_s = float(np.median(times))
     # the same traces as one plain launch per design (inputs resident on the GPU)
     plain_s = 0.0
@@ -505,7 +505,7 @@ def batched_leg(count: int, seed: int, processes: int, repeats: int) -> dict:
         b.record(stream)
         torch.cuda.synchronize()
         gtimes.append(a.elapsed_time(b) / 1e3)
-    graph_steps = int(bt._graph_state[1]["steps"].sum().item())
+    graph_steps = int(bt._graph_state[1]["steps"].numpy().sum())
     bt.close()
     return {
         "workload": f"{count} einzel candidates from seed {seed}: nx in [201,1025], ny in [51,257], "

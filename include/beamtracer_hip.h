@@ -369,6 +369,38 @@ int64_t bt_adjoint_workspace_bytes(int64_t n_active, int64_t max_step, int32_t i
                                    int64_t cylindrical, int32_t block_size);
 int bt_trace_adjoint_device(const bt_adjoint_args* args, void* stream);
 
+/* ---- device arrays owned by the library ------------------------------------------------
+   No counterpart in the reference (its caller keeps host arrays): the host code's
+   device-resident state (resident beams, batched designs, edge fields on the GPU) lives in
+   memory allocated here, and these are the operations it needs between traces.  `stream`
+   NULL is the null stream.  Copies to and from the host return once complete; the helpers
+   that report a count or a value return once their kernel has run.  An index out of range is
+   skipped and reported as BT_E_ARG.  The helpers share one per-device counter: calls on one
+   device must not overlap. */
+int bt_compute_units(void);                      /* of the current device (or an error code) */
+int bt_device_alloc(int64_t bytes, void** out);  /* *out NULL for 0 bytes */
+int bt_device_free(void* ptr);
+int bt_copy_to_device(void* dst, const void* src, int64_t bytes, void* stream);
+int bt_copy_to_host(void* dst, const void* src, int64_t bytes, void* stream);
+int bt_copy_on_device(void* dst, const void* src, int64_t bytes, void* stream);  /* asynchronous */
+int bt_fill_i64(int64_t* dst, int64_t value, int64_t n, void* stream);           /* asynchronous */
+int bt_fill_f64(double* dst, double value, int64_t n, void* stream);             /* asynchronous */
+int bt_synchronize(void* stream);
+/* *out = max |a[i]| (NaN if any is NaN; 0 for n = 0) */
+int bt_abs_max_f64(const double* a, int64_t n, double* out, void* stream);
+/* *out = max a[i] (n >= 1) */
+int bt_max_i64(const int64_t* a, int64_t n, int64_t* out, void* stream);
+/* idx[0 .. *count) = the i with a[i] == value, in no particular order (idx holds n) */
+int bt_select_equal_i64(const int64_t* a, int64_t n, int64_t value, int64_t* idx, int64_t* count, void* stream);
+/* dst (rows, m) = src (rows, n) at the columns idx[0 .. m) */
+int bt_gather_columns_f64(const double* src, int64_t rows, int64_t n, const int64_t* idx, int64_t m, double* dst,
+                          void* stream);
+/* dst (n)[idx[k]] = values[k] for k < m */
+int bt_scatter_i64(int64_t* dst, int64_t n, const int64_t* idx, const int64_t* values, int64_t m, void* stream);
+/* dst (m, 8) = hits (n, n_planes, capacity, 8)[trajectory[r], plane, slot[r], :] */
+int bt_gather_records(const double* hits, int64_t n, int64_t n_planes, int64_t capacity, int64_t plane,
+                      const int64_t* trajectory, const int64_t* slot, int64_t m, double* dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,8 +65,8 @@ def test_device_edge_field_equals_the_host_fields_on_real_meshes(which):
     phi = poisson.IonOpticsPoisson(m).solve()
     host = fields.StaggeredElectricField(m, phi)
     fx, fy = fields.DeviceEdgeField(m)(phi)
-    np.testing.assert_array_equal(fx.cpu().numpy(), host.fx)
-    np.testing.assert_array_equal(fy.cpu().numpy(), host.fy)
+    np.testing.assert_array_equal(fx.numpy(), host.fx)
+    np.testing.assert_array_equal(fy.numpy(), host.fy)
     ox, oy = oracle.edge_fields(phi, m.solid, m.near_solid, m.distance, m.dx, m.h, fields.DeviceEdgeField(m).faces)
     np.testing.assert_array_equal(host.fx, ox)
     np.testing.assert_array_equal(host.fy, oy)
