@@ -384,7 +384,10 @@ class BatchedTracer:
         design_of = host["design_of"].astype(np.int64)
         out = []
         for p in range(len(self.tracers[0].planes)):
-            k, c = np.nonzero(np.arange(capacity)[None, :] < np.minimum(counts[:, p], capacity)[:, None])
+            # every filled slot (trajectory k, slot c), k ascending and c ascending within k
+            filled = np.minimum(counts[:, p], capacity)
+            k = np.repeat(np.arange(filled.size, dtype=np.int64), filled)
+            c = np.arange(k.size, dtype=np.int64) - np.repeat(np.cumsum(filled) - filled, filled)
             record = device.gather_records(hits, p, k, c)
             # k ascends in nonzero's output; a stable sort on (design, step) keeps it
             # ascending among equal keys
@@ -400,13 +403,13 @@ class BatchedTracer:
         dead = host["status"] == ELECTRODE
         status = np.where(dead, host["status"], dev["status"].numpy()).astype(np.int8)
         electrode = np.where(dead, host["electrode"], dev["electrode"].numpy()).astype(np.int32)
-        design_of = host["design_of"].astype(np.int64)
-        n_designs = len(self._live)
         steps = dev["steps"].numpy()
-        longest = np.zeros(n_designs, dtype=np.int64)
-        np.maximum.at(longest, design_of, steps)
-        total = np.zeros(n_designs, dtype=np.int64)
-        np.add.at(total, design_of, steps)
+        # each design's trajectories are one run of the arrays (host["first"])
+        first = np.asarray(host["first"], dtype=np.int64)
+        sizes = np.diff(first)
+        starts = np.minimum(first[:-1], max(steps.size - 1, 0))
+        longest = np.where(sizes > 0, np.maximum.reduceat(steps, starts) if steps.size else 0, 0)
+        total = np.where(sizes > 0, np.add.reduceat(steps, starts) if steps.size else 0, 0)
         charge = self._dev["charge"].numpy()
         records = self._plane_records(dev, host)
         first = np.asarray(host["first"])
