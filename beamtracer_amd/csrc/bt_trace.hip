@@ -7,25 +7,37 @@
 // (status 4).  Each step takes the classical Runge--Kutta step (integrator 0, four field
 // evaluations) or Boris's kick--drift--kick (1, one evaluation: the end field is the next
 // start's), deposits dq dt at the Hermite midpoint, records plane crossings and path
-// points, and tests the end point.  The arithmetic follows the reference statement by
-// statement in the same order (the library is built with -ffp-contract=off), except that
-// hypot, the square roots of the step-size rule and the divisions by r, |v|, |a| and the
-// mesh spacing become products with reciprocal square roots and reciprocals (bt_device.h):
-// a few units in the last place per operation, against the CPU restatement's 1e-10 bar.
+// points, and tests the end point.  The operations are the reference's, in its order (the
+// library is built with -ffp-contract=off, so only the fusions written out below are fused),
+// with these changes of rounding, each a few units in the last place against the CPU
+// restatement's 1e-10 bar:
+// * hypot, the step-size rule's square roots and the divisions by r, |v| and sqrt|a| go
+//   through one hardware reciprocal square root with a third-order correction (rsqrt_nr,
+//   radial, step_rule in bt_device.h); the cylindrical deposit weight's division through a
+//   corrected reciprocal (recip_nr); divisions by the mesh spacing and by 6 are products
+//   with their reciprocals;
+// * fused multiply-adds (__builtin_fma) in the RK4 stage points and velocities
+//   (x + c k, v + c g), the weighted stage sums, the bilinear interpolation of the gather
+//   and the Hermite midpoint; the adjoint's forward recompute fuses the same operations,
+//   so the two stay bit-identical.
 //
 // MI355X design:
 // * persistent grid, one trajectory per lane, refilled from a device work queue: a
 //   wave whose lanes finish claims that many new trajectories with one atomic, so lanes
 //   freed by short trajectories pick up the remaining ones;
-// * state in registers (x, v component rows are coalesced loads / stores);
+// * state in registers (x, v component rows are coalesced loads / stores); the launch's
+//   rarely used fields are read from the kernel-argument segment where they are used, so
+//   the step loop's scalar registers hold only what every step needs;
 // * the FP64 edge fields (2.6 MB for an 801 x 201 mesh) stay resident in each XCD's 4 MB
-//   L2 and are read with plain loads;
-// * the deposit is exact fixed point (deterministic): each workgroup sums its deposits in
-//   an LDS window of node sums that follows its beam (integer LDS atomics) and adds a
-//   node's sum to global memory once when the window leaves it.  Global integer atomics
-//   on this GPU are executed at the memory side, where same-node adds from every wave
-//   serialize; the window turns one atomic per deposit into one per workgroup and node.
-//   A finalize kernel converts and adds to charge row 0;
+//   L2 and are read through buffer resources, one 16-byte load per row pair;
+// * the deposit is exact fixed point (deterministic): each lane sums its trajectory's
+//   deposits into the four nodes of its cell in registers, each wave sums its lanes' node
+//   sums in an LDS window of 16 (or 32) columns that follows its lanes (integer LDS
+//   atomics, no barriers: the window is the wave's own), and a window column that the wave
+//   leaves is added to global memory once.  Global integer atomics on this GPU are executed
+//   at the memory side, where same-node adds from every wave serialize; the window turns
+//   one atomic per deposit into one per wave and node.  A finalize kernel converts the
+//   node words and adds them to charge row 0;
 // * compile-time variants on integrator, geometry, magnetic field and the general
 //   features (refined patches, graded axis, relativistic motion), so the common case
 //   carries no dead branches.
