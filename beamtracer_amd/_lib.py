@@ -32,6 +32,7 @@ BT_OK, BT_E_ARG, BT_E_NOMEM, BT_E_LAUNCH, BT_E_DEVICE, BT_E_UNSUPPORTED = 0, -1,
 BT_REFILL_BLOCK = 1
 BT_NO_PATHS = 2
 BT_WIDE_WINDOW = 4
+BT_MAGNETIC = 8
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32
@@ -144,6 +145,7 @@ class BatchArgs(ctypes.Structure):
         ("cylindrical", _i64), ("integrator", _i64),
         ("step_fraction", _dbl), ("max_steps", _i64), ("refine", _i64),
         ("block_size", _i32), ("max_blocks", _i32), ("flags", _i32), ("reserved", _i32),
+        ("b0", _ptr), ("b1", _ptr), ("b2", _ptr),
     ]  # fmt: skip
 
 

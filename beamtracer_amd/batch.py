@@ -66,9 +66,10 @@ class LaunchShape:
 class BatchedTracer:
     """Trace the beams of many designs in one launch; one :class:`HipBeamTracer` per design.
 
-    Every design is a single uniform mesh (no refined patches, no graded axis), classical
-    and electrostatic, without saved paths; all share the integrator, geometry, step
-    fraction, step limit and number of diagnostic planes.
+    Every design is a single uniform mesh (no refined patches, no graded axis), classical,
+    without saved paths; all share the integrator, geometry, step fraction, step limit,
+    number of diagnostic planes and whether they carry a node magnetic field (each design
+    its own, on its mesh's nodes).
 
     A design that cannot be traced -- no field yet, a refined or non-finite field, a mesh
     beyond the kernel's limits, a non-finite beam -- is left out of the launches instead of
@@ -84,12 +85,13 @@ class BatchedTracer:
             raise ValueError("no designs")
         t0 = tracers[0]
         for t in tracers:
-            if t.mesh.graded or t.relativistic or t.magnetic is not None or t.save_trajectories:
-                raise ValueError("the batched form takes uniform, classical, electrostatic designs without paths")
-            if (t.integrator, geometry_code(t.mesh), t.step_fraction, t.max_steps, len(t.planes)) != (
-                t0.integrator, geometry_code(t0.mesh), t0.step_fraction, t0.max_steps, len(t0.planes)
-            ):  # fmt: skip
-                raise ValueError("the designs must share integrator, geometry, step rule and plane count")
+            if t.mesh.graded or t.relativistic or t.save_trajectories:
+                raise ValueError("the batched form takes uniform, classical designs without paths")
+            if (t.integrator, geometry_code(t.mesh), t.step_fraction, t.max_steps, len(t.planes),
+                    t.magnetic is None) != (t0.integrator, geometry_code(t0.mesh), t0.step_fraction, t0.max_steps,
+                                            len(t0.planes), t0.magnetic is None):  # fmt: skip
+                raise ValueError("the designs must share integrator, geometry, step rule, plane count and whether "
+                                 "they have a magnetic field")
         self.tracers = list(tracers)
         self.shape = shape or LaunchShape()
         self.capacity = 4
@@ -162,6 +164,16 @@ class BatchedTracer:
         self.total_nodes = ofn
         n_planes = len(self.tracers[0].planes)
         self._n_planes = n_planes
+        # the node magnetic fields, each component one arena over the designs' nodes (zeros
+        # for a design without that component; None when no design has it)
+        self._magnetic = [None, None, None]
+        if rows and self.tracers[0].magnetic is not None:
+            nodes = [self.tracers[d].magnetic_nodes() for d in self._live]
+            for c in range(3):
+                if any(m[c] is not None for m in nodes):
+                    self._magnetic[c] = np.concatenate([
+                        (m[c] if m[c] is not None else np.zeros(self.tracers[d].mesh.shape)).ravel()
+                        for d, m in zip(self._live, nodes)])  # fmt: skip
         if not rows:
             return
         self._fxs = np.concatenate(fxs)
@@ -218,6 +230,8 @@ class BatchedTracer:
         if "fxs" not in self._dev:
             for name in ("fxs", "fys", "cells", "subcells", "planes"):
                 self._dev[name] = DeviceArray.from_numpy(getattr(self, "_" + name))
+            for c, arena in enumerate(self._magnetic):
+                self._dev[f"b{c}"] = None if arena is None else DeviceArray.from_numpy(arena)
             self._dev["charge"] = DeviceArray.zeros(self.total_nodes, np.float64)
         self._table = list(exponents)
 
@@ -316,6 +330,7 @@ class BatchedTracer:
         a.block_size, a.max_blocks = self.shape.block_size, self.shape.max_blocks
         a.flags = (_lib.BT_REFILL_BLOCK if self.shape.refill_block else 0) | (
             _lib.BT_WIDE_WINDOW if self.shape.wide_window else 0)
+        a.b0, a.b1, a.b2 = (None if self._dev.get(f"b{c}") is None else self._dev[f"b{c}"].data_ptr() for c in range(3))
         return a
 
     def _upload(self, host: dict, capacity: int) -> dict:
@@ -604,7 +619,7 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
         tracer.shape = LaunchShape(0, 0, order)
         dev = tracer._upload(host, tracer.capacity)
         for block, wide in itertools.product(blocks, windows):
-            flags = _lib.BT_WIDE_WINDOW if wide else 0
+            flags = (_lib.BT_WIDE_WINDOW if wide else 0) | (_lib.BT_MAGNETIC if t0.magnetic is not None else 0)
             per_cu = L.bt_batch_occupancy(INTEGRATORS.index(t0.integrator), geometry_code(t0.mesh), block, flags)
             for fill in fills:
                 grid = max(1, int(per_cu * cus * fill))

@@ -412,17 +412,22 @@ bt::TraceParams batch_params(const BatchPlan* plan, const bt_batch_args* a) {
     p.charge = a->charge;
     p.designs = plan->d_designs; p.design_of = a->design_of; p.n_designs = (int)plan->designs.size();
     p.refill_block = (a->flags & BT_REFILL_BLOCK) != 0;
+    // the magnetic arenas: each design's lanes add its node offset (load_design)
+    p.b0 = a->b0; p.b1 = a->b1; p.b2 = a->b2;
     return p;
 }
+
+bool batch_has_b(const bt_batch_args* a) { return a->b0 != nullptr || a->b1 != nullptr || a->b2 != nullptr; }
 
 int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream) {
     const bt::TraceParams p = batch_params(plan, a);
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const int block = a->block_size > 0 ? a->block_size : 256;
     const bool wide = (a->flags & BT_WIDE_WINDOW) != 0;
+    const bool has_b = batch_has_b(a);
     int grid = 0;
     if (a->n_active > 0) {
-        int per_cu = bt::batched_occupancy(integ, geom, false, wide, block);
+        int per_cu = bt::batched_occupancy(integ, geom, has_b, wide, block);
         if (per_cu < 1) per_cu = 1;
         const i64 full = (i64)per_cu * plan->cus;
         const i64 need = (a->n_active + block - 1) / block;
@@ -431,7 +436,7 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
         if (grid < 1) grid = 1;
     }
     const bool reset = a->x0 != nullptr;
-    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, false, wide, plan->total_nodes,
+    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, has_b, wide, plan->total_nodes,
                            plan->max_nodes))
         return fail(BT_E_LAUNCH, std::string("batched launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;
@@ -1089,7 +1094,8 @@ int64_t bt_batch_plan_nodes(void* handle) {
 int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size, int32_t flags) {
     if (integrator < 0 || integrator > 1 || cylindrical < 0 || cylindrical > 2) return fail(BT_E_ARG, "bad variant");
     if (block_size % 64 != 0 || block_size < 64 || block_size > 256) return fail(BT_E_ARG, "bad block size");
-    return bt::batched_occupancy((int)integrator, (int)cylindrical, false, (flags & BT_WIDE_WINDOW) != 0, block_size);
+    return bt::batched_occupancy((int)integrator, (int)cylindrical, (flags & BT_MAGNETIC) != 0,
+                                 (flags & BT_WIDE_WINDOW) != 0, block_size);
 }
 
 int bt_batch_trace_device(void* handle, const bt_batch_args* a, void* stream) {

@@ -742,6 +742,10 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
     lp.planes_base = d * p.n_planes;
     lp.deposit_exponent = b.deposit_exponent;
     lp.design = d;
+    // a design's node magnetic field is its run of the launch's node arenas
+    lp.b0 = p.b0 != nullptr ? p.b0 + b.node_offset : nullptr;
+    lp.b1 = p.b1 != nullptr ? p.b1 + b.node_offset : nullptr;
+    lp.b2 = p.b2 != nullptr ? p.b2 + b.node_offset : nullptr;
 }
 
 // The persistent trace loop of one wave.  Waves run independently (no barrier): each claims
@@ -987,6 +991,8 @@ TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide) {
     if (integ == I && geom == G && has_b == B) return wide ? batched_kernel<I, G, B, kWideCols> : batched_kernel<I, G, B, kWinCols>;
     BT_SELB(0, 0, false) BT_SELB(0, 1, false) BT_SELB(0, 2, false)
     BT_SELB(1, 0, false) BT_SELB(1, 1, false) BT_SELB(1, 2, false)
+    BT_SELB(0, 0, true) BT_SELB(0, 1, true) BT_SELB(0, 2, true)
+    BT_SELB(1, 0, true) BT_SELB(1, 1, true) BT_SELB(1, 2, true)
 #undef BT_SELB
     return nullptr;
 }

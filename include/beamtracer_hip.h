@@ -66,6 +66,10 @@ extern "C" {
  * patches, a graded axis or relativistic motion). */
 #define BT_WIDE_WINDOW 4
 
+/* flag of bt_batch_occupancy only: the variant with a node magnetic field (a batch whose
+ * bt_batch_args carry magnetic arenas runs it). */
+#define BT_MAGNETIC 8
+
 /* Every argument of trace_trajectories_2d (trace_trajectories_2d_pyccel.py:42-78) as a
  * pointer with its extents, then the node magnetic field that the reference's compiled
  * kernel lacks (its NumPy path takes B, solver/tracer.py:519-529, 605-650), then launch
@@ -278,6 +282,13 @@ typedef struct bt_batch_args {
                                                      queue entries: mostly one design);
                                                      BT_WIDE_WINDOW: 32-column deposit
                                                      windows                                */
+    const double* b0; const double* b1; const double* b2;  /* node magnetic field arenas of
+                                                     bt_batch_plan_nodes() doubles, a design's
+                                                     nodes at its node_offset ((B_x, B_r,
+                                                     B_theta) cylindrical, (B_x, B_y, B_z)
+                                                     planar); all NULL: no field, the
+                                                     electrostatic variant; a NULL component
+                                                     is zero */
 } bt_batch_args;
 
 int64_t bt_batch_args_size(void);
@@ -293,7 +304,8 @@ int bt_batch_plan_update(void* plan, const bt_batch_design* designs, int64_t n_d
 /* Total nodes of the plan's designs (the length of `charge`). */
 int64_t bt_batch_plan_nodes(void* plan);
 /* Resident blocks per CU of the batched kernel (for launch-shape tuning); `flags` are the
-   trace's flags, of which BT_WIDE_WINDOW (the 32-column deposit windows) counts here. */
+   trace's flags, of which BT_WIDE_WINDOW (the 32-column deposit windows) and BT_MAGNETIC
+   (the variant with a magnetic field) count here. */
 int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_size, int32_t flags);
 /* One batched trace, asynchronous on `stream`; allocates nothing. */
 int bt_batch_trace_device(void* plan, const bt_batch_args* args, void* stream);

@@ -130,3 +130,37 @@ def test_designs_that_cannot_be_traced_are_left_out_and_the_others_are_unchanged
     assert set(bt.failures) == {1, 2, 4, 5} and again[5].failure is not None
     _assert_identical(again[0], got[0])
     _assert_identical(again[3], got[3])
+
+
+def _with_magnetic_field(tracer, integrator, strength):
+    """The design's tracer with a node magnetic field: an axial field peaked mid-mesh and the
+    radial field of its divergence, (B_x, B_r, no B_theta)."""
+    from beamtracer_amd.tracer import HipBeamTracer
+
+    mesh = tracer.mesh
+    x, r = mesh.nodes()
+    mid, width = 0.5 * (mesh.origin[0] + mesh.upper[0]), 0.2 * (mesh.upper[0] - mesh.origin[0])
+    g = np.exp(-(((x - mid) / width) ** 2))
+    bx = strength * g
+    br = strength * r * (x - mid) / width**2 * g  # -r/2 dBx/dx
+    out = HipBeamTracer(mesh, tracer.electric, (bx, br, None), planes=tracer.planes,
+                        step_fraction=tracer.step_fraction, max_steps=tracer.max_steps, integrator=integrator)  # fmt: skip
+    out._lookup = tracer._lookup
+    return out
+
+
+@pytest.mark.parametrize("integrator", ["boris", "rk4"])
+def test_batched_designs_with_magnetic_fields_equal_plain_traces(integrator):
+    ds, tracers = _small_batch(4, seed=13)
+    beams = [d.particles for d in ds]
+    magnetic = [_with_magnetic_field(t, integrator, 0.05 + 0.02 * k) for k, t in enumerate(tracers)]
+    bt = batch.BatchedTracer(magnetic)
+    got = bt.trace(beams)
+    for k, (t, g) in enumerate(zip(magnetic, got)):
+        assert g.failure is None
+        _assert_identical(g, t.trace(beams[k]))
+    # the field turns the beam: its space charge differs from the electrostatic design's
+    plain = tracers[0].trace(beams[0])
+    assert not np.array_equal(got[0].charge_density, plain.charge_density)
+    with pytest.raises(ValueError, match="magnetic"):
+        batch.BatchedTracer([magnetic[0], tracers[1]])
