@@ -24,11 +24,13 @@ wave's lanes start alike and stay together. The results do not depend on the ord
 from __future__ import annotations
 
 import itertools
+import time
 from collections.abc import Sequence
 
 import numpy as np
 
-from beamtracer_amd import kernels
+from beamtracer_amd import device, kernels
+from beamtracer_amd.device import DeviceArray
 
 #: queue orders of :func:`tune_trace`: the emitter's (``active`` as given) or along the
 #: Morton curve of the starting states (:func:`coherent_order`)
@@ -77,27 +79,49 @@ def coherent_order(active: np.ndarray, x: np.ndarray, v: np.ndarray, qm: np.ndar
 _OUTPUTS = (1, 2, 5, 6, 7, 25, 26, 28, 29, 31)
 
 
+def _host_copy(a) -> np.ndarray:
+    """A host copy of a device array (the library's or a torch tensor)."""
+    return a.numpy() if isinstance(a, DeviceArray) else a.detach().cpu().numpy()
+
+
+def _like(values: np.ndarray, like):
+    """``values`` on the device, as the same kind of array as ``like``."""
+    if isinstance(like, DeviceArray):
+        return DeviceArray.from_numpy(values)
+    import torch
+
+    return torch.from_numpy(values).to(like.device)
+
+
+def _synchronize(like) -> None:
+    if isinstance(like, DeviceArray):
+        device.synchronize()
+    else:
+        import torch
+
+        torch.cuda.synchronize()
+
+
 def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks: Sequence[int] = (64, 128, 256),
                refills: Sequence[bool] = (False, True), windows: Sequence[bool] = (False, True),
                orders: Sequence[str] = QUEUE_ORDERS, repeats: int = 2) -> dict:
-    """Time ``trace_trajectories_2d`` on ``args`` (the 35 arguments, torch CUDA tensors for the
-    arrays) for every (queue order, threads per workgroup, refill mode, window width),
-    restoring the outputs before each launch, and return ``{"timings_ms": {...}, "block_size":
-    b, "refill_block": r, "wide_window": w, "queue_order": o}`` (the fastest, best of
-    ``repeats``). On return ``args`` holds one trace with the fastest shape and its ``active``
-    in the chosen order."""
-    import torch
-
+    """Time ``trace_trajectories_2d`` on ``args`` (the 35 arguments, device arrays: the
+    library's :class:`~beamtracer_amd.device.DeviceArray` or GPU torch tensors) for every
+    (queue order, threads per workgroup, refill mode, window width), restoring the outputs
+    before each launch, and return ``{"timings_ms": {...}, "block_size": b, "refill_block":
+    r, "wide_window": w, "queue_order": o}`` (the fastest, best of ``repeats``; each launch
+    timed between two synchronizations). On return ``args`` holds one trace with the fastest
+    shape and its ``active`` in the chosen order."""
+    like = args[1]
     initial = {i: args[i].clone() for i in _OUTPUTS if hasattr(args[i], "clone")}
     queues = {"emitter": args[0].clone()}
     if "coherent" in orders:
-        host = [args[i].cpu().numpy() for i in (0, 1, 2, 3)]
-        queues["coherent"] = torch.from_numpy(coherent_order(*host, int(args[17]))).to(args[0].device)
+        host = [_host_copy(args[i]) for i in (0, 1, 2, 3)]
+        queues["coherent"] = _like(coherent_order(*host, int(args[17])), like)
     if exponent is None:
-        host = [args[i].cpu().numpy() for i in (0, 2, 4, 3, 8, 9)]
+        host = [_host_copy(args[i]) for i in (0, 2, 4, 3, 8, 9)]
         exponent = kernels.trace_exponent(*host, args[11], args[19], args[34])
-    record_paths = bool((args[27] >= 0).any())
-    stream = torch.cuda.current_stream()
+    record_paths = bool((_host_copy(args[27]) >= 0).any())
     timings = {}
     best = None
     for order, wide, refill, block in itertools.product(orders, windows, refills, blocks):
@@ -106,14 +130,12 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
         for _ in range(repeats):
             for i, a in initial.items():
                 args[i].copy_(a)
-            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            start.record(stream)
+            _synchronize(like)
+            start = time.perf_counter()
             kernels.trace_trajectories_2d(*args, magnetic=magnetic, exponent=exponent, block_size=block,
-                                          refill_block=refill, record_paths=record_paths, wide_window=wide,
-                                          stream=stream)  # fmt: skip
-            end.record(stream)
-            torch.cuda.synchronize()
-            times.append(start.elapsed_time(end))
+                                          refill_block=refill, record_paths=record_paths, wide_window=wide)  # fmt: skip
+            _synchronize(like)
+            times.append((time.perf_counter() - start) * 1e3)
         key = f"block{block}" + ("/refill-block" if refill else "") + ("/wide" if wide else "")
         if order != "emitter":
             key = f"{order}/{key}"
@@ -125,7 +147,7 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
     for i, a in initial.items():
         args[i].copy_(a)
     kernels.trace_trajectories_2d(*args, magnetic=magnetic, exponent=exponent, block_size=best[1],
-                                  refill_block=best[2], record_paths=record_paths, wide_window=best[3],
-                                  stream=stream)  # fmt: skip
+                                  refill_block=best[2], record_paths=record_paths, wide_window=best[3])  # fmt: skip
+    _synchronize(like)
     return {"timings_ms": timings, "block_size": best[1], "refill_block": best[2], "wide_window": best[3],
             "queue_order": best[4], "best_ms": best[0]}

@@ -11,22 +11,28 @@ from oracle import cases
 pytestmark = pytest.mark.gpu
 
 
-def test_tuned_trace_equals_the_default_trace():
-    import torch
-
+@pytest.mark.parametrize("arrays", ["library", "torch"])
+def test_tuned_trace_equals_the_default_trace(arrays):
     from beamtracer_amd import kernels, tuning
+    from beamtracer_amd.device import DeviceArray
 
     args = cases.kernel_case(1, 0, refined=False)
     args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
     host = cases.copy_args(args)
     kernels.trace_trajectories_2d(*host)
-    dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+    if arrays == "library":
+        dev = [DeviceArray.from_numpy(a) if isinstance(a, np.ndarray) else a for a in args]
+        back = DeviceArray.numpy
+    else:
+        import torch
+
+        dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+        back = lambda t: t.cpu().numpy()  # noqa: E731
     for i in (10, 11, 24):  # patch tables and planes stay on the host
         dev[i] = args[i]
     tune = tuning.tune_trace(dev, blocks=(64, 256), repeats=1)
     shapes = {"block64", "block256", "block64/refill-block", "block256/refill-block"}
     shapes = shapes | {f"{k}/wide" for k in shapes}
     assert set(tune["timings_ms"]) == shapes | {f"coherent/{k}" for k in shapes}
-    torch.cuda.synchronize()
     for i in (*cases.OUTPUTS, cases.CHARGE):
-        assert np.array_equal(dev[i].cpu().numpy(), host[i]), i
+        assert np.array_equal(back(dev[i]), host[i]), i
