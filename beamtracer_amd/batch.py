@@ -387,7 +387,9 @@ class BatchedTracer:
             active = paused[~inside]
             if not active.size:
                 return
-            self.launch(dev, DeviceArray.from_numpy(active.astype(np.int64)))
+            # (held until the next pass has synchronized: the launch is asynchronous)
+            resumed = DeviceArray.from_numpy(active.astype(np.int64))
+            self.launch(dev, resumed)
 
     def _plane_records(self, dev: dict, host: dict) -> list[tuple[np.ndarray, np.ndarray]]:
         """Per plane, the recorded crossings of every design as ``(trajectory, record)``:
