@@ -30,13 +30,14 @@ class _Block:
     __slots__ = ("ptr", "nbytes")
 
     def __init__(self, nbytes: int) -> None:
+        self.ptr = 0
+        self.nbytes = int(nbytes)
         out = ctypes.c_void_p()
         _lib.check(_lib.lib().bt_device_alloc(int(nbytes), ctypes.byref(out)), "bt_device_alloc")
         self.ptr = int(out.value or 0)
-        self.nbytes = int(nbytes)
 
     def __del__(self) -> None:
-        if self.ptr:
+        if getattr(self, "ptr", 0):
             try:
                 _lib.lib().bt_device_free(ctypes.c_void_p(self.ptr))
             except Exception:  # interpreter shutdown

@@ -34,3 +34,29 @@ def test_batch_leaves_out_a_design_its_plan_rejects(monkeypatch):
     assert "2^29 nodes" in bt.failures[1] and len(bt._rows) == 2
     assert bt.total_nodes == sum(tracers[k].mesh.shape[0] * tracers[k].mesh.shape[1] for k in (0, 2))
     assert bt._failed_results()[1].failure == bt.failures[1]
+
+
+def test_a_batch_whose_every_design_fails_returns_their_reasons_without_the_gpu():
+    from beamtracer_amd import batch, designs
+
+    ds = designs.einzel_batch(3, 2, nx_range=(41, 61), ny_range=(11, 21), particles_range=(10, 20))
+    tracers = [HipBeamTracer(d.mesh, None, planes=d.planes) for d in ds]
+    bt = batch.BatchedTracer(tracers)
+    assert bt._live == [] and set(bt.failures) == {0, 1, 2}
+    results = bt.trace([d.particles for d in ds])
+    assert [r.failure for r in results] == ["no electric field yet"] * 3
+    assert all(np.isnan(r.charge_density).all() and r.status.size == 0 for r in results)
+    assert results[0].charge_density.shape == ds[0].mesh.shape
+    assert len(results[1].planes) == len(ds[1].planes) and results[1].planes[0].point.shape == (3, 0)
+
+
+def test_device_arrays_fail_loudly_without_a_gpu():
+    import pytest
+
+    from beamtracer_amd import _lib
+    from beamtracer_amd.device import DeviceArray
+
+    if _lib.lib().bt_compute_units() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises((MemoryError, RuntimeError)):
+        DeviceArray.zeros(16)
