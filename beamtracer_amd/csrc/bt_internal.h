@@ -68,7 +68,7 @@ struct TraceParams {
 typedef void (*TraceKernel)(TraceParams);
 
 // `records`: paths or checkpoints may be written (else a variant without that code runs)
-// `wide`: the deposit window spans 128 columns instead of 16 (beams whose lanes spread along x)
+// `wide`: each wave's deposit window spans 32 columns instead of 16 (beams whose lanes spread along x)
 TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide);
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
                  bool has_b, bool general, bool records, bool wide, i64 nodes);

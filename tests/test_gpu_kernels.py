@@ -186,7 +186,7 @@ def test_trace_without_paths_runs_the_lean_variant_with_the_same_results(hip, cy
 
 @pytest.mark.parametrize("cylindrical,integrator,magnetic", list(itertools.product([0, 1, 2], [0, 1], [False, True])))
 def test_wide_deposit_window_gives_the_same_bits(hip, cylindrical, integrator, magnetic):
-    """A deposit window of 128 columns (``wide_window``) and the default 16 give the same trace
+    """Deposit windows of 32 columns (``wide_window``) and the default 16 give the same trace
     and the same charge, workgroup or lane refill."""
     import torch
 
