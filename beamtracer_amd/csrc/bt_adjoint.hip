@@ -169,72 +169,56 @@ __device__ __forceinline__ void add_node(KAdj kp, i64 cols, const GradWindow& W,
     unsafeAtomicAdd((A == 0 ? c->dfx : c->dfy) + (i64)ii * cols + jj, v);
 }
 
-// lambda_E times the bilinear weights of the gathered cell into the gradient of grid A
-// (0: fx, 1: fy; rows of `cols`).
-// A lane's running sums of its contributions to the four nodes of one cell of grid A (slot
-// q: the cell's node of column parity q >> 1 and row parity q & 1): consecutive stages and
-// steps mostly stay in one cell, so the node sums leave the lane -- into the wave's window
-// or global memory -- only when the lane moves to a cell without that node and when its
-// trajectory ends.
+// A lane's running sums of lambda_E times the bilinear weights of the gathered points in one
+// cell of grid A (0: fx, 1: fy; rows of `cols`), kept as the four moments sum(lam),
+// sum(lam u), sum(lam w), sum(lam u w) of the contributions: a contribution in the same cell
+// is four fused multiply-adds, and the four node values -- (1-u)(1-w), (1-u)w, u(1-w), uw
+// weighted -- are formed once, when the lane's points leave the cell or its trajectory ends,
+// and go to the wave's window or global memory.  Consecutive stages and steps mostly stay in
+// one cell.
 constexpr int kNoCell = -(1 << 30);
 
 struct GradCache {
     int c, r;  // the cell (kNoCell: none)
-    double q[4];
+    double s, su, sw, suw;
 };
 
 __device__ __forceinline__ void cache_reset(GradCache& g) {
     g.c = kNoCell;
     g.r = kNoCell;
-    g.q[0] = g.q[1] = g.q[2] = g.q[3] = 0.0;
+    g.s = g.su = g.sw = g.suw = 0.0;
 }
 
-// Moves the cache to cell (i, j): column c stays iff i is c - 1 or c, column c + 1 iff i is
-// c or c + 1 (rows alike); the other nodes' sums leave.
-template <int A>
-__device__ __forceinline__ void cache_move(KAdj kp, i64 cols, const GradWindow& W, GradCache& g,
-                                           int i, int j) {
-    if (i == g.c && j == g.r) return;
-    const int c = g.c, r = g.r;
-    const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
-    const bool keep_r = (unsigned)(r - j) <= 1u, keep_r1 = (unsigned)(r + 1 - j) <= 1u;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const bool at_c = ((c ^ (q >> 1)) & 1) == 0;  // slot q's column is c (else c + 1)
-        const bool at_r = ((r ^ (q & 1)) & 1) == 0;
-        if (!((at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1))) {
-            if (g.q[q] != 0.0) add_node<A>(kp, cols, W, at_c ? c : c + 1, at_r ? r : r + 1, g.q[q]);
-            g.q[q] = 0.0;
-        }
-    }
-    g.c = i;
-    g.r = j;
-}
-
+// The cell's four node values to the window or global memory; the cache empties.
 template <int A>
 __device__ __forceinline__ void cache_flush(KAdj kp, i64 cols, const GradWindow& W, GradCache& g) {
     const int c = g.c, r = g.r;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (g.q[q] != 0.0) add_node<A>(kp, cols, W, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1), g.q[q]);
+    if (c != kNoCell) {
+        const double n11 = g.suw, n10 = g.su - g.suw, n01 = g.sw - g.suw;
+        const double n00 = (g.s - g.su) - n01;
+        if (n00 != 0.0) add_node<A>(kp, cols, W, c, r, n00);
+        if (n01 != 0.0) add_node<A>(kp, cols, W, c, r + 1, n01);
+        if (n10 != 0.0) add_node<A>(kp, cols, W, c + 1, r, n10);
+        if (n11 != 0.0) add_node<A>(kp, cols, W, c + 1, r + 1, n11);
+    }
     cache_reset(g);
 }
 
-// lambda_E times the bilinear weights of the gathered cell into the lane's cache of grid A
-// (0: fx, 1: fy; rows of `cols`).
+// lambda_E times the bilinear weights of the gathered cell into the lane's moments of grid A.
 template <int A>
 __device__ __forceinline__ void scatter(KAdj kp, i64 cols, const GradWindow& W, GradCache& gc,
                                         const Gathered& g, double lam) {
     if (lam == 0.0) return;
-    cache_move<A>(kp, cols, W, gc, g.i, g.j);
-    const double u0 = 1.0 - g.u, w0 = 1.0 - g.w;
-    const bool io = (g.i & 1) != 0, jo = (g.j & 1) != 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const bool ia = io != ((q >> 1) != 0);  // the node is i + 1
-        const bool jb = jo != ((q & 1) != 0);   // the node is j + 1
-        gc.q[q] += ((ia ? g.u : u0) * (jb ? g.w : w0)) * lam;
+    if (__builtin_expect(g.i != gc.c || g.j != gc.r, 0)) {
+        cache_flush<A>(kp, cols, W, gc);
+        gc.c = g.i;
+        gc.r = g.j;
     }
+    const double lu = lam * g.u;
+    gc.s += lam;
+    gc.su += lu;
+    gc.sw = __builtin_fma(lam, g.w, gc.sw);
+    gc.suw = __builtin_fma(lu, g.w, gc.suw);
 }
 
 // Column `col` of the window (slot s) to global memory; lane r takes row base + r.
