@@ -40,8 +40,6 @@ __device__ __forceinline__ Vec3 vaxpy(Vec3 x, double s, Vec3 y) {
 __device__ __forceinline__ Vec3 vscale(double s, Vec3 y) { return {s * y.a, s * y.b, s * y.c}; }
 __device__ __forceinline__ double vdot(Vec3 x, Vec3 y) { return x.a * y.a + x.b * y.b + x.c * y.c; }
 
-// q/m E at point q: the trace kernel's edge_field for one patch whose origin and spacing are
-// the mesh's, then the radial projection (bt_trace.hip, same operations).
 // The two edge-field buffers (uniform).
 struct AdjFields {
     Rsrc fx, fy;
@@ -58,8 +56,80 @@ __device__ __forceinline__ KAdj cold(KAdj k) {
     return k;
 }
 
-template <int GEOM>
-__device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F, double qm, Vec3 q) {
+// The bilinear gather of the staggered grid f (rows of `cols`) at (s, t) with its partial
+// derivatives in s and t and its cell and weights (the same clamped cell as the forward).
+struct Gathered {
+    double value, ds, dt;
+    unsigned c;
+    int i, j;
+    double u, w;
+};
+
+// The first half of gather_d: the cell, its weights and the loads of its two row pairs
+// (issued here; their values are first used by gather_finish).
+struct GatherLoads {
+    Gathered g;
+    Pair lo, hi;
+};
+
+__device__ __forceinline__ GatherLoads gather_issue(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
+    GatherLoads L;
+    Gathered& g = L.g;
+    g.i = clamped_floor_index<true>(s, imax);
+    g.j = clamped_floor_index<true>(t, jmax);
+    g.u = s - (double)g.i;
+    g.w = t - (double)g.j;
+    g.c = mad24((unsigned)g.i, (unsigned)cols, (unsigned)g.j);
+    L.lo = load_pair(f, g.c);
+    L.hi = load_pair(f, g.c + (unsigned)cols);
+    return L;
+}
+
+__device__ __forceinline__ Gathered gather_finish(const GatherLoads& L) {
+    Gathered g = L.g;
+    const double f00 = L.lo.a, f01 = L.lo.b, f10 = L.hi.a, f11 = L.hi.b;
+    // (clamped_bilinear_buf's arithmetic)
+    const double low = __builtin_fma(g.w, f01 - f00, f00);
+    const double high = __builtin_fma(g.w, f11 - f10, f10);
+    g.value = __builtin_fma(g.u, high - low, low);
+    g.ds = high - low;
+    g.dt = (1.0 - g.u) * (f01 - f00) + g.u * (f11 - f10);
+    return g;
+}
+
+__device__ __forceinline__ Gathered gather_d(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
+    return gather_finish(gather_issue(f, cols, s, t, imax, jmax));
+}
+
+// The start state of the next backward step, read from the segment scratch while the step
+// before it runs: `src` is its first component (the others `lanes` apart).
+// Its loads are issued after the last field loads of the current step, so that no wait for
+// a field value also waits for them (vector loads complete in the order they are issued).
+struct StatePrefetch {
+    const double* src;
+    i64 lanes;
+    Vec3 x, v;
+};
+
+__device__ __forceinline__ void prefetch_state(StatePrefetch& pf) {
+    const double* s = pf.src;
+    const i64 n = pf.lanes;
+    pf.x = {s[0], s[n], s[2 * n]};
+    pf.v = {s[3 * n], s[4 * n], s[5 * n]};
+}
+
+// No memory operation between a field gather's loads and their use.
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// q/m E at point q: the trace kernel's edge_field for one patch whose origin and spacing are
+// the mesh's, then the radial projection (bt_trace.hip, same operations).  `hook` issues
+// memory operations right after the two field loads (an empty hook: none), where no wait for
+// the fields waits for them.
+template <int GEOM, typename Hook = NoHook>
+__device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F, double qm, Vec3 q,
+                                      Hook hook = Hook()) {
     double cy = 0.0, cz = 0.0, qy = q.b;
     if (GEOM == 1) {
         const Radial rd = radial(q.b, q.c);
@@ -71,37 +141,18 @@ __device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F
     }
     const double sx = (q.a - p.origin_x) * p.inv_h;
     const double sy = (qy - p.origin_y) * p.inv_h;
-    const double ex = clamped_bilinear_buf<true>(F.fx, 0u, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
-    const double ey = clamped_bilinear_buf<true>(F.fy, 0u, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    // (gather_issue / gather_finish: clamped_bilinear_buf's operations)
+    const GatherLoads lx = gather_issue(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const GatherLoads ly = gather_issue(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    if (!__is_same(Hook, NoHook)) {
+        __builtin_amdgcn_sched_barrier(0);
+        hook();
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const double ex = gather_finish(lx).value;
+    const double ey = gather_finish(ly).value;
     if (GEOM != 0) return {qm * ex, (qm * ey) * cy, (qm * ey) * cz};
     return {qm * ex, qm * ey, qm * 0.0};
-}
-
-// The bilinear gather of the staggered grid f (rows of `cols`) at (s, t) with its partial
-// derivatives in s and t and its cell and weights (the same clamped cell as the forward).
-struct Gathered {
-    double value, ds, dt;
-    unsigned c;
-    int i, j;
-    double u, w;
-};
-
-__device__ __forceinline__ Gathered gather_d(Rsrc f, i64 cols, double s, double t, int imax, int jmax) {
-    Gathered g;
-    g.i = clamped_floor_index<true>(s, imax);
-    g.j = clamped_floor_index<true>(t, jmax);
-    g.u = s - (double)g.i;
-    g.w = t - (double)g.j;
-    g.c = mad24((unsigned)g.i, (unsigned)cols, (unsigned)g.j);
-    const Pair lo = load_pair(f, g.c), hi = load_pair(f, g.c + (unsigned)cols);
-    const double f00 = lo.a, f01 = lo.b, f10 = hi.a, f11 = hi.b;
-    // (clamped_bilinear_buf's arithmetic)
-    const double low = __builtin_fma(g.w, f01 - f00, f00);
-    const double high = __builtin_fma(g.w, f11 - f10, f10);
-    g.value = __builtin_fma(g.u, high - low, low);
-    g.ds = high - low;
-    g.dt = (1.0 - g.u) * (f01 - f00) + g.u * (f11 - f10);
-    return g;
 }
 
 // ---- the wave's gradient window ------------------------------------------------------
@@ -275,11 +326,13 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, KAdj kp, G
 }
 
 // The transposed field evaluation: given lambda_g for g = q/m E(q), scatter into dJ/dfx,
-// dJ/dfy and return lambda_q; `g` receives g itself (accel's value, bit for bit).
+// dJ/dfy and return lambda_q; `g` receives g itself (accel's value, bit for bit).  With `pf`
+// (the last evaluation of a backward step), the next step's start state is requested right
+// after this evaluation's field loads.
 template <int GEOM>
 __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
                                               GradCache& fx_sums, GradCache& fy_sums, double qm, Vec3 q,
-                                              Vec3 lg, Vec3& g) {
+                                              Vec3 lg, Vec3& g, StatePrefetch* pf = nullptr) {
     const KAdj kp = adjoint_params();
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0, ir = 0.0;
     if (GEOM == 1) {
@@ -292,8 +345,16 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
     }
     const double sx = (q.a - p.origin_x) * p.inv_h;
     const double sy = (qy - p.origin_y) * p.inv_h;
-    const Gathered gx = gather_d(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
-    const Gathered gy = gather_d(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    const GatherLoads lx = gather_issue(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
+    const GatherLoads ly = gather_issue(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
+    if (pf != nullptr) {
+        // (the barriers keep the scheduler from moving these loads above the field loads)
+        __builtin_amdgcn_sched_barrier(0);
+        prefetch_state(*pf);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const Gathered gx = gather_finish(lx);
+    const Gathered gy = gather_finish(ly);
     if (GEOM != 0) g = {qm * gx.value, (qm * gy.value) * cy, (qm * gy.value) * cz};
     else g = {qm * gx.value, qm * gy.value, qm * 0.0};
     const double lex = qm * lg.a;
@@ -339,10 +400,11 @@ __device__ __forceinline__ double step_size(const AdjointParams& p, Vec3 v, Vec3
     return r.dt;
 }
 
-// One forward step (the trace kernel's arithmetic).
-template <int INTEG, int GEOM>
+// One forward step (the trace kernel's arithmetic); `hook` runs after the loads of the step's
+// last field gather (accel's hook).
+template <int INTEG, int GEOM, typename Hook = NoHook>
 __device__ __forceinline__ void forward_step(const AdjointParams& p, const AdjFields& F, double qm, Vec3& x,
-                                             Vec3& v) {
+                                             Vec3& v, Hook hook = Hook()) {
     const Vec3 a = accel<GEOM>(p, F, qm, x);
     int branch;
     double speed, acc;
@@ -358,14 +420,14 @@ __device__ __forceinline__ void forward_step(const AdjointParams& p, const AdjFi
         g = accel<GEOM>(p, F, qm, q);
         sx = vaxpy(sx, 2.0, k); sv = vaxpy(sv, 2.0, g);
         q = vaxpy(x, dt, k); k = vaxpy(v, dt, g);
-        g = accel<GEOM>(p, F, qm, q);
+        g = accel<GEOM>(p, F, qm, q, hook);
         sx = vadd(sx, k); sv = vadd(sv, g);
         x = vaxpy(x, dt6, sx);
         v = vaxpy(v, dt6, sv);
     } else {
         const Vec3 k = vaxpy(v, hdt, a);
         const Vec3 q = vaxpy(x, dt, k);
-        const Vec3 g = accel<GEOM>(p, F, qm, q);
+        const Vec3 g = accel<GEOM>(p, F, qm, q, hook);
         x = q;
         v = vaxpy(k, hdt, g);
     }
@@ -423,7 +485,8 @@ __device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 be
 template <int INTEG, int GEOM>
 __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
                                              GradCache& fx_sums, GradCache& fy_sums, i64 k,
-                                             i64 m, bool seeded, double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv) {
+                                             i64 m, bool seeded, double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv,
+                                             StatePrefetch& pf) {
     const Vec3 a = accel<GEOM>(p, F, qm, x);
     int branch;
     double speed, acc;
@@ -433,25 +496,28 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
     double Ldt = 0.0, Lhdt = 0.0;
     if (INTEG == 0) {
         const double dt6 = dt * (1.0 / 6.0);
-        // (the stage fields are not kept: the transposed evaluations below return them)
+        // The stages' points and the first stage's field; the fields at q2 and q3 come from their
+        // transposed evaluations below (bit for bit accel's), and the terms that need them are
+        // added once they are known.  s12 = v + 2 k1 + 2 k2 and a12 = a + 2 g1 are the stage
+        // sums before k3 and g2, g3 (the same operations as the full sums).
         const Vec3 q1 = vaxpy(x, hdt, v), k1 = vaxpy(v, hdt, a);
-        Vec3 g = accel<GEOM>(p, F, qm, q1);
-        Vec3 sv = vaxpy(a, 2.0, g);
-        const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g);
-        g = accel<GEOM>(p, F, qm, q2);
-        sv = vaxpy(sv, 2.0, g);  // (a + 2 g1 + 2 g2: g3 is added below)
-        const Vec3 q3 = vaxpy(x, dt, k2), k3 = vaxpy(v, dt, g);
-        const Vec3 sx = vadd(vaxpy(vaxpy(v, 2.0, k1), 2.0, k2), k3);
+        const Vec3 g1 = accel<GEOM>(p, F, qm, q1);
+        const Vec3 q2 = vaxpy(x, hdt, k1), k2 = vaxpy(v, hdt, g1);
+        const Vec3 q3 = vaxpy(x, dt, k2);
+        const Vec3 s12 = vaxpy(vaxpy(v, 2.0, k1), 2.0, k2);
+        const Vec3 a12 = vaxpy(a, 2.0, g1);
         Vec3 xs = {0.0, 0.0, 0.0}, vs = {0.0, 0.0, 0.0};
-        if (__builtin_expect(seeded, 0)) {  // the crossing's seeds need the step's end, hence g3 now
-            const Vec3 g3 = accel<GEOM>(p, F, qm, q3);
+        if (__builtin_expect(seeded, 0)) {  // the crossing's seeds need the step's end first
+            const Vec3 g2 = accel<GEOM>(p, F, qm, q2), g3 = accel<GEOM>(p, F, qm, q3);
+            const Vec3 sx = vadd(s12, vaxpy(v, dt, g2)), sv = vaxpy(a12, 2.0, g2);
             const Vec3 y = vaxpy(x, dt6, sx), w = vaxpy(v, dt6, vadd(sv, g3));
             crossing_seeds(p, k, m, x, v, y, w, xs, vs, lx, lv);
         }
         Lx = lx;
         Lv = lv;
-        // the step's derivative in dt6, lx.sx + lv.sv; lv.g3 is added once g3 is known
-        double Ldt6 = vdot(lx, sx) + vdot(lv, sv);
+        // the step's derivative in dt6, lx.sx + lv.sv: lx.k3 + 2 lv.g2 and lv.g3 are added once
+        // g2 and g3 are known
+        double Ldt6 = vdot(lx, s12) + vdot(lv, a12);
         const Vec3 Lsx = vscale(dt6, lx), Lsv = vscale(dt6, lv);
         Lv = vadd(Lv, Lsx);
         Vec3 Lk1 = vscale(2.0, Lsx), Lk2 = vscale(2.0, Lsx);
@@ -459,16 +525,18 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         La = Lsv;
         Vec3 Lg1 = vscale(2.0, Lsv), Lg2 = vscale(2.0, Lsv);
         const Vec3 Lg3 = Lsv;
+        Vec3 g;
         const Vec3 Lq3 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q3, Lg3, g);
-        Ldt6 += vdot(lv, g);  // g3, from the transposed evaluation
+        Ldt6 += vdot(lv, g);  // g3
         Lv = vadd(Lv, Lk3); Lg2 = vaxpy(Lg2, dt, Lk3);
         Lx = vadd(Lx, Lq3); Ldt += vdot(Lq3, k2); Lk2 = vaxpy(Lk2, dt, Lq3);
         const Vec3 Lq2 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q2, Lg2, g);
+        Ldt6 += vdot(lx, vaxpy(v, dt, g)) + 2.0 * vdot(lv, g);  // k3 = v + dt g2, and g2
         Ldt += vdot(Lk3, g);  // g2
         Lv = vadd(Lv, Lk2); Lg1 = vaxpy(Lg1, hdt, Lk2);
         Lx = vadd(Lx, Lq2); Lhdt += vdot(Lq2, k1); Lk1 = vaxpy(Lk1, hdt, Lq2);
         const Vec3 Lq1 = accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, q1, Lg1, g);
-        Lhdt += vdot(Lk2, g);  // g1
+        Lhdt += vdot(Lk2, g1);
         Lv = vadd(Lv, Lk1); Lhdt += vdot(Lk1, a); La = vaxpy(La, hdt, Lk1);
         Lx = vadd(Lx, Lq1); Lhdt += vdot(Lq1, v); Lv = vaxpy(Lv, hdt, Lq1);
         Ldt += Ldt6 * (1.0 / 6.0);
@@ -504,7 +572,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         La = vaxpy(La, -0.5 * Ldt * dt / (acc * acc), a);
     }
     Vec3 ga;
-    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, x, La, ga));
+    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, x, La, ga, &pf));
     lx = Lx;
     lv = Lv;
 }
@@ -529,8 +597,9 @@ __device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, int& ci,
     double qy = x.b;
     if (GEOM == 1) qy = radial(x.b, x.c).r;
     else if (GEOM == 2) qy = p.origin_y + fabs(x.b - p.origin_y);
-    ci = clamped_floor((x.a - p.origin_x) * p.inv_h, 0, (int)p.nx - 1);
-    cj = clamped_floor((qy - p.origin_y) * p.inv_h, 0, (int)p.ny - 1);
+    // (floor clamped to [0, n - 1] as the saturated conversion and one median: the same cell)
+    ci = clamped_floor_index<true>((x.a - p.origin_x) * p.inv_h, (int)p.nx - 1);
+    cj = clamped_floor_index<true>((qy - p.origin_y) * p.inv_h, (int)p.ny - 1);
 }
 
 template <int INTEG, int GEOM>
@@ -613,23 +682,41 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                 } else {
                     get_state(ck, p.lanes, c, gl, x, v);
                 }
-                for (i64 m = m0; m < m1; ++m) {
-                    put_state(seg, p.lanes, m - m0, gl, x, v);
-                    if (m + 1 < m1) forward_step<INTEG, GEOM>(p, F, qm, x, v);
+                // each step's start state goes to the scratch once the step's last field loads
+                // are issued (stores count with the loads: a store issued before a gather would
+                // hold up the wait for its fields); the segment's last state stays in x, v, where
+                // the backward walk starts
+                const i64 lanes = p.lanes;
+                for (i64 m = m0; m + 1 < m1; ++m) {
+                    double* const dst = seg + (m - m0) * 6 * lanes + gl;
+                    const Vec3 xs = x, vs = v;
+                    forward_step<INTEG, GEOM>(p, F, qm, x, v, [=] {
+                        dst[0] = xs.a; dst[lanes] = xs.b; dst[2 * lanes] = xs.c;
+                        dst[3 * lanes] = vs.a; dst[4 * lanes] = vs.b; dst[5 * lanes] = vs.c;
+                    });
                 }
             }
+            // x, v hold the start state of step m (the segment's last, then each step's
+            // prefetched predecessor)
             for (i64 r = K - 1; r >= 0; --r) {
                 const i64 m = m0 + r;
                 const bool act = has && m < m1;
+                // (the step works on copies taken here, where the wait for the prefetch falls;
+                // the prefetch then loads into x, v themselves, with no copy behind it)
+                const Vec3 xc = x, vc = v;
                 int ci = 0, cj = 0;
-                if (act) {
-                    get_state(seg, p.lanes, m - m0, gl, x, v);
-                    cell_of<GEOM>(p, x, ci, cj);
-                }
+                if (act) cell_of<GEOM>(p, xc, ci, cj);
                 window_follow(p, adjoint_params(), W, act, ci, cj);
                 if (act) {
                     const bool seeded = m + 1 == next;
-                    reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, x, v, lx, lv);
+                    // the prefetch always loads (slot 0 again at r = 0, unused): a load under a
+                    // branch would make the following waits count it as outstanding
+                    StatePrefetch pf;
+                    pf.src = seg + (r > 0 ? r - 1 : 0) * 6 * p.lanes + gl;
+                    pf.lanes = p.lanes;
+                    reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, xc, vc, lx, lv, pf);
+                    x = pf.x;
+                    v = pf.v;
                     if (seeded) next = seeded_step(p, k, m + 1);
                 }
             }

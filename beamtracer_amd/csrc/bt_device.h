@@ -248,18 +248,38 @@ __device__ __forceinline__ unsigned mad24(unsigned a, unsigned b, unsigned c) {
 // double), the three interpolations fused (fma: one rounding each instead of two).  Rows and
 // columns stay below 2^24 (the entry points check it).  UNIFORM: the bounds are the same in
 // every lane.
+// The gather in two halves: bilinear_issue finds the cell and its fractions and issues the
+// two row-pair loads; bilinear_value interpolates once they are in (first use of the loaded
+// values), so independent work can run between the two.
+struct Bilinear {
+    Pair lo, hi;
+    double u, w;
+};
+
+template <bool UNIFORM = false>
+__device__ __forceinline__ Bilinear bilinear_issue(Rsrc r, unsigned base, i64 cols, double s, double t, int imax,
+                                                   int jmax) {
+    Bilinear b;
+    const int i = clamped_floor_index<UNIFORM>(s, imax);
+    const int j = clamped_floor_index<UNIFORM>(t, jmax);
+    b.u = s - (double)i;
+    b.w = t - (double)j;
+    const unsigned c = mad24((unsigned)i, (unsigned)cols, base + (unsigned)j);
+    b.lo = load_pair(r, c);
+    b.hi = load_pair(r, c + (unsigned)cols);
+    return b;
+}
+
+__device__ __forceinline__ double bilinear_value(const Bilinear& b) {
+    const double low = __builtin_fma(b.w, b.lo.b - b.lo.a, b.lo.a);
+    const double high = __builtin_fma(b.w, b.hi.b - b.hi.a, b.hi.a);
+    return __builtin_fma(b.u, high - low, low);
+}
+
 template <bool UNIFORM = false>
 __device__ __forceinline__ double clamped_bilinear_buf(Rsrc r, unsigned base, i64 cols, double s, double t,
                                                        int imax, int jmax) {
-    const int i = clamped_floor_index<UNIFORM>(s, imax);
-    const int j = clamped_floor_index<UNIFORM>(t, jmax);
-    const double u = s - (double)i;
-    const double w = t - (double)j;
-    const unsigned c = mad24((unsigned)i, (unsigned)cols, base + (unsigned)j);
-    const Pair lo = load_pair(r, c), hi = load_pair(r, c + (unsigned)cols);
-    const double low = __builtin_fma(w, lo.b - lo.a, lo.a);
-    const double high = __builtin_fma(w, hi.b - hi.a, hi.a);
-    return __builtin_fma(u, high - low, low);
+    return bilinear_value(bilinear_issue<UNIFORM>(r, base, cols, s, t, imax, jmax));
 }
 
 // ---- fixed point ---------------------------------------------------------------------

@@ -73,6 +73,10 @@ struct LaneState {
     int cci, ccj;
     double cq[4];
     i64 ck_next, ck_index;  // the step count of the next checkpoint and its slot
+    // RK4 on one patch: the first-stage gathers of the next step at the current point (E_x, E_y),
+    // issued by the last step's end so that their loads run under its deposit (valid with
+    // `point`)
+    Bilinear pgx, pgy;
 };
 
 constexpr int kNoCell = -(1 << 30);
@@ -191,11 +195,14 @@ __device__ __forceinline__ KParams cold(KParams k) {
 // the mesh coordinates (pyccel.py:196-269).
 // With `known` (one patch that is the mesh), qy, cy, cz, sx and sy are the values the end
 // test of the last step computed for this same point: the same operations on the same
-// operands, so the same bits.  Otherwise they are computed and returned.
+// operands, so the same bits.  Otherwise they are computed and returned.  With `pre` as
+// well, the two gathers were issued then too (the same operations: `pgx`, `pgy`).
 template <int GEOM, bool GENERAL>
 __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F, double q0, double q1, double q2,
                                            double& ex, double& ey, double& cy, double& cz,
-                                           double& lq, double& qy, bool known, double& sx, double& sy) {
+                                           double& lq, double& qy, bool known, double& sx, double& sy,
+                                           bool pre = false, const Bilinear* pgx = nullptr,
+                                           const Bilinear* pgy = nullptr) {
     if (!known) {
         cy = 0.0; cz = 0.0; qy = q1;
         if (GEOM == 1) {
@@ -219,9 +226,24 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
             sy = (qy - v.oy) * p.inv_h;
         }
     }
+    if (!GENERAL && pre) {
+        ex = bilinear_value(*pgx);
+        ey = bilinear_value(*pgy);
+        return;
+    }
     ex = clamped_bilinear_buf<!GENERAL>(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1,
                                         (int)v.ny - 2);
     ey = clamped_bilinear_buf<!GENERAL>(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
+                              (int)v.ny - 1);
+}
+
+// The first-stage gathers of the next step at the end point, with its cell coordinates (sx, sy)
+// on the one patch that is the mesh (edge_field's gathers, issued now: RK4 only).
+__device__ __forceinline__ void issue_first_stage(const TraceParams& p, const Fields& F, double sx, double sy,
+                                                  Bilinear& gx, Bilinear& gy) {
+    const PatchView v = patch_view<false>(p, p.n_patches - 1);
+    gx = bilinear_issue<true>(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
+    gy = bilinear_issue<true>(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
                               (int)v.ny - 1);
 }
 
@@ -511,7 +533,8 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
             double ex, ey, cy, cz, lq, qy, psx, psy;
             const bool known = !GENERAL && stage == 0 && s.point != 0;
             if (known) { qy = s.pt_y; cy = s.pt_cy; cz = s.pt_cz; psx = s.pt_sx; psy = s.pt_sy; }
-            edge_field<GEOM, GENERAL>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy);
+            edge_field<GEOM, GENERAL>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy,
+                                      INTEG == 0 && known, &s.pgx, &s.pgy);
             if (INTEG != 0 && stage == 1) { end_y = qy; end_cy = cy; end_cz = cz; end_sx = psx; end_sy = psy; }
             if (GEOM != 0) {
                 const double qe = qmk * ey;
@@ -618,6 +641,9 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
     const int ei = clamped_floor_index<true>(ci, p.cell_imax);
     const int ej = clamped_floor_index<true>(cj, p.cell_jmax);
     const i64 cell = p.cells[mad24((unsigned)ei, (unsigned)p.cells_cols, (unsigned)p.cells_base + (unsigned)ej)];
+    // RK4 on one patch: the next step's first-stage gathers at the end point, issued before the
+    // deposit so that their loads run under it (used if the trajectory goes on: `point`)
+    if (INTEG == 0 && !GENERAL) issue_first_stage(p, F, ci, cj, s.pgx, s.pgy);
 
     // space charge: dq dt at the Hermite midpoint, area-weighted (pyccel.py:372-423)
     const double gw = 1.0 / lorentz<GENERAL>(w0, w1, w2, inv_c2);
