@@ -229,6 +229,10 @@ __device__ __forceinline__ void add_node(KAdj kp, i64 cols, const GradWindow& W,
 // one cell.
 constexpr int kNoCell = -(1 << 30);
 
+// Polls of a group's finished-piece count before a piece gives up waiting (each a round trip
+// to the coherent level and a short sleep: far longer than any piece takes).
+constexpr int kHandoffSpins = 1 << 19;
+
 struct GradCache {
     int c, r;  // the cell (kNoCell: none)
     double s, su, sw, suw;
@@ -618,62 +622,128 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
         adjoint_windows[W.wave][0][t] = 0.0;
         adjoint_windows[W.wave][1][t] = 0.0;
     }
-    const i64 gl = (i64)blockIdx.x * blockDim.x + threadIdx.x;  // this lane's scratch column
+    // (per-lane indices and step counts are 32-bit: the entry point bounds the trajectories,
+    // the lanes and the steps below 2^31, and the loop keeps fewer registers)
+    const int gl = (int)(blockIdx.x * blockDim.x + threadIdx.x);  // this lane's scratch column
     double* ck = p.scratch;
     double* seg = p.scratch + (i64)p.n_ckpt * 6 * p.lanes;
     const i64 K = p.interval;
+    const int P = p.pieces;
     bool exhausted = false;
     for (;;) {
         const KAdj c = cold(adjoint_params());
-        i64 k = -1;
-        const bool want = !exhausted;
-        const u64 need = __ballot(want);
-        if (need != 0) {
-            const int count = __popcll(need);
-            const int leader = __ffsll((long long)need) - 1;
-            u64 base = 0;
-            if ((int)lane == leader) base = atomicAdd(c->queue, (u64)count);
-            base = __shfl(base, leader);
-            if (want) {
-                const u64 mm = base + (u64)__popcll(need & below);
-                if (mm < (u64)c->n_active) {
-                    const i64 kk = c->active[mm];
-                    if (kk >= 0 && kk < c->n) k = kk;
-                } else {
-                    exhausted = true;
+        int k = -1;
+        int piece = 0;  // of the task: segments of piece 0 lie above those of piece 1, ...
+        i64 g = 0;      // the task's group (pieces > 1)
+        if (P == 1) {
+            // one task per trajectory: each lane that needs one claims the next queue entry
+            const bool want = !exhausted;
+            const u64 need = __ballot(want);
+            if (need != 0) {
+                const int count = __popcll(need);
+                const int leader = __ffsll((long long)need) - 1;
+                u64 base = 0;
+                if ((int)lane == leader) base = atomicAdd(c->queue, (u64)count);
+                base = __shfl(base, leader);
+                if (want) {
+                    const u64 mm = base + (u64)__popcll(need & below);
+                    if (mm < (u64)c->n_active) {
+                        const i64 kk = c->active[mm];
+                        if (kk >= 0 && kk < c->n) k = (int)kk;
+                    } else {
+                        exhausted = true;
+                    }
                 }
             }
+            if (__ballot(k >= 0 || !exhausted) == 0) break;
+        } else {
+            // one task per piece of a group of 64 consecutive entries, every group's first piece
+            // before any second piece, and so on (wave-uniform)
+            u64 t = 0;
+            if (lane == 0) t = atomicAdd(c->queue, 1ull);
+            t = __shfl(t, 0);
+            if (t >= (u64)P * (u64)c->groups) break;
+            piece = (int)(t / (u64)c->groups);
+            g = (i64)(t - (u64)piece * (u64)c->groups);
+            const i64 mm = g * 64 + (i64)lane;
+            if (mm < c->n_active) {
+                const i64 kk = c->active[mm];
+                if (kk >= 0 && kk < c->n) k = (int)kk;
+            }
         }
-        if (__ballot(k >= 0 || !exhausted) == 0) break;
-        i64 last = k >= 0 ? seeded_step(p, k, 0x7fffffffffffffffll) : 0;
+        const i64 last_seeded = k >= 0 ? seeded_step(p, k, 0x7fffffffffffffffll) : 0;
         // a trajectory with no seed, or seeded beyond the checkpoints, adds nothing; the
-        // latter (a max_step below the seeded steps) is counted for the caller
-        const bool valid = last > 0 && (last - 1) / K < p.n_ckpt;
-        if (__builtin_expect(last > 0 && !valid, 0) && c->dropped != nullptr)
+        // latter (a max_step below the seeded steps) is counted for the caller (once)
+        bool valid = last_seeded > 0 && (last_seeded - 1) / K < p.n_ckpt;
+        int last = valid ? (int)last_seeded : 0;
+        if (__builtin_expect(last_seeded > 0 && !valid, 0) && piece == 0 && c->dropped != nullptr)
             __hip_atomic_fetch_add(c->dropped, (i64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!valid) last = 0;
         const i64 N = c->n;
+        // the forward trace's checkpoints when they reach the last seeded step (pieces need them:
+        // another lane continues the walk; the launch uses pieces only when they do)
+        const bool stored = valid && c->fwd_ckpt != nullptr && (last - 1) / K < c->fwd_n_ckpt;
+        if (__builtin_expect(P > 1 && valid && !stored, 0)) {
+            if (piece == 0 && c->dropped != nullptr)
+                __hip_atomic_fetch_add(c->dropped, (i64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            valid = false;
+        }
+        if (!valid) last = 0;
         const double qm = valid ? c->qm[k] : 0.0;
         Vec3 x = {0.0, 0.0, 0.0}, v = {0.0, 0.0, 0.0};
-        // the forward trace's checkpoints when they reach the last seeded step
-        const bool stored = valid && c->fwd_ckpt != nullptr && (last - 1) / K < c->fwd_n_ckpt;
         if (valid && !stored) {
             x = {c->x[k], c->x[N + k], c->x[2 * N + k]};
             v = {c->v[k], c->v[N + k], c->v[2 * N + k]};
             // 1. forward to the last seeded step, a checkpoint every K steps
-            for (i64 m = 0; m < last; ++m) {
+            for (int m = 0; m < last; ++m) {
                 if (m % K == 0) put_state(ck, p.lanes, m / K, gl, x, v);
                 forward_step<INTEG, GEOM>(p, F, qm, x, v);
             }
         }
-        // 2. the segments backward, in step across the wave (the window moves with them)
+        // 2. the segments backward, in step across the wave (the window moves with them); a
+        // piece takes its share of the group's segments (all of them with one piece)
         Vec3 lx = {0.0, 0.0, 0.0}, lv = {0.0, 0.0, 0.0};
-        i64 next = last;
         const int top = wave_max(valid ? (int)((last - 1) / K) : -1);
-        for (int c = top; c >= 0; --c) {
-            const i64 m0 = (i64)c * K;
+        const int nseg = top + 1;
+        const int c_hi = top - (int)(((i64)piece * nseg) / P);
+        const int c_lo = top + 1 - (int)(((i64)(piece + 1) * nseg) / P);
+        // the largest seeded record step within or below this piece's segments
+        int next = piece == 0 ? last : (valid ? (int)seeded_step(p, k, (i64)(c_hi + 1) * K + 1) : 0);
+        if (piece > 0) {
+            // the adjoint state at the end of this piece, from the group's piece above: wait
+            // (bounded) until that piece is done, then read it through the coherent path
+            const KAdj h = cold(adjoint_params());
+            int* const flag = h->done + g;
+            int seen = 0;
+            for (int spin = 0; spin < kHandoffSpins; ++spin) {
+                seen = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (seen >= piece) break;
+                __builtin_amdgcn_s_sleep(8);
+            }
+            asm volatile("" ::: "memory");
+            if (__builtin_expect(seen >= piece, 1)) {
+                const i64 HL = h->groups * 64, at = g * 64 + (i64)lane;
+                double* const hd = h->hand;
+                lx = {__hip_atomic_load(hd + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(hd + HL + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(hd + 2 * HL + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+                lv = {__hip_atomic_load(hd + 3 * HL + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(hd + 4 * HL + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(hd + 5 * HL + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+            } else if (valid) {
+                // (never in a working launch) the trajectory's gradient would miss the pieces
+                // above: counted as dropped, and this piece adds nothing
+                if (h->dropped != nullptr)
+                    __hip_atomic_fetch_add(h->dropped, (i64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                valid = false;
+                last = 0;
+                next = 0;
+            }
+        }
+        for (int c = c_hi; c >= c_lo; --c) {
+            const int m0 = c * (int)K;
             const bool has = valid && m0 < last;
-            const i64 m1 = has ? (m0 + K < last ? m0 + K : last) : m0;
+            const int m1 = has ? (m0 + (int)K < last ? m0 + (int)K : last) : m0;
             if (has) {
                 if (stored) {
                     const double* cp = cold(adjoint_params())->fwd_ckpt + (i64)c * 6 * N + k;
@@ -687,7 +757,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                 // hold up the wait for its fields); the segment's last state stays in x, v, where
                 // the backward walk starts
                 const i64 lanes = p.lanes;
-                for (i64 m = m0; m + 1 < m1; ++m) {
+                for (int m = m0; m + 1 < m1; ++m) {
                     double* const dst = seg + (m - m0) * 6 * lanes + gl;
                     const Vec3 xs = x, vs = v;
                     forward_step<INTEG, GEOM>(p, F, qm, x, v, [=] {
@@ -699,7 +769,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
             // x, v hold the start state of step m (the segment's last, then each step's
             // prefetched predecessor)
             for (i64 r = K - 1; r >= 0; --r) {
-                const i64 m = m0 + r;
+                const int m = m0 + (int)r;
                 const bool act = has && m < m1;
                 // (the step works on copies taken here, where the wait for the prefetch falls;
                 // the prefetch then loads into x, v themselves, with no copy behind it)
@@ -717,12 +787,28 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                     reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, xc, vc, lx, lv, pf);
                     x = pf.x;
                     v = pf.v;
-                    if (seeded) next = seeded_step(p, k, m + 1);
+                    if (seeded) next = (int)seeded_step(p, k, m + 1);
                 }
             }
         }
         cache_flush<0>(adjoint_params(), p.ny, W, fx_sums);
         cache_flush<1>(adjoint_params(), p.ny + 1, W, fy_sums);
+        if (piece + 1 < P) {
+            // hand the adjoint state to the group's next piece: written through to the coherent
+            // level, complete, and only then counted in done[g]
+            const KAdj h = cold(adjoint_params());
+            const i64 HL = h->groups * 64, at = g * 64 + (i64)lane;
+            double* const hd = h->hand;
+            __hip_atomic_store(hd + at, lx.a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(hd + HL + at, lx.b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(hd + 2 * HL + at, lx.c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(hd + 3 * HL + at, lv.a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(hd + 4 * HL + at, lv.b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(hd + 5 * HL + at, lv.c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (lane == 0) __hip_atomic_fetch_add(h->done + g, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
         {
             const KAdj e = cold(adjoint_params());
             double* const dx = e->dx;
@@ -738,8 +824,12 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
     window_flush_all(adjoint_params(), W);
 }
 
-__global__ void adjoint_prepare_kernel(u64* __restrict__ queue) {
+// The queue head and the groups' finished-piece counts to 0 (a kernel: no memset node in a
+// captured graph).
+__global__ void adjoint_prepare_kernel(u64* __restrict__ queue, int* __restrict__ done, i64 groups) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0ull;
+    const i64 stride = (i64)gridDim.x * blockDim.x;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; done != nullptr && t < groups; t += stride) done[t] = 0;
 }
 
 typedef void (*AdjointKernel)(AdjointParams);
@@ -765,7 +855,10 @@ int adjoint_occupancy(int integ, int geom, int block) {
 int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom) {
     AdjointKernel kern = select_adjoint_kernel(integ, geom);
     if (kern == nullptr) return -1;
-    hipLaunchKernelGGL(adjoint_prepare_kernel, dim3(1), dim3(64), 0, stream, p.queue);
+    int pgrid = p.pieces > 1 ? (int)((p.groups + 255) / 256) : 1;
+    if (pgrid > 1024) pgrid = 1024;
+    hipLaunchKernelGGL(adjoint_prepare_kernel, dim3(pgrid), dim3(256), 0, stream, p.queue,
+                       p.pieces > 1 ? p.done : nullptr, p.groups);
     if (p.n_active > 0 && grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, p);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }

@@ -442,9 +442,23 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
     return BT_OK;
 }
 
-// The grid, checkpoint slots and scratch bytes of an adjoint call.
+// The pieces each group of 64 trajectories' backward walk is split into, when the groups
+// outnumber the waves the grid holds at once: whole groups would leave the last round of
+// waves partly filled (2e5 trajectories on 2048 waves: a second round of 1077 waves that
+// costs nearly a full one); pieces of 1/P of a walk make the last round's share 1/P as large.
+// Eight pieces (fewer when the walks have fewer segments) measured best on the einzel lens's
+// 24-segment walks: 2 pieces 22.0 ms, 4 20.9, 8 19.2, 12 19.2, 16 19.5 (uneven pieces of
+// one and two segments), one piece 24.5.
+int adjoint_pieces(i64 groups, i64 waves, i64 segments) {
+    if (waves < 1 || groups <= waves || segments < 2) return 1;
+    return (int)(segments < 8 ? segments : 8);
+}
+
+// The grid, checkpoint slots, groups and scratch bytes of an adjoint call: the queue head,
+// the per-lane checkpoint and segment states, and (when the groups outnumber the resident
+// waves) the groups' finished-piece counts and the adjoint states handed between pieces.
 size_t adjoint_shape(i64 n_active, i64 max_step, i64 interval, int integ, int geom, int block, int cus, int* grid_out,
-                     i64* n_ckpt_out) {
+                     i64* n_ckpt_out, i64* groups_out = nullptr, size_t* hand_at = nullptr) {
     int grid = 0;
     if (n_active > 0) {
         int per_cu = bt::adjoint_occupancy(integ, geom, block);
@@ -458,7 +472,17 @@ size_t adjoint_shape(i64 n_active, i64 max_step, i64 interval, int integ, int ge
     const i64 n_ckpt = max_step > 0 ? (max_step - 1) / interval + 1 : 1;
     *grid_out = grid;
     *n_ckpt_out = n_ckpt;
-    return 256 + (size_t)(n_ckpt + interval) * 6 * 8 * (size_t)(lanes > 0 ? lanes : 1);
+    size_t bytes = 256 + (size_t)(n_ckpt + interval) * 6 * 8 * (size_t)(lanes > 0 ? lanes : 1);
+    const i64 groups = (n_active + 63) / 64;
+    if (groups_out != nullptr) *groups_out = groups;
+    if (groups > lanes / 64) {
+        bytes = (bytes + 255) / 256 * 256;
+        if (hand_at != nullptr) *hand_at = bytes;  // done[groups], then hand[6][groups * 64]
+        bytes += ((size_t)groups * 4 + 255) / 256 * 256 + (size_t)groups * 64 * 6 * 8;
+    } else if (hand_at != nullptr) {
+        *hand_at = 0;
+    }
+    return bytes;
 }
 
 }  // namespace
@@ -976,6 +1000,8 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     if (a->capacity < 1) return fail(BT_E_ARG, "capacity must be positive");
     if (a->interval < 1) return fail(BT_E_ARG, "interval must be positive");
     if (a->max_step < 0) return fail(BT_E_ARG, "negative max_step");
+    if (a->n >= ((i64)1 << 31) || a->max_step + a->interval >= ((i64)1 << 31))
+        return fail(BT_E_ARG, "2^31 trajectories or steps or more");
     if (a->block_size != 0 && (a->block_size % 64 != 0 || a->block_size < 64 || a->block_size > 256))
         return fail(BT_E_ARG, "block_size must be a multiple of 64 in 64..256");
     if (a->n_active > 0 && (a->active == nullptr || a->x == nullptr || a->v == nullptr || a->qm == nullptr ||
@@ -989,9 +1015,10 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const int block = a->block_size > 0 ? a->block_size : 256;
     int grid = 0;
-    i64 n_ckpt = 1;
+    i64 n_ckpt = 1, groups = 0;
+    size_t hand_at = 0;
     const size_t bytes = adjoint_shape(a->n_active, a->max_step, a->interval, integ, geom, block, s->cus, &grid,
-                                       &n_ckpt);
+                                       &n_ckpt, &groups, &hand_at);
     const i64 lanes = (i64)grid * block;
     void* ws = a->workspace;
     if (ws == nullptr) {
@@ -1018,6 +1045,15 @@ int bt_trace_adjoint_device(const bt_adjoint_args* a, void* stream) {
     p.scratch = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
     p.dfx = a->dfx; p.dfy = a->dfy; p.dx = a->dx; p.dv = a->dv;
     p.dropped = a->dropped;
+    // pieces need the forward trace's checkpoints up to every seeded step (another lane
+    // continues a walk, so no lane's own checkpoints can serve)
+    p.pieces = 1;
+    p.groups = groups;
+    if (hand_at != 0 && p.fwd_ckpt != nullptr && p.fwd_n_ckpt >= n_ckpt) {
+        p.pieces = adjoint_pieces(groups, lanes / 64, n_ckpt);
+        p.done = reinterpret_cast<int*>(static_cast<char*>(ws) + hand_at);
+        p.hand = reinterpret_cast<double*>(static_cast<char*>(ws) + hand_at + ((size_t)groups * 4 + 255) / 256 * 256);
+    }
     if (bt::launch_adjoint(p, block, grid, static_cast<hipStream_t>(stream), integ, geom))
         return fail(BT_E_LAUNCH, std::string("adjoint launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;

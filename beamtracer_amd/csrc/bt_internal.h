@@ -115,6 +115,13 @@ struct AdjointParams {
     double* dfx; double* dfy; double* dx; double* dv;
     u64* queue;
     i64* dropped;  // += seeded trajectories beyond the checkpoints (their gradient is missing), or null
+    // the backward walk of each group of 64 queue entries in `pieces` tasks (1: one task per
+    // group, claimed per lane): the adjoint state passes from one piece to the next through
+    // `hand` [6][groups * 64], `done[g]` counting group g's finished pieces
+    int pieces;
+    i64 groups;
+    int* done;
+    double* hand;
 };
 
 int launch_adjoint(const AdjointParams& p, int block, int grid, hipStream_t stream, int integ, int geom);

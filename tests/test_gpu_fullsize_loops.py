@@ -129,6 +129,44 @@ def test_full_size_adjoint_of_a_sample_transposes_the_cpu_tangent(lens):
         assert adj == pytest.approx(tangent, rel=1e-9)
 
 
+def test_full_size_adjoint_in_pieces_equals_one_walk_per_lane(lens):
+    """2e5 seeded trajectories are 3,125 groups of 64 on the 2,048 waves the adjoint holds at
+    once: from the forward trace's checkpoints each group's backward walk runs in pieces,
+    handed from wave to wave; without them every lane steps its trajectory again from the
+    start and walks it whole.  Both give the same gradients (dJ/dfx, dJ/dfy, dJ/dx, dJ/dv) up
+    to the order of the FP64 sums, and drop nothing."""
+    import torch
+
+    design, tracer, problem, unit, base = lens
+    fx, fy = _fields(unit, torch.tensor(base, device="cuda"))
+    problem.objective(fx, fy)
+    r = problem.record
+    active, max_step = problem.queue()
+    assert active.size > 64 * 2048  # more groups than resident waves: pieces
+    dev = torch.device("cuda")
+    p = design.particles
+    common = (torch.from_numpy(active).to(dev), torch.from_numpy(np.ascontiguousarray(p.position)).to(dev),
+              torch.from_numpy(np.ascontiguousarray(p.velocity)).to(dev),
+              torch.from_numpy(np.ascontiguousarray(p.charge / p.mass)).to(dev), fx, fy,
+              torch.tensor(tracer.planes, dtype=torch.float64, device=dev), torch.from_numpy(r.hits).to(dev),
+              torch.from_numpy(r.counts).to(dev), torch.from_numpy(r.seeds).to(dev), max_step)  # fmt: skip
+    assert r.checkpoints_current(tracer, problem.interval)
+    results = []
+    for stored in (True, False):
+        dfx, dfy = torch.zeros_like(fx), torch.zeros_like(fy)
+        dx, dv = torch.zeros_like(common[1]), torch.zeros_like(common[2])
+        dropped = torch.zeros(1, dtype=torch.int64, device=dev)
+        problem.launch(*common, dfx, dfy, dx=dx, dv=dv, checkpoints=stored, dropped=dropped)
+        torch.cuda.synchronize()
+        assert int(dropped.item()) == 0
+        results.append([a.cpu().numpy() for a in (dfx, dfy, dx, dv)])
+    for a, b in zip(*results):
+        scale = np.abs(b).max()
+        assert scale > 0
+        print(f"pieces vs whole walks: max difference {np.abs(a - b).max() / scale:.3e} of the largest", flush=True)
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-11 * scale)
+
+
 def test_gradients_of_32_candidates_on_streams_equal_single_gradients():
     import torch
 
