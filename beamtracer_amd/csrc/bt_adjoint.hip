@@ -19,6 +19,12 @@
 //      lambda_E times the bilinear weights into dJ/dfx and dJ/dfy, through a window of
 //      node sums in LDS that each wave moves with its lanes (the segments run in step
 //      across the wave).
+// When the groups of 64 trajectories outnumber the waves the grid holds at once, the walk of
+// each group runs as several tasks (pieces) of its segments, upper segments first, from the
+// forward trace's checkpoints: the adjoint state at a piece's lower end is handed to the
+// group's next piece through stores written through to the coherent level and a per-group
+// count of finished pieces (the next piece waits for it, boundedly).  A last round of whole
+// walks would hold few waves on some SIMDs and two on others; pieces make it short.
 // The forward steps here repeat the trace kernel's operations in the same order, so they
 // reproduce its states bit for bit.
 #include <hip/hip_runtime.h>

@@ -319,11 +319,14 @@ int bt_graph_destroy(void* graph);
  * crossings (no counterpart in the reference, which has no gradients): given the forward
  * trace's records (hits, counts) and seeds dJ/d(y, z, vx, vy, vz) of every record, adds
  * dJ/dfx and dJ/dfy of the edge fields (and dJ/dx, dJ/dv of the initial states) for the
- * electrostatic, classical trace on one uniform mesh.  Each trajectory is stepped again
- * from its initial state to its last seeded step with a checkpoint every `interval`
- * steps, then walked backward segment by segment (recomputed from its checkpoint).
- * Device memory, asynchronous on `stream`; the library-owned scratch is
- * (max_step / interval + 1 + interval) * 48 bytes per lane of the resident grid. */
+ * electrostatic, classical trace on one uniform mesh.  Each trajectory is walked backward
+ * segment by segment, each segment recomputed from its checkpoint: the forward trace's
+ * (`checkpoints`), or its own, stepped again from the initial state.  With the forward
+ * trace's checkpoints and more groups of 64 trajectories than the grid holds waves, each
+ * group's walk runs in pieces handed between waves.  Device memory, asynchronous on
+ * `stream`; the scratch is (max_step / interval + 1 + interval) * 48 bytes per lane of the
+ * resident grid, plus 48 bytes per trajectory and 4 per group of 64 for the pieces.
+ * n and max_step + interval stay below 2^31. */
 typedef struct bt_adjoint_args {
     const int64_t* active; int64_t n_active;      /* the trajectories with seeded records  */
     const double* x; const double* v; int64_t n;  /* initial states (3, n)                 */
