@@ -33,6 +33,7 @@ BT_REFILL_BLOCK = 1
 BT_NO_PATHS = 2
 BT_WIDE_WINDOW = 4
 BT_MAGNETIC = 8
+BT_LANE_PAIRS = 16
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32

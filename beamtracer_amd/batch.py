@@ -54,13 +54,16 @@ class LaunchShape:
     a workgroup takes new trajectories together (consecutive queue entries, so its lanes
     mostly share one design and its deposit window) or each lane as it frees, and the
     deposit tile: each wave's window of node sums in LDS spans 16 mesh columns, or 32 with
-    ``wide_window`` (for lanes spread further along x; twice the LDS per wave)."""
+    ``wide_window`` (for lanes spread further along x; twice the LDS per wave); and the lanes
+    per trajectory: one, or two with ``lane_pairs`` (one gathering E_x, the other E_y, each
+    keeping the deposits of two of the cell's nodes)."""
 
     block_size: int = 0
     max_blocks: int = 0
     order: str = "longest"
     refill_block: bool = False
     wide_window: bool = False
+    lane_pairs: bool = False
 
 
 class BatchedTracer:
@@ -328,8 +331,9 @@ class BatchedTracer:
         a.integrator = INTEGRATORS.index(t0.integrator)
         a.step_fraction, a.max_steps, a.refine = t0.step_fraction, t0.max_steps, SUBCELLS
         a.block_size, a.max_blocks = self.shape.block_size, self.shape.max_blocks
-        a.flags = (_lib.BT_REFILL_BLOCK if self.shape.refill_block else 0) | (
-            _lib.BT_WIDE_WINDOW if self.shape.wide_window else 0)
+        a.flags = ((_lib.BT_REFILL_BLOCK if self.shape.refill_block else 0)
+                   | (_lib.BT_WIDE_WINDOW if self.shape.wide_window else 0)
+                   | (_lib.BT_LANE_PAIRS if self.shape.lane_pairs else 0))
         a.b0, a.b1, a.b2 = (None if self._dev.get(f"b{c}") is None else self._dev[f"b{c}"].data_ptr() for c in range(3))
         return a
 
@@ -598,16 +602,17 @@ def vacuum_tracers(designs: Sequence, processes: int = 0) -> list[HipBeamTracer]
 
 
 def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, blocks=(64, 128, 192, 256),
-             fills=(1.0, 0.5), orders=ORDERS, refills=(False, True), windows=(False, True),
+             fills=(1.0, 0.5), orders=ORDERS, refills=(False, True), windows=(False, True), lanes=(1, 2),
              repeats: int = 2) -> dict:  # fmt: skip
     """Time one batched trace per launch shape and keep the fastest in ``tracer.shape``.
 
     The candidates are threads per block, the fraction of the resident blocks the grid
-    fills, the queue order, the refill mode (lane by lane, or a workgroup together) and the
-    deposit tile (windows of 16 or 32 columns); each is timed (the launch between two
-    synchronizations of the null stream, best of ``repeats``) on the real batch. Every trajectory takes one lane: a
-    trajectory's steps are a chain of dependent field gathers, which more lanes would
-    duplicate rather than share. Returns every timing in ms and the choice.
+    fills, the queue order, the refill mode (lane by lane, or a workgroup together), the
+    deposit tile (windows of 16 or 32 columns) and the lanes per trajectory (one, or a pair
+    that splits the two field gathers and the four deposit nodes: fewer instructions per lane
+    for twice the lanes, which pays only where the GPU holds few waves per SIMD); each is
+    timed (the launch between two synchronizations of the null stream, best of
+    ``repeats``) on the real batch. Returns every timing in ms and the choice.
     """
     host = tracer._prepare(particles)
     tracer._ensure_plan(host["exponents"])
@@ -620,13 +625,14 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
     for order in orders:
         tracer.shape = LaunchShape(0, 0, order)
         dev = tracer._upload(host, tracer.capacity)
-        for block, wide in itertools.product(blocks, windows):
-            flags = (_lib.BT_WIDE_WINDOW if wide else 0) | (_lib.BT_MAGNETIC if t0.magnetic is not None else 0)
+        for block, wide, lpt in itertools.product(blocks, windows, lanes):
+            flags = ((_lib.BT_WIDE_WINDOW if wide else 0) | (_lib.BT_MAGNETIC if t0.magnetic is not None else 0)
+                     | (_lib.BT_LANE_PAIRS if lpt == 2 else 0))
             per_cu = L.bt_batch_occupancy(INTEGRATORS.index(t0.integrator), geometry_code(t0.mesh), block, flags)
             for fill in fills:
                 grid = max(1, int(per_cu * cus * fill))
                 for refill in refills:
-                    tracer.shape = LaunchShape(block, grid, order, refill, wide)
+                    tracer.shape = LaunchShape(block, grid, order, refill, wide, lpt == 2)
                     times = []
                     for _ in range(repeats):
                         tracer._restart(dev)
@@ -636,10 +642,10 @@ def autotune(tracer: BatchedTracer, particles: Sequence[BeamParticles], *, block
                         device.synchronize()
                         times.append((time.perf_counter() - start) * 1e3)
                     key = (f"{order}/block{block}/fill{fill}" + ("/refill-block" if refill else "")
-                           + ("/window32" if wide else ""))  # fmt: skip
+                           + ("/window32" if wide else "") + ("/pairs" if lpt == 2 else ""))  # fmt: skip
                     timings[key] = min(times)
                     if best is None or min(times) < best[0]:
-                        best = (min(times), LaunchShape(block, grid, order, refill, wide))
+                        best = (min(times), LaunchShape(block, grid, order, refill, wide, lpt == 2))
     assert best is not None
     tracer.shape = best[1]
     return {"timings_ms": timings, "choice": vars(best[1]), "best_ms": best[0], "time": time.time()}

@@ -263,20 +263,23 @@ int run_trace(const bt_trace2d_args* a, bt::TraceParams p, void* ws, hipStream_t
     const bool general = a->n_patches > 1 || a->graded != 0 || a->inv_c2 != 0.0 || p.reuse == 0;
     const bool records = (a->flags & BT_NO_PATHS) == 0 || p.ckpt != nullptr;
     const bool wide = (a->flags & BT_WIDE_WINDOW) != 0;
+    // two lanes per trajectory: without records, on one uniform patch (ignored otherwise)
+    const bool pairs = (a->flags & BT_LANE_PAIRS) != 0 && !general && !records;
+    const i64 lpt = pairs ? 2 : 1;
     int block = a->block_size > 0 ? a->block_size : 256;
     if (block % 64 != 0 || block > 256) return fail(BT_E_ARG, "block_size must be a multiple of 64, at most 256");
     int grid = 0;
     if (a->n_active > 0) {
-        int per_cu = bt::trace_occupancy(integ, geom, has_b, general, records, wide, block);
+        int per_cu = bt::trace_occupancy(integ, geom, has_b, general, records, wide, pairs, block);
         if (per_cu < 1) per_cu = 1;
         i64 full = (i64)per_cu * cus;
-        i64 need = (a->n_active + block - 1) / block;
+        i64 need = (a->n_active * lpt + block - 1) / block;
         grid = (int)(need < full ? need : full);
         if (a->max_blocks > 0 && grid > a->max_blocks) grid = a->max_blocks;
         if (grid < 1) grid = 1;
     }
     p.charge = a->charge;  // row 0
-    int rc = bt::launch_trace(p, block, grid, stream, integ, geom, has_b, general, records, wide, nodes);
+    int rc = bt::launch_trace(p, block, grid, stream, integ, geom, has_b, general, records, wide, pairs, nodes);
     if (rc) return fail(BT_E_LAUNCH, std::string("trace launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;
 }
@@ -424,19 +427,20 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
     const int integ = (int)a->integrator, geom = (int)a->cylindrical;
     const int block = a->block_size > 0 ? a->block_size : 256;
     const bool wide = (a->flags & BT_WIDE_WINDOW) != 0;
+    const bool pairs = (a->flags & BT_LANE_PAIRS) != 0;
     const bool has_b = batch_has_b(a);
     int grid = 0;
     if (a->n_active > 0) {
-        int per_cu = bt::batched_occupancy(integ, geom, has_b, wide, block);
+        int per_cu = bt::batched_occupancy(integ, geom, has_b, wide, pairs, block);
         if (per_cu < 1) per_cu = 1;
         const i64 full = (i64)per_cu * plan->cus;
-        const i64 need = (a->n_active + block - 1) / block;
+        const i64 need = (a->n_active * (pairs ? 2 : 1) + block - 1) / block;
         grid = (int)(need < full ? need : full);
         if (a->max_blocks > 0 && grid > a->max_blocks) grid = a->max_blocks;
         if (grid < 1) grid = 1;
     }
     const bool reset = a->x0 != nullptr;
-    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, has_b, wide, plan->total_nodes,
+    if (bt::launch_batched(p, a->x0, a->v0, reset, block, grid, stream, integ, geom, has_b, wide, pairs, plan->total_nodes,
                            plan->max_nodes))
         return fail(BT_E_LAUNCH, std::string("batched launch: ") + hipGetErrorString(hipGetLastError()));
     return BT_OK;
@@ -1131,7 +1135,7 @@ int bt_batch_occupancy(int64_t integrator, int64_t cylindrical, int32_t block_si
     if (integrator < 0 || integrator > 1 || cylindrical < 0 || cylindrical > 2) return fail(BT_E_ARG, "bad variant");
     if (block_size % 64 != 0 || block_size < 64 || block_size > 256) return fail(BT_E_ARG, "bad block size");
     return bt::batched_occupancy((int)integrator, (int)cylindrical, (flags & BT_MAGNETIC) != 0,
-                                 (flags & BT_WIDE_WINDOW) != 0, block_size);
+                                 (flags & BT_WIDE_WINDOW) != 0, (flags & BT_LANE_PAIRS) != 0, block_size);
 }
 
 int bt_batch_trace_device(void* handle, const bt_batch_args* a, void* stream) {

@@ -69,19 +69,21 @@ typedef void (*TraceKernel)(TraceParams);
 
 // `records`: paths or checkpoints may be written (else a variant without that code runs)
 // `wide`: each wave's deposit window spans 32 columns instead of 16 (beams whose lanes spread along x)
-TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide);
+// `pairs`: two lanes per trajectory (without records, on one uniform patch; ignored otherwise).
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide,
+                                bool pairs);
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
-                 bool has_b, bool general, bool records, bool wide, i64 nodes);
-int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, bool wide, int block);
+                 bool has_b, bool general, bool records, bool wide, bool pairs, i64 nodes);
+int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, bool wide, bool pairs, int block);
 
 // Batched form: one persistent launch over the trajectories of every design (single-patch,
 // uniform, classical meshes); `reset` first restores x, v from x0, v0 and clears steps,
 // status, electrode, counts and charge.
-TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide);
+TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide, bool pairs);
 int launch_batched(const TraceParams& p, const double* x0, const double* v0, bool reset, int block, int grid,
-                   hipStream_t stream, int integ, int geom, bool has_b, bool wide, i64 total_nodes,
+                   hipStream_t stream, int integ, int geom, bool has_b, bool wide, bool pairs, i64 total_nodes,
                    i64 max_design_nodes);
-int batched_occupancy(int integ, int geom, bool has_b, bool wide, int block);
+int batched_occupancy(int integ, int geom, bool has_b, bool wide, bool pairs, int block);
 
 // The device-array helpers (bt_memory.hip); `bad` and `count` are device counters.
 int launch_fill_i64(i64* dst, i64 value, i64 n, hipStream_t stream);

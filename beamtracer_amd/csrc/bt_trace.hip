@@ -197,12 +197,55 @@ __device__ __forceinline__ KParams cold(KParams k) {
 // test of the last step computed for this same point: the same operations on the same
 // operands, so the same bits.  Otherwise they are computed and returned.  With `pre` as
 // well, the two gathers were issued then too (the same operations: `pgx`, `pgy`).
-template <int GEOM, bool GENERAL>
+// The value of `v` in the other lane of a pair (lanes 2t and 2t + 1; both active).
+__device__ __forceinline__ double pair_swap(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0xB1, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Two lanes per trajectory: this lane's half of edge_field's two gathers on the one patch that
+// is the mesh -- E_x in even lanes, E_y in odd ones, each with the operations of that gather
+// alone -- issued; pair_fields then gives both lanes E_x and E_y.
+__device__ __forceinline__ Bilinear issue_pair_gather(const TraceParams& p, const Fields& F, double sx, double sy) {
+    const bool odd = (__lane_id() & 1u) != 0;
+    const PatchView v = patch_view<false>(p, p.n_patches - 1);
+    const double s = odd ? sx : sx + 0.5;
+    const double t = odd ? sy + 0.5 : sy;
+    const int imax = odd ? (int)v.nx - 2 : (int)v.nx - 1;
+    const int jmax = odd ? (int)v.ny - 1 : (int)v.ny - 2;
+    const unsigned cols = (unsigned)(odd ? v.ny + 1 : v.ny);
+    const unsigned base = odd ? (unsigned)(p.fy_base + v.ofy) : (unsigned)(p.fx_base + v.ofx);
+    Bilinear b;
+    const int i = clamped_floor_index<false>(s, imax);
+    const int j = clamped_floor_index<false>(t, jmax);
+    b.u = s - (double)i;
+    b.w = t - (double)j;
+    const unsigned c = mad24((unsigned)i, cols, base + (unsigned)j);
+    if (odd) {
+        b.lo = load_pair(F.fy, c);
+        b.hi = load_pair(F.fy, c + cols);
+    } else {
+        b.lo = load_pair(F.fx, c);
+        b.hi = load_pair(F.fx, c + cols);
+    }
+    return b;
+}
+
+__device__ __forceinline__ void pair_fields(const Bilinear& b, double& ex, double& ey) {
+    const bool odd = (__lane_id() & 1u) != 0;
+    const double mine = bilinear_value(b), other = pair_swap(mine);
+    ex = odd ? other : mine;
+    ey = odd ? mine : other;
+}
+
+template <int GEOM, bool GENERAL, int LPT = 1>
 __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F, double q0, double q1, double q2,
                                            double& ex, double& ey, double& cy, double& cz,
                                            double& lq, double& qy, bool known, double& sx, double& sy,
                                            bool pre = false, const Bilinear* pgx = nullptr,
                                            const Bilinear* pgy = nullptr) {
+    static_assert(LPT == 1 || !GENERAL, "two lanes per trajectory only on one uniform patch");
     if (!known) {
         cy = 0.0; cz = 0.0; qy = q1;
         if (GEOM == 1) {
@@ -226,6 +269,10 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
             sy = (qy - v.oy) * p.inv_h;
         }
     }
+    if (LPT == 2) {
+        pair_fields(pre ? *pgx : issue_pair_gather(p, F, sx, sy), ex, ey);
+        return;
+    }
     if (!GENERAL && pre) {
         ex = bilinear_value(*pgx);
         ey = bilinear_value(*pgy);
@@ -238,9 +285,15 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
 }
 
 // The first-stage gathers of the next step at the end point, with its cell coordinates (sx, sy)
-// on the one patch that is the mesh (edge_field's gathers, issued now: RK4 only).
+// on the one patch that is the mesh (edge_field's gathers, issued now: RK4 only; a pair's
+// lane issues its half into gx).
+template <int LPT = 1>
 __device__ __forceinline__ void issue_first_stage(const TraceParams& p, const Fields& F, double sx, double sy,
                                                   Bilinear& gx, Bilinear& gy) {
+    if (LPT == 2) {
+        gx = issue_pair_gather(p, F, sx, sy);
+        return;
+    }
     const PatchView v = patch_view<false>(p, p.n_patches - 1);
     gx = bilinear_issue<true>(F.fx, (unsigned)(p.fx_base + v.ofx), v.ny, sx + 0.5, sy, (int)v.nx - 1, (int)v.ny - 2);
     gy = bilinear_issue<true>(F.fy, (unsigned)(p.fy_base + v.ofy), v.ny + 1, sx, sy + 0.5, (int)v.nx - 2,
@@ -318,17 +371,29 @@ __device__ __forceinline__ void sum_leaves(const WaveWin& w, KParams kp, i64 ofn
 // their slots, the others leave the lane.  The cache's cell (c, r) holds columns c, c + 1 and
 // rows r, r + 1; column c stays iff i is c - 1 or c, column c + 1 iff i is c or c + 1 (rows
 // alike); slot q holds column c when q's column parity is c's, else c + 1.
-template <bool USE_WIN, int WC>
+//
+// With two lanes per trajectory (LPT 2) each lane of the pair keeps the two nodes whose column
+// parity is its own (the lane's parity): its slot b holds the node of row parity b, and the
+// pair together holds what one lane holds alone -- each node's sum is the same sum, added and
+// rounded the same way.
+template <int LPT>
+__device__ __forceinline__ int lane_column_parity() {
+    return LPT == 2 ? (int)(__lane_id() & 1u) : 0;
+}
+
+template <bool USE_WIN, int WC, int LPT = 1>
 __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, KParams kp, i64 ofn, i64 ny,
                                            int design, int i, int j) {
     if (i == s.cci && j == s.ccj) return;
     const int c = s.cci, r = s.ccj;
     const bool keep_c = (unsigned)(c - i) <= 1u, keep_c1 = (unsigned)(c + 1 - i) <= 1u;
     const bool keep_r = (unsigned)(r - j) <= 1u, keep_r1 = (unsigned)(r + 1 - j) <= 1u;
+    const int a = lane_column_parity<LPT>();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const bool at_c = ((c ^ (q >> 1)) & 1) == 0;  // slot q's column is c (else c + 1)
-        const bool at_r = ((r ^ (q & 1)) & 1) == 0;
+    for (int q = 0; q < 4 / LPT; ++q) {
+        const int qa = LPT == 2 ? a : (q >> 1), qb = LPT == 2 ? q : (q & 1);  // the slot's parities
+        const bool at_c = ((c ^ qa) & 1) == 0;  // slot q's column is c (else c + 1)
+        const bool at_r = ((r ^ qb) & 1) == 0;
         const bool kept = (at_c ? keep_c : keep_c1) && (at_r ? keep_r : keep_r1);
         if (!kept && s.cq[q] != 0.0)
             sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, at_c ? c : c + 1, at_r ? r : r + 1, s.cq[q]);
@@ -339,15 +404,16 @@ __device__ __forceinline__ void cache_move(LaneState& s, const WaveWin& w, KPara
 }
 
 // Empties the lane's cache (the trajectory ended).
-template <bool USE_WIN, int WC>
+template <bool USE_WIN, int WC, int LPT = 1>
 __device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, KParams kp, i64 ofn, i64 ny,
                                             int design) {
     const int c = s.cci, r = s.ccj;
+    const int a = lane_column_parity<LPT>();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 4 / LPT; ++q) {
+        const int qa = LPT == 2 ? a : (q >> 1), qb = LPT == 2 ? q : (q & 1);
         if (s.cq[q] != 0.0)
-            sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, c + ((c ^ (q >> 1)) & 1), r + ((r ^ (q & 1)) & 1),
-                                    s.cq[q]);
+            sum_leaves<USE_WIN, WC>(w, kp, ofn, ny, design, c + ((c ^ qa) & 1), r + ((r ^ qb) & 1), s.cq[q]);
         s.cq[q] = 0.0;
     }
     s.cci = kNoCell;
@@ -361,18 +427,20 @@ __device__ __forceinline__ void cache_flush(LaneState& s, const WaveWin& w, KPar
 // patch each value is rounded once to an integer (ties to even) and goes to the node words in
 // global memory.  The node totals are exact integer sums of rounded values, so the order of
 // the lanes does not change them; a non-finite deposit is dropped.
-template <bool USE_WIN, int WC>
+template <bool USE_WIN, int WC, int LPT = 1>
 __device__ __forceinline__ void deposit4(LaneState& s, const WaveWin& ww, KParams kp, i64 ofn, i64 ny,
                                          bool in_base, int design, int i, int j, double u, double w,
                                          double deposit) {
     if (__builtin_expect(!(fabs(deposit) < __builtin_inf()), 0)) return;  // (the four values would not be finite)
-    if (in_base) cache_move<USE_WIN, WC>(s, ww, kp, ofn, ny, design, i, j);
+    if (in_base) cache_move<USE_WIN, WC, LPT>(s, ww, kp, ofn, ny, design, i, j);
     const double u0 = 1.0 - u, w0 = 1.0 - w;
     const bool io = (i & 1) != 0, jo = (j & 1) != 0;
+    const int a = lane_column_parity<LPT>();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {  // slot q: the node of parities (q >> 1, q & 1)
-        const bool ia = io != ((q >> 1) != 0);  // the node is i + 1
-        const bool jb = jo != ((q & 1) != 0);   // the node is j + 1
+    for (int q = 0; q < 4 / LPT; ++q) {  // slot q: the node of parities (q >> 1, q & 1); (a, q) for a pair
+        const int qa = LPT == 2 ? a : (q >> 1), qb = LPT == 2 ? q : (q & 1);
+        const bool ia = io != (qa != 0);  // the node is i + 1
+        const bool jb = jo != (qb != 0);  // the node is j + 1
         const double value = ((ia ? u : u0) * (jb ? w : w0)) * deposit;
         if (in_base) {
             s.cq[q] += value;
@@ -483,9 +551,13 @@ __device__ __forceinline__ void record_crossing(KParams kp, const LaneState& s, 
 }
 
 // One step of the trajectory in `s`; returns the status set by the end tests (-1: go on).
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN, int WC>
+// LPT 2: both lanes of a pair step the same trajectory with the same operations (the same
+// bits), each gathering one of the two fields and keeping the deposits of its two nodes; the
+// even lane writes the trajectory's records.
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool USE_WIN, int WC, int LPT = 1>
 __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fields& F, LaneState& s,
                                     const WaveWin& ww, double plane0) {
+    const bool writer = LPT == 1 || (__lane_id() & 1u) == 0;
     const double inv_c2 = p.inv_c2;
     double x0 = s.x0, x1 = s.x1, x2 = s.x2, v0 = s.v0, v1 = s.v1, v2 = s.v2;
     const double qmk = s.qmk;
@@ -533,8 +605,8 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
             double ex, ey, cy, cz, lq, qy, psx, psy;
             const bool known = !GENERAL && stage == 0 && s.point != 0;
             if (known) { qy = s.pt_y; cy = s.pt_cy; cz = s.pt_cz; psx = s.pt_sx; psy = s.pt_sy; }
-            edge_field<GEOM, GENERAL>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy,
-                                      INTEG == 0 && known, &s.pgx, &s.pgy);
+            edge_field<GEOM, GENERAL, LPT>(p, F, q0, q1, q2, ex, ey, cy, cz, lq, qy, known, psx, psy,
+                                           INTEG == 0 && known, &s.pgx, &s.pgy);
             if (INTEG != 0 && stage == 1) { end_y = qy; end_cy = cy; end_cz = cz; end_sx = psx; end_sy = psy; }
             if (GEOM != 0) {
                 const double qe = qmk * ey;
@@ -643,7 +715,7 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
     const i64 cell = p.cells[mad24((unsigned)ei, (unsigned)p.cells_cols, (unsigned)p.cells_base + (unsigned)ej)];
     // RK4 on one patch: the next step's first-stage gathers at the end point, issued before the
     // deposit so that their loads run under it (used if the trajectory goes on: `point`)
-    if (INTEG == 0 && !GENERAL) issue_first_stage(p, F, ci, cj, s.pgx, s.pgy);
+    if (INTEG == 0 && !GENERAL) issue_first_stage<LPT>(p, F, ci, cj, s.pgx, s.pgy);
 
     // space charge: dq dt at the Hermite midpoint, area-weighted (pyccel.py:372-423)
     const double gw = 1.0 / lorentz<GENERAL>(w0, w1, w2, inv_c2);
@@ -674,8 +746,8 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
             w = sym - (double)j;
         }
         w = fmin(fmax(w, 0.0), 1.0);
-        deposit4<USE_WIN, WC>(s, ww, kp, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
-                              s.dqk * dt);
+        deposit4<USE_WIN, WC, LPT>(s, ww, kp, pv.ofn, pv.ny, !GENERAL || pv.ofn == p.bofn, p.design, i, j, u, w,
+                                   s.dqk * dt);
     }
     const i64 ns = s.ns + 1;
     s.ns = ns;
@@ -683,12 +755,12 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
     // plane crossings (pyccel.py:426-442); the first plane's position is in a register, and a
     // record's address is formed only for a crossing (rare)
     if (p.n_planes > 0) {
-        if (__builtin_expect((x0 - plane0 < 0.0) != (y0 - plane0 < 0.0), 0))
+        if (__builtin_expect((x0 - plane0 < 0.0) != (y0 - plane0 < 0.0), 0) && writer)
             record_crossing(kp, s, 0, plane0, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
         if (__builtin_expect(p.n_planes > 1, 0)) {
             for (int pl = 1; pl < p.n_planes; ++pl) {
                 const double xp = p.planes[p.planes_base + pl];
-                if ((x0 - xp < 0.0) != (y0 - xp < 0.0))
+                if ((x0 - xp < 0.0) != (y0 - xp < 0.0) && writer)
                     record_crossing(kp, s, pl, xp, x0, x1, x2, y0, y1, y2, gv, v0, v1, v2, gw, w0, w1, w2, ns);
             }
         }
@@ -724,7 +796,7 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
             const i64 e = at < p.n_subcells ? c->subcells[p.subcells_base + at] : -1;
             if (e >= 0) {
                 st = 3;
-                c->electrode[s.k] = e;
+                if (writer) c->electrode[s.k] = e;
             } else if (e == -2) {
                 st = -2;
             }
@@ -785,12 +857,17 @@ __device__ __forceinline__ void load_design(TraceParams& lp, const TraceParams& 
 //
 // RECORDS: the launch may record paths and checkpoints (else that code is compiled out; a
 // batched launch records neither).
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS, int WC = kWinCols>
+//
+// LPT 2: two lanes per trajectory (lanes 2t, 2t + 1 of a wave take the same trajectory and
+// step it together; RECORDS false).
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool BATCHED, bool RECORDS, int WC = kWinCols, int LPT = 1>
 __device__ __forceinline__ void trace_loop(const TraceParams& p) {
+    static_assert(LPT == 1 || (!GENERAL && !RECORDS), "two lanes per trajectory: one uniform patch, no records");
     const KParams kp = kernel_params();  // p itself, for the rarely used fields
     WaveWin ww = window_start<WC>();
     const unsigned lane = __lane_id();
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const bool writer = LPT == 1 || (lane & 1u) == 0;  // the lane that writes its trajectory's end
     TraceParams dp = p;  // BATCHED: the mesh of design wd
     int wd = -1;
     const TraceParams& q = BATCHED ? dp : p;  // the mesh of the lanes being stepped
@@ -812,13 +889,14 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
             const u64 need = __ballot(want);
             bool end_seen = false;
             if (need != 0) {
-                const int count = __popcll(need);
+                // (a pair's two lanes are free together and take one entry: the pair's rank)
+                const int count = __popcll(need) / LPT;
                 const int leader = __ffsll((long long)need) - 1;
                 u64 base = 0;
                 if ((int)lane == leader) base = atomicAdd(c->queue, (u64)count);
                 base = __shfl(base, leader);
                 if (want) {
-                    const u64 m = base + (u64)__popcll(need & below);
+                    const u64 m = base + (u64)(__popcll(need & below) / LPT);
                     if (m < (u64)c->n_active) {
                         const i64 k = c->active[m];
                         const i64 N = c->n;
@@ -854,7 +932,7 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                                 s.ck_next = s.ck_index * c->ckpt_interval;
                             }
                             if (s.ns >= c->max_steps) {  // no step: the loop of the reference ends at once
-                                store_end(kp, s, 4);
+                                if (writer) store_end(kp, s, 4);
                                 s.k = -1;
                             }
                         }
@@ -889,11 +967,11 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
                     s.ck_index += 1;
                     s.ck_next += p.ckpt_interval;
                 }
-                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC>(q, kp, F, s, ww, plane0);
+                const int st = step<INTEG, GEOM, HAS_B, GENERAL, true, WC, LPT>(q, kp, F, s, ww, plane0);
                 if (st != -1 || s.ns >= p.max_steps) {
-                    store_end(kp, s, st != -1 ? st : 4);
+                    if (writer) store_end(kp, s, st != -1 ? st : 4);
                     s.k = -1;
-                    cache_flush<true, WC>(s, ww, kp, q.bofn, q.ny, q.design);
+                    cache_flush<true, WC, LPT>(s, ww, kp, q.bofn, q.ny, q.design);
                 }
             }
         }
@@ -903,18 +981,18 @@ __device__ __forceinline__ void trace_loop(const TraceParams& p) {
     window_flush_all<WC>(ww, kp);
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC, int LPT = 1>
 __global__ __launch_bounds__(256) void trace_kernel(TraceParams p) {
-    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false, RECORDS, WC>(p);
+    trace_loop<INTEG, GEOM, HAS_B, GENERAL, false, RECORDS, WC, LPT>(p);
 }
 
 // Every design's trajectories in one persistent launch.  Held to three waves per SIMD (168
 // VGPRs): a batch always fills the GPU, so the third wave's latency hiding pays.  WC: the
 // columns of each wave's deposit window (a wide window of 32 holds lanes spread further
-// along x, at twice the LDS).
-template <int INTEG, int GEOM, bool HAS_B, int WC>
+// along x, at twice the LDS); LPT: lanes per trajectory (1 or 2).
+template <int INTEG, int GEOM, bool HAS_B, int WC, int LPT = 1>
 __global__ __launch_bounds__(256, 3) void batched_kernel(TraceParams p) {
-    trace_loop<INTEG, GEOM, HAS_B, false, true, false, WC>(p);
+    trace_loop<INTEG, GEOM, HAS_B, false, true, false, WC, LPT>(p);
 }
 
 // The start of a batched trace from the emitter: x, v from x0, v0; steps 0; status and
@@ -970,14 +1048,25 @@ static size_t window_bytes(int block, bool wide) {
     return (size_t)(block / 64) * (wide ? kWideCols : kWinCols) * kWinRows * sizeof(u64);
 }
 
-template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC>
+template <int INTEG, int GEOM, bool HAS_B, bool GENERAL, bool RECORDS, int WC, int LPT = 1>
 static TraceKernel kernel_ptr() {
-    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL, RECORDS, WC>;
+    return trace_kernel<INTEG, GEOM, HAS_B, GENERAL, RECORDS, WC, LPT>;
 }
 
 // The general variants (refined patches, graded axis, relativistic motion) always record and
-// use the narrow window; the others come with either window width.
-TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide) {
+// use the narrow window; the others come with either window width, and without records also
+// with two lanes per trajectory (`pairs`, ignored otherwise).
+TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, bool records, bool wide,
+                                bool pairs) {
+    if (pairs && !general && !records) {
+#define BT_SELP(I, G, B, W) \
+        if (integ == I && geom == G && has_b == B && wide == W) return kernel_ptr<I, G, B, false, false, W ? kWideCols : kWinCols, 2>();
+#define BT_SELP_G(I, G) BT_SELP(I, G, false, false) BT_SELP(I, G, false, true) BT_SELP(I, G, true, false) BT_SELP(I, G, true, true)
+        BT_SELP_G(0, 0) BT_SELP_G(0, 1) BT_SELP_G(0, 2) BT_SELP_G(1, 0) BT_SELP_G(1, 1) BT_SELP_G(1, 2)
+#undef BT_SELP_G
+#undef BT_SELP
+        return nullptr;
+    }
 #define BT_SEL(I, G, B, X, R, W) \
     if (integ == I && geom == G && has_b == B && general == X && (X || records == R) && (X || wide == W)) \
         return kernel_ptr<I, G, B, X, R, W ? kWideCols : kWinCols>();
@@ -994,8 +1083,8 @@ TraceKernel select_trace_kernel(int integ, int geom, bool has_b, bool general, b
 }
 
 int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, int integ, int geom,
-                 bool has_b, bool general, bool records, bool wide, i64 nodes) {
-    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records, wide);
+                 bool has_b, bool general, bool records, bool wide, bool pairs, i64 nodes) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records, wide, pairs);
     if (kern == nullptr) return -1;
     const i64 words = kWords * nodes;
     int zgrid = (int)((words + 255) / 256);
@@ -1012,9 +1101,11 @@ int launch_trace(const TraceParams& p, int block, int grid, hipStream_t stream, 
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide) {
+TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide, bool pairs) {
 #define BT_SELB(I, G, B) \
-    if (integ == I && geom == G && has_b == B) return wide ? batched_kernel<I, G, B, kWideCols> : batched_kernel<I, G, B, kWinCols>;
+    if (integ == I && geom == G && has_b == B) \
+        return pairs ? (wide ? batched_kernel<I, G, B, kWideCols, 2> : batched_kernel<I, G, B, kWinCols, 2>) \
+                     : (wide ? batched_kernel<I, G, B, kWideCols> : batched_kernel<I, G, B, kWinCols>);
     BT_SELB(0, 0, false) BT_SELB(0, 1, false) BT_SELB(0, 2, false)
     BT_SELB(1, 0, false) BT_SELB(1, 1, false) BT_SELB(1, 2, false)
     BT_SELB(0, 0, true) BT_SELB(0, 1, true) BT_SELB(0, 2, true)
@@ -1024,9 +1115,9 @@ TraceKernel select_batched_kernel(int integ, int geom, bool has_b, bool wide) {
 }
 
 int launch_batched(const TraceParams& p, const double* x0, const double* v0, bool reset, int block, int grid,
-                   hipStream_t stream, int integ, int geom, bool has_b, bool wide, i64 total_nodes,
+                   hipStream_t stream, int integ, int geom, bool has_b, bool wide, bool pairs, i64 total_nodes,
                    i64 max_design_nodes) {
-    TraceKernel kern = select_batched_kernel(integ, geom, has_b, wide);
+    TraceKernel kern = select_batched_kernel(integ, geom, has_b, wide, pairs);
     if (kern == nullptr) return -1;
     if (reset) {
         i64 work = 3 * p.n > total_nodes ? 3 * p.n : total_nodes;
@@ -1051,8 +1142,8 @@ int launch_batched(const TraceParams& p, const double* x0, const double* v0, boo
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int batched_occupancy(int integ, int geom, bool has_b, bool wide, int block) {
-    TraceKernel kern = select_batched_kernel(integ, geom, has_b, wide);
+int batched_occupancy(int integ, int geom, bool has_b, bool wide, bool pairs, int block) {
+    TraceKernel kern = select_batched_kernel(integ, geom, has_b, wide, pairs);
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),
@@ -1085,8 +1176,8 @@ int launch_arith_selftest(const double* y, const double* z, i64 n, double* r, do
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, bool wide, int block) {
-    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records, wide);
+int trace_occupancy(int integ, int geom, bool has_b, bool general, bool records, bool wide, bool pairs, int block) {
+    TraceKernel kern = select_trace_kernel(integ, geom, has_b, general, records, wide, pairs);
     if (kern == nullptr) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(kern),

@@ -384,7 +384,7 @@ def trace_trajectories_2d(
     path_count, num_threads, charge, graded, x_nodes, inv_c2,
     *, magnetic=None, block_size=0, max_blocks=0, exponent=None, workspace=None, stream=None,
     refill_block=False, checkpoints=None, checkpoint_interval=64, record_paths=True, wide_window=False,
-    noalloc=False,
+    lane_pairs=False, noalloc=False,
 ) -> None:  # fmt: skip
     """Trace the ``active`` trajectories until they end, or reach a surface.
 
@@ -402,7 +402,9 @@ def trace_trajectories_2d(
     ``path_slot`` is negative, else those paths go unrecorded) and runs the trace variant
     without the path code (the host form finds this itself); ``wide_window`` gives each wave's
     deposit window in LDS 32 mesh columns instead of 16 (``BT_WIDE_WINDOW``: beams whose lanes
-    spread along x). ``noalloc`` (device form, no ``workspace``) calls the allocation-free entry
+    spread along x); ``lane_pairs`` steps each trajectory on two lanes, one gathering E_x and the
+    other E_y (``BT_LANE_PAIRS``; ignored with paths, checkpoints or the general features).
+    ``noalloc`` (device form, no ``workspace``) calls the allocation-free entry
     point, whose fixed scratch holds at most ``BT_NOALLOC_MAX_NODES`` mesh nodes. None of these
     changes a result.
     """
@@ -469,7 +471,7 @@ def trace_trajectories_2d(
         int(magnetic is not None), _ptr(bx), _ptr(by), _ptr(bz),
         int(exponent), int(block_size), int(max_blocks),
         (_lib.BT_REFILL_BLOCK if refill_block else 0) | (0 if record_paths else _lib.BT_NO_PATHS)
-        | (_lib.BT_WIDE_WINDOW if wide_window else 0),
+        | (_lib.BT_WIDE_WINDOW if wide_window else 0) | (_lib.BT_LANE_PAIRS if lane_pairs else 0),
         _ptr(workspace), 0 if workspace is None else int(workspace.numel() * workspace.element_size()),
         None, 0, 0,
     )  # fmt: skip

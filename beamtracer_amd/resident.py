@@ -129,7 +129,8 @@ class ResidentTrace:
                 ]  # fmt: skip
                 kernels.trace_trajectories_2d(*call, magnetic=self._magnetic, exponent=exponent,
                                               block_size=tracer.block_size, refill_block=tracer.refill_block,
-                                              record_paths=self._records, wide_window=tracer.wide_window)  # fmt: skip
+                                              record_paths=self._records, wide_window=tracer.wide_window,
+                                              lane_pairs=tracer.lane_pairs)  # fmt: skip
                 active = resume_paused(tracer, active, s["x"], s["status"], s["electrode"])
             grown = outgrown(tracer._capacity, self._counts, self._path_count)
         steps = self._steps.numpy()

@@ -248,6 +248,7 @@ class HipBeamTracer:
         refill_block: bool = False,
         wide_window: bool = False,
         queue_order: str = "emitter",
+        lane_pairs: bool = False,
     ) -> None:
         if not (0.0 < step_fraction <= 1.0):
             raise ValueError(f"step_fraction must lie in (0, 1], got {step_fraction}")
@@ -269,6 +270,8 @@ class HipBeamTracer:
         self.refill_block = bool(refill_block)
         #: per-wave deposit windows of 32 columns instead of 16 (see :mod:`beamtracer_amd.tuning`)
         self.wide_window = bool(wide_window)
+        #: two lanes per trajectory (see :mod:`beamtracer_amd.tuning`)
+        self.lane_pairs = bool(lane_pairs)
         if queue_order not in tuning.QUEUE_ORDERS:
             raise ValueError(f"queue_order must be one of {tuning.QUEUE_ORDERS}, got {queue_order!r}")
         #: the order in which the kernel's queue hands out the trajectories (results do not
@@ -500,7 +503,8 @@ class HipBeamTracer:
             self._call_device(call, magnetic, exponent)
         else:
             kernels.trace_trajectories_2d(*call, magnetic=magnetic, exponent=exponent, block_size=self.block_size,
-                                          refill_block=self.refill_block, wide_window=self.wide_window)  # fmt: skip
+                                          refill_block=self.refill_block, wide_window=self.wide_window,
+                                          lane_pairs=self.lane_pairs)  # fmt: skip
 
     def _call_device(self, call: list, magnetic, exponent: int) -> None:
         """One kernel call with the arrays in the library's device memory (copied back into
@@ -512,7 +516,7 @@ class HipBeamTracer:
         ck, interval = self.checkpoints if self.checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=self.block_size,
                                       refill_block=self.refill_block, record_paths=bool((call[27] >= 0).any()),
-                                      wide_window=self.wide_window,
+                                      wide_window=self.wide_window, lane_pairs=self.lane_pairs,
                                       checkpoints=ck, checkpoint_interval=interval)  # fmt: skip
         for i in (1, 2, 5, 6, 7, 25, 26, 28, 29, 31):
             call[i][...] = dev[i].numpy()

@@ -1,4 +1,5 @@
-"""Launch-shape tuning of the plain trace (threads per workgroup, refill mode, window width).
+"""Launch-shape tuning of the plain trace (threads per workgroup, refill mode, window width,
+lanes per trajectory).
 
 The persistent trace kernel's launch shape changes only its speed, never its results (the
 node sums are exact integers and every lane steps its own trajectory): workgroups of 256
@@ -7,10 +8,13 @@ scattered times (electrodes, reflections) keep their waves together longer with 
 wave-wide refill (``BT_REFILL_BLOCK``: a wave takes new trajectories once all its lanes are
 free), and beams whose lanes spread along x (a fine mesh, a thermal start) deposit faster
 into per-wave deposit windows of 32 columns (``BT_WIDE_WINDOW``) than into the 16 that suit
-a coherent beam. :func:`tune_trace`
+a coherent beam. Lanes per trajectory: one lane steps each trajectory, or two lanes
+(``BT_LANE_PAIRS``) step it together, one gathering E_x and the other E_y and each keeping
+the deposits of its two nodes of the cell -- fewer instructions per lane, twice the lanes, so
+it pays only where the launch holds few waves per SIMD. :func:`tune_trace`
 times the candidates on the real arguments and returns the fastest; :class:`HipBeamTracer`
-takes the choice as ``block_size``, ``refill_block`` and ``wide_window``. (The batched form has
-its own tuner, :func:`beamtracer_amd.batch.autotune`.)
+takes the choice as ``block_size``, ``refill_block``, ``wide_window`` and ``lane_pairs``. (The
+batched form has its own tuner, :func:`beamtracer_amd.batch.autotune`.)
 
 The queue order is a tuning choice too: the kernel hands trajectories to lanes in the order
 of ``active``, and a wave's deposit window serves its lanes only while they stay together.
@@ -104,12 +108,13 @@ def _synchronize(like) -> None:
 
 def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks: Sequence[int] = (64, 128, 256),
                refills: Sequence[bool] = (False, True), windows: Sequence[bool] = (False, True),
-               orders: Sequence[str] = QUEUE_ORDERS, repeats: int = 2) -> dict:
+               orders: Sequence[str] = QUEUE_ORDERS, lanes: Sequence[int] = (1, 2), repeats: int = 2) -> dict:
     """Time ``trace_trajectories_2d`` on ``args`` (the 35 arguments, device arrays: the
     library's :class:`~beamtracer_amd.device.DeviceArray` or GPU torch tensors) for every
-    (queue order, threads per workgroup, refill mode, window width), restoring the outputs
-    before each launch, and return ``{"timings_ms": {...}, "block_size": b, "refill_block":
-    r, "wide_window": w, "queue_order": o}`` (the fastest, best of ``repeats``; each launch
+    (queue order, threads per workgroup, refill mode, window width, lanes per trajectory),
+    restoring the outputs before each launch, and return ``{"timings_ms": {...}, "block_size":
+    b, "refill_block": r, "wide_window": w, "queue_order": o, "lane_pairs": p}`` (the fastest,
+    best of ``repeats``; each launch
     timed between two synchronizations). On return ``args`` holds one trace with the fastest
     shape and its ``active`` in the chosen order."""
     like = args[1]
@@ -122,9 +127,11 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
         host = [_host_copy(args[i]) for i in (0, 2, 4, 3, 8, 9)]
         exponent = kernels.trace_exponent(*host, args[11], args[19], args[34])
     record_paths = bool((_host_copy(args[27]) >= 0).any())
+    if record_paths:  # (two lanes per trajectory do not record paths)
+        lanes = (1,)
     timings = {}
     best = None
-    for order, wide, refill, block in itertools.product(orders, windows, refills, blocks):
+    for order, wide, refill, block, lpt in itertools.product(orders, windows, refills, blocks, lanes):
         args[0] = queues[order]
         times = []
         for _ in range(repeats):
@@ -133,21 +140,24 @@ def tune_trace(args: list, *, magnetic=None, exponent: int | None = None, blocks
             _synchronize(like)
             start = time.perf_counter()
             kernels.trace_trajectories_2d(*args, magnetic=magnetic, exponent=exponent, block_size=block,
-                                          refill_block=refill, record_paths=record_paths, wide_window=wide)  # fmt: skip
+                                          refill_block=refill, record_paths=record_paths, wide_window=wide,
+                                          lane_pairs=lpt == 2)  # fmt: skip
             _synchronize(like)
             times.append((time.perf_counter() - start) * 1e3)
         key = f"block{block}" + ("/refill-block" if refill else "") + ("/wide" if wide else "")
+        key += "/pairs" if lpt == 2 else ""
         if order != "emitter":
             key = f"{order}/{key}"
         timings[key] = min(times)
         if best is None or min(times) < best[0]:
-            best = (min(times), block, refill, wide, order)
+            best = (min(times), block, refill, wide, order, lpt == 2)
     assert best is not None
     args[0] = queues[best[4]]
     for i, a in initial.items():
         args[i].copy_(a)
     kernels.trace_trajectories_2d(*args, magnetic=magnetic, exponent=exponent, block_size=best[1],
-                                  refill_block=best[2], record_paths=record_paths, wide_window=best[3])  # fmt: skip
+                                  refill_block=best[2], record_paths=record_paths, wide_window=best[3],
+                                  lane_pairs=best[5])  # fmt: skip
     _synchronize(like)
     return {"timings_ms": timings, "block_size": best[1], "refill_block": best[2], "wide_window": best[3],
-            "queue_order": best[4], "best_ms": best[0]}
+            "queue_order": best[4], "lane_pairs": best[5], "best_ms": best[0]}

@@ -206,7 +206,8 @@ def device_arguments(host, magnetic=None):
 
 
 def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkpoints=None,
-                refill_block: bool = False, wide_window: bool = False, queue_order: str = "emitter") -> dict:
+                refill_block: bool = False, wide_window: bool = False, queue_order: str = "emitter",
+                lane_pairs: bool = False) -> dict:
     """Trace every trajectory ``steps`` times (after ``warmup``), states reset before each
     (``checkpoints``: (tensor, interval) receiving the states, as for the adjoint)."""
     import torch
@@ -229,7 +230,8 @@ def time_traces(host, magnetic, steps: int, warmup: int, block_size: int, checkp
         ck, interval = checkpoints if checkpoints is not None else (None, 64)
         kernels.trace_trajectories_2d(*dev, magnetic=mag, exponent=exponent, block_size=block_size, stream=stream,
                                       checkpoints=ck, checkpoint_interval=interval, refill_block=refill_block,
-                                      record_paths=bool((host[27] >= 0).any()), wide_window=wide_window)  # fmt: skip
+                                      record_paths=bool((host[27] >= 0).any()), wide_window=wide_window,
+                                      lane_pairs=lane_pairs)  # fmt: skip
         if end is not None:
             end.record(stream)
 
@@ -379,10 +381,12 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
     dev, mag = device_arguments(host, design.magnetic)
     tune = tuning.tune_trace(dev, magnetic=mag, repeats=1)
     log(f"extraction tuned: block {tune['block_size']} refill_block {tune['refill_block']} "
-        f"wide_window {tune['wide_window']} queue {tune['queue_order']} ({tune['best_ms']:.1f} ms)")
+        f"wide_window {tune['wide_window']} queue {tune['queue_order']} lane_pairs {tune['lane_pairs']} "
+        f"({tune['best_ms']:.1f} ms)")
     del dev, mag
     r = time_traces(host, design.magnetic, steps, 1, tune["block_size"], refill_block=tune["refill_block"],
-                    wide_window=tune["wide_window"], queue_order=tune["queue_order"])  # fmt: skip
+                    wide_window=tune["wide_window"], queue_order=tune["queue_order"],
+                    lane_pairs=tune["lane_pairs"])  # fmt: skip
     rate = r["steps_per_trace"] * steps / r["wall"]
     flops = FLOP_PER_STEP_BORIS_B_CYL * r["steps_per_trace"] / r["launch_s"] / 1e12
     return {
@@ -397,7 +401,8 @@ def extraction_leg(particles: int, steps: int, block_size: int) -> dict:
         "frac_fp64_peak": flops / PEAK_FP64_TFLOPS,
         "flop_per_step": FLOP_PER_STEP_BORIS_B_CYL,
         "launch_shape": {"block_size": tune["block_size"], "refill_block": tune["refill_block"],
-                         "wide_window": tune["wide_window"], "queue_order": tune["queue_order"]},
+                         "wide_window": tune["wide_window"], "queue_order": tune["queue_order"],
+                         "lane_pairs": tune["lane_pairs"]},
         "tuning_ms": tune["timings_ms"],
         "default_shape_launch_ms": default["launch_s"] * 1e3,
     }

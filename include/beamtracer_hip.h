@@ -69,6 +69,12 @@ extern "C" {
 /* flag of bt_batch_occupancy only: the variant with a node magnetic field (a batch whose
  * bt_batch_args carry magnetic arenas runs it). */
 #define BT_MAGNETIC 8
+/* flags of a trace: two lanes per trajectory -- the lanes of a pair step the same trajectory
+ * with the same operations, one gathering E_x and the other E_y, each keeping the deposits of
+ * its two nodes of the cell (the launch holds twice the lanes; results unchanged, bit for
+ * bit; ignored with path records, checkpoints, refined patches, a graded axis or relativistic
+ * motion).  The launch-shape tuners time it against one lane per trajectory. */
+#define BT_LANE_PAIRS 16
 
 /* Every argument of trace_trajectories_2d (trace_trajectories_2d_pyccel.py:42-78) as a
  * pointer with its extents, then the node magnetic field that the reference's compiled

@@ -55,11 +55,14 @@ def test_queue_orders_and_launch_shapes_give_the_same_bits():
     bt = batch.BatchedTracer(tracers)
     ref = bt.trace(beams)
     timings = batch.autotune(bt, beams, blocks=(64, 256), fills=(1.0, 0.5), repeats=1)
-    assert len(timings["timings_ms"]) == 2 * 2 * len(batch.ORDERS) * 2 * 2
+    assert len(timings["timings_ms"]) == 2 * 2 * len(batch.ORDERS) * 2 * 2 * 2  # ... x windows x lanes
+    assert sum("/pairs" in k for k in timings["timings_ms"]) == len(timings["timings_ms"]) // 2
     for shape in (batch.LaunchShape(64, 7, "design"), batch.LaunchShape(192, 0, "longest"),
                   batch.LaunchShape(128, 0, "longest", refill_block=True),
                   batch.LaunchShape(256, 0, "design", wide_window=True),
-                  batch.LaunchShape(128, 5, "longest", refill_block=True, wide_window=True), bt.shape):
+                  batch.LaunchShape(128, 5, "longest", refill_block=True, wide_window=True),
+                  batch.LaunchShape(256, 0, "longest", refill_block=True, lane_pairs=True),
+                  batch.LaunchShape(64, 9, "design", wide_window=True, lane_pairs=True), bt.shape):
         bt.shape = shape
         for g, w in zip(bt.trace(beams), ref):
             _assert_identical(g, w)

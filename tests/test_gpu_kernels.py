@@ -219,6 +219,46 @@ def test_wide_deposit_window_on_a_beam_gives_the_same_bits(hip):
         assert np.array_equal(results[1][i], results[0][i]), i
 
 
+@pytest.mark.parametrize("cylindrical,integrator,magnetic", list(itertools.product([0, 1, 2], [0, 1], [False, True])))
+def test_two_lanes_per_trajectory_give_the_same_bits(hip, cylindrical, integrator, magnetic):
+    """Lane pairs (``lane_pairs``: one lane gathers E_x, the other E_y, each keeps the deposits of
+    two of the cell's nodes) and one lane per trajectory give the same trace and charge, bit for
+    bit, with every window width, refill mode and block size; the pair variant does not record
+    paths (``record_paths=False``)."""
+    import torch
+
+    args = cases.kernel_case(cylindrical, integrator)
+    args[27][:] = -1  # path_slot: no paths
+    args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
+    b = _magnetic(args) if magnetic else None
+    single = _run(hip.trace_trajectories_2d, args, magnetic=b)
+    mag = None if b is None else tuple(torch.from_numpy(c.copy()).cuda() for c in b)
+    for wide, refill, block in ((False, False, 256), (True, True, 64), (False, True, 128)):
+        dev = [torch.from_numpy(a.copy()).cuda() if isinstance(a, np.ndarray) else a for a in args]
+        hip.trace_trajectories_2d(*dev, magnetic=mag, record_paths=False, lane_pairs=True, wide_window=wide,
+                                  refill_block=refill, block_size=block)  # fmt: skip
+        torch.cuda.synchronize()
+        for i in (*cases.OUTPUTS, cases.CHARGE):
+            assert np.array_equal(dev[i].cpu().numpy(), single[i]), (wide, refill, block, i)
+
+
+def test_two_lanes_per_trajectory_on_a_beam_give_the_same_bits(hip):
+    """3000 einzel-lens trajectories, host form: lane pairs and single lanes agree bit for bit
+    (records, steps, statuses, charge)."""
+    import bench
+
+    design, tracer = bench.build_workload(3000)
+    host = bench.kernel_arguments(design, tracer)
+    results = []
+    for pairs in (False, True):
+        args = cases.copy_args(host)
+        hip.trace_trajectories_2d(*args, lane_pairs=pairs)
+        results.append(args)
+    assert (results[0][6] == 0).sum() > 1000
+    for i in (*cases.OUTPUTS, cases.CHARGE):
+        assert np.array_equal(results[1][i], results[0][i]), i
+
+
 def test_trace_without_active_and_at_max_steps(hip):
     args = cases.kernel_case(1, 0)
     args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])

@@ -11,12 +11,15 @@ from oracle import cases
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("arrays", ["library", "torch"])
-def test_tuned_trace_equals_the_default_trace(arrays):
+@pytest.mark.parametrize("arrays,paths", [("library", True), ("torch", True), ("library", False)])
+def test_tuned_trace_equals_the_default_trace(arrays, paths):
+    """Without paths the tuner also times two lanes per trajectory."""
     from beamtracer_amd import kernels, tuning
     from beamtracer_amd.device import DeviceArray
 
     args = cases.kernel_case(1, 0, refined=False)
+    if not paths:
+        args[27][:] = -1
     args[cases.CHARGE] = np.zeros_like(args[cases.CHARGE][:1])
     host = cases.copy_args(args)
     kernels.trace_trajectories_2d(*host)
@@ -33,6 +36,8 @@ def test_tuned_trace_equals_the_default_trace(arrays):
     tune = tuning.tune_trace(dev, blocks=(64, 256), repeats=1)
     shapes = {"block64", "block256", "block64/refill-block", "block256/refill-block"}
     shapes = shapes | {f"{k}/wide" for k in shapes}
+    if not paths:
+        shapes = shapes | {f"{k}/pairs" for k in shapes}
     assert set(tune["timings_ms"]) == shapes | {f"coherent/{k}" for k in shapes}
     for i in (*cases.OUTPUTS, cases.CHARGE):
         assert np.array_equal(back(dev[i]), host[i]), i
