@@ -214,6 +214,12 @@ __device__ __forceinline__ Pair load_pair(Rsrc r, unsigned elem) {
     return __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 8u, 0, 0));
 }
 
+// The same at element `elem` + `base`, with `base` the same in every lane: it goes into the
+// load's scalar offset, so the lanes' offsets need no add for it.
+__device__ __forceinline__ Pair load_pair_at(Rsrc r, unsigned elem, unsigned base) {
+    return __builtin_bit_cast(Pair, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 8u, base * 8u, 0));
+}
+
 // trunc(s) saturated to the 32-bit range, NaN to 0: the hardware conversion v_cvt_i32_f64
 // (a C++ conversion of an out-of-range double is undefined, so the instruction is named).
 __device__ __forceinline__ int trunc_sat_i32(double s) {
@@ -264,9 +270,14 @@ __device__ __forceinline__ Bilinear bilinear_issue(Rsrc r, unsigned base, i64 co
     const int j = clamped_floor_index<UNIFORM>(t, jmax);
     b.u = s - (double)i;
     b.w = t - (double)j;
-    const unsigned c = mad24((unsigned)i, (unsigned)cols, base + (unsigned)j);
-    b.lo = load_pair(r, c);
-    b.hi = load_pair(r, c + (unsigned)cols);
+    const unsigned c = mad24((unsigned)i, (unsigned)cols, (unsigned)j);
+    if (UNIFORM) {  // base and the row length are the same in every lane: scalar offsets
+        b.lo = load_pair_at(r, c, base);
+        b.hi = load_pair_at(r, c, base + (unsigned)cols);
+    } else {
+        b.lo = load_pair(r, c + base);
+        b.hi = load_pair(r, c + base + (unsigned)cols);
+    }
     return b;
 }
 
