@@ -135,10 +135,10 @@ struct NoHook {
 // the fields waits for them.
 template <int GEOM, typename Hook = NoHook>
 __device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F, double qm, Vec3 q,
-                                      Hook hook = Hook()) {
+                                      Hook hook = Hook(), const Radial* known = nullptr) {
     double cy = 0.0, cz = 0.0, qy = q.b;
     if (GEOM == 1) {
-        const Radial rd = radial(q.b, q.c);
+        const Radial rd = known != nullptr ? *known : radial(q.b, q.c);
         qy = rd.r; cy = rd.cy; cz = rd.cz;
     } else if (GEOM == 2) {
         cy = 1.0;
@@ -306,6 +306,12 @@ __device__ __forceinline__ void window_flush_all(KAdj kp, GradWindow& W) {
 // (ci, cj) of the active lanes with room below them, the direction of the backward walk.
 __device__ __forceinline__ void window_follow(const AdjointParams& p, KAdj kp, GradWindow& W, bool act, int ci,
                                               int cj) {
+    // the window stays when every active lane's cell lies inside it with the margins tested
+    // below (two columns on both sides, a row below and two above): one compare per lane
+    // instead of the wave-wide minima and maxima, which only a move needs
+    const bool inside = !act || ((unsigned)(ci - 2 - W.lo) <= (unsigned)(kWinCols - 5) &&
+                                 (unsigned)(cj - 1 - W.base) <= (unsigned)(kWinRows - 4));
+    if (W.lo >= 0 && __ballot(!inside) == 0) return;
     int imin, imax, jmin, jmax;
     if (p.nx < 32768 && p.ny < 32768) {  // cells below 2^15 on both axes: (ci, cj) in one word
         const int both = (ci & 0xffff) | (cj << 16);
@@ -325,10 +331,14 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, KAdj kp, G
     if (nlo > imin - 2) nlo = imin - 2;
     if (nlo < 0) nlo = 0;
     const int nbase = rows_ok ? W.base : (jmin - 2 > 0 ? jmin - 2 : 0);
+    // (lanes spread over more rows than the margins allow leave the rows where they are: then
+    // the columns that stay keep their sums, and a window already in place moves nothing)
+    const bool same_rows = nbase == W.base;
+    if (nlo == W.lo && same_rows) return;
     if (W.lo >= 0) {
         for (int s = 0; s < kWinCols; ++s) {
             const int col = W.lo + ((s - W.lo) & (kWinCols - 1));
-            if (!(rows_ok && col >= nlo && col < nlo + kWinCols)) flush_column(kp, W, s, col);
+            if (!(same_rows && col >= nlo && col < nlo + kWinCols)) flush_column(kp, W, s, col);
         }
     }
     W.lo = nlo;
@@ -342,11 +352,12 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, KAdj kp, G
 template <int GEOM>
 __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
                                               GradCache& fx_sums, GradCache& fy_sums, double qm, Vec3 q,
-                                              Vec3 lg, Vec3& g, StatePrefetch* pf = nullptr) {
+                                              Vec3 lg, Vec3& g, StatePrefetch* pf = nullptr,
+                                              const Radial* known = nullptr) {
     const KAdj kp = adjoint_params();
     double cy = 0.0, cz = 0.0, qy = q.b, r = 0.0, ir = 0.0;
     if (GEOM == 1) {
-        const Radial rd = radial(q.b, q.c);
+        const Radial rd = known != nullptr ? *known : radial(q.b, q.c);
         r = rd.r; qy = r; cy = rd.cy; cz = rd.cz; ir = rd.ir;
     } else if (GEOM == 2) {
         cy = 1.0;
@@ -491,13 +502,14 @@ __device__ __forceinline__ i64 seeded_step(const AdjointParams& p, i64 k, i64 be
     return best;
 }
 
-// The transposed step m (start state x, v): lambda of the end (lx, lv) -> of the start.
+// The transposed step m (start state x, v; `rx`: radial(x.b, x.c) when cylindrical): lambda of
+// the end (lx, lv) -> of the start.
 template <int INTEG, int GEOM>
 __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFields& F, const GradWindow& W,
                                              GradCache& fx_sums, GradCache& fy_sums, i64 k,
-                                             i64 m, bool seeded, double qm, Vec3 x, Vec3 v, Vec3& lx, Vec3& lv,
-                                             StatePrefetch& pf) {
-    const Vec3 a = accel<GEOM>(p, F, qm, x);
+                                             i64 m, bool seeded, double qm, Vec3 x, Vec3 v, const Radial& rx,
+                                             Vec3& lx, Vec3& lv, StatePrefetch& pf) {
+    const Vec3 a = accel<GEOM>(p, F, qm, x, NoHook(), &rx);
     int branch;
     double speed, acc;
     const double dt = step_size(p, v, a, branch, speed, acc);
@@ -582,7 +594,7 @@ __device__ __forceinline__ void reverse_step(const AdjointParams& p, const AdjFi
         La = vaxpy(La, -0.5 * Ldt * dt / (acc * acc), a);
     }
     Vec3 ga;
-    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, x, La, ga, &pf));
+    Lx = vadd(Lx, accel_adjoint<GEOM>(p, F, W, fx_sums, fy_sums, qm, x, La, ga, &pf, &rx));
     lx = Lx;
     lv = Lv;
 }
@@ -601,11 +613,12 @@ __device__ __forceinline__ void get_state(const double* __restrict__ base, i64 l
     v = {s[3 * lanes], s[4 * lanes], s[5 * lanes]};
 }
 
-// The mesh cell of point x (the window's key), clamped to the mesh.
+// The mesh cell of point x (the window's key), clamped to the mesh (rx: radial(x.b, x.c) when
+// cylindrical).
 template <int GEOM>
-__device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, int& ci, int& cj) {
+__device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, const Radial& rx, int& ci, int& cj) {
     double qy = x.b;
-    if (GEOM == 1) qy = radial(x.b, x.c).r;
+    if (GEOM == 1) qy = rx.r;
     else if (GEOM == 2) qy = p.origin_y + fabs(x.b - p.origin_y);
     // (floor clamped to [0, n - 1] as the saturated conversion and one median: the same cell)
     ci = clamped_floor_index<true>((x.a - p.origin_x) * p.inv_h, (int)p.nx - 1);
@@ -780,8 +793,12 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                 // (the step works on copies taken here, where the wait for the prefetch falls;
                 // the prefetch then loads into x, v themselves, with no copy behind it)
                 const Vec3 xc = x, vc = v;
+                // (the start point's distance from the axis, once for the cell and the step's two
+                // evaluations there)
+                Radial rx = {0.0, 0.0, 0.0, 0.0};
+                if (GEOM == 1) rx = radial(xc.b, xc.c);
                 int ci = 0, cj = 0;
-                if (act) cell_of<GEOM>(p, xc, ci, cj);
+                if (act) cell_of<GEOM>(p, xc, rx, ci, cj);
                 window_follow(p, adjoint_params(), W, act, ci, cj);
                 if (act) {
                     const bool seeded = m + 1 == next;
@@ -790,7 +807,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
                     StatePrefetch pf;
                     pf.src = seg + (r > 0 ? r - 1 : 0) * 6 * p.lanes + gl;
                     pf.lanes = p.lanes;
-                    reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, xc, vc, lx, lv, pf);
+                    reverse_step<INTEG, GEOM>(p, F, W, fx_sums, fy_sums, k, m + 1, seeded, qm, xc, vc, rx, lx, lv, pf);
                     x = pf.x;
                     v = pf.v;
                     if (seeded) next = (int)seeded_step(p, k, m + 1);
