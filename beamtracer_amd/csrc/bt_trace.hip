@@ -510,16 +510,20 @@ __device__ __forceinline__ void window_round(WaveWin& w, const TraceParams& q, K
     if (__builtin_expect(rows_ok && cols_ok && !force, 1)) return;
     int nlo = cmin - kRoundSteps;
     if (nlo < 0) nlo = 0;
-    if (rows_ok && !force) {  // only columns leave: those of [lo, lo + WC) outside [nlo, nlo + WC)
+    int nbase = ((rmin + rmax + 1) >> 1) - kWinRows / 2;
+    if (nbase > q.ny - kWinRows) nbase = q.ny - kWinRows;
+    if (nbase < 0) nbase = 0;
+    // rows that stay (in room, or lanes spread over more rows than the room allows, centred
+    // where the window already is): only columns leave, those of [lo, lo + WC) outside
+    // [nlo, nlo + WC) -- none when the columns stay too
+    if (same && !force && (rows_ok || nbase == w.base)) {
+        if (nlo == w.lo) return;
         if (nlo > w.lo) window_flush_columns<WC>(w, kp, w.lo, min(nlo, w.lo + WC) - 1);
         else window_flush_columns<WC>(w, kp, max(nlo + WC, w.lo), w.lo + WC - 1);
         w.lo = nlo;
         return;
     }
     window_flush_all<WC>(w, kp);
-    int nbase = ((rmin + rmax + 1) >> 1) - kWinRows / 2;
-    if (nbase > q.ny - kWinRows) nbase = q.ny - kWinRows;
-    if (nbase < 0) nbase = 0;
     w.lo = nlo; w.base = nbase; w.design = q.design; w.ny = q.ny; w.ofn = q.bofn;
 }
 
