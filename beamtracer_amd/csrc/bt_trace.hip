@@ -82,17 +82,37 @@ struct LaneState {
 constexpr int kNoCell = -(1 << 30);
 constexpr i64 kWindowLimit = (i64)1 << 46;  // a sum added to a window slot (the slots' bound)
 
-// Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113).
-__device__ __forceinline__ double node_bilinear(const double* __restrict__ f, double sx, double sy,
-                                                i64 nx, i64 ny) {
-    if (f == nullptr) return 0.0;
+// Node-field bilinear, constant beyond the mesh (solver/self_field.py:100-113): the point is
+// clamped to the nodes, the cell to the last one, and the four nodes weighted as
+// (1 - a)((1 - b) f00 + b f01) + a((1 - b) f10 + b f11).  The cell and weights are the same for
+// the three components of a node field, so they are formed once (in 32 bits: a mesh has fewer
+// than 2^24 nodes per axis), and each component reads its two node pairs (f00, f01) and
+// (f10, f11) as two 16-byte loads of consecutive nodes.
+struct NodeCell {
+    unsigned c;  // the cell's first node, i ny + j
+    double a, b;
+};
+
+__device__ __forceinline__ NodeCell node_cell(double sx, double sy, int nx, int ny) {
     sx = fmin(fmax(sx, 0.0), (double)(nx - 1));
     sy = fmin(fmax(sy, 0.0), (double)(ny - 1));
-    i64 i = (i64)sx; if (i > nx - 2) i = nx - 2;
-    i64 j = (i64)sy; if (j > ny - 2) j = ny - 2;
-    const double a = sx - (double)i, b = sy - (double)j;
-    return (1.0 - a) * ((1.0 - b) * f[i * ny + j] + b * f[i * ny + j + 1])
-         + a * ((1.0 - b) * f[(i + 1) * ny + j] + b * f[(i + 1) * ny + j + 1]);
+    int i = (int)sx;
+    if (i > nx - 2) i = nx - 2;
+    int j = (int)sy;
+    if (j > ny - 2) j = ny - 2;
+    NodeCell n;
+    n.a = sx - (double)i;
+    n.b = sy - (double)j;
+    n.c = mad24((unsigned)i, (unsigned)ny, (unsigned)j);
+    return n;
+}
+
+__device__ __forceinline__ double node_bilinear(const double* __restrict__ f, const NodeCell& n, int nx, int ny) {
+    if (f == nullptr) return 0.0;
+    const Rsrc r = field_rsrc(f, (i64)nx * ny);
+    const Pair lo = load_pair(r, n.c), hi = load_pair_at(r, n.c, (unsigned)ny);
+    const double a = n.a, b = n.b;
+    return (1.0 - a) * ((1.0 - b) * lo.a + b * lo.b) + a * ((1.0 - b) * hi.a + b * hi.b);
 }
 
 template <int GEOM>
@@ -100,9 +120,10 @@ __device__ __forceinline__ void magnetic_at(const TraceParams& p, double lq, dou
                                             double cz, double& B0, double& B1, double& B2) {
     const double sx = (lq - p.origin_x) / p.h;
     const double sy = (qy - p.origin_y) / p.h;
-    const double c0 = node_bilinear(p.b0, sx, sy, p.nx, p.ny);
-    const double c1 = node_bilinear(p.b1, sx, sy, p.nx, p.ny);
-    const double c2 = node_bilinear(p.b2, sx, sy, p.nx, p.ny);
+    const NodeCell n = node_cell(sx, sy, (int)p.nx, (int)p.ny);
+    const double c0 = node_bilinear(p.b0, n, (int)p.nx, (int)p.ny);
+    const double c1 = node_bilinear(p.b1, n, (int)p.nx, (int)p.ny);
+    const double c2 = node_bilinear(p.b2, n, (int)p.nx, (int)p.ny);
     if (GEOM == 1) {
         B0 = c0; B1 = c1 * cy - c2 * cz; B2 = c1 * cz + c2 * cy;
     } else if (GEOM == 2) {
