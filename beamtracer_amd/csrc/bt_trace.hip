@@ -139,7 +139,11 @@ __device__ __forceinline__ void boris_kick(double& u0, double& u1, double& u2, d
                                            double qm, double inv_c2) {
     const double hq = 0.5 * tau * qm;
     const double m0 = u0 + hq * e0, m1 = u1 + hq * e1, m2 = u2 + hq * e2;
-    const double g = 1.0 / lorentz_factor(m0, m1, m2, inv_c2);
+    // 1 / gamma; classical (inv_c2 = 0): sqrt(1 + |m|^2 0) is exactly 1 for a finite |m|^2 and
+    // NaN otherwise, which the compare gives without the square root and the division
+    double g;
+    if (inv_c2 == 0.0) g = (m0 * m0 + m1 * m1 + m2 * m2) < __builtin_inf() ? 1.0 : __builtin_nan("");
+    else g = 1.0 / lorentz_factor(m0, m1, m2, inv_c2);
     const double t0 = hq * b0 * g, t1 = hq * b1 * g, t2 = hq * b2 * g;
     const double f = 2.0 / (1.0 + (t0 * t0 + t1 * t1 + t2 * t2));
     const double p0 = m0 + (m1 * t2 - m2 * t1);
