@@ -118,8 +118,9 @@ __device__ __forceinline__ double node_bilinear(const double* __restrict__ f, co
 template <int GEOM>
 __device__ __forceinline__ void magnetic_at(const TraceParams& p, double lq, double qy, double cy,
                                             double cz, double& B0, double& B1, double& B2) {
-    const double sx = (lq - p.origin_x) / p.h;
-    const double sy = (qy - p.origin_y) / p.h;
+    // (the node coordinates as products with RN(1 / h), as the edge field's on one patch)
+    const double sx = (lq - p.origin_x) * p.inv_h;
+    const double sy = (qy - p.origin_y) * p.inv_h;
     const NodeCell n = node_cell(sx, sy, (int)p.nx, (int)p.ny);
     const double c0 = node_bilinear(p.b0, n, (int)p.nx, (int)p.ny);
     const double c1 = node_bilinear(p.b1, n, (int)p.nx, (int)p.ny);
