@@ -146,7 +146,8 @@ __device__ __forceinline__ void boris_kick(double& u0, double& u1, double& u2, d
     if (inv_c2 == 0.0) g = (m0 * m0 + m1 * m1 + m2 * m2) < __builtin_inf() ? 1.0 : __builtin_nan("");
     else g = 1.0 / lorentz_factor(m0, m1, m2, inv_c2);
     const double t0 = hq * b0 * g, t1 = hq * b1 * g, t2 = hq * b2 * g;
-    const double f = 2.0 / (1.0 + (t0 * t0 + t1 * t1 + t2 * t2));
+    // (2 / (1 + |t|^2) as a product with the corrected reciprocal: 1 + |t|^2 >= 1 is normal)
+    const double f = 2.0 * recip_nr(1.0 + (t0 * t0 + t1 * t1 + t2 * t2));
     const double p0 = m0 + (m1 * t2 - m2 * t1);
     const double p1 = m1 + (m2 * t0 - m0 * t2);
     const double p2 = m2 + (m0 * t1 - m1 * t0);
