@@ -145,8 +145,8 @@ __device__ __forceinline__ Vec3 accel(const AdjointParams& p, const AdjFields& F
         if (q.b < p.origin_y) cy = -1.0;
         qy = p.origin_y + fabs(q.b - p.origin_y);
     }
-    const double sx = (q.a - p.origin_x) * p.inv_h;
-    const double sy = (qy - p.origin_y) * p.inv_h;
+    const double sx = axis_coord(q.a, p.origin_x, p.inv_h);
+    const double sy = axis_coord(qy, p.origin_y, p.inv_h);
     // (gather_issue / gather_finish: clamped_bilinear_buf's operations)
     const GatherLoads lx = gather_issue(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
     const GatherLoads ly = gather_issue(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
@@ -364,8 +364,8 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
         if (q.b < p.origin_y) cy = -1.0;
         qy = p.origin_y + fabs(q.b - p.origin_y);
     }
-    const double sx = (q.a - p.origin_x) * p.inv_h;
-    const double sy = (qy - p.origin_y) * p.inv_h;
+    const double sx = axis_coord(q.a, p.origin_x, p.inv_h);
+    const double sy = axis_coord(qy, p.origin_y, p.inv_h);
     const GatherLoads lx = gather_issue(F.fx, p.ny, sx + 0.5, sy, (int)p.nx - 1, (int)p.ny - 2);
     const GatherLoads ly = gather_issue(F.fy, p.ny + 1, sx, sy + 0.5, (int)p.nx - 2, (int)p.ny - 1);
     if (pf != nullptr) {
@@ -621,8 +621,8 @@ __device__ __forceinline__ void cell_of(const AdjointParams& p, Vec3 x, const Ra
     if (GEOM == 1) qy = rx.r;
     else if (GEOM == 2) qy = p.origin_y + fabs(x.b - p.origin_y);
     // (floor clamped to [0, n - 1] as the saturated conversion and one median: the same cell)
-    ci = clamped_floor_index<true>((x.a - p.origin_x) * p.inv_h, (int)p.nx - 1);
-    cj = clamped_floor_index<true>((qy - p.origin_y) * p.inv_h, (int)p.ny - 1);
+    ci = clamped_floor_index<true>(axis_coord(x.a, p.origin_x, p.inv_h), (int)p.nx - 1);
+    cj = clamped_floor_index<true>(axis_coord(qy, p.origin_y, p.inv_h), (int)p.ny - 1);
 }
 
 template <int INTEG, int GEOM>

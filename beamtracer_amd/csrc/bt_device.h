@@ -85,6 +85,13 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     return __builtin_fma(y * e, p, y);
 }
 
+// (x - o) / h on a uniform axis as one fused multiply-add with inv_h = RN(1 / h):
+// RN(x inv_h - RN(o inv_h)); for o = 0 the same bits as RN(RN(x - o) inv_h), otherwise within
+// an ulp of the axis' extent in cells (the product is formed once per launch).
+__device__ __forceinline__ double axis_coord(double x, double o, double inv_h) {
+    return __builtin_fma(x, inv_h, -(o * inv_h));
+}
+
 // 1 / d for a positive normal d: the hardware estimate r (v_rcp_f64) and one third-order
 // correction, r (1 + e + e^2) with e = 1 - d r (truncation about e^3 < 2^-66).
 __device__ __forceinline__ double recip_nr(double d) {

@@ -119,8 +119,8 @@ template <int GEOM>
 __device__ __forceinline__ void magnetic_at(const TraceParams& p, double lq, double qy, double cy,
                                             double cz, double& B0, double& B1, double& B2) {
     // (the node coordinates as products with RN(1 / h), as the edge field's on one patch)
-    const double sx = (lq - p.origin_x) * p.inv_h;
-    const double sy = (qy - p.origin_y) * p.inv_h;
+    const double sx = axis_coord(lq, p.origin_x, p.inv_h);
+    const double sy = axis_coord(qy, p.origin_y, p.inv_h);
     const NodeCell n = node_cell(sx, sy, (int)p.nx, (int)p.ny);
     const double c0 = node_bilinear(p.b0, n, (int)p.nx, (int)p.ny);
     const double c1 = node_bilinear(p.b1, n, (int)p.nx, (int)p.ny);
@@ -292,8 +292,8 @@ __device__ __forceinline__ void edge_field(const TraceParams& p, const Fields& F
             sx = (lq - v.ox) / v.h;
             sy = (qy - v.oy) / v.h;
         } else {
-            sx = (lq - v.ox) * p.inv_h;
-            sy = (qy - v.oy) * p.inv_h;
+            sx = axis_coord(lq, v.ox, p.inv_h);
+            sy = axis_coord(qy, v.oy, p.inv_h);
         }
     }
     if (LPT == 2) {
@@ -736,8 +736,8 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
         py = p.origin_y + fabs(y1 - p.origin_y);
     }
     const double lx = locate_x<GENERAL>(p, y0);
-    const double ci = end_known ? end_sx : (GENERAL ? (lx - p.origin_x) / p.h : (lx - p.origin_x) * p.inv_h);
-    const double cj = end_known ? end_sy : (GENERAL ? (py - p.origin_y) / p.h : (py - p.origin_y) * p.inv_h);
+    const double ci = end_known ? end_sx : (GENERAL ? (lx - p.origin_x) / p.h : axis_coord(lx, p.origin_x, p.inv_h));
+    const double cj = end_known ? end_sy : (GENERAL ? (py - p.origin_y) / p.h : axis_coord(py, p.origin_y, p.inv_h));
     // clamped on both sides as trace_trajectories_2d_cuda.cu:340-341: min(trunc(max(c, 0)),
     // n - 2, cells - 1), taken as the saturated trunc(c) clamped to [0, that bound] (the
     // same integer for an integer bound)
@@ -763,8 +763,8 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
         // weight 1, one beyond the last on the last; a NaN point on the first): the cell index
         // by the saturated conversion and a median, the fractions clamped to [0, 1]
         const PatchView pv = patch_view<GENERAL>(p, patch_of<GENERAL>(p, m0, m1));
-        const double sxm = GENERAL ? (m0 - pv.ox) / pv.h : (m0 - pv.ox) * p.inv_h;
-        const double sym = GENERAL ? (m1 - pv.oy) / pv.h : (m1 - pv.oy) * p.inv_h;
+        const double sxm = GENERAL ? (m0 - pv.ox) / pv.h : axis_coord(m0, pv.ox, p.inv_h);
+        const double sym = GENERAL ? (m1 - pv.oy) / pv.h : axis_coord(m1, pv.oy, p.inv_h);
         const int i = clamped_floor_index<!GENERAL>(sxm, (int)pv.nx - 2);  // (a patch has fewer than 2^29 nodes)
         const int j = clamped_floor_index<!GENERAL>(sym, (int)pv.ny - 2);
         const double u = fmin(fmax(sxm - (double)i, 0.0), 1.0);
