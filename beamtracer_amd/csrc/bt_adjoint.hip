@@ -413,7 +413,7 @@ __device__ __forceinline__ Vec3 accel_adjoint(const AdjointParams& p, const AdjF
 // kernel's step_rule on the same operands).
 __device__ __forceinline__ double step_size(const AdjointParams& p, Vec3 v, Vec3 a, int& branch, double& speed,
                                             double& acc) {
-    const StepRule r = step_rule(v.a * v.a + v.b * v.b + v.c * v.c, a.a * a.a + a.b * a.b + a.c * a.c, p.h,
+    const StepRule r = step_rule(norm2(v.a, v.b, v.c), norm2(a.a, a.b, a.c), p.h,
                                  p.step_fraction, 1.0);
     branch = r.branch;
     speed = r.speed;

@@ -142,6 +142,11 @@ __device__ __forceinline__ Radial radial(double y, double z) {
     return o;
 }
 
+// |u|^2 as fused multiply-adds (within an ulp of the sum of the three products).
+__device__ __forceinline__ double norm2(double u0, double u1, double u2) {
+    return __builtin_fma(u0, u0, __builtin_fma(u1, u1, u2 * u2));
+}
+
 // The step-size rule (pyccel.py:282-295): dt = f min(h / |v|, sqrt(2 h / |a|)) over the
 // terms that exist, h when neither does; |v| = gv sqrt(v2), |a| = sqrt(a2) (gv = 1 / gamma,
 // exactly 1 for classical motion).  `branch`: 1 = h / |v|, 2 = sqrt(2 h / |a|), 0 = h;

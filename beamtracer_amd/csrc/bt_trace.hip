@@ -662,7 +662,7 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
         }
         if (stage == 0) {
             const double a0 = g0, a1 = g1, a2 = g2;
-            dt = step_rule(v0 * v0 + v1 * v1 + v2 * v2, a0 * a0 + a1 * a1 + a2 * a2, hloc, p.step_fraction, gv).dt;
+            dt = step_rule(norm2(v0, v1, v2), norm2(a0, a1, a2), hloc, p.step_fraction, gv).dt;
             hdt = 0.5 * dt;
             dt6 = dt * (1.0 / 6.0);
             if (INTEG != 0) {
@@ -772,7 +772,7 @@ __device__ __forceinline__ int step(const TraceParams& p, KParams kp, const Fiel
         if (GEOM == 1) {
             const double low = pv.oy + pv.h * (double)j;
             const double high = low + pv.h;
-            w = (m1 * m1 - low * low) * recip_nr(high * high - low * low);
+            w = __builtin_fma(m1, m1, -(low * low)) * recip_nr(high * high - low * low);
         } else {
             w = sym - (double)j;
         }
