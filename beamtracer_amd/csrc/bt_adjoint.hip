@@ -341,8 +341,9 @@ __device__ __forceinline__ void window_follow(const AdjointParams& p, KAdj kp, G
             if (!(same_rows && col >= nlo && col < nlo + kWinCols)) flush_column(kp, W, s, col);
         }
     }
-    W.lo = nlo;
-    W.base = nbase;
+    // (wave-uniform: kept in scalar registers)
+    W.lo = __builtin_amdgcn_readfirstlane(nlo);
+    W.base = __builtin_amdgcn_readfirstlane(nbase);
 }
 
 // The transposed field evaluation: given lambda_g for g = q/m E(q), scatter into dJ/dfx,
@@ -631,7 +632,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointParams p) {
     const u64 below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const AdjFields F = {field_rsrc(p.fxs, (p.nx + 1) * p.ny), field_rsrc(p.fys, p.nx * (p.ny + 1))};
     GradWindow W;
-    W.wave = (int)(threadIdx.x >> 6);
+    W.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     W.lo = -1;
     W.base = 0;
     GradCache fx_sums, fy_sums;  // the lane's node sums of dJ/dfx and dJ/dfy

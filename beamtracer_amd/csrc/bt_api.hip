@@ -450,12 +450,13 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
 // outnumber the waves the grid holds at once: whole groups would leave the last round of
 // waves partly filled (2e5 trajectories on 2048 waves: a second round of 1077 waves that
 // costs nearly a full one); pieces of 1/P of a walk make the last round's share 1/P as large.
-// Eight pieces (fewer when the walks have fewer segments) measured best on the einzel lens's
-// 24-segment walks: 2 pieces 22.0 ms, 4 20.9, 8 19.2, 12 19.2, 16 19.5 (uneven pieces of
-// one and two segments), one piece 24.5.
+// Twelve pieces (fewer when the walks have fewer segments) measured best on the einzel lens's
+// 24-segment walks (2e5 trajectories): 6 pieces 18.52 ms, 8 18.45, 10 18.50, 12 18.30,
+// 16 18.66 (uneven pieces of one and two segments), 24 18.61; one piece 24.5 on an earlier
+// build (2 pieces 22.0, 4 20.9, 8 19.2 there).
 int adjoint_pieces(i64 groups, i64 waves, i64 segments) {
     if (waves < 1 || groups <= waves || segments < 2) return 1;
-    return (int)(segments < 8 ? segments : 8);
+    return (int)(segments < 12 ? segments : 12);
 }
 
 // The grid, checkpoint slots, groups and scratch bytes of an adjoint call: the queue head,
