@@ -451,9 +451,9 @@ int run_batch(const BatchPlan* plan, const bt_batch_args* a, hipStream_t stream)
 // waves partly filled (2e5 trajectories on 2048 waves: a second round of 1077 waves that
 // costs nearly a full one); pieces of 1/P of a walk make the last round's share 1/P as large.
 // Twelve pieces (fewer when the walks have fewer segments) measured best on the einzel lens's
-// 24-segment walks (2e5 trajectories): 6 pieces 18.52 ms, 8 18.45, 10 18.50, 12 18.30,
-// 16 18.66 (uneven pieces of one and two segments), 24 18.61; one piece 24.5 on an earlier
-// build (2 pieces 22.0, 4 20.9, 8 19.2 there).
+// 24-segment walks (2e5 trajectories): 8 pieces 16.59 ms, 12 16.29, 16 16.60 (uneven pieces of
+// one and two segments); on an earlier build 6 18.52, 10 18.50, 24 18.61 against 12's 18.30,
+// and one piece 24.5 (2 pieces 22.0, 4 20.9).
 int adjoint_pieces(i64 groups, i64 waves, i64 segments) {
     if (waves < 1 || groups <= waves || segments < 2) return 1;
     return (int)(segments < 12 ? segments : 12);
