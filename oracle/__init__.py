@@ -61,6 +61,7 @@ MUTANTS = {
     "azimuthal_b": "-DORACLE_MUTANT_BTHETA=-1.0",
     "mirror_b": "-DORACLE_MUTANT_MIRROR_B=1",
     "midpoint": "-DORACLE_MUTANT_MIDPOINT=-1.0",
+    "lorentz": "-DORACLE_MUTANT_LORENTZ=1",
 }
 
 

@@ -31,12 +31,14 @@
  *
  * Mutation switches (tests only): ORACLE_MUTANT_BR = -1 flips the radial B, _BTHETA = -1
  * the azimuthal B, _MIRROR_B = 1 drops the sign of B_x below a symmetry plane and
- * _MIDPOINT = -1 flips the dt (v - w) / 8 term of the deposit point.  tests/ builds such
+ * _MIDPOINT = -1 flips the dt (v - w) / 8 term of the deposit point, _LORENTZ = 1 takes
+ * the first-order expansion 1 + u.u / 2c^2 for the Lorentz factor.  tests/ builds such
  * mutants to show that the pins above fail on each of these mistakes; the default build
  * is the reference's arithmetic.
  * The relativistic Lorentz factor (mpiparticles, un-vendored) is written from its
- * equation gamma = sqrt(1 + u.u / c^2): bit-level parity with mpiparticles is unpinned;
- * for inv_c2 = 0 it is exactly 1.
+ * equation gamma = sqrt(1 + u.u / c^2), pinned by energy conservation in a uniform electric
+ * field (gamma c^2 grows by q/m E dx; the speed at a plane follows from gamma alone): its
+ * bit-level parity with mpiparticles is unpinned; for inv_c2 = 0 it is exactly 1.
  */
 #include <math.h>
 #include <stdint.h>
@@ -59,11 +61,16 @@ typedef int64_t i64;
 #ifndef ORACLE_MUTANT_MIDPOINT
 #define ORACLE_MUTANT_MIDPOINT 1.0
 #endif
+#ifndef ORACLE_MUTANT_LORENTZ
+#define ORACLE_MUTANT_LORENTZ 0
+#endif
 
 /* gamma of the proper velocity u (mpiparticles lorentz_factor; equation of
  * solver/tracer.py:464-466): exactly 1 when inv_c2 == 0. */
 static double lorentz_factor(double u0, double u1, double u2, double inv_c2) {
-    return sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2) * inv_c2);
+    const double s = (u0 * u0 + u1 * u1 + u2 * u2) * inv_c2;
+    if (ORACLE_MUTANT_LORENTZ) return 1.0 + 0.5 * s;
+    return sqrt(1.0 + s);
 }
 
 /* trace_trajectories_2d_pyccel.py:19-33 */
